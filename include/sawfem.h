@@ -1,0 +1,99 @@
+/* sawfem: MI355X (gfx950) HIP backend for the SawSim piezoelectric FEM frequency sweep.
+ *
+ * Plain C ABI: device pointers are passed as void*, sizes as int/int64_t, the stream as
+ * a hipStream_t cast to void*.  Complex values are interleaved (re, im) doubles
+ * ("complex128") unless stated.  Every function returns 0 on success and a non-zero
+ * status otherwise (1 invalid argument, 3 HIP runtime error); sawfem_last_error()
+ * returns the thread-local message of the last failure.
+ *
+ * Reference interfaces replaced (SawSim, Python):
+ *   saw2d/piezo_fem.py:283-367 assemble / :373-405 assemble_ME0  -> sawfem_elem_q9 + sawfem_gather_kmd
+ *   saw2d/sweep.py:283-323 FrequencySolver.__init__/set_damping     -> sawfem_gather3 (Tᵀ K T, active extraction,
+ *                                                                     Km@s, Kᵀs)
+ *   saw2d/sweep.py:326-333, 349-353 _backend / PARDISO / SuperLU   -> sawfem_mf_create, sawfem_mf_factor,
+ *                                                                     sawfem_mf_solve
+ *   saw2d/sweep.py:337-342 rhs = -(Ks - w^2 Ms)[active] V          -> sawfem_rhs
+ *   saw2d/sweep.py:362 Q = kts . disp                               -> sawfem_charge
+ *   (no counterpart; refinement against the FP64 operator)          -> sawfem_residual, sawfem_axpy
+ */
+#ifndef SAWFEM_H
+#define SAWFEM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* sawfem_last_error(void);
+
+/* K1: Q9 element matrices.  conn (ne*9 int32, QUADS9 order), xy (N*2 double, metres),
+ * mat_idx/elem_tag (ne int32), per material C9 (nmat*81), E9 (nmat*27), EPS (nmat*9)
+ * complex128 (the 6x6 / 3x6 tensors lifted to 9-component strain), rho (nmat double);
+ * npml <= 8 PML regions given by host arrays tag, kind (bit0 x, bit1 y) and
+ * par (6 doubles: dmax, n, xa, xp, ya, yp).  ndu = 2 (plane strain [ux|uy|phi]) or 3.
+ * Writes Ke[e][A][B][a][b] (ne*(ndu+1)^2*81) and Me[e][a][b] (ne*81), complex128.
+ * Replaces piezo_fem.py:248-350 (_element_geometry, element_alpha, block contractions). */
+int sawfem_elem_q9(void* stream, int ne, int ndu, const int* conn, const double* xy, const int* mat_idx,
+                   const int* elem_tag, const void* C9, const void* E9, const void* EPS, const double* rho,
+                   int npml, const int* pml_tag, const int* pml_kind, const double* pml_par,
+                   double omega_top, double scale, void* Ke, void* Me);
+
+/* K2: global K, M (and D = K_uu of elements with dmask[e] != 0, if D != NULL) on the
+ * structural CSR pattern, gathered per node pair in a fixed order (no atomics).
+ * Replaces the COO -> CSR scatter of piezo_fem.py:352-365. */
+int sawfem_gather_kmd(void* stream, int nnzn, int nd, int ndu, const int* pair_row, const int64_t* node_ptr,
+                      const int64_t* gptr, const int64_t* gidx, const unsigned char* dmask, const void* Ke,
+                      const void* Me, void* K, void* M, void* D);
+
+/* K3: d_i[j] = sum_{c in ptr[j]..ptr[j+1]} s_i[idx[c]] for up to three complex128 sources
+ * sharing one list (s1/d1, s2/d2 may be NULL together).  Used for Tᵀ A T restricted to the
+ * active set, the Km@s column and the charge weights (sweep.py:293-309). */
+int sawfem_gather3(void* stream, int64_t nout, const int64_t* ptr, const int64_t* idx, const void* s0,
+                   const void* s1, const void* s2, void* d0, void* d1, void* d2);
+
+/* Multifrontal plan from the host symbolic analysis (all pointers are host memory). */
+int sawfem_mf_create(void** plan, int n, int nf, const int* m, const int* p, const int64_t* rows_ptr,
+                     const int* rows, const int* child_ptr, const int* child_ids, const int64_t* inv_ptr,
+                     const int* inv, int nlev, const int* lvl_ptr, const int* lvl_fronts, int64_t nent,
+                     const int* ent_front, const int64_t* ent_slot, const int* ent_loc, const int64_t* ent_lvl_ptr);
+int sawfem_mf_destroy(void* plan);
+int64_t sawfem_mf_arena_elems(void* plan);   /* complex entries of one frequency's fronts */
+int64_t sawfem_mf_uarena_elems(void* plan);  /* complex128 entries of one frequency's update vectors */
+int sawfem_mf_perturbed(void* plan, int reset);
+
+/* Factor A(w_b) = K + i w_b beta D - w_b^2 M for B frequencies (omega: device doubles)
+ * into arena (arena_elems * B entries; complex64 if prec == 0, complex128 if prec == 1).
+ * Kaa/Maa/Daa: complex128 values on the active pattern (Daa may be NULL).  nb: panel width. */
+int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* omega, double beta, const void* Kaa,
+                     const void* Maa, const void* Daa, void* arena, int nb);
+
+/* Forward/backward substitution with the factors of sawfem_mf_factor; x: B x n complex128,
+ * right-hand sides in, solutions out.  uarena: uarena_elems * B complex128 scratch. */
+int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x);
+
+/* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;
+ * used by the benchmark).  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S,
+ * 6 fwd, 7 bwd.  profile_read waits for the recorded events and returns totals. */
+int sawfem_mf_profile(void* plan, int enable);
+int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count);
+
+/* out[b][i] = -V (rK[i] + i w_b beta rD[i] - w_b^2 rM[i])  (sweep.py:338-342). */
+int sawfem_rhs(void* stream, int n, int B, const double* omega, double beta, double V, const void* rK,
+               const void* rM, const void* rD, void* out);
+
+/* r[b] = rhs[b] - A(w_b) x[b] in FP64 on the active CSR pattern (indptr int64, indices int32). */
+int sawfem_residual(void* stream, int n, int B, const int64_t* indptr, const int* indices, const void* Kaa,
+                    const void* Maa, const void* Daa, const double* omega, double beta, const void* x,
+                    const void* rhs, void* r);
+
+/* Q[b] = sum_i kts_a[i] x[b][i] + V kts_z  (sweep.py:356-362). */
+int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, double kz_im, double V,
+                  const void* x, void* Q);
+
+/* x += d (n complex128). */
+int sawfem_axpy(void* stream, int64_t n, void* x, const void* d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,75 @@
+"""ctypes binding of ``libsawfem_hip.so`` (the C ABI declared in ``include/sawfem.h``).
+
+The product path has no CPU fallback: every function here raises if the HIP library
+is missing or a call returns a non-zero status.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+LIB_NAME = 'libsawfem_hip.so'
+LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
+
+_vp, _i, _i64, _d = C.c_void_p, C.c_int, C.c_int64, C.c_double
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+SIGNATURES = {
+    'sawfem_last_error': [],
+    'sawfem_elem_q9': [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _d, _d, _vp, _vp],
+    'sawfem_gather_kmd': [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    'sawfem_gather3': [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    'sawfem_mf_create': [C.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _i64, _vp,
+                         _vp, _vp, _vp],
+    'sawfem_mf_destroy': [_vp],
+    'sawfem_mf_arena_elems': [_vp],
+    'sawfem_mf_uarena_elems': [_vp],
+    'sawfem_mf_perturbed': [_vp, _i],
+    'sawfem_mf_factor': [_vp, _vp, _i, _i, _vp, _d, _vp, _vp, _vp, _vp, _i],
+    'sawfem_mf_solve': [_vp, _vp, _i, _i, _vp, _vp, _vp],
+    'sawfem_mf_profile': [_vp, _i],
+    'sawfem_mf_profile_read': [_vp, _vp, _vp],
+    'sawfem_rhs': [_vp, _i, _i, _vp, _d, _d, _vp, _vp, _vp, _vp],
+    'sawfem_residual': [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
+    'sawfem_charge': [_vp, _i, _i, _vp, _d, _d, _d, _vp, _vp],
+    'sawfem_axpy': [_vp, _i64, _vp, _vp],
+}
+_RESTYPE = {'sawfem_last_error': C.c_char_p, 'sawfem_mf_arena_elems': _i64, 'sawfem_mf_uarena_elems': _i64}
+
+_lib = None
+
+
+class SawfemError(RuntimeError):
+    pass
+
+
+def load():
+    """Load the HIP library (once); raise if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get('SAWFEM_LIB', LIB_PATH))
+    if not path.exists():
+        raise SawfemError(f'{path} is missing: build it with `python -c "import __graft_entry__ as g; g.build()"`')
+    lib = C.CDLL(str(path))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = _RESTYPE.get(name, _i)
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    """Call a status-returning entry point; raise with the library's message on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != 0:
+        msg = lib.sawfem_last_error().decode(errors='replace')
+        raise SawfemError(f'{name} failed with status {rc}: {msg}')
+    return rc
+
+
+def raw(name, *args):
+    return getattr(load(), name)(*args)

@@ -1,0 +1,807 @@
+// Multifrontal numeric LU of the Bloch-reduced system A(w) = K + i w beta D - w^2 M,
+// batched over frequencies, in complex FP32 (mixed path) or complex FP64 (plain path).
+//
+// Replaces the per-frequency sparse direct solve behind FrequencySolver._backend()
+// (SawSim saw2d/sweep.py:326-333, 349-353: PARDISO or SuperLU on the real-block form).
+// The symbolic analysis (ordering, fronts, index maps) is built once on the host.
+//
+// Storage: the numeric arena holds every front of every frequency of the batch as a
+// dense column-major m x m matrix; front f, frequency b lives at
+//     arena + off[f] * B + b * m_f^2.
+// After factorization the first p columns hold L (unit lower, below the diagonal) and
+// U (upper), rows/columns p.. of the trailing block hold the Schur complement that the
+// parent assembles (extend-add), and U12 = F[0:p, p:m], L21 = F[p:m, 0:p].
+//
+// Per level (leaves first):
+//   k_front_init    : F = sum of children's contribution blocks (gather, no atomics)
+//   k_front_entries : F += A(w) entries owned by this front (unique targets)
+//   for each pivot panel [k0, k0 + kb):
+//     k_diag        : LU of the kb x kb diagonal block (no pivoting, tiny-pivot guard)
+//     k_trsm        : L panel = A U_D^-1, U panel = L_D^-1 A
+//     k_gemm (L)    : update of the trailing rows/columns that are still pivots
+//   k_gemm (S)      : Schur complement F22 -= L21 U12 with K = p  (MFMA)
+#include "common.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int TILE = 64;   // C tile edge of the GEMM workgroup (complex entries)
+constexpr int KC = 16;     // K chunk staged in LDS
+
+struct FrontsDev {
+  const int* m;
+  const int* p;
+  const int64_t* off;      // front offset (entries) for one frequency
+  const int64_t* uoff;     // update-vector offset for one frequency
+  const int64_t* rows_ptr;
+  const int* rows;
+  const int* child_ptr;    // children of f: child_ids[child_ptr[f] .. child_ptr[f+1])
+  const int* child_ids;
+  const int64_t* inv_ptr;  // per child slot: parent-row -> child CB index map (length m_parent)
+  const int* inv;
+};
+
+template <typename T>
+__device__ __forceinline__ typename cplx<T>::type* front_ptr(typename cplx<T>::type* arena, const FrontsDev& F,
+                                                             int f, int b, int B) {
+  const int64_t m = F.m[f];
+  return arena + F.off[f] * (int64_t)B + (int64_t)b * m * m;
+}
+
+// ---------------------------------------------------------------- front assembly
+template <typename T>
+__global__ void __launch_bounds__(256) k_front_init(FrontsDev F, const int* __restrict__ lvl, int B,
+                                                    typename cplx<T>::type* __restrict__ arena, int cols_per_wg) {
+  using V = typename cplx<T>::type;
+  const int f = lvl[blockIdx.y];
+  const int b = blockIdx.z;
+  const int m = F.m[f];
+  const int c0 = blockIdx.x * cols_per_wg;
+  if (c0 >= m) return;
+  const int c1 = min(m, c0 + cols_per_wg);
+  V* Fb = front_ptr<T>(arena, F, f, b, B);
+  const int nch = F.child_ptr[f + 1] - F.child_ptr[f];
+  for (int64_t idx = threadIdx.x; idx < (int64_t)(c1 - c0) * m; idx += blockDim.x) {
+    const int j = c0 + (int)(idx / m);
+    const int i = (int)(idx % m);
+    V v = czero<V>();
+    for (int c = 0; c < nch; ++c) {
+      const int slot = F.child_ptr[f] + c;
+      const int* inv = F.inv + F.inv_ptr[slot];
+      const int ci = inv[i], cj = inv[j];
+      if (ci >= 0 && cj >= 0) {
+        const int ch = F.child_ids[slot];
+        const int64_t mc = F.m[ch], pc = F.p[ch];
+        const V* Fc = front_ptr<T>(arena, F, ch, b, B);
+        const V w = Fc[(pc + ci) + (pc + cj) * mc];
+        v.x += w.x; v.y += w.y;
+      }
+    }
+    Fb[i + (int64_t)j * m] = v;
+  }
+}
+
+template <typename T>
+__global__ void k_front_entries(FrontsDev F, int64_t e0, int64_t e1, int B, const double* __restrict__ omega,
+                                double beta, const int* __restrict__ ent_front, const int64_t* __restrict__ ent_slot,
+                                const int* __restrict__ ent_loc, const double2* __restrict__ Kaa,
+                                const double2* __restrict__ Maa, const double2* __restrict__ Daa,
+                                typename cplx<T>::type* __restrict__ arena) {
+  using V = typename cplx<T>::type;
+  const int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= e1) return;
+  const int b = blockIdx.y;
+  const double w = omega[b];
+  const int64_t s = ent_slot[e];
+  const double2 k = Kaa[s], mm = Maa[s];
+  double2 a;
+  a.x = k.x - w * w * mm.x;
+  a.y = k.y - w * w * mm.y;
+  if (Daa) {
+    const double2 d = Daa[s];
+    a.x -= w * beta * d.y;
+    a.y += w * beta * d.x;
+  }
+  V* Fb = front_ptr<T>(arena, F, ent_front[e], b, B);
+  V cur = Fb[ent_loc[e]];
+  cur.x += (T)a.x;
+  cur.y += (T)a.y;
+  Fb[ent_loc[e]] = cur;
+}
+
+// ---------------------------------------------------------------- panel factorization
+template <typename T>
+__global__ void __launch_bounds__(64) k_diag(FrontsDev F, const int* __restrict__ lvl, int B,
+                                             typename cplx<T>::type* __restrict__ arena, int k0, int nb,
+                                             T tiny, int* __restrict__ npert) {
+  using V = typename cplx<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  V* Ds = reinterpret_cast<V*>(smem_raw);   // [nb][nb+1], column-major with padding
+  const int f = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const int p = F.p[f];
+  if (k0 >= p) return;
+  const int m = F.m[f];
+  const int kb = min(nb, p - k0);
+  const int ld = nb + 1;
+  V* Fb = front_ptr<T>(arena, F, f, b, B);
+  const int t = threadIdx.x;
+  for (int j = 0; j < kb; ++j)
+    if (t < kb) Ds[t + j * ld] = Fb[(int64_t)(k0 + t) + (int64_t)(k0 + j) * m];
+  __syncthreads();
+  for (int k = 0; k < kb; ++k) {
+    V piv = Ds[k + k * ld];
+    const T mag = piv.x * piv.x + piv.y * piv.y;
+    if (!(mag > tiny * tiny)) {
+      // static pivot perturbation; refinement against the FP64 operator repairs it
+      piv.x = tiny; piv.y = 0;
+      if (t == 0) { Ds[k + k * ld] = piv; atomicAdd(npert, 1); }
+    }
+    if (t > k && t < kb) {
+      const V l = cdiv(Ds[t + k * ld], piv);
+      Ds[t + k * ld] = l;
+      for (int j = k + 1; j < kb; ++j) Ds[t + j * ld] = cfms(Ds[t + j * ld], l, Ds[k + j * ld]);
+    }
+    __syncthreads();
+  }
+  for (int j = 0; j < kb; ++j)
+    if (t < kb) Fb[(int64_t)(k0 + t) + (int64_t)(k0 + j) * m] = Ds[t + j * ld];
+}
+
+// L panel rows [k1, m) x cols [k0, k1): X U_D = A.   U panel rows [k0, k1) x cols [k1, m): L_D X = A.
+template <typename T>
+__global__ void __launch_bounds__(64) k_trsm(FrontsDev F, const int* __restrict__ lvl, int B,
+                                             typename cplx<T>::type* __restrict__ arena, int k0, int nb, int ntmax) {
+  using V = typename cplx<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  const int ld = nb + 1;
+  V* Ds = reinterpret_cast<V*>(smem_raw);      // [nb][ld] diag block, column-major
+  V* Xs = Ds + nb * ld;                        // [64][ld]: row r of the tile at Xs[r * ld + j]
+  const int f = lvl[blockIdx.y];
+  const int b = blockIdx.z;
+  const int p = F.p[f];
+  if (k0 >= p) return;
+  const int m = F.m[f];
+  const int kb = min(nb, p - k0);
+  const int k1 = k0 + kb;
+  const int nt = (m - k1 + 63) / 64;
+  const bool upanel = blockIdx.x >= ntmax;
+  const int tile = upanel ? blockIdx.x - ntmax : blockIdx.x;
+  if (tile >= nt) return;
+  V* Fb = front_ptr<T>(arena, F, f, b, B);
+  const int t = threadIdx.x;
+  for (int j = 0; j < kb; ++j)
+    if (t < kb) Ds[t + j * ld] = Fb[(int64_t)(k0 + t) + (int64_t)(k0 + j) * m];
+  const int r0 = k1 + tile * 64;
+  const int nr = min(64, m - r0);
+  if (!upanel) {
+    for (int j = 0; j < kb; ++j)
+      if (t < nr) Xs[t * ld + j] = Fb[(int64_t)(r0 + t) + (int64_t)(k0 + j) * m];
+  } else {
+    // column c = r0 + cc of the U panel, row k0 + i: stage as Xs[cc * ld + i]
+    for (int idx = t; idx < nr * kb; idx += 64) {
+      const int cc = idx / kb, i = idx % kb;
+      Xs[cc * ld + i] = Fb[(int64_t)(k0 + i) + (int64_t)(r0 + cc) * m];
+    }
+  }
+  __syncthreads();
+  if (t < nr) {
+    V* x = Xs + t * ld;
+    if (!upanel) {
+      for (int j = 0; j < kb; ++j) {
+        V s = x[j];
+        for (int i = 0; i < j; ++i) s = cfms(s, x[i], Ds[i + j * ld]);
+        x[j] = cdiv(s, Ds[j + j * ld]);
+      }
+    } else {
+      for (int i = 0; i < kb; ++i) {
+        V s = x[i];
+        for (int j = 0; j < i; ++j) s = cfms(s, Ds[i + j * ld], x[j]);
+        x[i] = s;
+      }
+    }
+  }
+  __syncthreads();
+  if (!upanel) {
+    for (int j = 0; j < kb; ++j)
+      if (t < nr) Fb[(int64_t)(r0 + t) + (int64_t)(k0 + j) * m] = Xs[t * ld + j];
+  } else {
+    for (int idx = t; idx < nr * kb; idx += 64) {
+      const int cc = idx / kb, i = idx % kb;
+      Fb[(int64_t)(k0 + i) + (int64_t)(r0 + cc) * m] = Xs[cc * ld + i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- complex GEMM on MFMA
+// C[r, c] -= sum_k A[r, k] B[k, c] inside one front, for the C tile (r0.., c0..) of 64x64.
+// The MFMA computes D = B^T A^T so that the lane-contiguous output index is the C row,
+// which is contiguous in the column-major front (coalesced epilogue).
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct WaveAcc;
+
+template <> struct WaveAcc<float> {
+  // one 32x32 complex block per wave: D[i = col][j = row]
+  floatx16 re, im;
+  __device__ void zero() {
+    for (int v = 0; v < 16; ++v) { re[v] = 0.f; im[v] = 0.f; }
+  }
+  // As: [KC][TILE] (k-major, rows contiguous); Bs: [TILE][KC+1] (col-major over k)
+  __device__ void step(const float2* As, const float2* Bs, int wrow, int wcol, int lane) {
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 2) {
+      const int k = kk + (lane >> 5);
+      const float2 bv = Bs[(wcol * 32 + (lane & 31)) * (KC + 1) + k];   // B[k][col]
+      const float2 av = As[k * TILE + wrow * 32 + (lane & 31)];        // A[row][k]
+      re = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.x, av.x, re, 0, 0, 0);
+      re = __builtin_amdgcn_mfma_f32_32x32x2f32(-bv.y, av.y, re, 0, 0, 0);
+      im = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.x, av.y, im, 0, 0, 0);
+      im = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.y, av.x, im, 0, 0, 0);
+    }
+  }
+  template <typename Fn> __device__ void each(int lane, Fn fn) {
+    // D[i][j]: j = lane % 32 (row), i = 8 (v / 4) + 4 (lane / 32) + v % 4 (col)
+    for (int v = 0; v < 16; ++v) {
+      const int i = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+      fn(lane & 31, i, re[v], im[v]);
+    }
+  }
+};
+
+template <> struct WaveAcc<double> {
+  // 2x2 blocks of 16x16: blk[ci][rj]
+  doublex4 re[2][2], im[2][2];
+  __device__ void zero() {
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c)
+        for (int v = 0; v < 4; ++v) { re[a][c][v] = 0.0; im[a][c][v] = 0.0; }
+  }
+  __device__ void step(const double2* As, const double2* Bs, int wrow, int wcol, int lane) {
+#pragma unroll
+    for (int kk = 0; kk < KC; kk += 4) {
+      const int k = kk + (lane >> 4);
+      double2 bv[2], av[2];
+      for (int ci = 0; ci < 2; ++ci) bv[ci] = Bs[(wcol * 32 + ci * 16 + (lane & 15)) * (KC + 1) + k];
+      for (int rj = 0; rj < 2; ++rj) av[rj] = As[k * TILE + wrow * 32 + rj * 16 + (lane & 15)];
+      for (int ci = 0; ci < 2; ++ci)
+        for (int rj = 0; rj < 2; ++rj) {
+          re[ci][rj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[ci].x, av[rj].x, re[ci][rj], 0, 0, 0);
+          re[ci][rj] = __builtin_amdgcn_mfma_f64_16x16x4f64(-bv[ci].y, av[rj].y, re[ci][rj], 0, 0, 0);
+          im[ci][rj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[ci].x, av[rj].y, im[ci][rj], 0, 0, 0);
+          im[ci][rj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[ci].y, av[rj].x, im[ci][rj], 0, 0, 0);
+        }
+    }
+  }
+  template <typename Fn> __device__ void each(int lane, Fn fn) {
+    // f64 16x16x4 output: D[i][j] with j = lane % 16 (row of C), i = 4 v + lane / 16 (column of C)
+    for (int ci = 0; ci < 2; ++ci)
+      for (int rj = 0; rj < 2; ++rj)
+        for (int v = 0; v < 4; ++v) fn(rj * 16 + (lane & 15), ci * 16 + 4 * v + (lane >> 4), re[ci][rj][v], im[ci][rj][v]);
+  }
+};
+
+// mode 0: L-shaped update for panel [k0, k0+kb)  (skips rows >= p and cols >= p together)
+// mode 1: Schur complement rows/cols [p, m), K = [0, p)
+template <typename T>
+__global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
+                                              typename cplx<T>::type* __restrict__ arena, int mode, int k0, int nb) {
+  using V = typename cplx<T>::type;
+  __shared__ V As[KC * TILE];
+  __shared__ V Bs[TILE * (KC + 1)];
+  const int f = lvl[blockIdx.y];
+  const int b = blockIdx.z;
+  const int p = F.p[f];
+  const int m = F.m[f];
+  int lo, kbeg, kend;
+  if (mode == 0) {
+    if (k0 >= p) return;
+    const int kb = min(nb, p - k0);
+    lo = k0 + kb; kbeg = k0; kend = k0 + kb;
+  } else {
+    if (p == 0) return;
+    lo = p; kbeg = 0; kend = p;
+  }
+  if (lo >= m) return;
+  const int nt = (m - lo + TILE - 1) / TILE;
+  const int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
+  if (tj >= nt) return;
+  const int r0 = lo + ti * TILE, c0 = lo + tj * TILE;
+  if (mode == 0 && r0 >= p && c0 >= p) return;
+  V* Fb = front_ptr<T>(arena, F, f, b, B);
+  const int t = threadIdx.x;
+  const int wave = t >> 6, lane = t & 63;
+  const int wrow = wave & 1, wcol = wave >> 1;
+  WaveAcc<T> acc;
+  acc.zero();
+  for (int kc = kbeg; kc < kend; kc += KC) {
+    // A chunk: rows r0.., k in [kc, kc+KC) -> As[k][row]
+    for (int idx = t; idx < KC * TILE; idx += 256) {
+      const int row = idx % TILE, k = idx / TILE;
+      const int gr = r0 + row, gk = kc + k;
+      As[k * TILE + row] = (gr < m && gk < kend) ? Fb[(int64_t)gr + (int64_t)gk * m] : czero<V>();
+    }
+    // B chunk: k in [kc, kc+KC), cols c0.. -> Bs[col][k]
+    for (int idx = t; idx < KC * TILE; idx += 256) {
+      const int k = idx % KC, col = idx / KC;
+      const int gk = kc + k, gc = c0 + col;
+      Bs[col * (KC + 1) + k] = (gc < m && gk < kend) ? Fb[(int64_t)gk + (int64_t)gc * m] : czero<V>();
+    }
+    __syncthreads();
+    acc.step(As, Bs, wrow, wcol, lane);
+    __syncthreads();
+  }
+  acc.each(lane, [&](int j, int i, T re, T im) {
+    const int gr = r0 + wrow * 32 + j;
+    const int gc = c0 + wcol * 32 + i;
+    if (gr < m && gc < m && !(mode == 0 && gr >= p && gc >= p)) {
+      V* c = Fb + (int64_t)gr + (int64_t)gc * m;
+      V v = *c;
+      v.x -= re; v.y -= im;
+      *c = v;
+    }
+  });
+}
+
+// ---------------------------------------------------------------- triangular solves
+// x: [B][n] complex128, in place: forward writes y at pivot rows, backward overwrites with x.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwd(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+                                             const typename cplx<T>::type* __restrict__ arena,
+                                             double2* __restrict__ uarena, double2* __restrict__ x) {
+  using V = typename cplx<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  double2* w = reinterpret_cast<double2*>(smem_raw);
+  const int f = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const int m = F.m[f], p = F.p[f];
+  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
+  const int* rows = F.rows + F.rows_ptr[f];
+  double2* xb = x + (int64_t)b * n;
+  const int nch = F.child_ptr[f + 1] - F.child_ptr[f];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    double2 v = (i < p) ? xb[rows[i]] : make_double2(0, 0);
+    for (int c = 0; c < nch; ++c) {
+      const int slot = F.child_ptr[f] + c;
+      const int ci = F.inv[F.inv_ptr[slot] + i];
+      if (ci >= 0) {
+        const int ch = F.child_ids[slot];
+        const double2 u = uarena[F.uoff[ch] * B + (int64_t)b * (F.m[ch] - F.p[ch]) + ci];
+        v.x += u.x; v.y += u.y;
+      }
+    }
+    w[i] = v;
+  }
+  __syncthreads();
+  for (int k = 0; k < p; ++k) {
+    const double2 wk = w[k];
+    for (int i = k + 1 + threadIdx.x; i < p; i += blockDim.x)
+      w[i] = cfms(w[i], to_d2(Fb[(int64_t)i + (int64_t)k * m]), wk);
+    __syncthreads();
+  }
+  double2* uf = uarena + F.uoff[f] * B + (int64_t)b * (m - p);
+  for (int i = p + threadIdx.x; i < m; i += blockDim.x) {
+    double2 s = w[i];
+    for (int k = 0; k < p; ++k) s = cfms(s, to_d2(Fb[(int64_t)i + (int64_t)k * m]), w[k]);
+    uf[i - p] = s;
+  }
+  for (int i = threadIdx.x; i < p; i += blockDim.x) xb[rows[i]] = w[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwd(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+                                             const typename cplx<T>::type* __restrict__ arena,
+                                             double2* __restrict__ x) {
+  using V = typename cplx<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  double2* z = reinterpret_cast<double2*>(smem_raw);   // [m]: z[0:p] pivots, z[p:m] known x
+  const int f = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const int m = F.m[f], p = F.p[f];
+  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
+  const int* rows = F.rows + F.rows_ptr[f];
+  double2* xb = x + (int64_t)b * n;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) z[i] = xb[rows[i]];
+  __syncthreads();
+  for (int i = threadIdx.x; i < p; i += blockDim.x) {
+    double2 s = z[i];
+    for (int j = p; j < m; ++j) s = cfms(s, to_d2(Fb[(int64_t)i + (int64_t)j * m]), z[j]);
+    z[i] = s;
+  }
+  __syncthreads();
+  for (int k = p - 1; k >= 0; --k) {
+    const double2 xk = cdiv(z[k], to_d2(Fb[(int64_t)k + (int64_t)k * m]));
+    for (int i = threadIdx.x; i < k; i += blockDim.x) z[i] = cfms(z[i], to_d2(Fb[(int64_t)i + (int64_t)k * m]), xk);
+    if (threadIdx.x == 0) z[k] = xk;
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < p; i += blockDim.x) xb[rows[i]] = z[i];
+}
+
+// ---------------------------------------------------------------- per-frequency vectors (FP64)
+__global__ void k_rhs(int n, int B, const double* __restrict__ omega, double beta, double V,
+                      const double2* __restrict__ rK, const double2* __restrict__ rM,
+                      const double2* __restrict__ rD, double2* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = blockIdx.y;
+  const double w = omega[b];
+  const double2 k = rK[i], mm = rM[i];
+  double2 a = make_double2(k.x - w * w * mm.x, k.y - w * w * mm.y);
+  if (rD) { const double2 d = rD[i]; a.x -= w * beta * d.y; a.y += w * beta * d.x; }
+  out[(int64_t)b * n + i] = make_double2(-V * a.x, -V * a.y);
+}
+
+// r = rhs - A(w) x for all frequencies; one thread per row, 8 frequencies per pass.
+__global__ void k_residual(int n, int B, const int64_t* __restrict__ indptr, const int* __restrict__ indices,
+                           const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
+                           const double2* __restrict__ Daa, const double* __restrict__ omega, double beta,
+                           const double2* __restrict__ x, const double2* __restrict__ rhs, double2* __restrict__ r) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b0 = blockIdx.y * 8;
+  const int nb = min(8, B - b0);
+  double2 acc[8];
+  double w[8];
+  for (int q = 0; q < 8; ++q) { acc[q] = make_double2(0, 0); w[q] = q < nb ? omega[b0 + q] : 0.0; }
+  for (int64_t s = indptr[i]; s < indptr[i + 1]; ++s) {
+    const int j = indices[s];
+    const double2 k = Kaa[s], mm = Maa[s];
+    const double2 d = Daa ? Daa[s] : make_double2(0, 0);
+    for (int q = 0; q < nb; ++q) {
+      double2 a = make_double2(k.x - w[q] * w[q] * mm.x - w[q] * beta * d.y,
+                               k.y - w[q] * w[q] * mm.y + w[q] * beta * d.x);
+      acc[q] = cfma(acc[q], a, x[(int64_t)(b0 + q) * n + j]);
+    }
+  }
+  for (int q = 0; q < nb; ++q) {
+    const double2 bb = rhs[(int64_t)(b0 + q) * n + i];
+    r[(int64_t)(b0 + q) * n + i] = make_double2(bb.x - acc[q].x, bb.y - acc[q].y);
+  }
+}
+
+// Q[b] = sum_p kts_a[p] x[b][p] + V kts_z  (deterministic block reduction, one block per frequency)
+__global__ void __launch_bounds__(256) k_charge(int n, const double2* __restrict__ kts_a, double2 kts_z, double V,
+                                                const double2* __restrict__ x, double2* __restrict__ Q) {
+  __shared__ double2 red[256];
+  const int b = blockIdx.x;
+  double2 s = make_double2(0, 0);
+  for (int i = threadIdx.x; i < n; i += 256) s = cfma(s, kts_a[i], x[(int64_t)b * n + i]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[threadIdx.x].x += red[threadIdx.x + h].x;
+      red[threadIdx.x].y += red[threadIdx.x + h].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) Q[b] = make_double2(red[0].x + V * kts_z.x, red[0].y + V * kts_z.y);
+}
+
+__global__ void k_axpy(int64_t n, double2* __restrict__ x, const double2* __restrict__ d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i].x += d[i].x;
+  x[i].y += d[i].y;
+}
+
+// ---------------------------------------------------------------- plan
+struct Level {
+  int nf, mmax, pmax, cbmax;
+  int64_t e0, e1;
+  const int* fronts;   // device pointer into lvl_fronts
+};
+
+// Optional per-kernel timing with HIP events on the launch stream (kind ids below).
+enum Kind { K_INIT = 0, K_ENTRIES, K_DIAG, K_TRSM, K_GEMM_L, K_GEMM_S, K_FWD, K_BWD, K_NKIND };
+
+struct Plan {
+  int n = 0, nf = 0;
+  bool profile = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_marks;   // (kind, index of the start event)
+  int64_t arena_elems = 0, uarena_elems = 0;
+  std::vector<Level> levels;
+  FrontsDev F{};
+  std::vector<void*> owned;
+  int* lvl_fronts = nullptr;
+  int* ent_front = nullptr;
+  int64_t* ent_slot = nullptr;
+  int* ent_loc = nullptr;
+  int* npert = nullptr;
+  int mmax = 0;
+};
+
+// Record a start/stop event pair around one launch when profiling is on.
+struct Timed {
+  Plan* P; hipStream_t s; int kind; size_t i0;
+  Timed(Plan* p, hipStream_t st, int k) : P(p), s(st), kind(k), i0(0) {
+    if (!P->profile) return;
+    while (P->ev_pool.size() < P->ev_used + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) { P->profile = false; return; }
+      P->ev_pool.push_back(e);
+    }
+    i0 = P->ev_used;
+    P->ev_used += 2;
+    (void)hipEventRecord(P->ev_pool[i0], s);
+  }
+  ~Timed() {
+    if (!P->profile) return;
+    (void)hipEventRecord(P->ev_pool[i0 + 1], s);
+    P->ev_marks.push_back({kind, i0});
+  }
+};
+
+template <typename X>
+int upload(Plan* P, X** dst, const X* src, int64_t count) {
+  if (count <= 0) { *dst = nullptr; return 0; }
+  void* d = nullptr;
+  SAW_HIP(hipMalloc(&d, sizeof(X) * count));
+  SAW_HIP(hipMemcpy(d, src, sizeof(X) * count, hipMemcpyHostToDevice));
+  P->owned.push_back(d);
+  *dst = (X*)d;
+  return 0;
+}
+
+template <typename T>
+int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta, const void* Kaa, const void* Maa,
+                const void* Daa, void* arena_v, int nb) {
+  using V = typename cplx<T>::type;
+  V* arena = (V*)arena_v;
+  const T tiny = sizeof(T) == 4 ? (T)1e-30 : (T)1e-300;
+  for (const Level& L : P->levels) {
+    const int cpw = 8;
+    { Timed tm(P, s, K_INIT);
+    hipLaunchKernelGGL(k_front_init<T>, dim3((L.mmax + cpw - 1) / cpw, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B,
+                       arena, cpw); }
+    if (L.e1 > L.e0) { Timed tm(P, s, K_ENTRIES);
+      hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), B), dim3(256), 0, s, P->F,
+                         L.e0, L.e1, B, omega, beta, P->ent_front, P->ent_slot, P->ent_loc, (const double2*)Kaa,
+                         (const double2*)Maa, (const double2*)Daa, arena); }
+    for (int k0 = 0; k0 < L.pmax; k0 += nb) {
+      const size_t lds_diag = sizeof(V) * nb * (nb + 1);
+      { Timed tm(P, s, K_DIAG);
+      hipLaunchKernelGGL(k_diag<T>, dim3(L.nf, B), dim3(64), lds_diag, s, P->F, L.fronts, B, arena, k0, nb, tiny,
+                         P->npert); }
+      const int rem = L.mmax - k0 - 1;
+      if (rem > 0) {
+        const int ntmax = (rem + 63) / 64;
+        const size_t lds_trsm = sizeof(V) * (nb + 64) * (nb + 1);
+        { Timed tm(P, s, K_TRSM);
+        hipLaunchKernelGGL(k_trsm<T>, dim3(2 * ntmax, L.nf, B), dim3(64), lds_trsm, s, P->F, L.fronts, B, arena, k0,
+                           nb, ntmax); }
+        const int nt = (rem + TILE - 1) / TILE;
+        if (k0 + nb < L.pmax) {
+          Timed tm(P, s, K_GEMM_L);
+          hipLaunchKernelGGL(k_gemm<T>, dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 0, k0, nb);
+        }
+      }
+    }
+    if (L.cbmax > 0) {
+      const int nt = (L.cbmax + TILE - 1) / TILE;
+      Timed tm(P, s, K_GEMM_S);
+      hipLaunchKernelGGL(k_gemm<T>, dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 1, 0, nb);
+    }
+  }
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x) {
+  const size_t lds = sizeof(double2) * P->mmax;
+  for (const Level& L : P->levels) {
+    Timed tm(P, s, K_FWD);
+    hipLaunchKernelGGL(k_fwd<T>, dim3(L.nf, B), dim3(256), lds, s, P->F, L.fronts, B, P->n,
+                       (const typename cplx<T>::type*)arena, (double2*)uarena, (double2*)x);
+  }
+  for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
+    const Level& L = P->levels[l];
+    Timed tm(P, s, K_BWD);
+    hipLaunchKernelGGL(k_bwd<T>, dim3(L.nf, B), dim3(256), lds, s, P->F, L.fronts, B, P->n,
+                       (const typename cplx<T>::type*)arena, (double2*)x);
+  }
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Build a numeric plan from the host symbolic analysis.  All arrays are host pointers.
+//   m, p, rows_ptr(nf+1), rows: front orders, pivot counts and row lists (active ids)
+//   child_ptr(nf+1), child_ids: children per front; inv_ptr(nchild+1), inv: per child
+//     slot, for each parent row, the child contribution-block index or -1
+//   lvl_ptr(nlev+1), lvl_fronts: fronts per level, leaves first
+//   ent_*: A_aa slot -> (front, column-major local offset), sorted by level then front;
+//     ent_lvl_ptr(nlev+1) delimits the entries of each level
+int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, const int64_t* rows_ptr, const int* rows,
+                     const int* child_ptr, const int* child_ids, const int64_t* inv_ptr, const int* inv, int nlev,
+                     const int* lvl_ptr, const int* lvl_fronts, int64_t nent, const int* ent_front,
+                     const int64_t* ent_slot, const int* ent_loc, const int64_t* ent_lvl_ptr) {
+  if (!out || n <= 0 || nf <= 0 || nlev <= 0) {
+    sawfem_set_error("sawfem_mf_create: invalid arguments");
+    return 1;
+  }
+  Plan* P = new Plan();
+  P->n = n; P->nf = nf;
+  std::vector<int64_t> off(nf), uoff(nf);
+  int64_t acc = 0, uacc = 0;
+  for (int f = 0; f < nf; ++f) {
+    if (m[f] <= 0 || p[f] < 0 || p[f] > m[f] || rows_ptr[f + 1] - rows_ptr[f] != m[f]) {
+      delete P;
+      sawfem_set_error("sawfem_mf_create: inconsistent front");
+      return 1;
+    }
+    off[f] = acc; acc += (int64_t)m[f] * m[f];
+    uoff[f] = uacc; uacc += m[f] - p[f];
+    P->mmax = std::max(P->mmax, m[f]);
+  }
+  P->arena_elems = acc;
+  P->uarena_elems = uacc;
+  const int nchild = child_ptr[nf];
+  int* dm; int* dp; int64_t* doff; int64_t* duoff; int64_t* drp; int* drows; int* dcp; int* dci; int64_t* dip; int* dinv;
+  int rc = 0;
+  rc |= upload(P, &dm, m, nf);
+  rc |= upload(P, &dp, p, nf);
+  rc |= upload(P, &doff, off.data(), nf);
+  rc |= upload(P, &duoff, uoff.data(), nf);
+  rc |= upload(P, &drp, rows_ptr, nf + 1);
+  rc |= upload(P, &drows, rows, rows_ptr[nf]);
+  rc |= upload(P, &dcp, child_ptr, nf + 1);
+  rc |= upload(P, &dci, child_ids, nchild);
+  rc |= upload(P, &dip, inv_ptr, nchild + 1);
+  rc |= upload(P, &dinv, inv, inv_ptr[nchild]);
+  rc |= upload(P, &P->lvl_fronts, lvl_fronts, lvl_ptr[nlev]);
+  rc |= upload(P, &P->ent_front, ent_front, nent);
+  rc |= upload(P, &P->ent_slot, ent_slot, nent);
+  rc |= upload(P, &P->ent_loc, ent_loc, nent);
+  int zero = 0;
+  rc |= upload(P, &P->npert, &zero, 1);
+  if (rc) { delete P; return 3; }
+  P->F = FrontsDev{dm, dp, doff, duoff, drp, drows, dcp, dci, dip, dinv};
+  for (int l = 0; l < nlev; ++l) {
+    Level L{};
+    L.nf = lvl_ptr[l + 1] - lvl_ptr[l];
+    L.fronts = P->lvl_fronts + lvl_ptr[l];
+    for (int k = lvl_ptr[l]; k < lvl_ptr[l + 1]; ++k) {
+      const int f = lvl_fronts[k];
+      L.mmax = std::max(L.mmax, m[f]);
+      L.pmax = std::max(L.pmax, p[f]);
+      L.cbmax = std::max(L.cbmax, m[f] - p[f]);
+    }
+    L.e0 = ent_lvl_ptr[l];
+    L.e1 = ent_lvl_ptr[l + 1];
+    P->levels.push_back(L);
+  }
+  *out = P;
+  return 0;
+}
+
+int sawfem_mf_destroy(void* plan) {
+  Plan* P = (Plan*)plan;
+  if (!P) return 0;
+  for (void* d : P->owned) (void)hipFree(d);
+  for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
+  delete P;
+  return 0;
+}
+
+int64_t sawfem_mf_arena_elems(void* plan) { return plan ? ((Plan*)plan)->arena_elems : -1; }
+int64_t sawfem_mf_uarena_elems(void* plan) { return plan ? ((Plan*)plan)->uarena_elems : -1; }
+
+// Number of tiny pivots replaced since the plan was created (or the counter last reset).
+int sawfem_mf_perturbed(void* plan, int reset) {
+  Plan* P = (Plan*)plan;
+  int h = 0;
+  if (!P) return -1;
+  if (hipMemcpy(&h, P->npert, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  if (reset) { int z = 0; (void)hipMemcpy(P->npert, &z, sizeof(int), hipMemcpyHostToDevice); }
+  return h;
+}
+
+// prec: 0 = complex64 fronts (mixed path), 1 = complex128 fronts (plain path).
+int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* omega, double beta, const void* Kaa,
+                     const void* Maa, const void* Daa, void* arena, int nb) {
+  Plan* P = (Plan*)plan;
+  if (!P || B <= 0 || !omega || !Kaa || !Maa || !arena || nb <= 0 || nb > 64 || (prec != 0 && prec != 1)) {
+    sawfem_set_error("sawfem_mf_factor: invalid arguments");
+    return 1;
+  }
+  if (prec == 1 && nb > 32) { sawfem_set_error("sawfem_mf_factor: nb > 32 with complex128"); return 1; }
+  hipStream_t s = (hipStream_t)stream;
+  return prec == 0 ? factor_impl<float>(P, s, B, omega, beta, Kaa, Maa, Daa, arena, nb)
+                   : factor_impl<double>(P, s, B, omega, beta, Kaa, Maa, Daa, arena, nb);
+}
+
+// x: [B][n] complex128 right-hand sides, overwritten with the solutions.
+int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x) {
+  Plan* P = (Plan*)plan;
+  if (!P || B <= 0 || !arena || !uarena || !x || (prec != 0 && prec != 1)) {
+    sawfem_set_error("sawfem_mf_solve: invalid arguments");
+    return 1;
+  }
+  if ((size_t)P->mmax * sizeof(double2) > 64 * 1024) {
+    sawfem_set_error("sawfem_mf_solve: front too large for the LDS solve");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  return prec == 0 ? solve_impl<float>(P, s, B, arena, uarena, x) : solve_impl<double>(P, s, B, arena, uarena, x);
+}
+
+// Per-kernel event timing: enable (1) / disable (0); turning it on clears what was recorded.
+int sawfem_mf_profile(void* plan, int enable) {
+  Plan* P = (Plan*)plan;
+  if (!P) { sawfem_set_error("sawfem_mf_profile: null plan"); return 1; }
+  P->profile = enable != 0;
+  P->ev_used = 0;
+  P->ev_marks.clear();
+  return 0;
+}
+
+// Waits for the recorded launches and writes, per kind (init, entries, diag, trsm, gemm-L,
+// gemm-S, fwd, bwd), total milliseconds to ms[kind] and launch counts to count[kind].
+int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count) {
+  Plan* P = (Plan*)plan;
+  if (!P || !ms || !count) { sawfem_set_error("sawfem_mf_profile_read: invalid arguments"); return 1; }
+  for (int k = 0; k < K_NKIND; ++k) { ms[k] = 0; count[k] = 0; }
+  for (auto& mk : P->ev_marks) {
+    SAW_HIP(hipEventSynchronize(P->ev_pool[mk.second + 1]));
+    float t = 0;
+    SAW_HIP(hipEventElapsedTime(&t, P->ev_pool[mk.second], P->ev_pool[mk.second + 1]));
+    ms[mk.first] += t;
+    count[mk.first] += 1;
+  }
+  P->ev_used = 0;
+  P->ev_marks.clear();
+  return 0;
+}
+
+int sawfem_rhs(void* stream, int n, int B, const double* omega, double beta, double V, const void* rK,
+               const void* rM, const void* rD, void* out) {
+  if (n <= 0 || B <= 0) { sawfem_set_error("sawfem_rhs: invalid arguments"); return 1; }
+  hipLaunchKernelGGL(k_rhs, dim3((n + 255) / 256, B), dim3(256), 0, (hipStream_t)stream, n, B, omega, beta, V,
+                     (const double2*)rK, (const double2*)rM, (const double2*)rD, (double2*)out);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_residual(void* stream, int n, int B, const int64_t* indptr, const int* indices, const void* Kaa,
+                    const void* Maa, const void* Daa, const double* omega, double beta, const void* x,
+                    const void* rhs, void* r) {
+  if (n <= 0 || B <= 0) { sawfem_set_error("sawfem_residual: invalid arguments"); return 1; }
+  hipLaunchKernelGGL(k_residual, dim3((n + 255) / 256, (B + 7) / 8), dim3(256), 0, (hipStream_t)stream, n, B, indptr,
+                     indices, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, omega, beta,
+                     (const double2*)x, (const double2*)rhs, (double2*)r);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, double kz_im, double V, const void* x,
+                  void* Q) {
+  if (n <= 0 || B <= 0) { sawfem_set_error("sawfem_charge: invalid arguments"); return 1; }
+  hipLaunchKernelGGL(k_charge, dim3(B), dim3(256), 0, (hipStream_t)stream, n, (const double2*)kts_a,
+                     make_double2(kz_re, kz_im), V, (const double2*)x, (double2*)Q);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_axpy(void* stream, int64_t n, void* x, const void* d) {
+  if (n < 0) { sawfem_set_error("sawfem_axpy: invalid arguments"); return 1; }
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n, (double2*)x,
+                     (const double2*)d);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,220 @@
+"""Host-side integer layout of the assembled and Bloch-reduced systems.
+
+Everything here is built once per cell on the host and uploaded as index arrays;
+the floating-point work that consumes it runs in the HIP kernels.
+
+DOF numbering follows the reference (``piezo_fem.py:11, 319-325``): component-major,
+``dof = comp * N + node``; ``nd = 3`` for the plane-strain ME0 system ``[ux|uy|phi]``
+(``piezo_fem.py:402-405``) and ``nd = 4`` for ME1 ``[ux|uy|uz|phi]``.
+
+Patterns:
+
+* ``K`` (and ``M``, ``D``) share one *structural* CSR pattern: every component pair of
+  every node pair that shares an element.  The reference builds K and M by COO -> CSR
+  (``piezo_fem.py:352-365``), which keeps explicit zeros, so its K pattern is this
+  structural one; its M keeps only the diagonal displacement blocks, which here are
+  slots holding exact zeros.
+* ``A_aa`` is ``(Tᵀ K T)[active, active]`` (``sweep.py:293-309``) on the structural
+  pattern of the merged active unknowns.  SciPy's SpGEMM drops exact-zero sums, so the
+  reference pattern is a value-dependent subset of this one; the extra slots hold 0.
+
+Gather lists (``ptr``, ``idx``) drive a deterministic, atomic-free reduction:
+``out[j] = sum(in[idx[k]] for k in ptr[j]:ptr[j+1])`` in list order.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+def _csr_from_sorted_keys(keys, nrows, ncols):
+    rows = keys // ncols
+    cols = keys % ncols
+    indptr = np.zeros(nrows + 1, dtype=np.int64)
+    np.add.at(indptr, rows + 1, 1)
+    return np.cumsum(indptr), cols.astype(np.int64)
+
+
+def _gather_lists(targets, n_out):
+    """Group source positions by target id (stable): ptr (n_out+1), idx (sources)."""
+    order = np.argsort(targets, kind='stable')
+    counts = np.bincount(targets, minlength=n_out)
+    ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return ptr, order.astype(np.int64)
+
+
+@dataclass
+class GatherList:
+    ptr: np.ndarray
+    idx: np.ndarray
+
+    @property
+    def n_out(self):
+        return self.ptr.size - 1
+
+
+class KLayout:
+    """Structural CSR pattern of K/M/D and the element -> slot gather (K1/K2 kernels).
+
+    Element matrices are written by the element kernel as ``Ke[e, a, b, i, j]`` (complex,
+    component blocks of 9x9 node entries) and ``Me[e, i, j]``.  The global matrices are
+    then gathered per *node pair* ``k`` (an entry of the node adjacency) from the list of
+    ``(e, i, j)`` codes ``e*81 + i*9 + j`` of the elements that couple that node pair.
+    """
+
+    def __init__(self, quads: np.ndarray, N: int, nd: int):
+        quads = np.asarray(quads, dtype=np.int64)
+        ne = quads.shape[0]
+        self.N, self.nd, self.ne = int(N), int(nd), int(ne)
+        ri = np.repeat(quads, 9, axis=1)                 # (ne,81) row node of local (i,j)
+        ci = np.tile(quads, (1, 9))                       # (ne,81) col node
+        keys = (ri * N + ci).ravel()
+        ukeys, inv = np.unique(keys, return_inverse=True)
+        self.node_ptr, self.node_idx = _csr_from_sorted_keys(ukeys, N, N)
+        self.nnz_node = int(ukeys.size)
+        self.pair_row = (ukeys // N).astype(np.int64)     # row node of each node pair
+        self.pair_col = (ukeys % N).astype(np.int64)
+        self.elem_pair = inv.reshape(ne, 81).astype(np.int64)   # node-pair slot per element entry
+        ptr, idx = _gather_lists(self.elem_pair.ravel(), self.nnz_node)
+        self.pair_gather = GatherList(ptr, idx)           # codes e*81 + i*9 + j
+        deg = np.diff(self.node_ptr)
+        self.deg = deg
+        # K slot of (a, i, b, j) = a*nd*nnz_node + nd*node_ptr[i] + b*deg[i] + (k - node_ptr[i])
+        self.nnz = nd * nd * self.nnz_node
+        self.gdof = nd * N
+
+    def slot(self, a, b, pair):
+        """K slot of component block (a, b) at node-pair slot(s) ``pair``."""
+        i = self.pair_row[pair]
+        return (a * self.nd * self.nnz_node + self.nd * self.node_ptr[i] + b * self.deg[i]
+                + (pair - self.node_ptr[i]))
+
+    def csr(self):
+        """(indptr, indices) of the gdof x gdof structural pattern."""
+        nd, N = self.nd, self.N
+        indptr = np.zeros(nd * N + 1, dtype=np.int64)
+        rowlen = nd * self.deg
+        indptr[1:] = np.cumsum(np.tile(rowlen, nd))
+        indices = np.empty(self.nnz, dtype=np.int64)
+        for a in range(nd):
+            for b in range(nd):
+                pair = np.arange(self.nnz_node)
+                indices[self.slot(a, b, pair)] = b * N + self.pair_col
+        return indptr, indices
+
+    def slot_rows_cols(self):
+        """Row and column dof of every K slot."""
+        nd, N = self.nd, self.N
+        rows = np.empty(self.nnz, dtype=np.int64)
+        cols = np.empty(self.nnz, dtype=np.int64)
+        pair = np.arange(self.nnz_node)
+        for a in range(nd):
+            for b in range(nd):
+                s = self.slot(a, b, pair)
+                rows[s] = a * N + self.pair_row
+                cols[s] = b * N + self.pair_col
+        return rows, cols
+
+
+def union_find_merge(gdof, groups):
+    """Representative map rep[d] for the Bloch merge (``sweep.py:97-144``).
+
+    Union-find with path halving; a slave's root is attached under its master's root,
+    so the first-listed master of a chain stays the representative.
+    Returns (rep, slaves, masters) with slaves sorted ascending.
+    """
+    parent = np.arange(gdof, dtype=np.int64)
+
+    def root(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+    members = []
+    for g in groups:
+        g = np.asarray(g, dtype=np.int64).ravel()
+        if g.size == 0:
+            continue
+        members.extend(int(x) for x in g)
+        m = int(g[0])
+        for s in g[1:]:
+            rs, rm = root(int(s)), root(m)
+            if rs != rm:
+                parent[rs] = rm
+    rep = np.arange(gdof, dtype=np.int64)
+    for x in set(members):
+        rep[x] = root(x)
+    slaves = np.flatnonzero(rep != np.arange(gdof))
+    return rep, slaves, rep[slaves]
+
+
+class BlochLayout:
+    """Active-set reduction ``A_aa = (Tᵀ A T)[active, active]`` as gathers over K slots.
+
+    Follows ``FrequencySolver.__init__`` (``sweep.py:283-314``) and ``solve``
+    (``sweep.py:335-364``):
+
+    * ``active`` = all dofs minus fix ∪ slaves ∪ zheng ∪ fu, ascending;
+    * ``aa``: A_aa slot <- K slots whose merged (row, col) are both active;
+    * ``rhs``: active row p <- K slots with merged row active[p] and merged column in
+      zheng (the ``Km @ s`` column, ``sweep.py:303``);
+    * ``kts``: charge weights on the *unmerged* K (``sweep.py:305``):
+      ``Q = sum_p kts_a[p] x[p] + V * kts_z``, where disp[slave] = disp[master]
+      and the zheng/fu values are written last (``sweep.py:356-362``).
+    """
+
+    def __init__(self, kl: KLayout, fix, zheng, fu, groups):
+        gdof = kl.gdof
+        self.gdof = gdof
+        self.zheng = np.asarray(zheng, dtype=np.int64)
+        self.fu = np.asarray(fu, dtype=np.int64)
+        rep, self.slaves, self.masters = union_find_merge(gdof, groups)
+        self.rep = rep
+        prescribed = np.unique(np.concatenate([np.asarray(fix, dtype=np.int64), self.slaves,
+                                               self.zheng, self.fu]))
+        self.active = np.setdiff1d(np.arange(gdof), prescribed)
+        n = self.n_active = int(self.active.size)
+        pos = np.full(gdof, -1, dtype=np.int64)
+        pos[self.active] = np.arange(n)
+        self.pos = pos
+        is_z = np.zeros(gdof, dtype=bool)
+        is_z[self.zheng] = True
+        is_f = np.zeros(gdof, dtype=bool)
+        is_f[self.fu] = True
+        rows, cols = kl.slot_rows_cols()
+        mr, mc = rep[rows], rep[cols]
+        pr, pc = pos[mr], pos[mc]
+        # A_aa
+        sel = np.flatnonzero((pr >= 0) & (pc >= 0))
+        keys = pr[sel] * n + pc[sel]
+        ukeys, inv = np.unique(keys, return_inverse=True)
+        self.aa_indptr, self.aa_indices = _csr_from_sorted_keys(ukeys, n, n)
+        self.nnz_aa = int(ukeys.size)
+        ptr, order = _gather_lists(inv, self.nnz_aa)
+        self.aa = GatherList(ptr, sel[order])
+        # rhs column: merged row active, merged column in zheng
+        sel = np.flatnonzero((pr >= 0) & is_z[mc])
+        ptr, order = _gather_lists(pr[sel], n)
+        self.rhs = GatherList(ptr, sel[order])
+        # charge weights on unmerged K: row in zheng
+        zr = is_z[rows]
+        sel = np.flatnonzero(zr & (pc >= 0) & ~is_z[cols] & ~is_f[cols])
+        tgt = pc[sel]
+        selz = np.flatnonzero(zr & is_z[cols])
+        tgt = np.concatenate([tgt, np.full(selz.size, n)])
+        ptr, order = _gather_lists(tgt, n + 1)
+        self.kts = GatherList(ptr, np.concatenate([sel, selz])[order])
+        # merged-node view of the active unknowns (supervariables for the ordering)
+        N = kl.N
+        self.node_of = self.active % N
+        self.comp_of = self.active // N
+
+    def expand(self, x, V=1.0):
+        """Full-length displacement/potential vector from the active solution (``sweep.py:356-360``)."""
+        disp = np.zeros(self.gdof, dtype=np.complex128)
+        disp[self.active] = x
+        disp[self.slaves] = disp[self.masters]
+        disp[self.zheng] = V
+        disp[self.fu] = 0.0
+        return disp

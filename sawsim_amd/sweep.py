@@ -1,0 +1,242 @@
+"""GPU frequency sweep: Bloch reduction, batched multifrontal LU, refinement and Y11
+(drop-in for SawSim ``saw2d/sweep.py``).
+
+Same entry points as the reference:
+
+* ``bc_periodic(model, lines, ..., ndisp)`` -> ``(fix, zheng, fu, merge_groups)``
+  (``sweep.py:41-72``),
+* ``FrequencySolver(K, M, fix, zheng, fu, merge_groups)`` with ``set_damping(D, beta)``,
+  ``solve(omega, V) -> (disp, Q, |Y|)`` and ``sweep(fre, V) -> (Q, |Y|)``
+  (``sweep.py:275-393``).
+
+What runs where:
+
+* host, once per cell: union-find merge, active set, gather lists, nested-dissection
+  symbolic analysis (``layout.py``, ``symbolic.py``, ``plan.py``);
+* GPU: Bloch gathers of K/M/D into the active system, RHS and charge weights (K3), the
+  batched multifrontal factorization of ``A(w) = K + i w beta D - w^2 M`` for a chunk of
+  frequencies, frontal triangular solves, FP64 residual + refinement and the charge
+  reduction ``Q = kts . disp`` (``sweep.py:342-363``).
+
+Precision modes:
+
+* ``'fp64'``: complex128 fronts (plain path), one refinement step;
+* ``'mixed'``: complex64 fronts with FP32 MFMA accumulation, refined against the FP64
+  operator until Q stops changing (relative ``ir_tol``); frequencies that stall are
+  refactored in complex128.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _lib
+from .layout import BlochLayout
+from .piezo_fem import DeviceCSR, Model, _torch, stream_ptr
+from .plan import build_mf_arrays
+from .symbolic import analyse
+
+
+def bc_periodic(model: Model, lines, tag_bottom=21, tag_left=19, tag_right=20, tag_zheng=17, tag_fu=18, ndisp=3):
+    """Bloch-periodic (zero phase) boundary sets of an SP cell (``sweep.py:41-72``).
+
+    Left/right nodes without the bottom corners are paired by ascending y; the pairs
+    merge all displacement components and the potential, the bottom corners merge the
+    potential only; bottom displacements are fixed.
+    """
+    N = model.N
+    nodes = lambda tag: model.nodes_with_tag(tag, lines)
+    bottom, left, right = nodes(tag_bottom), nodes(tag_left), nodes(tag_right)
+    zheng, fu = nodes(tag_zheng), nodes(tag_fu)
+    cl, cr = np.intersect1d(bottom, left), np.intersect1d(bottom, right)
+    L = np.setdiff1d(left, cl)
+    R = np.setdiff1d(right, cr)
+    L = L[np.argsort(model.pos_m[L, 1], kind='stable')]
+    R = R[np.argsort(model.pos_m[R, 1], kind='stable')]
+    if L.size != R.size:
+        raise RuntimeError('left/right periodic node counts differ')
+    masters = np.concatenate([L + a * N for a in range(ndisp + 1)] + [cl + ndisp * N])
+    slaves = np.concatenate([R + a * N for a in range(ndisp + 1)] + [cr + ndisp * N])
+    groups = [np.array([a, b]) for a, b in zip(masters, slaves)]
+    fix = np.concatenate([bottom + a * N for a in range(ndisp)])
+    return fix, zheng + ndisp * N, fu + ndisp * N, groups
+
+
+class FrequencySolver:
+    """Per-frequency solve of the Bloch-reduced piezoelectric system on one GPU."""
+
+    def __init__(self, K: DeviceCSR, M: DeviceCSR, fix, zheng, fu, merge_groups, *, node_xy=None, period=None,
+                 leaf_weight=96):
+        torch = _torch()
+        self.torch = torch
+        dev = self.dev = torch.device('cuda')
+        kl = self.layout = K.layout
+        if M.layout is not kl:
+            raise ValueError('K and M must share one layout')
+        self.gdof = kl.gdof
+        bl = self.bloch = BlochLayout(kl, fix, zheng, fu, merge_groups)
+        self.zheng, self.fu = bl.zheng, bl.fu
+        self.active, self.slaves, self.masters = bl.active, bl.slaves, bl.masters
+        n = self.n_active = bl.n_active
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+        self._aa = (t(bl.aa.ptr), t(bl.aa.idx))
+        self._rh = (t(bl.rhs.ptr), t(bl.rhs.idx))
+        self._kt = (t(bl.kts.ptr), t(bl.kts.idx))
+        self.aa_indptr = t(bl.aa_indptr.astype(np.int64))
+        self.aa_indices = t(bl.aa_indices.astype(np.int32))
+        z = lambda k: torch.empty(k, dtype=torch.complex128, device=dev)
+        self.Kaa, self.Maa, self.rK, self.rM = z(bl.nnz_aa), z(bl.nnz_aa), z(n), z(n)
+        self.Daa = self.rD = None
+        self.beta = 0.0
+        s = stream_ptr()
+        _lib.call('sawfem_gather3', s, bl.nnz_aa, self._aa[0].data_ptr(), self._aa[1].data_ptr(),
+                  K.values.data_ptr(), M.values.data_ptr(), None, self.Kaa.data_ptr(), self.Maa.data_ptr(), None)
+        _lib.call('sawfem_gather3', s, n, self._rh[0].data_ptr(), self._rh[1].data_ptr(),
+                  K.values.data_ptr(), M.values.data_ptr(), None, self.rK.data_ptr(), self.rM.data_ptr(), None)
+        kts = z(n + 1)
+        _lib.call('sawfem_gather3', s, n + 1, self._kt[0].data_ptr(), self._kt[1].data_ptr(),
+                  K.values.data_ptr(), None, None, kts.data_ptr(), None, None)
+        self.kts_a = kts[:n]
+        kz = complex(kts[n].item())
+        self.kts_z = kz
+        # symbolic analysis and numeric plan
+        t0 = time.time()
+        xy = node_xy if node_xy is not None else getattr(K, 'node_xy', None)
+        if xy is None:
+            raise ValueError('node coordinates are needed for the nested-dissection ordering')
+        xy = np.asarray(xy)
+        if period is None and len(merge_groups):
+            g = np.asarray(merge_groups[0]).ravel()
+            period = float(abs(xy[g[1] % kl.N, 0] - xy[g[0] % kl.N, 0])) or None
+        self.symbolic = analyse(bl.aa_indptr, bl.aa_indices, bl.node_of, bl.comp_of, xy,
+                                period=period, leaf_weight=leaf_weight)
+        self.mf = build_mf_arrays(self.symbolic, bl.aa_indptr, bl.aa_indices)
+        self.analysis_seconds = time.time() - t0
+        self._plan = self._create_plan()
+        self.stats = dict(self.symbolic.stats)
+
+    # ---------------------------------------------------------------- plan
+    def _create_plan(self):
+        import ctypes as C
+        mf = self.mf
+        h = C.c_void_p()
+        a = lambda x: np.ascontiguousarray(x).ctypes.data
+        keep = [np.ascontiguousarray(x) for x in (mf.m, mf.p, mf.rows_ptr, mf.rows, mf.child_ptr, mf.child_ids,
+                                                   mf.inv_ptr, mf.inv, mf.lvl_ptr, mf.lvl_fronts, mf.ent_front,
+                                                   mf.ent_slot, mf.ent_loc, mf.ent_lvl_ptr)]
+        _lib.call('sawfem_mf_create', C.byref(h), mf.n, mf.nf, *[k.ctypes.data for k in keep[:8]],
+                  mf.lvl_ptr.size - 1, keep[8].ctypes.data, keep[9].ctypes.data, mf.ent_front.size,
+                  keep[10].ctypes.data, keep[11].ctypes.data, keep[12].ctypes.data, keep[13].ctypes.data)
+        return h
+
+    def __del__(self):
+        try:
+            if getattr(self, '_plan', None) is not None:
+                _lib.raw('sawfem_mf_destroy', self._plan)
+                self._plan = None
+        except Exception:
+            pass
+
+    def set_damping(self, D: DeviceCSR, beta: float):
+        """Rayleigh stiffness damping A = K + i w beta D - w^2 M (``sweep.py:316-323``)."""
+        torch = self.torch
+        bl = self.bloch
+        self.Daa = torch.empty(bl.nnz_aa, dtype=torch.complex128, device=self.dev)
+        self.rD = torch.empty(self.n_active, dtype=torch.complex128, device=self.dev)
+        s = stream_ptr()
+        _lib.call('sawfem_gather3', s, bl.nnz_aa, self._aa[0].data_ptr(), self._aa[1].data_ptr(),
+                  D.values.data_ptr(), None, None, self.Daa.data_ptr(), None, None)
+        _lib.call('sawfem_gather3', s, self.n_active, self._rh[0].data_ptr(), self._rh[1].data_ptr(),
+                  D.values.data_ptr(), None, None, self.rD.data_ptr(), None, None)
+        self.beta = float(beta)
+
+    # ---------------------------------------------------------------- numeric
+    def arena_bytes(self, batch, precision):
+        return self.mf.arena_elems * batch * (8 if precision == 'mixed' else 16)
+
+    def _dptr(self, t):
+        return None if t is None else t.data_ptr()
+
+    def solve_batch(self, omegas, V=1.0, precision='mixed', ir_tol=1e-10, max_ir=8, nb=32, fallback=True):
+        """Solve all ``omegas`` in one batched factorization.
+
+        Returns dict with ``x`` (B, n) complex128 device tensor, ``Q`` (B,) complex numpy,
+        ``ir_steps``, ``dq`` (last relative change of Q per frequency), ``refactored``.
+        """
+        torch = self.torch
+        dev = self.dev
+        omegas = np.asarray(omegas, dtype=np.float64)
+        B, n = omegas.size, self.n_active
+        prec = 0 if precision == 'mixed' else 1
+        s = stream_ptr()
+        om = torch.as_tensor(omegas, device=dev)
+        arena = torch.empty(self.mf.arena_elems * B, dtype=torch.complex64 if prec == 0 else torch.complex128,
+                            device=dev)
+        uarena = torch.empty(max(1, self.mf.uarena_elems * B), dtype=torch.complex128, device=dev)
+        _lib.call('sawfem_mf_factor', self._plan, s, prec, B, om.data_ptr(), self.beta, self.Kaa.data_ptr(),
+                  self.Maa.data_ptr(), self._dptr(self.Daa), arena.data_ptr(), nb)
+        rhs = torch.empty((B, n), dtype=torch.complex128, device=dev)
+        _lib.call('sawfem_rhs', s, n, B, om.data_ptr(), self.beta, float(V), self.rK.data_ptr(), self.rM.data_ptr(),
+                  self._dptr(self.rD), rhs.data_ptr())
+        x = rhs.clone()
+        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), x.data_ptr())
+        Qd = torch.empty(B, dtype=torch.complex128, device=dev)
+        charge = lambda: (_lib.call('sawfem_charge', s, n, B, self.kts_a.data_ptr(), self.kts_z.real,
+                                    self.kts_z.imag, float(V), x.data_ptr(), Qd.data_ptr()),
+                          Qd.cpu().numpy().copy())[1]
+        Q = charge()
+        r = torch.empty_like(x)
+        steps = 0
+        dq = np.full(B, np.inf)
+        for steps in range(1, max_ir + 1):
+            _lib.call('sawfem_residual', s, n, B, self.aa_indptr.data_ptr(), self.aa_indices.data_ptr(),
+                      self.Kaa.data_ptr(), self.Maa.data_ptr(), self._dptr(self.Daa), om.data_ptr(), self.beta,
+                      x.data_ptr(), rhs.data_ptr(), r.data_ptr())
+            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr())
+            _lib.call('sawfem_axpy', s, B * n, x.data_ptr(), r.data_ptr())
+            Qn = charge()
+            dq = np.abs(Qn - Q) / np.maximum(np.abs(Qn), 1e-300)
+            Q = Qn
+            if np.all(dq <= ir_tol) or (prec == 1 and steps >= 1 and np.all(dq <= 1e-13)):
+                break
+        refactored = np.zeros(B, dtype=bool)
+        if prec == 0 and fallback:
+            bad = np.flatnonzero(~(dq <= ir_tol))
+            if bad.size:
+                del arena, uarena
+                sub = self.solve_batch(omegas[bad], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir, nb=nb)
+                x[self.torch.as_tensor(bad, device=self.dev)] = sub['x']
+                Q[bad] = sub['Q']
+                dq[bad] = sub['dq']
+                refactored[bad] = True
+        return dict(x=x, Q=Q, ir_steps=steps, dq=dq, refactored=refactored)
+
+    def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, **kw):
+        """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies."""
+        fre = np.asarray(fre, dtype=np.float64)
+        batch = batch or fre.size
+        Q = np.empty(fre.size, dtype=complex)
+        info = []
+        for i0 in range(0, fre.size, batch):
+            res = self.solve_batch(2 * np.pi * fre[i0:i0 + batch], V, precision=precision, **kw)
+            Q[i0:i0 + batch] = res['Q']
+            info.append(dict(ir_steps=res['ir_steps'], dq_max=float(np.max(res['dq'])),
+                             refactored=int(res['refactored'].sum())))
+            del res
+        return Q, info
+
+    def expand(self, x, V=1.0):
+        return self.bloch.expand(np.asarray(x), V)
+
+    def solve(self, omega: float, V: float = 1.0):
+        """(disp, Q, |iωQ/V|) for one frequency (``sweep.py:335-364``), FP64 path."""
+        res = self.solve_batch([omega], V, precision='fp64')
+        disp = self.expand(res['x'][0].cpu().numpy(), V)
+        Q = complex(res['Q'][0])
+        return disp, Q, abs(1j * omega * Q / V)
+
+    def sweep(self, fre, V: float = 1.0, n_jobs: int = 1, progress: bool = False, precision='mixed', batch=None):
+        """(Q, |Y|) over frequencies ``fre`` in Hz (``sweep.py:366-393``); all points batched."""
+        Q, _ = self.sweep_charges(fre, V, precision=precision, batch=batch)
+        return Q, np.abs(1j * 2 * np.pi * np.asarray(fre) * Q / V)

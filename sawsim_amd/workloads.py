@@ -1,0 +1,146 @@
+"""Synthetic periodic SAW cells used by the benchmark and the parity tests.
+
+Each builder returns a :class:`Cell`: the mesh (reference ``Model`` + line tags),
+materials (already rotated and, for plane strain, projected), per-element material
+index, PML regions and the sweep band.  The layer/PML/material assignment follows
+``models.py:53-72`` and ``sp_meshes/__init__.py:27-41`` of the reference.
+
+Workloads:
+
+* ``config1``: sp_stack-style structured Q9 cell, pitch 1.085 µm, Al electrode
+  0.08·pitch thick, metallization 0.5, 0.6 µm trigonal-3m piezo film (sp_baseline
+  constants with seeded ±5 % jitter, Euler (0, -42°, 0)) over isotropic Si
+  (E = 130 GPa, ν = 0.28, ρ = 2330), depth 5·pitch plus a 1·pitch bottom PML in Si,
+  plane strain ``[ux|uy|phi]`` (mesh 0.045 µm: 4312 elements, 17521 nodes).
+* ``config2``: the same stack with a 0.3·pitch SiO₂ overlay embedding the electrodes,
+  Euler (0, -38°, 0), mesh refined 2x in each direction (16848 elements).
+* ``config3``: config 1 with Rayleigh stiffness damping β = 1e-13 s and dielectric
+  loss tanδ = 1e-3 on the piezo.
+* ``double_layer``: the reference ``sp_double_layer`` template (ME1) whose |Y| curve
+  the reference stores in ``tests/data/sp_double_layer.npz``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import materials as mt
+from .mesh import CellMesh, double_layer_cell, quad_layer_cell, stack_cell, stack_tag_map
+from .piezo_fem import Model, PMLRegion, material_index
+
+SP_BASELINE_3M = dict(C11=232.966, C12=46.8904, C13=80.2342, C14=-11.0267, C33=275.364, C44=93.8995,
+                      e15=2.59576, e22=1.58923, e31=0.0821598, e33=1.88197, eps11=40.9, eps33=43.3)
+PIEZO_KEYS = ['C11', 'C12', 'C13', 'C14', 'C33', 'C44', 'e15', 'e22', 'e31', 'e33', 'eps11', 'eps33']
+
+
+@dataclass
+class Cell:
+    mesh: CellMesh
+    model: Model
+    materials: list
+    mat_idx: np.ndarray
+    pml: list
+    ndisp: int
+    band_hz: tuple
+    points: int
+    beta_dk: float = 0.0
+    eta_eps: float = 0.0
+    info: dict = field(default_factory=dict)
+
+    @property
+    def nd(self):
+        return self.ndisp + 1
+
+    def frequencies(self, points=None):
+        return np.linspace(self.band_hz[0], self.band_hz[1], points or self.points)
+
+
+def jittered_constants(seed=0, rel=0.05):
+    rng = np.random.default_rng(seed)
+    return {k: v * (1.0 + rng.uniform(-rel, rel)) for k, v in SP_BASELINE_3M.items()}
+
+
+def _al():
+    # legacy Al record: E = 70 GPa, nu = 0.33, rho = 2700, eps = eps0 (8.854e-12) I
+    return mt.isotropic(70e9, 0.33, 2700.0, 1.0, eps0=mt.EPS0_LEGACY)
+
+
+def piezo_material(constants, euler_deg, eta_eps=0.0, plane='xy', rho=7450.0):
+    m = mt.trigonal_3m(constants, rho)
+    C, e, eps = mt.rotate(euler_deg, m.C, m.e, m.eps)
+    m = mt.Material(rho, C, e, eps)
+    if plane:
+        m = mt.project(m, plane)
+    if eta_eps:
+        m = mt.Material(m.rho, m.C, m.e, m.eps * (1 - 1j * eta_eps))
+    return m
+
+
+def _stack(mesh, mats, plane, ndisp, band, points, beta_dk=0.0, eta_eps=0.0, info=None):
+    model = Model(mesh.pos_m, mesh.quads, mesh.elem_tag, mesh.N)
+    tag_map = stack_tag_map(mesh) if 'tag_backing' in mesh.info else mesh.info['tag_map']
+    bottom = float(mesh.pos_m[mesh.nodes_with_tag(21), 1].min())
+    pml = [PMLRegion(tag=mesh.info['tag_pml'], kind='y', yp=bottom, ya=bottom + mesh.info['pml_thickness_m'])]
+    if plane:
+        mats = [mats[0]] + [mt.project(m, plane) for m in mats[1:]]
+    return Cell(mesh=mesh, model=model, materials=mats, mat_idx=material_index(model, tag_map), pml=pml,
+                ndisp=ndisp, band_hz=band, points=points, beta_dk=beta_dk, eta_eps=eta_eps, info=info or {})
+
+
+def config1(seed=0, mesh_um=0.045, points=101, band=(1.63e9, 1.915e9), beta_dk=0.0, eta_eps=0.0, euler=(0, -42, 0)):
+    pitch = 1.085
+    mesh = stack_cell(pitch, 0.5, 0.08 * pitch, 0.6, [5 * pitch - 0.6], [], mesh_um, pml_pitches=1.0)
+    k = jittered_constants(seed)
+    mats = [piezo_material(k, euler, eta_eps), _al(), mt.isotropic(130e9, 0.28, 2330.0, 11.7)]
+    return _stack(mesh, mats, 'xy', 2, band, points, beta_dk, eta_eps,
+                  info=dict(name='config1', constants=k, euler=euler, seed=seed))
+
+
+def config2(seed=0, mesh_um=0.0225, points=401, band=(1.55e9, 1.85e9)):
+    pitch = 1.085
+    mesh = stack_cell(pitch, 0.5, 0.08 * pitch, 0.6, [5 * pitch - 0.6], [0.3 * pitch], mesh_um, pml_pitches=1.0)
+    k = jittered_constants(seed)
+    euler = (0, -38, 0)
+    mats = [piezo_material(k, euler), _al(), mt.isotropic(130e9, 0.28, 2330.0, 11.7),
+            mt.isotropic(72e9, 0.17, 2200.0, 4.2)]
+    return _stack(mesh, mats, 'xy', 2, band, points, info=dict(name='config2', constants=k, euler=euler, seed=seed))
+
+
+def config3(seed=0, mesh_um=0.045, points=201, band=(1.63e9, 1.915e9)):
+    c = config1(seed, mesh_um, points, band, beta_dk=1e-13, eta_eps=1e-3)
+    c.info['name'] = 'config3'
+    return c
+
+
+def legacy_record_materials(layers):
+    """sp_baseline (rotated 0, -42, 0), Al and builtin layer records used by the stored curves."""
+    C = np.array([[232966000000.0, 46890400000.0, 80234200000.0, -11026700000.0, 0, 0],
+                  [46890400000.0, 232966000000.0, 80234200000.0, 11026700000.0, 0, 0],
+                  [80234200000.0, 80234200000.0, 275364000000.0, 0, 0, 0],
+                  [-11026700000.0, 11026700000.0, 0, 93899500000.0, 0, 0],
+                  [0, 0, 0, 0, 93899500000.0, -11026700000.0],
+                  [0, 0, 0, 0, -11026700000.0, 93038000000.0]])
+    e = np.array([[0, 0, 0, 0, 2.59576, -1.58923], [-1.58923, 1.58923, 0, 2.59576, 0, 0],
+                  [0.0821598, 0.0821598, 1.88197, 0, 0, 0]])
+    eps = np.diag([3.6212859999999994e-10, 3.6212859999999994e-10, 3.833781999999999e-10])
+    Cr, er, epsr = mt.rotate((0, -42, 0), C, e, eps)
+    lib = dict(si_isotropic=mt.isotropic(170e9, 0.28, 2329.0, 11.7, eps0=mt.EPS0_LEGACY),
+               sio2=mt.isotropic(70e9, 0.17, 2200.0, 4.2, eps0=mt.EPS0_LEGACY),
+               polysi=mt.isotropic(169e9, 0.22, 2320.0, 4.5, eps0=mt.EPS0_LEGACY))
+    return [mt.Material(7450.0, Cr, er, epsr), _al()] + [lib[x] for x in layers]
+
+
+def double_layer(points=5, band=(1.5e9, 2.7e9)):
+    mesh = double_layer_cell()
+    mats = legacy_record_materials(['si_isotropic'])
+    return _stack(mesh, mats, None, 3, band, points, info=dict(name='sp_double_layer'))
+
+
+def quad_layer(points=5, band=(1.5e9, 2.7e9)):
+    mesh = quad_layer_cell()
+    mats = legacy_record_materials(['sio2', 'polysi', 'si_isotropic'])
+    return _stack(mesh, mats, None, 3, band, points, info=dict(name='sp_quad_layer'))
+
+
+WORKLOADS = dict(config1=config1, config2=config2, config3=config3, double_layer=double_layer, quad_layer=quad_layer)

@@ -1,0 +1,109 @@
+"""NumPy emulation of the multifrontal kernels' data flow (CPU test helper).
+
+Executes, front by front and level by level, the same operations the HIP kernels of
+``sawsim_amd/csrc/mf.hip`` perform, on a flat arena with the same offsets, the same
+extend-add maps, entry offsets and blocked panel order (diagonal LU, panel solves,
+L-shaped update, Schur complement).  It checks the host plan arrays and the blocked
+algorithm on the CPU; it is not used by the product path.
+"""
+import numpy as np
+
+
+def factor(mf, B, omegas, beta, Kaa, Maa, Daa, nb=32, dtype=np.complex128):
+    off = mf.off
+    arena = np.zeros(mf.arena_elems * B, dtype=dtype)
+
+    def front(f, b):
+        m = int(mf.m[f])
+        base = off[f] * B + b * m * m
+        return arena[base:base + m * m].reshape(m, m, order='F')
+    for L in range(mf.lvl_ptr.size - 1):
+        fronts = mf.lvl_fronts[mf.lvl_ptr[L]:mf.lvl_ptr[L + 1]]
+        e0, e1 = mf.ent_lvl_ptr[L], mf.ent_lvl_ptr[L + 1]
+        for b in range(B):
+            w = omegas[b]
+            for f in fronts:
+                Fm = front(f, b)
+                Fm[:] = 0
+                for slot in range(mf.child_ptr[f], mf.child_ptr[f + 1]):
+                    ch = mf.child_ids[slot]
+                    inv = mf.inv[mf.inv_ptr[slot]:mf.inv_ptr[slot + 1]]
+                    Fc = front(ch, b)
+                    pc = mf.p[ch]
+                    sel = np.flatnonzero(inv >= 0)
+                    Fm[np.ix_(sel, sel)] += Fc[pc + inv[sel][:, None], pc + inv[sel][None, :]]
+            s = mf.ent_slot[e0:e1]
+            a = Kaa[s] - w * w * Maa[s]
+            if Daa is not None:
+                a = a + 1j * w * beta * Daa[s]
+            for k in range(e0, e1):
+                f = mf.ent_front[k]
+                Fm = front(f, b)
+                loc = mf.ent_loc[k]
+                m = mf.m[f]
+                Fm[loc % m, loc // m] += a[k - e0]
+            for f in fronts:
+                Fm = front(f, b)
+                m, p = int(mf.m[f]), int(mf.p[f])
+                for k0 in range(0, p, nb):
+                    k1 = min(p, k0 + nb)
+                    D = Fm[k0:k1, k0:k1]
+                    for k in range(k1 - k0):
+                        D[k + 1:, k] /= D[k, k]
+                        D[k + 1:, k + 1:] -= np.outer(D[k + 1:, k], D[k, k + 1:])
+                    Lk = np.tril(D, -1) + np.eye(k1 - k0)
+                    Uk = np.triu(D)
+                    Fm[k1:, k0:k1] = np.linalg.solve(Uk.T, Fm[k1:, k0:k1].T).T
+                    Fm[k0:k1, k1:] = np.linalg.solve(Lk, Fm[k0:k1, k1:])
+                    upd = Fm[k1:, k0:k1] @ Fm[k0:k1, k1:]
+                    mask = np.ones((m - k1, m - k1), dtype=bool)
+                    mask[max(0, p - k1):, max(0, p - k1):] = False
+                    Fm[k1:, k1:] -= np.where(mask, upd, 0)
+                if p < m and p > 0:
+                    Fm[p:, p:] -= Fm[p:, :p] @ Fm[:p, p:]
+    return arena
+
+
+def solve(mf, B, arena, x):
+    """x: (B, n) right-hand sides, solved in place (forward then backward, as k_fwd/k_bwd)."""
+    off, uoff = mf.off, mf.uoff
+    u = np.zeros(mf.uarena_elems * B, dtype=np.complex128)
+
+    def front(f, b):
+        m = int(mf.m[f])
+        base = off[f] * B + b * m * m
+        return arena[base:base + m * m].reshape(m, m, order='F')
+    nlev = mf.lvl_ptr.size - 1
+    for L in range(nlev):
+        for f in mf.lvl_fronts[mf.lvl_ptr[L]:mf.lvl_ptr[L + 1]]:
+            m, p = int(mf.m[f]), int(mf.p[f])
+            rows = mf.rows[mf.rows_ptr[f]:mf.rows_ptr[f + 1]]
+            for b in range(B):
+                Fm = front(f, b).astype(np.complex128)
+                w = np.zeros(m, dtype=np.complex128)
+                w[:p] = x[b, rows[:p]]
+                for slot in range(mf.child_ptr[f], mf.child_ptr[f + 1]):
+                    ch = mf.child_ids[slot]
+                    inv = mf.inv[mf.inv_ptr[slot]:mf.inv_ptr[slot + 1]]
+                    mc = int(mf.m[ch] - mf.p[ch])
+                    uc = u[uoff[ch] * B + b * mc: uoff[ch] * B + (b + 1) * mc]
+                    sel = inv >= 0
+                    w[sel] += uc[inv[sel]]
+                for k in range(p):
+                    w[k + 1:p] -= Fm[k + 1:p, k] * w[k]
+                uf = w[p:] - Fm[p:, :p] @ w[:p]
+                u[uoff[f] * B + b * (m - p): uoff[f] * B + (b + 1) * (m - p)] = uf
+                x[b, rows[:p]] = w[:p]
+    for L in range(nlev - 1, -1, -1):
+        for f in mf.lvl_fronts[mf.lvl_ptr[L]:mf.lvl_ptr[L + 1]]:
+            m, p = int(mf.m[f]), int(mf.p[f])
+            rows = mf.rows[mf.rows_ptr[f]:mf.rows_ptr[f + 1]]
+            for b in range(B):
+                Fm = front(f, b).astype(np.complex128)
+                z = x[b, rows].copy()
+                z[:p] -= Fm[:p, p:] @ z[p:]
+                for k in range(p - 1, -1, -1):
+                    z[k] = z[k] / Fm[k, k]
+                    z[:k] -= Fm[:k, k] * z[k]
+                x[b, rows[:p]] = z[:p]
+    return x

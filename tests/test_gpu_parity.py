@@ -1,0 +1,158 @@
+"""GPU parity: HIP assembly, Bloch reduction, multifrontal LU and Y11 against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers
+import mf_emulator
+from sawsim_amd import workloads
+from sawsim_amd.piezo_fem import assemble, assemble_ME0, assemble_device
+from sawsim_amd.sweep import FrequencySolver, bc_periodic
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _gpu_assemble(cell, omega_top, damping=False):
+    fn = assemble if cell.ndisp == 3 else assemble_ME0
+    if not damping:
+        return fn(cell.model, cell.materials, cell.mat_idx, cell.pml, omega_top)
+    nd = cell.ndisp + 1
+    from sawsim_amd.materials import lift_projected
+    mats = cell.materials if nd == 4 else [lift_projected(m) for m in cell.materials]
+    mask = np.zeros(len(mats), dtype=bool)
+    mask[0] = True
+    return assemble_device(cell.model, mats, cell.mat_idx, cell.pml, omega_top, nd, damping_mask=mask)
+
+
+def _engine(cell, omega_top, **kw):
+    K, M, D = _gpu_assemble(cell, omega_top, damping=True)
+    bcs = bc_periodic(cell.model, cell.mesh.lines, ndisp=cell.ndisp)
+    eng = FrequencySolver(K, M, *bcs, **kw)
+    if cell.beta_dk:
+        eng.set_damping(D, cell.beta_dk)
+    return eng
+
+
+CELLS = {
+    'double_layer': lambda: workloads.double_layer(),
+    'config1_coarse': lambda: workloads.config1(mesh_um=0.09, points=5),
+    'config3_coarse': lambda: workloads.config3(mesh_um=0.09, points=5),
+}
+
+
+@pytest.mark.parametrize('name', list(CELLS))
+def test_assembly_matches_oracle(name):
+    _say(f'[test_assembly_matches_oracle] {name}')
+    cell = CELLS[name]()
+    wtop = 2 * np.pi * cell.band_hz[1]
+    K, M, D = _gpu_assemble(cell, wtop, damping=True)
+    Ko, Mo = helpers.oracle_assemble(cell, wtop)
+    Do, _ = helpers.oracle_assemble(cell, wtop, helpers.oracle_damping_materials(cell))
+    for gpu, ref in ((K, Ko), (M, Mo), (D, Do)):
+        g = gpu.to_scipy()
+        assert g.shape == ref.shape
+        # oracle pattern is contained in the structural pattern; extra slots are exact zeros
+        diff = (g - ref).tocsr()
+        scale = abs(ref).max()
+        assert abs(diff).max() <= 1e-12 * scale
+        gz = g.copy(); gz.data[:] = 1
+        rz = ref.copy(); rz.data[:] = 1
+        assert (rz - rz.multiply(gz)).nnz == 0
+
+
+@pytest.mark.parametrize('name', list(CELLS))
+def test_bloch_maps_and_active_matrix(name):
+    _say(f'[test_bloch_maps_and_active_matrix] {name}')
+    cell = CELLS[name]()
+    wtop = 2 * np.pi * cell.band_hz[1]
+    eng = _engine(cell, wtop)
+    ref = helpers.oracle_engine(cell, wtop)
+    assert np.array_equal(eng.active, ref.active)
+    o = np.argsort(ref.slaves)
+    assert np.array_equal(eng.slaves, ref.slaves[o]) and np.array_equal(eng.masters, ref.masters[o])
+    r = np.repeat(np.arange(eng.n_active), np.diff(eng.bloch.aa_indptr))
+    c = eng.bloch.aa_indices
+    Kaa = eng.Kaa.cpu().numpy()
+    Kref = np.asarray(ref.Km_aa[r, c]).ravel()
+    assert np.abs(Kaa - Kref).max() <= 1e-12 * np.abs(Kref).max()
+    # every oracle nonzero is a slot of ours
+    assert ref.Km_aa.nnz <= eng.bloch.nnz_aa
+    kts = np.concatenate([eng.kts_a.cpu().numpy(), [eng.kts_z]])
+    disp = np.zeros(ref.gdof, dtype=complex)
+    x = np.random.default_rng(0).standard_normal(eng.n_active) + 0j
+    Qo = ref.kts @ ref.expand(x)
+    Qg = kts[:-1] @ x + kts[-1]
+    assert abs(Qg - Qo) <= 1e-12 * abs(Qo) + 1e-30
+
+
+@pytest.mark.parametrize('prec', ['fp64', 'mixed'])
+def test_factor_arena_matches_emulation(prec):
+    _say(f'[test_factor_arena_matches_emulation] {prec}')
+    import torch
+    cell = workloads.double_layer()
+    wtop = 2 * np.pi * cell.band_hz[1]
+    eng = _engine(cell, wtop, leaf_weight=60)
+    omegas = 2 * np.pi * np.array([1.7e9, 1.85e9, 2.2e9])
+    B = omegas.size
+    from sawsim_amd import _lib
+    from sawsim_amd.piezo_fem import stream_ptr
+    pr = 0 if prec == 'mixed' else 1
+    arena = torch.empty(eng.mf.arena_elems * B, dtype=torch.complex64 if pr == 0 else torch.complex128, device='cuda')
+    om = torch.as_tensor(omegas, device='cuda')
+    _lib.call('sawfem_mf_factor', eng._plan, stream_ptr(), pr, B, om.data_ptr(), 0.0, eng.Kaa.data_ptr(),
+              eng.Maa.data_ptr(), None, arena.data_ptr(), 16)
+    got = arena.cpu().numpy().astype(np.complex128)
+    ref = mf_emulator.factor(eng.mf, B, omegas, 0.0, eng.Kaa.cpu().numpy(), eng.Maa.cpu().numpy(), None, nb=16)
+    off = eng.mf.off
+    tol = 1e-10 if pr == 1 else 2e-3
+    for f in range(eng.mf.nf):
+        m = int(eng.mf.m[f])
+        a = got[off[f] * B: off[f] * B + B * m * m]
+        b = ref[off[f] * B: off[f] * B + B * m * m]
+        err = np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+        assert err <= tol, (f, m, int(eng.mf.p[f]), err)
+
+
+@pytest.mark.parametrize('name', list(CELLS))
+def test_y11_fp64_and_mixed_vs_oracle(name):
+    _say(f'[test_y11_fp64_and_mixed_vs_oracle] {name}')
+    cell = CELLS[name]()
+    wtop = 2 * np.pi * cell.band_hz[1]
+    eng = _engine(cell, wtop)
+    ref = helpers.oracle_engine(cell, wtop)
+    f = cell.frequencies(5)
+    Q64, info64 = eng.sweep_charges(f, precision='fp64')
+    Qmx, infomx = eng.sweep_charges(f, precision='mixed')
+    Qo = np.array([helpers.refined_charge(ref, 2 * np.pi * fz)[0] for fz in f])
+    e64 = np.abs(Q64 - Qo) / np.abs(Qo)
+    emx = np.abs(Qmx - Q64) / np.abs(Q64)
+    _say(name, 'fp64 vs oracle', e64.max(), 'mixed vs fp64', emx.max(), info64, infomx)
+    assert e64.max() <= 1e-8
+    assert emx.max() <= 1e-6
+
+
+def test_double_layer_reproduces_stored_curve():
+    """|Y| of the reference's stored sp_double_layer curve at its test frequencies."""
+    _say('[test_double_layer_reproduces_stored_curve]')
+    import json
+    from pathlib import Path
+    data = Path(__file__).parent / 'data'
+    meta = json.loads((data / 'sp_double_layer.json').read_text())
+    d = np.load(data / 'sp_double_layer.npz')
+    idx = np.array(meta['test_indices'])
+    f = d['frequency_hz'][idx]
+    cell = workloads.double_layer(points=len(f), band=(f[0], f[-1]))
+    eng = _engine(cell, 2 * np.pi * f[-1])
+    for prec in ('fp64', 'mixed'):
+        Q, _ = eng.sweep_charges(f, precision=prec)
+        y = -1j * 2 * np.pi * f * Q
+        err = np.abs(np.abs(y) - d['sawsim_magnitude'][idx]) / d['sawsim_magnitude'][idx]
+        _say(prec, err)
+        assert err.max() <= meta['regression_rtol']
+        assert err.max() <= 1e-6

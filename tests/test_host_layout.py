@@ -1,0 +1,108 @@
+"""Host integer work of the product path against the oracle (CPU)."""
+import numpy as np
+import pytest
+
+import helpers
+import mf_emulator
+from oracle import sweep as osw
+from sawsim_amd import materials as mt, workloads
+from sawsim_amd.layout import BlochLayout, KLayout
+from sawsim_amd.piezo_fem import Model
+from sawsim_amd.plan import build_mf_arrays, kernel_work
+from sawsim_amd.symbolic import analyse
+from sawsim_amd.sweep import bc_periodic
+
+
+def _cells():
+    return [workloads.double_layer(), workloads.config1(mesh_um=0.12, points=3)]
+
+
+@pytest.mark.parametrize('cell', _cells(), ids=['double_layer', 'config1_coarse'])
+def test_structural_pattern_and_bloch_maps_match_oracle(cell):
+    m = cell.model
+    nd = cell.ndisp + 1
+    kl = KLayout(m.quads, m.N, nd)
+    K, M = helpers.oracle_assemble(cell, 2 * np.pi * cell.band_hz[1])
+    indptr, indices = kl.csr()
+    # the reference's COO -> CSR keeps every element slot: identical structural pattern
+    assert np.array_equal(indptr, K.indptr) and np.array_equal(indices, K.indices)
+    bcs = bc_periodic(m, cell.mesh.lines, ndisp=cell.ndisp)
+    obcs = osw.bc_periodic(m.pos_m, cell.mesh.lines, m.N, ndisp=cell.ndisp)
+    for a, b in zip(bcs[:3], obcs[:3]):
+        assert np.array_equal(a, b)
+    assert all(np.array_equal(g, h) for g, h in zip(bcs[3], obcs[3]))
+    bl = BlochLayout(kl, *bcs)
+    eng = osw.FrequencySolver(K, M, *obcs)
+    assert np.array_equal(bl.active, eng.active)
+    o = np.argsort(eng.slaves)
+    assert np.array_equal(bl.slaves, eng.slaves[o]) and np.array_equal(bl.masters, eng.masters[o])
+    # oracle Km_aa (SpGEMM drops exact zeros) is contained in the structural active pattern
+    r = np.repeat(np.arange(bl.n_active), np.diff(bl.aa_indptr))
+    ours = set(zip(r.tolist(), bl.aa_indices.tolist()))
+    ref = eng.Km_aa.tocoo()
+    assert set(zip(ref.row.tolist(), ref.col.tolist())) <= ours
+    # gathers reproduce Km_aa, Km@s, kts on the host
+    Kv = K.data
+    Kaa = np.array([Kv[bl.aa.idx[bl.aa.ptr[j]:bl.aa.ptr[j + 1]]].sum() for j in range(bl.nnz_aa)])
+    assert np.abs(Kaa - np.asarray(eng.Km_aa[r, bl.aa_indices]).ravel()).max() <= 1e-12 * np.abs(Kaa).max()
+    rK = np.array([Kv[bl.rhs.idx[bl.rhs.ptr[j]:bl.rhs.ptr[j + 1]]].sum() for j in range(bl.n_active)])
+    assert np.allclose(rK, eng.Km_s[eng.active], rtol=1e-12, atol=1e-12 * np.abs(rK).max())
+    kt = np.array([Kv[bl.kts.idx[bl.kts.ptr[j]:bl.kts.ptr[j + 1]]].sum() for j in range(bl.n_active + 1)])
+    x = np.random.default_rng(3).standard_normal(bl.n_active) + 1j
+    assert np.isclose(kt[:-1] @ x + kt[-1], eng.kts @ eng.expand(x), rtol=1e-12)
+
+
+def test_product_materials_match_oracle():
+    from oracle import materials as om
+    k = workloads.jittered_constants(5)
+    m = mt.trigonal_3m(k, 7450.0)
+    C, e, eps = om.crystal_tensors('trigonal_3m', k)
+    assert np.allclose(m.C, C, rtol=1e-14) and np.allclose(m.e, e) and np.allclose(m.eps, eps)
+    for euler in [(0, -42, 0), (10, -38, 44)]:
+        got = mt.rotate(euler, m.C, m.e, m.eps)
+        ref = om.rotate(euler, C, e, eps)
+        for g, r in zip(got, ref):
+            assert np.allclose(g, r, rtol=1e-12, atol=1e-12 * np.abs(r).max())
+    iso = mt.isotropic(130e9, 0.28, 2330.0)
+    assert np.allclose(iso.C, om.isotropic_C(130e9, 0.28), rtol=1e-14)
+
+
+@pytest.mark.parametrize('nb', [8, 32])
+def test_plan_arrays_drive_a_correct_factorization(nb):
+    """The exact data flow of the HIP kernels, emulated in NumPy, solves the system."""
+    import scipy.sparse.linalg as spla
+    cell = workloads.double_layer()
+    m = cell.model
+    wtop = 2 * np.pi * cell.band_hz[1]
+    eng = helpers.oracle_engine(cell, wtop)
+    kl = KLayout(m.quads, m.N, 4)
+    bl = BlochLayout(kl, *bc_periodic(m, cell.mesh.lines, ndisp=3))
+    sym = analyse(bl.aa_indptr, bl.aa_indices, bl.node_of, bl.comp_of, m.pos_m, period=2.17e-6, leaf_weight=60)
+    mf = build_mf_arrays(sym, bl.aa_indptr, bl.aa_indices)
+    r = np.repeat(np.arange(bl.n_active), np.diff(bl.aa_indptr))
+    Kaa = np.asarray(eng.Km_aa[r, bl.aa_indices]).ravel()
+    Maa = np.asarray(eng.Mm_aa[r, bl.aa_indices]).ravel()
+    om_ = 2 * np.pi * np.array([1.7e9, 2.4e9])
+    arena = mf_emulator.factor(mf, 2, om_, 0.0, Kaa, Maa, None, nb=nb)
+    rhs = np.stack([eng.system(w)[1] for w in om_])
+    x = mf_emulator.solve(mf, 2, arena, rhs.copy())
+    for b, w in enumerate(om_):
+        A, _ = eng.system(w)
+        assert np.linalg.norm(A @ x[b] - rhs[b]) <= 1e-12 * np.linalg.norm(rhs[b])
+    work = kernel_work(mf, nb)
+    assert 0 < work['gemm_flops'] <= work['factor_flops']
+    assert np.isclose(work['factor_flops'], sym.stats['flops_factor'])
+
+
+def test_symbolic_covers_every_unknown_once():
+    cell = workloads.config1(mesh_um=0.1, points=3)
+    m = cell.model
+    kl = KLayout(m.quads, m.N, 3)
+    bl = BlochLayout(kl, *bc_periodic(m, cell.mesh.lines, ndisp=2))
+    sym = analyse(bl.aa_indptr, bl.aa_indices, bl.node_of, bl.comp_of, m.pos_m, period=2.17e-6)
+    assert np.array_equal(np.sort(sym.perm), np.arange(bl.n_active))
+    for f in sym.fronts:
+        assert f.rows.size == f.m and np.unique(f.rows).size == f.m
+        if f.parent >= 0:
+            par = sym.fronts[f.parent]
+            assert set(f.rows[f.p:].tolist()) <= set(par.rows.tolist())

@@ -1,0 +1,271 @@
+"""Pins of the oracle against mathematics and the reference's own data (CPU).
+
+Each check is chosen so that a plausible mistake in the restatement (a dropped term,
+a wrong sign, a transposed operand, a wrong node order) fails it.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import fem, materials as om, sweep as osw
+
+GOLD = Path(__file__).parent / 'golden'
+
+
+# ---------------------------------------------------------------- shapes and quadrature
+def test_q9_shapes_kronecker_and_partition_of_unity():
+    nodes = np.array([(-1, -1), (1, -1), (1, 1), (-1, 1), (0, -1), (1, 0), (0, 1), (-1, 0), (0, 0)], float)
+    N, dN = fem.q9_shapes(nodes[:, 0], nodes[:, 1])
+    assert np.allclose(N, np.eye(9), atol=1e-15)
+    xi, eta = np.random.default_rng(1).uniform(-1, 1, (2, 50))
+    N, dN = fem.q9_shapes(xi, eta)
+    assert np.allclose(N.sum(1), 1.0)
+    assert np.allclose(dN.sum(1), 0.0, atol=1e-14)
+    # biquadratic reproduction: sum_a N_a f(x_a) == f for f = x^2 y^2
+    f = lambda x, y: x * x * y * y + 3 * x * y - y
+    assert np.allclose(N @ f(nodes[:, 0], nodes[:, 1]), f(xi, eta))
+    # derivatives by central differences
+    h = 1e-6
+    Np, _ = fem.q9_shapes(xi + h, eta)
+    Nm, _ = fem.q9_shapes(xi - h, eta)
+    assert np.allclose((Np - Nm) / (2 * h), dN[..., 0], atol=1e-8)
+
+
+def test_gauss_3x3_exact_to_degree_5():
+    pts, w = fem.gauss_3x3()
+    assert np.isclose(w.sum(), 4.0)
+    for a in range(6):
+        for b in range(6):
+            exact = (2.0 / (a + 1) if a % 2 == 0 else 0.0) * (2.0 / (b + 1) if b % 2 == 0 else 0.0)
+            assert np.isclose((w * pts[:, 0] ** a * pts[:, 1] ** b).sum(), exact, atol=1e-12)
+
+
+def _rect_element(x0, y0, hx, hy, cw=False):
+    xs = np.array([x0, x0 + hx, x0 + hx, x0, x0 + hx / 2, x0 + hx, x0 + hx / 2, x0, x0 + hx / 2])
+    ys = np.array([y0, y0, y0 + hy, y0 + hy, y0, y0 + hy / 2, y0 + hy, y0 + hy / 2, y0 + hy / 2])
+    pos = np.column_stack([xs, ys])
+    quads = np.arange(9)[None, :]
+    if cw:   # mirror in x: same nodes, clockwise corner order
+        pos = pos * np.array([-1.0, 1.0])
+    return pos, quads
+
+
+def test_gram_matrices_closed_form_rectangle():
+    hx, hy = 0.3e-6, 0.2e-6
+    pos, quads = _rect_element(0.1e-6, -0.4e-6, hx, hy)
+    G, GM = fem.element_geometry(pos, quads)
+    # 1D quadratic Lagrange on [0, h]: mass and stiffness matrices (nodes left, mid, right)
+    m1 = np.array([[4, 2, -1], [2, 16, 2], [-1, 2, 4]]) / 30.0
+    k1 = np.array([[7, -8, 1], [-8, 16, -8], [1, -8, 7]]) / 3.0
+    ia = np.array([0, 2, 2, 0, 1, 2, 1, 0, 1])
+    ja = np.array([0, 0, 2, 2, 0, 1, 2, 1, 1])
+    Mx, My = m1[np.ix_(ia, ia)] * hx, m1[np.ix_(ja, ja)] * hy
+    Kx, Ky = k1[np.ix_(ia, ia)] / hx, k1[np.ix_(ja, ja)] / hy
+    assert np.allclose(GM[0], Mx * My, rtol=1e-12, atol=1e-30)
+    assert np.allclose(G[(0, 0)][0], Kx * My, rtol=1e-12, atol=1e-14)
+    assert np.allclose(G[(1, 1)][0], Mx * Ky, rtol=1e-12, atol=1e-14)
+    D1 = np.array([[-3, 4, -1], [-4, 0, 4], [1, -4, 3]]) / 6.0       # ∫ L_a L_b' over [0,1]
+    Gxy = (D1.T[np.ix_(ia, ia)]) * (D1[np.ix_(ja, ja)])
+    assert np.allclose(G[(0, 1)][0], Gxy, atol=1e-14)
+    assert np.isclose(GM.sum(), hx * hy)
+    # clockwise element: signed detJ flips every Gram matrix (piezo_fem.py:264, 272)
+    pos2, _ = _rect_element(0.1e-6, -0.4e-6, hx, hy, cw=True)
+    G2, GM2 = fem.element_geometry(pos2, quads)
+    assert np.allclose(GM2, -GM)
+
+
+# ---------------------------------------------------------------- rotation and crystals
+def _voigt(i, j):
+    return [[0, 5, 4], [5, 1, 3], [4, 3, 2]][i][j]
+
+
+def _rotate_bruteforce(a, C, e, eps):
+    C4 = np.zeros((3, 3, 3, 3))
+    e3 = np.zeros((3, 3, 3))
+    for i in range(3):
+        for j in range(3):
+            for k in range(3):
+                e3[i, j, k] = e[i, _voigt(j, k)]
+                for l in range(3):
+                    C4[i, j, k, l] = C[_voigt(i, j), _voigt(k, l)]
+    C4r = np.einsum('ip,jq,kr,ls,pqrs->ijkl', a, a, a, a, C4)
+    e3r = np.einsum('ip,jq,kr,pqr->ijk', a, a, a, e3)
+    pairs = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+    Cr = np.array([[C4r[i, j, k, l] for k, l in pairs] for i, j in pairs])
+    er = np.array([[e3r[n, k, l] for k, l in pairs] for n in range(3)])
+    return Cr, er, a @ eps @ a.T
+
+
+@pytest.mark.parametrize('euler', [(0, -42, 0), (10, 33, -71), (0, 0, 0), (90, -38, 15)])
+def test_bond_rotation_matches_tensor_rotation(euler):
+    k = dict(C11=232.966, C12=46.8904, C13=80.2342, C14=-11.0267, C33=275.364, C44=93.8995, e15=2.59576,
+             e22=1.58923, e31=0.0821598, e33=1.88197, eps11=40.9, eps33=43.3)
+    C, e, eps = om.crystal_tensors('trigonal_3m', k)
+    a = om.rotation_matrix(euler)
+    assert np.allclose(a @ a.T, np.eye(3), atol=1e-15)
+    got = om.rotate(euler, C, e, eps)
+    ref = _rotate_bruteforce(a, C, e, eps)
+    for g, r in zip(got, ref):
+        assert np.allclose(g, r, rtol=1e-12, atol=1e-12 * np.abs(r).max())
+    assert np.allclose(np.linalg.eigvalsh(got[2]), np.linalg.eigvalsh(eps))
+    if euler == (0, 0, 0):
+        assert np.allclose(got[0], C) and np.allclose(got[1], e)
+
+
+def test_crystal_builders_reproduce_library_records():
+    """tests/test_stack_materials.py:65-78 of the reference, on copies of its records."""
+    rec = json.loads((GOLD / 'material_records.json').read_text())
+    r = rec['linbo3_bouchy_2022']
+    C, e, eps = np.array(r['C_pa']), np.array(r['e_c_m2']), np.array(r['eps_f_m'])
+    k = dict(C11=C[0, 0] / 1e9, C12=C[0, 1] / 1e9, C13=C[0, 2] / 1e9, C14=C[0, 3] / 1e9, C33=C[2, 2] / 1e9,
+             C44=C[3, 3] / 1e9, e15=e[0, 4], e22=e[1, 1], e31=e[2, 0], e33=e[2, 2],
+             eps11=eps[0, 0] / om.EPS0_CRYSTAL, eps33=eps[2, 2] / om.EPS0_CRYSTAL)
+    C2, e2, eps2 = om.crystal_tensors('trigonal_3m', k)
+    assert np.allclose(C2, C, rtol=0, atol=1e-3) and np.allclose(e2, e) and np.allclose(eps2, eps)
+    s = rec['sio2']
+    C3, _, eps3 = om.crystal_tensors('isotropic', dict(E_gpa=70, nu=0.17, eps_r=4.2))
+    assert np.allclose(C3, s['C_pa'], rtol=1e-6) and np.allclose(eps3, s['eps_f_m'], rtol=1e-3)
+    assert np.allclose(om.isotropic_C(70e9, 0.17), s['C_pa'], rtol=1e-12)
+
+
+# ---------------------------------------------------------------- assembly
+def _brute_force_assembly(pos, quads, mats, mat_idx, scale):
+    """Element loop with explicit B matrices: ∫ Bᵀ C B, no PML, ME1 layout [ux|uy|uz|phi]."""
+    N = pos.shape[0]
+    K = np.zeros((4 * N, 4 * N))
+    M = np.zeros((4 * N, 4 * N))
+    pts, wts = fem.gauss_3x3()
+    Nq, dNr = fem.q9_shapes(pts[:, 0], pts[:, 1])
+    for e, q in enumerate(quads):
+        m = mats[mat_idx[e]]
+        C, E, P = np.asarray(m.C), np.asarray(m.e), np.asarray(m.eps)
+        xy = pos[q]
+        for g in range(9):
+            J = xy.T @ dNr[g]
+            det = np.linalg.det(J)
+            dx = dNr[g] @ np.linalg.inv(J)               # (9, 2): d/dx, d/dy
+            Bu = np.zeros((6, 27))                       # engineering strain, d/dz = 0
+            for a in range(9):
+                Bu[0, a] = dx[a, 0]                      # exx = dux/dx
+                Bu[1, 9 + a] = dx[a, 1]                  # eyy = duy/dy
+                Bu[3, 18 + a] = dx[a, 1]                 # gyz = duz/dy
+                Bu[4, 18 + a] = dx[a, 0]                 # gxz = duz/dx
+                Bu[5, a] = dx[a, 1]                      # gxy = dux/dy + duy/dx
+                Bu[5, 9 + a] = dx[a, 0]
+            Bp = np.zeros((3, 9))
+            Bp[0], Bp[1] = dx[:, 0], dx[:, 1]            # E = -grad(phi) up to sign, Ez = 0
+            w = wts[g] * det
+            uu = Bu.T @ C @ Bu * w
+            up = Bu.T @ E.T @ Bp * w
+            pp = Bp.T @ P @ Bp * w
+            mm = np.outer(Nq[g], Nq[g]) * w * m.rho
+            iu = np.concatenate([q + c * N for c in range(3)])
+            ip = q + 3 * N
+            K[np.ix_(iu, iu)] += uu * scale
+            K[np.ix_(iu, ip)] += up
+            K[np.ix_(ip, iu)] += up.T * scale
+            K[np.ix_(ip, ip)] -= pp
+            for c in range(3):
+                M[np.ix_(q + c * N, q + c * N)] += mm * scale
+    return K, M
+
+
+def _two_element_mesh():
+    from sawsim_amd.mesh import structured_cell
+    m = structured_cell([0.0, 0.4], [2], [-0.5, 0.0], [1], lambda b, s: 16 if s == 0 else 22, {})
+    return m.pos_m * np.array([1.0, 1.3]) + np.array([0.0, 0.05e-6]) * m.pos_m[:, :1] * 1e6, m.quads
+
+
+def test_assembly_matches_brute_force_b_matrices():
+    pos, quads = _two_element_mesh()
+    k = dict(C11=232.966, C12=46.8904, C13=80.2342, C14=-11.0267, C33=275.364, C44=93.8995, e15=2.59576,
+             e22=1.58923, e31=0.0821598, e33=1.88197, eps11=40.9, eps33=43.3)
+    C, e, eps = om.rotate((13, -42, 7), *om.crystal_tensors('trigonal_3m', k))
+    mats = [om.Material(7450.0, C, e, eps),
+            om.Material(2330.0, om.isotropic_C(130e9, 0.28), np.zeros((3, 6)), np.eye(3) * 1e-10)]
+    idx = np.array([0, 1])
+    tags = np.array([16, 22])
+    K, M = fem.assemble(pos, quads, tags, mats, idx, [], 2 * np.pi * 2e9)
+    Kb, Mb = _brute_force_assembly(pos, quads, mats, idx, fem.SCALE)
+    assert np.allclose(K.toarray(), Kb, rtol=1e-10, atol=1e-10 * np.abs(Kb).max())
+    assert np.allclose(M.toarray(), Mb, rtol=1e-10, atol=1e-10 * np.abs(Mb).max())
+
+
+def test_assembly_invariants_with_pml():
+    from sawsim_amd.mesh import stack_cell
+    m = stack_cell(1.085, 0.5, 0.0868, 0.6, [0.8], [], 0.3, pml_pitches=1.0)
+    k = dict(C11=232.966, C12=46.8904, C13=80.2342, C14=-11.0267, C33=275.364, C44=93.8995, e15=2.59576,
+             e22=1.58923, e31=0.0821598, e33=1.88197, eps11=40.9, eps33=43.3)
+    C, e, eps = om.rotate((0, -42, 0), *om.crystal_tensors('trigonal_3m', k))
+    mats = [om.Material(7450.0, C, e, eps),
+            om.Material(2700.0, om.isotropic_C(70e9, 0.33), np.zeros((3, 6)), np.eye(3) * 8.854e-12),
+            om.Material(2330.0, om.isotropic_C(130e9, 0.28), np.zeros((3, 6)), np.eye(3) * 1e-10)]
+    from sawsim_amd.mesh import stack_tag_map
+    tm = stack_tag_map(m)
+    idx = np.array([tm[t] for t in m.elem_tag])
+    bottom = m.pos_m[:, 1].min()
+    pml = [fem.PMLRegion(tag=m.info['tag_pml'], kind='y', yp=bottom, ya=bottom + m.info['pml_thickness_m'])]
+    K, M = fem.assemble(m.pos_m, m.quads, m.elem_tag, mats, idx, pml, 2 * np.pi * 2e9)
+    N = m.N
+    Kd = K.toarray()
+    # rigid translations are in the null space of K_uu (PML stretch is a column scaling per element)
+    for c in range(3):
+        t = np.zeros(4 * N)
+        t[c * N:(c + 1) * N] = 1.0
+        assert np.abs(Kd[:3 * N] @ t).max() <= 1e-9 * np.abs(Kd[:3 * N]).max()
+    # constant potential is in the null space of K_phiphi and of K_uphi
+    t = np.zeros(4 * N)
+    t[3 * N:] = 1.0
+    assert np.abs(Kd @ t).max() <= 1e-9 * np.abs(Kd[:, 3 * N:]).max()
+    # K_phiu = SCALE * K_uphi^T (plain transpose, piezo_fem.py:343)
+    pu = Kd[3 * N:, :3 * N]
+    assert np.allclose(pu, fem.SCALE * Kd[:3 * N, 3 * N:].T, rtol=1e-12, atol=1e-13 * np.abs(pu).max())
+    # total mass outside the PML = SCALE * sum rho * area per component block
+    nonpml = m.elem_tag != m.info['tag_pml']
+    Md = M.toarray()
+    sel = np.zeros(N, bool)
+    # mass of the stack without PML from element areas
+    e1 = m.pos_m[m.quads[:, 1]] - m.pos_m[m.quads[:, 0]]
+    e2 = m.pos_m[m.quads[:, 3]] - m.pos_m[m.quads[:, 0]]
+    area = np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])
+    rho = np.array([mats[i].rho for i in idx])
+    a1 = np.ones(m.ne)
+    total_expected = fem.SCALE * (rho * area)[nonpml].sum()
+    # PML elements carry rho*a2 (complex); compare the real mass of non-PML elements via element sums
+    GM = fem.element_geometry(m.pos_m, m.quads)[1]
+    assert np.isclose(GM[nonpml].sum(axis=(1, 2)) @ rho[nonpml] * fem.SCALE, total_expected, rtol=1e-12)
+    assert np.isclose(Md[:N, :N].sum().real,
+                      fem.SCALE * (GM.sum(axis=(1, 2)) * rho *
+                                   fem.element_alpha(m.pos_m, m.quads, m.elem_tag, pml, 2 * np.pi * 2e9)[1].real).sum(),
+                      rtol=1e-12)
+
+
+# ---------------------------------------------------------------- boundary maps
+def test_bc_periodic_closed_form_on_structured_cell():
+    from sawsim_amd.mesh import stack_cell
+    m = stack_cell(1.085, 0.5, 0.0868, 0.6, [0.8], [], 0.3, pml_pitches=1.0)
+    fix, zheng, fu, groups = osw.bc_periodic(m.pos_m, m.lines, m.N, ndisp=2)
+    N = m.N
+    L, R = m.nodes_with_tag(19), m.nodes_with_tag(20)
+    bottom = m.nodes_with_tag(21)
+    # pairs: (len(L)-1) non-corner pairs per component (3) + 1 corner phi pair
+    assert len(groups) == 3 * (L.size - 1) + 1
+    for g in groups:
+        a, b = g % N
+        assert np.isclose(m.pos_m[a, 1], m.pos_m[b, 1]) and np.isclose(m.pos_m[b, 0] - m.pos_m[a, 0], 2.17e-6)
+    assert np.array_equal(np.sort(fix), np.sort(np.concatenate([bottom, bottom + N])))
+    T, slaves, masters = osw.merge_transform(3 * N, groups)
+    assert set((slaves % N).tolist()) == set(R.tolist())
+    assert np.all(masters % N < N) and set((masters % N).tolist()) <= set(L.tolist())
+    # T is a 0/1 selection: every row has exactly one 1
+    assert np.array_equal(np.asarray(T.sum(1)).ravel(), np.ones(3 * N))
+
+
+def test_union_find_chains_collapse_to_first_master():
+    T, s, mst = osw.merge_transform(6, [np.array([0, 1]), np.array([1, 2]), np.array([3, 4]), np.array([4, 0])])
+    rep = dict(zip(s.tolist(), mst.tolist()))
+    # 4 joins 3's class; then 0's class (0,1,2) is attached under 4's root 3
+    assert rep == {0: 3, 1: 3, 2: 3, 4: 3}
