@@ -68,8 +68,10 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
                      const void* Maa, const void* Daa, void* arena, int nb);
 
 /* Forward/backward substitution with the factors of sawfem_mf_factor; x: B x n complex128,
- * right-hand sides in, solutions out.  uarena: uarena_elems * B complex128 scratch. */
-int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x);
+ * right-hand sides in, solutions out.  uarena: uarena_elems * B complex128 scratch.
+ * transpose != 0 solves A^T x = b with the same factors (U^T then L^T; adjoint solve). */
+int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x,
+                    int transpose);
 
 /* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;
  * used by the benchmark).  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S,
@@ -81,10 +83,11 @@ int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count);
 int sawfem_rhs(void* stream, int n, int B, const double* omega, double beta, double V, const void* rK,
                const void* rM, const void* rD, void* out);
 
-/* r[b] = rhs[b] - A(w_b) x[b] in FP64 on the active CSR pattern (indptr int64, indices int32). */
-int sawfem_residual(void* stream, int n, int B, const int64_t* indptr, const int* indices, const void* Kaa,
-                    const void* Maa, const void* Daa, const double* omega, double beta, const void* x,
-                    const void* rhs, void* r);
+/* r[b] = rhs[b] - A(w_b) x[b] in FP64 on the active CSR pattern (indptr int64, indices int32);
+ * with tslot != NULL (slot of the transposed entry, int64) the operator is A^T. */
+int sawfem_residual(void* stream, int n, int B, const int64_t* indptr, const int* indices, const int64_t* tslot,
+                    const void* Kaa, const void* Maa, const void* Daa, const double* omega, double beta,
+                    const void* x, const void* rhs, void* r);
 
 /* Q[b] = sum_i kts_a[i] x[b][i] + V kts_z  (sweep.py:356-362). */
 int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, double kz_im, double V,
@@ -92,6 +95,17 @@ int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, d
 
 /* x += d (n complex128). */
 int sawfem_axpy(void* stream, int64_t n, void* x, const void* d);
+
+/* K12 backward element kernel for dY11/dp: per frequency, the PML-weighted sensitivity
+ * tensors of l = s - T P_a^T lambda and the displacement d over the elements listed in
+ * elems (the parameterised material): W[b] = [Hc (NR x NR), He (2 x NR), Hp (2 x 2)],
+ * NR = 2 ndu.  pos_rep: active index of each dof's merge representative or -1; zmask:
+ * signal-electrode dofs (value V in d, 1 in s).  work: B*ne0*NT complex128 scratch.
+ * Replaces the reference's central finite differences of the sweep (2P+1 solves per point). */
+int sawfem_adjoint_elem(void* stream, int ne0, const int* elems, const int* conn, const double* xy,
+                        const int* elem_tag, int npml, const int* pml_tag, const int* pml_kind, const double* pml_par,
+                        double omega_top, int ndu, int N, const int* pos_rep, const unsigned char* zmask, double V,
+                        double scale, int B, int n, const void* x, const void* lam, void* work, void* W);
 
 #ifdef __cplusplus
 }

@@ -27,13 +27,15 @@ SIGNATURES = {
     'sawfem_mf_uarena_elems': [_vp],
     'sawfem_mf_perturbed': [_vp, _i],
     'sawfem_mf_factor': [_vp, _vp, _i, _i, _vp, _d, _vp, _vp, _vp, _vp, _i],
-    'sawfem_mf_solve': [_vp, _vp, _i, _i, _vp, _vp, _vp],
+    'sawfem_mf_solve': [_vp, _vp, _i, _i, _vp, _vp, _vp, _i],
     'sawfem_mf_profile': [_vp, _i],
     'sawfem_mf_profile_read': [_vp, _vp, _vp],
     'sawfem_rhs': [_vp, _i, _i, _vp, _d, _d, _vp, _vp, _vp, _vp],
-    'sawfem_residual': [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
+    'sawfem_residual': [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
     'sawfem_charge': [_vp, _i, _i, _vp, _d, _d, _d, _vp, _vp],
     'sawfem_axpy': [_vp, _i64, _vp, _vp],
+    'sawfem_adjoint_elem': [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _i, _vp, _vp, _d, _d, _i, _i,
+                            _vp, _vp, _vp, _vp],
 }
 _RESTYPE = {'sawfem_last_error': C.c_char_p, 'sawfem_mf_arena_elems': _i64, 'sawfem_mf_uarena_elems': _i64}
 

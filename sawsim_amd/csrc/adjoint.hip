@@ -1,0 +1,172 @@
+// K12: backward element kernel for dY11/dp.
+//
+// With Q = s^T K d (s: indicator of the signal-electrode potentials, d: full displacement /
+// potential vector) and A_aa x = b, the adjoint lambda = A_aa^-T kts_a gives
+//     dQ = l^T dK d + i w beta l^T dD d + i w dbeta l^T D d,     l = s - T P_a^T lambda,
+// (the u part of s vanishes, so -lambda_u = l_u).  K, D are linear in the lifted material
+// tensors of each element, so per element this kernel forms the bilinear "sensitivity"
+// tensors of the pair (l, d), already multiplied by the element's PML codes:
+//     Hc[i][j] = code(Ri, Rj) * sum_q dV_q epsl_q[Ri] epsd_q[Rj]                  (C9 entries)
+//     He[k][i] = code(k, Ri)  * sum_q dV_q (epsl_q[Ri] gd_q[k] + SCALE gl_q[k] epsd_q[Ri])
+//     Hp[j][k] = teps(j, k)   * sum_q dV_q gl_q[j] gd_q[k]                        (eps entries)
+// where Ri = 9-component strain row of displacement component A = i / 2 and direction
+// d = i % 2, and gl/gd are potential gradients.  The host contracts the element sums with
+// the derivatives of the lifted tensors (SCALE and the sign of K_phiphi applied there).
+// Element matrices follow SawSim saw2d/piezo_fem.py:312-350; the element loop runs over the
+// elements of the parameterised material only.
+#include "q9.h"
+
+namespace {
+
+using namespace q9;
+
+template <int NDU>
+__global__ void __launch_bounds__(64) k_adj_elem(int ne0, const int* __restrict__ elems, const int* __restrict__ conn,
+                                                 const double* __restrict__ xy, const int* __restrict__ elem_tag,
+                                                 int npml, const PmlSet pmlset, double omega_top, int N,
+                                                 const int* __restrict__ pos_rep, const unsigned char* __restrict__ zmask,
+                                                 double V, double scale, int B, int n, const double2* __restrict__ x,
+                                                 const double2* __restrict__ lam, double2* __restrict__ out) {
+  constexpr int ND = NDU + 1;
+  constexpr int NR = 2 * NDU;
+  constexpr int NT = NR * NR + 2 * NR + 4;
+  const int ei = blockIdx.x;
+  const int e = elems[ei];
+  const int t = threadIdx.x;
+  __shared__ double sx[9], sy[9];
+  __shared__ double dNx[9][9][2];
+  __shared__ double Nq[9][9];
+  __shared__ double dV[9];
+  __shared__ double2 al[3];
+  __shared__ int dofs[9 * ND];
+  if (t < 9) {
+    const int nd = conn[e * 9 + t];
+    sx[t] = xy[2 * nd];
+    sy[t] = xy[2 * nd + 1];
+    for (int c = 0; c < ND; ++c) dofs[c * 9 + t] = c * N + nd;
+  }
+  __syncthreads();
+  if (t < 9) gauss_point(t, sx, sy, Nq[t], dNx[t], &dV[t]);
+  if (t == 32) pml_alpha(pmlset, npml, elem_tag[e], sx[8], sy[8], omega_top, al);
+  __syncthreads();
+  const int b = blockIdx.y * 64 + t;
+  if (b >= B) return;
+  double2 dvl[ND][9], lvl[ND][9];
+  for (int c = 0; c < ND; ++c)
+    for (int a = 0; a < 9; ++a) {
+      const int dof = dofs[c * 9 + a];
+      const int p = pos_rep[dof];
+      const bool z = zmask[dof] != 0;
+      double2 xv = p >= 0 ? x[(int64_t)b * n + p] : make_double2(0, 0);
+      double2 lv = p >= 0 ? lam[(int64_t)b * n + p] : make_double2(0, 0);
+      dvl[c][a] = z ? make_double2(V, 0) : xv;
+      lvl[c][a] = make_double2((z ? 1.0 : 0.0) - lv.x, -lv.y);
+    }
+  double2 Hc[NR][NR], He[2][NR], Hp[2][2];
+  for (int i = 0; i < NR; ++i) {
+    for (int j = 0; j < NR; ++j) Hc[i][j] = make_double2(0, 0);
+    He[0][i] = He[1][i] = make_double2(0, 0);
+  }
+  Hp[0][0] = Hp[0][1] = Hp[1][0] = Hp[1][1] = make_double2(0, 0);
+  for (int q = 0; q < 9; ++q) {
+    double2 gd[ND][2], gl[ND][2];
+    for (int c = 0; c < ND; ++c)
+      for (int d = 0; d < 2; ++d) {
+        double2 sd = make_double2(0, 0), sl = make_double2(0, 0);
+        for (int a = 0; a < 9; ++a) {
+          const double g = dNx[q][a][d];
+          sd.x += g * dvl[c][a].x; sd.y += g * dvl[c][a].y;
+          sl.x += g * lvl[c][a].x; sl.y += g * lvl[c][a].y;
+        }
+        gd[c][d] = sd;
+        gl[c][d] = sl;
+      }
+    const double w = dV[q];
+    for (int i = 0; i < NR; ++i) {
+      const double2 li = gl[i / 2][i % 2];
+      const double2 di = gd[i / 2][i % 2];
+      for (int j = 0; j < NR; ++j) {
+        const double2 pr = cmul(li, gd[j / 2][j % 2]);
+        Hc[i][j].x += w * pr.x; Hc[i][j].y += w * pr.y;
+      }
+      for (int k = 0; k < 2; ++k) {
+        const double2 a1 = cmul(li, gd[NDU][k]);
+        const double2 a2 = cmul(gl[NDU][k], di);
+        He[k][i].x += w * (a1.x + scale * a2.x);
+        He[k][i].y += w * (a1.y + scale * a2.y);
+      }
+    }
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) {
+        const double2 pr = cmul(gl[NDU][j], gd[NDU][k]);
+        Hp[j][k].x += w * pr.x; Hp[j][k].y += w * pr.y;
+      }
+  }
+  double2 alc[3] = {al[0], al[1], al[2]};
+  double2 code[6];
+  pml_codes(alc, code);
+  double2* o = out + ((int64_t)b * ne0 + ei) * NT;
+  for (int i = 0; i < NR; ++i)
+    for (int j = 0; j < NR; ++j) {
+      const int ri = kStrainRow[i / 2][i % 2], rj = kStrainRow[j / 2][j % 2];
+      o[i * NR + j] = cmul(code[kT9Code[ri][rj]], Hc[i][j]);
+    }
+  for (int k = 0; k < 2; ++k)
+    for (int i = 0; i < NR; ++i) {
+      const int ri = kStrainRow[i / 2][i % 2];
+      o[NR * NR + k * NR + i] = cmul(code[kT9Code[k][ri]], He[k][i]);
+    }
+  for (int j = 0; j < 2; ++j)
+    for (int k = 0; k < 2; ++k) o[NR * NR + 2 * NR + j * 2 + k] = cmul(code[kTepsCode[j][k]], Hp[j][k]);
+}
+
+// W[b][t] = sum_e out[b][e][t], elements in list order (deterministic).
+__global__ void k_adj_reduce(int ne0, int NT, int B, const double2* __restrict__ out, double2* __restrict__ W) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)B * NT) return;
+  const int b = (int)(g / NT), tt = (int)(g % NT);
+  double2 s = make_double2(0, 0);
+  const double2* o = out + (int64_t)b * ne0 * NT + tt;
+  for (int e = 0; e < ne0; ++e) {
+    const double2 v = o[(int64_t)e * NT];
+    s.x += v.x; s.y += v.y;
+  }
+  W[g] = s;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Element sensitivities of the pair (l, d) for B frequencies; see the file header.
+// work: B * ne0 * NT complex128 scratch, W: B * NT complex128 output,
+// NT = (2 ndu)^2 + 4 ndu + 4.
+int sawfem_adjoint_elem(void* stream, int ne0, const int* elems, const int* conn, const double* xy,
+                        const int* elem_tag, int npml, const int* pml_tag, const int* pml_kind, const double* pml_par,
+                        double omega_top, int ndu, int N, const int* pos_rep, const unsigned char* zmask, double V,
+                        double scale, int B, int n, const void* x, const void* lam, void* work, void* W) {
+  if (ne0 < 0 || B <= 0 || n <= 0 || (ndu != 2 && ndu != 3) || npml < 0 || npml > 8) {
+    sawfem_set_error("sawfem_adjoint_elem: invalid arguments");
+    return 1;
+  }
+  const int NR = 2 * ndu, NT = NR * NR + 2 * NR + 4;
+  hipStream_t s = (hipStream_t)stream;
+  if (ne0 > 0) {
+    const q9::PmlSet ps = q9::make_pml_set(npml, pml_tag, pml_kind, pml_par);
+    dim3 grid(ne0, (B + 63) / 64);
+    if (ndu == 2)
+      hipLaunchKernelGGL(k_adj_elem<2>, grid, dim3(64), 0, s, ne0, elems, conn, xy, elem_tag, npml, ps, omega_top, N,
+                         pos_rep, zmask, V, scale, B, n, (const double2*)x, (const double2*)lam, (double2*)work);
+    else
+      hipLaunchKernelGGL(k_adj_elem<3>, grid, dim3(64), 0, s, ne0, elems, conn, xy, elem_tag, npml, ps, omega_top, N,
+                         pos_rep, zmask, V, scale, B, n, (const double2*)x, (const double2*)lam, (double2*)work);
+    SAW_HIP(hipGetLastError());
+  }
+  const int64_t tot = (int64_t)B * NT;
+  hipLaunchKernelGGL(k_adj_reduce, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, ne0, NT, B,
+                     (const double2*)work, (double2*)W);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

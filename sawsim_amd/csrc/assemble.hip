@@ -10,35 +10,12 @@
 //     M = SCALE rho_eff GM on each displacement block (:357-362).
 // One workgroup per element; thread t < 81 owns the node pair (a = t / 9, b = t % 9).
 // Output layout: Ke[e][A][B][a][b] (complex128), Me[e][a][b] (complex128).
-#include "common.h"
+#include "q9.h"
+#include <cstdio>
 
 namespace {
 
-__constant__ double kGaussPt[3] = {-0.774596669241483, 0.0, 0.774596669241483};
-__constant__ double kGaussW[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
-// QUADS9 node -> 1D Lagrange index in xi and eta (0: -1, 1: 0, 2: +1)
-__constant__ int kIa[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1};
-__constant__ int kJa[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
-// strain row of component A for derivative direction d (9-component strain)
-__constant__ int kStrainRow[3][2] = {{0, 5}, {8, 1}, {7, 6}};
-// PML multiplier codes (piezo_fem.py:127-142): 0 A, 1 b, 2 c, 3 D, 4 e, 5 F
-__constant__ int kT9Code[9][9] = {
-    {0, 1, 2, 2, 2, 1, 1, 0, 0}, {1, 3, 4, 4, 4, 3, 3, 1, 1}, {2, 4, 5, 5, 5, 4, 4, 2, 2},
-    {2, 4, 5, 5, 5, 4, 4, 2, 2}, {2, 4, 5, 5, 5, 4, 4, 2, 2}, {1, 3, 4, 4, 4, 3, 3, 1, 1},
-    {1, 3, 4, 4, 4, 3, 3, 1, 1}, {0, 1, 2, 2, 2, 1, 1, 0, 0}, {0, 1, 2, 2, 2, 1, 1, 0, 0}};
-
-__device__ inline double lag(int i, double t) {
-  return i == 0 ? 0.5 * t * (t - 1.0) : (i == 1 ? 1.0 - t * t : 0.5 * t * (t + 1.0));
-}
-__device__ inline double dlag(int i, double t) {
-  return i == 0 ? t - 0.5 : (i == 1 ? -2.0 * t : t + 0.5);
-}
-
-struct PmlRegionDev {
-  int tag, kind;            // kind bit 0: x stretch, bit 1: y stretch
-  double dmax, n, xa, xp, ya, yp;
-};
-struct PmlSet { PmlRegionDev r[8]; };
+using namespace q9;
 
 __global__ void __launch_bounds__(128) k_elem_q9(
     int ne, int ndu, const int* __restrict__ conn, const double* __restrict__ xy,
@@ -55,58 +32,13 @@ __global__ void __launch_bounds__(128) k_elem_q9(
   __shared__ double dV[9];
   __shared__ double2 al[3];
   if (t < 9) {
-    int nd = conn[e * 9 + t];
+    const int nd = conn[e * 9 + t];
     sx[t] = xy[2 * nd];
     sy[t] = xy[2 * nd + 1];
   }
   __syncthreads();
-  if (t < 9) {
-    const int q = t;
-    const double xi = kGaussPt[q % 3], eta = kGaussPt[q / 3];
-    const double w = kGaussW[q % 3] * kGaussW[q / 3];
-    double dr[9][2];
-    double J00 = 0, J01 = 0, J10 = 0, J11 = 0;
-    for (int a = 0; a < 9; ++a) {
-      const int ia = kIa[a], ja = kJa[a];
-      Nq[q][a] = lag(ia, xi) * lag(ja, eta);
-      dr[a][0] = dlag(ia, xi) * lag(ja, eta);
-      dr[a][1] = lag(ia, xi) * dlag(ja, eta);
-      J00 += sx[a] * dr[a][0]; J01 += sx[a] * dr[a][1];
-      J10 += sy[a] * dr[a][0]; J11 += sy[a] * dr[a][1];
-    }
-    // J[i][j] = sum_a xy[a][i] dN[a][j]
-    const double det = J00 * J11 - J01 * J10;
-    const double i00 = J11 / det, i01 = -J01 / det, i10 = -J10 / det, i11 = J00 / det;
-    for (int a = 0; a < 9; ++a) {
-      dNx[q][a][0] = dr[a][0] * i00 + dr[a][1] * i10;
-      dNx[q][a][1] = dr[a][0] * i01 + dr[a][1] * i11;
-    }
-    dV[q] = w * det;
-  }
-  if (t == 32) {
-    // PML stretch at the centre node (local 8)
-    double2 a1 = make_double2(1, 0), a2 = make_double2(1, 0), a3 = make_double2(1, 0);
-    const int tag = elem_tag[e];
-    const int cn = conn[e * 9 + 8];
-    const double xc = xy[2 * cn], yc = xy[2 * cn + 1];
-    const PmlRegionDev* pml = pmlset.r;
-    for (int r = 0; r < npml; ++r) {
-      if (pml[r].tag != tag) continue;
-      if (pml[r].kind & 1) {
-        double ratio = (xc - pml[r].xp) * (xc - pml[r].xp) / ((pml[r].xa - pml[r].xp) * (pml[r].xa - pml[r].xp));
-        ratio = fmin(fmax(ratio, 0.0), 1.0);
-        const double d = pml[r].dmax * pow(1.0 - ratio, pml[r].n);
-        a1 = make_double2(1.0, -d / omega_top);     // 1 + d / (i w)
-      }
-      if (pml[r].kind & 2) {
-        double ratio = (yc - pml[r].yp) * (yc - pml[r].yp) / ((pml[r].ya - pml[r].yp) * (pml[r].ya - pml[r].yp));
-        ratio = fmin(fmax(ratio, 0.0), 1.0);
-        const double d = pml[r].dmax * pow(1.0 - ratio, pml[r].n);
-        a2 = make_double2(1.0, -d / omega_top);
-      }
-    }
-    al[0] = a1; al[1] = a2; al[2] = a3;
-  }
+  if (t < 9) gauss_point(t, sx, sy, Nq[t], dNx[t], &dV[t]);
+  if (t == 32) pml_alpha(pmlset, npml, elem_tag[e], sx[8], sy[8], omega_top, al);
   __syncthreads();
   if (t >= 81) return;
   const int a = t / 9, b = t % 9;
@@ -119,21 +51,16 @@ __global__ void __launch_bounds__(128) k_elem_q9(
     G[1][1] += v * dNx[q][a][1] * dNx[q][b][1];
     GM += v * Nq[q][a] * Nq[q][b];
   }
-  const double2 a1 = al[0], a2 = al[1], a3 = al[2];
+  double2 alc[3] = {al[0], al[1], al[2]};
   double2 code[6];
-  code[0] = cdiv(cmul(a2, a3), a1);   // A
-  code[1] = a3;                        // b
-  code[2] = a2;                        // c
-  code[3] = cdiv(cmul(a1, a3), a2);    // D
-  code[4] = a1;                        // e
-  code[5] = cdiv(cmul(a1, a2), a3);    // F
+  pml_codes(alc, code);
   const int mat = mat_idx[e];
   const double2* Cm = C9 + mat * 81;
   const double2* Em = E9 + mat * 27;
   const double2* Pm = EPS + mat * 9;
   const int nd = ndu + 1;
   double2* Kout = Ke + (size_t)e * nd * nd * 81;
-  // K_uu and K_uphi / K_phiu
+  // K_uu: sum over direction pairs of C_eff[r(A,dv)][r(B,du)] G[dv][du]
   for (int A = 0; A < ndu; ++A) {
     for (int B = 0; B < ndu; ++B) {
       double2 acc = make_double2(0, 0);
@@ -163,17 +90,15 @@ __global__ void __launch_bounds__(128) k_elem_q9(
     Kout[(ndu * nd + A) * 81 + t] = make_double2(scale * pu.x, scale * pu.y);
   }
   {
-    // Teps (piezo_fem.py:145-155) reuses codes: [[A, b, c], [b, D, e], [c, e, F]]
-    const int tc[2][2] = {{0, 1}, {1, 3}};
     double2 pp = make_double2(0, 0);
     for (int j = 0; j < 2; ++j)
       for (int k = 0; k < 2; ++k) {
-        const double2 peff = cmul(code[tc[j][k]], Pm[j * 3 + k]);
+        const double2 peff = cmul(code[kTepsCode[j][k]], Pm[j * 3 + k]);
         pp.x += peff.x * G[j][k]; pp.y += peff.y * G[j][k];
       }
     Kout[(ndu * nd + ndu) * 81 + t] = make_double2(-pp.x, -pp.y);
   }
-  const double2 a123 = cmul(cmul(a1, a2), a3);
+  const double2 a123 = cmul(cmul(alc[0], alc[1]), alc[2]);
   const double rs = RHO[mat] * GM * scale;
   Me[(size_t)e * 81 + t] = make_double2(a123.x * rs, a123.y * rs);
 }
@@ -250,14 +175,7 @@ int sawfem_elem_q9(void* stream, int ne, int ndu, const int* conn, const double*
     return 1;
   }
   if (ne == 0) return 0;
-  PmlSet ps;
-  PmlRegionDev* hp = ps.r;
-  for (int r = 0; r < npml; ++r) {
-    hp[r].tag = pml_tag[r]; hp[r].kind = pml_kind[r];
-    hp[r].dmax = pml_par[6 * r + 0]; hp[r].n = pml_par[6 * r + 1];
-    hp[r].xa = pml_par[6 * r + 2]; hp[r].xp = pml_par[6 * r + 3];
-    hp[r].ya = pml_par[6 * r + 4]; hp[r].yp = pml_par[6 * r + 5];
-  }
+  const q9::PmlSet ps = q9::make_pml_set(npml, pml_tag, pml_kind, pml_par);
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_elem_q9, dim3(ne), dim3(128), 0, s, ne, ndu, conn, xy, mat_idx, elem_tag,
                      (const double2*)C9, (const double2*)E9, (const double2*)EPS, rho, npml, ps,

@@ -414,10 +414,95 @@ __global__ void __launch_bounds__(256) k_bwd(FrontsDev F, const int* __restrict_
     z[i] = s;
   }
   __syncthreads();
+  // every thread reads z[k] before anyone writes; solutions go to a separate array
+  double2* sol = z + m;
   for (int k = p - 1; k >= 0; --k) {
     const double2 xk = cdiv(z[k], to_d2(Fb[(int64_t)k + (int64_t)k * m]));
     for (int i = threadIdx.x; i < k; i += blockDim.x) z[i] = cfms(z[i], to_d2(Fb[(int64_t)i + (int64_t)k * m]), xk);
-    if (threadIdx.x == 0) z[k] = xk;
+    if (threadIdx.x == 0) sol[k] = xk;
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < p; i += blockDim.x) xb[rows[i]] = sol[i];
+}
+
+
+// Transposed solves for the adjoint A^T lambda = g with A = L U:  U^T w = g (bottom-up),
+// then L^T lambda = w (top-down).  Same storage and maps as k_fwd / k_bwd.
+template <typename T>
+__global__ void __launch_bounds__(256) k_fwdT(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+                                              const typename cplx<T>::type* __restrict__ arena,
+                                              double2* __restrict__ uarena, double2* __restrict__ x) {
+  using V = typename cplx<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  double2* w = reinterpret_cast<double2*>(smem_raw);
+  const int f = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const int m = F.m[f], p = F.p[f];
+  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
+  const int* rows = F.rows + F.rows_ptr[f];
+  double2* xb = x + (int64_t)b * n;
+  const int nch = F.child_ptr[f + 1] - F.child_ptr[f];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    double2 v = (i < p) ? xb[rows[i]] : make_double2(0, 0);
+    for (int c = 0; c < nch; ++c) {
+      const int slot = F.child_ptr[f] + c;
+      const int ci = F.inv[F.inv_ptr[slot] + i];
+      if (ci >= 0) {
+        const int ch = F.child_ids[slot];
+        const double2 u = uarena[F.uoff[ch] * B + (int64_t)b * (F.m[ch] - F.p[ch]) + ci];
+        v.x += u.x; v.y += u.y;
+      }
+    }
+    w[i] = v;
+  }
+  __syncthreads();
+  // U11^T w_p = w_p: w_k /= U[k][k]; w_i -= U[k][i] w_k for i > k
+  for (int k = 0; k < p; ++k) {
+    const double2 wk = cdiv(w[k], to_d2(Fb[(int64_t)k + (int64_t)k * m]));
+    for (int i = k + 1 + threadIdx.x; i < p; i += blockDim.x)
+      w[i] = cfms(w[i], to_d2(Fb[(int64_t)k + (int64_t)i * m]), wk);
+    __syncthreads();
+    if (threadIdx.x == 0) w[k] = wk;
+    __syncthreads();
+  }
+  // update for the parent: u_j = w[p + j] - sum_k U[k][p + j] w_k
+  double2* uf = uarena + F.uoff[f] * B + (int64_t)b * (m - p);
+  for (int j = p + threadIdx.x; j < m; j += blockDim.x) {
+    double2 s = w[j];
+    const V* col = Fb + (int64_t)j * m;
+    for (int k = 0; k < p; ++k) s = cfms(s, to_d2(col[k]), w[k]);
+    uf[j - p] = s;
+  }
+  for (int i = threadIdx.x; i < p; i += blockDim.x) xb[rows[i]] = w[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_bwdT(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+                                              const typename cplx<T>::type* __restrict__ arena,
+                                              double2* __restrict__ x) {
+  using V = typename cplx<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  double2* z = reinterpret_cast<double2*>(smem_raw);
+  const int f = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const int m = F.m[f], p = F.p[f];
+  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
+  const int* rows = F.rows + F.rows_ptr[f];
+  double2* xb = x + (int64_t)b * n;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) z[i] = xb[rows[i]];
+  __syncthreads();
+  // z_k -= sum_j L[p + j][k] lambda_cb[j]   (column k of L21 is contiguous)
+  for (int k = threadIdx.x; k < p; k += blockDim.x) {
+    double2 s = z[k];
+    const V* col = Fb + (int64_t)k * m;
+    for (int j = p; j < m; ++j) s = cfms(s, to_d2(col[j]), z[j]);
+    z[k] = s;
+  }
+  __syncthreads();
+  // L11^T (unit upper) backward: lambda_k = z_k; z_i -= L[k][i] lambda_k for i < k
+  for (int k = p - 1; k >= 0; --k) {
+    const double2 lk = z[k];
+    for (int i = threadIdx.x; i < k; i += blockDim.x) z[i] = cfms(z[i], to_d2(Fb[(int64_t)k + (int64_t)i * m]), lk);
     __syncthreads();
   }
   for (int i = threadIdx.x; i < p; i += blockDim.x) xb[rows[i]] = z[i];
@@ -438,7 +523,10 @@ __global__ void k_rhs(int n, int B, const double* __restrict__ omega, double bet
 }
 
 // r = rhs - A(w) x for all frequencies; one thread per row, 8 frequencies per pass.
+// With tslot != NULL the operator is A^T: the value of slot s is taken from slot tslot[s]
+// (the pattern is structurally symmetric).
 __global__ void k_residual(int n, int B, const int64_t* __restrict__ indptr, const int* __restrict__ indices,
+                           const int64_t* __restrict__ tslot,
                            const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
                            const double2* __restrict__ Daa, const double* __restrict__ omega, double beta,
                            const double2* __restrict__ x, const double2* __restrict__ rhs, double2* __restrict__ r) {
@@ -449,8 +537,9 @@ __global__ void k_residual(int n, int B, const int64_t* __restrict__ indptr, con
   double2 acc[8];
   double w[8];
   for (int q = 0; q < 8; ++q) { acc[q] = make_double2(0, 0); w[q] = q < nb ? omega[b0 + q] : 0.0; }
-  for (int64_t s = indptr[i]; s < indptr[i + 1]; ++s) {
-    const int j = indices[s];
+  for (int64_t s0 = indptr[i]; s0 < indptr[i + 1]; ++s0) {
+    const int j = indices[s0];
+    const int64_t s = tslot ? tslot[s0] : s0;
     const double2 k = Kaa[s], mm = Maa[s];
     const double2 d = Daa ? Daa[s] : make_double2(0, 0);
     for (int q = 0; q < nb; ++q) {
@@ -596,8 +685,23 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
 }
 
 template <typename T>
-int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x) {
+int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose) {
   const size_t lds = sizeof(double2) * P->mmax;
+  if (transpose) {
+    for (const Level& L : P->levels) {
+      Timed tm(P, s, K_FWD);
+      hipLaunchKernelGGL(k_fwdT<T>, dim3(L.nf, B), dim3(256), lds, s, P->F, L.fronts, B, P->n,
+                         (const typename cplx<T>::type*)arena, (double2*)uarena, (double2*)x);
+    }
+    for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
+      const Level& L = P->levels[l];
+      Timed tm(P, s, K_BWD);
+      hipLaunchKernelGGL(k_bwdT<T>, dim3(L.nf, B), dim3(256), lds, s, P->F, L.fronts, B, P->n,
+                         (const typename cplx<T>::type*)arena, (double2*)x);
+    }
+    SAW_HIP(hipGetLastError());
+    return 0;
+  }
   for (const Level& L : P->levels) {
     Timed tm(P, s, K_FWD);
     hipLaunchKernelGGL(k_fwd<T>, dim3(L.nf, B), dim3(256), lds, s, P->F, L.fronts, B, P->n,
@@ -606,7 +710,7 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
     const Level& L = P->levels[l];
     Timed tm(P, s, K_BWD);
-    hipLaunchKernelGGL(k_bwd<T>, dim3(L.nf, B), dim3(256), lds, s, P->F, L.fronts, B, P->n,
+    hipLaunchKernelGGL(k_bwd<T>, dim3(L.nf, B), dim3(256), 2 * lds, s, P->F, L.fronts, B, P->n,
                        (const typename cplx<T>::type*)arena, (double2*)x);
   }
   SAW_HIP(hipGetLastError());
@@ -724,18 +828,20 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
 }
 
 // x: [B][n] complex128 right-hand sides, overwritten with the solutions.
-int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x) {
+int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x,
+                    int transpose) {
   Plan* P = (Plan*)plan;
   if (!P || B <= 0 || !arena || !uarena || !x || (prec != 0 && prec != 1)) {
     sawfem_set_error("sawfem_mf_solve: invalid arguments");
     return 1;
   }
-  if ((size_t)P->mmax * sizeof(double2) > 64 * 1024) {
+  if ((size_t)P->mmax * sizeof(double2) * 2 > 64 * 1024) {
     sawfem_set_error("sawfem_mf_solve: front too large for the LDS solve");
     return 1;
   }
   hipStream_t s = (hipStream_t)stream;
-  return prec == 0 ? solve_impl<float>(P, s, B, arena, uarena, x) : solve_impl<double>(P, s, B, arena, uarena, x);
+  return prec == 0 ? solve_impl<float>(P, s, B, arena, uarena, x, transpose)
+                   : solve_impl<double>(P, s, B, arena, uarena, x, transpose);
 }
 
 // Per-kernel event timing: enable (1) / disable (0); turning it on clears what was recorded.
@@ -775,12 +881,12 @@ int sawfem_rhs(void* stream, int n, int B, const double* omega, double beta, dou
   return 0;
 }
 
-int sawfem_residual(void* stream, int n, int B, const int64_t* indptr, const int* indices, const void* Kaa,
-                    const void* Maa, const void* Daa, const double* omega, double beta, const void* x,
-                    const void* rhs, void* r) {
+int sawfem_residual(void* stream, int n, int B, const int64_t* indptr, const int* indices, const int64_t* tslot,
+                    const void* Kaa, const void* Maa, const void* Daa, const double* omega, double beta,
+                    const void* x, const void* rhs, void* r) {
   if (n <= 0 || B <= 0) { sawfem_set_error("sawfem_residual: invalid arguments"); return 1; }
   hipLaunchKernelGGL(k_residual, dim3((n + 255) / 256, (B + 7) / 8), dim3(256), 0, (hipStream_t)stream, n, B, indptr,
-                     indices, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, omega, beta,
+                     indices, tslot, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, omega, beta,
                      (const double2*)x, (const double2*)rhs, (double2*)r);
   SAW_HIP(hipGetLastError());
   return 0;
