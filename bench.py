@@ -34,20 +34,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_engine(cell):
-    from sawsim_amd.materials import lift_projected
-    from sawsim_amd.piezo_fem import assemble_device
-    from sawsim_amd.sweep import FrequencySolver, bc_periodic
-    nd = cell.ndisp + 1
-    mats = cell.materials if nd == 4 else [lift_projected(m) for m in cell.materials]
-    mask = np.zeros(len(mats), dtype=bool)
-    mask[0] = True
-    K, M, D = assemble_device(cell.model, mats, cell.mat_idx, cell.pml, 2 * np.pi * cell.band_hz[1], nd,
-                              damping_mask=mask)
-    eng = FrequencySolver(K, M, *bc_periodic(cell.model, cell.mesh.lines, ndisp=cell.ndisp))
-    if cell.beta_dk:
-        eng.set_damping(D, cell.beta_dk)
-    return eng
+def build_engine(cell, gradient=True):
+    from sawsim_amd.workloads import build_engine as _build
+    return _build(cell, gradient=gradient)
 
 
 def profile_read(eng):
@@ -122,24 +111,28 @@ def main():
     log(f'[bench] batch {batch} x {nchunk} chunks, arena {per_freq * batch / 1e9:.1f} GB')
 
     def sweep():
-        return eng.sweep_charges(freqs, precision='mixed', batch=batch)
+        return eng.sweep_charges(freqs, precision='mixed', batch=batch, gradient=True)
     for _ in range(args.warmup):
         sweep()
     torch.cuda.synchronize()
     _lib.call('sawfem_mf_profile', eng._plan, 1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        Q, info = sweep()
+        Q, dY, info = sweep()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     prof = profile_read(eng)
     _lib.call('sawfem_mf_profile', eng._plan, 0)
     value = args.steps * P / dt
-    log(f'[bench] mixed sweep {dt / args.steps:.3f}s/step -> {value:.1f} pts/s; ir {info}')
-    # accuracy of the mixed path against the plain FP64 path on a sample
+    log(f'[bench] mixed sweep + adjoint {dt / args.steps:.3f}s/step -> {value:.1f} pts/s; ir {info}')
+    # accuracy of the mixed path against the plain FP64 path (with its FP64 adjoint) on a sample
     idx = np.linspace(0, P - 1, min(args.check_points, P)).astype(int)
-    Q64, _ = eng.sweep_charges(freqs[idx], precision='fp64', batch=max(1, min(len(idx), batch // 2)))
+    Q64, dY64, _ = eng.sweep_charges(freqs[idx], precision='fp64', batch=max(1, min(len(idx), batch // 2)),
+                                     gradient=True)
     y_err = float(np.max(np.abs(Q[idx] - Q64) / np.abs(Q64)))
+    scale = np.abs(cell.params)
+    scale[12:15] = 1.0
+    g_err = float(np.max(np.abs((dY[idx] - dY64) * scale).max(axis=1) / np.abs(dY64 * scale).max(axis=1)))
     # plain FP64 GPU path: one frequency per factorization
     pp = min(args.plain_points, P)
     pidx = np.linspace(0, P - 1, pp).astype(int)
@@ -149,7 +142,29 @@ def main():
     eng.sweep_charges(freqs[pidx], precision='fp64', batch=1)
     torch.cuda.synchronize()
     plain = pp / (time.perf_counter() - t0)
-    log(f'[bench] plain fp64 {plain:.2f} pts/s; mixed vs fp64 max rel err {y_err:.2e}')
+    t0 = time.perf_counter()
+    eng.sweep_charges(freqs[pidx], precision='fp64', batch=1, gradient=True)
+    torch.cuda.synchronize()
+    plain_adj = pp / (time.perf_counter() - t0)
+    # plain FP64 with the gradient by central differences: 2P+1 = 35 operator rebuilds + solves, one point
+    fz = freqs[pidx[:1]]
+    p0 = cell.params.copy()
+    t0 = time.perf_counter()
+    eng.sweep_charges(fz, precision='fp64', batch=1)
+    for k in range(17):
+        h = 1e-6 * abs(p0[k]) if p0[k] != 0 else 1e-6 * 180 / np.pi
+        for sgn in (1, -1):
+            p = p0.copy()
+            p[k] += sgn * h
+            K, M, D = workloads.assemble_cell(cell, workloads.materials_for_params(cell, p))
+            eng.update_operator(K, M, D, beta=p[15])
+            eng.sweep_charges(fz, precision='fp64', batch=1)
+    torch.cuda.synchronize()
+    plain_fd = 1.0 / (time.perf_counter() - t0)
+    K, M, D = workloads.assemble_cell(cell)
+    eng.update_operator(K, M, D, beta=p0[15])
+    log(f'[bench] plain fp64: Y11 {plain:.2f} pts/s, +adjoint {plain_adj:.2f}, +central FD {plain_fd:.3f}; '
+        f'mixed vs fp64: Y11 {y_err:.2e}, dY/dp {g_err:.2e}')
     gemm_ms = prof['gemm_l'][0] + prof['gemm_s'][0]
     gemm_n = prof['gemm_l'][1] + prof['gemm_s'][1]
     gemm_flops = work['gemm_flops'] * P * args.steps
@@ -168,10 +183,13 @@ def main():
         config=dict(workload=f'{args.workload}: {cell.model.ne} Q9 elements, {eng.n_active} complex unknowns, '
                              f'{P} frequency points, batch {batch}',
                     points=P, n_active=eng.n_active, fronts=eng.mf.nf, batch=batch,
-                    gradient='not included: Y11 only (adjoint not built)'),
+                    gradient='dY11/dp for 17 parameters (3m constants, Euler angles, beta_dK, tan delta) by adjoint'),
         y11_max_rel_err_vs_fp64=y_err,
-        plain_fp64_gpu_pts_per_s=plain,
-        speedup_vs_plain_fp64=value / plain,
+        dy11_max_rel_err_vs_fp64_adjoint=g_err,
+        plain_fp64_gpu_pts_per_s=dict(y11_only=plain, y11_plus_fp64_adjoint=plain_adj,
+                                      y11_plus_central_fd=plain_fd),
+        speedup_vs_plain_fp64=dict(central_fd=value / plain_fd, fp64_adjoint=value / plain_adj,
+                                   y11_only=value / plain),
         ir=info,
         roofline=dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
                       frac=(achieved / PEAK_FP32_MFMA) if achieved else None, traffic=None,

@@ -89,17 +89,8 @@ class FrequencySolver:
         self.Kaa, self.Maa, self.rK, self.rM = z(bl.nnz_aa), z(bl.nnz_aa), z(n), z(n)
         self.Daa = self.rD = None
         self.beta = 0.0
-        s = stream_ptr()
-        _lib.call('sawfem_gather3', s, bl.nnz_aa, self._aa[0].data_ptr(), self._aa[1].data_ptr(),
-                  K.values.data_ptr(), M.values.data_ptr(), None, self.Kaa.data_ptr(), self.Maa.data_ptr(), None)
-        _lib.call('sawfem_gather3', s, n, self._rh[0].data_ptr(), self._rh[1].data_ptr(),
-                  K.values.data_ptr(), M.values.data_ptr(), None, self.rK.data_ptr(), self.rM.data_ptr(), None)
-        kts = z(n + 1)
-        _lib.call('sawfem_gather3', s, n + 1, self._kt[0].data_ptr(), self._kt[1].data_ptr(),
-                  K.values.data_ptr(), None, None, kts.data_ptr(), None, None)
-        self.kts_a = kts[:n]
-        kz = complex(kts[n].item())
-        self.kts_z = kz
+        self._kts = z(n + 1)
+        self.update_operator(K, M)
         # symbolic analysis and numeric plan
         t0 = time.time()
         xy = node_xy if node_xy is not None else getattr(K, 'node_xy', None)
@@ -138,6 +129,25 @@ class FrequencySolver:
         except Exception:
             pass
 
+    def update_operator(self, K: DeviceCSR, M: DeviceCSR, D: DeviceCSR = None, beta: float = None):
+        """Re-gather the active system from new K/M (and D) values on the same mesh and boundary
+        sets; the symbolic analysis and the numeric plan are kept (design loops, finite differences)."""
+        if K.layout.nnz != self.layout.nnz or M.layout.nnz != self.layout.nnz:
+            raise ValueError('K/M layout differs from the one this solver was built for')
+        bl = self.bloch
+        n = self.n_active
+        s = stream_ptr()
+        _lib.call('sawfem_gather3', s, bl.nnz_aa, self._aa[0].data_ptr(), self._aa[1].data_ptr(),
+                  K.values.data_ptr(), M.values.data_ptr(), None, self.Kaa.data_ptr(), self.Maa.data_ptr(), None)
+        _lib.call('sawfem_gather3', s, n, self._rh[0].data_ptr(), self._rh[1].data_ptr(),
+                  K.values.data_ptr(), M.values.data_ptr(), None, self.rK.data_ptr(), self.rM.data_ptr(), None)
+        _lib.call('sawfem_gather3', s, n + 1, self._kt[0].data_ptr(), self._kt[1].data_ptr(),
+                  K.values.data_ptr(), None, None, self._kts.data_ptr(), None, None)
+        self.kts_a = self._kts[:n]
+        self.kts_z = complex(self._kts[n].item())
+        if D is not None:
+            self.set_damping(D, self.beta if beta is None else beta)
+
     def set_damping(self, D: DeviceCSR, beta: float):
         """Rayleigh stiffness damping A = K + i w beta D - w^2 M (``sweep.py:316-323``)."""
         torch = self.torch
@@ -151,6 +161,63 @@ class FrequencySolver:
                   D.values.data_ptr(), None, None, self.rD.data_ptr(), None, None)
         self.beta = float(beta)
 
+    # ---------------------------------------------------------------- gradient context
+    def enable_gradient(self, model, mat_idx, pml_regions, omega_top, params, plane='xy', material=0):
+        """Prepare dY11/dp for the 17 parameters of material ``material`` (see ``params.py``).
+
+        ``model``/``mat_idx``/``pml_regions``/``omega_top`` are the assembly inputs; ``params`` the
+        parameter vector the material tensors were built from.
+        """
+        from . import params as prm
+        from .piezo_fem import _pml_arrays
+        torch = self.torch
+        dev = self.dev
+        bl = self.bloch
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=dev)
+        elems = np.flatnonzero(np.asarray(mat_idx) == material).astype(np.int32)
+        self._grad = dict(
+            elems=t(elems), ne0=int(elems.size),
+            conn=t(np.asarray(model.quads, dtype=np.int32)), xy=t(np.asarray(model.pos_m)[:, :2].astype(np.float64)),
+            tag=t(np.asarray(model.elem_tag, dtype=np.int32)), pml=_pml_arrays(pml_regions), npml=len(pml_regions),
+            omega_top=float(omega_top), N=int(model.N),
+            pos_rep=t(bl.pos[bl.rep].astype(np.int32)),
+            zmask=t(np.isin(np.arange(self.gdof), bl.zheng).astype(np.uint8)),
+            params=np.asarray(params, dtype=float), plane=plane)
+        self._grad['tensors'] = prm.lifted_tensors(params, plane)
+        self._grad['jac'] = prm.jacobian(params, plane)
+        # slot of the transposed entry on the (structurally symmetric) active pattern
+        n = self.n_active
+        r = np.repeat(np.arange(n, dtype=np.int64), np.diff(bl.aa_indptr))
+        c = bl.aa_indices.astype(np.int64)
+        key = r * n + c
+        tkey = c * n + r
+        self.tslot = t(np.searchsorted(key, tkey).astype(np.int64))
+        if not np.array_equal(key[np.searchsorted(key, tkey)], tkey):
+            raise ValueError('active pattern is not structurally symmetric')
+
+    def _param_gradient(self, W, omegas, V):
+        """dY11/dp (B, 17) from the element sensitivity sums W (B, NT)."""
+        from .piezo_fem import SCALE
+        g = self._grad
+        ndu = self.layout.nd - 1
+        NR = 2 * ndu
+        rows = [[0, 5], [8, 1], [7, 6]]
+        R = [rows[i // 2][i % 2] for i in range(NR)]
+        Wc = W[:, :NR * NR].reshape(-1, NR, NR)
+        We = W[:, NR * NR:NR * NR + 2 * NR].reshape(-1, 2, NR)
+        Wp = W[:, NR * NR + 2 * NR:].reshape(-1, 2, 2)
+        dC, dE, dP = g['jac']
+        C9 = g['tensors'][0]
+        dCs = dC[:, R][:, :, R]                      # (17, NR, NR)
+        dEs = dE[:, :2][:, :, R]                     # (17, 2, NR)
+        dPs = dP[:, :2, :2]
+        w = np.asarray(omegas)[:, None]
+        fac = (1 + 1j * w * self.beta)
+        dQ = (fac * SCALE * np.einsum('bij,kij->bk', Wc, dCs) + np.einsum('bij,kij->bk', We, dEs)
+              - np.einsum('bij,kij->bk', Wp, dPs))
+        dQ[:, 15] = (1j * w[:, 0]) * SCALE * np.einsum('bij,ij->b', Wc, C9[np.ix_(R, R)])
+        return -1j * w / V * dQ
+
     # ---------------------------------------------------------------- numeric
     def arena_bytes(self, batch, precision):
         return self.mf.arena_elems * batch * (8 if precision == 'mixed' else 16)
@@ -158,7 +225,8 @@ class FrequencySolver:
     def _dptr(self, t):
         return None if t is None else t.data_ptr()
 
-    def solve_batch(self, omegas, V=1.0, precision='mixed', ir_tol=1e-10, max_ir=8, nb=32, fallback=True):
+    def solve_batch(self, omegas, V=1.0, precision='mixed', ir_tol=1e-10, max_ir=8, nb=32, fallback=True,
+                    gradient=False):
         """Solve all ``omegas`` in one batched factorization.
 
         Returns dict with ``x`` (B, n) complex128 device tensor, ``Q`` (B,) complex numpy,
@@ -180,7 +248,7 @@ class FrequencySolver:
         _lib.call('sawfem_rhs', s, n, B, om.data_ptr(), self.beta, float(V), self.rK.data_ptr(), self.rM.data_ptr(),
                   self._dptr(self.rD), rhs.data_ptr())
         x = rhs.clone()
-        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), x.data_ptr())
+        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), x.data_ptr(), 0)
         Qd = torch.empty(B, dtype=torch.complex128, device=dev)
         charge = lambda: (_lib.call('sawfem_charge', s, n, B, self.kts_a.data_ptr(), self.kts_z.real,
                                     self.kts_z.imag, float(V), x.data_ptr(), Qd.data_ptr()),
@@ -190,40 +258,95 @@ class FrequencySolver:
         steps = 0
         dq = np.full(B, np.inf)
         for steps in range(1, max_ir + 1):
-            _lib.call('sawfem_residual', s, n, B, self.aa_indptr.data_ptr(), self.aa_indices.data_ptr(),
+            _lib.call('sawfem_residual', s, n, B, self.aa_indptr.data_ptr(), self.aa_indices.data_ptr(), None,
                       self.Kaa.data_ptr(), self.Maa.data_ptr(), self._dptr(self.Daa), om.data_ptr(), self.beta,
                       x.data_ptr(), rhs.data_ptr(), r.data_ptr())
-            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr())
+            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 0)
             _lib.call('sawfem_axpy', s, B * n, x.data_ptr(), r.data_ptr())
             Qn = charge()
             dq = np.abs(Qn - Q) / np.maximum(np.abs(Qn), 1e-300)
             Q = Qn
             if np.all(dq <= ir_tol) or (prec == 1 and steps >= 1 and np.all(dq <= 1e-13)):
                 break
+        out = dict(x=x, Q=Q, ir_steps=steps, dq=dq)
+        if gradient:
+            out.update(self._adjoint(om, omegas, V, prec, arena, uarena, rhs, x, ir_tol, max_ir))
         refactored = np.zeros(B, dtype=bool)
         if prec == 0 and fallback:
-            bad = np.flatnonzero(~(dq <= ir_tol))
+            bad = np.flatnonzero(~(dq <= ir_tol) | (~(out.get('adj_dq', dq) <= ir_tol) if gradient else False))
             if bad.size:
                 del arena, uarena
-                sub = self.solve_batch(omegas[bad], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir, nb=nb)
-                x[self.torch.as_tensor(bad, device=self.dev)] = sub['x']
-                Q[bad] = sub['Q']
-                dq[bad] = sub['dq']
+                sub = self.solve_batch(omegas[bad], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir, nb=nb,
+                                       gradient=gradient)
+                tb = self.torch.as_tensor(bad, device=self.dev)
+                x[tb] = sub['x']
+                for key in ('Q', 'dq', 'dY', 'adj_dq'):
+                    if key in out:
+                        out[key][bad] = sub[key]
                 refactored[bad] = True
-        return dict(x=x, Q=Q, ir_steps=steps, dq=dq, refactored=refactored)
+        out['refactored'] = refactored
+        return out
 
-    def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, **kw):
-        """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies."""
+    def _adjoint(self, om, omegas, V, prec, arena, uarena, rhs, x, ir_tol, max_ir):
+        """lambda = A^-T kts_a (refined), then K12 and the parameter contraction."""
+        torch = self.torch
+        g = self._grad
+        s = stream_ptr()
+        B, n = x.shape
+        gvec = self.kts_a.unsqueeze(0).expand(B, n).contiguous()
+        lam = gvec.clone()
+        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), lam.data_ptr(), 1)
+        # duality check: lambda^T b converges to kts_a^T x
+        mu = lambda: (lam * rhs).sum(dim=1).cpu().numpy()
+        m0 = mu()
+        r = torch.empty_like(lam)
+        dq = np.full(B, np.inf)
+        for _ in range(max_ir):
+            _lib.call('sawfem_residual', s, n, B, self.aa_indptr.data_ptr(), self.aa_indices.data_ptr(),
+                      self.tslot.data_ptr(), self.Kaa.data_ptr(), self.Maa.data_ptr(), self._dptr(self.Daa),
+                      om.data_ptr(), self.beta, lam.data_ptr(), gvec.data_ptr(), r.data_ptr())
+            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 1)
+            _lib.call('sawfem_axpy', s, B * n, lam.data_ptr(), r.data_ptr())
+            m1 = mu()
+            dq = np.abs(m1 - m0) / np.maximum(np.abs(m1), 1e-300)
+            m0 = m1
+            if np.all(dq <= ir_tol):
+                break
+        ndu = self.layout.nd - 1
+        NT = (2 * ndu) ** 2 + 4 * ndu + 4
+        work = torch.empty(max(1, B * g['ne0'] * NT), dtype=torch.complex128, device=self.dev)
+        W = torch.empty(B * NT, dtype=torch.complex128, device=self.dev)
+        tags, kinds, par = g['pml']
+        from .piezo_fem import SCALE
+        _lib.call('sawfem_adjoint_elem', s, g['ne0'], g['elems'].data_ptr(), g['conn'].data_ptr(), g['xy'].data_ptr(),
+                  g['tag'].data_ptr(), g['npml'], tags.ctypes.data, kinds.ctypes.data, par.ctypes.data,
+                  g['omega_top'], ndu, g['N'], g['pos_rep'].data_ptr(), g['zmask'].data_ptr(), float(V), SCALE, B, n,
+                  x.data_ptr(), lam.data_ptr(), work.data_ptr(), W.data_ptr())
+        Wh = W.cpu().numpy().reshape(B, NT)
+        return dict(dY=self._param_gradient(Wh, omegas, V), adj_dq=dq, lam=lam)
+
+    def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, **kw):
+        """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies.
+
+        With ``gradient=True`` returns (Q, dY, info) where dY (P, 17) is dY11/dp.
+        """
         fre = np.asarray(fre, dtype=np.float64)
         batch = batch or fre.size
         Q = np.empty(fre.size, dtype=complex)
+        dY = np.empty((fre.size, 17), dtype=complex) if gradient else None
         info = []
         for i0 in range(0, fre.size, batch):
-            res = self.solve_batch(2 * np.pi * fre[i0:i0 + batch], V, precision=precision, **kw)
+            res = self.solve_batch(2 * np.pi * fre[i0:i0 + batch], V, precision=precision, gradient=gradient, **kw)
             Q[i0:i0 + batch] = res['Q']
-            info.append(dict(ir_steps=res['ir_steps'], dq_max=float(np.max(res['dq'])),
-                             refactored=int(res['refactored'].sum())))
+            item = dict(ir_steps=res['ir_steps'], dq_max=float(np.max(res['dq'])),
+                        refactored=int(res['refactored'].sum()))
+            if gradient:
+                dY[i0:i0 + batch] = res['dY']
+                item['adj_dq_max'] = float(np.max(res['adj_dq']))
+            info.append(item)
             del res
+        if gradient:
+            return Q, dY, info
         return Q, info
 
     def expand(self, x, V=1.0):

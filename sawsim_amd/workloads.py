@@ -27,6 +27,7 @@ import numpy as np
 
 from . import materials as mt
 from .mesh import CellMesh, double_layer_cell, quad_layer_cell, stack_cell, stack_tag_map
+from .params import from_constants
 from .piezo_fem import Model, PMLRegion, material_index
 
 SP_BASELINE_3M = dict(C11=232.966, C12=46.8904, C13=80.2342, C14=-11.0267, C33=275.364, C44=93.8995,
@@ -47,6 +48,7 @@ class Cell:
     beta_dk: float = 0.0
     eta_eps: float = 0.0
     info: dict = field(default_factory=dict)
+    params: np.ndarray = None          # 17 design parameters of the piezo material (params.NAMES)
 
     @property
     def nd(self):
@@ -93,8 +95,10 @@ def config1(seed=0, mesh_um=0.045, points=101, band=(1.63e9, 1.915e9), beta_dk=0
     mesh = stack_cell(pitch, 0.5, 0.08 * pitch, 0.6, [5 * pitch - 0.6], [], mesh_um, pml_pitches=1.0)
     k = jittered_constants(seed)
     mats = [piezo_material(k, euler, eta_eps), _al(), mt.isotropic(130e9, 0.28, 2330.0, 11.7)]
-    return _stack(mesh, mats, 'xy', 2, band, points, beta_dk, eta_eps,
-                  info=dict(name='config1', constants=k, euler=euler, seed=seed))
+    c = _stack(mesh, mats, 'xy', 2, band, points, beta_dk, eta_eps,
+               info=dict(name='config1', constants=k, euler=euler, seed=seed))
+    c.params = from_constants(k, euler, beta_dk, eta_eps)
+    return c
 
 
 def config2(seed=0, mesh_um=0.0225, points=401, band=(1.55e9, 1.85e9)):
@@ -104,7 +108,9 @@ def config2(seed=0, mesh_um=0.0225, points=401, band=(1.55e9, 1.85e9)):
     euler = (0, -38, 0)
     mats = [piezo_material(k, euler), _al(), mt.isotropic(130e9, 0.28, 2330.0, 11.7),
             mt.isotropic(72e9, 0.17, 2200.0, 4.2)]
-    return _stack(mesh, mats, 'xy', 2, band, points, info=dict(name='config2', constants=k, euler=euler, seed=seed))
+    c = _stack(mesh, mats, 'xy', 2, band, points, info=dict(name='config2', constants=k, euler=euler, seed=seed))
+    c.params = from_constants(k, euler)
+    return c
 
 
 def config3(seed=0, mesh_um=0.045, points=201, band=(1.63e9, 1.915e9)):
@@ -144,3 +150,37 @@ def quad_layer(points=5, band=(1.5e9, 2.7e9)):
 
 
 WORKLOADS = dict(config1=config1, config2=config2, config3=config3, double_layer=double_layer, quad_layer=quad_layer)
+
+
+def assemble_cell(cell: Cell, materials=None):
+    """K, M, D on the GPU for a workload cell (D = K_uu of the piezo, the damping operator)."""
+    from .piezo_fem import assemble_device
+    mats = materials if materials is not None else cell.materials
+    nd = cell.ndisp + 1
+    if nd == 3:
+        mats = [mt.lift_projected(m) for m in mats]
+    mask = np.zeros(len(mats), dtype=bool)
+    mask[0] = True
+    return assemble_device(cell.model, mats, cell.mat_idx, cell.pml, 2 * np.pi * cell.band_hz[1], nd,
+                           damping_mask=mask)
+
+
+def build_engine(cell: Cell, gradient=False, **kw):
+    """GPU FrequencySolver for a cell, with damping and (optionally) the dY11/dp context."""
+    from .sweep import FrequencySolver, bc_periodic
+    K, M, D = assemble_cell(cell)
+    eng = FrequencySolver(K, M, *bc_periodic(cell.model, cell.mesh.lines, ndisp=cell.ndisp), **kw)
+    eng.set_damping(D, cell.beta_dk)
+    if gradient:
+        if cell.params is None:
+            raise ValueError('this cell has no design parameters')
+        eng.enable_gradient(cell.model, cell.mat_idx, cell.pml, 2 * np.pi * cell.band_hz[1], cell.params, 'xy')
+    return eng
+
+
+def materials_for_params(cell: Cell, p):
+    """The cell's material list with the piezo rebuilt from parameter vector ``p``."""
+    from .params import NAMES
+    k = {name: float(p[i]) for i, name in enumerate(NAMES[:12])}
+    pz = piezo_material(k, tuple(float(a) for a in p[12:15]), float(p[16]))
+    return [pz] + list(cell.materials[1:])

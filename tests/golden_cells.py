@@ -39,3 +39,21 @@ def tiny_cell():
                 ndisp=2, band_hz=(1.6e9, float(g['omega_top']) / (2 * np.pi)), points=3,
                 beta_dk=p['beta_dk'], eta_eps=p['tand'])
     return cell, g, p
+
+
+def cfg3_coarse_cell():
+    """Config-3 cell of golden_cfg3_coarse.npz (unjittered sp_baseline constants, mesh 0.09 µm)."""
+    from sawsim_amd.mesh import stack_cell, stack_tag_map
+    from sawsim_amd.piezo_fem import material_index
+    g = np.load(GOLD / 'golden_cfg3_coarse.npz')
+    p = dict(zip(NAMES, g['params']))
+    mesh = stack_cell(1.085, 0.5, 0.08 * 1.085, 0.6, [5 * 1.085 - 0.6], [], float(g['mesh_um']), pml_pitches=1.0)
+    model = Model(mesh.pos_m, mesh.quads, mesh.elem_tag, mesh.N)
+    bottom = float(mesh.pos_m[:, 1].min())
+    pml = [PMLRegion(tag=mesh.info['tag_pml'], kind='y', yp=bottom, ya=bottom + mesh.info['pml_thickness_m'])]
+    f = g['frequency_hz']
+    cell = Cell(mesh=mesh, model=model, materials=materials_from_params(p), mat_idx=material_index(model, stack_tag_map(mesh)),
+                pml=pml, ndisp=2, band_hz=(float(f[0]), float(f[-1])), points=len(f), beta_dk=p['beta_dk'],
+                eta_eps=p['tand'])
+    cell.params = np.array([p[k] for k in NAMES])
+    return cell, g
