@@ -89,6 +89,15 @@ int sawfem_residual(void* stream, int n, int B, const int64_t* indptr, const int
                     const void* Kaa, const void* Maa, const void* Daa, const double* omega, double beta,
                     const void* x, const void* rhs, void* r);
 
+/* Same residual in the frequency-contiguous layout: xt, rhst, rt are n x Bp complex128
+ * (row-major, Bp >= B a multiple of 64); one wave per row and 64 frequencies. */
+int sawfem_residual_t(void* stream, int n, int B, int Bp, const int64_t* indptr, const int* indices,
+                      const int64_t* tslot, const void* Kaa, const void* Maa, const void* Daa, const double* omega,
+                      double beta, const void* xt, const void* rhst, void* rt);
+
+/* out (C rows, leading dimension ld_out) = transpose of in (R rows, leading dimension ld_in). */
+int sawfem_transpose(void* stream, int R, int C, int ld_in, int ld_out, const void* in, void* out);
+
 /* Q[b] = sum_i kts_a[i] x[b][i] + V kts_z  (sweep.py:356-362). */
 int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, double kz_im, double V,
                   const void* x, void* Q);

@@ -32,6 +32,8 @@ SIGNATURES = {
     'sawfem_mf_profile_read': [_vp, _vp, _vp],
     'sawfem_rhs': [_vp, _i, _i, _vp, _d, _d, _vp, _vp, _vp, _vp],
     'sawfem_residual': [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
+    'sawfem_residual_t': [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
+    'sawfem_transpose': [_vp, _i, _i, _i, _i, _vp, _vp],
     'sawfem_charge': [_vp, _i, _i, _vp, _d, _d, _d, _vp, _vp],
     'sawfem_axpy': [_vp, _i64, _vp, _vp],
     'sawfem_adjoint_elem': [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _i, _vp, _vp, _d, _d, _i, _i,

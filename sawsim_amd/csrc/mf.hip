@@ -114,10 +114,15 @@ __global__ void k_front_entries(FrontsDev F, int64_t e0, int64_t e1, int B, cons
 }
 
 // ---------------------------------------------------------------- panel factorization
+// LU of the kb x kb diagonal block of pivot panel [k0, k0 + kb) (no pivoting; a pivot below
+// `tiny` is replaced by `tiny` and counted), then the inverses of its unit-lower and upper
+// factors written to the scratch slot of this (front, frequency): Linv then Uinv, each
+// column-major with leading dimension nb.  Threads 0..nb-1 build one column of Linv each,
+// threads 64..64+nb-1 one column of Uinv, with uniform loops (broadcast LDS reads).
 template <typename T>
-__global__ void __launch_bounds__(64) k_diag(FrontsDev F, const int* __restrict__ lvl, int B,
-                                             typename cplx<T>::type* __restrict__ arena, int k0, int nb,
-                                             T tiny, int* __restrict__ npert) {
+__global__ void __launch_bounds__(128) k_diag(FrontsDev F, const int* __restrict__ lvl, int B,
+                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb,
+                                              T tiny, int* __restrict__ npert, typename cplx<T>::type* __restrict__ inv) {
   using V = typename cplx<T>::type;
   extern __shared__ unsigned char smem_raw[];
   V* Ds = reinterpret_cast<V*>(smem_raw);   // [nb][nb+1], column-major with padding
@@ -137,7 +142,6 @@ __global__ void __launch_bounds__(64) k_diag(FrontsDev F, const int* __restrict_
     V piv = Ds[k + k * ld];
     const T mag = piv.x * piv.x + piv.y * piv.y;
     if (!(mag > tiny * tiny)) {
-      // static pivot perturbation; refinement against the FP64 operator repairs it
       piv.x = tiny; piv.y = 0;
       if (t == 0) { Ds[k + k * ld] = piv; atomicAdd(npert, 1); }
     }
@@ -150,69 +154,24 @@ __global__ void __launch_bounds__(64) k_diag(FrontsDev F, const int* __restrict_
   }
   for (int j = 0; j < kb; ++j)
     if (t < kb) Fb[(int64_t)(k0 + t) + (int64_t)(k0 + j) * m] = Ds[t + j * ld];
-}
-
-// L panel rows [k1, m) x cols [k0, k1): X U_D = A.   U panel rows [k0, k1) x cols [k1, m): L_D X = A.
-template <typename T>
-__global__ void __launch_bounds__(64) k_trsm(FrontsDev F, const int* __restrict__ lvl, int B,
-                                             typename cplx<T>::type* __restrict__ arena, int k0, int nb, int ntmax) {
-  using V = typename cplx<T>::type;
-  extern __shared__ unsigned char smem_raw[];
-  const int ld = nb + 1;
-  V* Ds = reinterpret_cast<V*>(smem_raw);      // [nb][ld] diag block, column-major
-  V* Xs = Ds + nb * ld;                        // [64][ld]: row r of the tile at Xs[r * ld + j]
-  const int f = lvl[blockIdx.y];
-  const int b = blockIdx.z;
-  const int p = F.p[f];
-  if (k0 >= p) return;
-  const int m = F.m[f];
-  const int kb = min(nb, p - k0);
-  const int k1 = k0 + kb;
-  const int nt = (m - k1 + 63) / 64;
-  const bool upanel = blockIdx.x >= ntmax;
-  const int tile = upanel ? blockIdx.x - ntmax : blockIdx.x;
-  if (tile >= nt) return;
-  V* Fb = front_ptr<T>(arena, F, f, b, B);
-  const int t = threadIdx.x;
-  for (int j = 0; j < kb; ++j)
-    if (t < kb) Ds[t + j * ld] = Fb[(int64_t)(k0 + t) + (int64_t)(k0 + j) * m];
-  const int r0 = k1 + tile * 64;
-  const int nr = min(64, m - r0);
-  if (!upanel) {
-    for (int j = 0; j < kb; ++j)
-      if (t < nr) Xs[t * ld + j] = Fb[(int64_t)(r0 + t) + (int64_t)(k0 + j) * m];
-  } else {
-    // column c = r0 + cc of the U panel, row k0 + i: stage as Xs[cc * ld + i]
-    for (int idx = t; idx < nr * kb; idx += 64) {
-      const int cc = idx / kb, i = idx % kb;
-      Xs[cc * ld + i] = Fb[(int64_t)(k0 + i) + (int64_t)(r0 + cc) * m];
+  V* slot = inv + ((int64_t)blockIdx.x * B + b) * 2 * nb * nb;
+  if (t < kb) {
+    // column j of L^{-1}: x_i = delta_ij - sum_{k<i} L[i][k] x_k
+    const int j = t;
+    V* x = slot + (int64_t)j * nb;
+    for (int i = 0; i < kb; ++i) {
+      V s = (i == j) ? V{1, 0} : czero<V>();
+      for (int k = 0; k < i; ++k) s = cfms(s, Ds[i + k * ld], (k >= j) ? x[k] : czero<V>());
+      x[i] = (i >= j) ? s : czero<V>();
     }
-  }
-  __syncthreads();
-  if (t < nr) {
-    V* x = Xs + t * ld;
-    if (!upanel) {
-      for (int j = 0; j < kb; ++j) {
-        V s = x[j];
-        for (int i = 0; i < j; ++i) s = cfms(s, x[i], Ds[i + j * ld]);
-        x[j] = cdiv(s, Ds[j + j * ld]);
-      }
-    } else {
-      for (int i = 0; i < kb; ++i) {
-        V s = x[i];
-        for (int j = 0; j < i; ++j) s = cfms(s, Ds[i + j * ld], x[j]);
-        x[i] = s;
-      }
-    }
-  }
-  __syncthreads();
-  if (!upanel) {
-    for (int j = 0; j < kb; ++j)
-      if (t < nr) Fb[(int64_t)(r0 + t) + (int64_t)(k0 + j) * m] = Xs[t * ld + j];
-  } else {
-    for (int idx = t; idx < nr * kb; idx += 64) {
-      const int cc = idx / kb, i = idx % kb;
-      Fb[(int64_t)(k0 + i) + (int64_t)(r0 + cc) * m] = Xs[cc * ld + i];
+  } else if (t >= 64 && t < 64 + kb) {
+    // column j of U^{-1}: x_i = (delta_ij - sum_{k>i} U[i][k] x_k) / U[i][i]
+    const int j = t - 64;
+    V* x = slot + (int64_t)nb * nb + (int64_t)j * nb;
+    for (int i = kb - 1; i >= 0; --i) {
+      V s = (i == j) ? V{1, 0} : czero<V>();
+      for (int k = i + 1; k < kb; ++k) s = cfms(s, Ds[i + k * ld], (k <= j) ? x[k] : czero<V>());
+      x[i] = (i <= j) ? cdiv(s, Ds[i + i * ld]) : czero<V>();
     }
   }
 }
@@ -288,9 +247,13 @@ template <> struct WaveAcc<double> {
 
 // mode 0: L-shaped update for panel [k0, k0+kb)  (skips rows >= p and cols >= p together)
 // mode 1: Schur complement rows/cols [p, m), K = [0, p)
+// mode 2: L panel  F[k1:m, k0:k1] = F[k1:m, k0:k1] * Uinv      (in place, overwrite)
+// mode 3: U panel  F[k0:k1, k1:m] = Linv * F[k0:k1, k1:m]      (in place, overwrite)
+// The inverses come from k_diag's scratch slot of this (front, frequency).
 template <typename T>
 __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
-                                              typename cplx<T>::type* __restrict__ arena, int mode, int k0, int nb) {
+                                              typename cplx<T>::type* __restrict__ arena, int mode, int k0, int nb,
+                                              const typename cplx<T>::type* __restrict__ inv) {
   using V = typename cplx<T>::type;
   __shared__ V As[KC * TILE];
   __shared__ V Bs[TILE * (KC + 1)];
@@ -298,39 +261,58 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
   const int b = blockIdx.z;
   const int p = F.p[f];
   const int m = F.m[f];
-  int lo, kbeg, kend;
-  if (mode == 0) {
-    if (k0 >= p) return;
-    const int kb = min(nb, p - k0);
-    lo = k0 + kb; kbeg = k0; kend = k0 + kb;
-  } else {
+  // C rows [r_lo, r_hi) x cols [c_lo, c_hi); reduction length K
+  int r_lo, r_hi, c_lo, c_hi, K, kb = 0;
+  if (mode == 1) {
     if (p == 0) return;
-    lo = p; kbeg = 0; kend = p;
+    r_lo = c_lo = p; r_hi = c_hi = m; K = p;
+  } else {
+    if (k0 >= p) return;
+    kb = min(nb, p - k0);
+    const int k1 = k0 + kb;
+    if (mode == 0) { r_lo = c_lo = k1; r_hi = c_hi = m; K = kb; }
+    else if (mode == 2) { r_lo = k1; r_hi = m; c_lo = k0; c_hi = k1; K = kb; }
+    else { r_lo = k0; r_hi = k1; c_lo = k1; c_hi = m; K = kb; }
   }
-  if (lo >= m) return;
-  const int nt = (m - lo + TILE - 1) / TILE;
-  const int ti = blockIdx.x % nt, tj = blockIdx.x / nt;
-  if (tj >= nt) return;
-  const int r0 = lo + ti * TILE, c0 = lo + tj * TILE;
+  if (r_lo >= r_hi || c_lo >= c_hi) return;
+  const int ntr = (r_hi - r_lo + TILE - 1) / TILE, ntc = (c_hi - c_lo + TILE - 1) / TILE;
+  const int ti = blockIdx.x % ntr, tj = blockIdx.x / ntr;
+  if (tj >= ntc) return;
+  const int r0 = r_lo + ti * TILE, c0 = c_lo + tj * TILE;
   if (mode == 0 && r0 >= p && c0 >= p) return;
   V* Fb = front_ptr<T>(arena, F, f, b, B);
+  const V* slot = (mode >= 2) ? inv + ((int64_t)blockIdx.y * B + b) * 2 * nb * nb : nullptr;
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63;
   const int wrow = wave & 1, wcol = wave >> 1;
   WaveAcc<T> acc;
   acc.zero();
-  for (int kc = kbeg; kc < kend; kc += KC) {
-    // A chunk: rows r0.., k in [kc, kc+KC) -> As[k][row]
+  for (int kc = 0; kc < K; kc += KC) {
+    // A chunk -> As[k][row]
     for (int idx = t; idx < KC * TILE; idx += 256) {
       const int row = idx % TILE, k = idx / TILE;
       const int gr = r0 + row, gk = kc + k;
-      As[k * TILE + row] = (gr < m && gk < kend) ? Fb[(int64_t)gr + (int64_t)gk * m] : czero<V>();
+      V v = czero<V>();
+      if (gr < r_hi && gk < K) {
+        if (mode == 0) v = Fb[(int64_t)gr + (int64_t)(k0 + gk) * m];
+        else if (mode == 1) v = Fb[(int64_t)gr + (int64_t)gk * m];
+        else if (mode == 2) v = Fb[(int64_t)gr + (int64_t)(k0 + gk) * m];
+        else v = slot[(int64_t)(gr - k0) + (int64_t)gk * nb];                 // Linv[row][k]
+      }
+      As[k * TILE + row] = v;
     }
-    // B chunk: k in [kc, kc+KC), cols c0.. -> Bs[col][k]
+    // B chunk -> Bs[col][k]
     for (int idx = t; idx < KC * TILE; idx += 256) {
       const int k = idx % KC, col = idx / KC;
       const int gk = kc + k, gc = c0 + col;
-      Bs[col * (KC + 1) + k] = (gc < m && gk < kend) ? Fb[(int64_t)gk + (int64_t)gc * m] : czero<V>();
+      V v = czero<V>();
+      if (gc < c_hi && gk < K) {
+        if (mode == 0) v = Fb[(int64_t)(k0 + gk) + (int64_t)gc * m];
+        else if (mode == 1) v = Fb[(int64_t)gk + (int64_t)gc * m];
+        else if (mode == 2) v = slot[(int64_t)nb * nb + gk + (int64_t)(gc - k0) * nb];   // Uinv[k][col]
+        else v = Fb[(int64_t)(k0 + gk) + (int64_t)gc * m];
+      }
+      Bs[col * (KC + 1) + k] = v;
     }
     __syncthreads();
     acc.step(As, Bs, wrow, wcol, lane);
@@ -339,173 +321,221 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
   acc.each(lane, [&](int j, int i, T re, T im) {
     const int gr = r0 + wrow * 32 + j;
     const int gc = c0 + wcol * 32 + i;
-    if (gr < m && gc < m && !(mode == 0 && gr >= p && gc >= p)) {
+    if (gr < r_hi && gc < c_hi && !(mode == 0 && gr >= p && gc >= p)) {
       V* c = Fb + (int64_t)gr + (int64_t)gc * m;
-      V v = *c;
-      v.x -= re; v.y -= im;
-      *c = v;
+      if (mode >= 2) {
+        V v; v.x = re; v.y = im;
+        *c = v;
+      } else {
+        V v = *c;
+        v.x -= re; v.y -= im;
+        *c = v;
+      }
     }
   });
 }
 
 // ---------------------------------------------------------------- triangular solves
-// x: [B][n] complex128, in place: forward writes y at pivot rows, backward overwrites with x.
-template <typename T>
-__global__ void __launch_bounds__(256) k_fwd(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+// Per level, each direction runs two kernels:
+//   * a pivot-block kernel, one workgroup per (front, frequency): the p x p triangle solved in
+//     SB-wide diagonal blocks staged in LDS (wave 0 solves a block in registers, lane i owning
+//     row i, values broadcast by lane shuffles), the remaining pivot rows updated by all waves;
+//   * a panel kernel over 64-wide tiles of the off-diagonal panel (L21, U12 or their
+//     transposes), many workgroups per front, coalesced loads (transposed panels go through
+//     an LDS tile).
+// Vectors: x [B][n] complex128 (pivot rows of each front are owned by that front); the update
+// vector of front f (its contribution to the parent's rows) lives in uarena.
+template <typename T> struct SolveTile;
+template <> struct SolveTile<float> { static constexpr int SB = 64; };
+template <> struct SolveTile<double> { static constexpr int SB = 32; };
+
+__device__ __forceinline__ double2 shfl2(double2 v, int src) {
+  return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
+}
+
+// w[i] (i < p) = x[rows[i]] + children's update entries; uf[i - p] (i >= p) = children's entries.
+__device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, int B, int m, int p, const int* rows,
+                                             const double2* xb, const double2* uarena, double2* w, double2* uf) {
+  const int nch = F.child_ptr[f + 1] - F.child_ptr[f];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    double2 v = (i < p) ? xb[rows[i]] : make_double2(0, 0);
+    for (int c = 0; c < nch; ++c) {
+      const int slot = F.child_ptr[f] + c;
+      const int ci = F.inv[F.inv_ptr[slot] + i];
+      if (ci >= 0) {
+        const int ch = F.child_ids[slot];
+        const double2 u = uarena[F.uoff[ch] * B + (int64_t)b * (F.m[ch] - F.p[ch]) + ci];
+        v.x += u.x; v.y += u.y;
+      }
+    }
+    if (i < p) w[i] = v; else uf[i - p] = v;
+  }
+}
+
+// Pivot triangle.  kind 0: L11 (unit lower, forward)   1: U11 (upper, backward)
+//                  2: U11^T (lower, forward)            3: L11^T (unit upper, backward)
+// Forward kinds gather the right-hand side and the children's updates first.
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict__ lvl, int B, int n,
                                              const typename cplx<T>::type* __restrict__ arena,
                                              double2* __restrict__ uarena, double2* __restrict__ x) {
   using V = typename cplx<T>::type;
+  constexpr int SB = SolveTile<T>::SB;
+  constexpr int LD = SB + 1;
   extern __shared__ unsigned char smem_raw[];
-  double2* w = reinterpret_cast<double2*>(smem_raw);
+  V* tile = reinterpret_cast<V*>(smem_raw);                           // [SB][LD] or [64][LD]
+  double2* w = reinterpret_cast<double2*>(smem_raw + sizeof(V) * 64 * LD);
   const int f = lvl[blockIdx.x];
   const int b = blockIdx.y;
   const int m = F.m[f], p = F.p[f];
   const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
   double2* xb = x + (int64_t)b * n;
-  const int nch = F.child_ptr[f + 1] - F.child_ptr[f];
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    double2 v = (i < p) ? xb[rows[i]] : make_double2(0, 0);
-    for (int c = 0; c < nch; ++c) {
-      const int slot = F.child_ptr[f] + c;
-      const int ci = F.inv[F.inv_ptr[slot] + i];
-      if (ci >= 0) {
-        const int ch = F.child_ids[slot];
-        const double2 u = uarena[F.uoff[ch] * B + (int64_t)b * (F.m[ch] - F.p[ch]) + ci];
-        v.x += u.x; v.y += u.y;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  constexpr bool fwd = (KIND == 0 || KIND == 2);
+  if (fwd) {
+    solve_gather(F, f, b, B, m, p, rows, xb, uarena, w, uarena + F.uoff[f] * B + (int64_t)b * (m - p));
+  } else {
+    for (int i = t; i < p; i += blockDim.x) w[i] = xb[rows[i]];
+  }
+  __syncthreads();
+  const int nblk = (p + SB - 1) / SB;
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int J0 = fwd ? bi * SB : (nblk - 1 - bi) * SB;
+    const int nbk = min(SB, p - J0);
+    // diagonal block to LDS: tile[r + c * LD] = F[J0 + r][J0 + c]
+    for (int idx = t; idx < SB * SB; idx += blockDim.x) {
+      const int r = idx % SB, c = idx / SB;
+      if (r < nbk && c < nbk) tile[r + c * LD] = Fb[(int64_t)(J0 + r) + (int64_t)(J0 + c) * m];
+    }
+    __syncthreads();
+    if (wave == 0) {
+      double2 v = lane < nbk ? w[J0 + lane] : make_double2(0, 0);
+      if (KIND == 0) {
+        for (int k = 0; k < nbk; ++k) {
+          const double2 vk = shfl2(v, k);
+          if (lane > k && lane < nbk) v = cfms(v, to_d2(tile[lane + k * LD]), vk);
+        }
+      } else if (KIND == 1) {
+        for (int k = nbk - 1; k >= 0; --k) {
+          const double2 xk = cdiv(shfl2(v, k), to_d2(tile[k + k * LD]));
+          if (lane == k) v = xk;
+          else if (lane < k) v = cfms(v, to_d2(tile[lane + k * LD]), xk);
+        }
+      } else if (KIND == 2) {
+        for (int k = 0; k < nbk; ++k) {
+          const double2 wk = cdiv(shfl2(v, k), to_d2(tile[k + k * LD]));
+          if (lane == k) v = wk;
+          else if (lane > k && lane < nbk) v = cfms(v, to_d2(tile[k + lane * LD]), wk);
+        }
+      } else {
+        for (int k = nbk - 1; k >= 0; --k) {
+          const double2 lk = shfl2(v, k);
+          if (lane < k) v = cfms(v, to_d2(tile[k + lane * LD]), lk);
+        }
+      }
+      if (lane < nbk) w[J0 + lane] = v;
+    }
+    __syncthreads();
+    // remaining pivot rows: forward kinds update [J0 + nbk, p), backward kinds [0, J0)
+    const int lo = fwd ? J0 + nbk : 0, hi = fwd ? p : J0;
+    if (KIND == 0 || KIND == 1) {
+      // column panel F[lo:hi, J0:J0+nbk] (contiguous rows): thread per row
+      for (int r = lo + t; r < hi; r += blockDim.x) {
+        double2 s = w[r];
+        for (int k = 0; k < nbk; ++k) s = cfms(s, to_d2(Fb[(int64_t)r + (int64_t)(J0 + k) * m]), w[J0 + k]);
+        w[r] = s;
+      }
+    } else {
+      // row panel F[J0:J0+nbk, lo:hi] read as its transpose through a 64-column LDS tile
+      for (int c0 = lo; c0 < hi; c0 += 64) {
+        const int nc = min(64, hi - c0);
+        __syncthreads();
+        for (int idx = t; idx < SB * 64; idx += blockDim.x) {
+          const int k = idx % SB, c = idx / SB;
+          if (k < nbk && c < nc) tile[c * LD + k] = Fb[(int64_t)(J0 + k) + (int64_t)(c0 + c) * m];
+        }
+        __syncthreads();
+        if (t < nc) {
+          double2 s = w[c0 + t];
+          for (int k = 0; k < nbk; ++k) s = cfms(s, to_d2(tile[t * LD + k]), w[J0 + k]);
+          w[c0 + t] = s;
+        }
       }
     }
-    w[i] = v;
-  }
-  __syncthreads();
-  for (int k = 0; k < p; ++k) {
-    const double2 wk = w[k];
-    for (int i = k + 1 + threadIdx.x; i < p; i += blockDim.x)
-      w[i] = cfms(w[i], to_d2(Fb[(int64_t)i + (int64_t)k * m]), wk);
     __syncthreads();
   }
-  double2* uf = uarena + F.uoff[f] * B + (int64_t)b * (m - p);
-  for (int i = p + threadIdx.x; i < m; i += blockDim.x) {
-    double2 s = w[i];
-    for (int k = 0; k < p; ++k) s = cfms(s, to_d2(Fb[(int64_t)i + (int64_t)k * m]), w[k]);
-    uf[i - p] = s;
-  }
-  for (int i = threadIdx.x; i < p; i += blockDim.x) xb[rows[i]] = w[i];
+  for (int i = t; i < p; i += blockDim.x) xb[rows[i]] = w[i];
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_bwd(FrontsDev F, const int* __restrict__ lvl, int B, int n,
-                                             const typename cplx<T>::type* __restrict__ arena,
-                                             double2* __restrict__ x) {
+// Off-diagonal panels, one workgroup per 64-wide output tile of one front and frequency.
+//  kind 0 (forward,  L21):   uf[r]  -= sum_k L[p + r][k] y[k]        r in CB rows
+//  kind 1 (backward, U12):   x_p[r] -= sum_j U[r][p + j] x_cb[j]      r in pivot rows
+//  kind 2 (forward,  U12^T): uf[j]  -= sum_k U[k][p + j] w[k]         j in CB rows
+//  kind 3 (backward, L21^T): x_p[k] -= sum_j L[p + j][k] x_cb[j]      k in pivot rows
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) k_panel(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+                                               const typename cplx<T>::type* __restrict__ arena,
+                                               double2* __restrict__ uarena, double2* __restrict__ x) {
   using V = typename cplx<T>::type;
   extern __shared__ unsigned char smem_raw[];
-  double2* z = reinterpret_cast<double2*>(smem_raw);   // [m]: z[0:p] pivots, z[p:m] known x
-  const int f = lvl[blockIdx.x];
-  const int b = blockIdx.y;
+  const int f = lvl[blockIdx.y];
+  const int b = blockIdx.z;
   const int m = F.m[f], p = F.p[f];
+  const int q = m - p;
+  const bool out_cb = (KIND == 0 || KIND == 2);
+  const int nout = out_cb ? q : p;     // outputs
+  const int nin = out_cb ? p : q;      // reduction length
+  const int o0 = blockIdx.x * 64;
+  if (o0 >= nout || nin == 0) return;
+  const int no = min(64, nout - o0);
   const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
   double2* xb = x + (int64_t)b * n;
-  for (int i = threadIdx.x; i < m; i += blockDim.x) z[i] = xb[rows[i]];
+  double2* uf = uarena + F.uoff[f] * B + (int64_t)b * q;
+  double2* v = reinterpret_cast<double2*>(smem_raw);        // [nin] input vector
+  double2* part = v + nin;                                    // [4][64] partial sums
+  const int t = threadIdx.x;
+  for (int i = t; i < nin; i += blockDim.x) v[i] = xb[rows[(out_cb ? 0 : p) + i]];
   __syncthreads();
-  for (int i = threadIdx.x; i < p; i += blockDim.x) {
-    double2 s = z[i];
-    for (int j = p; j < m; ++j) s = cfms(s, to_d2(Fb[(int64_t)i + (int64_t)j * m]), z[j]);
-    z[i] = s;
-  }
-  __syncthreads();
-  // every thread reads z[k] before anyone writes; solutions go to a separate array
-  double2* sol = z + m;
-  for (int k = p - 1; k >= 0; --k) {
-    const double2 xk = cdiv(z[k], to_d2(Fb[(int64_t)k + (int64_t)k * m]));
-    for (int i = threadIdx.x; i < k; i += blockDim.x) z[i] = cfms(z[i], to_d2(Fb[(int64_t)i + (int64_t)k * m]), xk);
-    if (threadIdx.x == 0) sol[k] = xk;
-    __syncthreads();
-  }
-  for (int i = threadIdx.x; i < p; i += blockDim.x) xb[rows[i]] = sol[i];
-}
-
-
-// Transposed solves for the adjoint A^T lambda = g with A = L U:  U^T w = g (bottom-up),
-// then L^T lambda = w (top-down).  Same storage and maps as k_fwd / k_bwd.
-template <typename T>
-__global__ void __launch_bounds__(256) k_fwdT(FrontsDev F, const int* __restrict__ lvl, int B, int n,
-                                              const typename cplx<T>::type* __restrict__ arena,
-                                              double2* __restrict__ uarena, double2* __restrict__ x) {
-  using V = typename cplx<T>::type;
-  extern __shared__ unsigned char smem_raw[];
-  double2* w = reinterpret_cast<double2*>(smem_raw);
-  const int f = lvl[blockIdx.x];
-  const int b = blockIdx.y;
-  const int m = F.m[f], p = F.p[f];
-  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
-  const int* rows = F.rows + F.rows_ptr[f];
-  double2* xb = x + (int64_t)b * n;
-  const int nch = F.child_ptr[f + 1] - F.child_ptr[f];
-  for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    double2 v = (i < p) ? xb[rows[i]] : make_double2(0, 0);
-    for (int c = 0; c < nch; ++c) {
-      const int slot = F.child_ptr[f] + c;
-      const int ci = F.inv[F.inv_ptr[slot] + i];
-      if (ci >= 0) {
-        const int ch = F.child_ids[slot];
-        const double2 u = uarena[F.uoff[ch] * B + (int64_t)b * (F.m[ch] - F.p[ch]) + ci];
-        v.x += u.x; v.y += u.y;
+  const int r = t & 63, g = t >> 6;   // output within tile, reduction quarter
+  double2 s = make_double2(0, 0);
+  if (KIND == 0 || KIND == 1) {
+    // direct: output o is a row of the panel; coalesced across r
+    const int row = (KIND == 0 ? p : 0) + o0 + r;
+    const int cbase = KIND == 0 ? 0 : p;
+    if (r < no)
+      for (int k = g; k < nin; k += 4) s = cfma(s, to_d2(Fb[(int64_t)row + (int64_t)(cbase + k) * m]), v[k]);
+  } else {
+    // transposed: output o is a column of the stored panel; stage 64 x 64 tiles through LDS
+    constexpr int KC2 = SolveTile<T>::SB;                      // reduction chunk
+    V* tile = reinterpret_cast<V*>(part + 256);                // [64][KC2 + 1]
+    const int rbase = KIND == 2 ? 0 : p;                       // stored rows (reduction index)
+    const int cbase = (KIND == 2 ? p : 0) + o0;                // stored columns (outputs)
+    for (int k0 = 0; k0 < nin; k0 += KC2) {
+      const int nk = min(KC2, nin - k0);
+      __syncthreads();
+      for (int idx = t; idx < 64 * KC2; idx += blockDim.x) {
+        const int kk = idx % KC2, c = idx / KC2;
+        if (kk < nk && c < no) tile[c * (KC2 + 1) + kk] = Fb[(int64_t)(rbase + k0 + kk) + (int64_t)(cbase + c) * m];
       }
+      __syncthreads();
+      if (r < no)
+        for (int kk = g; kk < nk; kk += 4) s = cfma(s, to_d2(tile[r * (KC2 + 1) + kk]), v[k0 + kk]);
     }
-    w[i] = v;
   }
+  part[g * 64 + r] = s;
   __syncthreads();
-  // U11^T w_p = w_p: w_k /= U[k][k]; w_i -= U[k][i] w_k for i > k
-  for (int k = 0; k < p; ++k) {
-    const double2 wk = cdiv(w[k], to_d2(Fb[(int64_t)k + (int64_t)k * m]));
-    for (int i = k + 1 + threadIdx.x; i < p; i += blockDim.x)
-      w[i] = cfms(w[i], to_d2(Fb[(int64_t)k + (int64_t)i * m]), wk);
-    __syncthreads();
-    if (threadIdx.x == 0) w[k] = wk;
-    __syncthreads();
+  if (t < no) {
+    double2 tot = part[t];
+    for (int gg = 1; gg < 4; ++gg) { tot.x += part[gg * 64 + t].x; tot.y += part[gg * 64 + t].y; }
+    if (out_cb) {
+      uf[o0 + t].x -= tot.x; uf[o0 + t].y -= tot.y;
+    } else {
+      double2* dst = xb + rows[o0 + t];
+      dst->x -= tot.x; dst->y -= tot.y;
+    }
   }
-  // update for the parent: u_j = w[p + j] - sum_k U[k][p + j] w_k
-  double2* uf = uarena + F.uoff[f] * B + (int64_t)b * (m - p);
-  for (int j = p + threadIdx.x; j < m; j += blockDim.x) {
-    double2 s = w[j];
-    const V* col = Fb + (int64_t)j * m;
-    for (int k = 0; k < p; ++k) s = cfms(s, to_d2(col[k]), w[k]);
-    uf[j - p] = s;
-  }
-  for (int i = threadIdx.x; i < p; i += blockDim.x) xb[rows[i]] = w[i];
-}
-
-template <typename T>
-__global__ void __launch_bounds__(256) k_bwdT(FrontsDev F, const int* __restrict__ lvl, int B, int n,
-                                              const typename cplx<T>::type* __restrict__ arena,
-                                              double2* __restrict__ x) {
-  using V = typename cplx<T>::type;
-  extern __shared__ unsigned char smem_raw[];
-  double2* z = reinterpret_cast<double2*>(smem_raw);
-  const int f = lvl[blockIdx.x];
-  const int b = blockIdx.y;
-  const int m = F.m[f], p = F.p[f];
-  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
-  const int* rows = F.rows + F.rows_ptr[f];
-  double2* xb = x + (int64_t)b * n;
-  for (int i = threadIdx.x; i < m; i += blockDim.x) z[i] = xb[rows[i]];
-  __syncthreads();
-  // z_k -= sum_j L[p + j][k] lambda_cb[j]   (column k of L21 is contiguous)
-  for (int k = threadIdx.x; k < p; k += blockDim.x) {
-    double2 s = z[k];
-    const V* col = Fb + (int64_t)k * m;
-    for (int j = p; j < m; ++j) s = cfms(s, to_d2(col[j]), z[j]);
-    z[k] = s;
-  }
-  __syncthreads();
-  // L11^T (unit upper) backward: lambda_k = z_k; z_i -= L[k][i] lambda_k for i < k
-  for (int k = p - 1; k >= 0; --k) {
-    const double2 lk = z[k];
-    for (int i = threadIdx.x; i < k; i += blockDim.x) z[i] = cfms(z[i], to_d2(Fb[(int64_t)k + (int64_t)i * m]), lk);
-    __syncthreads();
-  }
-  for (int i = threadIdx.x; i < p; i += blockDim.x) xb[rows[i]] = z[i];
 }
 
 // ---------------------------------------------------------------- per-frequency vectors (FP64)
@@ -551,6 +581,51 @@ __global__ void k_residual(int n, int B, const int64_t* __restrict__ indptr, con
   for (int q = 0; q < nb; ++q) {
     const double2 bb = rhs[(int64_t)(b0 + q) * n + i];
     r[(int64_t)(b0 + q) * n + i] = make_double2(bb.x - acc[q].x, bb.y - acc[q].y);
+  }
+}
+
+// Frequency-contiguous residual: xt, rhst, rt are [n][Bp] (row-major, Bp = frequencies padded to
+// a multiple of 64).  One wave per (row, 64 frequencies); matrix values are uniform across the wave,
+// the ω-dependent coefficient and the x entries are per lane (coalesced).
+__global__ void __launch_bounds__(256) k_residual_t(int n, int B, int Bp, const int64_t* __restrict__ indptr,
+                                                    const int* __restrict__ indices, const int64_t* __restrict__ tslot,
+                                                    const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
+                                                    const double2* __restrict__ Daa, const double* __restrict__ omega,
+                                                    double beta, const double2* __restrict__ xt,
+                                                    const double2* __restrict__ rhst, double2* __restrict__ rt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int b = blockIdx.y * 64 + lane;
+  const double w = b < B ? omega[b] : 0.0;
+  const double w2 = w * w, wb = w * beta;
+  double2 acc = make_double2(0, 0);
+  for (int64_t s0 = indptr[i]; s0 < indptr[i + 1]; ++s0) {
+    const int j = indices[s0];
+    const int64_t s = tslot ? tslot[s0] : s0;
+    const double2 k = Kaa[s], mm = Maa[s];
+    const double2 d = Daa ? Daa[s] : make_double2(0, 0);
+    const double2 a = make_double2(k.x - w2 * mm.x - wb * d.y, k.y - w2 * mm.y + wb * d.x);
+    acc = cfma(acc, a, xt[(int64_t)j * Bp + b]);
+  }
+  const double2 r = rhst[i * Bp + b];
+  rt[i * Bp + b] = make_double2(r.x - acc.x, r.y - acc.y);
+}
+
+// out[j][i] = in[i][j] for an R x C complex128 matrix (tiled through LDS).
+__global__ void __launch_bounds__(256) k_transpose(int R, int C, int ld_in, int ld_out, const double2* __restrict__ in,
+                                                   double2* __restrict__ out) {
+  __shared__ double2 tile[32][33];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + tx;
+    if (r < R && c < C) tile[k][tx] = in[(int64_t)r * ld_in + c];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + k, r = r0 + tx;
+    if (r < R && c < C) out[(int64_t)c * ld_out + r] = tile[tx][k];
   }
 }
 
@@ -606,6 +681,9 @@ struct Plan {
   int* ent_loc = nullptr;
   int* npert = nullptr;
   int mmax = 0;
+  int pmax = 0;
+  void* inv = nullptr;        // diagonal-block inverses (grown on demand)
+  size_t inv_bytes = 0;
 };
 
 // Record a start/stop event pair around one launch when profiling is on.
@@ -646,6 +724,16 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
   using V = typename cplx<T>::type;
   V* arena = (V*)arena_v;
   const T tiny = sizeof(T) == 4 ? (T)1e-30 : (T)1e-300;
+  // scratch for the diagonal-block inverses: one slot (2 nb^2) per front of a level and frequency
+  size_t need = 0;
+  for (const Level& L : P->levels) need = std::max(need, (size_t)L.nf * B * 2 * nb * nb * sizeof(V));
+  if (need > P->inv_bytes) {
+    if (P->inv) SAW_HIP(hipFree(P->inv));
+    P->inv = nullptr;
+    P->inv_bytes = 0;
+    SAW_HIP(hipMalloc(&P->inv, need));
+    P->inv_bytes = need;
+  }
   for (const Level& L : P->levels) {
     const int cpw = 8;
     { Timed tm(P, s, K_INIT);
@@ -655,29 +743,32 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
       hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), B), dim3(256), 0, s, P->F,
                          L.e0, L.e1, B, omega, beta, P->ent_front, P->ent_slot, P->ent_loc, (const double2*)Kaa,
                          (const double2*)Maa, (const double2*)Daa, arena); }
-    for (int k0 = 0; k0 < L.pmax; k0 += nb) {
-      const size_t lds_diag = sizeof(V) * nb * (nb + 1);
+    const int nbl = (nb == 64 && L.pmax <= 128) ? 32 : nb;   // narrow panels for small fronts
+    for (int k0 = 0; k0 < L.pmax; k0 += nbl) {
+      const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
       { Timed tm(P, s, K_DIAG);
-      hipLaunchKernelGGL(k_diag<T>, dim3(L.nf, B), dim3(64), lds_diag, s, P->F, L.fronts, B, arena, k0, nb, tiny,
-                         P->npert); }
+      hipLaunchKernelGGL(k_diag<T>, dim3(L.nf, B), dim3(128), lds_diag, s, P->F, L.fronts, B, arena, k0, nbl, tiny,
+                         P->npert, (V*)P->inv); }
       const int rem = L.mmax - k0 - 1;
       if (rem > 0) {
-        const int ntmax = (rem + 63) / 64;
-        const size_t lds_trsm = sizeof(V) * (nb + 64) * (nb + 1);
-        { Timed tm(P, s, K_TRSM);
-        hipLaunchKernelGGL(k_trsm<T>, dim3(2 * ntmax, L.nf, B), dim3(64), lds_trsm, s, P->F, L.fronts, B, arena, k0,
-                           nb, ntmax); }
         const int nt = (rem + TILE - 1) / TILE;
-        if (k0 + nb < L.pmax) {
+        { Timed tm(P, s, K_TRSM);
+        hipLaunchKernelGGL(k_gemm<T>, dim3(nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 2, k0, nbl,
+                           (const V*)P->inv);
+        hipLaunchKernelGGL(k_gemm<T>, dim3(nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 3, k0, nbl,
+                           (const V*)P->inv); }
+        if (k0 + nbl < L.pmax) {
           Timed tm(P, s, K_GEMM_L);
-          hipLaunchKernelGGL(k_gemm<T>, dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 0, k0, nb);
+          hipLaunchKernelGGL(k_gemm<T>, dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 0, k0, nbl,
+                             (const V*)nullptr);
         }
       }
     }
     if (L.cbmax > 0) {
       const int nt = (L.cbmax + TILE - 1) / TILE;
       Timed tm(P, s, K_GEMM_S);
-      hipLaunchKernelGGL(k_gemm<T>, dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 1, 0, nb);
+      hipLaunchKernelGGL(k_gemm<T>, dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 1, 0, nb,
+                         (const V*)nullptr);
     }
   }
   SAW_HIP(hipGetLastError());
@@ -686,32 +777,45 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
 
 template <typename T>
 int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose) {
-  const size_t lds = sizeof(double2) * P->mmax;
-  if (transpose) {
-    for (const Level& L : P->levels) {
-      Timed tm(P, s, K_FWD);
-      hipLaunchKernelGGL(k_fwdT<T>, dim3(L.nf, B), dim3(256), lds, s, P->F, L.fronts, B, P->n,
-                         (const typename cplx<T>::type*)arena, (double2*)uarena, (double2*)x);
+  using V = typename cplx<T>::type;
+  constexpr int LD = SolveTile<T>::SB + 1;
+  const size_t lds_tri = sizeof(V) * 64 * LD + sizeof(double2) * P->pmax;
+  const V* A = (const V*)arena;
+  double2* U = (double2*)uarena;
+  double2* X = (double2*)x;
+  auto tri = [&](const Level& L, int kind) {
+    Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
+    switch (kind) {
+      case 0: hipLaunchKernelGGL((k_tri<T, 0>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, A, U, X); break;
+      case 1: hipLaunchKernelGGL((k_tri<T, 1>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, A, U, X); break;
+      case 2: hipLaunchKernelGGL((k_tri<T, 2>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, A, U, X); break;
+      default: hipLaunchKernelGGL((k_tri<T, 3>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, A, U, X);
     }
-    for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
-      const Level& L = P->levels[l];
-      Timed tm(P, s, K_BWD);
-      hipLaunchKernelGGL(k_bwdT<T>, dim3(L.nf, B), dim3(256), lds, s, P->F, L.fronts, B, P->n,
-                         (const typename cplx<T>::type*)arena, (double2*)x);
+  };
+  auto panel = [&](const Level& L, int kind) {
+    const bool out_cb = (kind == 0 || kind == 2);
+    const int nout = out_cb ? L.cbmax : L.pmax;
+    const int nin = out_cb ? L.pmax : L.cbmax;
+    if (nout == 0 || nin == 0) return;
+    const size_t lds = sizeof(double2) * (nin + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
+    if (lds > 64 * 1024) return;   // rejected up front by sawfem_mf_solve
+    Timed tm(P, s, out_cb ? K_FWD : K_BWD);
+    dim3 grid((nout + 63) / 64, L.nf, B);
+    switch (kind) {
+      case 0: hipLaunchKernelGGL((k_panel<T, 0>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
+      case 1: hipLaunchKernelGGL((k_panel<T, 1>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
+      case 2: hipLaunchKernelGGL((k_panel<T, 2>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
+      default: hipLaunchKernelGGL((k_panel<T, 3>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X);
     }
-    SAW_HIP(hipGetLastError());
-    return 0;
-  }
+  };
+  const int kf = transpose ? 2 : 0, kb = transpose ? 3 : 1;
   for (const Level& L : P->levels) {
-    Timed tm(P, s, K_FWD);
-    hipLaunchKernelGGL(k_fwd<T>, dim3(L.nf, B), dim3(256), lds, s, P->F, L.fronts, B, P->n,
-                       (const typename cplx<T>::type*)arena, (double2*)uarena, (double2*)x);
+    tri(L, kf);
+    panel(L, kf);
   }
   for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
-    const Level& L = P->levels[l];
-    Timed tm(P, s, K_BWD);
-    hipLaunchKernelGGL(k_bwd<T>, dim3(L.nf, B), dim3(256), 2 * lds, s, P->F, L.fronts, B, P->n,
-                       (const typename cplx<T>::type*)arena, (double2*)x);
+    panel(P->levels[l], kb);
+    tri(P->levels[l], kb);
   }
   SAW_HIP(hipGetLastError());
   return 0;
@@ -749,6 +853,7 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
     off[f] = acc; acc += (int64_t)m[f] * m[f];
     uoff[f] = uacc; uacc += m[f] - p[f];
     P->mmax = std::max(P->mmax, m[f]);
+    P->pmax = std::max(P->pmax, p[f]);
   }
   P->arena_elems = acc;
   P->uarena_elems = uacc;
@@ -795,6 +900,7 @@ int sawfem_mf_destroy(void* plan) {
   Plan* P = (Plan*)plan;
   if (!P) return 0;
   for (void* d : P->owned) (void)hipFree(d);
+  if (P->inv) (void)hipFree(P->inv);
   for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
   delete P;
   return 0;
@@ -817,7 +923,7 @@ int sawfem_mf_perturbed(void* plan, int reset) {
 int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* omega, double beta, const void* Kaa,
                      const void* Maa, const void* Daa, void* arena, int nb) {
   Plan* P = (Plan*)plan;
-  if (!P || B <= 0 || !omega || !Kaa || !Maa || !arena || nb <= 0 || nb > 64 || (prec != 0 && prec != 1)) {
+  if (!P || B <= 0 || !omega || !Kaa || !Maa || !arena || (nb != 32 && nb != 64) || (prec != 0 && prec != 1)) {
     sawfem_set_error("sawfem_mf_factor: invalid arguments");
     return 1;
   }
@@ -835,7 +941,10 @@ int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena
     sawfem_set_error("sawfem_mf_solve: invalid arguments");
     return 1;
   }
-  if ((size_t)P->mmax * sizeof(double2) * 2 > 64 * 1024) {
+  int nin_max = 0;
+  for (const Level& L : P->levels) nin_max = std::max(nin_max, std::max(L.pmax, L.cbmax));
+  if ((size_t)P->pmax * sizeof(double2) + 64 * 33 * 16 > 64 * 1024 ||
+      (size_t)nin_max * sizeof(double2) + 256 * 16 + 64 * 33 * 16 > 64 * 1024) {
     sawfem_set_error("sawfem_mf_solve: front too large for the LDS solve");
     return 1;
   }
@@ -888,6 +997,28 @@ int sawfem_residual(void* stream, int n, int B, const int64_t* indptr, const int
   hipLaunchKernelGGL(k_residual, dim3((n + 255) / 256, (B + 7) / 8), dim3(256), 0, (hipStream_t)stream, n, B, indptr,
                      indices, tslot, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, omega, beta,
                      (const double2*)x, (const double2*)rhs, (double2*)r);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+// Residual in the frequency-contiguous layout: xt/rhst/rt are n x Bp (Bp >= B, multiple of 64).
+int sawfem_residual_t(void* stream, int n, int B, int Bp, const int64_t* indptr, const int* indices,
+                      const int64_t* tslot, const void* Kaa, const void* Maa, const void* Daa, const double* omega,
+                      double beta, const void* xt, const void* rhst, void* rt) {
+  if (n <= 0 || B <= 0 || Bp < B || Bp % 64) { sawfem_set_error("sawfem_residual_t: invalid arguments"); return 1; }
+  hipLaunchKernelGGL(k_residual_t, dim3((n + 3) / 4, Bp / 64), dim3(256), 0, (hipStream_t)stream, n, B, Bp, indptr,
+                     indices, tslot, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, omega, beta,
+                     (const double2*)xt, (const double2*)rhst, (double2*)rt);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+// out (C x ld_out) = transpose of in (R x ld_in), complex128.
+int sawfem_transpose(void* stream, int R, int C, int ld_in, int ld_out, const void* in, void* out) {
+  if (R < 0 || C < 0 || ld_in < C || ld_out < R) { sawfem_set_error("sawfem_transpose: invalid arguments"); return 1; }
+  if (R == 0 || C == 0) return 0;
+  hipLaunchKernelGGL(k_transpose, dim3((C + 31) / 32, (R + 31) / 32), dim3(256), 0, (hipStream_t)stream, R, C, ld_in,
+                     ld_out, (const double2*)in, (double2*)out);
   SAW_HIP(hipGetLastError());
   return 0;
 }

@@ -225,7 +225,7 @@ class FrequencySolver:
     def _dptr(self, t):
         return None if t is None else t.data_ptr()
 
-    def solve_batch(self, omegas, V=1.0, precision='mixed', ir_tol=1e-10, max_ir=8, nb=32, fallback=True,
+    def solve_batch(self, omegas, V=1.0, precision='mixed', ir_tol=1e-9, max_ir=8, nb=None, fallback=True,
                     gradient=False):
         """Solve all ``omegas`` in one batched factorization.
 
@@ -237,6 +237,7 @@ class FrequencySolver:
         omegas = np.asarray(omegas, dtype=np.float64)
         B, n = omegas.size, self.n_active
         prec = 0 if precision == 'mixed' else 1
+        nb = nb or (64 if prec == 0 else 32)
         s = stream_ptr()
         om = torch.as_tensor(omegas, device=dev)
         arena = torch.empty(self.mf.arena_elems * B, dtype=torch.complex64 if prec == 0 else torch.complex128,
@@ -255,20 +256,22 @@ class FrequencySolver:
                           Qd.cpu().numpy().copy())[1]
         Q = charge()
         r = torch.empty_like(x)
+        res = self._residual_op(B, om)
+        rhst = res.to_t(rhs)
         steps = 0
         dq = np.full(B, np.inf)
+        hist = []
         for steps in range(1, max_ir + 1):
-            _lib.call('sawfem_residual', s, n, B, self.aa_indptr.data_ptr(), self.aa_indices.data_ptr(), None,
-                      self.Kaa.data_ptr(), self.Maa.data_ptr(), self._dptr(self.Daa), om.data_ptr(), self.beta,
-                      x.data_ptr(), rhs.data_ptr(), r.data_ptr())
+            res(x, rhst, r, None)
             _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 0)
             _lib.call('sawfem_axpy', s, B * n, x.data_ptr(), r.data_ptr())
             Qn = charge()
             dq = np.abs(Qn - Q) / np.maximum(np.abs(Qn), 1e-300)
+            hist.append(float(dq.max()))
             Q = Qn
-            if np.all(dq <= ir_tol) or (prec == 1 and steps >= 1 and np.all(dq <= 1e-13)):
+            if np.all(dq <= ir_tol):
                 break
-        out = dict(x=x, Q=Q, ir_steps=steps, dq=dq)
+        out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, dq_hist=hist)
         if gradient:
             out.update(self._adjoint(om, omegas, V, prec, arena, uarena, rhs, x, ir_tol, max_ir))
         refactored = np.zeros(B, dtype=bool)
@@ -276,7 +279,7 @@ class FrequencySolver:
             bad = np.flatnonzero(~(dq <= ir_tol) | (~(out.get('adj_dq', dq) <= ir_tol) if gradient else False))
             if bad.size:
                 del arena, uarena
-                sub = self.solve_batch(omegas[bad], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir, nb=nb,
+                sub = self.solve_batch(omegas[bad], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir,
                                        gradient=gradient)
                 tb = self.torch.as_tensor(bad, device=self.dev)
                 x[tb] = sub['x']
@@ -286,6 +289,30 @@ class FrequencySolver:
                 refactored[bad] = True
         out['refactored'] = refactored
         return out
+
+    def _residual_op(self, B, om):
+        """r = rhs - A(w) x (or A^T with tslot) through the frequency-contiguous residual kernel."""
+        torch = self.torch
+        n = self.n_active
+        Bp = -(-B // 64) * 64
+        xt = torch.empty((n, Bp), dtype=torch.complex128, device=self.dev)
+        rt = torch.empty((n, Bp), dtype=torch.complex128, device=self.dev)
+        solver = self
+
+        class Op:
+            def to_t(self, v):
+                out = torch.empty((n, Bp), dtype=torch.complex128, device=solver.dev)
+                _lib.call('sawfem_transpose', stream_ptr(), B, n, n, Bp, v.data_ptr(), out.data_ptr())
+                return out
+
+            def __call__(self, x, rhst, r, tslot):
+                s = stream_ptr()
+                _lib.call('sawfem_transpose', s, B, n, n, Bp, x.data_ptr(), xt.data_ptr())
+                _lib.call('sawfem_residual_t', s, n, B, Bp, solver.aa_indptr.data_ptr(), solver.aa_indices.data_ptr(),
+                          solver._dptr(tslot), solver.Kaa.data_ptr(), solver.Maa.data_ptr(), solver._dptr(solver.Daa),
+                          om.data_ptr(), solver.beta, xt.data_ptr(), rhst.data_ptr(), rt.data_ptr())
+                _lib.call('sawfem_transpose', s, n, B, Bp, n, rt.data_ptr(), r.data_ptr())
+        return Op()
 
     def _adjoint(self, om, omegas, V, prec, arena, uarena, rhs, x, ir_tol, max_ir):
         """lambda = A^-T kts_a (refined), then K12 and the parameter contraction."""
@@ -301,10 +328,10 @@ class FrequencySolver:
         m0 = mu()
         r = torch.empty_like(lam)
         dq = np.full(B, np.inf)
+        res = self._residual_op(B, om)
+        gt = res.to_t(gvec)
         for _ in range(max_ir):
-            _lib.call('sawfem_residual', s, n, B, self.aa_indptr.data_ptr(), self.aa_indices.data_ptr(),
-                      self.tslot.data_ptr(), self.Kaa.data_ptr(), self.Maa.data_ptr(), self._dptr(self.Daa),
-                      om.data_ptr(), self.beta, lam.data_ptr(), gvec.data_ptr(), r.data_ptr())
+            res(lam, gt, r, self.tslot)
             _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 1)
             _lib.call('sawfem_axpy', s, B * n, lam.data_ptr(), r.data_ptr())
             m1 = mu()
@@ -343,6 +370,7 @@ class FrequencySolver:
             if gradient:
                 dY[i0:i0 + batch] = res['dY']
                 item['adj_dq_max'] = float(np.max(res['adj_dq']))
+            item['dq_hist'] = res.get('dq_hist')
             info.append(item)
             del res
         if gradient:

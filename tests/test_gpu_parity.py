@@ -91,13 +91,13 @@ def test_bloch_maps_and_active_matrix(name):
     assert abs(Qg - Qo) <= 1e-12 * abs(Qo) + 1e-30
 
 
-@pytest.mark.parametrize('prec', ['fp64', 'mixed'])
-def test_factor_arena_matches_emulation(prec):
-    _say(f'[test_factor_arena_matches_emulation] {prec}')
+@pytest.mark.parametrize('prec,nb', [('fp64', 32), ('mixed', 32), ('mixed', 64)])
+def test_factor_arena_matches_emulation(prec, nb):
+    _say(f'[test_factor_arena_matches_emulation] {prec} nb={nb}')
     import torch
     cell = workloads.double_layer()
     wtop = 2 * np.pi * cell.band_hz[1]
-    eng = _engine(cell, wtop, leaf_weight=60)
+    eng = _engine(cell, wtop, leaf_weight=160)   # larger leaves: several panels per front
     omegas = 2 * np.pi * np.array([1.7e9, 1.85e9, 2.2e9])
     B = omegas.size
     from sawsim_amd import _lib
@@ -106,11 +106,13 @@ def test_factor_arena_matches_emulation(prec):
     arena = torch.empty(eng.mf.arena_elems * B, dtype=torch.complex64 if pr == 0 else torch.complex128, device='cuda')
     om = torch.as_tensor(omegas, device='cuda')
     _lib.call('sawfem_mf_factor', eng._plan, stream_ptr(), pr, B, om.data_ptr(), 0.0, eng.Kaa.data_ptr(),
-              eng.Maa.data_ptr(), None, arena.data_ptr(), 16)
+              eng.Maa.data_ptr(), None, arena.data_ptr(), nb)
     got = arena.cpu().numpy().astype(np.complex128)
-    ref = mf_emulator.factor(eng.mf, B, omegas, 0.0, eng.Kaa.cpu().numpy(), eng.Maa.cpu().numpy(), None, nb=16)
+    ref = mf_emulator.factor(eng.mf, B, omegas, 0.0, eng.Kaa.cpu().numpy(), eng.Maa.cpu().numpy(), None, nb=nb)
     off = eng.mf.off
-    tol = 1e-8 if pr == 1 else 2e-3   # intermediate fronts: summation-order roundoff only
+    # intermediate fronts without pivoting: roundoff differences (operation order) are amplified by
+    # pivot growth in a few fronts; indexing mistakes give O(1) errors
+    tol = 1e-6 if pr == 1 else 5e-3
     for f in range(eng.mf.nf):
         m = int(eng.mf.m[f])
         a = got[off[f] * B: off[f] * B + B * m * m]
