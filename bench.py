@@ -10,7 +10,7 @@ Printed JSON line: metric/value (whole-job frequency points per second), the pla
 GPU path (one frequency per launch) measured on a sample of points, the Y11 error of the
 mixed path against the FP64 path, a roofline entry for the dominant kernel (the MFMA
 update GEMM, timed live with HIP events) and a CPU baseline (the SuperLU oracle on a
-bounded sample of the same workload).
+bounded sample of the same workload: one frequency per worker process).
 """
 from __future__ import annotations
 
@@ -49,9 +49,10 @@ def profile_read(eng):
     return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(names)}
 
 
-def cpu_baseline(cell, seconds_budget=20.0):
-    """Oracle SuperLU sweep (real-block system per frequency, as the reference's fallback
-    backend) on a bounded sample of the workload's frequencies, one process per point."""
+def cpu_baseline(cell, workers=8):
+    """The oracle (SciPy SuperLU, one complex sparse LU + solve per frequency, the reference's
+    per-frequency direct solve) on a bounded sample of the workload: one frequency per worker
+    of a fork pool, ``workers`` frequencies spread over the band."""
     sys.path.insert(0, str(ROOT / 'tests'))
     import helpers
     from oracle import sweep as osweep
@@ -59,20 +60,17 @@ def cpu_baseline(cell, seconds_budget=20.0):
     eng = helpers.oracle_engine(cell, 2 * np.pi * cell.band_hz[1])
     setup = time.time() - t0
     f = cell.frequencies()
-    cores = min(8, os.cpu_count() or 1)
-    # one probe solve decides the sample size
-    t0 = time.time()
-    eng.solve(2 * np.pi * f[len(f) // 2])
-    one = time.time() - t0
-    k = int(max(1, min(cores, seconds_budget // max(one, 1e-3))))
+    k = max(1, min(workers, os.cpu_count() or 1, f.size))
     sample = f[np.linspace(0, len(f) - 1, k).astype(int)]
+    from threadpoolctl import threadpool_limits
     t0 = time.time()
-    osweep.sweep_charges(eng, sample, workers=k)
+    with threadpool_limits(limits=1):          # one BLAS thread per worker: `cores` = workers
+        osweep.sweep_charges(eng, sample, workers=k, complex_form=True)
     dt = time.time() - t0
     return dict(value=k / dt, unit='freq pts/s (Y11 only)', cores=k, kind='port',
-                sample=f'{k} of {len(f)} frequencies, SciPy SuperLU on the real-block system '
-                       f'(n={eng.n_active} complex), fork pool of {k}, setup {setup:.1f}s excluded; '
-                       f'host os.cpu_count()={os.cpu_count()}')
+                sample=f'{k} of {len(f)} frequencies, one per process of a fork pool of {k} '
+                       f'(single-threaded SciPy SuperLU on the complex system, n={eng.n_active}); '
+                       f'{dt:.1f}s wall, host setup {setup:.1f}s excluded; os.cpu_count()={os.cpu_count()}')
 
 
 def main():
@@ -85,7 +83,7 @@ def main():
     ap.add_argument('--plain-points', type=int, default=8)
     ap.add_argument('--check-points', type=int, default=16)
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--cpu-budget', type=float, default=20.0)
+    ap.add_argument('--cpu-workers', type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps(dict(error='this build runs on exactly one GPU; multi-GPU sharding of the '
@@ -103,7 +101,8 @@ def main():
     torch.cuda.synchronize()
     log(f'[bench] {args.workload}: n={eng.n_active} fronts={eng.mf.nf} setup {time.time() - t0:.1f}s '
         f'(analysis {eng.analysis_seconds:.1f}s)')
-    work = kernel_work(eng.mf, 32)
+    nb = 64                                   # panel width of the complex64 factorization
+    work = kernel_work(eng.mf, nb)
     per_freq = eng.arena_bytes(1, 'mixed')
     batch = args.batch or max(1, min(P, int(120e9 // per_freq)))
     nchunk = -(-P // batch)
@@ -165,14 +164,20 @@ def main():
     eng.update_operator(K, M, D, beta=p0[15])
     log(f'[bench] plain fp64: Y11 {plain:.2f} pts/s, +adjoint {plain_adj:.2f}, +central FD {plain_fd:.3f}; '
         f'mixed vs fp64: Y11 {y_err:.2e}, dY/dp {g_err:.2e}')
-    gemm_ms = prof['gemm_l'][0] + prof['gemm_s'][0]
-    gemm_n = prof['gemm_l'][1] + prof['gemm_s'][1]
-    gemm_flops = work['gemm_flops'] * P * args.steps
-    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else None
+    # dominant MFMA kernel: the Schur-complement GEMM k_gemm<float, 1> (one launch per level)
+    schur_ms, schur_n = prof['gemm_s']
+    schur_flops = work['schur_flops'] * P * args.steps
+    achieved = schur_flops / (schur_ms * 1e-3) / 1e12 if schur_ms > 0 else None
+    lupd_tf = work['lupd_flops'] * P * args.steps / (prof['gemm_l'][0] * 1e-3) / 1e12 if prof['gemm_l'][0] else None
+    # solves: each forward+backward pass streams the stored L/U entries (complex64) once
+    nsolve_pts = sum(it['points'] * (2 + it['ir_steps'] + it.get('adj_steps', 0)) for it in info) * args.steps
+    solve_bytes = nsolve_pts * work['lu_entries'] * 8
+    solve_ms = prof['fwd'][0] + prof['bwd'][0]
+    solve_gbs = solve_bytes / (solve_ms * 1e-3) / 1e9 if solve_ms else None
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(cell, args.cpu_budget)
+            cpu = cpu_baseline(cell, args.cpu_workers)
         except Exception as exc:  # the baseline is reported, never required
             log(f'[bench] cpu baseline failed: {exc!r}')
     out = dict(
@@ -193,8 +198,14 @@ def main():
         ir=info,
         roofline=dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
                       frac=(achieved / PEAK_FP32_MFMA) if achieved else None, traffic=None,
-                      kernel='k_gemm<float> (L-shaped panel updates + Schur complement)',
-                      launches=gemm_n, kernel_ms=gemm_ms, algorithmic_flops=gemm_flops),
+                      kernel='k_gemm<float, 1> (Schur complement, complex64 on FP32 MFMA)',
+                      launches=schur_n, kernel_ms=schur_ms, algorithmic_flops=schur_flops,
+                      avg_launch_ms=schur_ms / max(schur_n, 1)),
+        other_kernels=dict(
+            lupdate_gemm_tflops=lupd_tf,
+            solve=dict(bound='hbm', achieved_gbs=solve_gbs, peak_gbs=8000.0,
+                       frac=(solve_gbs / 8000.0) if solve_gbs else None, bytes=solve_bytes, ms=solve_ms,
+                       note='forward+backward passes; algorithmic bytes = stored L/U entries x 8 B per point')),
         kernel_ms={k: v[0] for k, v in prof.items()},
         factor_flops_per_point=work['factor_flops'],
         cpu_baseline=cpu,

@@ -69,9 +69,11 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
 
 /* Forward/backward substitution with the factors of sawfem_mf_factor; x: B x n complex128,
  * right-hand sides in, solutions out.  uarena: uarena_elems * B complex128 scratch.
- * transpose != 0 solves A^T x = b with the same factors (U^T then L^T; adjoint solve). */
+ * transpose != 0 solves A^T x = b with the same factors (U^T then L^T; adjoint solve).
+ * nb: the panel width the arena was factored with (the diagonal blocks store the inverses of
+ * their triangular factors, so the solve has to walk the same blocks). */
 int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x,
-                    int transpose);
+                    int transpose, int nb);
 
 /* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;
  * used by the benchmark).  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S,

@@ -164,20 +164,24 @@ _POOL_ENGINE = None
 
 
 def _pool_charge(args):
-    f, V = args
-    return complex(_POOL_ENGINE.solve(2 * np.pi * f, V)[1])
+    f, V, cplx = args
+    solve = _POOL_ENGINE.solve_complex if cplx else _POOL_ENGINE.solve
+    return complex(solve(2 * np.pi * f, V)[1])
 
 
-def sweep_charges(engine, freqs, V=1.0, workers=None):
-    """Charges in frequency order over a fork pool (frequency_tasks.py:12-55)."""
+def sweep_charges(engine, freqs, V=1.0, workers=None, complex_form=False):
+    """Charges in frequency order over a fork pool (frequency_tasks.py:12-55).
+
+    ``complex_form`` solves the complex system directly instead of the real-block form."""
     global _POOL_ENGINE
     freqs = np.asarray(freqs, dtype=float)
     if workers is None:
         workers = 8 if freqs.size > 41 else 1
     workers = max(1, min(int(workers), freqs.size, os.cpu_count() or 1))
     if workers == 1:
-        return np.array([engine.solve(2 * np.pi * f, V)[1] for f in freqs], dtype=complex)
+        solve = engine.solve_complex if complex_form else engine.solve
+        return np.array([solve(2 * np.pi * f, V)[1] for f in freqs], dtype=complex)
     _POOL_ENGINE = engine
     with mp.get_context('fork').Pool(workers) as pool:
-        q = pool.map(_pool_charge, [(f, V) for f in freqs], chunksize=1)
+        q = pool.map(_pool_charge, [(f, V, complex_form) for f in freqs], chunksize=1)
     return np.asarray(q, dtype=complex)

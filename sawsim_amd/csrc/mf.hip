@@ -116,13 +116,15 @@ __global__ void k_front_entries(FrontsDev F, int64_t e0, int64_t e1, int B, cons
 // ---------------------------------------------------------------- panel factorization
 // LU of the kb x kb diagonal block of pivot panel [k0, k0 + kb) (no pivoting; a pivot below
 // `tiny` is replaced by `tiny` and counted), then the inverses of its unit-lower and upper
-// factors written to the scratch slot of this (front, frequency): Linv then Uinv, each
-// column-major with leading dimension nb.  Threads 0..nb-1 build one column of Linv each,
-// threads 64..64+nb-1 one column of Uinv, with uniform loops (broadcast LDS reads).
+// factors stored IN PLACE of the factors: the strictly lower triangle of the block holds
+// L_D^{-1} (unit diagonal implied) and the upper triangle U_D^{-1}.  Neither the panel solves
+// (GEMMs with these inverses) nor the triangular solves need L_D, U_D themselves.
+// Threads 0..kb-1 build one column of L_D^{-1} each, threads 64..64+kb-1 one column of
+// U_D^{-1}, with uniform loops (broadcast LDS reads of the factors).
 template <typename T>
 __global__ void __launch_bounds__(128) k_diag(FrontsDev F, const int* __restrict__ lvl, int B,
                                               typename cplx<T>::type* __restrict__ arena, int k0, int nb,
-                                              T tiny, int* __restrict__ npert, typename cplx<T>::type* __restrict__ inv) {
+                                              T tiny, int* __restrict__ npert) {
   using V = typename cplx<T>::type;
   extern __shared__ unsigned char smem_raw[];
   V* Ds = reinterpret_cast<V*>(smem_raw);   // [nb][nb+1], column-major with padding
@@ -152,26 +154,24 @@ __global__ void __launch_bounds__(128) k_diag(FrontsDev F, const int* __restrict
     }
     __syncthreads();
   }
-  for (int j = 0; j < kb; ++j)
-    if (t < kb) Fb[(int64_t)(k0 + t) + (int64_t)(k0 + j) * m] = Ds[t + j * ld];
-  V* slot = inv + ((int64_t)blockIdx.x * B + b) * 2 * nb * nb;
+  // column j of L^{-1}: x_i = delta_ij - sum_{j<=k<i} L[i][k] x_k, stored below the diagonal
+  // column j of U^{-1}: x_i = (delta_ij - sum_{i<k<=j} U[i][k] x_k) / U[i][i], stored on/above it
+  V* D = Fb + (int64_t)k0 + (int64_t)k0 * m;
   if (t < kb) {
-    // column j of L^{-1}: x_i = delta_ij - sum_{k<i} L[i][k] x_k
     const int j = t;
-    V* x = slot + (int64_t)j * nb;
-    for (int i = 0; i < kb; ++i) {
-      V s = (i == j) ? V{1, 0} : czero<V>();
-      for (int k = 0; k < i; ++k) s = cfms(s, Ds[i + k * ld], (k >= j) ? x[k] : czero<V>());
-      x[i] = (i >= j) ? s : czero<V>();
+    V* x = D + (int64_t)j * m;
+    for (int i = j + 1; i < kb; ++i) {
+      V s = cmul(Ds[i + j * ld], V{-1, 0});
+      for (int k = j + 1; k < i; ++k) s = cfms(s, Ds[i + k * ld], x[k]);
+      x[i] = s;
     }
   } else if (t >= 64 && t < 64 + kb) {
-    // column j of U^{-1}: x_i = (delta_ij - sum_{k>i} U[i][k] x_k) / U[i][i]
     const int j = t - 64;
-    V* x = slot + (int64_t)nb * nb + (int64_t)j * nb;
-    for (int i = kb - 1; i >= 0; --i) {
+    V* x = D + (int64_t)j * m;
+    for (int i = j; i >= 0; --i) {
       V s = (i == j) ? V{1, 0} : czero<V>();
-      for (int k = i + 1; k < kb; ++k) s = cfms(s, Ds[i + k * ld], (k <= j) ? x[k] : czero<V>());
-      x[i] = (i <= j) ? cdiv(s, Ds[i + i * ld]) : czero<V>();
+      for (int k = i + 1; k <= j; ++k) s = cfms(s, Ds[i + k * ld], x[k]);
+      x[i] = cdiv(s, Ds[i + i * ld]);
     }
   }
 }
@@ -249,11 +249,10 @@ template <> struct WaveAcc<double> {
 // mode 1: Schur complement rows/cols [p, m), K = [0, p)
 // mode 2: L panel  F[k1:m, k0:k1] = F[k1:m, k0:k1] * Uinv      (in place, overwrite)
 // mode 3: U panel  F[k0:k1, k1:m] = Linv * F[k0:k1, k1:m]      (in place, overwrite)
-// The inverses come from k_diag's scratch slot of this (front, frequency).
-template <typename T>
+// The inverses are the ones k_diag stored in the diagonal block.
+template <typename T, int mode>
 __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
-                                              typename cplx<T>::type* __restrict__ arena, int mode, int k0, int nb,
-                                              const typename cplx<T>::type* __restrict__ inv) {
+                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb) {
   using V = typename cplx<T>::type;
   __shared__ V As[KC * TILE];
   __shared__ V Bs[TILE * (KC + 1)];
@@ -281,7 +280,7 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
   const int r0 = r_lo + ti * TILE, c0 = c_lo + tj * TILE;
   if (mode == 0 && r0 >= p && c0 >= p) return;
   V* Fb = front_ptr<T>(arena, F, f, b, B);
-  const V* slot = (mode >= 2) ? inv + ((int64_t)blockIdx.y * B + b) * 2 * nb * nb : nullptr;
+  const V* Dinv = Fb + (int64_t)k0 + (int64_t)k0 * m;      // diagonal block holding the inverses
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63;
   const int wrow = wave & 1, wcol = wave >> 1;
@@ -297,7 +296,10 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
         if (mode == 0) v = Fb[(int64_t)gr + (int64_t)(k0 + gk) * m];
         else if (mode == 1) v = Fb[(int64_t)gr + (int64_t)gk * m];
         else if (mode == 2) v = Fb[(int64_t)gr + (int64_t)(k0 + gk) * m];
-        else v = slot[(int64_t)(gr - k0) + (int64_t)gk * nb];                 // Linv[row][k]
+        else {                                                                    // Linv[row][k]
+          const int rr = gr - k0;
+          v = rr > gk ? Dinv[(int64_t)rr + (int64_t)gk * m] : (rr == gk ? V{1, 0} : czero<V>());
+        }
       }
       As[k * TILE + row] = v;
     }
@@ -309,7 +311,10 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
       if (gc < c_hi && gk < K) {
         if (mode == 0) v = Fb[(int64_t)(k0 + gk) + (int64_t)gc * m];
         else if (mode == 1) v = Fb[(int64_t)gk + (int64_t)gc * m];
-        else if (mode == 2) v = slot[(int64_t)nb * nb + gk + (int64_t)(gc - k0) * nb];   // Uinv[k][col]
+        else if (mode == 2) {                                                     // Uinv[k][col]
+          const int cc = gc - k0;
+          v = gk <= cc ? Dinv[(int64_t)gk + (int64_t)cc * m] : czero<V>();
+        }
         else v = Fb[(int64_t)(k0 + gk) + (int64_t)gc * m];
       }
       Bs[col * (KC + 1) + k] = v;
@@ -337,21 +342,19 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
 
 // ---------------------------------------------------------------- triangular solves
 // Per level, each direction runs two kernels:
-//   * a pivot-block kernel, one workgroup per (front, frequency): the p x p triangle solved in
-//     SB-wide diagonal blocks staged in LDS (wave 0 solves a block in registers, lane i owning
-//     row i, values broadcast by lane shuffles), the remaining pivot rows updated by all waves;
+//   * a pivot-block kernel, one workgroup per (front, frequency): the p x p triangle walked in
+//     the factorization's nbl-wide diagonal blocks, staged in LDS.  A diagonal block holds the
+//     inverses of its triangular factors, so its solve is a mat-vec (all 256 threads, four
+//     partial sums per row); the off-diagonal part of the triangle then updates the remaining
+//     pivot rows;
 //   * a panel kernel over 64-wide tiles of the off-diagonal panel (L21, U12 or their
 //     transposes), many workgroups per front, coalesced loads (transposed panels go through
 //     an LDS tile).
 // Vectors: x [B][n] complex128 (pivot rows of each front are owned by that front); the update
 // vector of front f (its contribution to the parent's rows) lives in uarena.
 template <typename T> struct SolveTile;
-template <> struct SolveTile<float> { static constexpr int SB = 64; };
-template <> struct SolveTile<double> { static constexpr int SB = 32; };
-
-__device__ __forceinline__ double2 shfl2(double2 v, int src) {
-  return make_double2(__shfl(v.x, src, 64), __shfl(v.y, src, 64));
-}
+template <> struct SolveTile<float> { static constexpr int SB = 64; };    // widest panel, complex64
+template <> struct SolveTile<double> { static constexpr int SB = 32; };   // widest panel, complex128
 
 // w[i] (i < p) = x[rows[i]] + children's update entries; uf[i - p] (i >= p) = children's entries.
 __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, int B, int m, int p, const int* rows,
@@ -374,24 +377,27 @@ __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, i
 
 // Pivot triangle.  kind 0: L11 (unit lower, forward)   1: U11 (upper, backward)
 //                  2: U11^T (lower, forward)            3: L11^T (unit upper, backward)
-// Forward kinds gather the right-hand side and the children's updates first.
+// Forward kinds gather the right-hand side and the children's updates first.  nbl is the
+// panel width the front was factored with (the diagonal blocks hold L_D^{-1} below and
+// U_D^{-1} on/above the diagonal).
 template <typename T, int KIND>
-__global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+__global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
                                              const typename cplx<T>::type* __restrict__ arena,
                                              double2* __restrict__ uarena, double2* __restrict__ x) {
   using V = typename cplx<T>::type;
   constexpr int SB = SolveTile<T>::SB;
   constexpr int LD = SB + 1;
   extern __shared__ unsigned char smem_raw[];
-  V* tile = reinterpret_cast<V*>(smem_raw);                           // [SB][LD] or [64][LD]
-  double2* w = reinterpret_cast<double2*>(smem_raw + sizeof(V) * 64 * LD);
+  V* tile = reinterpret_cast<V*>(smem_raw);                                   // [64][LD]
+  double2* part = reinterpret_cast<double2*>(smem_raw + sizeof(V) * 64 * LD);  // [4][64]
+  double2* w = part + 256;                                                    // [p]
   const int f = lvl[blockIdx.x];
   const int b = blockIdx.y;
   const int m = F.m[f], p = F.p[f];
   const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
   double2* xb = x + (int64_t)b * n;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int t = threadIdx.x;
   constexpr bool fwd = (KIND == 0 || KIND == 2);
   if (fwd) {
     solve_gather(F, f, b, B, m, p, rows, xb, uarena, w, uarena + F.uoff[f] * B + (int64_t)b * (m - p));
@@ -399,42 +405,38 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
     for (int i = t; i < p; i += blockDim.x) w[i] = xb[rows[i]];
   }
   __syncthreads();
-  const int nblk = (p + SB - 1) / SB;
+  const int nblk = (p + nbl - 1) / nbl;
   for (int bi = 0; bi < nblk; ++bi) {
-    const int J0 = fwd ? bi * SB : (nblk - 1 - bi) * SB;
-    const int nbk = min(SB, p - J0);
+    const int J0 = fwd ? bi * nbl : (nblk - 1 - bi) * nbl;
+    const int nbk = min(nbl, p - J0);
     // diagonal block to LDS: tile[r + c * LD] = F[J0 + r][J0 + c]
-    for (int idx = t; idx < SB * SB; idx += blockDim.x) {
-      const int r = idx % SB, c = idx / SB;
+    for (int idx = t; idx < nbl * nbl; idx += blockDim.x) {
+      const int r = idx % nbl, c = idx / nbl;
       if (r < nbk && c < nbk) tile[r + c * LD] = Fb[(int64_t)(J0 + r) + (int64_t)(J0 + c) * m];
     }
     __syncthreads();
-    if (wave == 0) {
-      double2 v = lane < nbk ? w[J0 + lane] : make_double2(0, 0);
-      if (KIND == 0) {
-        for (int k = 0; k < nbk; ++k) {
-          const double2 vk = shfl2(v, k);
-          if (lane > k && lane < nbk) v = cfms(v, to_d2(tile[lane + k * LD]), vk);
-        }
-      } else if (KIND == 1) {
-        for (int k = nbk - 1; k >= 0; --k) {
-          const double2 xk = cdiv(shfl2(v, k), to_d2(tile[k + k * LD]));
-          if (lane == k) v = xk;
-          else if (lane < k) v = cfms(v, to_d2(tile[lane + k * LD]), xk);
-        }
-      } else if (KIND == 2) {
-        for (int k = 0; k < nbk; ++k) {
-          const double2 wk = cdiv(shfl2(v, k), to_d2(tile[k + k * LD]));
-          if (lane == k) v = wk;
-          else if (lane > k && lane < nbk) v = cfms(v, to_d2(tile[k + lane * LD]), wk);
-        }
-      } else {
-        for (int k = nbk - 1; k >= 0; --k) {
-          const double2 lk = shfl2(v, k);
-          if (lane < k) v = cfms(v, to_d2(tile[k + lane * LD]), lk);
+    {
+      const int i = t & 63, g = t >> 6;   // output row, quarter of the reduction
+      double2 s = make_double2(0, 0);
+      if (i < nbk) {
+        if (KIND == 0) {          // L_D^{-1} (unit lower): w_i + sum_{k<i} Li[i][k] w_k
+          for (int k = g; k < i; k += 4) s = cfma(s, to_d2(tile[i + k * LD]), w[J0 + k]);
+        } else if (KIND == 1) {   // U_D^{-1}: sum_{k>=i} Ui[i][k] w_k
+          for (int k = i + g; k < nbk; k += 4) s = cfma(s, to_d2(tile[i + k * LD]), w[J0 + k]);
+        } else if (KIND == 2) {   // U_D^{-T}: sum_{k<=i} Ui[k][i] w_k
+          for (int k = g; k <= i; k += 4) s = cfma(s, to_d2(tile[k + i * LD]), w[J0 + k]);
+        } else {                  // L_D^{-T} (unit upper): w_i + sum_{k>i} Li[k][i] w_k
+          for (int k = i + 1 + g; k < nbk; k += 4) s = cfma(s, to_d2(tile[k + i * LD]), w[J0 + k]);
         }
       }
-      if (lane < nbk) w[J0 + lane] = v;
+      part[g * 64 + i] = s;
+    }
+    __syncthreads();
+    if (t < nbk) {
+      double2 tot = part[t];
+      for (int g = 1; g < 4; ++g) { tot.x += part[g * 64 + t].x; tot.y += part[g * 64 + t].y; }
+      if (KIND == 0 || KIND == 3) { tot.x += w[J0 + t].x; tot.y += w[J0 + t].y; }
+      w[J0 + t] = tot;
     }
     __syncthreads();
     // remaining pivot rows: forward kinds update [J0 + nbk, p), backward kinds [0, J0)
@@ -451,8 +453,8 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
       for (int c0 = lo; c0 < hi; c0 += 64) {
         const int nc = min(64, hi - c0);
         __syncthreads();
-        for (int idx = t; idx < SB * 64; idx += blockDim.x) {
-          const int k = idx % SB, c = idx / SB;
+        for (int idx = t; idx < nbl * 64; idx += blockDim.x) {
+          const int k = idx % nbl, c = idx / nbl;
           if (k < nbk && c < nc) tile[c * LD + k] = Fb[(int64_t)(J0 + k) + (int64_t)(c0 + c) * m];
         }
         __syncthreads();
@@ -682,8 +684,6 @@ struct Plan {
   int* npert = nullptr;
   int mmax = 0;
   int pmax = 0;
-  void* inv = nullptr;        // diagonal-block inverses (grown on demand)
-  size_t inv_bytes = 0;
 };
 
 // Record a start/stop event pair around one launch when profiling is on.
@@ -718,22 +718,16 @@ int upload(Plan* P, X** dst, const X* src, int64_t count) {
   return 0;
 }
 
+// Panel width of a level: narrow panels for levels of small fronts.  The factorization and the
+// solves must agree on it (the diagonal blocks hold inverses of nbl x nbl triangles).
+int panel_width(int nb, const Level& L) { return (nb == 64 && L.pmax <= 128) ? 32 : nb; }
+
 template <typename T>
 int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta, const void* Kaa, const void* Maa,
                 const void* Daa, void* arena_v, int nb) {
   using V = typename cplx<T>::type;
   V* arena = (V*)arena_v;
   const T tiny = sizeof(T) == 4 ? (T)1e-30 : (T)1e-300;
-  // scratch for the diagonal-block inverses: one slot (2 nb^2) per front of a level and frequency
-  size_t need = 0;
-  for (const Level& L : P->levels) need = std::max(need, (size_t)L.nf * B * 2 * nb * nb * sizeof(V));
-  if (need > P->inv_bytes) {
-    if (P->inv) SAW_HIP(hipFree(P->inv));
-    P->inv = nullptr;
-    P->inv_bytes = 0;
-    SAW_HIP(hipMalloc(&P->inv, need));
-    P->inv_bytes = need;
-  }
   for (const Level& L : P->levels) {
     const int cpw = 8;
     { Timed tm(P, s, K_INIT);
@@ -743,32 +737,28 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
       hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), B), dim3(256), 0, s, P->F,
                          L.e0, L.e1, B, omega, beta, P->ent_front, P->ent_slot, P->ent_loc, (const double2*)Kaa,
                          (const double2*)Maa, (const double2*)Daa, arena); }
-    const int nbl = (nb == 64 && L.pmax <= 128) ? 32 : nb;   // narrow panels for small fronts
+    const int nbl = panel_width(nb, L);
     for (int k0 = 0; k0 < L.pmax; k0 += nbl) {
       const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
       { Timed tm(P, s, K_DIAG);
       hipLaunchKernelGGL(k_diag<T>, dim3(L.nf, B), dim3(128), lds_diag, s, P->F, L.fronts, B, arena, k0, nbl, tiny,
-                         P->npert, (V*)P->inv); }
+                         P->npert); }
       const int rem = L.mmax - k0 - 1;
       if (rem > 0) {
         const int nt = (rem + TILE - 1) / TILE;
         { Timed tm(P, s, K_TRSM);
-        hipLaunchKernelGGL(k_gemm<T>, dim3(nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 2, k0, nbl,
-                           (const V*)P->inv);
-        hipLaunchKernelGGL(k_gemm<T>, dim3(nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 3, k0, nbl,
-                           (const V*)P->inv); }
+        hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, k0, nbl);
+        hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, k0, nbl); }
         if (k0 + nbl < L.pmax) {
           Timed tm(P, s, K_GEMM_L);
-          hipLaunchKernelGGL(k_gemm<T>, dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 0, k0, nbl,
-                             (const V*)nullptr);
+          hipLaunchKernelGGL((k_gemm<T, 0>), dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, k0, nbl);
         }
       }
     }
     if (L.cbmax > 0) {
       const int nt = (L.cbmax + TILE - 1) / TILE;
       Timed tm(P, s, K_GEMM_S);
-      hipLaunchKernelGGL(k_gemm<T>, dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 1, 0, nb,
-                         (const V*)nullptr);
+      hipLaunchKernelGGL((k_gemm<T, 1>), dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 0, nb);
     }
   }
   SAW_HIP(hipGetLastError());
@@ -776,20 +766,21 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
 }
 
 template <typename T>
-int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose) {
+int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose, int nb) {
   using V = typename cplx<T>::type;
   constexpr int LD = SolveTile<T>::SB + 1;
-  const size_t lds_tri = sizeof(V) * 64 * LD + sizeof(double2) * P->pmax;
+  const size_t lds_tri = sizeof(V) * 64 * LD + sizeof(double2) * (256 + P->pmax);
   const V* A = (const V*)arena;
   double2* U = (double2*)uarena;
   double2* X = (double2*)x;
   auto tri = [&](const Level& L, int kind) {
     Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
+    const int nbl = panel_width(nb, L);
     switch (kind) {
-      case 0: hipLaunchKernelGGL((k_tri<T, 0>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, A, U, X); break;
-      case 1: hipLaunchKernelGGL((k_tri<T, 1>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, A, U, X); break;
-      case 2: hipLaunchKernelGGL((k_tri<T, 2>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, A, U, X); break;
-      default: hipLaunchKernelGGL((k_tri<T, 3>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, A, U, X);
+      case 0: hipLaunchKernelGGL((k_tri<T, 0>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
+      case 1: hipLaunchKernelGGL((k_tri<T, 1>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
+      case 2: hipLaunchKernelGGL((k_tri<T, 2>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
+      default: hipLaunchKernelGGL((k_tri<T, 3>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X);
     }
   };
   auto panel = [&](const Level& L, int kind) {
@@ -900,7 +891,6 @@ int sawfem_mf_destroy(void* plan) {
   Plan* P = (Plan*)plan;
   if (!P) return 0;
   for (void* d : P->owned) (void)hipFree(d);
-  if (P->inv) (void)hipFree(P->inv);
   for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
   delete P;
   return 0;
@@ -933,24 +923,27 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
                    : factor_impl<double>(P, s, B, omega, beta, Kaa, Maa, Daa, arena, nb);
 }
 
-// x: [B][n] complex128 right-hand sides, overwritten with the solutions.
+// x: [B][n] complex128 right-hand sides, overwritten with the solutions.  nb must be the
+// value the arena was factored with.
 int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x,
-                    int transpose) {
+                    int transpose, int nb) {
   Plan* P = (Plan*)plan;
-  if (!P || B <= 0 || !arena || !uarena || !x || (prec != 0 && prec != 1)) {
+  if (!P || B <= 0 || !arena || !uarena || !x || (prec != 0 && prec != 1) || (nb != 32 && nb != 64) ||
+      (prec == 1 && nb > 32)) {
     sawfem_set_error("sawfem_mf_solve: invalid arguments");
     return 1;
   }
   int nin_max = 0;
   for (const Level& L : P->levels) nin_max = std::max(nin_max, std::max(L.pmax, L.cbmax));
-  if ((size_t)P->pmax * sizeof(double2) + 64 * 33 * 16 > 64 * 1024 ||
+  // LDS of the widest tile: 64 x 65 complex64 or 64 x 33 complex128 (both <= 64 * 33 * 16 bytes)
+  if ((size_t)(P->pmax + 256) * sizeof(double2) + 64 * 33 * 16 > 64 * 1024 ||
       (size_t)nin_max * sizeof(double2) + 256 * 16 + 64 * 33 * 16 > 64 * 1024) {
     sawfem_set_error("sawfem_mf_solve: front too large for the LDS solve");
     return 1;
   }
   hipStream_t s = (hipStream_t)stream;
-  return prec == 0 ? solve_impl<float>(P, s, B, arena, uarena, x, transpose)
-                   : solve_impl<double>(P, s, B, arena, uarena, x, transpose);
+  return prec == 0 ? solve_impl<float>(P, s, B, arena, uarena, x, transpose, nb)
+                   : solve_impl<double>(P, s, B, arena, uarena, x, transpose, nb);
 }
 
 // Per-kernel event timing: enable (1) / disable (0); turning it on clears what was recorded.

@@ -121,28 +121,41 @@ def build_mf_arrays(sym: Symbolic, aa_indptr: np.ndarray, aa_indices: np.ndarray
                     ent_slot=ent_slot, ent_loc=ent_loc, ent_lvl_ptr=ent_lvl_ptr)
 
 
+def panel_width(nb: int, level_pmax: int) -> int:
+    """Panel width the factorization uses on a level (narrow panels for small fronts)."""
+    return 32 if (nb == 64 and level_pmax <= 128) else nb
+
+
 def kernel_work(mf: MFArrays, nb: int = 32) -> dict:
-    """Algorithmic work per frequency, counted from the fronts.
+    """Algorithmic work per frequency, counted from the fronts, for panel width ``nb``.
 
     * ``factor_flops``: 8·Σ_f Σ_{k<p} (m-k-1)² (complex LU, 8 real flops per complex
       multiply-add);
-    * ``gemm_flops``: the part of it done by the MFMA update kernel — L-shaped panel
-      updates 8·kb·((m-k1)² - (m-p)²) per panel plus the Schur complement 8·p·(m-p)²;
+    * ``lupd_flops``: L-shaped panel updates 8·kb·((m-k1)² - (m-p)²) per panel (all panels
+      but the last of a front), with the level's panel width;
+    * ``schur_flops``: Schur complement 8·p·(m-p)²;
+    * ``gemm_flops``: their sum (the MFMA update kernels);
     * ``solve_flops``: 8·(nnz(L)+nnz(U)) for one forward + backward substitution;
-    * ``factor_bytes``: stored L/U entries (one complex entry each).
+    * ``lu_entries``: stored L/U entries (pivot triangle + both panels), the data one solve
+      streams; ``arena_entries``: Σ m².
     """
     m = mf.m.astype(np.float64)
     p = mf.p.astype(np.float64)
     ff = 0.0
-    gf = 0.0
-    for mi, pi in zip(mf.m.astype(np.int64), mf.p.astype(np.int64)):
-        k = np.arange(pi, dtype=np.float64)
-        ff += float(((mi - k - 1) ** 2).sum())
-        for k0 in range(0, int(pi), nb):
-            k1 = min(int(pi), k0 + nb)
-            if k1 < pi:
-                gf += (k1 - k0) * float((mi - k1) ** 2 - (mi - pi) ** 2)
-        gf += float(pi) * float((mi - pi) ** 2)
+    lf = 0.0
+    sf = 0.0
+    for L in range(mf.lvl_ptr.size - 1):
+        fronts = mf.lvl_fronts[mf.lvl_ptr[L]:mf.lvl_ptr[L + 1]]
+        nbl = panel_width(nb, int(mf.p[fronts].max()))
+        for f in fronts:
+            mi, pi = int(mf.m[f]), int(mf.p[f])
+            k = np.arange(pi, dtype=np.float64)
+            ff += float(((mi - k - 1) ** 2).sum())
+            for k0 in range(0, pi, nbl):
+                k1 = min(pi, k0 + nbl)
+                if k1 < pi:
+                    lf += (k1 - k0) * float((mi - k1) ** 2 - (mi - pi) ** 2)
+            sf += float(pi) * float((mi - pi) ** 2)
     nnz_lu = float((p * p + 2 * p * (m - p)).sum())
-    return dict(factor_flops=8 * ff, gemm_flops=8 * gf, solve_flops=8 * nnz_lu, lu_entries=nnz_lu,
-                arena_entries=float((m * m).sum()))
+    return dict(factor_flops=8 * ff, lupd_flops=8 * lf, schur_flops=8 * sf, gemm_flops=8 * (lf + sf),
+                solve_flops=8 * nnz_lu, lu_entries=nnz_lu, arena_entries=float((m * m).sum()))

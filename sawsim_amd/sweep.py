@@ -22,8 +22,8 @@ Precision modes:
 
 * ``'fp64'``: complex128 fronts (plain path), one refinement step;
 * ``'mixed'``: complex64 fronts with FP32 MFMA accumulation, refined against the FP64
-  operator until Q stops changing (relative ``ir_tol``); frequencies that stall are
-  refactored in complex128.
+  operator until the estimated relative error of Q is below ``ir_tol`` (see
+  :func:`ir_error`); frequencies that stall are refactored in complex128.
 """
 from __future__ import annotations
 
@@ -36,6 +36,21 @@ from .layout import BlochLayout
 from .piezo_fem import DeviceCSR, Model, _torch, stream_ptr
 from .plan import build_mf_arrays
 from .symbolic import analyse
+
+
+def ir_error(dq, dq_prev):
+    """Estimated relative error of a refined quantity after its latest correction.
+
+    ``dq`` is the relative size of the latest correction, ``dq_prev`` that of the one before
+    (inf after the first).  Refinement converges linearly; with the observed rate
+    r = dq / dq_prev below 1/2, the corrections still to come sum to about dq r / (1 - r)
+    (geometric series).  Otherwise (first step, slow or stalled contraction) the estimate is
+    dq itself.
+    """
+    dq = np.asarray(dq, dtype=float)
+    prev = np.asarray(dq_prev, dtype=float)
+    r = np.where(np.isfinite(prev), dq / np.maximum(prev, 1e-300), 1.0)
+    return np.where(r < 0.5, dq * r / (1.0 - r), dq)
 
 
 def bc_periodic(model: Model, lines, tag_bottom=21, tag_left=19, tag_right=20, tag_zheng=17, tag_fu=18, ndisp=3):
@@ -249,7 +264,7 @@ class FrequencySolver:
         _lib.call('sawfem_rhs', s, n, B, om.data_ptr(), self.beta, float(V), self.rK.data_ptr(), self.rM.data_ptr(),
                   self._dptr(self.rD), rhs.data_ptr())
         x = rhs.clone()
-        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), x.data_ptr(), 0)
+        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), x.data_ptr(), 0, nb)
         Qd = torch.empty(B, dtype=torch.complex128, device=dev)
         charge = lambda: (_lib.call('sawfem_charge', s, n, B, self.kts_a.data_ptr(), self.kts_z.real,
                                     self.kts_z.imag, float(V), x.data_ptr(), Qd.data_ptr()),
@@ -260,30 +275,33 @@ class FrequencySolver:
         rhst = res.to_t(rhs)
         steps = 0
         dq = np.full(B, np.inf)
+        err = dq
         hist = []
         for steps in range(1, max_ir + 1):
             res(x, rhst, r, None)
-            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 0)
+            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 0, nb)
             _lib.call('sawfem_axpy', s, B * n, x.data_ptr(), r.data_ptr())
             Qn = charge()
-            dq = np.abs(Qn - Q) / np.maximum(np.abs(Qn), 1e-300)
+            dq_new = np.abs(Qn - Q) / np.maximum(np.abs(Qn), 1e-300)
+            err = ir_error(dq_new, dq)
+            dq = dq_new
             hist.append(float(dq.max()))
             Q = Qn
-            if np.all(dq <= ir_tol):
+            if np.all(err <= ir_tol):
                 break
-        out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, dq_hist=hist)
+        out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist)
         if gradient:
-            out.update(self._adjoint(om, omegas, V, prec, arena, uarena, rhs, x, ir_tol, max_ir))
+            out.update(self._adjoint(om, omegas, V, prec, nb, arena, uarena, rhs, x, ir_tol, max_ir))
         refactored = np.zeros(B, dtype=bool)
         if prec == 0 and fallback:
-            bad = np.flatnonzero(~(dq <= ir_tol) | (~(out.get('adj_dq', dq) <= ir_tol) if gradient else False))
+            bad = np.flatnonzero(~(err <= ir_tol) | (~(out.get('adj_err', err) <= ir_tol) if gradient else False))
             if bad.size:
                 del arena, uarena
                 sub = self.solve_batch(omegas[bad], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir,
                                        gradient=gradient)
                 tb = self.torch.as_tensor(bad, device=self.dev)
                 x[tb] = sub['x']
-                for key in ('Q', 'dq', 'dY', 'adj_dq'):
+                for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err'):
                     if key in out:
                         out[key][bad] = sub[key]
                 refactored[bad] = True
@@ -314,7 +332,7 @@ class FrequencySolver:
                 _lib.call('sawfem_transpose', s, n, B, Bp, n, rt.data_ptr(), r.data_ptr())
         return Op()
 
-    def _adjoint(self, om, omegas, V, prec, arena, uarena, rhs, x, ir_tol, max_ir):
+    def _adjoint(self, om, omegas, V, prec, nb, arena, uarena, rhs, x, ir_tol, max_ir):
         """lambda = A^-T kts_a (refined), then K12 and the parameter contraction."""
         torch = self.torch
         g = self._grad
@@ -322,22 +340,26 @@ class FrequencySolver:
         B, n = x.shape
         gvec = self.kts_a.unsqueeze(0).expand(B, n).contiguous()
         lam = gvec.clone()
-        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), lam.data_ptr(), 1)
+        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), lam.data_ptr(), 1, nb)
         # duality check: lambda^T b converges to kts_a^T x
         mu = lambda: (lam * rhs).sum(dim=1).cpu().numpy()
         m0 = mu()
         r = torch.empty_like(lam)
         dq = np.full(B, np.inf)
+        err = dq
         res = self._residual_op(B, om)
         gt = res.to_t(gvec)
-        for _ in range(max_ir):
+        steps = 0
+        for steps in range(1, max_ir + 1):
             res(lam, gt, r, self.tslot)
-            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 1)
+            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 1, nb)
             _lib.call('sawfem_axpy', s, B * n, lam.data_ptr(), r.data_ptr())
             m1 = mu()
-            dq = np.abs(m1 - m0) / np.maximum(np.abs(m1), 1e-300)
+            dq_new = np.abs(m1 - m0) / np.maximum(np.abs(m1), 1e-300)
+            err = ir_error(dq_new, dq)
+            dq = dq_new
             m0 = m1
-            if np.all(dq <= ir_tol):
+            if np.all(err <= ir_tol):
                 break
         ndu = self.layout.nd - 1
         NT = (2 * ndu) ** 2 + 4 * ndu + 4
@@ -350,7 +372,7 @@ class FrequencySolver:
                   g['omega_top'], ndu, g['N'], g['pos_rep'].data_ptr(), g['zmask'].data_ptr(), float(V), SCALE, B, n,
                   x.data_ptr(), lam.data_ptr(), work.data_ptr(), W.data_ptr())
         Wh = W.cpu().numpy().reshape(B, NT)
-        return dict(dY=self._param_gradient(Wh, omegas, V), adj_dq=dq, lam=lam)
+        return dict(dY=self._param_gradient(Wh, omegas, V), adj_dq=dq, adj_err=err, adj_steps=steps, lam=lam)
 
     def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, **kw):
         """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies.
@@ -366,10 +388,13 @@ class FrequencySolver:
             res = self.solve_batch(2 * np.pi * fre[i0:i0 + batch], V, precision=precision, gradient=gradient, **kw)
             Q[i0:i0 + batch] = res['Q']
             item = dict(ir_steps=res['ir_steps'], dq_max=float(np.max(res['dq'])),
-                        refactored=int(res['refactored'].sum()))
+                        err_est_max=float(np.max(res['err'])), refactored=int(res['refactored'].sum()))
             if gradient:
                 dY[i0:i0 + batch] = res['dY']
                 item['adj_dq_max'] = float(np.max(res['adj_dq']))
+                item['adj_err_est_max'] = float(np.max(res['adj_err']))
+                item['adj_steps'] = res['adj_steps']
+            item['points'] = int(min(batch, fre.size - i0))
             item['dq_hist'] = res.get('dq_hist')
             info.append(item)
             del res

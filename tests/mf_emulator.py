@@ -3,15 +3,32 @@
 Executes, front by front and level by level, the same operations the HIP kernels of
 ``sawsim_amd/csrc/mf.hip`` perform, on a flat arena with the same offsets, the same
 extend-add maps, entry offsets and blocked panel order (diagonal LU, panel solves,
-L-shaped update, Schur complement).  It checks the host plan arrays and the blocked
-algorithm on the CPU; it is not used by the product path.
+L-shaped update, Schur complement), including the storage convention of the diagonal
+blocks: after the factorization each nbl x nbl diagonal block holds L_D^{-1} strictly below
+its diagonal (unit diagonal implied) and U_D^{-1} on and above it, nbl being the level's
+panel width (32 for levels whose largest pivot block is <= 128 when nb is 64, else nb).
+It checks the host plan arrays and the blocked algorithm on the CPU; it is not used by the
+product path.
 """
 import numpy as np
+
+
+def panel_width(nb, level_pmax):
+    return 32 if (nb == 64 and level_pmax <= 128) else nb
+
+
+def _level_widths(mf, nb):
+    out = []
+    for L in range(mf.lvl_ptr.size - 1):
+        fronts = mf.lvl_fronts[mf.lvl_ptr[L]:mf.lvl_ptr[L + 1]]
+        out.append(panel_width(nb, int(mf.p[fronts].max())))
+    return out
 
 
 def factor(mf, B, omegas, beta, Kaa, Maa, Daa, nb=32, dtype=np.complex128):
     off = mf.off
     arena = np.zeros(mf.arena_elems * B, dtype=dtype)
+    widths = _level_widths(mf, nb)
 
     def front(f, b):
         m = int(mf.m[f])
@@ -42,11 +59,12 @@ def factor(mf, B, omegas, beta, Kaa, Maa, Daa, nb=32, dtype=np.complex128):
                 loc = mf.ent_loc[k]
                 m = mf.m[f]
                 Fm[loc % m, loc // m] += a[k - e0]
+            nbl = widths[L]
             for f in fronts:
                 Fm = front(f, b)
                 m, p = int(mf.m[f]), int(mf.p[f])
-                for k0 in range(0, p, nb):
-                    k1 = min(p, k0 + nb)
+                for k0 in range(0, p, nbl):
+                    k1 = min(p, k0 + nbl)
                     D = Fm[k0:k1, k0:k1]
                     for k in range(k1 - k0):
                         D[k + 1:, k] /= D[k, k]
@@ -55,6 +73,8 @@ def factor(mf, B, omegas, beta, Kaa, Maa, Daa, nb=32, dtype=np.complex128):
                     Uk = np.triu(D)
                     Fm[k1:, k0:k1] = np.linalg.solve(Uk.T, Fm[k1:, k0:k1].T).T
                     Fm[k0:k1, k1:] = np.linalg.solve(Lk, Fm[k0:k1, k1:])
+                    Li, Ui = np.linalg.inv(Lk), np.linalg.inv(Uk)
+                    D[:] = np.tril(Li, -1) + np.triu(Ui)
                     upd = Fm[k1:, k0:k1] @ Fm[k0:k1, k1:]
                     mask = np.ones((m - k1, m - k1), dtype=bool)
                     mask[max(0, p - k1):, max(0, p - k1):] = False
@@ -64,9 +84,11 @@ def factor(mf, B, omegas, beta, Kaa, Maa, Daa, nb=32, dtype=np.complex128):
     return arena
 
 
-def solve(mf, B, arena, x):
-    """x: (B, n) right-hand sides, solved in place (forward then backward, as k_fwd/k_bwd)."""
+def solve(mf, B, arena, x, nb=32):
+    """x: (B, n) right-hand sides, solved in place (forward then backward, as the pivot-block
+    and panel kernels); ``nb`` as given to :func:`factor`."""
     off, uoff = mf.off, mf.uoff
+    widths = _level_widths(mf, nb)
     u = np.zeros(mf.uarena_elems * B, dtype=np.complex128)
 
     def front(f, b):
@@ -89,8 +111,11 @@ def solve(mf, B, arena, x):
                     uc = u[uoff[ch] * B + b * mc: uoff[ch] * B + (b + 1) * mc]
                     sel = inv >= 0
                     w[sel] += uc[inv[sel]]
-                for k in range(p):
-                    w[k + 1:p] -= Fm[k + 1:p, k] * w[k]
+                nbl = widths[L]
+                for k0 in range(0, p, nbl):
+                    k1 = min(p, k0 + nbl)
+                    w[k0:k1] = (np.tril(Fm[k0:k1, k0:k1], -1) + np.eye(k1 - k0)) @ w[k0:k1]
+                    w[k1:p] -= Fm[k1:p, k0:k1] @ w[k0:k1]
                 uf = w[p:] - Fm[p:, :p] @ w[:p]
                 u[uoff[f] * B + b * (m - p): uoff[f] * B + (b + 1) * (m - p)] = uf
                 x[b, rows[:p]] = w[:p]
@@ -102,8 +127,10 @@ def solve(mf, B, arena, x):
                 Fm = front(f, b).astype(np.complex128)
                 z = x[b, rows].copy()
                 z[:p] -= Fm[:p, p:] @ z[p:]
-                for k in range(p - 1, -1, -1):
-                    z[k] = z[k] / Fm[k, k]
-                    z[:k] -= Fm[:k, k] * z[k]
+                nbl = widths[L]
+                for k0 in reversed(range(0, p, nbl)):
+                    k1 = min(p, k0 + nbl)
+                    z[k0:k1] = np.triu(Fm[k0:k1, k0:k1]) @ z[k0:k1]
+                    z[:k0] -= Fm[:k0, k0:k1] @ z[k0:k1]
                 x[b, rows[:p]] = z[:p]
     return x
