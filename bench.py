@@ -170,7 +170,9 @@ def main():
     achieved = schur_flops / (schur_ms * 1e-3) / 1e12 if schur_ms > 0 else None
     lupd_tf = work['lupd_flops'] * P * args.steps / (prof['gemm_l'][0] * 1e-3) / 1e12 if prof['gemm_l'][0] else None
     # solves: each forward+backward pass streams the stored L/U entries (complex64) once
-    nsolve_pts = sum(it['points'] * (2 + it['ir_steps'] + it.get('adj_steps', 0)) for it in info) * args.steps
+    # (passes over the factors: paired forward+adjoint solves share each pass)
+    nsolve_pts = sum(it['points'] * ((1 + it['ir_steps']) if it.get('paired_adjoint')
+                                     else (2 + it['ir_steps'] + it.get('adj_steps', 0))) for it in info) * args.steps
     solve_bytes = nsolve_pts * work['lu_entries'] * 8
     solve_ms = prof['fwd'][0] + prof['bwd'][0]
     solve_gbs = solve_bytes / (solve_ms * 1e-3) / 1e9 if solve_ms else None

@@ -67,12 +67,14 @@ int sawfem_mf_perturbed(void* plan, int reset);
 int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* omega, double beta, const void* Kaa,
                      const void* Maa, const void* Daa, void* arena, int nb);
 
-/* Forward/backward substitution with the factors of sawfem_mf_factor; x: B x n complex128,
- * right-hand sides in, solutions out.  uarena: uarena_elems * B complex128 scratch.
+/* Forward/backward substitution with the factors of sawfem_mf_factor; x: B x nrhs x n
+ * complex128 (nrhs = 1 or 2 right-hand sides per frequency, sharing its factors: each factor
+ * entry is read once for all of them), right-hand sides in, solutions out.
+ * uarena: uarena_elems * B * nrhs complex128 scratch.
  * transpose != 0 solves A^T x = b with the same factors (U^T then L^T; adjoint solve).
  * nb: the panel width the arena was factored with (the diagonal blocks store the inverses of
  * their triangular factors, so the solve has to walk the same blocks). */
-int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x,
+int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const void* arena, void* uarena, void* x,
                     int transpose, int nb);
 
 /* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;

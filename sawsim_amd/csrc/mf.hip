@@ -350,37 +350,56 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
 //   * a panel kernel over 64-wide tiles of the off-diagonal panel (L21, U12 or their
 //     transposes), many workgroups per front, coalesced loads (transposed panels go through
 //     an LDS tile).
-// Vectors: x [B][n] complex128 (pivot rows of each front are owned by that front); the update
-// vector of front f (its contribution to the parent's rows) lives in uarena.
+// Each frequency carries R right-hand sides (1 or 2) that share its factors: every factor
+// entry is read once and applied to all R vectors.
+// Vectors: x [B][R][n] complex128 (pivot rows of each front are owned by that front); the
+// update vectors of front f (its contribution to the parent's rows) live in uarena at
+// uoff[f] * B * R + (b * R + r) * (m - p).
 template <typename T> struct SolveTile;
 template <> struct SolveTile<float> { static constexpr int SB = 64; };    // widest panel, complex64
 template <> struct SolveTile<double> { static constexpr int SB = 32; };   // widest panel, complex128
 
-// w[i] (i < p) = x[rows[i]] + children's update entries; uf[i - p] (i >= p) = children's entries.
-__device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, int B, int m, int p, const int* rows,
-                                             const double2* xb, const double2* uarena, double2* w, double2* uf) {
+__device__ __forceinline__ double2 d2add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+
+// w[r][i] (i < p) = x_r[rows[i]] + children's update entries; uf_r[i - p] (i >= p) = children's
+// entries.  w is [R][p] in LDS.
+template <int R>
+__device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, int B, int n, int m, int p,
+                                             const int* rows, const double2* x, double2* uarena, double2* w) {
   const int nch = F.child_ptr[f + 1] - F.child_ptr[f];
+  const int q = m - p;
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
-    double2 v = (i < p) ? xb[rows[i]] : make_double2(0, 0);
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = (i < p) ? x[((int64_t)b * R + r) * n + rows[i]] : make_double2(0, 0);
     for (int c = 0; c < nch; ++c) {
       const int slot = F.child_ptr[f] + c;
       const int ci = F.inv[F.inv_ptr[slot] + i];
       if (ci >= 0) {
         const int ch = F.child_ids[slot];
-        const double2 u = uarena[F.uoff[ch] * B + (int64_t)b * (F.m[ch] - F.p[ch]) + ci];
-        v.x += u.x; v.y += u.y;
+        const int qc = F.m[ch] - F.p[ch];
+        const double2* u = uarena + F.uoff[ch] * B * R + (int64_t)b * R * qc + ci;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[r] = d2add(v[r], u[(int64_t)r * qc]);
       }
     }
-    if (i < p) w[i] = v; else uf[i - p] = v;
+    if (i < p) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) w[r * p + i] = v[r];
+    } else {
+      double2* uf = uarena + F.uoff[f] * B * R + (int64_t)b * R * q + (i - p);
+#pragma unroll
+      for (int r = 0; r < R; ++r) uf[(int64_t)r * q] = v[r];
+    }
   }
 }
 
 // Pivot triangle.  kind 0: L11 (unit lower, forward)   1: U11 (upper, backward)
 //                  2: U11^T (lower, forward)            3: L11^T (unit upper, backward)
-// Forward kinds gather the right-hand side and the children's updates first.  nbl is the
+// Forward kinds gather the right-hand sides and the children's updates first.  nbl is the
 // panel width the front was factored with (the diagonal blocks hold L_D^{-1} below and
 // U_D^{-1} on/above the diagonal).
-template <typename T, int KIND>
+template <typename T, int KIND, int R>
 __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
                                              const typename cplx<T>::type* __restrict__ arena,
                                              double2* __restrict__ uarena, double2* __restrict__ x) {
@@ -389,20 +408,21 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
   constexpr int LD = SB + 1;
   extern __shared__ unsigned char smem_raw[];
   V* tile = reinterpret_cast<V*>(smem_raw);                                   // [64][LD]
-  double2* part = reinterpret_cast<double2*>(smem_raw + sizeof(V) * 64 * LD);  // [4][64]
-  double2* w = part + 256;                                                    // [p]
+  double2* part = reinterpret_cast<double2*>(smem_raw + sizeof(V) * 64 * LD);  // [R][4][64]
+  double2* w = part + R * 256;                                                // [R][p]
   const int f = lvl[blockIdx.x];
   const int b = blockIdx.y;
   const int m = F.m[f], p = F.p[f];
   const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
-  double2* xb = x + (int64_t)b * n;
   const int t = threadIdx.x;
   constexpr bool fwd = (KIND == 0 || KIND == 2);
   if (fwd) {
-    solve_gather(F, f, b, B, m, p, rows, xb, uarena, w, uarena + F.uoff[f] * B + (int64_t)b * (m - p));
+    solve_gather<R>(F, f, b, B, n, m, p, rows, x, uarena, w);
   } else {
-    for (int i = t; i < p; i += blockDim.x) w[i] = xb[rows[i]];
+    for (int i = t; i < p; i += blockDim.x)
+#pragma unroll
+      for (int r = 0; r < R; ++r) w[r * p + i] = x[((int64_t)b * R + r) * n + rows[i]];
   }
   __syncthreads();
   const int nblk = (p + nbl - 1) / nbl;
@@ -417,36 +437,65 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
     __syncthreads();
     {
       const int i = t & 63, g = t >> 6;   // output row, quarter of the reduction
-      double2 s = make_double2(0, 0);
+      double2 s[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) s[r] = make_double2(0, 0);
       if (i < nbk) {
         if (KIND == 0) {          // L_D^{-1} (unit lower): w_i + sum_{k<i} Li[i][k] w_k
-          for (int k = g; k < i; k += 4) s = cfma(s, to_d2(tile[i + k * LD]), w[J0 + k]);
+          for (int k = g; k < i; k += 4) {
+            const double2 a = to_d2(tile[i + k * LD]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * p + J0 + k]);
+          }
         } else if (KIND == 1) {   // U_D^{-1}: sum_{k>=i} Ui[i][k] w_k
-          for (int k = i + g; k < nbk; k += 4) s = cfma(s, to_d2(tile[i + k * LD]), w[J0 + k]);
+          for (int k = i + g; k < nbk; k += 4) {
+            const double2 a = to_d2(tile[i + k * LD]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * p + J0 + k]);
+          }
         } else if (KIND == 2) {   // U_D^{-T}: sum_{k<=i} Ui[k][i] w_k
-          for (int k = g; k <= i; k += 4) s = cfma(s, to_d2(tile[k + i * LD]), w[J0 + k]);
+          for (int k = g; k <= i; k += 4) {
+            const double2 a = to_d2(tile[k + i * LD]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * p + J0 + k]);
+          }
         } else {                  // L_D^{-T} (unit upper): w_i + sum_{k>i} Li[k][i] w_k
-          for (int k = i + 1 + g; k < nbk; k += 4) s = cfma(s, to_d2(tile[k + i * LD]), w[J0 + k]);
+          for (int k = i + 1 + g; k < nbk; k += 4) {
+            const double2 a = to_d2(tile[k + i * LD]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * p + J0 + k]);
+          }
         }
       }
-      part[g * 64 + i] = s;
+#pragma unroll
+      for (int r = 0; r < R; ++r) part[r * 256 + g * 64 + i] = s[r];
     }
     __syncthreads();
     if (t < nbk) {
-      double2 tot = part[t];
-      for (int g = 1; g < 4; ++g) { tot.x += part[g * 64 + t].x; tot.y += part[g * 64 + t].y; }
-      if (KIND == 0 || KIND == 3) { tot.x += w[J0 + t].x; tot.y += w[J0 + t].y; }
-      w[J0 + t] = tot;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        double2 tot = part[r * 256 + t];
+        for (int g = 1; g < 4; ++g) tot = d2add(tot, part[r * 256 + g * 64 + t]);
+        if (KIND == 0 || KIND == 3) tot = d2add(tot, w[r * p + J0 + t]);
+        w[r * p + J0 + t] = tot;
+      }
     }
     __syncthreads();
     // remaining pivot rows: forward kinds update [J0 + nbk, p), backward kinds [0, J0)
     const int lo = fwd ? J0 + nbk : 0, hi = fwd ? p : J0;
     if (KIND == 0 || KIND == 1) {
       // column panel F[lo:hi, J0:J0+nbk] (contiguous rows): thread per row
-      for (int r = lo + t; r < hi; r += blockDim.x) {
-        double2 s = w[r];
-        for (int k = 0; k < nbk; ++k) s = cfms(s, to_d2(Fb[(int64_t)r + (int64_t)(J0 + k) * m]), w[J0 + k]);
-        w[r] = s;
+      for (int i = lo + t; i < hi; i += blockDim.x) {
+        double2 s[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[r] = w[r * p + i];
+        for (int k = 0; k < nbk; ++k) {
+          const double2 a = to_d2(Fb[(int64_t)i + (int64_t)(J0 + k) * m]);
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * p + J0 + k]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) w[r * p + i] = s[r];
       }
     } else {
       // row panel F[J0:J0+nbk, lo:hi] read as its transpose through a 64-column LDS tile
@@ -459,15 +508,24 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
         }
         __syncthreads();
         if (t < nc) {
-          double2 s = w[c0 + t];
-          for (int k = 0; k < nbk; ++k) s = cfms(s, to_d2(tile[t * LD + k]), w[J0 + k]);
-          w[c0 + t] = s;
+          double2 s[R];
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[r] = w[r * p + c0 + t];
+          for (int k = 0; k < nbk; ++k) {
+            const double2 a = to_d2(tile[t * LD + k]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * p + J0 + k]);
+          }
+#pragma unroll
+          for (int r = 0; r < R; ++r) w[r * p + c0 + t] = s[r];
         }
       }
     }
     __syncthreads();
   }
-  for (int i = t; i < p; i += blockDim.x) xb[rows[i]] = w[i];
+  for (int i = t; i < p; i += blockDim.x)
+#pragma unroll
+    for (int r = 0; r < R; ++r) x[((int64_t)b * R + r) * n + rows[i]] = w[r * p + i];
 }
 
 // Off-diagonal panels, one workgroup per 64-wide output tile of one front and frequency.
@@ -475,7 +533,7 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
 //  kind 1 (backward, U12):   x_p[r] -= sum_j U[r][p + j] x_cb[j]      r in pivot rows
 //  kind 2 (forward,  U12^T): uf[j]  -= sum_k U[k][p + j] w[k]         j in CB rows
 //  kind 3 (backward, L21^T): x_p[k] -= sum_j L[p + j][k] x_cb[j]      k in pivot rows
-template <typename T, int KIND>
+template <typename T, int KIND, int R>
 __global__ void __launch_bounds__(256) k_panel(FrontsDev F, const int* __restrict__ lvl, int B, int n,
                                                const typename cplx<T>::type* __restrict__ arena,
                                                double2* __restrict__ uarena, double2* __restrict__ x) {
@@ -493,54 +551,74 @@ __global__ void __launch_bounds__(256) k_panel(FrontsDev F, const int* __restric
   const int no = min(64, nout - o0);
   const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
-  double2* xb = x + (int64_t)b * n;
-  double2* uf = uarena + F.uoff[f] * B + (int64_t)b * q;
-  double2* v = reinterpret_cast<double2*>(smem_raw);        // [nin] input vector
-  double2* part = v + nin;                                    // [4][64] partial sums
+  double2* uf = uarena + F.uoff[f] * B * R + (int64_t)b * R * q;   // + r * q
+  constexpr int VC = 256;                                     // input entries staged per chunk
+  double2* v = reinterpret_cast<double2*>(smem_raw);        // [R][VC] input vector chunk
+  double2* part = v + R * VC;                                 // [R][4][64] partial sums
+  V* tile = reinterpret_cast<V*>(part + R * 256);             // [64][KC2 + 1] (transposed kinds)
   const int t = threadIdx.x;
-  for (int i = t; i < nin; i += blockDim.x) v[i] = xb[rows[(out_cb ? 0 : p) + i]];
-  __syncthreads();
-  const int r = t & 63, g = t >> 6;   // output within tile, reduction quarter
-  double2 s = make_double2(0, 0);
-  if (KIND == 0 || KIND == 1) {
-    // direct: output o is a row of the panel; coalesced across r
-    const int row = (KIND == 0 ? p : 0) + o0 + r;
-    const int cbase = KIND == 0 ? 0 : p;
-    if (r < no)
-      for (int k = g; k < nin; k += 4) s = cfma(s, to_d2(Fb[(int64_t)row + (int64_t)(cbase + k) * m]), v[k]);
-  } else {
-    // transposed: output o is a column of the stored panel; stage 64 x 64 tiles through LDS
-    constexpr int KC2 = SolveTile<T>::SB;                      // reduction chunk
-    V* tile = reinterpret_cast<V*>(part + 256);                // [64][KC2 + 1]
-    const int rbase = KIND == 2 ? 0 : p;                       // stored rows (reduction index)
-    const int cbase = (KIND == 2 ? p : 0) + o0;                // stored columns (outputs)
-    for (int k0 = 0; k0 < nin; k0 += KC2) {
-      const int nk = min(KC2, nin - k0);
-      __syncthreads();
-      for (int idx = t; idx < 64 * KC2; idx += blockDim.x) {
-        const int kk = idx % KC2, c = idx / KC2;
-        if (kk < nk && c < no) tile[c * (KC2 + 1) + kk] = Fb[(int64_t)(rbase + k0 + kk) + (int64_t)(cbase + c) * m];
+  const int o = t & 63, g = t >> 6;   // output within tile, reduction quarter
+  double2 s[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) s[r] = make_double2(0, 0);
+  for (int v0 = 0; v0 < nin; v0 += VC) {
+    const int nv = min(VC, nin - v0);
+    __syncthreads();
+    for (int i = t; i < nv; i += blockDim.x)
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r * VC + i] = x[((int64_t)b * R + r) * n + rows[(out_cb ? 0 : p) + v0 + i]];
+    __syncthreads();
+    if (KIND == 0 || KIND == 1) {
+      // direct: output o is a row of the panel; coalesced across o
+      const int row = (KIND == 0 ? p : 0) + o0 + o;
+      const int cbase = (KIND == 0 ? 0 : p) + v0;
+      if (o < no)
+        for (int k = g; k < nv; k += 4) {
+          const double2 a = to_d2(Fb[(int64_t)row + (int64_t)(cbase + k) * m]);
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, v[r * VC + k]);
+        }
+    } else {
+      // transposed: output o is a column of the stored panel; stage 64 x KC2 tiles through LDS
+      constexpr int KC2 = SolveTile<T>::SB;                    // reduction sub-chunk
+      const int rbase = (KIND == 2 ? 0 : p) + v0;              // stored rows (reduction index)
+      const int cbase = (KIND == 2 ? p : 0) + o0;              // stored columns (outputs)
+      for (int k0 = 0; k0 < nv; k0 += KC2) {
+        const int nk = min(KC2, nv - k0);
+        __syncthreads();
+        for (int idx = t; idx < 64 * KC2; idx += blockDim.x) {
+          const int kk = idx % KC2, c = idx / KC2;
+          if (kk < nk && c < no) tile[c * (KC2 + 1) + kk] = Fb[(int64_t)(rbase + k0 + kk) + (int64_t)(cbase + c) * m];
+        }
+        __syncthreads();
+        if (o < no)
+          for (int kk = g; kk < nk; kk += 4) {
+            const double2 a = to_d2(tile[o * (KC2 + 1) + kk]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, v[r * VC + k0 + kk]);
+          }
       }
-      __syncthreads();
-      if (r < no)
-        for (int kk = g; kk < nk; kk += 4) s = cfma(s, to_d2(tile[r * (KC2 + 1) + kk]), v[k0 + kk]);
     }
   }
-  part[g * 64 + r] = s;
+#pragma unroll
+  for (int r = 0; r < R; ++r) part[r * 256 + g * 64 + o] = s[r];
   __syncthreads();
   if (t < no) {
-    double2 tot = part[t];
-    for (int gg = 1; gg < 4; ++gg) { tot.x += part[gg * 64 + t].x; tot.y += part[gg * 64 + t].y; }
-    if (out_cb) {
-      uf[o0 + t].x -= tot.x; uf[o0 + t].y -= tot.y;
-    } else {
-      double2* dst = xb + rows[o0 + t];
-      dst->x -= tot.x; dst->y -= tot.y;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      double2 tot = part[r * 256 + t];
+      for (int gg = 1; gg < 4; ++gg) tot = d2add(tot, part[r * 256 + gg * 64 + t]);
+      if (out_cb) {
+        double2* d = uf + (int64_t)r * q + o0 + t;
+        d->x -= tot.x; d->y -= tot.y;
+      } else {
+        double2* d = x + ((int64_t)b * R + r) * n + rows[o0 + t];
+        d->x -= tot.x; d->y -= tot.y;
+      }
     }
   }
 }
 
-// ---------------------------------------------------------------- per-frequency vectors (FP64)
 __global__ void k_rhs(int n, int B, const double* __restrict__ omega, double beta, double V,
                       const double2* __restrict__ rK, const double2* __restrict__ rM,
                       const double2* __restrict__ rD, double2* __restrict__ out) {
@@ -765,22 +843,23 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
   return 0;
 }
 
-template <typename T>
+template <typename T, int R>
 int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose, int nb) {
   using V = typename cplx<T>::type;
   constexpr int LD = SolveTile<T>::SB + 1;
-  const size_t lds_tri = sizeof(V) * 64 * LD + sizeof(double2) * (256 + P->pmax);
+  const size_t lds_tri = sizeof(V) * 64 * LD + sizeof(double2) * R * (256 + P->pmax);
   const V* A = (const V*)arena;
   double2* U = (double2*)uarena;
   double2* X = (double2*)x;
   auto tri = [&](const Level& L, int kind) {
     Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
     const int nbl = panel_width(nb, L);
+    const dim3 grid(L.nf, B);
     switch (kind) {
-      case 0: hipLaunchKernelGGL((k_tri<T, 0>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
-      case 1: hipLaunchKernelGGL((k_tri<T, 1>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
-      case 2: hipLaunchKernelGGL((k_tri<T, 2>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
-      default: hipLaunchKernelGGL((k_tri<T, 3>), dim3(L.nf, B), dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X);
+      case 0: hipLaunchKernelGGL((k_tri<T, 0, R>), grid, dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
+      case 1: hipLaunchKernelGGL((k_tri<T, 1, R>), grid, dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
+      case 2: hipLaunchKernelGGL((k_tri<T, 2, R>), grid, dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
+      default: hipLaunchKernelGGL((k_tri<T, 3, R>), grid, dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X);
     }
   };
   auto panel = [&](const Level& L, int kind) {
@@ -788,15 +867,14 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     const int nout = out_cb ? L.cbmax : L.pmax;
     const int nin = out_cb ? L.pmax : L.cbmax;
     if (nout == 0 || nin == 0) return;
-    const size_t lds = sizeof(double2) * (nin + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
-    if (lds > 64 * 1024) return;   // rejected up front by sawfem_mf_solve
+    const size_t lds = sizeof(double2) * R * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
     Timed tm(P, s, out_cb ? K_FWD : K_BWD);
     dim3 grid((nout + 63) / 64, L.nf, B);
     switch (kind) {
-      case 0: hipLaunchKernelGGL((k_panel<T, 0>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
-      case 1: hipLaunchKernelGGL((k_panel<T, 1>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
-      case 2: hipLaunchKernelGGL((k_panel<T, 2>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
-      default: hipLaunchKernelGGL((k_panel<T, 3>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X);
+      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
+      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
+      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
+      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X);
     }
   };
   const int kf = transpose ? 2 : 0, kb = transpose ? 3 : 1;
@@ -923,27 +1001,30 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
                    : factor_impl<double>(P, s, B, omega, beta, Kaa, Maa, Daa, arena, nb);
 }
 
-// x: [B][n] complex128 right-hand sides, overwritten with the solutions.  nb must be the
-// value the arena was factored with.
-int sawfem_mf_solve(void* plan, void* stream, int prec, int B, const void* arena, void* uarena, void* x,
+// x: [B][nrhs][n] complex128 right-hand sides (nrhs = 1 or 2 per frequency, sharing that
+// frequency's factors), overwritten with the solutions.  nb must be the value the arena was
+// factored with.  uarena: uarena_elems * B * nrhs complex128.
+int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const void* arena, void* uarena, void* x,
                     int transpose, int nb) {
   Plan* P = (Plan*)plan;
   if (!P || B <= 0 || !arena || !uarena || !x || (prec != 0 && prec != 1) || (nb != 32 && nb != 64) ||
-      (prec == 1 && nb > 32)) {
+      (prec == 1 && nb > 32) || (nrhs != 1 && nrhs != 2)) {
     sawfem_set_error("sawfem_mf_solve: invalid arguments");
     return 1;
   }
-  int nin_max = 0;
-  for (const Level& L : P->levels) nin_max = std::max(nin_max, std::max(L.pmax, L.cbmax));
-  // LDS of the widest tile: 64 x 65 complex64 or 64 x 33 complex128 (both <= 64 * 33 * 16 bytes)
-  if ((size_t)(P->pmax + 256) * sizeof(double2) + 64 * 33 * 16 > 64 * 1024 ||
-      (size_t)nin_max * sizeof(double2) + 256 * 16 + 64 * 33 * 16 > 64 * 1024) {
+  // LDS of the pivot-block kernel: the widest tile (64 x 65 complex64 or 64 x 33 complex128,
+  // both <= 64 * 33 * 16 bytes), partial sums and the pivot entries of the nrhs vectors
+  const size_t tile = 64 * 33 * 16;
+  if ((size_t)nrhs * (P->pmax + 256) * sizeof(double2) + tile > 64 * 1024) {
     sawfem_set_error("sawfem_mf_solve: front too large for the LDS solve");
     return 1;
   }
   hipStream_t s = (hipStream_t)stream;
-  return prec == 0 ? solve_impl<float>(P, s, B, arena, uarena, x, transpose, nb)
-                   : solve_impl<double>(P, s, B, arena, uarena, x, transpose, nb);
+  if (nrhs == 1)
+    return prec == 0 ? solve_impl<float, 1>(P, s, B, arena, uarena, x, transpose, nb)
+                     : solve_impl<double, 1>(P, s, B, arena, uarena, x, transpose, nb);
+  return prec == 0 ? solve_impl<float, 2>(P, s, B, arena, uarena, x, transpose, nb)
+                   : solve_impl<double, 2>(P, s, B, arena, uarena, x, transpose, nb);
 }
 
 // Per-kernel event timing: enable (1) / disable (0); turning it on clears what was recorded.

@@ -49,8 +49,9 @@ def ir_error(dq, dq_prev):
     """
     dq = np.asarray(dq, dtype=float)
     prev = np.asarray(dq_prev, dtype=float)
-    r = np.where(np.isfinite(prev), dq / np.maximum(prev, 1e-300), 1.0)
-    return np.where(r < 0.5, dq * r / (1.0 - r), dq)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        r = np.where(np.isfinite(prev), dq / np.maximum(prev, 1e-300), 1.0)
+        return np.where(r < 0.5, dq * r / (1.0 - r), dq)
 
 
 def bc_periodic(model: Model, lines, tag_bottom=21, tag_left=19, tag_right=20, tag_zheng=17, tag_fu=18, ndisp=3):
@@ -160,6 +161,7 @@ class FrequencySolver:
                   K.values.data_ptr(), None, None, self._kts.data_ptr(), None, None)
         self.kts_a = self._kts[:n]
         self.kts_z = complex(self._kts[n].item())
+        self._sym = None
         if D is not None:
             self.set_damping(D, self.beta if beta is None else beta)
 
@@ -175,6 +177,7 @@ class FrequencySolver:
         _lib.call('sawfem_gather3', s, self.n_active, self._rh[0].data_ptr(), self._rh[1].data_ptr(),
                   D.values.data_ptr(), None, None, self.rD.data_ptr(), None, None)
         self.beta = float(beta)
+        self._sym = None
 
     # ---------------------------------------------------------------- gradient context
     def enable_gradient(self, model, mat_idx, pml_regions, omega_top, params, plane='xy', material=0):
@@ -209,6 +212,30 @@ class FrequencySolver:
         self.tslot = t(np.searchsorted(key, tkey).astype(np.int64))
         if not np.array_equal(key[np.searchsorted(key, tkey)], tkey):
             raise ValueError('active pattern is not structurally symmetric')
+        from .piezo_fem import SCALE
+        is_phi = bl.comp_of == (self.layout.nd - 1)
+        self._dscale = t(np.where(is_phi, 1.0 / SCALE, 1.0))
+        self._arow = t(r)
+        self._sym = None
+
+    def scaled_symmetric(self, rtol=1e-12):
+        """True when S = D A_aa is symmetric for D = diag(1 on displacements, 1/SCALE on the
+        potential), checked on K, M and D separately.  The piezoelectric blocks satisfy
+        K_phi,u = SCALE K_u,phi^T, the Bloch merge and the Dirichlet eliminations keep the
+        symmetry of S, so A^-T g = D A^-1 (D^-1 g): the adjoint solve becomes a second
+        right-hand side of the forward solve."""
+        if self._sym is None:
+            d = self._dscale
+            dr, dc = d[self._arow], d[self.aa_indices.long()]
+            ok = True
+            for A in (self.Kaa, self.Maa, self.Daa):
+                if A is None:
+                    continue
+                S = A * dr
+                St = A[self.tslot] * dc
+                ok &= bool((S - St).abs().max() <= rtol * S.abs().max())
+            self._sym = ok
+        return self._sym
 
     def _param_gradient(self, W, omegas, V):
         """dY11/dp (B, 17) from the element sensitivity sums W (B, NT)."""
@@ -241,11 +268,15 @@ class FrequencySolver:
         return None if t is None else t.data_ptr()
 
     def solve_batch(self, omegas, V=1.0, precision='mixed', ir_tol=1e-9, max_ir=8, nb=None, fallback=True,
-                    gradient=False):
+                    gradient=False, adjoint='auto'):
         """Solve all ``omegas`` in one batched factorization.
 
         Returns dict with ``x`` (B, n) complex128 device tensor, ``Q`` (B,) complex numpy,
-        ``ir_steps``, ``dq`` (last relative change of Q per frequency), ``refactored``.
+        ``ir_steps``, ``dq`` (last relative change of Q per frequency), ``err`` (its
+        estimated remaining error), ``refactored``; with ``gradient`` also ``dY`` and the
+        adjoint's ``adj_*`` entries.  ``adjoint``: 'auto' pairs the adjoint with the forward
+        solve when :meth:`scaled_symmetric` holds, 'paired' requires it, 'transposed' always
+        uses separate transposed solves.
         """
         torch = self.torch
         dev = self.dev
@@ -253,18 +284,54 @@ class FrequencySolver:
         B, n = omegas.size, self.n_active
         prec = 0 if precision == 'mixed' else 1
         nb = nb or (64 if prec == 0 else 32)
+        paired = False
+        if gradient and adjoint != 'transposed':
+            paired = self.scaled_symmetric()
+            if adjoint == 'paired' and not paired:
+                raise ValueError('D A is not symmetric: the paired adjoint does not apply')
+        R = 2 if paired else 1
         s = stream_ptr()
         om = torch.as_tensor(omegas, device=dev)
         arena = torch.empty(self.mf.arena_elems * B, dtype=torch.complex64 if prec == 0 else torch.complex128,
                             device=dev)
-        uarena = torch.empty(max(1, self.mf.uarena_elems * B), dtype=torch.complex128, device=dev)
+        uarena = torch.empty(max(1, self.mf.uarena_elems * B * R), dtype=torch.complex128, device=dev)
         _lib.call('sawfem_mf_factor', self._plan, s, prec, B, om.data_ptr(), self.beta, self.Kaa.data_ptr(),
                   self.Maa.data_ptr(), self._dptr(self.Daa), arena.data_ptr(), nb)
         rhs = torch.empty((B, n), dtype=torch.complex128, device=dev)
         _lib.call('sawfem_rhs', s, n, B, om.data_ptr(), self.beta, float(V), self.rK.data_ptr(), self.rM.data_ptr(),
                   self._dptr(self.rD), rhs.data_ptr())
+        if paired:
+            out = self._solve_paired(om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir)
+        else:
+            out = self._solve_single(om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir)
+            if gradient:
+                out.update(self._adjoint(om, omegas, V, prec, nb, arena, uarena, rhs, out['x'], ir_tol, max_ir))
+        x, err = out['x'], out['err']
+        refactored = np.zeros(B, dtype=bool)
+        if prec == 0 and fallback:
+            bad = np.flatnonzero(~(err <= ir_tol) | (~(out.get('adj_err', err) <= ir_tol) if gradient else False))
+            if bad.size:
+                del arena, uarena
+                sub = self.solve_batch(omegas[bad], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir,
+                                       gradient=gradient, adjoint=adjoint)
+                tb = self.torch.as_tensor(bad, device=self.dev)
+                x[tb] = sub['x']
+                for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err'):
+                    if key in out:
+                        out[key][bad] = sub[key]
+                refactored[bad] = True
+        out['refactored'] = refactored
+        return out
+
+    def _solve_single(self, om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir):
+        """x = A^-1 rhs refined against the FP64 operator until Q's estimated error <= ir_tol."""
+        torch = self.torch
+        dev = self.dev
+        s = stream_ptr()
+        B, n = rhs.shape
         x = rhs.clone()
-        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), x.data_ptr(), 0, nb)
+        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), x.data_ptr(), 0,
+                  nb)
         Qd = torch.empty(B, dtype=torch.complex128, device=dev)
         charge = lambda: (_lib.call('sawfem_charge', s, n, B, self.kts_a.data_ptr(), self.kts_z.real,
                                     self.kts_z.imag, float(V), x.data_ptr(), Qd.data_ptr()),
@@ -279,7 +346,8 @@ class FrequencySolver:
         hist = []
         for steps in range(1, max_ir + 1):
             res(x, rhst, r, None)
-            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 0, nb)
+            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(),
+                      0, nb)
             _lib.call('sawfem_axpy', s, B * n, x.data_ptr(), r.data_ptr())
             Qn = charge()
             dq_new = np.abs(Qn - Q) / np.maximum(np.abs(Qn), 1e-300)
@@ -289,23 +357,60 @@ class FrequencySolver:
             Q = Qn
             if np.all(err <= ir_tol):
                 break
+        return dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist)
+
+    def _solve_paired(self, om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir):
+        """Forward and adjoint systems as two right-hand sides of one refined solve.
+
+        With S = D A symmetric (see :meth:`scaled_symmetric`), lambda = A^-T kts_a =
+        D z where A z = D^-1 kts_a.  Both columns are refined together against the FP64
+        operator: Q's estimated error and that of the duality product lambda^T rhs must
+        both reach ``ir_tol``.
+        """
+        torch = self.torch
+        s = stream_ptr()
+        B, n = rhs.shape
+        d = self._dscale
+        X = torch.stack([rhs, (self.kts_a / d).unsqueeze(0).expand(B, n)], dim=1).contiguous()   # [B][2][n]
+        rhs2 = X.clone()
+        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 2, arena.data_ptr(), uarena.data_ptr(), X.data_ptr(), 0,
+                  nb)
+        Qd = torch.empty(2 * B, dtype=torch.complex128, device=self.dev)
+
+        def measures():
+            _lib.call('sawfem_charge', s, n, 2 * B, self.kts_a.data_ptr(), self.kts_z.real, self.kts_z.imag, float(V),
+                      X.data_ptr(), Qd.data_ptr())
+            mu = (X[:, 1] * d * rhs).sum(dim=1)
+            return Qd[0::2].cpu().numpy().copy(), mu.cpu().numpy()
+        Q, mu = measures()
+        om2 = om.repeat_interleave(2)
+        res = self._residual_op(2 * B, om2)
+        rhst = res.to_t(rhs2)
+        r = torch.empty_like(X)
+        dq = np.full(B, np.inf)
+        dm = np.full(B, np.inf)
+        err = aerr = dq
+        hist = []
+        steps = 0
+        for steps in range(1, max_ir + 1):
+            res(X, rhst, r, None)
+            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 2, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(),
+                      0, nb)
+            _lib.call('sawfem_axpy', s, 2 * B * n, X.data_ptr(), r.data_ptr())
+            Qn, mun = measures()
+            dq_new = np.abs(Qn - Q) / np.maximum(np.abs(Qn), 1e-300)
+            dm_new = np.abs(mun - mu) / np.maximum(np.abs(mun), 1e-300)
+            err, aerr = ir_error(dq_new, dq), ir_error(dm_new, dm)
+            dq, dm, Q, mu = dq_new, dm_new, Qn, mun
+            hist.append(float(dq.max()))
+            if np.all(err <= ir_tol) and np.all(aerr <= ir_tol):
+                break
+        x = X[:, 0].contiguous()
+        lam = (X[:, 1] * d).contiguous()
+        del X, r, rhs2, rhst
         out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist)
-        if gradient:
-            out.update(self._adjoint(om, omegas, V, prec, nb, arena, uarena, rhs, x, ir_tol, max_ir))
-        refactored = np.zeros(B, dtype=bool)
-        if prec == 0 and fallback:
-            bad = np.flatnonzero(~(err <= ir_tol) | (~(out.get('adj_err', err) <= ir_tol) if gradient else False))
-            if bad.size:
-                del arena, uarena
-                sub = self.solve_batch(omegas[bad], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir,
-                                       gradient=gradient)
-                tb = self.torch.as_tensor(bad, device=self.dev)
-                x[tb] = sub['x']
-                for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err'):
-                    if key in out:
-                        out[key][bad] = sub[key]
-                refactored[bad] = True
-        out['refactored'] = refactored
+        out.update(self._contract(omegas, V, x, lam))
+        out.update(adj_dq=dm, adj_err=aerr, adj_steps=steps, lam=lam, paired=True)
         return out
 
     def _residual_op(self, B, om):
@@ -340,7 +445,8 @@ class FrequencySolver:
         B, n = x.shape
         gvec = self.kts_a.unsqueeze(0).expand(B, n).contiguous()
         lam = gvec.clone()
-        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), lam.data_ptr(), 1, nb)
+        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), lam.data_ptr(), 1,
+                  nb)
         # duality check: lambda^T b converges to kts_a^T x
         mu = lambda: (lam * rhs).sum(dim=1).cpu().numpy()
         m0 = mu()
@@ -352,7 +458,8 @@ class FrequencySolver:
         steps = 0
         for steps in range(1, max_ir + 1):
             res(lam, gt, r, self.tslot)
-            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(), 1, nb)
+            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(),
+                      1, nb)
             _lib.call('sawfem_axpy', s, B * n, lam.data_ptr(), r.data_ptr())
             m1 = mu()
             dq_new = np.abs(m1 - m0) / np.maximum(np.abs(m1), 1e-300)
@@ -361,6 +468,17 @@ class FrequencySolver:
             m0 = m1
             if np.all(err <= ir_tol):
                 break
+        out = self._contract(omegas, V, x, lam)
+        out.update(adj_dq=dq, adj_err=err, adj_steps=steps, lam=lam)
+        return out
+
+    def _contract(self, omegas, V, x, lam):
+        """dY11/dp from the forward solution x and the adjoint lambda (element sensitivities
+        on the GPU, parameter contraction on the host)."""
+        torch = self.torch
+        g = self._grad
+        s = stream_ptr()
+        B, n = x.shape
         ndu = self.layout.nd - 1
         NT = (2 * ndu) ** 2 + 4 * ndu + 4
         work = torch.empty(max(1, B * g['ne0'] * NT), dtype=torch.complex128, device=self.dev)
@@ -372,7 +490,7 @@ class FrequencySolver:
                   g['omega_top'], ndu, g['N'], g['pos_rep'].data_ptr(), g['zmask'].data_ptr(), float(V), SCALE, B, n,
                   x.data_ptr(), lam.data_ptr(), work.data_ptr(), W.data_ptr())
         Wh = W.cpu().numpy().reshape(B, NT)
-        return dict(dY=self._param_gradient(Wh, omegas, V), adj_dq=dq, adj_err=err, adj_steps=steps, lam=lam)
+        return dict(dY=self._param_gradient(Wh, omegas, V))
 
     def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, **kw):
         """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies.
@@ -394,6 +512,7 @@ class FrequencySolver:
                 item['adj_dq_max'] = float(np.max(res['adj_dq']))
                 item['adj_err_est_max'] = float(np.max(res['adj_err']))
                 item['adj_steps'] = res['adj_steps']
+                item['paired_adjoint'] = bool(res.get('paired', False))
             item['points'] = int(min(batch, fre.size - i0))
             item['dq_hist'] = res.get('dq_hist')
             info.append(item)

@@ -66,3 +66,24 @@ def test_adjoint_matches_reference_finite_differences():
     _say('Y11 vs reference', ey, 'dY vs reference FD', err)
     assert ey.max() <= 1e-8
     assert err.max() <= 1e-4
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_paired_adjoint_matches_transposed_solves(precision):
+    """The adjoint as a second right-hand side (lambda = D A^-1 D^-1 g) against transposed
+    solves with the same factors, on config 3 (damping, dielectric loss, PML)."""
+    _say(f'[test_paired_adjoint_matches_transposed_solves] {precision}')
+    cell = workloads.config3(mesh_um=0.12, points=4)
+    eng = workloads.build_engine(cell, gradient=True)
+    assert eng.scaled_symmetric()
+    om = 2 * np.pi * cell.frequencies(4)
+    a = eng.solve_batch(om, precision=precision, gradient=True, adjoint='paired')
+    b = eng.solve_batch(om, precision=precision, gradient=True, adjoint='transposed')
+    assert a.get('paired') and not b.get('paired')
+    lam_err = float(((a['lam'] - b['lam']).abs().max() / b['lam'].abs().max()).item())
+    scale = np.abs(cell.params)
+    scale[12:15] = 1.0
+    dy_err = np.abs((a['dY'] - b['dY']) * scale).max() / np.abs(b['dY'] * scale).max()
+    q_err = np.abs(a['Q'] - b['Q']).max() / np.abs(b['Q']).max()
+    _say(precision, 'lambda', lam_err, 'dY', dy_err, 'Q', q_err, 'steps', a['ir_steps'], b['ir_steps'], b['adj_steps'])
+    assert lam_err <= 1e-8 and dy_err <= 1e-9 and q_err <= 1e-12

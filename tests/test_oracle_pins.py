@@ -269,3 +269,20 @@ def test_union_find_chains_collapse_to_first_master():
     rep = dict(zip(s.tolist(), mst.tolist()))
     # 4 joins 3's class; then 0's class (0,1,2) is attached under 4's root 3
     assert rep == {0: 3, 1: 3, 2: 3, 4: 3}
+
+
+def test_scaled_active_system_is_symmetric():
+    """D A_aa is symmetric for D = diag(1 on displacements, 1/SCALE on the potential) (oracle
+    system with damping, dielectric loss, PML and the Bloch merge; ME0 and ME1): the property
+    that turns the adjoint solve into a second right-hand side of the forward solve."""
+    import helpers
+    from oracle.fem import SCALE
+    from sawsim_amd import workloads
+    for cell in (workloads.config3(mesh_um=0.12, points=3), workloads.double_layer()):
+        eng = helpers.oracle_engine(cell, 2 * np.pi * cell.band_hz[1])
+        A, _ = eng.system(2 * np.pi * cell.band_hz[0])
+        A = sp.csr_matrix(A)
+        is_phi = (eng.active // cell.model.N) == cell.ndisp
+        S = sp.diags(np.where(is_phi, 1.0 / SCALE, 1.0)) @ A
+        assert abs(S - S.T).max() <= 1e-14 * abs(S).max()
+        assert abs(A - A.T).max() >= 1e-3 * abs(A).max()      # A itself is not symmetric
