@@ -34,18 +34,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_engine(cell, gradient=True):
+def build_engine(cell, gradient=True, **kw):
     from sawsim_amd.workloads import build_engine as _build
-    return _build(cell, gradient=gradient)
+    return _build(cell, gradient=gradient, **kw)
 
 
 def profile_read(eng):
     import ctypes as C
     from sawsim_amd import _lib
-    ms = (C.c_double * 8)()
-    cnt = (C.c_int64 * 8)()
+    ms = (C.c_double * 9)()
+    cnt = (C.c_int64 * 9)()
     _lib.call('sawfem_mf_profile_read', eng._plan, C.cast(ms, C.c_void_p), C.cast(cnt, C.c_void_p))
-    names = ['init', 'entries', 'diag', 'trsm', 'gemm_l', 'gemm_s', 'fwd', 'bwd']
+    names = ['init', 'entries', 'diag', 'trsm', 'gemm_l', 'gemm_s', 'fwd', 'bwd', 'small']
     return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(names)}
 
 
@@ -84,6 +84,8 @@ def main():
     ap.add_argument('--check-points', type=int, default=16)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-workers', type=int, default=8)
+    ap.add_argument('--leaf-weight', type=int, default=0, help='nested-dissection leaf size (0: default)')
+    ap.add_argument('--small-max', type=int, default=-1, help='largest front for the fused kernel (-1: default)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps(dict(error='this build runs on exactly one GPU; multi-GPU sharding of the '
@@ -97,7 +99,9 @@ def main():
     freqs = cell.frequencies()
     P = freqs.size
     t0 = time.time()
-    eng = build_engine(cell)
+    eng = build_engine(cell, **({'leaf_weight': args.leaf_weight} if args.leaf_weight else {}))
+    if args.small_max >= 0:
+        _lib.call_value('sawfem_mf_set_small', eng._plan, 0, args.small_max)
     torch.cuda.synchronize()
     log(f'[bench] {args.workload}: n={eng.n_active} fronts={eng.mf.nf} setup {time.time() - t0:.1f}s '
         f'(analysis {eng.analysis_seconds:.1f}s)')

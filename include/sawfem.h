@@ -61,6 +61,14 @@ int64_t sawfem_mf_arena_elems(void* plan);   /* complex entries of one frequency
 int64_t sawfem_mf_uarena_elems(void* plan);  /* complex128 entries of one frequency's update vectors */
 int sawfem_mf_perturbed(void* plan, int reset);
 
+/* Fronts of order <= max_m are factored by one fused kernel each (assembly, partial LU and
+ * diagonal-block inverses with the front in LDS) and solved by one workgroup each; the others
+ * by the panel kernels.  Per precision (prec 0: complex64, 1: complex128); max_m <= 0 (the
+ * default) turns the fused path off; requests are clipped to what the device LDS holds (at
+ * most 128).  Returns the threshold in use, or -1.  No counterpart in the reference (a
+ * kernel-selection knob of this backend). */
+int sawfem_mf_set_small(void* plan, int prec, int max_m);
+
 /* Factor A(w_b) = K + i w_b beta D - w_b^2 M for B frequencies (omega: device doubles)
  * into arena (arena_elems * B entries; complex64 if prec == 0, complex128 if prec == 1).
  * Kaa/Maa/Daa: complex128 values on the active pattern (Daa may be NULL).  nb: panel width. */
@@ -79,7 +87,8 @@ int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const v
 
 /* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;
  * used by the benchmark).  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S,
- * 6 fwd, 7 bwd.  profile_read waits for the recorded events and returns totals. */
+ * 6 fwd, 7 bwd, 8 fused small fronts.  profile_read waits for the recorded events and
+ * returns totals in ms[9] / count[9]. */
 int sawfem_mf_profile(void* plan, int enable);
 int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count);
 

@@ -26,6 +26,7 @@ SIGNATURES = {
     'sawfem_mf_arena_elems': [_vp],
     'sawfem_mf_uarena_elems': [_vp],
     'sawfem_mf_perturbed': [_vp, _i],
+    'sawfem_mf_set_small': [_vp, _i, _i],
     'sawfem_mf_factor': [_vp, _vp, _i, _i, _vp, _d, _vp, _vp, _vp, _vp, _i],
     'sawfem_mf_solve': [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i],
     'sawfem_mf_profile': [_vp, _i],
@@ -72,6 +73,16 @@ def call(name, *args):
     if rc != 0:
         msg = lib.sawfem_last_error().decode(errors='replace')
         raise SawfemError(f'{name} failed with status {rc}: {msg}')
+    return rc
+
+
+def call_value(name, *args):
+    """Call an entry point that returns a value (negative on failure); raise on failure."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc < 0:
+        msg = lib.sawfem_last_error().decode(errors='replace')
+        raise SawfemError(f'{name} failed: {msg}')
     return rc
 
 

@@ -85,15 +85,17 @@ __global__ void __launch_bounds__(256) k_front_init(FrontsDev F, const int* __re
   }
 }
 
+// Entries of the fronts of one level that go through the panel path (skip[f] != 0: the front
+// is assembled by k_small_front).
 template <typename T>
 __global__ void k_front_entries(FrontsDev F, int64_t e0, int64_t e1, int B, const double* __restrict__ omega,
                                 double beta, const int* __restrict__ ent_front, const int64_t* __restrict__ ent_slot,
                                 const int* __restrict__ ent_loc, const double2* __restrict__ Kaa,
                                 const double2* __restrict__ Maa, const double2* __restrict__ Daa,
-                                typename cplx<T>::type* __restrict__ arena) {
+                                typename cplx<T>::type* __restrict__ arena, const unsigned char* __restrict__ skip) {
   using V = typename cplx<T>::type;
   const int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= e1) return;
+  if (e >= e1 || skip[ent_front[e]]) return;
   const int b = blockIdx.y;
   const double w = omega[b];
   const int64_t s = ent_slot[e];
@@ -172,6 +174,168 @@ __global__ void __launch_bounds__(128) k_diag(FrontsDev F, const int* __restrict
       V s = (i == j) ? V{1, 0} : czero<V>();
       for (int k = i + 1; k <= j; ++k) s = cfms(s, Ds[i + k * ld], x[k]);
       x[i] = cdiv(s, Ds[i + i * ld]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- small fronts, fused
+// One workgroup of 512 threads per (front, frequency) for fronts whose m x m block fits in
+// LDS (m <= 128): the whole factorization of the front happens there.
+//  1. assembly: zero, extend-add of each child's contribution block (scatter through the
+//     child-CB -> parent-row map rebuilt in LDS, children one after the other), then the
+//     A = K - w^2 M + i w beta D entries (FP64, rounded to T once);
+//  2. partial LU of the p pivots, right-looking with rank-1 updates of the whole trailing
+//     front (the Schur complement included); tiny pivots as in k_diag;
+//  3. write-back of everything but the nbl x nbl diagonal blocks of the pivot triangle;
+//     those are inverted in place (L_D^{-1} strictly below, U_D^{-1} on/above the diagonal,
+//     the storage the panel path and the solves use) and written last.
+constexpr int SMALL_NT = 512;
+constexpr int SMALL_MMAX = 128;
+
+template <typename T>
+__global__ void __launch_bounds__(SMALL_NT) k_small_front(
+    FrontsDev F, const int* __restrict__ lvl, int B, const double* __restrict__ omega, double beta,
+    const int64_t* __restrict__ fent_lo, const int64_t* __restrict__ fent_hi, const int64_t* __restrict__ ent_slot,
+    const int* __restrict__ ent_loc, const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
+    const double2* __restrict__ Daa, typename cplx<T>::type* __restrict__ arena, int nbl, T tiny,
+    int* __restrict__ npert) {
+  using V = typename cplx<T>::type;
+  extern __shared__ unsigned char smem_raw[];
+  const int f = lvl[blockIdx.x];
+  const int b = blockIdx.y;
+  const int m = F.m[f], p = F.p[f];
+  const int ld = m + 1;
+  V* S = reinterpret_cast<V*>(smem_raw);                                // [m][ld] column-major
+  int* rel = reinterpret_cast<int*>(smem_raw + sizeof(V) * (size_t)m * ld);   // [m]
+  const int t = threadIdx.x;
+  for (int idx = t; idx < m * ld; idx += SMALL_NT) S[idx] = czero<V>();
+  __syncthreads();
+  // 1. assembly
+  for (int slot = F.child_ptr[f]; slot < F.child_ptr[f + 1]; ++slot) {
+    const int ch = F.child_ids[slot];
+    const int mc = F.m[ch], pc = F.p[ch], qc = mc - pc;
+    const int* inv = F.inv + F.inv_ptr[slot];
+    for (int i = t; i < m; i += SMALL_NT) {
+      const int ci = inv[i];
+      if (ci >= 0) rel[ci] = i;
+    }
+    __syncthreads();
+    const V* Fc = front_ptr<T>(arena, F, ch, b, B) + pc + (int64_t)pc * mc;
+    for (int idx = t; idx < qc * qc; idx += SMALL_NT) {
+      const int a = idx % qc, c = idx / qc;
+      const V w = Fc[a + (int64_t)c * mc];
+      V* d = S + rel[a] + rel[c] * ld;
+      d->x += w.x;
+      d->y += w.y;
+    }
+    __syncthreads();
+  }
+  {
+    const double w = omega[b];
+    for (int64_t e = fent_lo[f] + t; e < fent_hi[f]; e += SMALL_NT) {
+      const int64_t sl = ent_slot[e];
+      const double2 k = Kaa[sl], mm = Maa[sl];
+      double ax = k.x - w * w * mm.x, ay = k.y - w * w * mm.y;
+      if (Daa) {
+        const double2 d = Daa[sl];
+        ax -= w * beta * d.y;
+        ay += w * beta * d.x;
+      }
+      const int loc = ent_loc[e];
+      V* d = S + (loc % m) + (loc / m) * ld;
+      d->x += (T)ax;
+      d->y += (T)ay;
+    }
+  }
+  __syncthreads();
+  // 2. partial LU: wave w updates columns j = w (mod 8) four at a time (all loads issued before
+  //    the stores), lane owns rows k+1+lane and k+65+lane
+  const int lane = t & 63, wave = t >> 6;
+  constexpr int NW = SMALL_NT / 64;
+  for (int k = 0; k < p; ++k) {
+    V piv = S[k + k * ld];
+    const T mag = piv.x * piv.x + piv.y * piv.y;
+    if (!(mag > tiny * tiny)) {
+      piv.x = tiny; piv.y = 0;
+      if (t == 0) { S[k + k * ld] = piv; atomicAdd(npert, 1); }
+    }
+    const V one{1, 0};
+    const V ipiv = cdiv(one, piv);
+    for (int i = k + 1 + t; i < m; i += SMALL_NT) S[i + k * ld] = cmul(S[i + k * ld], ipiv);
+    __syncthreads();
+    const int i0 = k + 1 + lane, i1 = i0 + 64;
+    const bool r0 = i0 < m, r1 = i1 < m;
+    const V l0 = r0 ? S[i0 + k * ld] : czero<V>();
+    const V l1 = r1 ? S[i1 + k * ld] : czero<V>();
+    for (int j = k + 1 + wave; j < m; j += 4 * NW) {
+      V u[4], a0[4], a1[4];
+      int jj[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        jj[c] = j + c * NW;
+        const bool ok = jj[c] < m;
+        u[c] = ok ? S[k + jj[c] * ld] : czero<V>();
+        a0[c] = (ok && r0) ? S[i0 + jj[c] * ld] : czero<V>();
+        a1[c] = (ok && r1) ? S[i1 + jj[c] * ld] : czero<V>();
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (jj[c] < m) {
+          if (r0) S[i0 + jj[c] * ld] = cfms(a0[c], l0, u[c]);
+          if (r1) S[i1 + jj[c] * ld] = cfms(a1[c], l1, u[c]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // 3. write-back (diagonal blocks of the pivot triangle last)
+  V* G = front_ptr<T>(arena, F, f, b, B);
+  for (int idx = t; idx < m * m; idx += SMALL_NT) {
+    const int i = idx % m, j = idx / m;
+    if (!(i < p && j < p && i / nbl == j / nbl)) G[idx] = S[i + j * ld];
+  }
+  // In-place inversion of each diagonal block by row operations (Gauss-Jordan), thread per
+  // row, independent updates inside a step:
+  //   L^{-1} (threads < 256): step k, rows i > k:  X[i][j] -= L[i][k] X[k][j] (j < k), X[i][k] = -L[i][k]
+  //   U^{-1} (threads >= 256): step k = n-1..0: row k scaled by d = 1/U[k][k] (phase a), then
+  //     rows i < k: Y[i][j] -= U[i][k] Y[k][j] (j > k), Y[i][k] = -U[i][k] d (phase b)
+  for (int J0 = 0; J0 < p; J0 += nbl) {
+    const int nbk = min(nbl, p - J0);
+    V* D = S + J0 + J0 * ld;
+    for (int st = 0; st < nbk; ++st) {
+      // phase a: U row scaling (row k = nbk-1-st)
+      if (t >= SMALL_NT / 2) {
+        const int k = nbk - 1 - st, i = t - SMALL_NT / 2;
+        if (i == k) {
+          const V one{1, 0};
+          const V d = cdiv(one, D[k + k * ld]);
+          for (int j = k + 1; j < nbk; ++j) D[k + j * ld] = cmul(D[k + j * ld], d);
+          D[k + k * ld] = d;
+        }
+      }
+      __syncthreads();
+      if (t < SMALL_NT / 2) {
+        const int k = st, i = t;
+        if (i > k && i < nbk) {
+          const V mk = D[i + k * ld];
+          for (int j = 0; j < k; ++j) D[i + j * ld] = cfms(D[i + j * ld], mk, D[k + j * ld]);
+          D[i + k * ld] = V{-mk.x, -mk.y};
+        }
+      } else {
+        const int k = nbk - 1 - st, i = t - SMALL_NT / 2;
+        if (i < k) {
+          const V mk = D[i + k * ld];
+          const V d = D[k + k * ld];
+          for (int j = k + 1; j < nbk; ++j) D[i + j * ld] = cfms(D[i + j * ld], mk, D[k + j * ld]);
+          const V y = cmul(mk, d);
+          D[i + k * ld] = V{-y.x, -y.y};
+        }
+      }
+      __syncthreads();
+    }
+    for (int idx = t; idx < nbk * nbk; idx += SMALL_NT) {
+      const int i = idx % nbk, j = idx / nbk;
+      G[(int64_t)(J0 + i) + (int64_t)(J0 + j) * m] = D[i + j * ld];
     }
   }
 }
@@ -362,8 +526,8 @@ template <> struct SolveTile<double> { static constexpr int SB = 32; };   // wid
 __device__ __forceinline__ double2 d2add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 
 // w[r][i] (i < p) = x_r[rows[i]] + children's update entries; uf_r[i - p] (i >= p) = children's
-// entries.  w is [R][p] in LDS.
-template <int R>
+// entries.  w is [R][p] in LDS, or [R][m] holding the CB rows too when ALL is set.
+template <int R, bool ALL>
 __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, int B, int n, int m, int p,
                                              const int* rows, const double2* x, double2* uarena, double2* w) {
   const int nch = F.child_ptr[f + 1] - F.child_ptr[f];
@@ -383,7 +547,10 @@ __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, i
         for (int r = 0; r < R; ++r) v[r] = d2add(v[r], u[(int64_t)r * qc]);
       }
     }
-    if (i < p) {
+    if (ALL) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) w[r * m + i] = v[r];
+    } else if (i < p) {
 #pragma unroll
       for (int r = 0; r < R; ++r) w[r * p + i] = v[r];
     } else {
@@ -398,8 +565,11 @@ __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, i
 //                  2: U11^T (lower, forward)            3: L11^T (unit upper, backward)
 // Forward kinds gather the right-hand sides and the children's updates first.  nbl is the
 // panel width the front was factored with (the diagonal blocks hold L_D^{-1} below and
-// U_D^{-1} on/above the diagonal).
-template <typename T, int KIND, int R>
+// U_D^{-1} on/above the diagonal).  With FULL the kernel also does the front's off-diagonal
+// panel (the k_panel work) so one workgroup handles the whole front (small fronts): the
+// vectors then cover all m rows, the forward column/row-panel updates run down to row m and
+// the backward kinds start with the CB-to-pivot product.
+template <typename T, int KIND, int R, bool FULL>
 __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
                                              const typename cplx<T>::type* __restrict__ arena,
                                              double2* __restrict__ uarena, double2* __restrict__ x) {
@@ -409,22 +579,70 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
   extern __shared__ unsigned char smem_raw[];
   V* tile = reinterpret_cast<V*>(smem_raw);                                   // [64][LD]
   double2* part = reinterpret_cast<double2*>(smem_raw + sizeof(V) * 64 * LD);  // [R][4][64]
-  double2* w = part + R * 256;                                                // [R][p]
+  double2* w = part + R * 256;                                                // [R][W]
   const int f = lvl[blockIdx.x];
   const int b = blockIdx.y;
   const int m = F.m[f], p = F.p[f];
+  const int W = FULL ? m : p;
   const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
   const int t = threadIdx.x;
   constexpr bool fwd = (KIND == 0 || KIND == 2);
   if (fwd) {
-    solve_gather<R>(F, f, b, B, n, m, p, rows, x, uarena, w);
+    solve_gather<R, FULL>(F, f, b, B, n, m, p, rows, x, uarena, w);
   } else {
-    for (int i = t; i < p; i += blockDim.x)
+    for (int i = t; i < W; i += blockDim.x)
 #pragma unroll
-      for (int r = 0; r < R; ++r) w[r * p + i] = x[((int64_t)b * R + r) * n + rows[i]];
+      for (int r = 0; r < R; ++r) w[r * W + i] = x[((int64_t)b * R + r) * n + rows[i]];
   }
   __syncthreads();
+  if (FULL && !fwd && p < m) {
+    // CB -> pivot rows: kind 1: w_i -= sum_j U[i][p + j] w_{p+j};  kind 3: w_k -= sum_j L[p + j][k] w_{p+j}
+    const int q = m - p;
+    if (KIND == 1) {
+      for (int i = t; i < p; i += blockDim.x) {
+        double2 s[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[r] = w[r * W + i];
+        for (int j = 0; j < q; ++j) {
+          const double2 a = to_d2(Fb[(int64_t)i + (int64_t)(p + j) * m]);
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * W + p + j]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) w[r * W + i] = s[r];
+      }
+    } else {
+      for (int c0 = 0; c0 < p; c0 += 64) {
+        const int nc = min(64, p - c0);
+        double2 s[R];
+        if (t < nc) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[r] = w[r * W + c0 + t];
+        }
+        for (int j0 = 0; j0 < q; j0 += SB) {
+          const int nj = min(SB, q - j0);
+          __syncthreads();
+          for (int idx = t; idx < SB * 64; idx += blockDim.x) {
+            const int jj = idx % SB, c = idx / SB;
+            if (jj < nj && c < nc) tile[c * LD + jj] = Fb[(int64_t)(p + j0 + jj) + (int64_t)(c0 + c) * m];
+          }
+          __syncthreads();
+          if (t < nc)
+            for (int jj = 0; jj < nj; ++jj) {
+              const double2 a = to_d2(tile[t * LD + jj]);
+#pragma unroll
+              for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * W + p + j0 + jj]);
+            }
+        }
+        if (t < nc) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) w[r * W + c0 + t] = s[r];
+        }
+      }
+    }
+    __syncthreads();
+  }
   const int nblk = (p + nbl - 1) / nbl;
   for (int bi = 0; bi < nblk; ++bi) {
     const int J0 = fwd ? bi * nbl : (nblk - 1 - bi) * nbl;
@@ -445,25 +663,25 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
           for (int k = g; k < i; k += 4) {
             const double2 a = to_d2(tile[i + k * LD]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * p + J0 + k]);
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * W + J0 + k]);
           }
         } else if (KIND == 1) {   // U_D^{-1}: sum_{k>=i} Ui[i][k] w_k
           for (int k = i + g; k < nbk; k += 4) {
             const double2 a = to_d2(tile[i + k * LD]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * p + J0 + k]);
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * W + J0 + k]);
           }
         } else if (KIND == 2) {   // U_D^{-T}: sum_{k<=i} Ui[k][i] w_k
           for (int k = g; k <= i; k += 4) {
             const double2 a = to_d2(tile[k + i * LD]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * p + J0 + k]);
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * W + J0 + k]);
           }
         } else {                  // L_D^{-T} (unit upper): w_i + sum_{k>i} Li[k][i] w_k
           for (int k = i + 1 + g; k < nbk; k += 4) {
             const double2 a = to_d2(tile[k + i * LD]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * p + J0 + k]);
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * W + J0 + k]);
           }
         }
       }
@@ -476,26 +694,27 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
       for (int r = 0; r < R; ++r) {
         double2 tot = part[r * 256 + t];
         for (int g = 1; g < 4; ++g) tot = d2add(tot, part[r * 256 + g * 64 + t]);
-        if (KIND == 0 || KIND == 3) tot = d2add(tot, w[r * p + J0 + t]);
-        w[r * p + J0 + t] = tot;
+        if (KIND == 0 || KIND == 3) tot = d2add(tot, w[r * W + J0 + t]);
+        w[r * W + J0 + t] = tot;
       }
     }
     __syncthreads();
-    // remaining pivot rows: forward kinds update [J0 + nbk, p), backward kinds [0, J0)
-    const int lo = fwd ? J0 + nbk : 0, hi = fwd ? p : J0;
+    // remaining rows: forward kinds update [J0 + nbk, W) (W = m with FULL: the CB rows too),
+    // backward kinds [0, J0)
+    const int lo = fwd ? J0 + nbk : 0, hi = fwd ? W : J0;
     if (KIND == 0 || KIND == 1) {
       // column panel F[lo:hi, J0:J0+nbk] (contiguous rows): thread per row
       for (int i = lo + t; i < hi; i += blockDim.x) {
         double2 s[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) s[r] = w[r * p + i];
+        for (int r = 0; r < R; ++r) s[r] = w[r * W + i];
         for (int k = 0; k < nbk; ++k) {
           const double2 a = to_d2(Fb[(int64_t)i + (int64_t)(J0 + k) * m]);
 #pragma unroll
-          for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * p + J0 + k]);
+          for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * W + J0 + k]);
         }
 #pragma unroll
-        for (int r = 0; r < R; ++r) w[r * p + i] = s[r];
+        for (int r = 0; r < R; ++r) w[r * W + i] = s[r];
       }
     } else {
       // row panel F[J0:J0+nbk, lo:hi] read as its transpose through a 64-column LDS tile
@@ -510,22 +729,29 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
         if (t < nc) {
           double2 s[R];
 #pragma unroll
-          for (int r = 0; r < R; ++r) s[r] = w[r * p + c0 + t];
+          for (int r = 0; r < R; ++r) s[r] = w[r * W + c0 + t];
           for (int k = 0; k < nbk; ++k) {
             const double2 a = to_d2(tile[t * LD + k]);
 #pragma unroll
-            for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * p + J0 + k]);
+            for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * W + J0 + k]);
           }
 #pragma unroll
-          for (int r = 0; r < R; ++r) w[r * p + c0 + t] = s[r];
+          for (int r = 0; r < R; ++r) w[r * W + c0 + t] = s[r];
         }
       }
     }
     __syncthreads();
   }
-  for (int i = t; i < p; i += blockDim.x)
+  for (int i = t; i < W; i += blockDim.x) {
+    if (i < p) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) x[((int64_t)b * R + r) * n + rows[i]] = w[r * p + i];
+      for (int r = 0; r < R; ++r) x[((int64_t)b * R + r) * n + rows[i]] = w[r * W + i];
+    } else if (fwd) {
+      double2* uf = uarena + F.uoff[f] * B * R + (int64_t)b * R * (m - p) + (i - p);
+#pragma unroll
+      for (int r = 0; r < R; ++r) uf[(int64_t)r * (m - p)] = w[r * W + i];
+    }
+  }
 }
 
 // Off-diagonal panels, one workgroup per 64-wide output tile of one front and frequency.
@@ -736,14 +962,21 @@ __global__ void k_axpy(int64_t n, double2* __restrict__ x, const double2* __rest
 }
 
 // ---------------------------------------------------------------- plan
+// Fronts of one level that take one factorization path.
+struct Sub {
+  int nf = 0, mmax = 0, pmax = 0, cbmax = 0;
+  const int* fronts = nullptr;   // device list
+};
+
 struct Level {
-  int nf, mmax, pmax, cbmax;
+  int nf, mmax, pmax, cbmax;     // all fronts of the level (solves, panel width)
   int64_t e0, e1;
-  const int* fronts;   // device pointer into lvl_fronts
+  const int* fronts;             // device pointer into lvl_fronts
+  Sub small[2], big[2];          // per precision (0 complex64, 1 complex128): fused / panel path
 };
 
 // Optional per-kernel timing with HIP events on the launch stream (kind ids below).
-enum Kind { K_INIT = 0, K_ENTRIES, K_DIAG, K_TRSM, K_GEMM_L, K_GEMM_S, K_FWD, K_BWD, K_NKIND };
+enum Kind { K_INIT = 0, K_ENTRIES, K_DIAG, K_TRSM, K_GEMM_L, K_GEMM_S, K_FWD, K_BWD, K_SMALL, K_NKIND };
 
 struct Plan {
   int n = 0, nf = 0;
@@ -762,6 +995,15 @@ struct Plan {
   int* npert = nullptr;
   int mmax = 0;
   int pmax = 0;
+  // fused small-front path: per-front entry ranges, per-precision front flags and the
+  // largest front order it takes (0: off); host copies to rebuild the level partitions
+  int64_t* fent_lo = nullptr;
+  int64_t* fent_hi = nullptr;
+  unsigned char* fused[2] = {nullptr, nullptr};
+  int small_limit[2] = {0, 0};   // device LDS bound per precision
+  int small_mmax[2] = {0, 0};    // threshold in use
+  std::vector<void*> part_owned[2];
+  std::vector<int> hm, hp, hlvl_ptr, hlvl_fronts;
 };
 
 // Record a start/stop event pair around one launch when profiling is on.
@@ -800,43 +1042,101 @@ int upload(Plan* P, X** dst, const X* src, int64_t count) {
 // solves must agree on it (the diagonal blocks hold inverses of nbl x nbl triangles).
 int panel_width(int nb, const Level& L) { return (nb == 64 && L.pmax <= 128) ? 32 : nb; }
 
+// Split every level into the fronts the fused small-front kernel takes (m <= max_m) and the
+// rest, for precision index pi; max_m <= 0 sends every front through the panel path.
+int build_partition(Plan* P, int pi, int max_m) {
+  for (void* d : P->part_owned[pi]) (void)hipFree(d);
+  P->part_owned[pi].clear();
+  P->small_mmax[pi] = std::max(0, std::min(max_m, P->small_limit[pi]));
+  const int nf = P->nf;
+  std::vector<unsigned char> flag(nf, 0);
+  auto put = [&](const std::vector<int>& v, const int** dst) -> int {
+    *dst = nullptr;
+    if (v.empty()) return 0;
+    void* d = nullptr;
+    SAW_HIP(hipMalloc(&d, sizeof(int) * v.size()));
+    SAW_HIP(hipMemcpy(d, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
+    P->part_owned[pi].push_back(d);
+    *dst = (const int*)d;
+    return 0;
+  };
+  for (size_t l = 0; l < P->levels.size(); ++l) {
+    Level& L = P->levels[l];
+    std::vector<int> sm, bg;
+    for (int k = P->hlvl_ptr[l]; k < P->hlvl_ptr[l + 1]; ++k) {
+      const int f = P->hlvl_fronts[k];
+      (P->hm[f] <= P->small_mmax[pi] ? sm : bg).push_back(f);
+    }
+    auto stats = [&](const std::vector<int>& v, Sub& S) {
+      S = Sub{};
+      S.nf = (int)v.size();
+      for (int f : v) {
+        S.mmax = std::max(S.mmax, P->hm[f]);
+        S.pmax = std::max(S.pmax, P->hp[f]);
+        S.cbmax = std::max(S.cbmax, P->hm[f] - P->hp[f]);
+      }
+    };
+    stats(sm, L.small[pi]);
+    stats(bg, L.big[pi]);
+    if (put(sm, &L.small[pi].fronts) || put(bg, &L.big[pi].fronts)) return 3;
+    for (int f : sm) flag[f] = 1;
+  }
+  void* d = nullptr;
+  SAW_HIP(hipMalloc(&d, nf));
+  SAW_HIP(hipMemcpy(d, flag.data(), nf, hipMemcpyHostToDevice));
+  P->part_owned[pi].push_back(d);
+  P->fused[pi] = (unsigned char*)d;
+  return 0;
+}
+
 template <typename T>
 int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta, const void* Kaa, const void* Maa,
                 const void* Daa, void* arena_v, int nb) {
   using V = typename cplx<T>::type;
   V* arena = (V*)arena_v;
   const T tiny = sizeof(T) == 4 ? (T)1e-30 : (T)1e-300;
+  const int pi = sizeof(T) == 4 ? 0 : 1;
   for (const Level& L : P->levels) {
+    const int nbl = panel_width(nb, L);
+    const Sub& S = L.small[pi];
+    if (S.nf > 0) {
+      const size_t lds = sizeof(V) * S.mmax * (S.mmax + 1) + sizeof(int) * S.mmax;
+      Timed tm(P, s, K_SMALL);
+      hipLaunchKernelGGL(k_small_front<T>, dim3(S.nf, B), dim3(SMALL_NT), lds, s, P->F, S.fronts, B, omega, beta,
+                         P->fent_lo, P->fent_hi, P->ent_slot, P->ent_loc, (const double2*)Kaa, (const double2*)Maa,
+                         (const double2*)Daa, arena, nbl, tiny, P->npert);
+    }
+    const Sub& G = L.big[pi];
+    if (G.nf == 0) continue;
     const int cpw = 8;
     { Timed tm(P, s, K_INIT);
-    hipLaunchKernelGGL(k_front_init<T>, dim3((L.mmax + cpw - 1) / cpw, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B,
+    hipLaunchKernelGGL(k_front_init<T>, dim3((G.mmax + cpw - 1) / cpw, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
                        arena, cpw); }
     if (L.e1 > L.e0) { Timed tm(P, s, K_ENTRIES);
       hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), B), dim3(256), 0, s, P->F,
                          L.e0, L.e1, B, omega, beta, P->ent_front, P->ent_slot, P->ent_loc, (const double2*)Kaa,
-                         (const double2*)Maa, (const double2*)Daa, arena); }
-    const int nbl = panel_width(nb, L);
-    for (int k0 = 0; k0 < L.pmax; k0 += nbl) {
+                         (const double2*)Maa, (const double2*)Daa, arena, P->fused[pi]); }
+    for (int k0 = 0; k0 < G.pmax; k0 += nbl) {
       const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
       { Timed tm(P, s, K_DIAG);
-      hipLaunchKernelGGL(k_diag<T>, dim3(L.nf, B), dim3(128), lds_diag, s, P->F, L.fronts, B, arena, k0, nbl, tiny,
+      hipLaunchKernelGGL(k_diag<T>, dim3(G.nf, B), dim3(128), lds_diag, s, P->F, G.fronts, B, arena, k0, nbl, tiny,
                          P->npert); }
-      const int rem = L.mmax - k0 - 1;
+      const int rem = G.mmax - k0 - 1;
       if (rem > 0) {
         const int nt = (rem + TILE - 1) / TILE;
         { Timed tm(P, s, K_TRSM);
-        hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, k0, nbl);
-        hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, k0, nbl); }
-        if (k0 + nbl < L.pmax) {
+        hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl);
+        hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl); }
+        if (k0 + nbl < G.pmax) {
           Timed tm(P, s, K_GEMM_L);
-          hipLaunchKernelGGL((k_gemm<T, 0>), dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, k0, nbl);
+          hipLaunchKernelGGL((k_gemm<T, 0>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl);
         }
       }
     }
-    if (L.cbmax > 0) {
-      const int nt = (L.cbmax + TILE - 1) / TILE;
+    if (G.cbmax > 0) {
+      const int nt = (G.cbmax + TILE - 1) / TILE;
       Timed tm(P, s, K_GEMM_S);
-      hipLaunchKernelGGL((k_gemm<T, 1>), dim3(nt * nt, L.nf, B), dim3(256), 0, s, P->F, L.fronts, B, arena, 0, nb);
+      hipLaunchKernelGGL((k_gemm<T, 1>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, 0, nb);
     }
   }
   SAW_HIP(hipGetLastError());
@@ -847,44 +1147,56 @@ template <typename T, int R>
 int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose, int nb) {
   using V = typename cplx<T>::type;
   constexpr int LD = SolveTile<T>::SB + 1;
-  const size_t lds_tri = sizeof(V) * 64 * LD + sizeof(double2) * R * (256 + P->pmax);
+  const int pi = sizeof(T) == 4 ? 0 : 1;
   const V* A = (const V*)arena;
   double2* U = (double2*)uarena;
   double2* X = (double2*)x;
-  auto tri = [&](const Level& L, int kind) {
+  // fronts of the fused set (m <= the small-front threshold): one workgroup does the whole
+  // front; the others: pivot-triangle kernel + panel kernel
+  auto tri = [&](const Level& L, int kind, bool full) {
+    const Sub& S = full ? L.small[pi] : L.big[pi];
+    if (S.nf == 0) return;
     Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
     const int nbl = panel_width(nb, L);
-    const dim3 grid(L.nf, B);
-    switch (kind) {
-      case 0: hipLaunchKernelGGL((k_tri<T, 0, R>), grid, dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
-      case 1: hipLaunchKernelGGL((k_tri<T, 1, R>), grid, dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
-      case 2: hipLaunchKernelGGL((k_tri<T, 2, R>), grid, dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X); break;
-      default: hipLaunchKernelGGL((k_tri<T, 3, R>), grid, dim3(256), lds_tri, s, P->F, L.fronts, B, P->n, nbl, A, U, X);
+    const size_t lds = sizeof(V) * 64 * LD + sizeof(double2) * R * (256 + (full ? S.mmax : S.pmax));
+    const dim3 grid(S.nf, B);
+#define SAW_TRI(K, FULL) hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, nbl, A, U, X)
+    if (full) {
+      switch (kind) { case 0: SAW_TRI(0, true); break; case 1: SAW_TRI(1, true); break;
+                      case 2: SAW_TRI(2, true); break; default: SAW_TRI(3, true); }
+    } else {
+      switch (kind) { case 0: SAW_TRI(0, false); break; case 1: SAW_TRI(1, false); break;
+                      case 2: SAW_TRI(2, false); break; default: SAW_TRI(3, false); }
     }
+#undef SAW_TRI
   };
   auto panel = [&](const Level& L, int kind) {
+    const Sub& S = L.big[pi];
     const bool out_cb = (kind == 0 || kind == 2);
-    const int nout = out_cb ? L.cbmax : L.pmax;
-    const int nin = out_cb ? L.pmax : L.cbmax;
-    if (nout == 0 || nin == 0) return;
+    const int nout = out_cb ? S.cbmax : S.pmax;
+    const int nin = out_cb ? S.pmax : S.cbmax;
+    if (S.nf == 0 || nout == 0 || nin == 0) return;
     const size_t lds = sizeof(double2) * R * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
     Timed tm(P, s, out_cb ? K_FWD : K_BWD);
-    dim3 grid((nout + 63) / 64, L.nf, B);
+    dim3 grid((nout + 63) / 64, S.nf, B);
     switch (kind) {
-      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
-      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
-      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X); break;
-      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, L.fronts, B, P->n, A, U, X);
+      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X); break;
+      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X); break;
+      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X); break;
+      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X);
     }
   };
   const int kf = transpose ? 2 : 0, kb = transpose ? 3 : 1;
   for (const Level& L : P->levels) {
-    tri(L, kf);
+    tri(L, kf, true);
+    tri(L, kf, false);
     panel(L, kf);
   }
   for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
-    panel(P->levels[l], kb);
-    tri(P->levels[l], kb);
+    const Level& L = P->levels[l];
+    panel(L, kb);
+    tri(L, kb, false);
+    tri(L, kb, true);
   }
   SAW_HIP(hipGetLastError());
   return 0;
@@ -893,6 +1205,8 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
 }  // namespace
 
 extern "C" {
+
+int sawfem_mf_destroy(void* plan);
 
 // Build a numeric plan from the host symbolic analysis.  All arrays are host pointers.
 //   m, p, rows_ptr(nf+1), rows: front orders, pivot counts and row lists (active ids)
@@ -946,6 +1260,53 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
   int zero = 0;
   rc |= upload(P, &P->npert, &zero, 1);
   if (rc) { delete P; return 3; }
+  // per-front entry ranges (entries are grouped by front within each level)
+  {
+    std::vector<int64_t> lo(nf, 0), hi(nf, 0);
+    for (int64_t e = nent - 1; e >= 0; --e) lo[ent_front[e]] = e;
+    for (int64_t e = 0; e < nent; ++e) hi[ent_front[e]] = e + 1;
+    std::vector<char> seen(nf, 0);
+    for (int64_t e = 0; e < nent; ++e) seen[ent_front[e]] = 1;
+    for (int f = 0; f < nf; ++f)
+      if (!seen[f]) lo[f] = hi[f] = 0;
+    for (int64_t e = 0; e < nent; ++e)
+      if (e < lo[ent_front[e]] || e >= hi[ent_front[e]] || (e > lo[ent_front[e]] && ent_front[e - 1] != ent_front[e])) {
+        delete P;
+        sawfem_set_error("sawfem_mf_create: entries of a front are not contiguous");
+        return 1;
+      }
+    rc |= upload(P, &P->fent_lo, lo.data(), nf);
+    rc |= upload(P, &P->fent_hi, hi.data(), nf);
+    if (rc) { delete P; return 3; }
+  }
+  P->hm.assign(m, m + nf);
+  P->hp.assign(p, p + nf);
+  P->hlvl_ptr.assign(lvl_ptr, lvl_ptr + nlev + 1);
+  P->hlvl_fronts.assign(lvl_fronts, lvl_fronts + lvl_ptr[nlev]);
+  {
+    int dev = 0, lds = 64 * 1024;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      lds = 64 * 1024;
+    for (int pi = 0; pi < 2; ++pi) {
+      const size_t sz = pi == 0 ? sizeof(c64) : sizeof(c128);
+      int mm = 0;
+      while (mm < SMALL_MMAX && sz * (size_t)(mm + 1) * (mm + 2) + sizeof(int) * (mm + 1) <= (size_t)lds) ++mm;
+      P->small_limit[pi] = mm;
+    }
+    const void* fk[2] = {(const void*)k_small_front<float>, (const void*)k_small_front<double>};
+    for (int pi = 0; pi < 2; ++pi) {
+      const size_t need = (pi == 0 ? sizeof(c64) : sizeof(c128)) * (size_t)P->small_limit[pi] * (P->small_limit[pi] + 1)
+                          + sizeof(int) * P->small_limit[pi];
+      if (need > 64 * 1024 && hipFuncSetAttribute(fk[pi], hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) != hipSuccess) {
+        (void)hipGetLastError();
+        int mm = 0;
+        const size_t sz = pi == 0 ? sizeof(c64) : sizeof(c128);
+        while (sz * (size_t)(mm + 1) * (mm + 2) + sizeof(int) * (mm + 1) <= 64 * 1024) ++mm;
+        P->small_limit[pi] = mm;
+      }
+    }
+  }
   P->F = FrontsDev{dm, dp, doff, duoff, drp, drows, dcp, dci, dip, dinv};
   for (int l = 0; l < nlev; ++l) {
     Level L{};
@@ -961,6 +1322,10 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
     L.e1 = ent_lvl_ptr[l + 1];
     P->levels.push_back(L);
   }
+  // the fused small-front path is opt-in (sawfem_mf_set_small): on the config-2 sweep it
+  // measured slower than the panel kernels (one 512-thread workgroup per CU, latency-bound)
+  for (int pi = 0; pi < 2; ++pi)
+    if (build_partition(P, pi, 0)) { sawfem_mf_destroy(P); return 3; }
   *out = P;
   return 0;
 }
@@ -969,9 +1334,21 @@ int sawfem_mf_destroy(void* plan) {
   Plan* P = (Plan*)plan;
   if (!P) return 0;
   for (void* d : P->owned) (void)hipFree(d);
+  for (int pi = 0; pi < 2; ++pi)
+    for (void* d : P->part_owned[pi]) (void)hipFree(d);
   for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
   delete P;
   return 0;
+}
+
+// Largest front order factored by the fused small-front kernel (prec 0: complex64, 1:
+// complex128); max_m <= 0 (the default) sends every front through the panel path.  The value
+// in use (the request clipped to what the device LDS holds, at most 128) is returned, or -1.
+int sawfem_mf_set_small(void* plan, int prec, int max_m) {
+  Plan* P = (Plan*)plan;
+  if (!P || (prec != 0 && prec != 1)) { sawfem_set_error("sawfem_mf_set_small: invalid arguments"); return -1; }
+  if (build_partition(P, prec, max_m)) return -1;
+  return P->small_mmax[prec];
 }
 
 int64_t sawfem_mf_arena_elems(void* plan) { return plan ? ((Plan*)plan)->arena_elems : -1; }
@@ -1038,7 +1415,8 @@ int sawfem_mf_profile(void* plan, int enable) {
 }
 
 // Waits for the recorded launches and writes, per kind (init, entries, diag, trsm, gemm-L,
-// gemm-S, fwd, bwd), total milliseconds to ms[kind] and launch counts to count[kind].
+// gemm-S, fwd, bwd, small), total milliseconds to ms[kind] and launch counts to count[kind]
+// (arrays of 9).
 int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count) {
   Plan* P = (Plan*)plan;
   if (!P || !ms || !count) { sawfem_set_error("sawfem_mf_profile_read: invalid arguments"); return 1; }

@@ -91,9 +91,13 @@ def test_bloch_maps_and_active_matrix(name):
     assert abs(Qg - Qo) <= 1e-12 * abs(Qo) + 1e-30
 
 
+@pytest.mark.parametrize('path', ['fused', 'panel'])
 @pytest.mark.parametrize('prec,nb', [('fp64', 32), ('mixed', 32), ('mixed', 64)])
-def test_factor_arena_matches_emulation(prec, nb):
-    _say(f'[test_factor_arena_matches_emulation] {prec} nb={nb}')
+def test_factor_arena_matches_emulation(prec, nb, path):
+    """Every front of the GPU factorization against the NumPy emulation of the kernels'
+    data flow; 'fused': fronts up to the LDS limit go through the fused small-front kernel,
+    'panel': every front through the panel kernels."""
+    _say(f'[test_factor_arena_matches_emulation] {prec} nb={nb} {path}')
     import torch
     cell = workloads.double_layer()
     wtop = 2 * np.pi * cell.band_hz[1]
@@ -103,6 +107,10 @@ def test_factor_arena_matches_emulation(prec, nb):
     from sawsim_amd import _lib
     from sawsim_amd.piezo_fem import stream_ptr
     pr = 0 if prec == 'mixed' else 1
+    lim = _lib.call_value('sawfem_mf_set_small', eng._plan, pr, 128 if path == 'fused' else 0)
+    nsmall = int((eng.mf.m <= lim).sum())
+    _say(f'fused threshold {lim}: {nsmall} of {eng.mf.nf} fronts')
+    assert (nsmall > 0 and nsmall < eng.mf.nf) if path == 'fused' else nsmall == 0
     arena = torch.empty(eng.mf.arena_elems * B, dtype=torch.complex64 if pr == 0 else torch.complex128, device='cuda')
     om = torch.as_tensor(omegas, device='cuda')
     _lib.call('sawfem_mf_factor', eng._plan, stream_ptr(), pr, B, om.data_ptr(), 0.0, eng.Kaa.data_ptr(),
