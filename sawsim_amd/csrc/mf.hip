@@ -413,10 +413,15 @@ template <> struct WaveAcc<double> {
 // mode 1: Schur complement rows/cols [p, m), K = [0, p)
 // mode 2: L panel  F[k1:m, k0:k1] = F[k1:m, k0:k1] * Uinv      (in place, overwrite)
 // mode 3: U panel  F[k0:k1, k1:m] = Linv * F[k0:k1, k1:m]      (in place, overwrite)
+// mode 4: left-looking panel update before the diagonal LU of panel [k0, k1), K = [0, k0):
+//         column panel F[k0:m, k0:k1] -= F[k0:m, 0:k0] F[0:k0, k0:k1]  (x tiles 0 .. nc-1)
+//         row panel    F[k0:k1, k1:m] -= F[k0:k1, 0:k0] F[0:k0, k1:m]  (x tiles nc ..)
+//         with nc = ceil((mmax - k0) / TILE) from the launch (uniform over the level)
 // The inverses are the ones k_diag stored in the diagonal block.
 template <typename T, int mode>
 __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
-                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb) {
+                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb,
+                                              int nc = 0) {
   using V = typename cplx<T>::type;
   __shared__ V As[KC * TILE];
   __shared__ V Bs[TILE * (KC + 1)];
@@ -426,9 +431,17 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
   const int m = F.m[f];
   // C rows [r_lo, r_hi) x cols [c_lo, c_hi); reduction length K
   int r_lo, r_hi, c_lo, c_hi, K, kb = 0;
+  int ti, tj;
   if (mode == 1) {
     if (p == 0) return;
     r_lo = c_lo = p; r_hi = c_hi = m; K = p;
+  } else if (mode == 4) {
+    if (k0 >= p) return;
+    kb = min(nb, p - k0);
+    const int k1 = k0 + kb;
+    K = k0;
+    if ((int)blockIdx.x < nc) { r_lo = k0; r_hi = m; c_lo = k0; c_hi = k1; ti = blockIdx.x; tj = 0; }
+    else { r_lo = k0; r_hi = k1; c_lo = k1; c_hi = m; ti = 0; tj = blockIdx.x - nc; }
   } else {
     if (k0 >= p) return;
     kb = min(nb, p - k0);
@@ -439,8 +452,8 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
   }
   if (r_lo >= r_hi || c_lo >= c_hi) return;
   const int ntr = (r_hi - r_lo + TILE - 1) / TILE, ntc = (c_hi - c_lo + TILE - 1) / TILE;
-  const int ti = blockIdx.x % ntr, tj = blockIdx.x / ntr;
-  if (tj >= ntc) return;
+  if (mode != 4) { ti = blockIdx.x % ntr; tj = blockIdx.x / ntr; }
+  if (ti >= ntr || tj >= ntc) return;
   const int r0 = r_lo + ti * TILE, c0 = c_lo + tj * TILE;
   if (mode == 0 && r0 >= p && c0 >= p) return;
   V* Fb = front_ptr<T>(arena, F, f, b, B);
@@ -458,7 +471,7 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
       V v = czero<V>();
       if (gr < r_hi && gk < K) {
         if (mode == 0) v = Fb[(int64_t)gr + (int64_t)(k0 + gk) * m];
-        else if (mode == 1) v = Fb[(int64_t)gr + (int64_t)gk * m];
+        else if (mode == 1 || mode == 4) v = Fb[(int64_t)gr + (int64_t)gk * m];
         else if (mode == 2) v = Fb[(int64_t)gr + (int64_t)(k0 + gk) * m];
         else {                                                                    // Linv[row][k]
           const int rr = gr - k0;
@@ -474,7 +487,7 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
       V v = czero<V>();
       if (gc < c_hi && gk < K) {
         if (mode == 0) v = Fb[(int64_t)(k0 + gk) + (int64_t)gc * m];
-        else if (mode == 1) v = Fb[(int64_t)gk + (int64_t)gc * m];
+        else if (mode == 1 || mode == 4) v = Fb[(int64_t)gk + (int64_t)gc * m];
         else if (mode == 2) {                                                     // Uinv[k][col]
           const int cc = gc - k0;
           v = gk <= cc ? Dinv[(int64_t)gk + (int64_t)cc * m] : czero<V>();
@@ -492,7 +505,7 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
     const int gc = c0 + wcol * 32 + i;
     if (gr < r_hi && gc < c_hi && !(mode == 0 && gr >= p && gc >= p)) {
       V* c = Fb + (int64_t)gr + (int64_t)gc * m;
-      if (mode >= 2) {
+      if (mode == 2 || mode == 3) {
         V v; v.x = re; v.y = im;
         *c = v;
       } else {
@@ -1116,7 +1129,17 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
       hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), B), dim3(256), 0, s, P->F,
                          L.e0, L.e1, B, omega, beta, P->ent_front, P->ent_slot, P->ent_loc, (const double2*)Kaa,
                          (const double2*)Maa, (const double2*)Daa, arena, P->fused[pi]); }
+    // left-looking panels: update panel k0 with the factored panels before it (one GEMM with
+    // K = k0 for its column and row parts), then its diagonal LU + inverses, then the two
+    // panel solves; the contribution block waits for the final Schur complement
     for (int k0 = 0; k0 < G.pmax; k0 += nbl) {
+      if (k0 > 0) {
+        const int nc = (G.mmax - k0 + TILE - 1) / TILE;
+        const int nr = (G.mmax - k0 - 1 + TILE - 1) / TILE;
+        Timed tm(P, s, K_GEMM_L);
+        hipLaunchKernelGGL((k_gemm<T, 4>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl,
+                           nc);
+      }
       const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
       { Timed tm(P, s, K_DIAG);
       hipLaunchKernelGGL(k_diag<T>, dim3(G.nf, B), dim3(128), lds_diag, s, P->F, G.fronts, B, arena, k0, nbl, tiny,
@@ -1124,13 +1147,9 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
       const int rem = G.mmax - k0 - 1;
       if (rem > 0) {
         const int nt = (rem + TILE - 1) / TILE;
-        { Timed tm(P, s, K_TRSM);
+        Timed tm(P, s, K_TRSM);
         hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl);
-        hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl); }
-        if (k0 + nbl < G.pmax) {
-          Timed tm(P, s, K_GEMM_L);
-          hipLaunchKernelGGL((k_gemm<T, 0>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl);
-        }
+        hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl);
       }
     }
     if (G.cbmax > 0) {
