@@ -39,14 +39,22 @@ def build_engine(cell, gradient=True, **kw):
     return _build(cell, gradient=gradient, **kw)
 
 
+KINDS = ['init', 'entries', 'diag', 'trsm', 'gemm_l', 'gemm_s', 'fwd', 'bwd', 'small']
+
+
 def profile_read(eng):
+    """Per-kind event totals of the recorded launches, and the same per tree level."""
     import ctypes as C
     from sawsim_amd import _lib
     ms = (C.c_double * 9)()
     cnt = (C.c_int64 * 9)()
     _lib.call('sawfem_mf_profile_read', eng._plan, C.cast(ms, C.c_void_p), C.cast(cnt, C.c_void_p))
-    names = ['init', 'entries', 'diag', 'trsm', 'gemm_l', 'gemm_s', 'fwd', 'bwd', 'small']
-    return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(names)}
+    nlev = eng.mf.lvl_ptr.size - 1
+    lv = (C.c_double * (9 * nlev))()
+    _lib.call_value('sawfem_mf_profile_levels', eng._plan, C.cast(lv, C.c_void_p), nlev)
+    per_level = [{n: round(float(lv[l * 9 + i]), 2) for i, n in enumerate(KINDS) if lv[l * 9 + i] > 0}
+                 for l in range(nlev)]
+    return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(KINDS)}, per_level
 
 
 def cpu_baseline(cell, workers=8):
@@ -124,7 +132,7 @@ def main():
         Q, dY, info = sweep()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    prof = profile_read(eng)
+    prof, prof_levels = profile_read(eng)
     _lib.call('sawfem_mf_profile', eng._plan, 0)
     value = args.steps * P / dt
     log(f'[bench] mixed sweep + adjoint {dt / args.steps:.3f}s/step -> {value:.1f} pts/s; ir {info}')
@@ -213,6 +221,7 @@ def main():
                        frac=(solve_gbs / 8000.0) if solve_gbs else None, bytes=solve_bytes, ms=solve_ms,
                        note='forward+backward passes; algorithmic bytes = stored L/U entries x 8 B per point')),
         kernel_ms={k: v[0] for k, v in prof.items()},
+        kernel_ms_per_level=prof_levels,
         factor_flops_per_point=work['factor_flops'],
         cpu_baseline=cpu,
     )

@@ -91,6 +91,9 @@ int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const v
  * returns totals in ms[9] / count[9]. */
 int sawfem_mf_profile(void* plan, int enable);
 int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count);
+/* The same totals per level of the assembly tree (leaves first), ms[level * 9 + kind], from the
+ * last profile_read; returns the number of levels written (at most max_levels) or -1. */
+int sawfem_mf_profile_levels(void* plan, double* ms, int max_levels);
 
 /* out[b][i] = -V (rK[i] + i w_b beta rD[i] - w_b^2 rM[i])  (sweep.py:338-342). */
 int sawfem_rhs(void* stream, int n, int B, const double* omega, double beta, double V, const void* rK,

@@ -53,94 +53,209 @@ __device__ __forceinline__ typename cplx<T>::type* front_ptr(typename cplx<T>::t
 }
 
 // ---------------------------------------------------------------- front assembly
+// Front f = sum of its children's contribution blocks (zero elsewhere), columns [c0, c1) of
+// one frequency per workgroup.  Threads cover rows in slabs of RS (64, 128 or 256) and CG =
+// 256 / RS columns at a time.  Two children (nested dissection): the parent-row -> child-CB
+// maps of a thread's row stay in registers and the values of CJ columns are loaded
+// unconditionally from clamped in-front addresses before any is selected or stored, so the
+// loads are in flight together; other child counts take a generic loop.
 template <typename T>
-__global__ void __launch_bounds__(256) k_front_init(FrontsDev F, const int* __restrict__ lvl, int B,
-                                                    typename cplx<T>::type* __restrict__ arena, int cols_per_wg) {
+__device__ __forceinline__ void front_init_body(FrontsDev F, const int* __restrict__ lvl, int B,
+                                                typename cplx<T>::type* __restrict__ arena, int cols_per_wg, int b) {
   using V = typename cplx<T>::type;
+  constexpr int CJ = 8;   // columns per thread whose loads are issued before any store
   const int f = lvl[blockIdx.y];
-  const int b = blockIdx.z;
   const int m = F.m[f];
   const int c0 = blockIdx.x * cols_per_wg;
   if (c0 >= m) return;
   const int c1 = min(m, c0 + cols_per_wg);
   V* Fb = front_ptr<T>(arena, F, f, b, B);
-  const int nch = F.child_ptr[f + 1] - F.child_ptr[f];
-  for (int64_t idx = threadIdx.x; idx < (int64_t)(c1 - c0) * m; idx += blockDim.x) {
-    const int j = c0 + (int)(idx / m);
-    const int i = (int)(idx % m);
-    V v = czero<V>();
-    for (int c = 0; c < nch; ++c) {
-      const int slot = F.child_ptr[f] + c;
-      const int* inv = F.inv + F.inv_ptr[slot];
-      const int ci = inv[i], cj = inv[j];
-      if (ci >= 0 && cj >= 0) {
-        const int ch = F.child_ids[slot];
-        const int64_t mc = F.m[ch], pc = F.p[ch];
-        const V* Fc = front_ptr<T>(arena, F, ch, b, B);
-        const V w = Fc[(pc + ci) + (pc + cj) * mc];
-        v.x += w.x; v.y += w.y;
+  const int cp0 = F.child_ptr[f];
+  const int nch = F.child_ptr[f + 1] - cp0;
+  const int RS = m <= 64 ? 64 : (m <= 128 ? 128 : 256);
+  const int CG = 256 / RS;
+  const int rsub = threadIdx.x % RS, cg = threadIdx.x / RS;
+  if (nch == 0) {                       // leaf: zeros
+    for (int j = c0 + cg; j < c1; j += CG)
+      for (int i = rsub; i < m; i += RS) Fb[i + (int64_t)j * m] = czero<V>();
+    return;
+  }
+  if (nch == 2) {                       // nested dissection: two children, loads batched
+    const V* cb[2];
+    const int* cinv[2];
+    int cmc[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int ch = F.child_ids[cp0 + c];
+      const int mc = F.m[ch], pc = F.p[ch];
+      cb[c] = front_ptr<T>(arena, F, ch, b, B) + pc + (int64_t)pc * mc;
+      cinv[c] = F.inv + F.inv_ptr[cp0 + c];
+      cmc[c] = mc;
+    }
+    for (int i0 = 0; i0 < m; i0 += RS) {
+      const int i = i0 + rsub;
+      const bool row_ok = i < m;
+      const int ir = min(i, m - 1);
+      const int ci0 = cinv[0][ir], ci1 = cinv[1][ir];
+      for (int jb = c0 + cg; jb < c1; jb += CG * CJ) {
+        int cj0[CJ], cj1[CJ];
+        V w0[CJ], w1[CJ];
+#pragma unroll
+        for (int q = 0; q < CJ; ++q) {
+          const int j = min(jb + q * CG, c1 - 1);
+          cj0[q] = cinv[0][j];
+          cj1[q] = cinv[1][j];
+        }
+#pragma unroll
+        for (int q = 0; q < CJ; ++q) {
+          w0[q] = cb[0][max(ci0, 0) + (int64_t)max(cj0[q], 0) * cmc[0]];
+          w1[q] = cb[1][max(ci1, 0) + (int64_t)max(cj1[q], 0) * cmc[1]];
+        }
+#pragma unroll
+        for (int q = 0; q < CJ; ++q) {
+          const int j = jb + q * CG;
+          V v = czero<V>();
+          if (ci0 >= 0 && cj0[q] >= 0) { v.x += w0[q].x; v.y += w0[q].y; }
+          if (ci1 >= 0 && cj1[q] >= 0) { v.x += w1[q].x; v.y += w1[q].y; }
+          if (row_ok && j < c1) Fb[i + (int64_t)j * m] = v;
+        }
       }
     }
-    Fb[i + (int64_t)j * m] = v;
+    return;
   }
+  // any other number of children: one entry at a time
+  for (int j = c0 + cg; j < c1; j += CG)
+    for (int i = rsub; i < m; i += RS) {
+      V v = czero<V>();
+      for (int c = 0; c < nch; ++c) {
+        const int slot = cp0 + c;
+        const int* inv = F.inv + F.inv_ptr[slot];
+        const int ci = inv[i], cj = inv[j];
+        if (ci >= 0 && cj >= 0) {
+          const int ch = F.child_ids[slot];
+          const int64_t mc = F.m[ch], pc = F.p[ch];
+          const V u = front_ptr<T>(arena, F, ch, b, B)[(pc + ci) + (pc + cj) * mc];
+          v.x += u.x; v.y += u.y;
+        }
+      }
+      Fb[i + (int64_t)j * m] = v;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_front_init(FrontsDev F, const int* __restrict__ lvl, int B,
+                                                    typename cplx<T>::type* __restrict__ arena, int cols_per_wg) {
+  front_init_body<T>(F, lvl, B, arena, cols_per_wg, blockIdx.z);
+
 }
 
 // Entries of the fronts of one level that go through the panel path (skip[f] != 0: the front
-// is assembled by k_small_front).
+// is assembled by k_small_front).  One entry per thread and ENT_FB frequencies per thread:
+// the FP64 K, M, D values are read once for those frequencies; consecutive entries of a front
+// are consecutive column-major offsets, so the front updates are coalesced.
+constexpr int ENT_FB = 16;
+
 template <typename T>
-__global__ void k_front_entries(FrontsDev F, int64_t e0, int64_t e1, int B, const double* __restrict__ omega,
-                                double beta, const int* __restrict__ ent_front, const int64_t* __restrict__ ent_slot,
-                                const int* __restrict__ ent_loc, const double2* __restrict__ Kaa,
-                                const double2* __restrict__ Maa, const double2* __restrict__ Daa,
-                                typename cplx<T>::type* __restrict__ arena, const unsigned char* __restrict__ skip) {
+__global__ void __launch_bounds__(256) k_front_entries(FrontsDev F, int64_t e0, int64_t e1, int B,
+                                const double* __restrict__ omega, double beta, const int* __restrict__ ent_front,
+                                const int64_t* __restrict__ ent_slot, const int* __restrict__ ent_loc,
+                                const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
+                                const double2* __restrict__ Daa, typename cplx<T>::type* __restrict__ arena,
+                                const unsigned char* __restrict__ skip) {
   using V = typename cplx<T>::type;
   const int64_t e = e0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= e1 || skip[ent_front[e]]) return;
-  const int b = blockIdx.y;
-  const double w = omega[b];
-  const int64_t s = ent_slot[e];
-  const double2 k = Kaa[s], mm = Maa[s];
-  double2 a;
-  a.x = k.x - w * w * mm.x;
-  a.y = k.y - w * w * mm.y;
-  if (Daa) {
-    const double2 d = Daa[s];
-    a.x -= w * beta * d.y;
-    a.y += w * beta * d.x;
+  if (e >= e1) return;
+  const int f = ent_front[e];
+  if (skip[f]) return;
+  const int64_t sl = ent_slot[e];
+  const double2 k = Kaa[sl], mm = Maa[sl];
+  const double2 d = Daa ? Daa[sl] : make_double2(0, 0);
+  const int64_t loc = ent_loc[e];
+  const int64_t msq = (int64_t)F.m[f] * F.m[f];
+  V* base = arena + F.off[f] * (int64_t)B + loc;
+  const int b0 = blockIdx.y * ENT_FB, b1 = min(B, b0 + ENT_FB);
+  for (int b = b0; b < b1; ++b) {
+    const double w = omega[b];
+    const double ax = k.x - w * w * mm.x - w * beta * d.y;
+    const double ay = k.y - w * w * mm.y + w * beta * d.x;
+    V* c = base + (int64_t)b * msq;
+    V cur = *c;
+    cur.x += (T)ax;
+    cur.y += (T)ay;
+    *c = cur;
   }
-  V* Fb = front_ptr<T>(arena, F, ent_front[e], b, B);
-  V cur = Fb[ent_loc[e]];
-  cur.x += (T)a.x;
-  cur.y += (T)a.y;
-  Fb[ent_loc[e]] = cur;
 }
 
 // ---------------------------------------------------------------- panel factorization
+// In-place inversion of the unit-lower and upper triangles of an n x n block D (column-major,
+// leading dimension ld, in LDS) by row operations (Gauss-Jordan): thread per row, the updates
+// inside a step independent of each other.
+//   L^{-1} (threads [0, h)):  step k, rows i > k:  X[i][j] -= L[i][k] X[k][j] (j < k), X[i][k] = -L[i][k]
+//   U^{-1} (threads [h, 2h)): step k = n-1..0: row k scaled by d = 1/U[k][k] (phase a), then
+//     rows i < k: Y[i][j] -= U[i][k] Y[k][j] (j > k), Y[i][k] = -U[i][k] d (phase b); the
+//     diagonal entries become 1/U[k][k] at the end
+// h >= n; all threads of the block call it (it synchronizes).
+template <typename V>
+__device__ void invert_lu_block(V* D, int ld, int n, int h) {
+  const int t = threadIdx.x;
+  const V one{1, 0};
+  for (int st = 0; st < n; ++st) {
+    // phase a: row k of Y scaled by d = 1/U[k][k], one column per thread; the diagonal itself
+    // keeps U[k][k] until the end (phase b of later steps never reads it)
+    if (t >= h && t < 2 * h) {
+      const int k = n - 1 - st, j = t - h;
+      if (j > k && j < n) D[k + j * ld] = cmul(D[k + j * ld], cdiv(one, D[k + k * ld]));
+    }
+    __syncthreads();
+    if (t < h) {
+      const int k = st, i = t;
+      if (i > k && i < n) {
+        const V mk = D[i + k * ld];
+        for (int j = 0; j < k; ++j) D[i + j * ld] = cfms(D[i + j * ld], mk, D[k + j * ld]);
+        D[i + k * ld] = V{-mk.x, -mk.y};
+      }
+    } else if (t < 2 * h) {
+      const int k = n - 1 - st, i = t - h;
+      if (i < k) {
+        const V mk = D[i + k * ld];
+        const V d = cdiv(one, D[k + k * ld]);
+        for (int j = k + 1; j < n; ++j) D[i + j * ld] = cfms(D[i + j * ld], mk, D[k + j * ld]);
+        const V y = cmul(mk, d);
+        D[i + k * ld] = V{-y.x, -y.y};
+      }
+    }
+    __syncthreads();
+  }
+  if (t >= h && t - h < n) D[(t - h) * (ld + 1)] = cdiv(one, D[(t - h) * (ld + 1)]);
+  __syncthreads();
+}
+
 // LU of the kb x kb diagonal block of pivot panel [k0, k0 + kb) (no pivoting; a pivot below
 // `tiny` is replaced by `tiny` and counted), then the inverses of its unit-lower and upper
 // factors stored IN PLACE of the factors: the strictly lower triangle of the block holds
 // L_D^{-1} (unit diagonal implied) and the upper triangle U_D^{-1}.  Neither the panel solves
-// (GEMMs with these inverses) nor the triangular solves need L_D, U_D themselves.
-// Threads 0..kb-1 build one column of L_D^{-1} each, threads 64..64+kb-1 one column of
-// U_D^{-1}, with uniform loops (broadcast LDS reads of the factors).
+// (GEMMs with these inverses) nor the triangular solves need L_D, U_D themselves.  Everything
+// happens in LDS; the block is written back once.
 template <typename T>
-__global__ void __launch_bounds__(128) k_diag(FrontsDev F, const int* __restrict__ lvl, int B,
-                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb,
-                                              T tiny, int* __restrict__ npert) {
+__device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ lvl, int B,
+                                          typename cplx<T>::type* __restrict__ arena, int k0, int nb,
+                                          T tiny, int* __restrict__ npert, int b) {
   using V = typename cplx<T>::type;
   extern __shared__ unsigned char smem_raw[];
   V* Ds = reinterpret_cast<V*>(smem_raw);   // [nb][nb+1], column-major with padding
   const int f = lvl[blockIdx.x];
-  const int b = blockIdx.y;
   const int p = F.p[f];
   if (k0 >= p) return;
   const int m = F.m[f];
   const int kb = min(nb, p - k0);
   const int ld = nb + 1;
   V* Fb = front_ptr<T>(arena, F, f, b, B);
+  V* G = Fb + (int64_t)k0 + (int64_t)k0 * m;
   const int t = threadIdx.x;
-  for (int j = 0; j < kb; ++j)
-    if (t < kb) Ds[t + j * ld] = Fb[(int64_t)(k0 + t) + (int64_t)(k0 + j) * m];
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    const int i = idx % kb, j = idx / kb;
+    Ds[i + j * ld] = G[i + (int64_t)j * m];
+  }
   __syncthreads();
   for (int k = 0; k < kb; ++k) {
     V piv = Ds[k + k * ld];
@@ -156,26 +271,19 @@ __global__ void __launch_bounds__(128) k_diag(FrontsDev F, const int* __restrict
     }
     __syncthreads();
   }
-  // column j of L^{-1}: x_i = delta_ij - sum_{j<=k<i} L[i][k] x_k, stored below the diagonal
-  // column j of U^{-1}: x_i = (delta_ij - sum_{i<k<=j} U[i][k] x_k) / U[i][i], stored on/above it
-  V* D = Fb + (int64_t)k0 + (int64_t)k0 * m;
-  if (t < kb) {
-    const int j = t;
-    V* x = D + (int64_t)j * m;
-    for (int i = j + 1; i < kb; ++i) {
-      V s = cmul(Ds[i + j * ld], V{-1, 0});
-      for (int k = j + 1; k < i; ++k) s = cfms(s, Ds[i + k * ld], x[k]);
-      x[i] = s;
-    }
-  } else if (t >= 64 && t < 64 + kb) {
-    const int j = t - 64;
-    V* x = D + (int64_t)j * m;
-    for (int i = j; i >= 0; --i) {
-      V s = (i == j) ? V{1, 0} : czero<V>();
-      for (int k = i + 1; k <= j; ++k) s = cfms(s, Ds[i + k * ld], x[k]);
-      x[i] = cdiv(s, Ds[i + i * ld]);
-    }
+  invert_lu_block(Ds, ld, kb, 64);
+  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
+    const int i = idx % kb, j = idx / kb;
+    G[i + (int64_t)j * m] = Ds[i + j * ld];
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(128) k_diag(FrontsDev F, const int* __restrict__ lvl, int B,
+                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb,
+                                              T tiny, int* __restrict__ npert) {
+  diag_body<T>(F, lvl, B, arena, k0, nb, tiny, npert, blockIdx.y);
+
 }
 
 // ---------------------------------------------------------------- small fronts, fused
@@ -294,45 +402,11 @@ __global__ void __launch_bounds__(SMALL_NT) k_small_front(
     const int i = idx % m, j = idx / m;
     if (!(i < p && j < p && i / nbl == j / nbl)) G[idx] = S[i + j * ld];
   }
-  // In-place inversion of each diagonal block by row operations (Gauss-Jordan), thread per
-  // row, independent updates inside a step:
-  //   L^{-1} (threads < 256): step k, rows i > k:  X[i][j] -= L[i][k] X[k][j] (j < k), X[i][k] = -L[i][k]
-  //   U^{-1} (threads >= 256): step k = n-1..0: row k scaled by d = 1/U[k][k] (phase a), then
-  //     rows i < k: Y[i][j] -= U[i][k] Y[k][j] (j > k), Y[i][k] = -U[i][k] d (phase b)
+  // in-place inversion of each diagonal block, then its write-back
   for (int J0 = 0; J0 < p; J0 += nbl) {
     const int nbk = min(nbl, p - J0);
     V* D = S + J0 + J0 * ld;
-    for (int st = 0; st < nbk; ++st) {
-      // phase a: U row scaling (row k = nbk-1-st)
-      if (t >= SMALL_NT / 2) {
-        const int k = nbk - 1 - st, i = t - SMALL_NT / 2;
-        if (i == k) {
-          const V one{1, 0};
-          const V d = cdiv(one, D[k + k * ld]);
-          for (int j = k + 1; j < nbk; ++j) D[k + j * ld] = cmul(D[k + j * ld], d);
-          D[k + k * ld] = d;
-        }
-      }
-      __syncthreads();
-      if (t < SMALL_NT / 2) {
-        const int k = st, i = t;
-        if (i > k && i < nbk) {
-          const V mk = D[i + k * ld];
-          for (int j = 0; j < k; ++j) D[i + j * ld] = cfms(D[i + j * ld], mk, D[k + j * ld]);
-          D[i + k * ld] = V{-mk.x, -mk.y};
-        }
-      } else {
-        const int k = nbk - 1 - st, i = t - SMALL_NT / 2;
-        if (i < k) {
-          const V mk = D[i + k * ld];
-          const V d = D[k + k * ld];
-          for (int j = k + 1; j < nbk; ++j) D[i + j * ld] = cfms(D[i + j * ld], mk, D[k + j * ld]);
-          const V y = cmul(mk, d);
-          D[i + k * ld] = V{-y.x, -y.y};
-        }
-      }
-      __syncthreads();
-    }
+    invert_lu_block(D, ld, nbk, SMALL_NT / 2);
     for (int idx = t; idx < nbk * nbk; idx += SMALL_NT) {
       const int i = idx % nbk, j = idx / nbk;
       G[(int64_t)(J0 + i) + (int64_t)(J0 + j) * m] = D[i + j * ld];
@@ -419,14 +493,12 @@ template <> struct WaveAcc<double> {
 //         with nc = ceil((mmax - k0) / TILE) from the launch (uniform over the level)
 // The inverses are the ones k_diag stored in the diagonal block.
 template <typename T, int mode>
-__global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
-                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb,
-                                              int nc = 0) {
+__device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ lvl, int B,
+                                          typename cplx<T>::type* __restrict__ arena, int k0, int nb, int nc, int b) {
   using V = typename cplx<T>::type;
-  __shared__ V As[KC * TILE];
-  __shared__ V Bs[TILE * (KC + 1)];
+  __shared__ V As[2][KC * TILE];
+  __shared__ V Bs[2][TILE * (KC + 1)];
   const int f = lvl[blockIdx.y];
-  const int b = blockIdx.z;
   const int p = F.p[f];
   const int m = F.m[f];
   // C rows [r_lo, r_hi) x cols [c_lo, c_hi); reduction length K
@@ -457,47 +529,75 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
   const int r0 = r_lo + ti * TILE, c0 = c_lo + tj * TILE;
   if (mode == 0 && r0 >= p && c0 >= p) return;
   V* Fb = front_ptr<T>(arena, F, f, b, B);
-  const V* Dinv = Fb + (int64_t)k0 + (int64_t)k0 * m;      // diagonal block holding the inverses
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63;
   const int wrow = wave & 1, wcol = wave >> 1;
-  WaveAcc<T> acc;
-  acc.zero();
-  for (int kc = 0; kc < K; kc += KC) {
-    // A chunk -> As[k][row]
-    for (int idx = t; idx < KC * TILE; idx += 256) {
+  // Loads are issued unconditionally from clamped in-front addresses and selected when stored
+  // to LDS, so a chunk's loads are all in flight together; the next chunk is loaded into
+  // registers while the current one feeds the MFMAs (double-buffered LDS, one barrier a chunk).
+  constexpr int NE = KC * TILE / 256;
+  V ra[NE], rb[NE];
+  unsigned char sa[NE], sb[NE];   // 0: zero, 1: loaded value, 2: one (unit diagonal of Linv)
+  auto load = [&](int kc) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int idx = t + 256 * e;
       const int row = idx % TILE, k = idx / TILE;
       const int gr = r0 + row, gk = kc + k;
-      V v = czero<V>();
-      if (gr < r_hi && gk < K) {
-        if (mode == 0) v = Fb[(int64_t)gr + (int64_t)(k0 + gk) * m];
-        else if (mode == 1 || mode == 4) v = Fb[(int64_t)gr + (int64_t)gk * m];
-        else if (mode == 2) v = Fb[(int64_t)gr + (int64_t)(k0 + gk) * m];
-        else {                                                                    // Linv[row][k]
-          const int rr = gr - k0;
-          v = rr > gk ? Dinv[(int64_t)rr + (int64_t)gk * m] : (rr == gk ? V{1, 0} : czero<V>());
-        }
+      const bool ok = gr < r_hi && gk < K;
+      const int grc = min(gr, r_hi - 1), gkc = min(gk, K - 1);
+      int64_t addr;
+      if (mode == 0 || mode == 2) addr = (int64_t)grc + (int64_t)(k0 + gkc) * m;
+      else if (mode == 1 || mode == 4) addr = (int64_t)grc + (int64_t)gkc * m;
+      else addr = (int64_t)grc + (int64_t)(k0 + gkc) * m;                  // Linv[row][k]
+      ra[e] = Fb[addr];
+      if (mode == 3) {
+        const int rr = gr - k0;
+        sa[e] = !ok ? 0 : (rr > gk ? 1 : (rr == gk ? 2 : 0));
+      } else {
+        sa[e] = ok ? 1 : 0;
       }
-      As[k * TILE + row] = v;
     }
-    // B chunk -> Bs[col][k]
-    for (int idx = t; idx < KC * TILE; idx += 256) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int idx = t + 256 * e;
       const int k = idx % KC, col = idx / KC;
       const int gk = kc + k, gc = c0 + col;
-      V v = czero<V>();
-      if (gc < c_hi && gk < K) {
-        if (mode == 0) v = Fb[(int64_t)(k0 + gk) + (int64_t)gc * m];
-        else if (mode == 1 || mode == 4) v = Fb[(int64_t)gk + (int64_t)gc * m];
-        else if (mode == 2) {                                                     // Uinv[k][col]
-          const int cc = gc - k0;
-          v = gk <= cc ? Dinv[(int64_t)gk + (int64_t)cc * m] : czero<V>();
-        }
-        else v = Fb[(int64_t)(k0 + gk) + (int64_t)gc * m];
-      }
-      Bs[col * (KC + 1) + k] = v;
+      const bool ok = gc < c_hi && gk < K;
+      const int gcc = min(gc, c_hi - 1), gkc = min(gk, K - 1);
+      int64_t addr;
+      if (mode == 1 || mode == 4) addr = (int64_t)gkc + (int64_t)gcc * m;
+      else addr = (int64_t)(k0 + gkc) + (int64_t)gcc * m;                     // mode 2: Uinv[k][col]
+      rb[e] = Fb[addr];
+      if (mode == 2) sb[e] = (ok && gk <= gc - k0) ? 1 : 0;
+      else sb[e] = ok ? 1 : 0;
     }
-    __syncthreads();
-    acc.step(As, Bs, wrow, wcol, lane);
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int idx = t + 256 * e;
+      const int row = idx % TILE, k = idx / TILE;
+      As[buf][k * TILE + row] = sa[e] == 1 ? ra[e] : (sa[e] == 2 ? V{1, 0} : czero<V>());
+    }
+#pragma unroll
+    for (int e = 0; e < NE; ++e) {
+      const int idx = t + 256 * e;
+      const int k = idx % KC, col = idx / KC;
+      Bs[buf][col * (KC + 1) + k] = sb[e] ? rb[e] : czero<V>();
+    }
+  };
+  WaveAcc<T> acc;
+  acc.zero();
+  const int nch = (K + KC - 1) / KC;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int c = 0; c < nch; ++c) {
+    const int buf = c & 1;
+    if (c + 1 < nch) load((c + 1) * KC);
+    acc.step(As[buf], Bs[buf], wrow, wcol, lane);
+    if (c + 1 < nch) store(buf ^ 1);
     __syncthreads();
   }
   acc.each(lane, [&](int j, int i, T re, T im) {
@@ -515,6 +615,13 @@ __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict
       }
     }
   });
+}
+
+template <typename T, int mode>
+__global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
+                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb, int nc) {
+  gemm_body<T, mode>(F, lvl, B, arena, k0, nb, nc, blockIdx.z);
+
 }
 
 // ---------------------------------------------------------------- triangular solves
@@ -583,18 +690,16 @@ __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, i
 // vectors then cover all m rows, the forward column/row-panel updates run down to row m and
 // the backward kinds start with the CB-to-pivot product.
 template <typename T, int KIND, int R, bool FULL>
-__global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
-                                             const typename cplx<T>::type* __restrict__ arena,
-                                             double2* __restrict__ uarena, double2* __restrict__ x) {
+__device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
+                                         const typename cplx<T>::type* __restrict__ arena,
+                                         double2* __restrict__ uarena, double2* __restrict__ x, int b) {
   using V = typename cplx<T>::type;
-  constexpr int SB = SolveTile<T>::SB;
-  constexpr int LD = SB + 1;
+  const int LD = nbl + 1;
   extern __shared__ unsigned char smem_raw[];
   V* tile = reinterpret_cast<V*>(smem_raw);                                   // [64][LD]
   double2* part = reinterpret_cast<double2*>(smem_raw + sizeof(V) * 64 * LD);  // [R][4][64]
   double2* w = part + R * 256;                                                // [R][W]
   const int f = lvl[blockIdx.x];
-  const int b = blockIdx.y;
   const int m = F.m[f], p = F.p[f];
   const int W = FULL ? m : p;
   const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
@@ -633,11 +738,11 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
 #pragma unroll
           for (int r = 0; r < R; ++r) s[r] = w[r * W + c0 + t];
         }
-        for (int j0 = 0; j0 < q; j0 += SB) {
-          const int nj = min(SB, q - j0);
+        for (int j0 = 0; j0 < q; j0 += nbl) {
+          const int nj = min(nbl, q - j0);
           __syncthreads();
-          for (int idx = t; idx < SB * 64; idx += blockDim.x) {
-            const int jj = idx % SB, c = idx / SB;
+          for (int idx = t; idx < nbl * 64; idx += blockDim.x) {
+            const int jj = idx % nbl, c = idx / nbl;
             if (jj < nj && c < nc) tile[c * LD + jj] = Fb[(int64_t)(p + j0 + jj) + (int64_t)(c0 + c) * m];
           }
           __syncthreads();
@@ -767,19 +872,26 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
   }
 }
 
+template <typename T, int KIND, int R, bool FULL>
+__global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
+                                             const typename cplx<T>::type* __restrict__ arena,
+                                             double2* __restrict__ uarena, double2* __restrict__ x) {
+  tri_body<T, KIND, R, FULL>(F, lvl, B, n, nbl, arena, uarena, x, blockIdx.y);
+
+}
+
 // Off-diagonal panels, one workgroup per 64-wide output tile of one front and frequency.
 //  kind 0 (forward,  L21):   uf[r]  -= sum_k L[p + r][k] y[k]        r in CB rows
 //  kind 1 (backward, U12):   x_p[r] -= sum_j U[r][p + j] x_cb[j]      r in pivot rows
 //  kind 2 (forward,  U12^T): uf[j]  -= sum_k U[k][p + j] w[k]         j in CB rows
 //  kind 3 (backward, L21^T): x_p[k] -= sum_j L[p + j][k] x_cb[j]      k in pivot rows
 template <typename T, int KIND, int R>
-__global__ void __launch_bounds__(256) k_panel(FrontsDev F, const int* __restrict__ lvl, int B, int n,
-                                               const typename cplx<T>::type* __restrict__ arena,
-                                               double2* __restrict__ uarena, double2* __restrict__ x) {
+__device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+                                           const typename cplx<T>::type* __restrict__ arena,
+                                           double2* __restrict__ uarena, double2* __restrict__ x, int b) {
   using V = typename cplx<T>::type;
   extern __shared__ unsigned char smem_raw[];
   const int f = lvl[blockIdx.y];
-  const int b = blockIdx.z;
   const int m = F.m[f], p = F.p[f];
   const int q = m - p;
   const bool out_cb = (KIND == 0 || KIND == 2);
@@ -856,6 +968,14 @@ __global__ void __launch_bounds__(256) k_panel(FrontsDev F, const int* __restric
       }
     }
   }
+}
+
+template <typename T, int KIND, int R>
+__global__ void __launch_bounds__(256) k_panel(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+                                               const typename cplx<T>::type* __restrict__ arena,
+                                               double2* __restrict__ uarena, double2* __restrict__ x) {
+  panel_body<T, KIND, R>(F, lvl, B, n, arena, uarena, x, blockIdx.z);
+
 }
 
 __global__ void k_rhs(int n, int B, const double* __restrict__ omega, double beta, double V,
@@ -996,7 +1116,10 @@ struct Plan {
   bool profile = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
-  std::vector<std::pair<int, size_t>> ev_marks;   // (kind, index of the start event)
+  struct Mark { int kind, level; size_t i0; };
+  std::vector<Mark> ev_marks;                      // (kind, level, index of the start event)
+  int cur_level = 0;                               // level of the launches being recorded
+  std::vector<double> level_ms;                    // [level][kind] totals of the last read
   int64_t arena_elems = 0, uarena_elems = 0;
   std::vector<Level> levels;
   FrontsDev F{};
@@ -1036,7 +1159,7 @@ struct Timed {
   ~Timed() {
     if (!P->profile) return;
     (void)hipEventRecord(P->ev_pool[i0 + 1], s);
-    P->ev_marks.push_back({kind, i0});
+    P->ev_marks.push_back({kind, P->cur_level, i0});
   }
 };
 
@@ -1109,7 +1232,9 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
   V* arena = (V*)arena_v;
   const T tiny = sizeof(T) == 4 ? (T)1e-30 : (T)1e-300;
   const int pi = sizeof(T) == 4 ? 0 : 1;
-  for (const Level& L : P->levels) {
+  for (size_t li = 0; li < P->levels.size(); ++li) {
+    const Level& L = P->levels[li];
+    P->cur_level = (int)li;
     const int nbl = panel_width(nb, L);
     const Sub& S = L.small[pi];
     if (S.nf > 0) {
@@ -1126,7 +1251,8 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
     hipLaunchKernelGGL(k_front_init<T>, dim3((G.mmax + cpw - 1) / cpw, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
                        arena, cpw); }
     if (L.e1 > L.e0) { Timed tm(P, s, K_ENTRIES);
-      hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), B), dim3(256), 0, s, P->F,
+      hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), (B + ENT_FB - 1) / ENT_FB),
+                         dim3(256), 0, s, P->F,
                          L.e0, L.e1, B, omega, beta, P->ent_front, P->ent_slot, P->ent_loc, (const double2*)Kaa,
                          (const double2*)Maa, (const double2*)Daa, arena, P->fused[pi]); }
     // left-looking panels: update panel k0 with the factored panels before it (one GEMM with
@@ -1148,14 +1274,14 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
       if (rem > 0) {
         const int nt = (rem + TILE - 1) / TILE;
         Timed tm(P, s, K_TRSM);
-        hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl);
-        hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl);
+        hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
+        hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
       }
     }
     if (G.cbmax > 0) {
-      const int nt = (G.cbmax + TILE - 1) / TILE;
       Timed tm(P, s, K_GEMM_S);
-      hipLaunchKernelGGL((k_gemm<T, 1>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, 0, nb);
+      const int nt = (G.cbmax + TILE - 1) / TILE;
+      hipLaunchKernelGGL((k_gemm<T, 1>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, 0, nb, 0);
     }
   }
   SAW_HIP(hipGetLastError());
@@ -1165,7 +1291,7 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
 template <typename T, int R>
 int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose, int nb) {
   using V = typename cplx<T>::type;
-  constexpr int LD = SolveTile<T>::SB + 1;
+  constexpr int LD = SolveTile<T>::SB + 1;   // panel kernels' transposition tile
   const int pi = sizeof(T) == 4 ? 0 : 1;
   const V* A = (const V*)arena;
   double2* U = (double2*)uarena;
@@ -1177,7 +1303,7 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     if (S.nf == 0) return;
     Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
     const int nbl = panel_width(nb, L);
-    const size_t lds = sizeof(V) * 64 * LD + sizeof(double2) * R * (256 + (full ? S.mmax : S.pmax));
+    const size_t lds = sizeof(V) * 64 * (nbl + 1) + sizeof(double2) * R * (256 + (full ? S.mmax : S.pmax));
     const dim3 grid(S.nf, B);
 #define SAW_TRI(K, FULL) hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, nbl, A, U, X)
     if (full) {
@@ -1206,13 +1332,16 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     }
   };
   const int kf = transpose ? 2 : 0, kb = transpose ? 3 : 1;
-  for (const Level& L : P->levels) {
+  for (size_t li = 0; li < P->levels.size(); ++li) {
+    const Level& L = P->levels[li];
+    P->cur_level = (int)li;
     tri(L, kf, true);
     tri(L, kf, false);
     panel(L, kf);
   }
   for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
     const Level& L = P->levels[l];
+    P->cur_level = l;
     panel(L, kb);
     tri(L, kb, false);
     tri(L, kb, true);
@@ -1440,16 +1569,30 @@ int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count) {
   Plan* P = (Plan*)plan;
   if (!P || !ms || !count) { sawfem_set_error("sawfem_mf_profile_read: invalid arguments"); return 1; }
   for (int k = 0; k < K_NKIND; ++k) { ms[k] = 0; count[k] = 0; }
+  P->level_ms.assign(P->levels.size() * K_NKIND, 0.0);
   for (auto& mk : P->ev_marks) {
-    SAW_HIP(hipEventSynchronize(P->ev_pool[mk.second + 1]));
+    SAW_HIP(hipEventSynchronize(P->ev_pool[mk.i0 + 1]));
     float t = 0;
-    SAW_HIP(hipEventElapsedTime(&t, P->ev_pool[mk.second], P->ev_pool[mk.second + 1]));
-    ms[mk.first] += t;
-    count[mk.first] += 1;
+    SAW_HIP(hipEventElapsedTime(&t, P->ev_pool[mk.i0], P->ev_pool[mk.i0 + 1]));
+    ms[mk.kind] += t;
+    count[mk.kind] += 1;
+    if (mk.level >= 0 && mk.level < (int)P->levels.size()) P->level_ms[(size_t)mk.level * K_NKIND + mk.kind] += t;
   }
   P->ev_used = 0;
   P->ev_marks.clear();
   return 0;
+}
+
+// Per-level totals of the last sawfem_mf_profile_read: ms[level * 9 + kind] for every level
+// (leaves first).  Returns the number of levels written (at most max_levels), or -1.
+int sawfem_mf_profile_levels(void* plan, double* ms, int max_levels) {
+  Plan* P = (Plan*)plan;
+  if (!P || !ms || max_levels < 0) { sawfem_set_error("sawfem_mf_profile_levels: invalid arguments"); return -1; }
+  const int nl = std::min(max_levels, (int)P->levels.size());
+  for (int l = 0; l < nl; ++l)
+    for (int k = 0; k < K_NKIND; ++k)
+      ms[l * K_NKIND + k] = P->level_ms.empty() ? 0.0 : P->level_ms[(size_t)l * K_NKIND + k];
+  return nl;
 }
 
 int sawfem_rhs(void* stream, int n, int B, const double* omega, double beta, double V, const void* rK,

@@ -6,7 +6,8 @@ From the symbolic analysis and the active pattern this builds, once per cell:
 * per (front, child): the map parent-row -> child contribution-block index (or -1),
   used by the gather-form extend-add and by the forward solve;
 * per A_aa slot: the front that owns it (the one eliminating the earlier of its row and
-  column) and its column-major offset inside that front;
+  column) and its column-major offset inside that front, ordered by level, front and
+  offset (consecutive entries of a front land on consecutive addresses);
 * levels (leaves first) and the entry ranges of each level.
 """
 from __future__ import annotations
@@ -111,7 +112,7 @@ def build_mf_arrays(sym: Symbolic, aa_indptr: np.ndarray, aa_indices: np.ndarray
         return lk[pos]
     li, lj = local(own, r), local(own, c)
     loc = li + lj * m[own].astype(np.int64)
-    order = np.lexsort((own, level_of[own]))
+    order = np.lexsort((loc, own, level_of[own]))      # by level, front, column-major offset
     ent_front = own[order].astype(np.int32)
     ent_slot = np.arange(r.size, dtype=np.int64)[order]
     ent_loc = loc[order].astype(np.int32)
