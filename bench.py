@@ -89,7 +89,8 @@ def main():
     ap.add_argument('--workload', default='config2')
     ap.add_argument('--batch', type=int, default=0, help='frequencies per factorization chunk (0: fit 120 GB)')
     ap.add_argument('--plain-points', type=int, default=8)
-    ap.add_argument('--check-points', type=int, default=16)
+    ap.add_argument('--check-points', type=int, default=0,
+                    help='frequencies of the FP64 accuracy check (0: every point of the sweep)')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-workers', type=int, default=8)
     ap.add_argument('--leaf-weight', type=int, default=0, help='nested-dissection leaf size (0: default)')
@@ -137,10 +138,22 @@ def main():
     value = args.steps * P / dt
     log(f'[bench] mixed sweep + adjoint {dt / args.steps:.3f}s/step -> {value:.1f} pts/s; ir {info}')
     # accuracy of the mixed path against the plain FP64 path (with its FP64 adjoint) on a sample
-    idx = np.linspace(0, P - 1, min(args.check_points, P)).astype(int)
+    # accuracy of the mixed path against the FP64 path (batched complex128 fronts, FP64 adjoint)
+    idx = np.arange(P) if not args.check_points else np.linspace(0, P - 1, min(args.check_points, P)).astype(int)
+    t0 = time.perf_counter()
     Q64, dY64, _ = eng.sweep_charges(freqs[idx], precision='fp64', batch=max(1, min(len(idx), batch // 2)),
                                      gradient=True)
+    log(f'[bench] FP64 check sweep of {len(idx)} points: {time.perf_counter() - t0:.1f}s')
     y_err = float(np.max(np.abs(Q[idx] - Q64) / np.abs(Q64)))
+    frfa = None
+    if len(idx) == P:
+        from sawsim_amd.metrics import compare, resonances
+        ym, y64 = -1j * 2 * np.pi * freqs * Q, -1j * 2 * np.pi * freqs * Q64
+        rm, r64 = resonances(freqs, ym), resonances(freqs, y64)
+        d = compare(r64, rm)
+        frfa = dict(fp64_fr_hz=r64['fr_hz'], fp64_fa_hz=r64['fa_hz'], mixed_fr_hz=rm['fr_hz'], mixed_fa_hz=rm['fa_hz'],
+                    fr_diff_hz=d['fr_diff_hz'], fa_diff_hz=d['fa_diff_hz'], step_hz=r64['step_hz'],
+                    warnings=r64['warnings'])
     scale = np.abs(cell.params)
     scale[12:15] = 1.0
     g_err = float(np.max(np.abs((dY[idx] - dY64) * scale).max(axis=1) / np.abs(dY64 * scale).max(axis=1)))
@@ -204,6 +217,8 @@ def main():
                     points=P, n_active=eng.n_active, fronts=eng.mf.nf, batch=batch,
                     gradient='dY11/dp for 17 parameters (3m constants, Euler angles, beta_dK, tan delta) by adjoint'),
         y11_max_rel_err_vs_fp64=y_err,
+        y11_check_points=int(len(idx)),
+        fr_fa_vs_fp64=frfa,
         dy11_max_rel_err_vs_fp64_adjoint=g_err,
         plain_fp64_gpu_pts_per_s=dict(y11_only=plain, y11_plus_fp64_adjoint=plain_adj,
                                       y11_plus_central_fd=plain_fd),
