@@ -28,6 +28,10 @@ import numpy as np  # noqa: E402
 
 PEAK_FP32_MFMA = 157.3   # TFLOP/s dense, MI355X
 PEAK_FP64_MFMA = 78.6
+# HBM bytes per launch of the Schur GEMM k_gemm<float, 1> on the config-2 sweep: rocprofv3
+# FETCH_SIZE and WRITE_SIZE (separate runs), each scaled by what the counter reads for a plain
+# 1 GiB device copy (`--copy-mib 1024`: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000)
+SCHUR_TRAFFIC_BYTES = {'config2': 2.991e10}
 
 
 def log(*a):
@@ -81,6 +85,42 @@ def cpu_baseline(cell, workers=8):
                        f'{dt:.1f}s wall, host setup {setup:.1f}s excluded; os.cpu_count()={os.cpu_count()}')
 
 
+def plain_fp64(eng, cell, freqs, pp):
+    """The plain FP64 GPU path, one frequency per factorization, on pp points of the sweep:
+    Y11 only, Y11 with its FP64 adjoint, and Y11 with central differences for the 17 parameters
+    (2P+1 = 35 operator rebuilds and solves, measured on one point).  Points per second each."""
+    import torch
+    from sawsim_amd import workloads
+    pidx = np.linspace(0, freqs.size - 1, pp).astype(int)
+    eng.sweep_charges(freqs[pidx[:1]], precision='fp64', batch=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.sweep_charges(freqs[pidx], precision='fp64', batch=1)
+    torch.cuda.synchronize()
+    plain = pp / (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    eng.sweep_charges(freqs[pidx], precision='fp64', batch=1, gradient=True)
+    torch.cuda.synchronize()
+    plain_adj = pp / (time.perf_counter() - t0)
+    fz = freqs[pidx[:1]]
+    p0 = cell.params.copy()
+    t0 = time.perf_counter()
+    eng.sweep_charges(fz, precision='fp64', batch=1)
+    for k in range(17):
+        h = 1e-6 * abs(p0[k]) if p0[k] != 0 else 1e-6 * 180 / np.pi
+        for sgn in (1, -1):
+            p = p0.copy()
+            p[k] += sgn * h
+            K, M, D = workloads.assemble_cell(cell, workloads.materials_for_params(cell, p))
+            eng.update_operator(K, M, D, beta=p[15])
+            eng.sweep_charges(fz, precision='fp64', batch=1)
+    torch.cuda.synchronize()
+    plain_fd = 1.0 / (time.perf_counter() - t0)
+    K, M, D = workloads.assemble_cell(cell)
+    eng.update_operator(K, M, D, beta=p0[15])
+    return plain, plain_adj, plain_fd
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -95,7 +135,18 @@ def main():
     ap.add_argument('--cpu-workers', type=int, default=8)
     ap.add_argument('--leaf-weight', type=int, default=0, help='nested-dissection leaf size (0: default)')
     ap.add_argument('--small-max', type=int, default=-1, help='largest front for the fused kernel (-1: default)')
+    ap.add_argument('--copy-mib', type=int, default=0,
+                    help='only run a plain device copy of this many MiB (HBM counter calibration) and exit')
     args = ap.parse_args()
+    if args.copy_mib:
+        import torch
+        n = args.copy_mib * 2 ** 20 // 4
+        src = torch.ones(n, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_bytes_read=4 * n, copy_bytes_written=4 * n)), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps(dict(error='this build runs on exactly one GPU; multi-GPU sharding of the '
                                     'frequency axis is designed but not built', n_gpus=args.gpus)))
@@ -138,6 +189,7 @@ def main():
     value = args.steps * P / dt
     log(f'[bench] mixed sweep + adjoint {dt / args.steps:.3f}s/step -> {value:.1f} pts/s; ir {info}')
     # accuracy of the mixed path against the plain FP64 path (with its FP64 adjoint) on a sample
+    torch.cuda.empty_cache()          # the FP64 chunks have another size than the timed ones
     # accuracy of the mixed path against the FP64 path (batched complex128 fronts, FP64 adjoint)
     idx = np.arange(P) if not args.check_points else np.linspace(0, P - 1, min(args.check_points, P)).astype(int)
     t0 = time.perf_counter()
@@ -159,36 +211,12 @@ def main():
     g_err = float(np.max(np.abs((dY[idx] - dY64) * scale).max(axis=1) / np.abs(dY64 * scale).max(axis=1)))
     # plain FP64 GPU path: one frequency per factorization
     pp = min(args.plain_points, P)
-    pidx = np.linspace(0, P - 1, pp).astype(int)
-    eng.sweep_charges(freqs[pidx[:1]], precision='fp64', batch=1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    eng.sweep_charges(freqs[pidx], precision='fp64', batch=1)
-    torch.cuda.synchronize()
-    plain = pp / (time.perf_counter() - t0)
-    t0 = time.perf_counter()
-    eng.sweep_charges(freqs[pidx], precision='fp64', batch=1, gradient=True)
-    torch.cuda.synchronize()
-    plain_adj = pp / (time.perf_counter() - t0)
-    # plain FP64 with the gradient by central differences: 2P+1 = 35 operator rebuilds + solves, one point
-    fz = freqs[pidx[:1]]
-    p0 = cell.params.copy()
-    t0 = time.perf_counter()
-    eng.sweep_charges(fz, precision='fp64', batch=1)
-    for k in range(17):
-        h = 1e-6 * abs(p0[k]) if p0[k] != 0 else 1e-6 * 180 / np.pi
-        for sgn in (1, -1):
-            p = p0.copy()
-            p[k] += sgn * h
-            K, M, D = workloads.assemble_cell(cell, workloads.materials_for_params(cell, p))
-            eng.update_operator(K, M, D, beta=p[15])
-            eng.sweep_charges(fz, precision='fp64', batch=1)
-    torch.cuda.synchronize()
-    plain_fd = 1.0 / (time.perf_counter() - t0)
-    K, M, D = workloads.assemble_cell(cell)
-    eng.update_operator(K, M, D, beta=p0[15])
-    log(f'[bench] plain fp64: Y11 {plain:.2f} pts/s, +adjoint {plain_adj:.2f}, +central FD {plain_fd:.3f}; '
-        f'mixed vs fp64: Y11 {y_err:.2e}, dY/dp {g_err:.2e}')
+    plain = plain_adj = plain_fd = None
+    if pp > 0:
+        plain, plain_adj, plain_fd = plain_fp64(eng, cell, freqs, pp)
+    if pp > 0:
+        log(f'[bench] plain fp64: Y11 {plain:.2f} pts/s, +adjoint {plain_adj:.2f}, +central FD {plain_fd:.3f}')
+    log(f'[bench] mixed vs fp64: Y11 {y_err:.2e}, dY/dp {g_err:.2e}')
     # dominant MFMA kernel: the Schur-complement GEMM k_gemm<float, 1> (one launch per level)
     schur_ms, schur_n = prof['gemm_s']
     schur_flops = work['schur_flops'] * P * args.steps
@@ -196,11 +224,22 @@ def main():
     lupd_tf = work['lupd_flops'] * P * args.steps / (prof['gemm_l'][0] * 1e-3) / 1e12 if prof['gemm_l'][0] else None
     # solves: each forward+backward pass streams the stored L/U entries (complex64) once
     # (passes over the factors: paired forward+adjoint solves share each pass)
-    nsolve_pts = sum(it['points'] * ((1 + it['ir_steps']) if it.get('paired_adjoint')
-                                     else (2 + it['ir_steps'] + it.get('adj_steps', 0))) for it in info) * args.steps
+    nsolve_pts = sum(sum(it['solved_per_pass']) if it.get('paired_adjoint')
+                     else it['points'] * (2 + it['ir_steps'] + it.get('adj_steps', 0)) for it in info) * args.steps
     solve_bytes = nsolve_pts * work['lu_entries'] * 8
     solve_ms = prof['fwd'][0] + prof['bwd'][0]
     solve_gbs = solve_bytes / (solve_ms * 1e-3) / 1e9 if solve_ms else None
+    # whole sweep: algorithmic flops of factorizations, solves (every right-hand side) and FP64
+    # residuals, over the wall time of the timed region
+    nnz_aa = int(eng.bloch.nnz_aa)
+    def chunk_flops(it):
+        if it.get('paired_adjoint'):       # two right-hand sides per solved frequency and pass
+            return (it['points'] * work['factor_flops'] + 2 * sum(it['solved_per_pass']) * work['solve_flops']
+                    + 2 * it['points'] * it['ir_steps'] * 8 * nnz_aa)
+        return it['points'] * (work['factor_flops'] + (2 + it['ir_steps'] + it.get('adj_steps', 0)) * work['solve_flops']
+                               + (it['ir_steps'] + it.get('adj_steps', 0)) * 8 * nnz_aa)
+    sweep_flops = args.steps * sum(chunk_flops(it) for it in info)
+    sweep_tf = sweep_flops / dt / 1e12
     cpu = None
     if not args.no_cpu:
         try:
@@ -221,16 +260,21 @@ def main():
         fr_fa_vs_fp64=frfa,
         dy11_max_rel_err_vs_fp64_adjoint=g_err,
         plain_fp64_gpu_pts_per_s=dict(y11_only=plain, y11_plus_fp64_adjoint=plain_adj,
-                                      y11_plus_central_fd=plain_fd),
+                                      y11_plus_central_fd=plain_fd) if pp > 0 else None,
         speedup_vs_plain_fp64=dict(central_fd=value / plain_fd, fp64_adjoint=value / plain_adj,
-                                   y11_only=value / plain),
+                                   y11_only=value / plain) if pp > 0 else None,
         ir=info,
         roofline=dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
-                      frac=(achieved / PEAK_FP32_MFMA) if achieved else None, traffic=None,
+                      frac=(achieved / PEAK_FP32_MFMA) if achieved else None,
+                      traffic=SCHUR_TRAFFIC_BYTES.get(args.workload),
+                      traffic_note='HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy',
                       kernel='k_gemm<float, 1> (Schur complement, complex64 on FP32 MFMA)',
                       launches=schur_n, kernel_ms=schur_ms, algorithmic_flops=schur_flops,
                       avg_launch_ms=schur_ms / max(schur_n, 1)),
         other_kernels=dict(
+            whole_sweep=dict(bound='mfma', achieved=sweep_tf, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
+                             frac=sweep_tf / PEAK_FP32_MFMA, algorithmic_flops=sweep_flops,
+                             note='(factor + every solve + FP64 residuals) / wall time of the timed sweep'),
             lupdate_gemm_tflops=lupd_tf,
             solve=dict(bound='hbm', achieved_gbs=solve_gbs, peak_gbs=8000.0,
                        frac=(solve_gbs / 8000.0) if solve_gbs else None, bytes=solve_bytes, ms=solve_ms,

@@ -81,9 +81,11 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
  * uarena: uarena_elems * B * nrhs complex128 scratch.
  * transpose != 0 solves A^T x = b with the same factors (U^T then L^T; adjoint solve).
  * nb: the panel width the arena was factored with (the diagonal blocks store the inverses of
- * their triangular factors, so the solve has to walk the same blocks). */
+ * their triangular factors, so the solve has to walk the same blocks).  bsel: device list of
+ * nsel frequencies of the chunk to solve (refinement steps skip converged frequencies), or
+ * NULL for all B. */
 int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const void* arena, void* uarena, void* x,
-                    int transpose, int nb);
+                    int transpose, int nb, const int* bsel, int nsel);
 
 /* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;
  * used by the benchmark).  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S,

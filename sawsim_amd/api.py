@@ -1,0 +1,239 @@
+"""Model -> sweep -> Result, with the reference's result-directory layout.
+
+Mirrors ``sawsim.Model`` / ``sawsim.sweep`` / ``sawsim.Result`` (``src/sawsim/api.py:12-90``) and
+the artifacts ``run_simulation`` writes (``src/sawsim/solver.py:56-157``) for the cells this
+backend builds itself (the structured SP templates and the synthetic stack cells of
+``workloads``):
+
+* ``input.json`` (the model's fields), ``progress.json`` (stage events),
+* ``mesh.npz`` {position_m, quads, material_tags},
+* ``curve.json`` {frequency_ghz, real, imag, magnitude}, ``admittance.npz`` {frequency_hz,
+  charge, admittance}, ``admittance.csv`` (frequency_hz, real_y, imag_y, abs_y),
+* ``fields.npz``: the FP64 re-solve at the largest |Y| sample, displacement scaled by SCALE
+  (``solver.py:131-140``),
+* ``gradient.npz`` {frequency_hz, names, dy11_dp} when the gradient is requested (no
+  counterpart in the reference),
+* ``metadata.json`` and ``manifest.json`` (artifact list).
+
+The admittance convention is the reference's passive sign, Y = -i w Q / V.  Images, the
+material snapshot and Gmsh meshes are not produced.
+"""
+from __future__ import annotations
+
+import json
+import shutil
+import tempfile
+import time
+from pathlib import Path
+from typing import Callable, Optional
+
+import numpy as np
+
+TEMPLATES = {
+    'sp_double_layer': 'structured double-layer SP cell (reference template, ME1)',
+    'sp_quad_layer': 'structured quad-layer SP cell (reference template, ME1)',
+    'config1': 'sp_stack-style 3m film on Si, plane strain (ME0)',
+    'config2': 'config1 with a SiO2 overlay, refined mesh (ME0)',
+    'config3': 'config1 with Rayleigh damping and dielectric loss (ME0)',
+}
+_BUILDERS = {'sp_double_layer': 'double_layer', 'sp_quad_layer': 'quad_layer', 'config1': 'config1',
+             'config2': 'config2', 'config3': 'config3'}
+_DEFAULTS = dict(points=None, start_ghz=None, stop_ghz=None, voltage=1.0, precision='mixed', gradient=False,
+                 batch=None)
+
+
+def _atomic_json(path: Path, data):
+    tmp = path.with_suffix('.tmp')
+    tmp.write_text(json.dumps(data, allow_nan=False), encoding='utf-8')
+    tmp.replace(path)
+
+
+class Model:
+    """A unit-cell model: template id plus sweep fields (points, start_ghz, stop_ghz, voltage,
+    precision 'mixed' | 'fp64', gradient, batch)."""
+
+    def __init__(self, model_id: str = 'sp_double_layer', **params):
+        if model_id not in TEMPLATES:
+            raise ValueError(f'unknown model_id {model_id!r}; available: {sorted(TEMPLATES)}')
+        unknown = set(params) - set(_DEFAULTS)
+        if unknown:
+            raise ValueError(f'unknown fields {sorted(unknown)}')
+        self.model_id = model_id
+        self.config = dict(_DEFAULTS, **params, model_id=model_id)
+
+    @classmethod
+    def from_json(cls, path) -> 'Model':
+        data = json.loads(Path(path).read_text(encoding='utf-8'))
+        return cls(**data)
+
+    @staticmethod
+    def templates() -> dict:
+        return dict(TEMPLATES)
+
+    def to_dict(self) -> dict:
+        return dict(self.config)
+
+    def cell(self):
+        """The workload cell (mesh, materials, band) this model sweeps."""
+        from . import workloads
+        c = self.config
+        builder = workloads.WORKLOADS[_BUILDERS[self.model_id]]
+        kw = {}
+        if c['points']:
+            kw['points'] = int(c['points'])
+        cell = builder(**kw)
+        if c['start_ghz'] is not None or c['stop_ghz'] is not None:
+            lo = cell.band_hz[0] if c['start_ghz'] is None else c['start_ghz'] * 1e9
+            hi = cell.band_hz[1] if c['stop_ghz'] is None else c['stop_ghz'] * 1e9
+            cell = builder(band=(lo, hi), **kw)
+        return cell
+
+    def __repr__(self):
+        c = self.config
+        return f"Model({self.model_id!r}, points={c['points']}, {c['start_ghz']}-{c['stop_ghz']} GHz)"
+
+
+class Result:
+    """Results of one sweep, backed by an output directory."""
+
+    def __init__(self, output_dir):
+        self.dir = Path(output_dir)
+        self.manifest = self._load('manifest.json')
+        self.metadata = self._load('metadata.json')
+        self.curve = self._load('curve.json')
+
+    def _load(self, name):
+        p = self.dir / name
+        return json.loads(p.read_text(encoding='utf-8')) if p.exists() else {}
+
+    @property
+    def frequency_ghz(self) -> np.ndarray:
+        return np.asarray(self.curve.get('frequency_ghz', []), dtype=float)
+
+    @property
+    def admittance(self) -> np.ndarray:
+        return (np.asarray(self.curve.get('real', []), dtype=float)
+                + 1j * np.asarray(self.curve.get('imag', []), dtype=float))
+
+    @property
+    def magnitude(self) -> np.ndarray:
+        return np.abs(self.admittance)
+
+    @property
+    def peak_frequency_ghz(self) -> Optional[float]:
+        return self.metadata.get('peak_frequency_ghz')
+
+    def resonances(self) -> dict:
+        from .metrics import resonances
+        return resonances(self.frequency_ghz * 1e9, self.admittance)
+
+    def artifact(self, name: str) -> Path:
+        return self.dir / name
+
+    def save(self, target) -> Path:
+        target = Path(target)
+        if target.resolve() != self.dir.resolve():
+            shutil.copytree(self.dir, target, dirs_exist_ok=True)
+        return target
+
+    def __repr__(self):
+        return f'Result({self.dir}, {len(self.frequency_ghz)} points, peak={self.peak_frequency_ghz} GHz)'
+
+
+def write_sweep_results(out: Path, fre, charge, voltage, *, mesh=None, fields=None, gradient=None, metadata=None):
+    """Write the curve, admittance, mesh, fields, gradient, metadata and manifest artifacts.
+
+    ``fre`` Hz, ``charge`` complex Q per frequency; ``mesh`` = (position_m, quads, tags);
+    ``fields`` = dict for fields.npz; ``gradient`` = (names, dY (P, k)).
+    """
+    out = Path(out)
+    fre = np.asarray(fre, dtype=float)
+    q = np.asarray(charge, dtype=complex)
+    y = -1j * 2 * np.pi * fre * q / voltage
+    _atomic_json(out / 'curve.json', dict(frequency_ghz=(fre * 1e-9).tolist(), real=y.real.tolist(),
+                                          imag=y.imag.tolist(), magnitude=np.abs(y).tolist()))
+    np.savez_compressed(out / 'admittance.npz', frequency_hz=fre, charge=q, admittance=y)
+    np.savetxt(out / 'admittance.csv', np.column_stack([fre, y.real, y.imag, np.abs(y)]), delimiter=',',
+               header='frequency_hz,real_y,imag_y,abs_y', comments='')
+    if mesh is not None:
+        pos, quads, tags = mesh
+        np.savez_compressed(out / 'mesh.npz', position_m=pos, quads=quads, material_tags=tags)
+    if fields is not None:
+        np.savez_compressed(out / 'fields.npz', **fields)
+    if gradient is not None:
+        names, dy = gradient
+        np.savez_compressed(out / 'gradient.npz', frequency_hz=fre, names=np.asarray(names), dy11_dp=np.asarray(dy))
+    peak = int(np.argmax(np.abs(y)))
+    meta = dict(frequency_count=int(fre.size), peak_frequency_ghz=float(fre[peak] * 1e-9),
+                peak_is_boundary=peak in (0, fre.size - 1),
+                admittance_convention='passive: Y = -i w Q_reaction / V',
+                units={'frequency': 'Hz', 'position': 'm', 'displacement': 'm', 'potential': 'V',
+                       'admittance': 'S/m (unit out-of-plane aperture)', 'charge': 'C/m'})
+    meta.update(metadata or {})
+    _atomic_json(out / 'metadata.json', meta)
+    media = {'.json': 'application/json', '.csv': 'text/csv', '.npz': 'application/octet-stream'}
+    artifacts = [dict(name=p.name, kind='data', media_type=media[p.suffix]) for p in sorted(out.iterdir())
+                 if p.suffix in media and p.name not in ('manifest.json', 'progress.json')]
+    artifacts.append(dict(name='manifest.json', kind='manifest', media_type='application/json'))
+    _atomic_json(out / 'manifest.json', dict(model_id=meta.get('model_id'), artifacts=artifacts))
+    return y
+
+
+def sweep(model: Model | dict, output_dir=None, *, on_progress: Optional[Callable] = None) -> Result:
+    """Run the frequency sweep of ``model`` on the GPU and return a Result.
+
+    A fresh temporary directory is used when ``output_dir`` is None; a directory that already
+    holds results is refused (``FileExistsError``), as in the reference.
+    """
+    from . import params as prm
+    from .piezo_fem import SCALE
+    from .workloads import build_engine
+    model = model if isinstance(model, Model) else Model(**model)
+    c = model.config
+    out = Path(output_dir) if output_dir else Path(tempfile.mkdtemp(prefix='sawsim_amd_'))
+    out.mkdir(parents=True, exist_ok=True)
+    if any((out / n).exists() for n in ('admittance.npz', 'manifest.json', 'mesh.npz')):
+        raise FileExistsError(f'{out} already holds results')
+    _atomic_json(out / 'input.json', model.to_dict())
+    started = time.time()
+
+    def progress(stage, done, total):
+        event = dict(stage=stage, completed=done, total=total, elapsed_seconds=round(time.time() - started, 2))
+        _atomic_json(out / 'progress.json', event)
+        if on_progress is not None:
+            on_progress(event)
+
+    cell = model.cell()
+    fre = cell.frequencies()
+    progress('assembly', 0, fre.size)
+    gradient = bool(c['gradient']) and cell.params is not None
+    eng = build_engine(cell, gradient=gradient)
+    progress('sweep', 0, fre.size)
+    V = float(c['voltage'])
+    if gradient:
+        Q, dY, info = eng.sweep_charges(fre, V, precision=c['precision'], batch=c['batch'], gradient=True)
+    else:
+        Q, info = eng.sweep_charges(fre, V, precision=c['precision'], batch=c['batch'])
+        dY = None
+    if not np.all(np.isfinite(Q)):
+        raise ValueError('non-finite charge in the sweep')
+    progress('postprocess', fre.size, fre.size)
+    y = -1j * 2 * np.pi * fre * Q / V
+    peak = int(np.argmax(np.abs(y)))
+    disp, _, _ = eng.solve(2 * np.pi * fre[peak], V)
+    n, nd = cell.model.N, cell.ndisp
+    u = np.column_stack([disp[i * n:(i + 1) * n] for i in range(nd)]) * SCALE
+    phi = disp[nd * n:(nd + 1) * n]
+    labels = ['ux', 'uy', 'uz'][:nd]
+    fields = dict(position_m=cell.model.pos_m, displacement_m=u, potential_v=phi, frequency_hz=fre[peak],
+                  quads=cell.model.quads, mesh_dimension=2, displacement_components=nd, component_labels=labels)
+    meta = dict(model_id=model.model_id, nodes=int(cell.model.N), elements=int(cell.model.ne),
+                dofs=int(eng.gdof), active_unknowns=int(eng.n_active), pml_reference_frequency_hz=float(fre[-1]),
+                backend='MI355X batched multifrontal LU (HIP, libsawfem_hip)', precision=c['precision'],
+                refinement=[{k: v for k, v in it.items() if k != 'dq_hist'} for it in info],
+                elapsed_seconds=time.time() - started, displacement_components=nd, component_labels=labels,
+                field_selection='maximum sampled admittance magnitude')
+    write_sweep_results(out, fre, Q, V, mesh=(cell.model.pos_m, cell.model.quads, cell.model.elem_tag),
+                        fields=fields, gradient=(prm.NAMES, dY) if dY is not None else None, metadata=meta)
+    progress('complete', fre.size, fre.size)
+    return Result(out)

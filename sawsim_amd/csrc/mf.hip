@@ -875,8 +875,9 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
 template <typename T, int KIND, int R, bool FULL>
 __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
                                              const typename cplx<T>::type* __restrict__ arena,
-                                             double2* __restrict__ uarena, double2* __restrict__ x) {
-  tri_body<T, KIND, R, FULL>(F, lvl, B, n, nbl, arena, uarena, x, blockIdx.y);
+                                             double2* __restrict__ uarena, double2* __restrict__ x,
+                                             const int* __restrict__ bsel) {
+  tri_body<T, KIND, R, FULL>(F, lvl, B, n, nbl, arena, uarena, x, bsel ? bsel[blockIdx.y] : (int)blockIdx.y);
 
 }
 
@@ -973,8 +974,9 @@ __device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ 
 template <typename T, int KIND, int R>
 __global__ void __launch_bounds__(256) k_panel(FrontsDev F, const int* __restrict__ lvl, int B, int n,
                                                const typename cplx<T>::type* __restrict__ arena,
-                                               double2* __restrict__ uarena, double2* __restrict__ x) {
-  panel_body<T, KIND, R>(F, lvl, B, n, arena, uarena, x, blockIdx.z);
+                                               double2* __restrict__ uarena, double2* __restrict__ x,
+                                               const int* __restrict__ bsel) {
+  panel_body<T, KIND, R>(F, lvl, B, n, arena, uarena, x, bsel ? bsel[blockIdx.z] : (int)blockIdx.z);
 
 }
 
@@ -1289,7 +1291,8 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
 }
 
 template <typename T, int R>
-int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose, int nb) {
+int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose, int nb,
+               const int* bsel, int nsel) {
   using V = typename cplx<T>::type;
   constexpr int LD = SolveTile<T>::SB + 1;   // panel kernels' transposition tile
   const int pi = sizeof(T) == 4 ? 0 : 1;
@@ -1304,8 +1307,8 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
     const int nbl = panel_width(nb, L);
     const size_t lds = sizeof(V) * 64 * (nbl + 1) + sizeof(double2) * R * (256 + (full ? S.mmax : S.pmax));
-    const dim3 grid(S.nf, B);
-#define SAW_TRI(K, FULL) hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, nbl, A, U, X)
+    const dim3 grid(S.nf, nsel);
+#define SAW_TRI(K, FULL) hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, nbl, A, U, X, bsel)
     if (full) {
       switch (kind) { case 0: SAW_TRI(0, true); break; case 1: SAW_TRI(1, true); break;
                       case 2: SAW_TRI(2, true); break; default: SAW_TRI(3, true); }
@@ -1323,12 +1326,12 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     if (S.nf == 0 || nout == 0 || nin == 0) return;
     const size_t lds = sizeof(double2) * R * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
     Timed tm(P, s, out_cb ? K_FWD : K_BWD);
-    dim3 grid((nout + 63) / 64, S.nf, B);
+    dim3 grid((nout + 63) / 64, S.nf, nsel);
     switch (kind) {
-      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X); break;
-      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X); break;
-      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X); break;
-      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X);
+      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
+      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
+      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
+      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel);
     }
   };
   const int kf = transpose ? 2 : 0, kb = transpose ? 3 : 1;
@@ -1528,12 +1531,14 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
 
 // x: [B][nrhs][n] complex128 right-hand sides (nrhs = 1 or 2 per frequency, sharing that
 // frequency's factors), overwritten with the solutions.  nb must be the value the arena was
-// factored with.  uarena: uarena_elems * B * nrhs complex128.
+// factored with.  uarena: uarena_elems * B * nrhs complex128.  bsel (device int[nsel], or
+// NULL for all B): solve only these frequencies of the chunk; the others' x is untouched.
 int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const void* arena, void* uarena, void* x,
-                    int transpose, int nb) {
+                    int transpose, int nb, const int* bsel, int nsel) {
   Plan* P = (Plan*)plan;
+  if (!bsel) nsel = B;
   if (!P || B <= 0 || !arena || !uarena || !x || (prec != 0 && prec != 1) || (nb != 32 && nb != 64) ||
-      (prec == 1 && nb > 32) || (nrhs != 1 && nrhs != 2)) {
+      (prec == 1 && nb > 32) || (nrhs != 1 && nrhs != 2) || nsel < 0 || nsel > B) {
     sawfem_set_error("sawfem_mf_solve: invalid arguments");
     return 1;
   }
@@ -1545,11 +1550,12 @@ int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const v
     return 1;
   }
   hipStream_t s = (hipStream_t)stream;
+  if (nsel == 0) return 0;
   if (nrhs == 1)
-    return prec == 0 ? solve_impl<float, 1>(P, s, B, arena, uarena, x, transpose, nb)
-                     : solve_impl<double, 1>(P, s, B, arena, uarena, x, transpose, nb);
-  return prec == 0 ? solve_impl<float, 2>(P, s, B, arena, uarena, x, transpose, nb)
-                   : solve_impl<double, 2>(P, s, B, arena, uarena, x, transpose, nb);
+    return prec == 0 ? solve_impl<float, 1>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel)
+                     : solve_impl<double, 1>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel);
+  return prec == 0 ? solve_impl<float, 2>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel)
+                   : solve_impl<double, 2>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel);
 }
 
 // Per-kernel event timing: enable (1) / disable (0); turning it on clears what was recorded.

@@ -267,7 +267,26 @@ class FrequencySolver:
     def _dptr(self, t):
         return None if t is None else t.data_ptr()
 
-    def solve_batch(self, omegas, V=1.0, precision='mixed', ir_tol=1e-9, max_ir=8, nb=None, fallback=True,
+    def _arena(self, count, dtype):
+        """Front arena of ``count`` entries, carved from one workspace kept across calls and
+        grown on demand: chunks of either precision reuse the same HBM instead of leaving the
+        allocator with cached 100 GB blocks of the wrong size."""
+        torch = self.torch
+        nbytes = count * (8 if dtype == torch.complex64 else 16)
+        ws = getattr(self, '_ws', None)
+        if ws is None or ws.numel() < nbytes:
+            self._ws = None
+            del ws
+            torch.cuda.empty_cache()
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+        return self._ws[:nbytes].view(dtype)
+
+    def release_workspace(self):
+        """Free the arena workspace (it is otherwise kept for the next solve_batch)."""
+        self._ws = None
+        self.torch.cuda.empty_cache()
+
+    def solve_batch(self, omegas, V=1.0, precision='mixed', ir_tol=1e-8, max_ir=8, nb=None, fallback=True,
                     gradient=False, adjoint='auto'):
         """Solve all ``omegas`` in one batched factorization.
 
@@ -292,8 +311,7 @@ class FrequencySolver:
         R = 2 if paired else 1
         s = stream_ptr()
         om = torch.as_tensor(omegas, device=dev)
-        arena = torch.empty(self.mf.arena_elems * B, dtype=torch.complex64 if prec == 0 else torch.complex128,
-                            device=dev)
+        arena = self._arena(self.mf.arena_elems * B, torch.complex64 if prec == 0 else torch.complex128)
         uarena = torch.empty(max(1, self.mf.uarena_elems * B * R), dtype=torch.complex128, device=dev)
         _lib.call('sawfem_mf_factor', self._plan, s, prec, B, om.data_ptr(), self.beta, self.Kaa.data_ptr(),
                   self.Maa.data_ptr(), self._dptr(self.Daa), arena.data_ptr(), nb)
@@ -323,41 +341,75 @@ class FrequencySolver:
         out['refactored'] = refactored
         return out
 
+    def _refine(self, prec, nb, arena, uarena, X, transpose, res, rhst, tslot, measure, ir_tol, max_ir):
+        """Iterative refinement of X ([B][R][n], holding the first solve) in place.
+
+        Each step: FP64 residual of every frequency, a solve of the frequencies whose estimated
+        error (:func:`ir_error` of each quantity ``measure()`` returns, per frequency) is still
+        above ``ir_tol`` (converged frequencies are left out of the solve and their correction
+        is zero), update.  Returns (values, last changes, error estimates, steps, history of the
+        largest change of the first quantity).
+        """
+        torch = self.torch
+        s = stream_ptr()
+        B = X.shape[0]
+        R = X.shape[1] if X.dim() == 3 else 1
+        vals = measure()
+        dqs = [np.full(B, np.inf) for _ in vals]
+        errs = [np.full(B, np.inf) for _ in vals]
+        active = np.ones(B, dtype=bool)
+        r = torch.empty_like(X)
+        hist, steps = [], 0
+        self._solved = [B]                 # frequencies solved per pass (first solve, then each step)
+        self._stalled = np.zeros(B, dtype=bool)
+        for steps in range(1, max_ir + 1):
+            self._solved.append(int(active.sum()))
+            res(X, rhst, r, tslot)
+            sel, nsel = None, 0
+            if not active.all():
+                r[torch.as_tensor(~active, device=self.dev)] = 0
+                idx = torch.as_tensor(np.flatnonzero(active).astype(np.int32), device=self.dev)
+                sel, nsel = idx.data_ptr(), int(idx.numel())
+            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, R, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(),
+                      transpose, nb, sel, nsel)
+            _lib.call('sawfem_axpy', s, X.numel(), X.data_ptr(), r.data_ptr())
+            new = measure()
+            stalled = np.zeros(B, dtype=bool)
+            for k in range(len(vals)):
+                dq = np.abs(new[k] - vals[k]) / np.maximum(np.abs(new[k]), 1e-300)
+                e = ir_error(dq, dqs[k])
+                # a correction that does not shrink by half after the first step: refinement with
+                # these factors does not converge here (left to the complex128 refactorization)
+                if steps >= 2:
+                    stalled |= active & (dq > 0.5 * dqs[k]) & (e > ir_tol)
+                errs[k][active] = e[active]
+                dqs[k][active] = dq[active]
+                vals[k] = new[k]
+            hist.append(float(dqs[0][active].max()))
+            active = np.any([e > ir_tol for e in errs], axis=0) & ~stalled & ~self._stalled
+            self._stalled |= stalled
+            if not active.any():
+                break
+        return vals, dqs, errs, steps, hist
+
     def _solve_single(self, om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir):
         """x = A^-1 rhs refined against the FP64 operator until Q's estimated error <= ir_tol."""
         torch = self.torch
-        dev = self.dev
         s = stream_ptr()
         B, n = rhs.shape
         x = rhs.clone()
         _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), x.data_ptr(), 0,
-                  nb)
-        Qd = torch.empty(B, dtype=torch.complex128, device=dev)
-        charge = lambda: (_lib.call('sawfem_charge', s, n, B, self.kts_a.data_ptr(), self.kts_z.real,
-                                    self.kts_z.imag, float(V), x.data_ptr(), Qd.data_ptr()),
-                          Qd.cpu().numpy().copy())[1]
-        Q = charge()
-        r = torch.empty_like(x)
+                  nb, None, 0)
+        Qd = torch.empty(B, dtype=torch.complex128, device=self.dev)
+
+        def measure():
+            _lib.call('sawfem_charge', s, n, B, self.kts_a.data_ptr(), self.kts_z.real, self.kts_z.imag, float(V),
+                      x.data_ptr(), Qd.data_ptr())
+            return [Qd.cpu().numpy().copy()]
         res = self._residual_op(B, om)
-        rhst = res.to_t(rhs)
-        steps = 0
-        dq = np.full(B, np.inf)
-        err = dq
-        hist = []
-        for steps in range(1, max_ir + 1):
-            res(x, rhst, r, None)
-            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(),
-                      0, nb)
-            _lib.call('sawfem_axpy', s, B * n, x.data_ptr(), r.data_ptr())
-            Qn = charge()
-            dq_new = np.abs(Qn - Q) / np.maximum(np.abs(Qn), 1e-300)
-            err = ir_error(dq_new, dq)
-            dq = dq_new
-            hist.append(float(dq.max()))
-            Q = Qn
-            if np.all(err <= ir_tol):
-                break
-        return dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist)
+        (Q,), (dq,), (err,), steps, hist = self._refine(prec, nb, arena, uarena, x, 0, res, res.to_t(rhs), None,
+                                                         measure, ir_tol, max_ir)
+        return dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist, solved=list(self._solved))
 
     def _solve_paired(self, om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir):
         """Forward and adjoint systems as two right-hand sides of one refined solve.
@@ -374,41 +426,21 @@ class FrequencySolver:
         X = torch.stack([rhs, (self.kts_a / d).unsqueeze(0).expand(B, n)], dim=1).contiguous()   # [B][2][n]
         rhs2 = X.clone()
         _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 2, arena.data_ptr(), uarena.data_ptr(), X.data_ptr(), 0,
-                  nb)
+                  nb, None, 0)
         Qd = torch.empty(2 * B, dtype=torch.complex128, device=self.dev)
 
-        def measures():
+        def measure():
             _lib.call('sawfem_charge', s, n, 2 * B, self.kts_a.data_ptr(), self.kts_z.real, self.kts_z.imag, float(V),
                       X.data_ptr(), Qd.data_ptr())
             mu = (X[:, 1] * d * rhs).sum(dim=1)
-            return Qd[0::2].cpu().numpy().copy(), mu.cpu().numpy()
-        Q, mu = measures()
-        om2 = om.repeat_interleave(2)
-        res = self._residual_op(2 * B, om2)
-        rhst = res.to_t(rhs2)
-        r = torch.empty_like(X)
-        dq = np.full(B, np.inf)
-        dm = np.full(B, np.inf)
-        err = aerr = dq
-        hist = []
-        steps = 0
-        for steps in range(1, max_ir + 1):
-            res(X, rhst, r, None)
-            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 2, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(),
-                      0, nb)
-            _lib.call('sawfem_axpy', s, 2 * B * n, X.data_ptr(), r.data_ptr())
-            Qn, mun = measures()
-            dq_new = np.abs(Qn - Q) / np.maximum(np.abs(Qn), 1e-300)
-            dm_new = np.abs(mun - mu) / np.maximum(np.abs(mun), 1e-300)
-            err, aerr = ir_error(dq_new, dq), ir_error(dm_new, dm)
-            dq, dm, Q, mu = dq_new, dm_new, Qn, mun
-            hist.append(float(dq.max()))
-            if np.all(err <= ir_tol) and np.all(aerr <= ir_tol):
-                break
+            return [Qd[0::2].cpu().numpy().copy(), mu.cpu().numpy()]
+        res = self._residual_op(2 * B, om.repeat_interleave(2))
+        (Q, mu), (dq, dm), (err, aerr), steps, hist = self._refine(prec, nb, arena, uarena, X, 0, res, res.to_t(rhs2),
+                                                                  None, measure, ir_tol, max_ir)
         x = X[:, 0].contiguous()
         lam = (X[:, 1] * d).contiguous()
-        del X, r, rhs2, rhst
-        out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist)
+        del X, rhs2
+        out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist, solved=list(self._solved))
         out.update(self._contract(omegas, V, x, lam))
         out.update(adj_dq=dm, adj_err=aerr, adj_steps=steps, lam=lam, paired=True)
         return out
@@ -438,36 +470,18 @@ class FrequencySolver:
         return Op()
 
     def _adjoint(self, om, omegas, V, prec, nb, arena, uarena, rhs, x, ir_tol, max_ir):
-        """lambda = A^-T kts_a (refined), then K12 and the parameter contraction."""
-        torch = self.torch
-        g = self._grad
+        """lambda = A^-T kts_a by transposed solves (refined with the transposed FP64 residual
+        until the duality product lambda^T b converges), then K12 and the parameter contraction."""
         s = stream_ptr()
         B, n = x.shape
         gvec = self.kts_a.unsqueeze(0).expand(B, n).contiguous()
         lam = gvec.clone()
         _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), lam.data_ptr(), 1,
-                  nb)
-        # duality check: lambda^T b converges to kts_a^T x
-        mu = lambda: (lam * rhs).sum(dim=1).cpu().numpy()
-        m0 = mu()
-        r = torch.empty_like(lam)
-        dq = np.full(B, np.inf)
-        err = dq
+                  nb, None, 0)
+        measure = lambda: [(lam * rhs).sum(dim=1).cpu().numpy()]
         res = self._residual_op(B, om)
-        gt = res.to_t(gvec)
-        steps = 0
-        for steps in range(1, max_ir + 1):
-            res(lam, gt, r, self.tslot)
-            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(),
-                      1, nb)
-            _lib.call('sawfem_axpy', s, B * n, lam.data_ptr(), r.data_ptr())
-            m1 = mu()
-            dq_new = np.abs(m1 - m0) / np.maximum(np.abs(m1), 1e-300)
-            err = ir_error(dq_new, dq)
-            dq = dq_new
-            m0 = m1
-            if np.all(err <= ir_tol):
-                break
+        _, (dq,), (err,), steps, _ = self._refine(prec, nb, arena, uarena, lam, 1, res, res.to_t(gvec), self.tslot,
+                                                  measure, ir_tol, max_ir)
         out = self._contract(omegas, V, x, lam)
         out.update(adj_dq=dq, adj_err=err, adj_steps=steps, lam=lam)
         return out
@@ -515,6 +529,7 @@ class FrequencySolver:
                 item['paired_adjoint'] = bool(res.get('paired', False))
             item['points'] = int(min(batch, fre.size - i0))
             item['dq_hist'] = res.get('dq_hist')
+            item['solved_per_pass'] = res.get('solved')
             info.append(item)
             del res
         if gradient:

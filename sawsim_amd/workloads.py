@@ -11,9 +11,11 @@ Workloads:
   0.08·pitch thick, metallization 0.5, 0.6 µm trigonal-3m piezo film (sp_baseline
   constants with seeded ±5 % jitter, Euler (0, -42°, 0)) over isotropic Si
   (E = 130 GPa, ν = 0.28, ρ = 2330), depth 5·pitch plus a 1·pitch bottom PML in Si,
-  plane strain ``[ux|uy|phi]`` (mesh 0.045 µm: 4312 elements, 17521 nodes).
+  plane strain ``[ux|uy|phi]`` (mesh 0.045 µm: 4312 elements, 17521 nodes); 101 points
+  over fR·[0.92, 1.08].
 * ``config2``: the same stack with a 0.3·pitch SiO₂ overlay embedding the electrodes,
-  Euler (0, -38°, 0), mesh refined 2x in each direction (16848 elements).
+  Euler (0, -38°, 0), mesh refined 2x in each direction (16848 elements); 401 points over
+  fR·[0.92, 1.08].
 * ``config3``: config 1 with Rayleigh stiffness damping β = 1e-13 s and dielectric
   loss tanδ = 1e-3 on the piezo.
 * ``double_layer``: the reference ``sp_double_layer`` template (ME1) whose |Y| curve
@@ -90,7 +92,14 @@ def _stack(mesh, mats, plane, ndisp, band, points, beta_dk=0.0, eta_eps=0.0, inf
                 ndisp=ndisp, band_hz=band, points=points, beta_dk=beta_dk, eta_eps=eta_eps, info=info or {})
 
 
-def config1(seed=0, mesh_um=0.045, points=101, band=(1.63e9, 1.915e9), beta_dk=0.0, eta_eps=0.0, euler=(0, -42, 0)):
+# Sweep bands: fR * [0.92, 1.08] around the main resonance fR of a 201-point pilot sweep over
+# 1.0-3.0 GHz (fr of `metrics.resonances`), measured once on the default cells: config 1
+# fR = 1.4286 GHz, config 2 fR = 2.5124 GHz (`pilot_resonance` recomputes them).
+BAND1 = (1.3143e9, 1.5429e9)
+BAND2 = (2.3114e9, 2.7134e9)
+
+
+def config1(seed=0, mesh_um=0.045, points=101, band=BAND1, beta_dk=0.0, eta_eps=0.0, euler=(0, -42, 0)):
     pitch = 1.085
     mesh = stack_cell(pitch, 0.5, 0.08 * pitch, 0.6, [5 * pitch - 0.6], [], mesh_um, pml_pitches=1.0)
     k = jittered_constants(seed)
@@ -101,7 +110,7 @@ def config1(seed=0, mesh_um=0.045, points=101, band=(1.63e9, 1.915e9), beta_dk=0
     return c
 
 
-def config2(seed=0, mesh_um=0.0225, points=401, band=(1.55e9, 1.85e9)):
+def config2(seed=0, mesh_um=0.0225, points=401, band=BAND2):
     pitch = 1.085
     mesh = stack_cell(pitch, 0.5, 0.08 * pitch, 0.6, [5 * pitch - 0.6], [0.3 * pitch], mesh_um, pml_pitches=1.0)
     k = jittered_constants(seed)
@@ -113,7 +122,7 @@ def config2(seed=0, mesh_um=0.0225, points=401, band=(1.55e9, 1.85e9)):
     return c
 
 
-def config3(seed=0, mesh_um=0.045, points=201, band=(1.63e9, 1.915e9)):
+def config3(seed=0, mesh_um=0.045, points=201, band=BAND1):
     c = config1(seed, mesh_um, points, band, beta_dk=1e-13, eta_eps=1e-3)
     c.info['name'] = 'config3'
     return c
@@ -176,6 +185,18 @@ def build_engine(cell: Cell, gradient=False, **kw):
             raise ValueError('this cell has no design parameters')
         eng.enable_gradient(cell.model, cell.mat_idx, cell.pml, 2 * np.pi * cell.band_hz[1], cell.params, 'xy')
     return eng
+
+
+def pilot_resonance(builder, points=201, band=(1.0e9, 3.0e9), **kw):
+    """fR of the main resonance of a cell from a pilot sweep (GPU, mixed precision refined to
+    FP64 accuracy), and the sweep band fR * [0.92, 1.08]."""
+    from .metrics import resonances
+    cell = builder(points=points, band=band, **kw)
+    eng = build_engine(cell)
+    f = cell.frequencies()
+    Q, _ = eng.sweep_charges(f, precision='mixed')
+    fr = resonances(f, -1j * 2 * np.pi * f * Q)['fr_hz']
+    return fr, (None if fr is None else (0.92 * fr, 1.08 * fr))
 
 
 def materials_for_params(cell: Cell, p):

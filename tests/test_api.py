@@ -1,0 +1,58 @@
+"""Model/sweep/Result mirror: result-directory layout (CPU) and one GPU sweep."""
+import json
+import sys
+
+import numpy as np
+import pytest
+
+from sawsim_amd.api import Model, Result, sweep, write_sweep_results
+
+
+def test_model_fields_are_checked():
+    m = Model('sp_double_layer', points=5, precision='fp64')
+    assert m.to_dict()['points'] == 5 and 'sp_quad_layer' in Model.templates()
+    with pytest.raises(ValueError):
+        Model('nope')
+    with pytest.raises(ValueError):
+        Model('config1', pitch=1.0)
+
+
+def test_result_directory_round_trip(tmp_path):
+    f = np.linspace(1.6e9, 2.0e9, 201)
+    fr, fa = 1.7512345e9, 1.8123456e9           # between samples
+    y_true = 2j * np.pi * f * 1e-12 * (fa ** 2 - f ** 2) / (fr ** 2 - f ** 2)
+    q = y_true / (-1j * 2 * np.pi * f)            # Y = -i w Q / V with V = 1
+    pos = np.zeros((9, 2))
+    write_sweep_results(tmp_path, f, q, 1.0, mesh=(pos, np.arange(9)[None], np.array([16])),
+                        gradient=(['a', 'b'], np.ones((f.size, 2))), metadata=dict(model_id='synthetic'))
+    r = Result(tmp_path)
+    assert np.allclose(r.frequency_ghz, f * 1e-9) and np.allclose(r.admittance, y_true)
+    assert r.peak_frequency_ghz == pytest.approx(f[np.argmax(np.abs(y_true))] * 1e-9)
+    names = {a['name'] for a in r.manifest['artifacts']}
+    assert {'curve.json', 'admittance.npz', 'admittance.csv', 'mesh.npz', 'gradient.npz', 'metadata.json',
+            'manifest.json'} <= names
+    z = np.load(tmp_path / 'admittance.npz')
+    assert np.allclose(z['admittance'], y_true) and np.allclose(z['charge'], q)
+    csv = np.loadtxt(tmp_path / 'admittance.csv', delimiter=',', skiprows=1)
+    assert np.allclose(csv[:, 3], np.abs(y_true))
+    res = r.resonances()
+    assert abs(res['fr_hz'] - fr) < res['uncertainty_hz'] and abs(res['fa_hz'] - fa) < res['uncertainty_hz']
+    assert r.save(tmp_path / 'copy').joinpath('curve.json').exists()
+
+
+@pytest.mark.gpu
+def test_sweep_reproduces_the_stored_double_layer_curve(tmp_path):
+    print('[test_sweep_reproduces_the_stored_double_layer_curve]', flush=True)
+    sys.stdout.flush()
+    from pathlib import Path
+    r = sweep(Model('sp_double_layer', points=5), tmp_path / 'run')
+    z = np.load(Path(__file__).parent / 'data' / 'sp_double_layer.npz')
+    ref = z['sawsim_magnitude'][::200]
+    assert np.allclose(z['frequency_hz'][::200] * 1e-9, r.frequency_ghz)
+    assert np.max(np.abs(r.magnitude - ref) / ref) <= 2e-3
+    meta = json.loads((tmp_path / 'run' / 'metadata.json').read_text())
+    assert meta['model_id'] == 'sp_double_layer' and meta['frequency_count'] == 5
+    f = np.load(tmp_path / 'run' / 'fields.npz')
+    assert f['displacement_m'].shape[1] == 3 and np.all(np.isfinite(f['potential_v']))
+    with pytest.raises(FileExistsError):
+        sweep(Model('sp_double_layer', points=5), tmp_path / 'run')

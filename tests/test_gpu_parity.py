@@ -188,3 +188,36 @@ def test_gpu_assembly_matches_reference_golden_tiny_me0():
         assert (rz - rz.multiply(mz)).nnz == 0
     eng = FrequencySolver(K, M, *bc_periodic(cell.model, cell.mesh.lines, ndisp=2))
     assert np.array_equal(eng.active, g['active'])
+
+
+def test_solve_on_a_frequency_subset():
+    """sawfem_mf_solve with a frequency list: the listed frequencies get the full solve's
+    result bit for bit, the others are left untouched."""
+    _say('[test_solve_on_a_frequency_subset]')
+    import torch
+    from sawsim_amd import _lib
+    from sawsim_amd.piezo_fem import stream_ptr
+    cell = workloads.double_layer()
+    eng = _engine(cell, 2 * np.pi * cell.band_hz[1])
+    omegas = 2 * np.pi * np.linspace(1.6e9, 2.4e9, 6)
+    B, n = omegas.size, eng.n_active
+    om = torch.as_tensor(omegas, device='cuda')
+    arena = torch.empty(eng.mf.arena_elems * B, dtype=torch.complex64, device='cuda')
+    uarena = torch.empty(eng.mf.uarena_elems * B * 2, dtype=torch.complex128, device='cuda')
+    s = stream_ptr()
+    _lib.call('sawfem_mf_factor', eng._plan, s, 0, B, om.data_ptr(), 0.0, eng.Kaa.data_ptr(), eng.Maa.data_ptr(),
+              None, arena.data_ptr(), 64)
+    g = torch.Generator(device='cpu').manual_seed(3)
+    rhs = torch.complex(torch.randn(B, 2, n, generator=g, dtype=torch.float64),
+                        torch.randn(B, 2, n, generator=g, dtype=torch.float64)).cuda()
+    full = rhs.clone()
+    _lib.call('sawfem_mf_solve', eng._plan, s, 0, B, 2, arena.data_ptr(), uarena.data_ptr(), full.data_ptr(), 0, 64,
+              None, 0)
+    part = rhs.clone()
+    sel = torch.tensor([1, 4], dtype=torch.int32, device='cuda')
+    _lib.call('sawfem_mf_solve', eng._plan, s, 0, B, 2, arena.data_ptr(), uarena.data_ptr(), part.data_ptr(), 0, 64,
+              sel.data_ptr(), 2)
+    torch.cuda.synchronize()
+    for b in range(B):
+        want = full[b] if b in (1, 4) else rhs[b]
+        assert torch.equal(part[b], want), b
