@@ -8,6 +8,9 @@
  *
  * Reference interfaces replaced (SawSim, Python):
  *   saw2d/piezo_fem.py:283-367 assemble / :373-405 assemble_ME0  -> sawfem_elem_q9 + sawfem_gather_kmd
+ *   saw2d/saw3d/assembly.py:230-327 form_stiffness_mass_3d         -> sawfem_elem_h27 + sawfem_gather_h27
+ *   saw2d/saw3d/bc_solve.py:121-175 FrequencySolver3D (two-stage fold, active set, Q rows)
+ *                                                                   -> sawfem_gather3 (fold as a representative map)
  *   saw2d/sweep.py:283-323 FrequencySolver.__init__/set_damping     -> sawfem_gather3 (Tᵀ K T, active extraction,
  *                                                                     Km@s, Kᵀs)
  *   saw2d/sweep.py:326-333, 349-353 _backend / PARDISO / SuperLU   -> sawfem_mf_create, sawfem_mf_factor,
@@ -44,6 +47,25 @@ int sawfem_elem_q9(void* stream, int ne, int ndu, const int* conn, const double*
 int sawfem_gather_kmd(void* stream, int nnzn, int nd, int ndu, const int* pair_row, const int64_t* node_ptr,
                       const int64_t* gptr, const int64_t* gidx, const unsigned char* dmask, const void* Ke,
                       const void* Me, void* K, void* M, void* D);
+
+/* K1' (Hex27): element matrices of the 27-node slab.  conn (ne*27, reference H27 node order),
+ * xyz (N*3, metres), mat_idx/pml_idx per element (pml_idx -1: no stretch); record tensors
+ * C (nrec*36), E (nrec*18), EPS (nrec*9) complex128 row-major, rho (nrec) real; npml <= 8 PML
+ * parameter sets: kind (bit 0: x stretch, bit 1: z stretch) and par (6 doubles: dmax, n, xa, xp,
+ * za, zp); the stretch is taken at local node 26 and frozen at omega_top.  scale = KM_SCALE.
+ * Writes Ke[e][A*4+B][a*27+b] (ne*16*729) and Me[e][a*27+b] (ne*729), complex128.
+ * Replaces saw3d/assembly.py:31-308 (_geometry, _b_matrix with the gamma_yz quirk, _bfi_matrix,
+ * _stretch_alphas, _c/_e/_eps_stretch, _pml_fields, B^T D B products, conj(K_uphi)^T). */
+int sawfem_elem_h27(void* stream, int ne, const int* conn, const double* xyz, const int* mat_idx, const int* pml_idx,
+                    const void* C, const void* E, const void* EPS, const double* rho, int npml, const int* pml_kind,
+                    const double* pml_par, double omega_top, double scale, void* Ke, void* Me);
+
+/* K2' (Hex27): K, M (and D = K_uu of elements with dmask[e] != 0, if D != NULL) on the 27-node
+ * structural pattern, 4 components, block-major slots; codes e*729 + i*27 + j per node pair.
+ * Replaces the sparse products and _scatter_mass of saw3d/assembly.py:288-327. */
+int sawfem_gather_h27(void* stream, int nnzn, const int* pair_row, const int64_t* node_ptr, const int64_t* gptr,
+                      const int64_t* gidx, const unsigned char* dmask, const void* Ke, const void* Me, void* K, void* M,
+                      void* D);
 
 /* K3: d_i[j] = sum_{c in ptr[j]..ptr[j+1]} s_i[idx[c]] for up to three complex128 sources
  * sharing one list (s1/d1, s2/d2 may be NULL together).  Used for Tᵀ A T restricted to the

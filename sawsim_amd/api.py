@@ -15,8 +15,11 @@ backend builds itself (the structured SP templates and the synthetic stack cells
   counterpart in the reference),
 * ``metadata.json`` and ``manifest.json`` (artifact list).
 
-The admittance convention is the reference's passive sign, Y = -i w Q / V.  Images, the
-material snapshot and Gmsh meshes are not produced.
+The admittance convention is the reference's passive sign, Y = -i w Q / V, for the plane
+cells.  The Hex27 slab templates follow ``run_hex_simulation`` (``src/sawsim/hex_solver.py:88-120``):
+Y = +i w Q / V, and charge and admittance per unit aperture (raw slice values kept as
+``raw_charge_c``/``raw_admittance_s``).  Images, the material snapshot and Gmsh meshes are not
+produced.
 """
 from __future__ import annotations
 
@@ -35,9 +38,16 @@ TEMPLATES = {
     'config1': 'sp_stack-style 3m film on Si, plane strain (ME0)',
     'config2': 'config1 with a SiO2 overlay, refined mesh (ME0)',
     'config3': 'config1 with Rayleigh damping and dielectric loss (ME0)',
+    'sp_2p5d_single_layer': 'Hex27 slab, piezo on its own PML (reference template)',
+    'sp_2p5d_double_layer': 'Hex27 slab, piezo on cubic Si (reference template)',
+    'sp_2p5d_triple_layer': 'Hex27 slab, piezo / SiO2 / poly-Si (reference template)',
+    'sp_2p5d_quad_layer': 'Hex27 slab, piezo / SiO2 / poly-Si / anisotropic Si (reference template)',
+    'config4': 'Hex27 slab, 3m film / SiO2 / poly-Si / Si, ~8k elements',
 }
 _BUILDERS = {'sp_double_layer': 'double_layer', 'sp_quad_layer': 'quad_layer', 'config1': 'config1',
-             'config2': 'config2', 'config3': 'config3'}
+             'config2': 'config2', 'config3': 'config3', 'config4': 'config4'}
+_HEX = {'sp_2p5d_single_layer': 'single', 'sp_2p5d_double_layer': 'double', 'sp_2p5d_triple_layer': 'triple',
+        'sp_2p5d_quad_layer': 'quad'}
 _DEFAULTS = dict(points=None, start_ghz=None, stop_ghz=None, voltage=1.0, precision='mixed', gradient=False,
                  batch=None)
 
@@ -77,10 +87,16 @@ class Model:
         """The workload cell (mesh, materials, band) this model sweeps."""
         from . import workloads
         c = self.config
-        builder = workloads.WORKLOADS[_BUILDERS[self.model_id]]
+        if self.model_id in _HEX:
+            name = _HEX[self.model_id]
+            builder = lambda **kw: workloads.hex_template(name, **kw)
+        else:
+            builder = workloads.WORKLOADS[_BUILDERS[self.model_id]]
         kw = {}
         if c['points']:
             kw['points'] = int(c['points'])
+        if self.model_id in _HEX:
+            kw.setdefault('points', 201)
         cell = builder(**kw)
         if c['start_ghz'] is not None or c['stop_ghz'] is not None:
             lo = cell.band_hz[0] if c['start_ghz'] is None else c['start_ghz'] * 1e9
@@ -179,6 +195,60 @@ def write_sweep_results(out: Path, fre, charge, voltage, *, mesh=None, fields=No
     return y
 
 
+def _sweep_hex(model, cell, fre, out, progress, started):
+    """Hex27 slab sweep and artifacts (``hex_solver.py:88-120``): Y = +i w Q / V per unit aperture."""
+    from .hex27 import KM_SCALE
+    from .workloads import build_hex_engine
+    c = model.config
+    eng = build_hex_engine(cell)
+    progress('sweep', 0, fre.size)
+    V = float(c['voltage'])
+    Q, info = eng.sweep_charges(fre, V, precision=c['precision'], batch=c['batch'])
+    if not np.all(np.isfinite(Q)):
+        raise ValueError('non-finite charge in the sweep')
+    progress('postprocess', fre.size, fre.size)
+    ap = cell.aperture_m
+    raw_y = 1j * 2 * np.pi * fre * Q / V
+    y = raw_y / ap
+    _atomic_json(out / 'curve.json', dict(frequency_ghz=(fre * 1e-9).tolist(), real=y.real.tolist(),
+                                          imag=y.imag.tolist(), magnitude=np.abs(y).tolist()))
+    np.savez_compressed(out / 'admittance.npz', frequency_hz=fre, charge=Q / ap, admittance=y, raw_charge_c=Q,
+                        raw_admittance_s=raw_y, aperture_m=ap)
+    np.savetxt(out / 'admittance.csv', np.column_stack([fre, y.real, y.imag, np.abs(y), raw_y.real, raw_y.imag,
+                                                        Q.real, Q.imag]), delimiter=',',
+               header='frequency_hz,real_y_s_per_m,imag_y_s_per_m,abs_y_s_per_m,raw_real_y_s,raw_imag_y_s,'
+                      'raw_real_reaction_q_c,raw_imag_reaction_q_c', comments='')
+    mesh = cell.mesh
+    quads, qtags = mesh.front_section()
+    np.savez_compressed(out / 'mesh.npz', position_m=mesh.pos_m, hexas27=mesh.hexas, material_tags=mesh.hex_tag,
+                        quads=quads, preview_material_tags=qtags, mesh_view_axes=[0, 2])
+    peak = int(np.argmax(np.abs(y)))
+    disp, _, _ = eng.solve(2 * np.pi * fre[peak], V)
+    n = mesh.N
+    np.savez_compressed(out / 'fields.npz', position_m=mesh.pos_m, displacement_m=disp[:3 * n].reshape(n, 3) * KM_SCALE,
+                        potential_v=disp[3 * n:], frequency_hz=fre[peak], hexas27=mesh.hexas, quads=quads,
+                        mesh_dimension=3, displacement_components=3, component_labels=['ux', 'uy', 'uz'],
+                        mesh_view_axes=[0, 2])
+    meta = dict(model_id=model.model_id, model='2.5D periodic thin slice / Hex27', mesh_dimension=3,
+                frequency_count=int(fre.size), peak_frequency_ghz=float(fre[peak] * 1e-9),
+                peak_is_boundary=peak in (0, fre.size - 1), aperture_m=ap, aperture_um=ap * 1e6,
+                nodes=int(n), elements=int(mesh.ne), dofs=int(4 * n), active_dofs=int(eng.n_active),
+                pml_reference_frequency_hz=float(cell.band_hz[1]),
+                backend='MI355X batched multifrontal LU (HIP, libsawfem_hip)', precision=c['precision'],
+                sign_convention='Y = +i w Q / V (reaction charge of the potential equation)',
+                normalization='charge = raw_charge_c / aperture_m; admittance = raw_admittance_s / aperture_m',
+                units={'frequency': 'Hz', 'position': 'm', 'displacement': 'm', 'potential': 'V',
+                       'admittance': 'S/m', 'raw_admittance_s': 'S', 'raw_charge_c': 'C reaction'},
+                refinement=[{k: v for k, v in it.items() if k != 'dq_hist'} for it in info],
+                elapsed_seconds=time.time() - started, field_selection='maximum sampled admittance magnitude')
+    _atomic_json(out / 'metadata.json', meta)
+    media = {'.json': 'application/json', '.csv': 'text/csv', '.npz': 'application/octet-stream'}
+    artifacts = [dict(name=p.name, kind='data', media_type=media[p.suffix]) for p in sorted(out.iterdir())
+                 if p.suffix in media and p.name not in ('manifest.json', 'progress.json')]
+    artifacts.append(dict(name='manifest.json', kind='manifest', media_type='application/json'))
+    _atomic_json(out / 'manifest.json', dict(model_id=model.model_id, artifacts=artifacts))
+
+
 def sweep(model: Model | dict, output_dir=None, *, on_progress: Optional[Callable] = None) -> Result:
     """Run the frequency sweep of ``model`` on the GPU and return a Result.
 
@@ -206,6 +276,11 @@ def sweep(model: Model | dict, output_dir=None, *, on_progress: Optional[Callabl
     cell = model.cell()
     fre = cell.frequencies()
     progress('assembly', 0, fre.size)
+    from .workloads import HexCell
+    if isinstance(cell, HexCell):
+        _sweep_hex(model, cell, fre, out, progress, started)
+        progress('complete', fre.size, fre.size)
+        return Result(out)
     gradient = bool(c['gradient']) and cell.params is not None
     eng = build_engine(cell, gradient=gradient)
     progress('sweep', 0, fre.size)

@@ -57,32 +57,55 @@ class GatherList:
 class KLayout:
     """Structural CSR pattern of K/M/D and the element -> slot gather (K1/K2 kernels).
 
-    Element matrices are written by the element kernel as ``Ke[e, a, b, i, j]`` (complex,
-    component blocks of 9x9 node entries) and ``Me[e, i, j]``.  The global matrices are
-    then gathered per *node pair* ``k`` (an entry of the node adjacency) from the list of
-    ``(e, i, j)`` codes ``e*81 + i*9 + j`` of the elements that couple that node pair.
+    Element matrices are written by the element kernels as ``Ke[e, A, B, i, j]`` (complex,
+    component blocks of npe x npe node entries, npe = 9 for Q9 and 27 for Hex27) and
+    ``Me[e, i, j]``.  The global matrices are then gathered per *node pair* ``k`` (an entry
+    of the node adjacency) from the list of ``(e, i, j)`` codes ``e*npe² + i*npe + j`` of the
+    elements that couple that node pair.
+
+    Storage is block-major: slot = A*nd*nnz_node + nd*node_ptr[i] + B*deg[i] + (k - node_ptr[i]).
+    ``numbering`` maps (component A, node) to a global dof: ``'component'`` = A*N + node
+    (the 2D reference, ``piezo_fem.py:11``), ``'interleaved'`` = 3*node + A for the three
+    displacements and 3N + node for the potential (the Hex27 reference, ``saw3d/assembly.py:4-5``).
     """
 
-    def __init__(self, quads: np.ndarray, N: int, nd: int):
+    def __init__(self, quads: np.ndarray, N: int, nd: int, numbering: str = 'component'):
         quads = np.asarray(quads, dtype=np.int64)
-        ne = quads.shape[0]
-        self.N, self.nd, self.ne = int(N), int(nd), int(ne)
-        ri = np.repeat(quads, 9, axis=1)                 # (ne,81) row node of local (i,j)
-        ci = np.tile(quads, (1, 9))                       # (ne,81) col node
+        ne, npe = quads.shape
+        if numbering not in ('component', 'interleaved') or (numbering == 'interleaved' and nd != 4):
+            raise ValueError(f'unsupported numbering {numbering!r} for nd={nd}')
+        self.N, self.nd, self.ne, self.npe = int(N), int(nd), int(ne), int(npe)
+        self.numbering = numbering
+        ri = np.repeat(quads, npe, axis=1)               # (ne,npe²) row node of local (i,j)
+        ci = np.tile(quads, (1, npe))                     # (ne,npe²) col node
         keys = (ri * N + ci).ravel()
         ukeys, inv = np.unique(keys, return_inverse=True)
         self.node_ptr, self.node_idx = _csr_from_sorted_keys(ukeys, N, N)
         self.nnz_node = int(ukeys.size)
         self.pair_row = (ukeys // N).astype(np.int64)     # row node of each node pair
         self.pair_col = (ukeys % N).astype(np.int64)
-        self.elem_pair = inv.reshape(ne, 81).astype(np.int64)   # node-pair slot per element entry
+        self.elem_pair = inv.reshape(ne, npe * npe).astype(np.int64)   # node-pair slot per element entry
         ptr, idx = _gather_lists(self.elem_pair.ravel(), self.nnz_node)
-        self.pair_gather = GatherList(ptr, idx)           # codes e*81 + i*9 + j
+        self.pair_gather = GatherList(ptr, idx)           # codes e*npe² + i*npe + j
         deg = np.diff(self.node_ptr)
         self.deg = deg
-        # K slot of (a, i, b, j) = a*nd*nnz_node + nd*node_ptr[i] + b*deg[i] + (k - node_ptr[i])
         self.nnz = nd * nd * self.nnz_node
         self.gdof = nd * N
+
+    def dof(self, a, node):
+        """Global dof of component ``a`` at ``node``."""
+        node = np.asarray(node, dtype=np.int64)
+        if self.numbering == 'component':
+            return a * self.N + node
+        return 3 * node + a if a < 3 else 3 * self.N + node
+
+    def node_comp(self, dofs):
+        """(node, component) of global dofs."""
+        dofs = np.asarray(dofs, dtype=np.int64)
+        if self.numbering == 'component':
+            return dofs % self.N, dofs // self.N
+        u = dofs < 3 * self.N
+        return np.where(u, dofs // 3, dofs - 3 * self.N), np.where(u, dofs % 3, 3)
 
     def slot(self, a, b, pair):
         """K slot of component block (a, b) at node-pair slot(s) ``pair``."""
@@ -91,7 +114,10 @@ class KLayout:
                 + (pair - self.node_ptr[i]))
 
     def csr(self):
-        """(indptr, indices) of the gdof x gdof structural pattern."""
+        """(indptr, indices) of the gdof x gdof structural pattern (component numbering:
+        the slots are already in CSR order)."""
+        if self.numbering != 'component':
+            raise ValueError('csr() is the slot order of the component numbering; use to_scipy()')
         nd, N = self.nd, self.N
         indptr = np.zeros(nd * N + 1, dtype=np.int64)
         rowlen = nd * self.deg
@@ -103,17 +129,26 @@ class KLayout:
                 indices[self.slot(a, b, pair)] = b * N + self.pair_col
         return indptr, indices
 
+    def to_scipy(self, values):
+        """SciPy CSR of slot values (all structural slots kept)."""
+        import scipy.sparse as sp
+        if self.numbering == 'component':
+            indptr, indices = self.csr()
+            return sp.csr_matrix((values, indices, indptr), shape=(self.gdof, self.gdof))
+        rows, cols = self.slot_rows_cols()
+        return sp.csr_matrix((values, (rows, cols)), shape=(self.gdof, self.gdof))
+
     def slot_rows_cols(self):
         """Row and column dof of every K slot."""
-        nd, N = self.nd, self.N
+        nd = self.nd
         rows = np.empty(self.nnz, dtype=np.int64)
         cols = np.empty(self.nnz, dtype=np.int64)
         pair = np.arange(self.nnz_node)
         for a in range(nd):
             for b in range(nd):
                 s = self.slot(a, b, pair)
-                rows[s] = a * N + self.pair_row
-                cols[s] = b * N + self.pair_col
+                rows[s] = self.dof(a, self.pair_row)
+                cols[s] = self.dof(b, self.pair_col)
         return rows, cols
 
 
@@ -164,12 +199,19 @@ class BlochLayout:
       and the zheng/fu values are written last (``sweep.py:356-362``).
     """
 
-    def __init__(self, kl: KLayout, fix, zheng, fu, groups):
+    def __init__(self, kl: KLayout, fix, zheng, fu, groups=None, rep=None):
         gdof = kl.gdof
         self.gdof = gdof
         self.zheng = np.asarray(zheng, dtype=np.int64)
         self.fu = np.asarray(fu, dtype=np.int64)
-        rep, self.slaves, self.masters = union_find_merge(gdof, groups)
+        if rep is None:
+            rep, self.slaves, self.masters = union_find_merge(gdof, groups)
+        else:
+            rep = np.asarray(rep, dtype=np.int64)
+            if rep.shape != (gdof,) or (rep[rep] != rep).any():
+                raise ValueError('rep must map every dof to a representative that maps to itself')
+            self.slaves = np.flatnonzero(rep != np.arange(gdof))
+            self.masters = rep[self.slaves]
         self.rep = rep
         prescribed = np.unique(np.concatenate([np.asarray(fix, dtype=np.int64), self.slaves,
                                                self.zheng, self.fu]))
@@ -206,9 +248,7 @@ class BlochLayout:
         ptr, order = _gather_lists(tgt, n + 1)
         self.kts = GatherList(ptr, np.concatenate([sel, selz])[order])
         # merged-node view of the active unknowns (supervariables for the ordering)
-        N = kl.N
-        self.node_of = self.active % N
-        self.comp_of = self.active // N
+        self.node_of, self.comp_of = kl.node_comp(self.active)
 
     def expand(self, x, V=1.0):
         """Full-length displacement/potential vector from the active solution (``sweep.py:356-360``)."""

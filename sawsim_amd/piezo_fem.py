@@ -90,9 +90,7 @@ class DeviceCSR:
         return self.layout.nnz
 
     def to_scipy(self):
-        import scipy.sparse as sp
-        indptr, indices = self.layout.csr()
-        return sp.csr_matrix((self.values.cpu().numpy(), indices, indptr), shape=self.shape)
+        return self.layout.to_scipy(self.values.cpu().numpy())
 
 
 class _Geometry:

@@ -84,6 +84,22 @@ class FrequencySolver:
 
     def __init__(self, K: DeviceCSR, M: DeviceCSR, fix, zheng, fu, merge_groups, *, node_xy=None, period=None,
                  leaf_weight=96):
+        bl = BlochLayout(K.layout, fix, zheng, fu, merge_groups)
+        xy = node_xy if node_xy is not None else getattr(K, 'node_xy', None)
+        if xy is None:
+            raise ValueError('node coordinates are needed for the nested-dissection ordering')
+        xy = np.asarray(xy)
+        if period is None and len(merge_groups):
+            g = np.asarray(merge_groups[0]).ravel()
+            period = float(abs(xy[g[1] % K.layout.N, 0] - xy[g[0] % K.layout.N, 0])) or None
+        self._setup(K, M, bl, bl.node_of, xy, period, leaf_weight)
+
+    def _setup(self, K, M, bl, sv_of, sv_xy, period, leaf_weight):
+        """Device copies of the reduction maps, the active operators and the numeric plan.
+
+        ``sv_of`` groups the active unknowns into supervariables (ids into ``sv_xy``, the
+        coordinates the nested dissection cuts on); ``period`` is the Bloch period in x.
+        """
         torch = _torch()
         self.torch = torch
         dev = self.dev = torch.device('cuda')
@@ -91,7 +107,7 @@ class FrequencySolver:
         if M.layout is not kl:
             raise ValueError('K and M must share one layout')
         self.gdof = kl.gdof
-        bl = self.bloch = BlochLayout(kl, fix, zheng, fu, merge_groups)
+        self.bloch = bl
         self.zheng, self.fu = bl.zheng, bl.fu
         self.active, self.slaves, self.masters = bl.active, bl.slaves, bl.masters
         n = self.n_active = bl.n_active
@@ -109,14 +125,7 @@ class FrequencySolver:
         self.update_operator(K, M)
         # symbolic analysis and numeric plan
         t0 = time.time()
-        xy = node_xy if node_xy is not None else getattr(K, 'node_xy', None)
-        if xy is None:
-            raise ValueError('node coordinates are needed for the nested-dissection ordering')
-        xy = np.asarray(xy)
-        if period is None and len(merge_groups):
-            g = np.asarray(merge_groups[0]).ravel()
-            period = float(abs(xy[g[1] % kl.N, 0] - xy[g[0] % kl.N, 0])) or None
-        self.symbolic = analyse(bl.aa_indptr, bl.aa_indices, bl.node_of, bl.comp_of, xy,
+        self.symbolic = analyse(bl.aa_indptr, bl.aa_indices, sv_of, bl.comp_of, np.asarray(sv_xy),
                                 period=period, leaf_weight=leaf_weight)
         self.mf = build_mf_arrays(self.symbolic, bl.aa_indptr, bl.aa_indices)
         self.analysis_seconds = time.time() - t0

@@ -158,7 +158,135 @@ def quad_layer(points=5, band=(1.5e9, 2.7e9)):
     return _stack(mesh, mats, None, 3, band, points, info=dict(name='sp_quad_layer'))
 
 
-WORKLOADS = dict(config1=config1, config2=config2, config3=config3, double_layer=double_layer, quad_layer=quad_layer)
+# ------------------------------------------------------------------ Hex27 slabs
+@dataclass
+class HexCell:
+    """A Hex27 slab cell: mesh, material records (piezo, electrode, backing layers from the top
+    down; the PML is the last layer, or the piezo without layers), sweep band."""
+    mesh: object
+    materials: list
+    band_hz: tuple
+    points: int
+    beta_dk: float = 0.0
+    eta_eps: float = 0.0
+    info: dict = field(default_factory=dict)
+    params: np.ndarray = None
+
+    ndisp = 3
+
+    @property
+    def aperture_m(self):
+        return self.mesh.info['aperture_m']
+
+    def frequencies(self, points=None):
+        return np.linspace(self.band_hz[0], self.band_hz[1], points or self.points)
+
+    def material_list(self, materials=None):
+        """Reference-style material records (``hex_models.py:194-196``), PML elements excluded."""
+        mats = materials if materials is not None else self.materials
+        tm, tag_pml = self.mesh.info['tag_map'], self.mesh.info['tag_pml']
+        return [dict(rho=mats[i].rho, C=mats[i].C, e=mats[i].e, eps=mats[i].eps,
+                     indx=np.flatnonzero(self.mesh.hex_tag == tag)) for tag, i in tm.items() if tag != tag_pml]
+
+    def pml_list(self, materials=None):
+        """Bottom PML record: region 3 (z stretch) between the bottom and 2 pitch above it,
+        dmax 1e11, n 3, with the PML region's own material (``hex_models.py:197-202``)."""
+        mats = materials if materials is not None else self.materials
+        info = self.mesh.info
+        pm = mats[info['tag_map'][info['tag_pml']]]
+        bottom = float(self.mesh.pos_m[self.mesh.sets['bottom'], 2].min())
+        return [dict(region=3, indx=np.flatnonzero(self.mesh.hex_tag == info['tag_pml']),
+                     za=bottom + info['pml_thickness_m'], zp=bottom, xa=0.0, xp=0.0, dmax=1e11, n=3,
+                     rho=pm.rho, C=pm.C, e=pm.e, eps=pm.eps)]
+
+
+def _cubic(c11, c12, c44, rho, eps_r):
+    C = np.zeros((6, 6))
+    C[:3, :3] = c12
+    C[np.arange(3), np.arange(3)] = c11
+    C[np.arange(3, 6), np.arange(3, 6)] = c44
+    return mt.Material(rho, C, np.zeros((3, 6)), np.eye(3) * eps_r * mt.EPS0_LEGACY)
+
+
+def hex_record_materials(euler, electrode, layers):
+    """sp_baseline rotated by ``euler``, the electrode record and the layer records of the
+    reference's slab templates (builtin legacy records, eps0 = 8.854e-12)."""
+    C = np.array([[232966000000.0, 46890400000.0, 80234200000.0, -11026700000.0, 0, 0],
+                  [46890400000.0, 232966000000.0, 80234200000.0, 11026700000.0, 0, 0],
+                  [80234200000.0, 80234200000.0, 275364000000.0, 0, 0, 0],
+                  [-11026700000.0, 11026700000.0, 0, 93899500000.0, 0, 0],
+                  [0, 0, 0, 0, 93899500000.0, -11026700000.0],
+                  [0, 0, 0, 0, -11026700000.0, 93038000000.0]])
+    e = np.array([[0, 0, 0, 0, 2.59576, -1.58923], [-1.58923, 1.58923, 0, 2.59576, 0, 0],
+                  [0.0821598, 0.0821598, 1.88197, 0, 0, 0]])
+    eps = np.diag([3.6212859999999994e-10, 3.6212859999999994e-10, 3.833781999999999e-10])
+    Cr, er, epsr = mt.rotate(euler, C, e, eps)
+    G = 1e9
+    aniso = np.array([[166, 64, 64, 0, 0, 0], [64, 195, 35, 0, 0, 0], [64, 35, 195, 0, 0, 0],
+                      [0, 0, 0, 51, 0, 0], [0, 0, 0, 0, 80, 0], [0, 0, 0, 0, 0, 80]]) * G
+    lib = dict(al=_al(), al_hex35=mt.isotropic(70e9, 0.35, 2700.0, 1.0, eps0=mt.EPS0_LEGACY),
+               si_cubic_hex=_cubic(166 * G, 64 * G, 80 * G, 2330.0, 11.68),
+               si_aniso_hex=mt.Material(2330.0, aniso, np.zeros((3, 6)), np.eye(3) * 11.8 * mt.EPS0_LEGACY),
+               sio2=mt.isotropic(70e9, 0.17, 2200.0, 4.2, eps0=mt.EPS0_LEGACY),
+               polysi=mt.isotropic(169e9, 0.22, 2320.0, 4.5, eps0=mt.EPS0_LEGACY),
+               polysi_160=mt.isotropic(160e9, 0.22, 2320.0, 4.5, eps0=mt.EPS0_LEGACY))
+    return [mt.Material(7450.0, Cr, er, epsr), lib[electrode]] + [lib[x] for x in layers]
+
+
+# The reference's four slab templates (tests/data/sp_2p5d_*_layer.json configurations).
+HEX_TEMPLATES = {
+    'single': dict(pitch=1.085, aperture=0.217, electrode=0.17, substrate=6.51, mesh=0.217, layers=[],
+                   electrode_mat='al', band=(1.5e9, 2.7e9)),
+    'double': dict(pitch=1.085, aperture=0.217, electrode=0.17, substrate=0.6, mesh=0.217,
+                   layers=[('si_cubic_hex', 6.51)], electrode_mat='al', band=(1.75e9, 2.0e9)),
+    'triple': dict(pitch=1.085, aperture=0.217, electrode=0.17, substrate=0.6, mesh=0.217,
+                   layers=[('sio2', 0.5), ('polysi', 6.51)], electrode_mat='al', band=(1.6e9, 2.0e9)),
+    'quad': dict(pitch=1.0, aperture=0.25, electrode=0.17, substrate=0.6, mesh=0.2,
+                 layers=[('sio2', 0.5), ('polysi_160', 1.0), ('si_aniso_hex', 6.0)], electrode_mat='al_hex35',
+                 band=(1.8e9, 2.1e9)),
+}
+
+
+def hex_template(name, points=5, band=None):
+    """The reference's sp_2p5d_<name>_layer slab (Euler (0, 48, 0), metallization 0.5)."""
+    from .hexmesh import template_slab
+    t = HEX_TEMPLATES[name]
+    mesh = template_slab(name, t['pitch'], t['aperture'], t['electrode'], t['substrate'], 0.5, t['mesh'],
+                         [th for _, th in t['layers']])
+    mats = hex_record_materials((0, 48, 0), t['electrode_mat'], [m for m, _ in t['layers']])
+    return HexCell(mesh=mesh, materials=mats, band_hz=band or t['band'], points=points,
+                   info=dict(name=f'sp_2p5d_{name}_layer'))
+
+
+# config 4: fR of a 201-point pilot over 1.0-3.0 GHz (see pilot_resonance), band fR * [0.92, 1.08]
+BAND4 = (1.6e9, 2.0e9)
+
+
+def config4(seed=0, mesh_um=0.047, points=201, band=BAND4, euler=(0, 52, 0), beta_dk=0.0, eta_eps=0.0):
+    """Hex27 slab, one element across a 0.217 µm aperture: pitch 1.085 µm, Al electrodes
+    0.08 pitch thick (metallization 0.5) on a 0.6 µm 3m film (sp_baseline constants with seeded
+    ±5 % jitter; Euler (0, 52°, 0) is config 2's (0, -38°, 0) cut in slab axes, z = depth) over
+    0.5 µm SiO2, 0.3 µm poly-Si (E = 169 GPa) and Si down to 5 pitch, plus a 2 pitch PML in Si.
+    Element layers of about ``mesh_um`` everywhere (≈ 8k Hex27 elements at 0.047 µm)."""
+    from .hexmesh import slab_cell, gmsh_segments
+    pitch = 1.085
+    nz = lambda h: max(1, int(np.ceil(h / mesh_um - 1e-12)))
+    h_si = 5 * pitch - 0.6 - 0.5 - 0.3
+    layers = [(0.5, nz(0.5)), (0.3, nz(0.3)), (h_si, nz(h_si))]
+    d = 0.08 * pitch
+    mesh = slab_cell(pitch, 0.217, d, 0.5, layers, 2 * pitch, nz(2 * pitch), nz(0.6), 0.6, nz(d), mesh_um)
+    k = jittered_constants(seed)
+    pz = piezo_material(k, euler, eta_eps, plane=None)
+    mats = [pz, _al(), mt.isotropic(72e9, 0.17, 2200.0, 4.2), mt.isotropic(169e9, 0.22, 2320.0, 4.5),
+            mt.isotropic(130e9, 0.28, 2330.0, 11.7)]
+    c = HexCell(mesh=mesh, materials=mats, band_hz=band, points=points, beta_dk=beta_dk, eta_eps=eta_eps,
+                info=dict(name='config4', constants=k, euler=euler, seed=seed, mesh_um=mesh_um))
+    c.params = np.append(from_constants(k, euler, beta_dk, eta_eps), 7450.0)
+    return c
+
+
+WORKLOADS = dict(config1=config1, config2=config2, config3=config3, double_layer=double_layer, quad_layer=quad_layer,
+                 config4=config4)
 
 
 def assemble_cell(cell: Cell, materials=None):
@@ -174,9 +302,26 @@ def assemble_cell(cell: Cell, materials=None):
                            damping_mask=mask)
 
 
+def build_hex_engine(cell: HexCell, **kw):
+    """GPU FrequencySolver3D for a slab cell (damping D = K_uu of the piezo)."""
+    from . import hex27
+    mesh = cell.mesh
+    omega_top = 2 * np.pi * cell.band_hz[1]
+    K, M, D = hex27.assemble_h27(mesh.hexas, mesh.pos_m, cell.material_list(), cell.pml_list(), omega_top,
+                                 damping=[True] + [False] * (len(cell.material_list()) - 1))
+    bnd = hex27.boundary_idx_vect_3ds(hex27.find_saw_boundary_3ds(mesh), mesh.pos_m, 3 * mesh.N)
+    eng = hex27.FrequencySolver3D(K, M, bnd, **kw)
+    eng.set_damping(D, cell.beta_dk)
+    return eng
+
+
 def build_engine(cell: Cell, gradient=False, **kw):
     """GPU FrequencySolver for a cell, with damping and (optionally) the dY11/dp context."""
     from .sweep import FrequencySolver, bc_periodic
+    if isinstance(cell, HexCell):
+        if gradient:
+            raise NotImplementedError('dY11/dp of Hex27 cells is not built')
+        return build_hex_engine(cell, **kw)
     K, M, D = assemble_cell(cell)
     eng = FrequencySolver(K, M, *bc_periodic(cell.model, cell.mesh.lines, ndisp=cell.ndisp), **kw)
     eng.set_damping(D, cell.beta_dk)
