@@ -61,15 +61,29 @@ def profile_read(eng):
     return {n: (float(ms[i]), int(cnt[i])) for i, n in enumerate(KINDS)}, per_level
 
 
+def _oracle_engine(cell):
+    from sawsim_amd.workloads import HexCell
+    if isinstance(cell, HexCell):
+        from oracle import fem3d
+        mesh = cell.mesh
+        M, K = fem3d.assemble(mesh.hexas, mesh.pos_m, cell.material_list(), cell.pml_list(),
+                              2 * np.pi * cell.band_hz[1])
+        m, el = fem3d.boundary_dofs(mesh.sets, mesh.pos_m, 3 * mesh.N)
+        eng = fem3d.Solver3D(K, M, m, el)
+        eng.n_active = eng.active.size
+        return eng
+    sys.path.insert(0, str(ROOT / 'tests'))
+    import helpers
+    return helpers.oracle_engine(cell, 2 * np.pi * cell.band_hz[1])
+
+
 def cpu_baseline(cell, workers=8):
     """The oracle (SciPy SuperLU, one complex sparse LU + solve per frequency, the reference's
     per-frequency direct solve) on a bounded sample of the workload: one frequency per worker
     of a fork pool, ``workers`` frequencies spread over the band."""
-    sys.path.insert(0, str(ROOT / 'tests'))
-    import helpers
     from oracle import sweep as osweep
     t0 = time.time()
-    eng = helpers.oracle_engine(cell, 2 * np.pi * cell.band_hz[1])
+    eng = _oracle_engine(cell)
     setup = time.time() - t0
     f = cell.frequencies()
     k = max(1, min(workers, os.cpu_count() or 1, f.size))
@@ -87,10 +101,14 @@ def cpu_baseline(cell, workers=8):
 
 def plain_fp64(eng, cell, freqs, pp):
     """The plain FP64 GPU path, one frequency per factorization, on pp points of the sweep:
-    Y11 only, Y11 with its FP64 adjoint, and Y11 with central differences for the 17 parameters
-    (2P+1 = 35 operator rebuilds and solves, measured on one point).  Points per second each."""
+    Y11 only, Y11 with its FP64 adjoint, and Y11 with central differences for the P parameters
+    (2P+1 operator rebuilds and solves, measured on one point).  Points per second each."""
     import torch
     from sawsim_amd import workloads
+    if isinstance(cell, workloads.HexCell):
+        assemble, for_params = workloads.assemble_hex_cell, workloads.hex_materials_for_params
+    else:
+        assemble, for_params = workloads.assemble_cell, workloads.materials_for_params
     pidx = np.linspace(0, freqs.size - 1, pp).astype(int)
     eng.sweep_charges(freqs[pidx[:1]], precision='fp64', batch=1)
     torch.cuda.synchronize()
@@ -106,17 +124,17 @@ def plain_fp64(eng, cell, freqs, pp):
     p0 = cell.params.copy()
     t0 = time.perf_counter()
     eng.sweep_charges(fz, precision='fp64', batch=1)
-    for k in range(17):
+    for k in range(eng.n_params):
         h = 1e-6 * abs(p0[k]) if p0[k] != 0 else 1e-6 * 180 / np.pi
         for sgn in (1, -1):
             p = p0.copy()
             p[k] += sgn * h
-            K, M, D = workloads.assemble_cell(cell, workloads.materials_for_params(cell, p))
+            K, M, D = assemble(cell, for_params(cell, p))
             eng.update_operator(K, M, D, beta=p[15])
             eng.sweep_charges(fz, precision='fp64', batch=1)
     torch.cuda.synchronize()
     plain_fd = 1.0 / (time.perf_counter() - t0)
-    K, M, D = workloads.assemble_cell(cell)
+    K, M, D = assemble(cell)
     eng.update_operator(K, M, D, beta=p0[15])
     return plain, plain_adj, plain_fd
 
@@ -128,6 +146,7 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--workload', default='config2')
     ap.add_argument('--batch', type=int, default=0, help='frequencies per factorization chunk (0: fit 120 GB)')
+    ap.add_argument('--arena-gb', type=float, default=120.0, help='front arena per chunk when --batch is 0')
     ap.add_argument('--plain-points', type=int, default=8)
     ap.add_argument('--check-points', type=int, default=0,
                     help='frequencies of the FP64 accuracy check (0: every point of the sweep)')
@@ -156,6 +175,7 @@ def main():
     from sawsim_amd.plan import kernel_work
     assert torch.cuda.is_available()
     cell = workloads.WORKLOADS[args.workload]()
+    hex_cell = isinstance(cell, workloads.HexCell)
     freqs = cell.frequencies()
     P = freqs.size
     t0 = time.time()
@@ -168,7 +188,7 @@ def main():
     nb = 64                                   # panel width of the complex64 factorization
     work = kernel_work(eng.mf, nb)
     per_freq = eng.arena_bytes(1, 'mixed')
-    batch = args.batch or max(1, min(P, int(120e9 // per_freq)))
+    batch = args.batch or max(1, min(P, int(args.arena_gb * 1e9 // per_freq)))
     nchunk = -(-P // batch)
     batch = -(-P // nchunk)
     log(f'[bench] batch {batch} x {nchunk} chunks, arena {per_freq * batch / 1e9:.1f} GB')
@@ -200,7 +220,8 @@ def main():
     frfa = None
     if len(idx) == P:
         from sawsim_amd.metrics import compare, resonances
-        ym, y64 = -1j * 2 * np.pi * freqs * Q, -1j * 2 * np.pi * freqs * Q64
+        sign = 1j if hex_cell else -1j           # Y = +i w Q / V on the slabs, -i w Q / V in 2D
+        ym, y64 = sign * 2 * np.pi * freqs * Q, sign * 2 * np.pi * freqs * Q64
         rm, r64 = resonances(freqs, ym), resonances(freqs, y64)
         d = compare(r64, rm)
         frfa = dict(fp64_fr_hz=r64['fr_hz'], fp64_fa_hz=r64['fa_hz'], mixed_fr_hz=rm['fr_hz'], mixed_fa_hz=rm['fa_hz'],
@@ -251,10 +272,12 @@ def main():
         value=value, unit='freq pts/s', n_gpus=1, steps=args.steps, warmup=args.warmup,
         ms_per_step=1e3 * dt / args.steps, higher_is_better=True, scaling='weak', vs_baseline=None,
         dtype='complex64 fronts (fp32 MFMA) + fp64 refinement', data='synthetic',
-        config=dict(workload=f'{args.workload}: {cell.model.ne} Q9 elements, {eng.n_active} complex unknowns, '
+        config=dict(workload=f'{args.workload}: {cell.mesh.ne if hex_cell else cell.model.ne} '
+                             f'{"Hex27" if hex_cell else "Q9"} elements, {eng.n_active} complex unknowns, '
                              f'{P} frequency points, batch {batch}',
                     points=P, n_active=eng.n_active, fronts=eng.mf.nf, batch=batch,
-                    gradient='dY11/dp for 17 parameters (3m constants, Euler angles, beta_dK, tan delta) by adjoint'),
+                    gradient=f'dY11/dp for {eng.n_params} parameters (3m constants, Euler angles, beta_dK, '
+                             f'tan delta{", rho" if hex_cell else ""}) by adjoint'),
         y11_max_rel_err_vs_fp64=y_err,
         y11_check_points=int(len(idx)),
         fr_fa_vs_fp64=frfa,

@@ -67,6 +67,18 @@ int sawfem_gather_h27(void* stream, int nnzn, const int* pair_row, const int64_t
                       const int64_t* gidx, const unsigned char* dmask, const void* Ke, const void* Me, void* K, void* M,
                       void* D);
 
+/* K12' (Hex27 backward): per element of elems (ne0) and frequency, the sensitivity sums of
+ * l = s - T P_a^T lambda and the full vector d, contracted on the host with the tensor
+ * derivatives: W[b] = (Hc 6x6, He 3x6, Hp 3x3, Hm), 64 complex128, already multiplied by the
+ * element's stretch codes.  xt, lt: x and lambda as [n][Bp] (Bp >= B); pos_rep: active position
+ * of each dof's representative (-1 if none); zmask: signal-electrode potentials; work:
+ * B*ne0*64 complex128.  No counterpart in the reference (it has no Hex27 gradient); the
+ * element operators are those of sawfem_elem_h27. */
+int sawfem_adjoint_h27(void* stream, int ne0, const int* elems, const int* conn, const double* xyz, const int* pml_idx,
+                       int npml, const int* pml_kind, const double* pml_par, double omega_top, int N,
+                       const int* pos_rep, const unsigned char* zmask, double V, double scale, int B, int Bp,
+                       const void* xt, const void* lt, void* work, void* W);
+
 /* K3: d_i[j] = sum_{c in ptr[j]..ptr[j+1]} s_i[idx[c]] for up to three complex128 sources
  * sharing one list (s1/d1, s2/d2 may be NULL together).  Used for Tᵀ A T restricted to the
  * active set, the Km@s column and the charge weights (sweep.py:293-309). */

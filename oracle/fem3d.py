@@ -294,6 +294,10 @@ class Solver3D:
         disp[el['fu']] = 0.0
         return disp
 
+    def solve_complex(self, omega, V=1.0):
+        """The complex system solved directly (the reference's SuperLU fallback, bc_solve.py:160)."""
+        return self.solve(omega, V)
+
     def solve(self, omega, V=1.0):
         """(disp, Q, Y) with Y = +i w Q / V (hex_solver.py:99)."""
         a_aa = (self.k_aa - omega ** 2 * self.m_aa).tocsc()

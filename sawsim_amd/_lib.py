@@ -21,6 +21,8 @@ SIGNATURES = {
     'sawfem_gather_kmd': [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'sawfem_elem_h27': [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _d, _d, _vp, _vp],
     'sawfem_gather_h27': [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    'sawfem_adjoint_h27': [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _d, _i, _vp, _vp, _d, _d, _i, _i, _vp, _vp,
+                           _vp, _vp],
     'sawfem_gather3': [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'sawfem_mf_create': [C.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _i64, _vp,
                          _vp, _vp, _vp],

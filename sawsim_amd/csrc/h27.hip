@@ -245,6 +245,133 @@ __global__ void k_gather_h27(int nnzn, const int* __restrict__ pair_row, const i
   if (D) D[slot] = dv;
 }
 
+// K12' (Hex27 backward): sensitivity sums of the pair (l, d) per element and frequency.
+// With Q = s^T K d and A_aa x = b, the adjoint lambda = A_aa^-T kts_a gives
+//   dQ = l^T dK d + i w beta l^T dD d - w^2 l^T dM d,   l = s - T P_a^T lambda,
+// d the full displacement/potential vector (x on active and slave dofs through the fold,
+// V on the signal electrode, 0 on fixed and ground dofs).  K is linear in each element's
+// C, e, eps and M in rho, so per element the kernel forms, already multiplied by the
+// element's stretch codes (assembly order of the forward kernel):
+//   Hc[s][t] = code_C(s,t) sum_q w (B l)_s (B d)_t                                 (pass 0)
+//   He[k][s] = code_e(k,s) sum_q w (B l)_s (G d)_k + SCALE conj(code_e(k,s)) sum_q w (G l)_k (B d)_s
+//   Hp[j][k] = code_eps(j,k) sum_q w (G l)_j (G d)_k
+//   Hm       = a1 a2 a3 sum_q w sum_c (N l_c)(N d_c)                                (pass 1)
+// B is the forward kernel's strain operator (gamma_yz = duy/dz), G the potential gradient.
+// The host contracts the element sums with the tensor derivatives (SCALE on K_uu and M, the
+// sign of K_phiphi and the damping factor applied there).  x and lambda come transposed,
+// [dof][Bp], so the 64 frequencies of a wave read contiguous values.
+template <int PASS>
+__global__ void __launch_bounds__(64) k_adj_h27(int ne0, const int* __restrict__ elems, const int* __restrict__ conn,
+                                                const double* __restrict__ xyz, const int* __restrict__ pml_idx,
+                                                const Pml3Set pset, double omega_top, int N,
+                                                const int* __restrict__ pos_rep, const unsigned char* __restrict__ zmask,
+                                                double V, double scale, int B, int Bp, const double2* __restrict__ xt,
+                                                const double2* __restrict__ lt, double2* __restrict__ out) {
+  const int ei = blockIdx.x;
+  const int e = elems[ei];
+  const int t = threadIdx.x;
+  __shared__ double X[27][3];
+  __shared__ double sg[27][27][3];
+  __shared__ double sN[27][27];
+  __shared__ double sw[27];
+  __shared__ double2 code[7];
+  __shared__ int sp[27][4];
+  __shared__ unsigned char sz[27][4];
+  if (t < 27) {
+    const int nd = conn[(int64_t)e * 27 + t];
+    for (int d = 0; d < 3; ++d) X[t][d] = xyz[3 * (int64_t)nd + d];
+    for (int c = 0; c < 4; ++c) {
+      const int64_t dof = c < 3 ? 3 * (int64_t)nd + c : 3 * (int64_t)N + nd;
+      sp[t][c] = pos_rep[dof];
+      sz[t][c] = zmask[dof];
+    }
+  }
+  __syncthreads();
+  if (t < 27) h27_point(t, X, sN[t], sg[t], &sw[t]);
+  if (t == 32) h27_codes(pset, pml_idx[e], X[26][0], X[26][2], omega_top, code);
+  __syncthreads();
+  const int b = blockIdx.y * 64 + t;
+  if (b >= B) return;
+  constexpr int NA = PASS == 0 ? 36 : 46;     // pass 0: Hc; pass 1: He (two parts), Hp, Hm
+  double2 H[NA];
+  for (int i = 0; i < NA; ++i) H[i] = make_double2(0, 0);
+  for (int q = 0; q < 27; ++q) {
+    double2 el[6], ed[6], gl[3], gd[3], nl[3], nd[3];
+    for (int i = 0; i < 6; ++i) el[i] = ed[i] = make_double2(0, 0);
+    for (int i = 0; i < 3; ++i) gl[i] = gd[i] = nl[i] = nd[i] = make_double2(0, 0);
+    for (int a = 0; a < 27; ++a) {
+      double2 lv[4], dv[4];
+      for (int c = 0; c < 4; ++c) {
+        const int p = sp[a][c];
+        const bool z = sz[a][c] != 0;
+        const double2 xv = p >= 0 ? xt[(int64_t)p * Bp + b] : make_double2(0, 0);
+        const double2 lm = p >= 0 ? lt[(int64_t)p * Bp + b] : make_double2(0, 0);
+        dv[c] = z ? make_double2(V, 0) : xv;
+        lv[c] = make_double2((z ? 1.0 : 0.0) - lm.x, -lm.y);
+      }
+      const double g0 = sg[q][a][0], g1 = sg[q][a][1], g2 = sg[q][a][2], Nn = sN[q][a];
+      // strains (Voigt xx, yy, zz, yz = duy/dz, xz, xy)
+      el[0] = cadd(el[0], cscale(lv[0], g0)); ed[0] = cadd(ed[0], cscale(dv[0], g0));
+      el[1] = cadd(el[1], cscale(lv[1], g1)); ed[1] = cadd(ed[1], cscale(dv[1], g1));
+      el[2] = cadd(el[2], cscale(lv[2], g2)); ed[2] = cadd(ed[2], cscale(dv[2], g2));
+      el[3] = cadd(el[3], cscale(lv[1], g2)); ed[3] = cadd(ed[3], cscale(dv[1], g2));
+      el[4] = cadd(el[4], cadd(cscale(lv[0], g2), cscale(lv[2], g0)));
+      ed[4] = cadd(ed[4], cadd(cscale(dv[0], g2), cscale(dv[2], g0)));
+      el[5] = cadd(el[5], cadd(cscale(lv[0], g1), cscale(lv[1], g0)));
+      ed[5] = cadd(ed[5], cadd(cscale(dv[0], g1), cscale(dv[1], g0)));
+      if (PASS == 1) {
+        gl[0] = cadd(gl[0], cscale(lv[3], g0)); gd[0] = cadd(gd[0], cscale(dv[3], g0));
+        gl[1] = cadd(gl[1], cscale(lv[3], g1)); gd[1] = cadd(gd[1], cscale(dv[3], g1));
+        gl[2] = cadd(gl[2], cscale(lv[3], g2)); gd[2] = cadd(gd[2], cscale(dv[3], g2));
+        for (int c = 0; c < 3; ++c) { nl[c] = cadd(nl[c], cscale(lv[c], Nn)); nd[c] = cadd(nd[c], cscale(dv[c], Nn)); }
+      }
+    }
+    const double w = sw[q];
+    if (PASS == 0) {
+      for (int s = 0; s < 6; ++s) {
+        const double2 ls = cscale(el[s], w);
+        for (int u = 0; u < 6; ++u) H[s * 6 + u] = cfma(H[s * 6 + u], ls, ed[u]);
+      }
+    } else {
+      for (int k = 0; k < 3; ++k) {
+        const double2 gdk = cscale(gd[k], w), glk = cscale(gl[k], w);
+        for (int s = 0; s < 6; ++s) {
+          H[k * 6 + s] = cfma(H[k * 6 + s], el[s], gdk);          // u-phi part
+          H[18 + k * 6 + s] = cfma(H[18 + k * 6 + s], glk, ed[s]);  // phi-u part
+        }
+        for (int j = 0; j < 3; ++j) H[36 + k * 3 + j] = cfma(H[36 + k * 3 + j], glk, gd[j]);   // Hp[k][j]
+      }
+      for (int c = 0; c < 3; ++c) H[45] = cfma(H[45], cscale(nl[c], w), nd[c]);
+    }
+  }
+  double2* o = out + ((int64_t)b * ne0 + ei) * 64;
+  if (PASS == 0) {
+    for (int s = 0; s < 6; ++s)
+      for (int u = 0; u < 6; ++u) o[s * 6 + u] = cmul(code[kSC[s][u]], H[s * 6 + u]);
+  } else {
+    for (int k = 0; k < 3; ++k)
+      for (int s = 0; s < 6; ++s) {
+        const double2 c = code[kSE[k][s]];
+        const double2 cc = make_double2(scale * c.x, -scale * c.y);
+        o[36 + k * 6 + s] = cadd(cmul(c, H[k * 6 + s]), cmul(cc, H[18 + k * 6 + s]));
+      }
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) o[54 + j * 3 + k] = cmul(code[kSP[j][k]], H[36 + j * 3 + k]);
+    o[63] = cmul(code[6], H[45]);
+  }
+}
+
+// W[b][k] = sum_e out[b][e][k], elements in list order (deterministic).
+__global__ void k_adj_reduce_h27(int ne0, int B, const double2* __restrict__ out, double2* __restrict__ W) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (int64_t)B * 64) return;
+  const int b = (int)(g / 64), k = (int)(g % 64);
+  double2 s = make_double2(0, 0);
+  const double2* o = out + (int64_t)b * ne0 * 64 + k;
+  for (int e = 0; e < ne0; ++e) s = cadd(s, o[(int64_t)e * 64]);
+  W[g] = s;
+}
+
 }  // namespace
 
 extern "C" {
@@ -282,6 +409,40 @@ int sawfem_gather_h27(void* stream, int nnzn, const int* pair_row, const int64_t
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(k_gather_h27, dim3((nnzn + 255) / 256, 16), dim3(256), 0, s, nnzn, pair_row, node_ptr, gptr,
                      gidx, dmask, (const double2*)Ke, (const double2*)Me, (double2*)K, (double2*)M, (double2*)D);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+// Element sensitivities of the pair (l, d) for B frequencies (see k_adj_h27).  xt, lt: x and
+// lambda transposed to [n][Bp] (Bp >= B); work: B * ne0 * 64 complex128; W: B * 64 complex128,
+// W[b] = (Hc 6x6, He 3x6, Hp 3x3, Hm).
+int sawfem_adjoint_h27(void* stream, int ne0, const int* elems, const int* conn, const double* xyz, const int* pml_idx,
+                       int npml, const int* pml_kind, const double* pml_par, double omega_top, int N,
+                       const int* pos_rep, const unsigned char* zmask, double V, double scale, int B, int Bp,
+                       const void* xt, const void* lt, void* work, void* W) {
+  if (ne0 < 0 || B <= 0 || Bp < B || npml < 0 || npml > 8 || !pos_rep || !zmask || !xt || !lt || !work || !W) {
+    sawfem_set_error("sawfem_adjoint_h27: invalid arguments");
+    return 1;
+  }
+  Pml3Set ps;
+  for (int r = 0; r < npml; ++r) {
+    ps.r[r].kind = pml_kind[r];
+    ps.r[r].dmax = pml_par[6 * r + 0]; ps.r[r].n = pml_par[6 * r + 1];
+    ps.r[r].xa = pml_par[6 * r + 2]; ps.r[r].xp = pml_par[6 * r + 3];
+    ps.r[r].za = pml_par[6 * r + 4]; ps.r[r].zp = pml_par[6 * r + 5];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (ne0 > 0) {
+    dim3 grid(ne0, (B + 63) / 64);
+    hipLaunchKernelGGL(k_adj_h27<0>, grid, dim3(64), 0, s, ne0, elems, conn, xyz, pml_idx, ps, omega_top, N, pos_rep,
+                       zmask, V, scale, B, Bp, (const double2*)xt, (const double2*)lt, (double2*)work);
+    hipLaunchKernelGGL(k_adj_h27<1>, grid, dim3(64), 0, s, ne0, elems, conn, xyz, pml_idx, ps, omega_top, N, pos_rep,
+                       zmask, V, scale, B, Bp, (const double2*)xt, (const double2*)lt, (double2*)work);
+    SAW_HIP(hipGetLastError());
+  }
+  const int64_t tot = (int64_t)B * 64;
+  hipLaunchKernelGGL(k_adj_reduce_h27, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, ne0, B,
+                     (const double2*)work, (double2*)W);
   SAW_HIP(hipGetLastError());
   return 0;
 }

@@ -300,3 +300,69 @@ class FrequencySolver3D(_frequency_solver_cls()):
             period = float(abs(coord[R[0] // 3, 0] - coord[L[0] // 3, 0]))
         col, col_xz = _columns(coord, period)
         self._setup(K, M, bl, col[bl.node_of], col_xz, period, leaf_weight)
+
+    # ---------------------------------------------------------------- gradient
+    n_params = 18          # params.NAMES3: the 17 of the plane cells and the piezo density
+
+    def enable_gradient(self, elem, coord, material_list, pml_list, omega_top, params, material=0):
+        """Prepare dY11/dp for the 18 parameters (``params.NAMES3``) of material record
+        ``material`` (its elements must lie outside the PML).  The other arguments are the
+        assembly inputs; ``params`` is the vector the record's tensors were built from."""
+        from . import params as prm
+        torch = self.torch
+        bl = self.bloch
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=self.dev)
+        elem = np.asarray(elem, dtype=np.int64)
+        _, pml_idx, _, _, _, _, kinds, par, npml = material_tables(elem.shape[0], material_list, pml_list)
+        elems = np.asarray(material_list[material]['indx'], dtype=np.int64)
+        if (pml_idx[elems] >= 0).any():
+            raise ValueError('the parameterised material must not be a PML record')
+        self._grad = dict(elems=t(elems.astype(np.int32)), ne0=int(elems.size), conn=t(elem.astype(np.int32)),
+                          xyz=t(np.asarray(coord, dtype=np.float64)), pml_idx=t(pml_idx), npml=npml, kinds=kinds,
+                          par=par, omega_top=float(omega_top), N=int(np.asarray(coord).shape[0]),
+                          pos_rep=t(bl.pos[bl.rep].astype(np.int32)),
+                          zmask=t(np.isin(np.arange(self.gdof), bl.zheng).astype(np.uint8)),
+                          params=np.asarray(params, dtype=float))
+        self._grad['tensors'] = prm.full_tensors(params)
+        self._grad['jac'] = prm.jacobian_full(params)
+        self._gradient_maps(KM_SCALE)
+
+    def _contract(self, omegas, V, x, lam):
+        """dY11/dp from the forward solution x and the adjoint lambda: element sensitivities on
+        the GPU (``sawfem_adjoint_h27``), parameter contraction on the host."""
+        torch = self.torch
+        g = self._grad
+        s = stream_ptr()
+        B, n = x.shape
+        Bp = -(-B // 64) * 64
+        xt = torch.empty((n, Bp), dtype=torch.complex128, device=self.dev)
+        lt = torch.empty((n, Bp), dtype=torch.complex128, device=self.dev)
+        _lib.call('sawfem_transpose', s, B, n, n, Bp, x.data_ptr(), xt.data_ptr())
+        _lib.call('sawfem_transpose', s, B, n, n, Bp, lam.data_ptr(), lt.data_ptr())
+        work = torch.empty(max(1, B * g['ne0'] * 64), dtype=torch.complex128, device=self.dev)
+        W = torch.empty(B * 64, dtype=torch.complex128, device=self.dev)
+        _lib.call('sawfem_adjoint_h27', s, g['ne0'], g['elems'].data_ptr(), g['conn'].data_ptr(), g['xyz'].data_ptr(),
+                  g['pml_idx'].data_ptr(), g['npml'], g['kinds'].ctypes.data, g['par'].ctypes.data, g['omega_top'],
+                  g['N'], g['pos_rep'].data_ptr(), g['zmask'].data_ptr(), float(V), KM_SCALE, B, Bp, xt.data_ptr(),
+                  lt.data_ptr(), work.data_ptr(), W.data_ptr())
+        Wh = W.cpu().numpy().reshape(B, 64)
+        return dict(dY=self._param_gradient(Wh, omegas, V))
+
+    def _param_gradient(self, W, omegas, V):
+        """dY11/dp (B, 18) from the element sensitivity sums W (B, 64) = (Hc, He, Hp, Hm):
+        dQ = (1 + i w beta) KM_SCALE Hc:dC + He:dE - Hp:dEps - w^2 KM_SCALE Hm drho, the damping
+        parameter through i w KM_SCALE Hc:C, and dY = +i w dQ / V."""
+        g = self._grad
+        B = W.shape[0]
+        Hc = W[:, :36].reshape(B, 6, 6)
+        He = W[:, 36:54].reshape(B, 3, 6)
+        Hp = W[:, 54:63].reshape(B, 3, 3)
+        Hm = W[:, 63]
+        dC, dE, dP, drho = g['jac']
+        C = g['tensors'][0]
+        w = np.asarray(omegas, dtype=float)[:, None]
+        fac = 1 + 1j * w * self.beta
+        dQ = (fac * KM_SCALE * np.einsum('bst,kst->bk', Hc, dC) + np.einsum('bks,pks->bp', He, dE)
+              - np.einsum('bjk,pjk->bp', Hp, dP) - w ** 2 * KM_SCALE * Hm[:, None] * drho[None, :])
+        dQ[:, 15] = (1j * w[:, 0]) * KM_SCALE * np.einsum('bst,st->b', Hc, C)
+        return 1j * w / V * dQ

@@ -96,5 +96,52 @@ def jacobian(p, plane='xy', h=1e-30):
     return dC, dE, dP
 
 
+# Hex27 slabs add the piezo density (kg/m^3) as an 18th parameter
+NAMES3 = NAMES + ['rho']
+
+
+def full_tensors(p):
+    """Piezo (C 6x6, e 3x6, eps 3x3) complex128 of a slab cell for parameter vector ``p`` (18):
+    rotated full tensors, eps with the dielectric loss (1 - i tan delta)."""
+    v = np.asarray(p, dtype=float)
+    C, e, eps = _full(v)
+    return C.astype(complex), e.astype(complex), eps * (1 - 1j * v[16])
+
+
+def _full(v):
+    G = 1e9
+    c11, c12, c13, c14, c33, c44 = (v[i] * G for i in range(6))
+    e15, e22, e31, e33, k11, k33 = v[6:12]
+    z = 0 * c11
+    C = np.array([[c11, c12, c13, c14, z, z], [c12, c11, c13, -c14, z, z], [c13, c13, c33, z, z, z],
+                  [c14, -c14, z, c44, z, z], [z, z, z, z, c44, c14], [z, z, z, z, c14, (c11 - c12) / 2]])
+    e = np.array([[z, z, z, z, e15, -e22], [-e22, e22, z, e15, z, z], [e31, e31, e33, z, z, z]])
+    eps = np.array([[k11, z, z], [z, k11, z], [z, z, k33]]) * EPS0
+    a = _rot(*(np.asarray(v[12:15]) * np.pi / 180.0))
+    M = _bond(a)
+    return M @ C @ M.T, a @ e @ M.T, a @ eps @ a.T
+
+
+def jacobian_full(p, h=1e-30):
+    """dC (18,6,6), dE (18,3,6), dEPS (18,3,3), drho (18,) of the slab piezo (complex step for
+    the 15 tensor parameters, closed form for tan delta and rho)."""
+    v = np.asarray(p, dtype=float)
+    dC = np.zeros((18, 6, 6), dtype=complex)
+    dE = np.zeros((18, 3, 6), dtype=complex)
+    dP = np.zeros((18, 3, 3), dtype=complex)
+    drho = np.zeros(18)
+    loss = 1 - 1j * v[16]
+    for k in range(15):
+        w = v[:15].astype(complex)
+        w[k] += 1j * h
+        C, e, eps = _full(w)
+        dC[k] = C.imag / h
+        dE[k] = e.imag / h
+        dP[k] = eps.imag / h * loss
+    dP[16] = -1j * _full(v)[2]
+    drho[17] = 1.0
+    return dC, dE, dP, drho
+
+
 def from_constants(constants, euler, beta_dk=0.0, tand=0.0):
     return np.array([constants[k] for k in NAMES[:12]] + list(euler) + [beta_dk, tand], dtype=float)

@@ -82,6 +82,8 @@ def bc_periodic(model: Model, lines, tag_bottom=21, tag_left=19, tag_right=20, t
 class FrequencySolver:
     """Per-frequency solve of the Bloch-reduced piezoelectric system on one GPU."""
 
+    n_params = 17          # dY11/dp parameters (params.NAMES)
+
     def __init__(self, K: DeviceCSR, M: DeviceCSR, fix, zheng, fu, merge_groups, *, node_xy=None, period=None,
                  leaf_weight=96):
         bl = BlochLayout(K.layout, fix, zheng, fu, merge_groups)
@@ -212,7 +214,15 @@ class FrequencySolver:
             params=np.asarray(params, dtype=float), plane=plane)
         self._grad['tensors'] = prm.lifted_tensors(params, plane)
         self._grad['jac'] = prm.jacobian(params, plane)
-        # slot of the transposed entry on the (structurally symmetric) active pattern
+        from .piezo_fem import SCALE
+        self._gradient_maps(SCALE)
+
+    def _gradient_maps(self, scale):
+        """Transposed-entry slots of the active pattern (transposed residual), the scaling
+        D = diag(1 on displacements, 1/scale on the potential) of the paired adjoint."""
+        torch = self.torch
+        bl = self.bloch
+        t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=self.dev)
         n = self.n_active
         r = np.repeat(np.arange(n, dtype=np.int64), np.diff(bl.aa_indptr))
         c = bl.aa_indices.astype(np.int64)
@@ -221,9 +231,8 @@ class FrequencySolver:
         self.tslot = t(np.searchsorted(key, tkey).astype(np.int64))
         if not np.array_equal(key[np.searchsorted(key, tkey)], tkey):
             raise ValueError('active pattern is not structurally symmetric')
-        from .piezo_fem import SCALE
         is_phi = bl.comp_of == (self.layout.nd - 1)
-        self._dscale = t(np.where(is_phi, 1.0 / SCALE, 1.0))
+        self._dscale = t(np.where(is_phi, 1.0 / scale, 1.0))
         self._arow = t(r)
         self._sym = None
 
@@ -339,13 +348,18 @@ class FrequencySolver:
             bad = np.flatnonzero(~(err <= ir_tol) | (~(out.get('adj_err', err) <= ir_tol) if gradient else False))
             if bad.size:
                 del arena, uarena
-                sub = self.solve_batch(omegas[bad], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir,
-                                       gradient=gradient, adjoint=adjoint)
-                tb = self.torch.as_tensor(bad, device=self.dev)
-                x[tb] = sub['x']
-                for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err'):
-                    if key in out:
-                        out[key][bad] = sub[key]
+                # complex128 refactorization in chunks that fit the workspace of this call
+                nfit = max(1, int(self._ws.numel() // self.arena_bytes(1, 'fp64')))
+                for c0 in range(0, bad.size, nfit):
+                    part = bad[c0:c0 + nfit]
+                    sub = self.solve_batch(omegas[part], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir,
+                                           gradient=gradient, adjoint=adjoint)
+                    tb = self.torch.as_tensor(part, device=self.dev)
+                    x[tb] = sub['x']
+                    for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err'):
+                        if key in out:
+                            out[key][part] = sub[key]
+                    del sub
                 refactored[bad] = True
         out['refactored'] = refactored
         return out
@@ -515,15 +529,28 @@ class FrequencySolver:
         Wh = W.cpu().numpy().reshape(B, NT)
         return dict(dY=self._param_gradient(Wh, omegas, V))
 
-    def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, **kw):
-        """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies.
+    def auto_batch(self, precision='mixed', gradient=False, fraction=0.6):
+        """Frequencies per chunk that fit ``fraction`` of the free HBM: front arena, update
+        vectors (two right-hand sides with the gradient) and about a dozen length-n vectors."""
+        free, _ = self.torch.cuda.mem_get_info()
+        ws = getattr(self, '_ws', None)
+        free += 0 if ws is None else ws.numel()
+        R = 2 if gradient else 1
+        per = self.arena_bytes(1, precision) + 16 * self.mf.uarena_elems * R + 16 * 12 * R * self.n_active
+        return max(1, int(fraction * free // per))
 
-        With ``gradient=True`` returns (Q, dY, info) where dY (P, 17) is dY11/dp.
+    def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, **kw):
+        """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies (default: as many
+        as fit the free HBM, see :meth:`auto_batch`).
+
+        With ``gradient=True`` returns (Q, dY, info) where dY (P, n_params) is dY11/dp.
         """
         fre = np.asarray(fre, dtype=np.float64)
-        batch = batch or fre.size
+        if not batch:
+            nb = self.auto_batch(precision, gradient)
+            batch = -(-fre.size // -(-fre.size // nb))          # equal chunks
         Q = np.empty(fre.size, dtype=complex)
-        dY = np.empty((fre.size, 17), dtype=complex) if gradient else None
+        dY = np.empty((fre.size, self.n_params), dtype=complex) if gradient else None
         info = []
         for i0 in range(0, fre.size, batch):
             res = self.solve_batch(2 * np.pi * fre[i0:i0 + batch], V, precision=precision, gradient=gradient, **kw)

@@ -258,8 +258,9 @@ def hex_template(name, points=5, band=None):
                    info=dict(name=f'sp_2p5d_{name}_layer'))
 
 
-# config 4: fR of a 201-point pilot over 1.0-3.0 GHz (see pilot_resonance), band fR * [0.92, 1.08]
-BAND4 = (1.6e9, 2.0e9)
+# config 4: fR = 1.8278 GHz from a 201-point pilot over 1.0-3.0 GHz (pilot_resonance on the
+# default slab), band fR * [0.92, 1.08]
+BAND4 = (1.6816e9, 1.9741e9)
 
 
 def config4(seed=0, mesh_um=0.047, points=201, band=BAND4, euler=(0, 52, 0), beta_dk=0.0, eta_eps=0.0):
@@ -302,26 +303,44 @@ def assemble_cell(cell: Cell, materials=None):
                            damping_mask=mask)
 
 
-def build_hex_engine(cell: HexCell, **kw):
-    """GPU FrequencySolver3D for a slab cell (damping D = K_uu of the piezo)."""
+def assemble_hex_cell(cell: HexCell, materials=None):
+    """K, M, D on the GPU for a slab cell (D = K_uu of the piezo record)."""
     from . import hex27
     mesh = cell.mesh
-    omega_top = 2 * np.pi * cell.band_hz[1]
-    K, M, D = hex27.assemble_h27(mesh.hexas, mesh.pos_m, cell.material_list(), cell.pml_list(), omega_top,
-                                 damping=[True] + [False] * (len(cell.material_list()) - 1))
+    recs = cell.material_list(materials)
+    return hex27.assemble_h27(mesh.hexas, mesh.pos_m, recs, cell.pml_list(materials), 2 * np.pi * cell.band_hz[1],
+                              damping=[True] + [False] * (len(recs) - 1))
+
+
+def build_hex_engine(cell: HexCell, gradient=False, **kw):
+    """GPU FrequencySolver3D for a slab cell, with damping and (optionally) the dY11/dp context."""
+    from . import hex27
+    mesh = cell.mesh
+    K, M, D = assemble_hex_cell(cell)
     bnd = hex27.boundary_idx_vect_3ds(hex27.find_saw_boundary_3ds(mesh), mesh.pos_m, 3 * mesh.N)
     eng = hex27.FrequencySolver3D(K, M, bnd, **kw)
     eng.set_damping(D, cell.beta_dk)
+    if gradient:
+        if cell.params is None:
+            raise ValueError('this cell has no design parameters')
+        eng.enable_gradient(mesh.hexas, mesh.pos_m, cell.material_list(), cell.pml_list(), 2 * np.pi * cell.band_hz[1],
+                            cell.params)
     return eng
+
+
+def hex_materials_for_params(cell: HexCell, p):
+    """The slab cell's records with the piezo rebuilt from the 18-parameter vector ``p``."""
+    from .params import NAMES
+    k = {name: float(p[i]) for i, name in enumerate(NAMES[:12])}
+    pz = piezo_material(k, tuple(float(a) for a in p[12:15]), float(p[16]), plane=None, rho=float(p[17]))
+    return [pz] + list(cell.materials[1:])
 
 
 def build_engine(cell: Cell, gradient=False, **kw):
     """GPU FrequencySolver for a cell, with damping and (optionally) the dY11/dp context."""
     from .sweep import FrequencySolver, bc_periodic
     if isinstance(cell, HexCell):
-        if gradient:
-            raise NotImplementedError('dY11/dp of Hex27 cells is not built')
-        return build_hex_engine(cell, **kw)
+        return build_hex_engine(cell, gradient=gradient, **kw)
     K, M, D = assemble_cell(cell)
     eng = FrequencySolver(K, M, *bc_periodic(cell.model, cell.mesh.lines, ndisp=cell.ndisp), **kw)
     eng.set_damping(D, cell.beta_dk)

@@ -137,3 +137,58 @@ def test_h27_api_sweep_layout(tmp_path):
     assert np.allclose(z['admittance'] * ap, z['raw_admittance_s'])
     fields = np.load(tmp_path / 'run' / 'fields.npz')
     assert fields['displacement_m'].shape == (r.metadata['nodes'], 3)
+
+
+def _coarse_slab():
+    return workloads.config4(mesh_um=0.2, points=3, beta_dk=1e-13, eta_eps=1e-3)
+
+
+def _fd_hex(cell, eng, f, p0, V=1.0):
+    """Central differences of the FP64 GPU sweep over the 18 slab parameters (relative step
+    1e-6; 1e-6 rad for the zero Euler angles), reusing the symbolic plan."""
+    out = np.zeros((18, f.size), dtype=complex)
+    for k in range(18):
+        h = 1e-6 * abs(p0[k]) if p0[k] != 0 else 1e-6 * 180 / np.pi
+        ys = []
+        for sgn in (1, -1):
+            p = p0.copy()
+            p[k] += sgn * h
+            K, M, D = workloads.assemble_hex_cell(cell, workloads.hex_materials_for_params(cell, p))
+            eng.update_operator(K, M, D, beta=p[15])
+            Q, _ = eng.sweep_charges(f, precision='fp64')
+            ys.append(1j * 2 * np.pi * f * Q / V)
+        out[k] = (ys[0] - ys[1]) / (2 * h)
+    K, M, D = workloads.assemble_hex_cell(cell)
+    eng.update_operator(K, M, D, beta=p0[15])
+    return out
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_h27_adjoint_matches_finite_differences(precision):
+    """dY11/dp for the 18 slab parameters (3m constants, Euler angles, beta_dK, tan delta, rho)
+    against central differences of the FP64 GPU sweep, on a coarse config-4 slab with damping,
+    dielectric loss and the bottom PML."""
+    _say(f'[test_h27_adjoint_matches_finite_differences] {precision}')
+    cell = _coarse_slab()
+    eng = workloads.build_hex_engine(cell, gradient=True)
+    f = cell.frequencies(3)
+    Q, dY, info = eng.sweep_charges(f, precision=precision, gradient=True)
+    fd = _fd_hex(cell, eng, f, cell.params.copy())
+    scale = np.abs(cell.params)[:, None]
+    scale[12:15] = 1.0
+    a, b = dY.T * scale, fd * scale
+    err = np.abs(a - b).max(axis=0) / np.abs(b).max(axis=0)
+    _say(precision, 'adjoint vs FD per frequency', err, 'paired', info[0].get('paired_adjoint'))
+    assert err.max() <= 1e-5
+
+
+def test_h27_paired_adjoint_matches_transposed_solves():
+    _say('[test_h27_paired_adjoint_matches_transposed_solves]')
+    cell = _coarse_slab()
+    eng = workloads.build_hex_engine(cell, gradient=True)
+    assert eng.scaled_symmetric()
+    om = 2 * np.pi * cell.frequencies(4)
+    a = eng.solve_batch(om, precision='mixed', gradient=True, adjoint='paired')
+    b = eng.solve_batch(om, precision='mixed', gradient=True, adjoint='transposed')
+    assert np.max(np.abs(a['Q'] - b['Q']) / np.abs(b['Q'])) <= 1e-8
+    assert np.max(np.abs(a['dY'] - b['dY']).max(axis=1) / np.abs(b['dY']).max(axis=1)) <= 1e-7
