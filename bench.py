@@ -28,10 +28,10 @@ import numpy as np  # noqa: E402
 
 PEAK_FP32_MFMA = 157.3   # TFLOP/s dense, MI355X
 PEAK_FP64_MFMA = 78.6
-# HBM bytes per launch of the Schur GEMM k_gemm<float, 1> on the config-2 sweep: rocprofv3
-# FETCH_SIZE and WRITE_SIZE (separate runs), each scaled by what the counter reads for a plain
-# 1 GiB device copy (`--copy-mib 1024`: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000)
-SCHUR_TRAFFIC_BYTES = {'config2': 2.991e10}
+# HBM bytes per launch of the Schur GEMM k_gemm<float, 1> on the config-2 / config-4 sweeps:
+# rocprofv3 FETCH_SIZE and WRITE_SIZE (separate runs), each scaled by what the counter reads for
+# a plain 1 GiB device copy (`--copy-mib 1024`: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000)
+SCHUR_TRAFFIC_BYTES = {'config2': 2.991e10, 'config4': 2.784e10}
 
 
 def log(*a):

@@ -1139,6 +1139,7 @@ struct Plan {
   int64_t* fent_hi = nullptr;
   unsigned char* fused[2] = {nullptr, nullptr};
   int small_limit[2] = {0, 0};   // device LDS bound per precision
+  int lds_max = 64 * 1024;       // LDS one workgroup may allocate on this device
   int small_mmax[2] = {0, 0};    // threshold in use
   std::vector<void*> part_owned[2];
   std::vector<int> hm, hp, hlvl_ptr, hlvl_fronts;
@@ -1308,7 +1309,15 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     const int nbl = panel_width(nb, L);
     const size_t lds = sizeof(V) * 64 * (nbl + 1) + sizeof(double2) * R * (256 + (full ? S.mmax : S.pmax));
     const dim3 grid(S.nf, nsel);
-#define SAW_TRI(K, FULL) hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, nbl, A, U, X, bsel)
+    // above 64 KB the kernel must be allowed the larger dynamic LDS (checked against the device
+    // limit by sawfem_mf_solve)
+#define SAW_TRI(K, FULL)                                                                                  \
+  do {                                                                                                   \
+    if (lds > 64 * 1024)                                                                                 \
+      (void)hipFuncSetAttribute((const void*)k_tri<T, K, R, FULL>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)lds);                                                               \
+    hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, nbl, A, U, X, bsel); \
+  } while (0)
     if (full) {
       switch (kind) { case 0: SAW_TRI(0, true); break; case 1: SAW_TRI(1, true); break;
                       case 2: SAW_TRI(2, true); break; default: SAW_TRI(3, true); }
@@ -1439,6 +1448,7 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       lds = 64 * 1024;
+    P->lds_max = lds;
     for (int pi = 0; pi < 2; ++pi) {
       const size_t sz = pi == 0 ? sizeof(c64) : sizeof(c128);
       int mm = 0;
@@ -1545,7 +1555,7 @@ int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const v
   // LDS of the pivot-block kernel: the widest tile (64 x 65 complex64 or 64 x 33 complex128,
   // both <= 64 * 33 * 16 bytes), partial sums and the pivot entries of the nrhs vectors
   const size_t tile = 64 * 33 * 16;
-  if ((size_t)nrhs * (P->pmax + 256) * sizeof(double2) + tile > 64 * 1024) {
+  if ((size_t)nrhs * (P->pmax + 256) * sizeof(double2) + tile > (size_t)P->lds_max) {
     sawfem_set_error("sawfem_mf_solve: front too large for the LDS solve");
     return 1;
   }
