@@ -139,6 +139,120 @@ def plain_fp64(eng, cell, freqs, pp):
     return plain, plain_adj, plain_fd
 
 
+def design_set(args):
+    """Config 5: the design set of 64 seeded stack cells (2k-20k Q9 elements, 128 points each,
+    Y11 + dY11/dp for 17 parameters), swept back to back on one shared HBM workspace after every
+    cell's assembly and symbolic analysis.  Per-cell latency percentiles beside the aggregate."""
+    import torch
+    from sawsim_amd import _lib, workloads
+    from sawsim_amd.plan import kernel_work
+    cells = workloads.config5(n_cells=args.cells)
+    t0 = time.time()
+    engines = []
+    for i, c in enumerate(cells):
+        engines.append(build_engine(c))
+        torch.cuda.synchronize()
+        log(f'[bench] cell {i}: {c.model.ne} elements, n={engines[-1].n_active}, {time.time() - t0:.0f}s')
+    setup = time.time() - t0
+    P = cells[0].points
+    batches = []
+    for eng in engines:
+        per = eng.arena_bytes(1, 'mixed')
+        b = max(1, min(P, int(args.arena_gb * 1e9 // per)))
+        b = -(-P // -(-P // b))
+        batches.append(b)
+    ws_bytes = max(eng.arena_bytes(b, 'mixed') for eng, b in zip(engines, batches))
+    shared = torch.empty(ws_bytes, dtype=torch.uint8, device='cuda')
+    for eng in engines:
+        eng._ws = shared                       # every cell reuses the one front workspace
+    ne = [c.model.ne for c in cells]
+    log(f'[bench] config5: {len(cells)} cells, {sum(ne)} elements ({min(ne)}-{max(ne)}), setup {setup:.0f}s, '
+        f'workspace {ws_bytes / 1e9:.0f} GB')
+
+    def sweep(i):
+        return engines[i].sweep_charges(cells[i].frequencies(), precision='mixed', batch=batches[i], gradient=True)
+    for _ in range(args.warmup):
+        sweep(int(np.argmax(ne)))
+    torch.cuda.synchronize()
+    for eng in engines:
+        _lib.call('sawfem_mf_profile', eng._plan, 1)
+    lat, results = [], []
+    t0 = time.perf_counter()
+    for i in range(len(cells)):
+        t1 = time.perf_counter()
+        results.append(sweep(i))
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t1)
+    dt = time.perf_counter() - t0
+    schur_ms = schur_n = 0.0
+    schur_flops = 0.0
+    for eng, c in zip(engines, cells):
+        prof, _ = profile_read(eng)
+        schur_ms += prof['gemm_s'][0]
+        schur_n += prof['gemm_s'][1]
+        schur_flops += kernel_work(eng.mf, 64)['schur_flops'] * c.points
+        _lib.call('sawfem_mf_profile', eng._plan, 0)
+    npts = sum(c.points for c in cells)
+    value = npts / dt
+    lat_ms = np.array(lat) * 1e3
+    log(f'[bench] design set: {dt:.2f}s for {npts} points -> {value:.1f} pts/s; latency p50 '
+        f'{np.percentile(lat_ms, 50):.0f} ms p90 {np.percentile(lat_ms, 90):.0f} ms')
+    # accuracy against the FP64 path on every 8th cell (all points)
+    errs, gerrs = [], []
+    for i in range(0, len(cells), 8):
+        Q, dY, _ = results[i]
+        Q64, dY64, _ = engines[i].sweep_charges(cells[i].frequencies(), precision='fp64',
+                                                batch=max(1, batches[i] // 2), gradient=True)
+        errs.append(float(np.max(np.abs(Q - Q64) / np.abs(Q64))))
+        scale = np.abs(cells[i].params)
+        scale[12:15] = 1.0
+        gerrs.append(float(np.max(np.abs((dY - dY64) * scale).max(axis=1) / np.abs(dY64 * scale).max(axis=1))))
+    log(f'[bench] mixed vs fp64 on {len(errs)} cells: Y11 {max(errs):.2e}, dY/dp {max(gerrs):.2e}')
+    # the plain FP64 path on the median-size cell, and the mixed rate on that same cell
+    mid = int(np.argsort(ne)[len(ne) // 2])
+    plain = plain_adj = plain_fd = None
+    if args.plain_points > 0:
+        plain, plain_adj, plain_fd = plain_fp64(engines[mid], cells[mid], cells[mid].frequencies(),
+                                                min(args.plain_points, P))
+    mid_rate = cells[mid].points / lat[mid]
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(cells[mid], args.cpu_workers)
+        except Exception as exc:
+            log(f'[bench] cpu baseline failed: {exc!r}')
+    achieved = schur_flops / (schur_ms * 1e-3) / 1e12 if schur_ms > 0 else None
+    out = dict(
+        metric='freq pts/s (Y11 + dY11/dp) at Y11 rel err <=1e-6 vs FP64; vs plain FP64 GPU',
+        value=value, unit='freq pts/s', n_gpus=1, steps=1, warmup=args.warmup, ms_per_step=1e3 * dt,
+        higher_is_better=True, scaling='weak', vs_baseline=None,
+        dtype='complex64 fronts (fp32 MFMA) + fp64 refinement', data='synthetic',
+        config=dict(workload=f'config5: {len(cells)} design cells (seeds 0..{len(cells) - 1}), {sum(ne)} Q9 elements '
+                             f'({min(ne)}-{max(ne)} per cell), {P} points each, swept back to back',
+                    cells=len(cells), points_per_cell=P, elements=ne,
+                    n_active=[int(e.n_active) for e in engines], batches=batches,
+                    gradient='dY11/dp for 17 parameters by adjoint'),
+        latency_ms_per_cell=dict(p50=float(np.percentile(lat_ms, 50)), p90=float(np.percentile(lat_ms, 90)),
+                                 p99=float(np.percentile(lat_ms, 99)), max=float(lat_ms.max()),
+                                 min=float(lat_ms.min())),
+        per_cell_ms=[round(float(x), 1) for x in lat_ms],
+        y11_max_rel_err_vs_fp64=max(errs), dy11_max_rel_err_vs_fp64_adjoint=max(gerrs),
+        accuracy_cells=list(range(0, len(cells), 8)),
+        plain_fp64_gpu_pts_per_s=dict(cell=mid, elements=ne[mid], y11_only=plain, y11_plus_fp64_adjoint=plain_adj,
+                                      y11_plus_central_fd=plain_fd, mixed_on_that_cell=mid_rate)
+        if plain else None,
+        speedup_vs_plain_fp64=dict(central_fd=mid_rate / plain_fd, fp64_adjoint=mid_rate / plain_adj,
+                                   y11_only=mid_rate / plain) if plain else None,
+        roofline=dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
+                      frac=(achieved / PEAK_FP32_MFMA) if achieved else None, traffic=None,
+                      kernel='k_gemm<float, 1> (Schur complement) over all cells', launches=int(schur_n),
+                      kernel_ms=schur_ms, algorithmic_flops=schur_flops),
+        setup_seconds=setup,
+        cpu_baseline=cpu,
+    )
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -147,6 +261,7 @@ def main():
     ap.add_argument('--workload', default='config2')
     ap.add_argument('--batch', type=int, default=0, help='frequencies per factorization chunk (0: fit 120 GB)')
     ap.add_argument('--arena-gb', type=float, default=120.0, help='front arena per chunk when --batch is 0')
+    ap.add_argument('--cells', type=int, default=64, help='design-set size (config5)')
     ap.add_argument('--plain-points', type=int, default=8)
     ap.add_argument('--check-points', type=int, default=0,
                     help='frequencies of the FP64 accuracy check (0: every point of the sweep)')
@@ -174,6 +289,8 @@ def main():
     from sawsim_amd import _lib, workloads
     from sawsim_amd.plan import kernel_work
     assert torch.cuda.is_available()
+    if args.workload == 'config5':
+        return design_set(args)
     cell = workloads.WORKLOADS[args.workload]()
     hex_cell = isinstance(cell, workloads.HexCell)
     freqs = cell.frequencies()

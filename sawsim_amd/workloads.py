@@ -158,6 +158,70 @@ def quad_layer(points=5, band=(1.5e9, 2.7e9)):
     return _stack(mesh, mats, None, 3, band, points, info=dict(name='sp_quad_layer'))
 
 
+# ------------------------------------------------------------------ design set (config 5)
+def _layer_material(rng):
+    """An isotropic or cubic layer record with ±5 % jitter on each independent constant."""
+    j = lambda v: v * (1.0 + rng.uniform(-0.05, 0.05))
+    kind = int(rng.integers(0, 4))
+    if kind == 0:
+        return mt.isotropic(j(130e9), j(0.28), j(2330.0), j(11.7))           # Si
+    if kind == 1:
+        return mt.isotropic(j(72e9), j(0.17), j(2200.0), j(4.2))             # SiO2
+    if kind == 2:
+        return mt.isotropic(j(169e9), j(0.22), j(2320.0), j(4.5))            # poly-Si
+    C = np.zeros((6, 6))                                                      # cubic Si
+    c11, c12, c44 = j(166e9), j(64e9), j(80e9)
+    C[:3, :3] = c12
+    C[np.arange(3), np.arange(3)] = c11
+    C[np.arange(3, 6), np.arange(3, 6)] = c44
+    return mt.Material(j(2330.0), C, np.zeros((3, 6)), np.eye(3) * j(11.68) * mt.EPS0)
+
+
+def design_cell(seed, points=128, ne_range=(2000, 20000)):
+    """One cell of the design set: pitch U(0.8, 2.0) µm, 0-3 backings and 0-2 coatings with
+    thicknesses U(0.1, 0.8) pitch (the deepest backing fills the stack to 5 pitch; without
+    backings the piezo is the substrate), isotropic/cubic layer records and a 3m piezo, all
+    jittered ±5 %, Euler (0, U(-50, -35)°, 0); Q9 plane strain with a mesh size that puts the
+    element count near a target drawn uniformly from ``ne_range``; Al electrodes 0.08 pitch thick;
+    1 pitch bottom PML; ``points`` frequencies over [0.7, 1.3] x (3100 m/s) / (2 pitch)."""
+    rng = np.random.default_rng(seed)
+    pitch = float(rng.uniform(0.8, 2.0))
+    nb, nc = int(rng.integers(0, 4)), int(rng.integers(0, 3))
+    thick = lambda: float(rng.uniform(0.1, 0.8)) * pitch
+    if nb:
+        piezo = thick()
+        backs = [thick() for _ in range(nb - 1)]
+        backs.append(max(0.2 * pitch, 5 * pitch - piezo - sum(backs)))
+    else:
+        piezo, backs = 5 * pitch, []
+    coats = [thick() for _ in range(nc)]
+    k = {key: v * (1.0 + rng.uniform(-0.05, 0.05)) for key, v in SP_BASELINE_3M.items()}
+    euler = (0.0, float(rng.uniform(-50.0, -35.0)), 0.0)
+    mats = ([piezo_material(k, euler), _al()] + [_layer_material(rng) for _ in range(nb)]
+            + [_layer_material(rng) for _ in range(nc)])
+    target = float(rng.uniform(*ne_range))
+    build = lambda h: stack_cell(pitch, 0.5, 0.08 * pitch, piezo, backs, coats, h, pml_pitches=1.0)
+    lo, hi = 0.005 * pitch, 0.5 * pitch                  # mesh size bisection on log scale
+    for _ in range(30):
+        h = float(np.sqrt(lo * hi))
+        ne = build(h).quads.shape[0]
+        if abs(ne - target) < 0.05 * target:
+            break
+        lo, hi = (h, hi) if ne > target else (lo, h)
+    mesh = build(h)
+    f0 = 3100.0 / (2 * pitch * 1e-6)
+    c = _stack(mesh, mats, 'xy', 2, (0.7 * f0, 1.3 * f0), points,
+               info=dict(name=f'design_{seed}', seed=seed, pitch=pitch, backings=backs, coatings=coats,
+                         piezo_um=piezo, mesh_um=h, target_elements=target))
+    c.params = from_constants(k, euler)
+    return c
+
+
+def config5(n_cells=64, points=128):
+    """The design set: ``n_cells`` cells with seeds 0..n_cells-1 (:func:`design_cell`)."""
+    return [design_cell(s, points) for s in range(n_cells)]
+
+
 # ------------------------------------------------------------------ Hex27 slabs
 @dataclass
 class HexCell:
