@@ -259,8 +259,8 @@ def main():
     ap.add_argument('--steps', type=int, default=1)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--workload', default='config2')
-    ap.add_argument('--batch', type=int, default=0, help='frequencies per factorization chunk (0: fit 120 GB)')
-    ap.add_argument('--arena-gb', type=float, default=120.0, help='front arena per chunk when --batch is 0')
+    ap.add_argument('--batch', type=int, default=0, help='frequencies per factorization chunk (0: fit --arena-gb)')
+    ap.add_argument('--arena-gb', type=float, default=200.0, help='front arena per chunk when --batch is 0')
     ap.add_argument('--cells', type=int, default=64, help='design-set size (config5)')
     ap.add_argument('--plain-points', type=int, default=8)
     ap.add_argument('--check-points', type=int, default=0,
@@ -269,6 +269,8 @@ def main():
     ap.add_argument('--cpu-workers', type=int, default=8)
     ap.add_argument('--leaf-weight', type=int, default=0, help='nested-dissection leaf size (0: default)')
     ap.add_argument('--small-max', type=int, default=-1, help='largest front for the fused kernel (-1: default)')
+    ap.add_argument('--solve-full', type=int, default=-1,
+                    help='largest front solved by one workgroup (-1: the library default)')
     ap.add_argument('--copy-mib', type=int, default=0,
                     help='only run a plain device copy of this many MiB (HBM counter calibration) and exit')
     args = ap.parse_args()
@@ -299,6 +301,8 @@ def main():
     eng = build_engine(cell, **({'leaf_weight': args.leaf_weight} if args.leaf_weight else {}))
     if args.small_max >= 0:
         _lib.call_value('sawfem_mf_set_small', eng._plan, 0, args.small_max)
+    if args.solve_full >= 0:
+        eng.set_solve_full(args.solve_full)
     torch.cuda.synchronize()
     log(f'[bench] {args.workload}: n={eng.n_active} fronts={eng.mf.nf} setup {time.time() - t0:.1f}s '
         f'(analysis {eng.analysis_seconds:.1f}s)')

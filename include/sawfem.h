@@ -102,6 +102,12 @@ int sawfem_mf_perturbed(void* plan, int reset);
  * most 128).  Returns the threshold in use, or -1.  No counterpart in the reference (a
  * kernel-selection knob of this backend). */
 int sawfem_mf_set_small(void* plan, int prec, int max_m);
+/* In the forward passes, fronts of order <= max_m are solved by one workgroup each (pivot
+ * triangle and off-diagonal panel in one kernel) instead of the pivot-triangle + panel kernel
+ * pair; 0 keeps the pair.
+ * Returns the order in use (capped at 1024), or -1.  A tuning knob of the frontal solves
+ * (part of the PARDISO/SuperLU replacement, sweep.py:349-353); no separate reference interface. */
+int sawfem_mf_set_solve_full(void* plan, int max_m);
 
 /* Factor A(w_b) = K + i w_b beta D - w_b^2 M for B frequencies (omega: device doubles)
  * into arena (arena_elems * B entries; complex64 if prec == 0, complex128 if prec == 1).

@@ -31,6 +31,7 @@ SIGNATURES = {
     'sawfem_mf_uarena_elems': [_vp],
     'sawfem_mf_perturbed': [_vp, _i],
     'sawfem_mf_set_small': [_vp, _i, _i],
+    'sawfem_mf_set_solve_full': [_vp, _i],
     'sawfem_mf_profile_levels': [_vp, _vp, _i],
     'sawfem_mf_factor': [_vp, _vp, _i, _i, _vp, _d, _vp, _vp, _vp, _vp, _i],
     'sawfem_mf_solve': [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _i],

@@ -31,6 +31,7 @@ namespace {
 
 constexpr int TILE = 64;   // C tile edge of the GEMM workgroup (complex entries)
 constexpr int KC = 16;     // K chunk staged in LDS
+constexpr int SHORT_K = 128;   // GEMM reductions up to this length prefetch their C tile
 
 struct FrontsDev {
   const int* m;
@@ -429,13 +430,14 @@ template <> struct WaveAcc<float> {
   __device__ void zero() {
     for (int v = 0; v < 16; ++v) { re[v] = 0.f; im[v] = 0.f; }
   }
-  // As: [KC][TILE] (k-major, rows contiguous); Bs: [TILE][KC+1] (col-major over k)
+  // As: [KC][TR] (k-major, rows contiguous); Bs: [TC][KC+1] (col-major over k)
+  template <int TR>
   __device__ void step(const float2* As, const float2* Bs, int wrow, int wcol, int lane) {
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 2) {
       const int k = kk + (lane >> 5);
       const float2 bv = Bs[(wcol * 32 + (lane & 31)) * (KC + 1) + k];   // B[k][col]
-      const float2 av = As[k * TILE + wrow * 32 + (lane & 31)];        // A[row][k]
+      const float2 av = As[k * TR + wrow * 32 + (lane & 31)];          // A[row][k]
       re = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.x, av.x, re, 0, 0, 0);
       re = __builtin_amdgcn_mfma_f32_32x32x2f32(-bv.y, av.y, re, 0, 0, 0);
       im = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.x, av.y, im, 0, 0, 0);
@@ -446,8 +448,12 @@ template <> struct WaveAcc<float> {
     // D[i][j]: j = lane % 32 (row), i = 8 (v / 4) + 4 (lane / 32) + v % 4 (col)
     for (int v = 0; v < 16; ++v) {
       const int i = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-      fn(lane & 31, i, re[v], im[v]);
+      fn(v, lane & 31, i, re[v], im[v]);
     }
+  }
+  // (row, col) of the lane's 16 outputs, in the order of each()
+  template <typename Fn> static __device__ void coords(int lane, Fn fn) {
+    for (int v = 0; v < 16; ++v) fn(v, lane & 31, 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3));
   }
 };
 
@@ -459,13 +465,14 @@ template <> struct WaveAcc<double> {
       for (int c = 0; c < 2; ++c)
         for (int v = 0; v < 4; ++v) { re[a][c][v] = 0.0; im[a][c][v] = 0.0; }
   }
+  template <int TR>
   __device__ void step(const double2* As, const double2* Bs, int wrow, int wcol, int lane) {
 #pragma unroll
     for (int kk = 0; kk < KC; kk += 4) {
       const int k = kk + (lane >> 4);
       double2 bv[2], av[2];
       for (int ci = 0; ci < 2; ++ci) bv[ci] = Bs[(wcol * 32 + ci * 16 + (lane & 15)) * (KC + 1) + k];
-      for (int rj = 0; rj < 2; ++rj) av[rj] = As[k * TILE + wrow * 32 + rj * 16 + (lane & 15)];
+      for (int rj = 0; rj < 2; ++rj) av[rj] = As[k * TR + wrow * 32 + rj * 16 + (lane & 15)];
       for (int ci = 0; ci < 2; ++ci)
         for (int rj = 0; rj < 2; ++rj) {
           re[ci][rj] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[ci].x, av[rj].x, re[ci][rj], 0, 0, 0);
@@ -479,7 +486,13 @@ template <> struct WaveAcc<double> {
     // f64 16x16x4 output: D[i][j] with j = lane % 16 (row of C), i = 4 v + lane / 16 (column of C)
     for (int ci = 0; ci < 2; ++ci)
       for (int rj = 0; rj < 2; ++rj)
-        for (int v = 0; v < 4; ++v) fn(rj * 16 + (lane & 15), ci * 16 + 4 * v + (lane >> 4), re[ci][rj][v], im[ci][rj][v]);
+        for (int v = 0; v < 4; ++v)
+          fn((ci * 2 + rj) * 4 + v, rj * 16 + (lane & 15), ci * 16 + 4 * v + (lane >> 4), re[ci][rj][v], im[ci][rj][v]);
+  }
+  template <typename Fn> static __device__ void coords(int lane, Fn fn) {
+    for (int ci = 0; ci < 2; ++ci)
+      for (int rj = 0; rj < 2; ++rj)
+        for (int v = 0; v < 4; ++v) fn((ci * 2 + rj) * 4 + v, rj * 16 + (lane & 15), ci * 16 + 4 * v + (lane >> 4));
   }
 };
 
@@ -492,12 +505,15 @@ template <> struct WaveAcc<double> {
 //         row panel    F[k0:k1, k1:m] -= F[k0:k1, 0:k0] F[0:k0, k1:m]  (x tiles nc ..)
 //         with nc = ceil((mmax - k0) / TILE) from the launch (uniform over the level)
 // The inverses are the ones k_diag stored in the diagonal block.
-template <typename T, int mode>
+// C tiles are TR x TC (4 waves of 32 x 32): 64 x 64 in general; 128 x 32 and 32 x 128 for the
+// 32-wide panels of the small-front levels, so no wave works on columns outside the panel.
+template <typename T, int mode, int TR = TILE, int TC = TILE, bool PRE = false>
 __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ lvl, int B,
                                           typename cplx<T>::type* __restrict__ arena, int k0, int nb, int nc, int b) {
   using V = typename cplx<T>::type;
-  __shared__ V As[2][KC * TILE];
-  __shared__ V Bs[2][TILE * (KC + 1)];
+  static_assert((TR / 32) * (TC / 32) == 4, "four 32x32 waves per tile");
+  __shared__ V As[2][KC * TR];
+  __shared__ V Bs[2][TC * (KC + 1)];
   const int f = lvl[blockIdx.y];
   const int p = F.p[f];
   const int m = F.m[f];
@@ -523,26 +539,27 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
     else { r_lo = k0; r_hi = k1; c_lo = k1; c_hi = m; K = kb; }
   }
   if (r_lo >= r_hi || c_lo >= c_hi) return;
-  const int ntr = (r_hi - r_lo + TILE - 1) / TILE, ntc = (c_hi - c_lo + TILE - 1) / TILE;
+  const int ntr = (r_hi - r_lo + TR - 1) / TR, ntc = (c_hi - c_lo + TC - 1) / TC;
   if (mode != 4) { ti = blockIdx.x % ntr; tj = blockIdx.x / ntr; }
   if (ti >= ntr || tj >= ntc) return;
-  const int r0 = r_lo + ti * TILE, c0 = c_lo + tj * TILE;
+  const int r0 = r_lo + ti * TR, c0 = c_lo + tj * TC;
   if (mode == 0 && r0 >= p && c0 >= p) return;
   V* Fb = front_ptr<T>(arena, F, f, b, B);
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63;
-  const int wrow = wave & 1, wcol = wave >> 1;
+  constexpr int WR = TR / 32;
+  const int wrow = wave % WR, wcol = wave / WR;
   // Loads are issued unconditionally from clamped in-front addresses and selected when stored
   // to LDS, so a chunk's loads are all in flight together; the next chunk is loaded into
   // registers while the current one feeds the MFMAs (double-buffered LDS, one barrier a chunk).
-  constexpr int NE = KC * TILE / 256;
-  V ra[NE], rb[NE];
-  unsigned char sa[NE], sb[NE];   // 0: zero, 1: loaded value, 2: one (unit diagonal of Linv)
+  constexpr int NEA = KC * TR / 256, NEB = KC * TC / 256;
+  V ra[NEA], rb[NEB];
+  unsigned char sa[NEA], sb[NEB];   // 0: zero, 1: loaded value, 2: one (unit diagonal of Linv)
   auto load = [&](int kc) {
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
+    for (int e = 0; e < NEA; ++e) {
       const int idx = t + 256 * e;
-      const int row = idx % TILE, k = idx / TILE;
+      const int row = idx % TR, k = idx / TR;
       const int gr = r0 + row, gk = kc + k;
       const bool ok = gr < r_hi && gk < K;
       const int grc = min(gr, r_hi - 1), gkc = min(gk, K - 1);
@@ -559,7 +576,7 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
       }
     }
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
+    for (int e = 0; e < NEB; ++e) {
       const int idx = t + 256 * e;
       const int k = idx % KC, col = idx / KC;
       const int gk = kc + k, gc = c0 + col;
@@ -575,13 +592,13 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
+    for (int e = 0; e < NEA; ++e) {
       const int idx = t + 256 * e;
-      const int row = idx % TILE, k = idx / TILE;
-      As[buf][k * TILE + row] = sa[e] == 1 ? ra[e] : (sa[e] == 2 ? V{1, 0} : czero<V>());
+      const int row = idx % TR, k = idx / TR;
+      As[buf][k * TR + row] = sa[e] == 1 ? ra[e] : (sa[e] == 2 ? V{1, 0} : czero<V>());
     }
 #pragma unroll
-    for (int e = 0; e < NE; ++e) {
+    for (int e = 0; e < NEB; ++e) {
       const int idx = t + 256 * e;
       const int k = idx % KC, col = idx / KC;
       Bs[buf][col * (KC + 1) + k] = sb[e] ? rb[e] : czero<V>();
@@ -591,36 +608,48 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
   acc.zero();
   const int nch = (K + KC - 1) / KC;
   load(0);
+  // read-modify-write modes: with PRE (short K) the C tile is fetched with the first chunk
+  // (clamped in-front addresses) so its latency overlaps the K loop instead of following it
+  constexpr bool RMW = mode == 0 || mode == 1 || mode == 4;
+  V cpre[RMW && PRE ? 16 : 1];
+  if (RMW && PRE)
+    WaveAcc<T>::coords(lane, [&](int v, int j, int i) {
+      const int gr = min(r0 + wrow * 32 + j, r_hi - 1), gc = min(c0 + wcol * 32 + i, c_hi - 1);
+      cpre[v] = Fb[(int64_t)gr + (int64_t)gc * m];
+    });
   store(0);
   __syncthreads();
   for (int c = 0; c < nch; ++c) {
     const int buf = c & 1;
     if (c + 1 < nch) load((c + 1) * KC);
-    acc.step(As[buf], Bs[buf], wrow, wcol, lane);
+    acc.template step<TR>(As[buf], Bs[buf], wrow, wcol, lane);
     if (c + 1 < nch) store(buf ^ 1);
     __syncthreads();
   }
-  acc.each(lane, [&](int j, int i, T re, T im) {
+  acc.each(lane, [&](int v, int j, int i, T re, T im) {
     const int gr = r0 + wrow * 32 + j;
     const int gc = c0 + wcol * 32 + i;
     if (gr < r_hi && gc < c_hi && !(mode == 0 && gr >= p && gc >= p)) {
       V* c = Fb + (int64_t)gr + (int64_t)gc * m;
-      if (mode == 2 || mode == 3) {
-        V v; v.x = re; v.y = im;
-        *c = v;
+      V o;
+      if (RMW && PRE) {
+        o = cpre[v];
+        o.x -= re; o.y -= im;
+      } else if (RMW) {
+        o = *c;
+        o.x -= re; o.y -= im;
       } else {
-        V v = *c;
-        v.x -= re; v.y -= im;
-        *c = v;
+        o.x = re; o.y = im;
       }
+      *c = o;
     }
   });
 }
 
-template <typename T, int mode>
+template <typename T, int mode, int TR = TILE, int TC = TILE, bool PRE = false>
 __global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
                                               typename cplx<T>::type* __restrict__ arena, int k0, int nb, int nc) {
-  gemm_body<T, mode>(F, lvl, B, arena, k0, nb, nc, blockIdx.z);
+  gemm_body<T, mode, TR, TC, PRE>(F, lvl, B, arena, k0, nb, nc, blockIdx.z);
 
 }
 
@@ -1108,6 +1137,7 @@ struct Level {
   int64_t e0, e1;
   const int* fronts;             // device pointer into lvl_fronts
   Sub small[2], big[2];          // per precision (0 complex64, 1 complex128): fused / panel path
+  Sub ssmall[2], sbig[2];        // solves: one-workgroup (pivot triangle + panel) / two-kernel fronts
 };
 
 // Optional per-kernel timing with HIP events on the launch stream (kind ids below).
@@ -1141,6 +1171,7 @@ struct Plan {
   int small_limit[2] = {0, 0};   // device LDS bound per precision
   int lds_max = 64 * 1024;       // LDS one workgroup may allocate on this device
   int small_mmax[2] = {0, 0};    // threshold in use
+  int solve_full_m = 0;          // fronts up to this order are solved by one workgroup each
   std::vector<void*> part_owned[2];
   std::vector<int> hm, hp, hlvl_ptr, hlvl_fronts;
 };
@@ -1199,12 +1230,15 @@ int build_partition(Plan* P, int pi, int max_m) {
     *dst = (const int*)d;
     return 0;
   };
+  // the solves' one-workgroup set must fit the pivot-triangle kernel's LDS with the full front
+  const int full_m = std::min(std::max(P->solve_full_m, P->small_mmax[pi]), 1024);
   for (size_t l = 0; l < P->levels.size(); ++l) {
     Level& L = P->levels[l];
-    std::vector<int> sm, bg;
+    std::vector<int> sm, bg, ssm, sbg;
     for (int k = P->hlvl_ptr[l]; k < P->hlvl_ptr[l + 1]; ++k) {
       const int f = P->hlvl_fronts[k];
       (P->hm[f] <= P->small_mmax[pi] ? sm : bg).push_back(f);
+      (P->hm[f] <= full_m ? ssm : sbg).push_back(f);
     }
     auto stats = [&](const std::vector<int>& v, Sub& S) {
       S = Sub{};
@@ -1217,7 +1251,11 @@ int build_partition(Plan* P, int pi, int max_m) {
     };
     stats(sm, L.small[pi]);
     stats(bg, L.big[pi]);
-    if (put(sm, &L.small[pi].fronts) || put(bg, &L.big[pi].fronts)) return 3;
+    stats(ssm, L.ssmall[pi]);
+    stats(sbg, L.sbig[pi]);
+    if (put(sm, &L.small[pi].fronts) || put(bg, &L.big[pi].fronts) || put(ssm, &L.ssmall[pi].fronts) ||
+        put(sbg, &L.sbig[pi].fronts))
+      return 3;
     for (int f : sm) flag[f] = 1;
   }
   void* d = nullptr;
@@ -1261,13 +1299,40 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
     // left-looking panels: update panel k0 with the factored panels before it (one GEMM with
     // K = k0 for its column and row parts), then its diagonal LU + inverses, then the two
     // panel solves; the contribution block waits for the final Schur complement
+    // 32-wide panels of fronts with long column/row parts use 128 x 32 / 32 x 128 tiles; on the
+    // levels of short fronts (all m < 192) the 64 x 64 tiles measured faster
+    const bool narrow = nbl == 32 && G.mmax >= 192;
     for (int k0 = 0; k0 < G.pmax; k0 += nbl) {
       if (k0 > 0) {
-        const int nc = (G.mmax - k0 + TILE - 1) / TILE;
-        const int nr = (G.mmax - k0 - 1 + TILE - 1) / TILE;
         Timed tm(P, s, K_GEMM_L);
-        hipLaunchKernelGGL((k_gemm<T, 4>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl,
-                           nc);
+        const bool pre = k0 <= SHORT_K;   // short reduction (K = k0): prefetch the C tile
+        if (narrow) {         // column part in 128 x 32 tiles, row part in 32 x 128 tiles
+          const int nc = (G.mmax - k0 + 127) / 128;
+          const int nr = (G.mmax - k0 - 1 + 127) / 128;
+          if (pre)
+            hipLaunchKernelGGL((k_gemm<T, 4, 128, 32, true>), dim3(nc, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
+                               arena, k0, nbl, nc);
+          else
+            hipLaunchKernelGGL((k_gemm<T, 4, 128, 32>), dim3(nc, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena,
+                               k0, nbl, nc);
+          if (nr > 0) {
+            if (pre)
+              hipLaunchKernelGGL((k_gemm<T, 4, 32, 128, true>), dim3(nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
+                                 arena, k0, nbl, 0);
+            else
+              hipLaunchKernelGGL((k_gemm<T, 4, 32, 128>), dim3(nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
+                                 arena, k0, nbl, 0);
+          }
+        } else {
+          const int nc = (G.mmax - k0 + TILE - 1) / TILE;
+          const int nr = (G.mmax - k0 - 1 + TILE - 1) / TILE;
+          if (pre)
+            hipLaunchKernelGGL((k_gemm<T, 4, TILE, TILE, true>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F,
+                               G.fronts, B, arena, k0, nbl, nc);
+          else
+            hipLaunchKernelGGL((k_gemm<T, 4>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
+                               nbl, nc);
+        }
       }
       const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
       { Timed tm(P, s, K_DIAG);
@@ -1275,16 +1340,28 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
                          P->npert); }
       const int rem = G.mmax - k0 - 1;
       if (rem > 0) {
-        const int nt = (rem + TILE - 1) / TILE;
         Timed tm(P, s, K_TRSM);
-        hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
-        hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
+        if (narrow) {
+          const int nt = (rem + 127) / 128;
+          hipLaunchKernelGGL((k_gemm<T, 2, 128, 32>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
+                             nbl, 0);
+          hipLaunchKernelGGL((k_gemm<T, 3, 32, 128>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
+                             nbl, 0);
+        } else {
+          const int nt = (rem + TILE - 1) / TILE;
+          hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
+          hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
+        }
       }
     }
     if (G.cbmax > 0) {
       Timed tm(P, s, K_GEMM_S);
       const int nt = (G.cbmax + TILE - 1) / TILE;
-      hipLaunchKernelGGL((k_gemm<T, 1>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, 0, nb, 0);
+      if (G.pmax <= SHORT_K)   // short reduction: the contribution-block read would follow the K loop
+        hipLaunchKernelGGL((k_gemm<T, 1, TILE, TILE, true>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
+                           arena, 0, nb, 0);
+      else
+        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, 0, nb, 0);
     }
   }
   SAW_HIP(hipGetLastError());
@@ -1302,8 +1379,12 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   double2* X = (double2*)x;
   // fronts of the fused set (m <= the small-front threshold): one workgroup does the whole
   // front; the others: pivot-triangle kernel + panel kernel
+  // forward passes take the one-workgroup set of the solve partition; backward passes keep the
+  // factorization's partition (their CB-to-pivot product has only p active rows per workgroup,
+  // which measured slower than the panel kernel on the small fronts)
   auto tri = [&](const Level& L, int kind, bool full) {
-    const Sub& S = full ? L.small[pi] : L.big[pi];
+    const bool fw = kind == 0 || kind == 2;
+    const Sub& S = full ? (fw ? L.ssmall[pi] : L.small[pi]) : (fw ? L.sbig[pi] : L.big[pi]);
     if (S.nf == 0) return;
     Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
     const int nbl = panel_width(nb, L);
@@ -1328,7 +1409,7 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
 #undef SAW_TRI
   };
   auto panel = [&](const Level& L, int kind) {
-    const Sub& S = L.big[pi];
+    const Sub& S = (kind == 0 || kind == 2) ? L.sbig[pi] : L.big[pi];
     const bool out_cb = (kind == 0 || kind == 2);
     const int nout = out_cb ? S.cbmax : S.pmax;
     const int nin = out_cb ? S.pmax : S.cbmax;
@@ -1510,6 +1591,19 @@ int sawfem_mf_set_small(void* plan, int prec, int max_m) {
   if (!P || (prec != 0 && prec != 1)) { sawfem_set_error("sawfem_mf_set_small: invalid arguments"); return -1; }
   if (build_partition(P, prec, max_m)) return -1;
   return P->small_mmax[prec];
+}
+
+// In the forward passes, fronts of order <= max_m are solved by one workgroup each (pivot
+// triangle and off-diagonal panel together, k_tri with FULL) instead of the pivot-triangle +
+// panel kernel pair; 0 keeps the pair for every front that the factorization did not fuse.
+// Returns the order in use.
+int sawfem_mf_set_solve_full(void* plan, int max_m) {
+  Plan* P = (Plan*)plan;
+  if (!P || max_m < 0) { sawfem_set_error("sawfem_mf_set_solve_full: invalid arguments"); return -1; }
+  P->solve_full_m = std::min(max_m, 1024);
+  for (int pi = 0; pi < 2; ++pi)
+    if (build_partition(P, pi, P->small_mmax[pi])) return -1;
+  return P->solve_full_m;
 }
 
 int64_t sawfem_mf_arena_elems(void* plan) { return plan ? ((Plan*)plan)->arena_elems : -1; }

@@ -286,7 +286,7 @@ class FrequencySolver3D(_frequency_solver_cls()):
     """
 
     def __init__(self, K: DeviceCSR, M: DeviceCSR, boundary: Boundary, t_fb=None, t_lr=None, *, period=None,
-                 leaf_weight=96):
+                 leaf_weight=64):
         from .layout import BlochLayout
         kl = K.layout
         if kl.numbering != 'interleaved':

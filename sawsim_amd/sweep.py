@@ -38,6 +38,11 @@ from .plan import build_mf_arrays
 from .symbolic import analyse
 
 
+# in the forward passes, fronts up to this order are solved by one workgroup (pivot triangle +
+# panel) per frequency (measured on config 2: 192 is the best of 0/128/192/256)
+SOLVE_FULL_M = 192
+
+
 def ir_error(dq, dq_prev):
     """Estimated relative error of a refined quantity after its latest correction.
 
@@ -85,7 +90,7 @@ class FrequencySolver:
     n_params = 17          # dY11/dp parameters (params.NAMES)
 
     def __init__(self, K: DeviceCSR, M: DeviceCSR, fix, zheng, fu, merge_groups, *, node_xy=None, period=None,
-                 leaf_weight=96):
+                 leaf_weight=64):
         bl = BlochLayout(K.layout, fix, zheng, fu, merge_groups)
         xy = node_xy if node_xy is not None else getattr(K, 'node_xy', None)
         if xy is None:
@@ -132,7 +137,12 @@ class FrequencySolver:
         self.mf = build_mf_arrays(self.symbolic, bl.aa_indptr, bl.aa_indices)
         self.analysis_seconds = time.time() - t0
         self._plan = self._create_plan()
+        self.set_solve_full(SOLVE_FULL_M)
         self.stats = dict(self.symbolic.stats)
+
+    def set_solve_full(self, max_m):
+        """Solve fronts of order <= ``max_m`` with one workgroup each (``sawfem_mf_set_solve_full``)."""
+        return _lib.call_value('sawfem_mf_set_solve_full', self._plan, int(max_m))
 
     # ---------------------------------------------------------------- plan
     def _create_plan(self):

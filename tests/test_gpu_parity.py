@@ -221,3 +221,21 @@ def test_solve_on_a_frequency_subset():
     for b in range(B):
         want = full[b] if b in (1, 4) else rhs[b]
         assert torch.equal(part[b], want), b
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'mixed'])
+def test_one_workgroup_solves_match_the_kernel_pair(precision):
+    """Solving the small fronts with one workgroup each (pivot triangle + panel together) gives
+    the same sweep as the pivot-triangle + panel kernel pair, forward and paired adjoint."""
+    _say(f'[test_one_workgroup_solves_match_the_kernel_pair] {precision}')
+    cell = workloads.config3(mesh_um=0.12, points=4)
+    eng = workloads.build_engine(cell, gradient=True)
+    om = 2 * np.pi * cell.frequencies(4)
+    assert eng.set_solve_full(0) == 0
+    a = eng.solve_batch(om, precision=precision, gradient=True)
+    assert eng.set_solve_full(400) == 400
+    b = eng.solve_batch(om, precision=precision, gradient=True)
+    eng.set_solve_full(0)
+    tol = 1e-12 if precision == 'fp64' else 1e-9
+    assert np.max(np.abs(a['Q'] - b['Q']) / np.abs(a['Q'])) <= tol
+    assert np.max(np.abs(a['dY'] - b['dY']).max(axis=1) / np.abs(a['dY']).max(axis=1)) <= 100 * tol
