@@ -1136,8 +1136,13 @@ struct Level {
   int nf, mmax, pmax, cbmax;     // all fronts of the level (solves, panel width)
   int64_t e0, e1;
   const int* fronts;             // device pointer into lvl_fronts
-  Sub small[2], big[2];          // per precision (0 complex64, 1 complex128): fused / panel path
-  Sub ssmall[2], sbig[2];        // solves: one-workgroup (pivot triangle + panel) / two-kernel fronts
+  // per precision (0 complex64, 1 complex128): the fused set and the panel-path fronts, the
+  // latter in size buckets so that each launch's grid fits its fronts
+  Sub small[2];
+  std::vector<Sub> big[2];
+  // solves: one-workgroup (pivot triangle + panel) set and the two-kernel fronts, bucketed
+  Sub ssmall[2];
+  std::vector<Sub> sbig[2];
 };
 
 // Optional per-kernel timing with HIP events on the launch stream (kind ids below).
@@ -1249,12 +1254,28 @@ int build_partition(Plan* P, int pi, int max_m) {
         S.cbmax = std::max(S.cbmax, P->hm[f] - P->hp[f]);
       }
     };
+    // size buckets by front order (edges roughly 1.7x apart): fronts of one bucket need grids
+    // of similar size, so few workgroups of a launch find no tile
+    auto buckets = [&](std::vector<int> v, std::vector<Sub>& out) -> int {
+      out.clear();
+      std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return P->hm[a] < P->hm[b]; });
+      static const int edges[] = {64, 128, 192, 320, 576, 1088, 1 << 30};
+      size_t i = 0;
+      for (int e : edges) {
+        std::vector<int> part;
+        while (i < v.size() && P->hm[v[i]] <= e) part.push_back(v[i++]);
+        if (part.empty()) continue;
+        Sub S;
+        stats(part, S);
+        if (put(part, &S.fronts)) return 3;
+        out.push_back(S);
+      }
+      return 0;
+    };
     stats(sm, L.small[pi]);
-    stats(bg, L.big[pi]);
     stats(ssm, L.ssmall[pi]);
-    stats(sbg, L.sbig[pi]);
-    if (put(sm, &L.small[pi].fronts) || put(bg, &L.big[pi].fronts) || put(ssm, &L.ssmall[pi].fronts) ||
-        put(sbg, &L.sbig[pi].fronts))
+    if (put(sm, &L.small[pi].fronts) || put(ssm, &L.ssmall[pi].fronts) || buckets(bg, L.big[pi]) ||
+        buckets(sbg, L.sbig[pi]))
       return 3;
     for (int f : sm) flag[f] = 1;
   }
@@ -1285,74 +1306,77 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
                          P->fent_lo, P->fent_hi, P->ent_slot, P->ent_loc, (const double2*)Kaa, (const double2*)Maa,
                          (const double2*)Daa, arena, nbl, tiny, P->npert);
     }
-    const Sub& G = L.big[pi];
-    if (G.nf == 0) continue;
-    const int cpw = 8;
-    { Timed tm(P, s, K_INIT);
-    hipLaunchKernelGGL(k_front_init<T>, dim3((G.mmax + cpw - 1) / cpw, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
-                       arena, cpw); }
+    if (L.big[pi].empty()) continue;
+    for (const Sub& G : L.big[pi]) {
+      // columns per workgroup: about 4096 entries of the largest front, in whole batches of 8
+      const int cpw = 8 * std::max(1, std::min(8, (4096 / std::max(1, G.mmax) + 4) / 8));
+      { Timed tm(P, s, K_INIT);
+      hipLaunchKernelGGL(k_front_init<T>, dim3((G.mmax + cpw - 1) / cpw, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
+                         arena, cpw); }
+    }
     if (L.e1 > L.e0) { Timed tm(P, s, K_ENTRIES);
       hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), (B + ENT_FB - 1) / ENT_FB),
                          dim3(256), 0, s, P->F,
                          L.e0, L.e1, B, omega, beta, P->ent_front, P->ent_slot, P->ent_loc, (const double2*)Kaa,
                          (const double2*)Maa, (const double2*)Daa, arena, P->fused[pi]); }
-    // left-looking panels: update panel k0 with the factored panels before it (one GEMM with
-    // K = k0 for its column and row parts), then its diagonal LU + inverses, then the two
-    // panel solves; the contribution block waits for the final Schur complement
-    // 32-wide panels of fronts with long column/row parts use 128 x 32 / 32 x 128 tiles; on the
-    // levels of short fronts (all m < 192) the 64 x 64 tiles measured faster
-    const bool narrow = nbl == 32 && G.mmax >= 192;
-    for (int k0 = 0; k0 < G.pmax; k0 += nbl) {
-      if (k0 > 0) {
-        Timed tm(P, s, K_GEMM_L);
-        const bool pre = k0 <= SHORT_K;   // short reduction (K = k0): prefetch the C tile
-        if (narrow) {         // column part in 128 x 32 tiles, row part in 32 x 128 tiles
-          const int nc = (G.mmax - k0 + 127) / 128;
-          const int nr = (G.mmax - k0 - 1 + 127) / 128;
-          if (pre)
-            hipLaunchKernelGGL((k_gemm<T, 4, 128, 32, true>), dim3(nc, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
-                               arena, k0, nbl, nc);
-          else
-            hipLaunchKernelGGL((k_gemm<T, 4, 128, 32>), dim3(nc, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena,
-                               k0, nbl, nc);
-          if (nr > 0) {
+    for (const Sub& G : L.big[pi]) {
+      // left-looking panels: update panel k0 with the factored panels before it (one GEMM with
+      // K = k0 for its column and row parts), then its diagonal LU + inverses, then the two
+      // panel solves; the contribution block waits for the final Schur complement
+      // 32-wide panels of fronts with long column/row parts use 128 x 32 / 32 x 128 tiles; on the
+      // levels of short fronts (all m < 192) the 64 x 64 tiles measured faster
+      const bool narrow = nbl == 32 && G.mmax >= 192;
+      for (int k0 = 0; k0 < G.pmax; k0 += nbl) {
+        if (k0 > 0) {
+          Timed tm(P, s, K_GEMM_L);
+          const bool pre = k0 <= SHORT_K;   // short reduction (K = k0): prefetch the C tile
+          if (narrow) {         // column part in 128 x 32 tiles, row part in 32 x 128 tiles
+            const int nc = (G.mmax - k0 + 127) / 128;
+            const int nr = (G.mmax - k0 - 1 + 127) / 128;
             if (pre)
-              hipLaunchKernelGGL((k_gemm<T, 4, 32, 128, true>), dim3(nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
-                                 arena, k0, nbl, 0);
+              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32, true>), dim3(nc, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
+                                 arena, k0, nbl, nc);
             else
-              hipLaunchKernelGGL((k_gemm<T, 4, 32, 128>), dim3(nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
-                                 arena, k0, nbl, 0);
+              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32>), dim3(nc, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena,
+                                 k0, nbl, nc);
+            if (nr > 0) {
+              if (pre)
+                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128, true>), dim3(nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
+                                   arena, k0, nbl, 0);
+              else
+                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128>), dim3(nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
+                                   arena, k0, nbl, 0);
+            }
+          } else {
+            const int nc = (G.mmax - k0 + TILE - 1) / TILE;
+            const int nr = (G.mmax - k0 - 1 + TILE - 1) / TILE;
+            if (pre)
+              hipLaunchKernelGGL((k_gemm<T, 4, TILE, TILE, true>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F,
+                                 G.fronts, B, arena, k0, nbl, nc);
+            else
+              hipLaunchKernelGGL((k_gemm<T, 4>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
+                                 nbl, nc);
           }
-        } else {
-          const int nc = (G.mmax - k0 + TILE - 1) / TILE;
-          const int nr = (G.mmax - k0 - 1 + TILE - 1) / TILE;
-          if (pre)
-            hipLaunchKernelGGL((k_gemm<T, 4, TILE, TILE, true>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F,
-                               G.fronts, B, arena, k0, nbl, nc);
-          else
-            hipLaunchKernelGGL((k_gemm<T, 4>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
-                               nbl, nc);
         }
-      }
-      const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
-      { Timed tm(P, s, K_DIAG);
-      hipLaunchKernelGGL(k_diag<T>, dim3(G.nf, B), dim3(128), lds_diag, s, P->F, G.fronts, B, arena, k0, nbl, tiny,
-                         P->npert); }
-      const int rem = G.mmax - k0 - 1;
-      if (rem > 0) {
-        Timed tm(P, s, K_TRSM);
-        if (narrow) {
-          const int nt = (rem + 127) / 128;
-          hipLaunchKernelGGL((k_gemm<T, 2, 128, 32>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
-                             nbl, 0);
-          hipLaunchKernelGGL((k_gemm<T, 3, 32, 128>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
-                             nbl, 0);
-        } else {
-          const int nt = (rem + TILE - 1) / TILE;
-          hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
-          hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
+        const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
+        { Timed tm(P, s, K_DIAG);
+        hipLaunchKernelGGL(k_diag<T>, dim3(G.nf, B), dim3(128), lds_diag, s, P->F, G.fronts, B, arena, k0, nbl, tiny,
+                           P->npert); }
+        const int rem = G.mmax - k0 - 1;
+        if (rem > 0) {
+          Timed tm(P, s, K_TRSM);
+          if (narrow) {
+            const int nt = (rem + 127) / 128;
+            hipLaunchKernelGGL((k_gemm<T, 2, 128, 32>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
+                               nbl, 0);
+            hipLaunchKernelGGL((k_gemm<T, 3, 32, 128>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
+                               nbl, 0);
+          } else {
+            const int nt = (rem + TILE - 1) / TILE;
+            hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
+            hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
+          }
         }
-      }
     }
     if (G.cbmax > 0) {
       Timed tm(P, s, K_GEMM_S);
@@ -1362,6 +1386,7 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
                            arena, 0, nb, 0);
       else
         hipLaunchKernelGGL((k_gemm<T, 1>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, 0, nb, 0);
+    }
     }
   }
   SAW_HIP(hipGetLastError());
@@ -1382,9 +1407,7 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   // forward passes take the one-workgroup set of the solve partition; backward passes keep the
   // factorization's partition (their CB-to-pivot product has only p active rows per workgroup,
   // which measured slower than the panel kernel on the small fronts)
-  auto tri = [&](const Level& L, int kind, bool full) {
-    const bool fw = kind == 0 || kind == 2;
-    const Sub& S = full ? (fw ? L.ssmall[pi] : L.small[pi]) : (fw ? L.sbig[pi] : L.big[pi]);
+  auto tri_sub = [&](const Level& L, const Sub& S, int kind, bool full) {
     if (S.nf == 0) return;
     Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
     const int nbl = panel_width(nb, L);
@@ -1408,20 +1431,29 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     }
 #undef SAW_TRI
   };
+  auto tri = [&](const Level& L, int kind, bool full) {
+    const bool fw = kind == 0 || kind == 2;
+    if (full) {
+      tri_sub(L, fw ? L.ssmall[pi] : L.small[pi], kind, true);
+      return;
+    }
+    for (const Sub& S : (fw ? L.sbig[pi] : L.big[pi])) tri_sub(L, S, kind, false);
+  };
   auto panel = [&](const Level& L, int kind) {
-    const Sub& S = (kind == 0 || kind == 2) ? L.sbig[pi] : L.big[pi];
     const bool out_cb = (kind == 0 || kind == 2);
-    const int nout = out_cb ? S.cbmax : S.pmax;
-    const int nin = out_cb ? S.pmax : S.cbmax;
-    if (S.nf == 0 || nout == 0 || nin == 0) return;
-    const size_t lds = sizeof(double2) * R * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
-    Timed tm(P, s, out_cb ? K_FWD : K_BWD);
-    dim3 grid((nout + 63) / 64, S.nf, nsel);
-    switch (kind) {
-      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
-      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
-      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
-      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel);
+    for (const Sub& S : (out_cb ? L.sbig[pi] : L.big[pi])) {
+      const int nout = out_cb ? S.cbmax : S.pmax;
+      const int nin = out_cb ? S.pmax : S.cbmax;
+      if (S.nf == 0 || nout == 0 || nin == 0) continue;
+      const size_t lds = sizeof(double2) * R * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
+      Timed tm(P, s, out_cb ? K_FWD : K_BWD);
+      dim3 grid((nout + 63) / 64, S.nf, nsel);
+      switch (kind) {
+        case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
+        case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
+        case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
+        default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel);
+      }
     }
   };
   const int kf = transpose ? 2 : 0, kb = transpose ? 3 : 1;
