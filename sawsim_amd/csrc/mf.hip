@@ -724,10 +724,12 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
                                          double2* __restrict__ uarena, double2* __restrict__ x, int b) {
   using V = typename cplx<T>::type;
   const int LD = nbl + 1;
+  constexpr bool ROWTILE = KIND >= 2;          // the tile also stages 64 columns of a row panel
   extern __shared__ unsigned char smem_raw[];
-  V* tile = reinterpret_cast<V*>(smem_raw);                                   // [64][LD]
-  double2* part = reinterpret_cast<double2*>(smem_raw + sizeof(V) * 64 * LD);  // [R][4][64]
-  double2* w = part + R * 256;                                                // [R][W]
+  V* tile = reinterpret_cast<V*>(smem_raw);                                   // [nbl or 64][LD]
+  double2* part = reinterpret_cast<double2*>(smem_raw + sizeof(V) * (ROWTILE ? 64 : nbl) * LD);  // [R][4][64]
+  const int PS = nbl <= 32 ? 32 : 64;                                         // rows of a block
+  double2* w = part + R * 4 * PS;                                             // [R][W]
   const int f = lvl[blockIdx.x];
   const int m = F.m[f], p = F.p[f];
   const int W = FULL ? m : p;
@@ -801,11 +803,11 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
     }
     __syncthreads();
     {
-      const int i = t & 63, g = t >> 6;   // output row, quarter of the reduction
+      const int i = t % PS, g = t / PS;   // output row, quarter of the reduction (g < 4)
       double2 s[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) s[r] = make_double2(0, 0);
-      if (i < nbk) {
+      if (g < 4 && i < nbk) {
         if (KIND == 0) {          // L_D^{-1} (unit lower): w_i + sum_{k<i} Li[i][k] w_k
           for (int k = g; k < i; k += 4) {
             const double2 a = to_d2(tile[i + k * LD]);
@@ -832,15 +834,17 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
           }
         }
       }
+      if (g < 4) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) part[r * 256 + g * 64 + i] = s[r];
+        for (int r = 0; r < R; ++r) part[r * 4 * PS + g * PS + i] = s[r];
+      }
     }
     __syncthreads();
     if (t < nbk) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        double2 tot = part[r * 256 + t];
-        for (int g = 1; g < 4; ++g) tot = d2add(tot, part[r * 256 + g * 64 + t]);
+        double2 tot = part[r * 4 * PS + t];
+        for (int g = 1; g < 4; ++g) tot = d2add(tot, part[r * 4 * PS + g * PS + t]);
         if (KIND == 0 || KIND == 3) tot = d2add(tot, w[r * W + J0 + t]);
         w[r * W + J0 + t] = tot;
       }
@@ -1411,7 +1415,11 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     if (S.nf == 0) return;
     Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
     const int nbl = panel_width(nb, L);
-    const size_t lds = sizeof(V) * 64 * (nbl + 1) + sizeof(double2) * R * (256 + (full ? S.mmax : S.pmax));
+    // kinds 0/1 keep only the nbl x nbl diagonal block in the tile; kinds 2/3 also stage 64
+    // columns of a row panel through it
+    const int tile_rows = kind <= 1 ? nbl : 64;
+    const int ps = nbl <= 32 ? 32 : 64;     // partial sums: 4 x ps per vector
+    const size_t lds = sizeof(V) * tile_rows * (nbl + 1) + sizeof(double2) * R * (4 * ps + (full ? S.mmax : S.pmax));
     const dim3 grid(S.nf, nsel);
     // above 64 KB the kernel must be allowed the larger dynamic LDS (checked against the device
     // limit by sawfem_mf_solve)
