@@ -1327,9 +1327,9 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
       // left-looking panels: update panel k0 with the factored panels before it (one GEMM with
       // K = k0 for its column and row parts), then its diagonal LU + inverses, then the two
       // panel solves; the contribution block waits for the final Schur complement
-      // 32-wide panels of fronts with long column/row parts use 128 x 32 / 32 x 128 tiles; on the
-      // levels of short fronts (all m < 192) the 64 x 64 tiles measured faster
-      const bool narrow = nbl == 32 && G.mmax >= 192;
+      // 32-wide panels use 128 x 32 / 32 x 128 tiles (no wave on columns outside the panel);
+      // the bucket of fronts with m <= 64 keeps 64 x 64 tiles (measured equal or better there)
+      const bool narrow = nbl == 32 && G.mmax > 64;
       for (int k0 = 0; k0 < G.pmax; k0 += nbl) {
         if (k0 > 0) {
           Timed tm(P, s, K_GEMM_L);
