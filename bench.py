@@ -31,7 +31,7 @@ PEAK_FP64_MFMA = 78.6
 # HBM bytes per launch of the Schur GEMM k_gemm<float, 1> on the config-2 / config-4 sweeps:
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (separate runs), each scaled by what the counter reads for
 # a plain 1 GiB device copy (`--copy-mib 1024`: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000)
-SCHUR_TRAFFIC_BYTES = {'config2': 2.991e10, 'config4': 2.784e10}
+SCHUR_TRAFFIC_BYTES = {'config2': 1.848e10, 'config4': 2.784e10}
 
 
 def log(*a):
@@ -412,7 +412,7 @@ def main():
                       frac=(achieved / PEAK_FP32_MFMA) if achieved else None,
                       traffic=SCHUR_TRAFFIC_BYTES.get(args.workload),
                       traffic_note='HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy',
-                      kernel='k_gemm<float, 1> (Schur complement, complex64 on FP32 MFMA)',
+                      kernel='k_gemm<float, 1, 64, 64, *> (Schur complement, complex64 on FP32 MFMA)',
                       launches=schur_n, kernel_ms=schur_ms, algorithmic_flops=schur_flops,
                       avg_launch_ms=schur_ms / max(schur_n, 1)),
         other_kernels=dict(
