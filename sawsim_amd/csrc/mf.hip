@@ -197,37 +197,45 @@ __global__ void __launch_bounds__(256) k_front_entries(FrontsDev F, int64_t e0, 
 //     diagonal entries become 1/U[k][k] at the end
 // h >= n; all threads of the block call it (it synchronizes).
 template <typename V>
-__device__ void invert_lu_block(V* D, int ld, int n, int h) {
-  const int t = threadIdx.x;
+__device__ void invert_lu_block(V* D, int ld, int n) {
+  // In-place inverses of the unit-lower L and upper U held in D (LDS): afterwards the strict
+  // lower triangle holds L^{-1} (unit diagonal implied) and the upper triangle U^{-1}.  Step st
+  // eliminates column st of L and column n-1-st of U (Gauss-Jordan); the two halves touch
+  // disjoint triangles, and each step's updates are spread over all threads.
+  const int t = threadIdx.x, nt = blockDim.x;
   const V one{1, 0};
   for (int st = 0; st < n; ++st) {
-    // phase a: row k of Y scaled by d = 1/U[k][k], one column per thread; the diagonal itself
-    // keeps U[k][k] until the end (phase b of later steps never reads it)
-    if (t >= h && t < 2 * h) {
-      const int k = n - 1 - st, j = t - h;
-      if (j > k && j < n) D[k + j * ld] = cmul(D[k + j * ld], cdiv(one, D[k + k * ld]));
+    const int kL = st, kU = n - 1 - st;
+    const V dU = cdiv(one, D[kU + kU * ld]);          // U[kU][kU] stays in place until the end
+    // a: row kU of U right of the diagonal scaled by 1 / U[kU][kU]
+    for (int j = kU + 1 + t; j < n; j += nt) D[kU + j * ld] = cmul(D[kU + j * ld], dU);
+    __syncthreads();
+    // b: L rows i > kL, columns j < kL and U rows i < kU, columns j > kU
+    const int aL = n - 1 - kL, cL = aL * kL;
+    const int aU = kU, cU = aU * (n - 1 - kU);
+    for (int idx = t; idx < cL + cU; idx += nt) {
+      if (idx < cL) {
+        const int i = kL + 1 + idx % aL, j = idx / aL;
+        D[i + j * ld] = cfms(D[i + j * ld], D[i + kL * ld], D[kL + j * ld]);
+      } else {
+        const int q = idx - cL;
+        const int i = q % aU, j = kU + 1 + q / aU;
+        D[i + j * ld] = cfms(D[i + j * ld], D[i + kU * ld], D[kU + j * ld]);
+      }
     }
     __syncthreads();
-    if (t < h) {
-      const int k = st, i = t;
-      if (i > k && i < n) {
-        const V mk = D[i + k * ld];
-        for (int j = 0; j < k; ++j) D[i + j * ld] = cfms(D[i + j * ld], mk, D[k + j * ld]);
-        D[i + k * ld] = V{-mk.x, -mk.y};
-      }
-    } else if (t < 2 * h) {
-      const int k = n - 1 - st, i = t - h;
-      if (i < k) {
-        const V mk = D[i + k * ld];
-        const V d = cdiv(one, D[k + k * ld]);
-        for (int j = k + 1; j < n; ++j) D[i + j * ld] = cfms(D[i + j * ld], mk, D[k + j * ld]);
-        const V y = cmul(mk, d);
-        D[i + k * ld] = V{-y.x, -y.y};
-      }
+    // c: the eliminated columns
+    for (int i = kL + 1 + t; i < n; i += nt) {
+      const V v = D[i + kL * ld];
+      D[i + kL * ld] = V{-v.x, -v.y};
+    }
+    for (int i = t; i < kU; i += nt) {
+      const V y = cmul(D[i + kU * ld], dU);
+      D[i + kU * ld] = V{-y.x, -y.y};
     }
     __syncthreads();
   }
-  if (t >= h && t - h < n) D[(t - h) * (ld + 1)] = cdiv(one, D[(t - h) * (ld + 1)]);
+  for (int k = t; k < n; k += nt) D[k * (ld + 1)] = cdiv(one, D[k * (ld + 1)]);
   __syncthreads();
 }
 
@@ -253,11 +261,14 @@ __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ l
   V* Fb = front_ptr<T>(arena, F, f, b, B);
   V* G = Fb + (int64_t)k0 + (int64_t)k0 * m;
   const int t = threadIdx.x;
+#pragma unroll 8
   for (int idx = t; idx < kb * kb; idx += blockDim.x) {
     const int i = idx % kb, j = idx / kb;
     Ds[i + j * ld] = G[i + (int64_t)j * m];
   }
   __syncthreads();
+  // right-looking LU, each step's multipliers and rank-1 update spread over all threads
+  const int nt = blockDim.x;
   for (int k = 0; k < kb; ++k) {
     V piv = Ds[k + k * ld];
     const T mag = piv.x * piv.x + piv.y * piv.y;
@@ -265,14 +276,17 @@ __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ l
       piv.x = tiny; piv.y = 0;
       if (t == 0) { Ds[k + k * ld] = piv; atomicAdd(npert, 1); }
     }
-    if (t > k && t < kb) {
-      const V l = cdiv(Ds[t + k * ld], piv);
-      Ds[t + k * ld] = l;
-      for (int j = k + 1; j < kb; ++j) Ds[t + j * ld] = cfms(Ds[t + j * ld], l, Ds[k + j * ld]);
+    const int nr = kb - k - 1;
+    for (int i = k + 1 + t; i < kb; i += nt) Ds[i + k * ld] = cdiv(Ds[i + k * ld], piv);
+    __syncthreads();
+    for (int idx = t; idx < nr * nr; idx += nt) {
+      const int i = k + 1 + idx % nr, j = k + 1 + idx / nr;
+      Ds[i + j * ld] = cfms(Ds[i + j * ld], Ds[i + k * ld], Ds[k + j * ld]);
     }
     __syncthreads();
   }
-  invert_lu_block(Ds, ld, kb, 64);
+  invert_lu_block(Ds, ld, kb);
+#pragma unroll 8
   for (int idx = t; idx < kb * kb; idx += blockDim.x) {
     const int i = idx % kb, j = idx / kb;
     G[i + (int64_t)j * m] = Ds[i + j * ld];
@@ -407,7 +421,7 @@ __global__ void __launch_bounds__(SMALL_NT) k_small_front(
   for (int J0 = 0; J0 < p; J0 += nbl) {
     const int nbk = min(nbl, p - J0);
     V* D = S + J0 + J0 * ld;
-    invert_lu_block(D, ld, nbk, SMALL_NT / 2);
+    invert_lu_block(D, ld, nbk);
     for (int idx = t; idx < nbk * nbk; idx += SMALL_NT) {
       const int i = idx % nbk, j = idx / nbk;
       G[(int64_t)(J0 + i) + (int64_t)(J0 + j) * m] = D[i + j * ld];
@@ -1103,16 +1117,20 @@ __global__ void __launch_bounds__(256) k_transpose(int R, int C, int ld_in, int 
   }
 }
 
-// Q[b] = sum_p kts_a[p] x[b][p] + V kts_z  (deterministic block reduction, one block per frequency)
-__global__ void __launch_bounds__(256) k_charge(int n, const double2* __restrict__ kts_a, double2 kts_z, double V,
-                                                const double2* __restrict__ x, double2* __restrict__ Q) {
-  __shared__ double2 red[256];
+// Q[b] = sum_p kts_a[p] x[b][p] + V kts_z: one 1024-thread block per frequency (many loads in
+// flight for the long dot product), fixed-order tree reduction (deterministic)
+constexpr int CHARGE_NT = 1024;
+__global__ void __launch_bounds__(CHARGE_NT) k_charge(int n, const double2* __restrict__ kts_a, double2 kts_z,
+                                                      double V, const double2* __restrict__ x,
+                                                      double2* __restrict__ Q) {
+  __shared__ double2 red[CHARGE_NT];
   const int b = blockIdx.x;
   double2 s = make_double2(0, 0);
-  for (int i = threadIdx.x; i < n; i += 256) s = cfma(s, kts_a[i], x[(int64_t)b * n + i]);
+#pragma unroll 4
+  for (int i = threadIdx.x; i < n; i += CHARGE_NT) s = cfma(s, kts_a[i], x[(int64_t)b * n + i]);
   red[threadIdx.x] = s;
   __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
+  for (int h = CHARGE_NT / 2; h > 0; h >>= 1) {
     if (threadIdx.x < h) {
       red[threadIdx.x].x += red[threadIdx.x + h].x;
       red[threadIdx.x].y += red[threadIdx.x + h].y;
@@ -1790,7 +1808,7 @@ int sawfem_transpose(void* stream, int R, int C, int ld_in, int ld_out, const vo
 int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, double kz_im, double V, const void* x,
                   void* Q) {
   if (n <= 0 || B <= 0) { sawfem_set_error("sawfem_charge: invalid arguments"); return 1; }
-  hipLaunchKernelGGL(k_charge, dim3(B), dim3(256), 0, (hipStream_t)stream, n, (const double2*)kts_a,
+  hipLaunchKernelGGL(k_charge, dim3(B), dim3(CHARGE_NT), 0, (hipStream_t)stream, n, (const double2*)kts_a,
                      make_double2(kz_re, kz_im), V, (const double2*)x, (double2*)Q);
   SAW_HIP(hipGetLastError());
   return 0;
