@@ -53,6 +53,23 @@ __device__ __forceinline__ typename cplx<T>::type* front_ptr(typename cplx<T>::t
   return arena + F.off[f] * (int64_t)B + (int64_t)b * m * m;
 }
 
+// Strided loop over idx = first, first + step, ... < count with its global loads issued U at a
+// time: load(idx) for U indices (clamped to count - 1, so every address is one the loop
+// visits), then use(idx, value) for those in range.  A plain loop waits for each load before
+// the next iteration; here U loads are in flight together.  count must be positive when
+// first < count.
+template <int U, typename Load, typename Use>
+__device__ __forceinline__ void batched(int first, int count, int step, Load load, Use use) {
+  for (int base = first; base < count; base += U * step) {
+    decltype(load(0)) a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = load(min(base + u * step, count - 1));
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + u * step < count) use(base + u * step, a[u]);
+  }
+}
+
 // ---------------------------------------------------------------- front assembly
 // Front f = sum of its children's contribution blocks (zero elsewhere), columns [c0, c1) of
 // one frequency per workgroup.  Threads cover rows in slabs of RS (64, 128 or 256) and CG =
@@ -261,11 +278,8 @@ __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ l
   V* Fb = front_ptr<T>(arena, F, f, b, B);
   V* G = Fb + (int64_t)k0 + (int64_t)k0 * m;
   const int t = threadIdx.x;
-#pragma unroll 8
-  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-    const int i = idx % kb, j = idx / kb;
-    Ds[i + j * ld] = G[i + (int64_t)j * m];
-  }
+  batched<8>(t, kb * kb, blockDim.x, [&](int idx) { return G[idx % kb + (int64_t)(idx / kb) * m]; },
+             [&](int idx, V v) { Ds[idx % kb + (idx / kb) * ld] = v; });
   __syncthreads();
   // right-looking LU, each step's multipliers and rank-1 update spread over all threads
   const int nt = blockDim.x;
@@ -286,7 +300,6 @@ __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ l
     __syncthreads();
   }
   invert_lu_block(Ds, ld, kb);
-#pragma unroll 8
   for (int idx = t; idx < kb * kb; idx += blockDim.x) {
     const int i = idx % kb, j = idx / kb;
     G[i + (int64_t)j * m] = Ds[i + j * ld];
@@ -767,11 +780,12 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
         double2 s[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) s[r] = w[r * W + i];
-        for (int j = 0; j < q; ++j) {
-          const double2 a = to_d2(Fb[(int64_t)i + (int64_t)(p + j) * m]);
+        const V* row = Fb + (int64_t)i + (int64_t)p * m;
+        batched<8>(0, q, 1, [&](int j) { return row[(int64_t)j * m]; }, [&](int j, V v) {
+          const double2 a = to_d2(v);
 #pragma unroll
           for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * W + p + j]);
-        }
+        });
 #pragma unroll
         for (int r = 0; r < R; ++r) w[r * W + i] = s[r];
       }
@@ -786,10 +800,9 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
         for (int j0 = 0; j0 < q; j0 += nbl) {
           const int nj = min(nbl, q - j0);
           __syncthreads();
-          for (int idx = t; idx < nbl * 64; idx += blockDim.x) {
-            const int jj = idx % nbl, c = idx / nbl;
-            if (jj < nj && c < nc) tile[c * LD + jj] = Fb[(int64_t)(p + j0 + jj) + (int64_t)(c0 + c) * m];
-          }
+          batched<8>(t, nj * nc, blockDim.x,
+                     [&](int idx) { return Fb[(int64_t)(p + j0 + idx % nj) + (int64_t)(c0 + idx / nj) * m]; },
+                     [&](int idx, V v) { tile[(idx / nj) * LD + idx % nj] = v; });
           __syncthreads();
           if (t < nc)
             for (int jj = 0; jj < nj; ++jj) {
@@ -811,10 +824,9 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
     const int J0 = fwd ? bi * nbl : (nblk - 1 - bi) * nbl;
     const int nbk = min(nbl, p - J0);
     // diagonal block to LDS: tile[r + c * LD] = F[J0 + r][J0 + c]
-    for (int idx = t; idx < nbl * nbl; idx += blockDim.x) {
-      const int r = idx % nbl, c = idx / nbl;
-      if (r < nbk && c < nbk) tile[r + c * LD] = Fb[(int64_t)(J0 + r) + (int64_t)(J0 + c) * m];
-    }
+    batched<8>(t, nbk * nbk, blockDim.x,
+               [&](int idx) { return Fb[(int64_t)(J0 + idx % nbk) + (int64_t)(J0 + idx / nbk) * m]; },
+               [&](int idx, V v) { tile[idx % nbk + (idx / nbk) * LD] = v; });
     __syncthreads();
     {
       const int i = t % PS, g = t / PS;   // output row, quarter of the reduction (g < 4)
@@ -873,11 +885,12 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
         double2 s[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) s[r] = w[r * W + i];
-        for (int k = 0; k < nbk; ++k) {
-          const double2 a = to_d2(Fb[(int64_t)i + (int64_t)(J0 + k) * m]);
+        const V* row = Fb + (int64_t)i + (int64_t)J0 * m;
+        batched<8>(0, nbk, 1, [&](int k) { return row[(int64_t)k * m]; }, [&](int k, V v) {
+          const double2 a = to_d2(v);
 #pragma unroll
           for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * W + J0 + k]);
-        }
+        });
 #pragma unroll
         for (int r = 0; r < R; ++r) w[r * W + i] = s[r];
       }
@@ -886,10 +899,9 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
       for (int c0 = lo; c0 < hi; c0 += 64) {
         const int nc = min(64, hi - c0);
         __syncthreads();
-        for (int idx = t; idx < nbl * 64; idx += blockDim.x) {
-          const int k = idx % nbl, c = idx / nbl;
-          if (k < nbk && c < nc) tile[c * LD + k] = Fb[(int64_t)(J0 + k) + (int64_t)(c0 + c) * m];
-        }
+        batched<8>(t, nbk * nc, blockDim.x,
+                   [&](int idx) { return Fb[(int64_t)(J0 + idx % nbk) + (int64_t)(c0 + idx / nbk) * m]; },
+                   [&](int idx, V v) { tile[(idx / nbk) * LD + idx % nbk] = v; });
         __syncthreads();
         if (t < nc) {
           double2 s[R];
@@ -972,11 +984,11 @@ __device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ 
       const int row = (KIND == 0 ? p : 0) + o0 + o;
       const int cbase = (KIND == 0 ? 0 : p) + v0;
       if (o < no)
-        for (int k = g; k < nv; k += 4) {
-          const double2 a = to_d2(Fb[(int64_t)row + (int64_t)(cbase + k) * m]);
+        batched<8>(g, nv, 4, [&](int k) { return Fb[(int64_t)row + (int64_t)(cbase + k) * m]; }, [&](int k, V e) {
+          const double2 a = to_d2(e);
 #pragma unroll
           for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, v[r * VC + k]);
-        }
+        });
     } else {
       // transposed: output o is a column of the stored panel; stage 64 x KC2 tiles through LDS
       constexpr int KC2 = SolveTile<T>::SB;                    // reduction sub-chunk
@@ -985,10 +997,9 @@ __device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ 
       for (int k0 = 0; k0 < nv; k0 += KC2) {
         const int nk = min(KC2, nv - k0);
         __syncthreads();
-        for (int idx = t; idx < 64 * KC2; idx += blockDim.x) {
-          const int kk = idx % KC2, c = idx / KC2;
-          if (kk < nk && c < no) tile[c * (KC2 + 1) + kk] = Fb[(int64_t)(rbase + k0 + kk) + (int64_t)(cbase + c) * m];
-        }
+        batched<8>(t, nk * no, blockDim.x,
+                   [&](int idx) { return Fb[(int64_t)(rbase + k0 + idx % nk) + (int64_t)(cbase + idx / nk) * m]; },
+                   [&](int idx, V e) { tile[(idx / nk) * (KC2 + 1) + idx % nk] = e; });
         __syncthreads();
         if (o < no)
           for (int kk = g; kk < nk; kk += 4) {
