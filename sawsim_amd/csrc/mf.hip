@@ -452,10 +452,13 @@ typedef double doublex4 __attribute__((ext_vector_type(4)));
 template <typename T> struct WaveAcc;
 
 template <> struct WaveAcc<float> {
-  // one 32x32 complex block per wave: D[i = col][j = row]
-  floatx16 re, im;
+  // one 32x32 complex block per wave: D[i = col][j = row].  Three real products per complex
+  // product (Gauss): P1 = Ar Br, P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); then Re = P1 - P2 and
+  // Im = P3 - P1 - P2.  Normwise as accurate as the four-product form; FP64 refinement of the
+  // solves absorbs the difference.
+  floatx16 p1, p2, p3;
   __device__ void zero() {
-    for (int v = 0; v < 16; ++v) { re[v] = 0.f; im[v] = 0.f; }
+    for (int v = 0; v < 16; ++v) { p1[v] = 0.f; p2[v] = 0.f; p3[v] = 0.f; }
   }
   // As: [KC][TR] (k-major, rows contiguous); Bs: [TC][KC+1] (col-major over k)
   template <int TR>
@@ -465,17 +468,16 @@ template <> struct WaveAcc<float> {
       const int k = kk + (lane >> 5);
       const float2 bv = Bs[(wcol * 32 + (lane & 31)) * (KC + 1) + k];   // B[k][col]
       const float2 av = As[k * TR + wrow * 32 + (lane & 31)];          // A[row][k]
-      re = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.x, av.x, re, 0, 0, 0);
-      re = __builtin_amdgcn_mfma_f32_32x32x2f32(-bv.y, av.y, re, 0, 0, 0);
-      im = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.x, av.y, im, 0, 0, 0);
-      im = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.y, av.x, im, 0, 0, 0);
+      p1 = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.x, av.x, p1, 0, 0, 0);
+      p2 = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.y, av.y, p2, 0, 0, 0);
+      p3 = __builtin_amdgcn_mfma_f32_32x32x2f32(bv.x + bv.y, av.x + av.y, p3, 0, 0, 0);
     }
   }
   template <typename Fn> __device__ void each(int lane, Fn fn) {
     // D[i][j]: j = lane % 32 (row), i = 8 (v / 4) + 4 (lane / 32) + v % 4 (col)
     for (int v = 0; v < 16; ++v) {
       const int i = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-      fn(v, lane & 31, i, re[v], im[v]);
+      fn(v, lane & 31, i, p1[v] - p2[v], p3[v] - p1[v] - p2[v]);
     }
   }
   // (row, col) of the lane's 16 outputs, in the order of each()
@@ -674,7 +676,7 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
 }
 
 template <typename T, int mode, int TR = TILE, int TC = TILE, bool PRE = false>
-__global__ void __launch_bounds__(256) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
+__global__ void __launch_bounds__(256, (PRE && sizeof(T) == 4) ? 3 : 1) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
                                               typename cplx<T>::type* __restrict__ arena, int k0, int nb, int nc) {
   gemm_body<T, mode, TR, TC, PRE>(F, lvl, B, arena, k0, nb, nc, blockIdx.z);
 
