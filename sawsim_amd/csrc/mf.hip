@@ -206,20 +206,23 @@ __global__ void __launch_bounds__(256) k_front_entries(FrontsDev F, int64_t e0, 
 
 // ---------------------------------------------------------------- panel factorization
 // In-place inversion of the unit-lower and upper triangles of an n x n block D (column-major,
-// leading dimension ld, in LDS) by row operations (Gauss-Jordan): thread per row, the updates
-// inside a step independent of each other.
-//   L^{-1} (threads [0, h)):  step k, rows i > k:  X[i][j] -= L[i][k] X[k][j] (j < k), X[i][k] = -L[i][k]
-//   U^{-1} (threads [h, 2h)): step k = n-1..0: row k scaled by d = 1/U[k][k] (phase a), then
-//     rows i < k: Y[i][j] -= U[i][k] Y[k][j] (j > k), Y[i][k] = -U[i][k] d (phase b); the
-//     diagonal entries become 1/U[k][k] at the end
-// h >= n; all threads of the block call it (it synchronizes).
+// leading dimension ld, in LDS) by row operations (Gauss-Jordan):
+//   L^{-1}: step k, rows i > k:  X[i][j] -= L[i][k] X[k][j] (j < k), X[i][k] = -L[i][k]
+//   U^{-1}: step k = n-1..0: row k scaled by d = 1/U[k][k] (phase a), then rows i < k:
+//     Y[i][j] -= U[i][k] Y[k][j] (j > k), Y[i][k] = -U[i][k] d (phase b); the diagonal
+//     entries become 1/U[k][k] at the end
+// n <= 64 and at least 64 threads when n > 32; all threads of the block call it (it synchronizes).
+// Thread t owns row t % RW (RW = 32 or 64, a power of two >= n) and every (nt / RW)-th column of
+// a step's update, so no index needs a division.
 template <typename V>
 __device__ void invert_lu_block(V* D, int ld, int n) {
   // In-place inverses of the unit-lower L and upper U held in D (LDS): afterwards the strict
   // lower triangle holds L^{-1} (unit diagonal implied) and the upper triangle U^{-1}.  Step st
   // eliminates column st of L and column n-1-st of U (Gauss-Jordan); the two halves touch
-  // disjoint triangles, and each step's updates are spread over all threads.
+  // disjoint triangles.
   const int t = threadIdx.x, nt = blockDim.x;
+  const int rs = n <= 32 ? 5 : 6;
+  const int i = t & ((1 << rs) - 1), tc = t >> rs, ncg = nt >> rs;
   const V one{1, 0};
   for (int st = 0; st < n; ++st) {
     const int kL = st, kU = n - 1 - st;
@@ -228,25 +231,21 @@ __device__ void invert_lu_block(V* D, int ld, int n) {
     for (int j = kU + 1 + t; j < n; j += nt) D[kU + j * ld] = cmul(D[kU + j * ld], dU);
     __syncthreads();
     // b: L rows i > kL, columns j < kL and U rows i < kU, columns j > kU
-    const int aL = n - 1 - kL, cL = aL * kL;
-    const int aU = kU, cU = aU * (n - 1 - kU);
-    for (int idx = t; idx < cL + cU; idx += nt) {
-      if (idx < cL) {
-        const int i = kL + 1 + idx % aL, j = idx / aL;
-        D[i + j * ld] = cfms(D[i + j * ld], D[i + kL * ld], D[kL + j * ld]);
-      } else {
-        const int q = idx - cL;
-        const int i = q % aU, j = kU + 1 + q / aU;
-        D[i + j * ld] = cfms(D[i + j * ld], D[i + kU * ld], D[kU + j * ld]);
-      }
+    if (i > kL && i < n) {
+      const V l = D[i + kL * ld];
+      for (int j = tc; j < kL; j += ncg) D[i + j * ld] = cfms(D[i + j * ld], l, D[kL + j * ld]);
+    }
+    if (i < kU) {
+      const V u = D[i + kU * ld];
+      for (int j = kU + 1 + tc; j < n; j += ncg) D[i + j * ld] = cfms(D[i + j * ld], u, D[kU + j * ld]);
     }
     __syncthreads();
     // c: the eliminated columns
-    for (int i = kL + 1 + t; i < n; i += nt) {
+    if (tc == 0 && i > kL && i < n) {
       const V v = D[i + kL * ld];
       D[i + kL * ld] = V{-v.x, -v.y};
     }
-    for (int i = t; i < kU; i += nt) {
+    if (tc == ncg - 1 && i < kU) {
       const V y = cmul(D[i + kU * ld], dU);
       D[i + kU * ld] = V{-y.x, -y.y};
     }
@@ -277,12 +276,15 @@ __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ l
   const int ld = nb + 1;
   V* Fb = front_ptr<T>(arena, F, f, b, B);
   V* G = Fb + (int64_t)k0 + (int64_t)k0 * m;
-  const int t = threadIdx.x;
-  batched<8>(t, kb * kb, blockDim.x, [&](int idx) { return G[idx % kb + (int64_t)(idx / kb) * m]; },
-             [&](int idx, V v) { Ds[idx % kb + (idx / kb) * ld] = v; });
+  // thread t: row i = t % RW of the block (RW = 32 or 64 >= kb), columns tc, tc + ncg, ...
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int rs = nb <= 32 ? 5 : 6;
+  const int i = t & ((1 << rs) - 1), tc = t >> rs, ncg = nt >> rs;
+  const int ic = min(i, kb - 1);
+  batched<8>(tc, kb, ncg, [&](int j) { return G[ic + (int64_t)j * m]; },
+             [&](int j, V v) { if (i < kb) Ds[i + j * ld] = v; });
   __syncthreads();
-  // right-looking LU, each step's multipliers and rank-1 update spread over all threads
-  const int nt = blockDim.x;
+  // right-looking LU: multipliers of column k, then the rank-1 update of the trailing block
   for (int k = 0; k < kb; ++k) {
     V piv = Ds[k + k * ld];
     const T mag = piv.x * piv.x + piv.y * piv.y;
@@ -290,20 +292,17 @@ __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ l
       piv.x = tiny; piv.y = 0;
       if (t == 0) { Ds[k + k * ld] = piv; atomicAdd(npert, 1); }
     }
-    const int nr = kb - k - 1;
-    for (int i = k + 1 + t; i < kb; i += nt) Ds[i + k * ld] = cdiv(Ds[i + k * ld], piv);
+    if (tc == 0 && i > k && i < kb) Ds[i + k * ld] = cdiv(Ds[i + k * ld], piv);
     __syncthreads();
-    for (int idx = t; idx < nr * nr; idx += nt) {
-      const int i = k + 1 + idx % nr, j = k + 1 + idx / nr;
-      Ds[i + j * ld] = cfms(Ds[i + j * ld], Ds[i + k * ld], Ds[k + j * ld]);
+    if (i > k && i < kb) {
+      const V l = Ds[i + k * ld];
+      for (int j = k + 1 + tc; j < kb; j += ncg) Ds[i + j * ld] = cfms(Ds[i + j * ld], l, Ds[k + j * ld]);
     }
     __syncthreads();
   }
   invert_lu_block(Ds, ld, kb);
-  for (int idx = t; idx < kb * kb; idx += blockDim.x) {
-    const int i = idx % kb, j = idx / kb;
-    G[i + (int64_t)j * m] = Ds[i + j * ld];
-  }
+  if (i < kb)
+    for (int j = tc; j < kb; j += ncg) G[i + (int64_t)j * m] = Ds[i + j * ld];
 }
 
 template <typename T>
@@ -1394,9 +1393,11 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
           }
         }
         const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
+        // 32-wide blocks: one wave per block (its barriers cost next to nothing and more blocks
+        // share a CU); 64-wide blocks: two waves
         { Timed tm(P, s, K_DIAG);
-        hipLaunchKernelGGL(k_diag<T>, dim3(G.nf, B), dim3(128), lds_diag, s, P->F, G.fronts, B, arena, k0, nbl, tiny,
-                           P->npert); }
+        hipLaunchKernelGGL(k_diag<T>, dim3(G.nf, B), dim3(nbl <= 32 ? 64 : 128), lds_diag, s, P->F, G.fronts, B, arena,
+                           k0, nbl, tiny, P->npert); }
         const int rem = G.mmax - k0 - 1;
         if (rem > 0) {
           Timed tm(P, s, K_TRSM);
