@@ -31,7 +31,7 @@ PEAK_FP64_MFMA = 78.6
 # HBM bytes per launch of the Schur GEMM k_gemm<float, 1> on the config-2 / config-4 sweeps:
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (separate runs), each scaled by what the counter reads for
 # a plain 1 GiB device copy (`--copy-mib 1024`: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000)
-SCHUR_TRAFFIC_BYTES = {'config2': 1.837e10, 'config4': 2.784e10}
+SCHUR_TRAFFIC_BYTES = {'config2': 1.837e10, 'config4': 1.676e10}
 
 
 def log(*a):
@@ -414,6 +414,7 @@ def main():
                       traffic_note='HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy',
                       kernel='k_gemm<float, 1, 64, 64, *> (Schur complement, complex64 on FP32 MFMA)',
                       launches=schur_n, kernel_ms=schur_ms, algorithmic_flops=schur_flops,
+                      algorithmic_bytes_per_launch=work['schur_bytes'] * P * args.steps / max(schur_n, 1),
                       avg_launch_ms=schur_ms / max(schur_n, 1)),
         other_kernels=dict(
             whole_sweep=dict(bound='mfma', achieved=sweep_tf, peak=PEAK_FP32_MFMA, unit='TFLOP/s',

@@ -138,7 +138,9 @@ def kernel_work(mf: MFArrays, nb: int = 32) -> dict:
     * ``gemm_flops``: their sum (the MFMA update kernels);
     * ``solve_flops``: 8·(nnz(L)+nnz(U)) for one forward + backward substitution;
     * ``lu_entries``: stored L/U entries (pivot triangle + both panels), the data one solve
-      streams; ``arena_entries``: Σ m².
+      streams; ``arena_entries``: Σ m²;
+    * ``schur_bytes``: the Schur complement's algorithmic complex64 bytes, 8·(2·p·q + 2·q²) with
+      q = m - p (L21 and U12 read once, the contribution block read and written once).
     """
     m = mf.m.astype(np.float64)
     p = mf.p.astype(np.float64)
@@ -158,5 +160,7 @@ def kernel_work(mf: MFArrays, nb: int = 32) -> dict:
                     lf += (k1 - k0) * float((mi - k1) ** 2 - (mi - pi) ** 2)
             sf += float(pi) * float((mi - pi) ** 2)
     nnz_lu = float((p * p + 2 * p * (m - p)).sum())
+    q = m - p
     return dict(factor_flops=8 * ff, lupd_flops=8 * lf, schur_flops=8 * sf, gemm_flops=8 * (lf + sf),
-                solve_flops=8 * nnz_lu, lu_entries=nnz_lu, arena_entries=float((m * m).sum()))
+                solve_flops=8 * nnz_lu, lu_entries=nnz_lu, arena_entries=float((m * m).sum()),
+                schur_bytes=float((8 * (2 * p * q + 2 * q * q))[(q > 0) & (p > 0)].sum()))

@@ -96,6 +96,9 @@ def test_plan_arrays_drive_a_correct_factorization(nb):
     work = kernel_work(mf, nb)
     assert 0 < work['gemm_flops'] <= work['factor_flops']
     assert np.isclose(work['factor_flops'], sym.stats['flops_factor'])
+    # Schur bytes, front by front: L21 (q x p) and U12 (p x q) read, the q x q block read and written
+    sb = sum(8 * (2 * f.p * (f.m - f.p) + 2 * (f.m - f.p) ** 2) for f in sym.fronts if f.m > f.p and f.p > 0)
+    assert np.isclose(work['schur_bytes'], sb)
 
 
 def test_symbolic_covers_every_unknown_once():
