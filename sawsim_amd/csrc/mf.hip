@@ -542,7 +542,8 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
   static_assert((TR / 32) * (TC / 32) == 4, "four 32x32 waves per tile");
   __shared__ V As[2][KC * TR];
   __shared__ V Bs[2][TC * (KC + 1)];
-  const int f = lvl[blockIdx.y];
+  // mode 1 walks a tile list (lvl = packed front / tile row / tile column per workgroup)
+  const int f = mode == 1 ? (lvl[blockIdx.x] >> 12) : lvl[blockIdx.y];
   const int p = F.p[f];
   const int m = F.m[f];
   // C rows [r_lo, r_hi) x cols [c_lo, c_hi); reduction length K
@@ -568,7 +569,8 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
   }
   if (r_lo >= r_hi || c_lo >= c_hi) return;
   const int ntr = (r_hi - r_lo + TR - 1) / TR, ntc = (c_hi - c_lo + TC - 1) / TC;
-  if (mode != 4) { ti = blockIdx.x % ntr; tj = blockIdx.x / ntr; }
+  if (mode == 1) { ti = (lvl[blockIdx.x] >> 6) & 63; tj = lvl[blockIdx.x] & 63; }
+  else if (mode != 4) { ti = blockIdx.x % ntr; tj = blockIdx.x / ntr; }
   if (ti >= ntr || tj >= ntc) return;
   const int r0 = r_lo + ti * TR, c0 = c_lo + tj * TC;
   if (mode == 0 && r0 >= p && c0 >= p) return;
@@ -1164,6 +1166,10 @@ __global__ void k_axpy(int64_t n, double2* __restrict__ x, const double2* __rest
 struct Sub {
   int nf = 0, mmax = 0, pmax = 0, cbmax = 0;
   const int* fronts = nullptr;   // device list
+  // Schur-complement tiles of these fronts, one entry per TILE x TILE tile of a contribution
+  // block: front << 12 | tile row << 6 | tile column (so a launch has no empty workgroups)
+  const int* tiles = nullptr;
+  int ntiles = 0;
 };
 
 struct Level {
@@ -1258,6 +1264,10 @@ int build_partition(Plan* P, int pi, int max_m) {
   P->part_owned[pi].clear();
   P->small_mmax[pi] = std::max(0, std::min(max_m, P->small_limit[pi]));
   const int nf = P->nf;
+  // Schur tile lists pack front (19 bits) and tile row / column (6 bits each)
+  if (nf >= (1 << 19)) { sawfem_set_error("sawfem_mf: more than 524287 fronts"); return 2; }
+  for (int f = 0; f < nf; ++f)
+    if (P->hm[f] - P->hp[f] > 64 * TILE) { sawfem_set_error("sawfem_mf: contribution block above 4096 rows"); return 2; }
   std::vector<unsigned char> flag(nf, 0);
   auto put = [&](const std::vector<int>& v, const int** dst) -> int {
     *dst = nullptr;
@@ -1302,6 +1312,16 @@ int build_partition(Plan* P, int pi, int max_m) {
         Sub S;
         stats(part, S);
         if (put(part, &S.fronts)) return 3;
+        std::vector<int> tl;
+        for (int f : part) {
+          const int q = P->hm[f] - P->hp[f];
+          if (q <= 0 || P->hp[f] <= 0) continue;
+          const int nt = (q + TILE - 1) / TILE;
+          for (int tj = 0; tj < nt; ++tj)
+            for (int ti = 0; ti < nt; ++ti) tl.push_back(f << 12 | ti << 6 | tj);
+        }
+        S.ntiles = (int)tl.size();
+        if (put(tl, &S.tiles)) return 3;
         out.push_back(S);
       }
       return 0;
@@ -1414,14 +1434,13 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
           }
         }
     }
-    if (G.cbmax > 0) {
+    if (G.ntiles > 0) {
       Timed tm(P, s, K_GEMM_S);
-      const int nt = (G.cbmax + TILE - 1) / TILE;
       if (G.pmax <= SHORT_K)   // short reduction: the contribution-block read would follow the K loop
-        hipLaunchKernelGGL((k_gemm<T, 1, TILE, TILE, true>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
+        hipLaunchKernelGGL((k_gemm<T, 1, TILE, TILE, true>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles, B,
                            arena, 0, nb, 0);
       else
-        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(nt * nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, 0, nb, 0);
+        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles, B, arena, 0, nb, 0);
     }
     }
   }
