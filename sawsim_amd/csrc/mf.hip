@@ -524,26 +524,27 @@ template <> struct WaveAcc<double> {
   }
 };
 
-// mode 0: L-shaped update for panel [k0, k0+kb)  (skips rows >= p and cols >= p together)
 // mode 1: Schur complement rows/cols [p, m), K = [0, p)
 // mode 2: L panel  F[k1:m, k0:k1] = F[k1:m, k0:k1] * Uinv      (in place, overwrite)
 // mode 3: U panel  F[k0:k1, k1:m] = Linv * F[k0:k1, k1:m]      (in place, overwrite)
 // mode 4: left-looking panel update before the diagonal LU of panel [k0, k1), K = [0, k0):
-//         column panel F[k0:m, k0:k1] -= F[k0:m, 0:k0] F[0:k0, k0:k1]  (x tiles 0 .. nc-1)
-//         row panel    F[k0:k1, k1:m] -= F[k0:k1, 0:k0] F[0:k0, k1:m]  (x tiles nc ..)
-//         with nc = ceil((mmax - k0) / TILE) from the launch (uniform over the level)
-// The inverses are the ones k_diag stored in the diagonal block.
+//         column panel F[k0:m, k0:k1] -= F[k0:m, 0:k0] F[0:k0, k0:k1]  (part 0 tiles)
+//         row panel    F[k0:k1, k1:m] -= F[k0:k1, 0:k0] F[0:k0, k1:m]  (part 1 tiles)
+// The inverses are the ones k_diag stored in the diagonal block.  Each workgroup takes one
+// entry of a tile list (lvl): the front and the C tile.
 // C tiles are TR x TC (4 waves of 32 x 32): 64 x 64 in general; 128 x 32 and 32 x 128 for the
 // 32-wide panels of the small-front levels, so no wave works on columns outside the panel.
 template <typename T, int mode, int TR = TILE, int TC = TILE, bool PRE = false>
 __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ lvl, int B,
-                                          typename cplx<T>::type* __restrict__ arena, int k0, int nb, int nc, int b) {
+                                          typename cplx<T>::type* __restrict__ arena, int k0, int nb, int b) {
   using V = typename cplx<T>::type;
   static_assert((TR / 32) * (TC / 32) == 4, "four 32x32 waves per tile");
   __shared__ V As[2][KC * TR];
   __shared__ V Bs[2][TC * (KC + 1)];
-  // mode 1 walks a tile list (lvl = packed front / tile row / tile column per workgroup)
-  const int f = mode == 1 ? (lvl[blockIdx.x] >> 12) : lvl[blockIdx.y];
+  // lvl is a tile list, one entry per workgroup: front << 12 | tile row << 6 | tile column
+  // (mode 1), front << 12 | tile (modes 2, 3), front << 12 | part << 11 | tile (mode 4)
+  const int e = lvl[blockIdx.x];
+  const int f = e >> 12;
   const int p = F.p[f];
   const int m = F.m[f];
   // C rows [r_lo, r_hi) x cols [c_lo, c_hi); reduction length K
@@ -557,23 +558,22 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
     kb = min(nb, p - k0);
     const int k1 = k0 + kb;
     K = k0;
-    if ((int)blockIdx.x < nc) { r_lo = k0; r_hi = m; c_lo = k0; c_hi = k1; ti = blockIdx.x; tj = 0; }
-    else { r_lo = k0; r_hi = k1; c_lo = k1; c_hi = m; ti = 0; tj = blockIdx.x - nc; }
+    if (!(e & 2048)) { r_lo = k0; r_hi = m; c_lo = k0; c_hi = k1; ti = e & 2047; tj = 0; }
+    else { r_lo = k0; r_hi = k1; c_lo = k1; c_hi = m; ti = 0; tj = e & 2047; }
   } else {
     if (k0 >= p) return;
     kb = min(nb, p - k0);
     const int k1 = k0 + kb;
-    if (mode == 0) { r_lo = c_lo = k1; r_hi = c_hi = m; K = kb; }
-    else if (mode == 2) { r_lo = k1; r_hi = m; c_lo = k0; c_hi = k1; K = kb; }
+    if (mode == 2) { r_lo = k1; r_hi = m; c_lo = k0; c_hi = k1; K = kb; }
     else { r_lo = k0; r_hi = k1; c_lo = k1; c_hi = m; K = kb; }
   }
   if (r_lo >= r_hi || c_lo >= c_hi) return;
   const int ntr = (r_hi - r_lo + TR - 1) / TR, ntc = (c_hi - c_lo + TC - 1) / TC;
-  if (mode == 1) { ti = (lvl[blockIdx.x] >> 6) & 63; tj = lvl[blockIdx.x] & 63; }
-  else if (mode != 4) { ti = blockIdx.x % ntr; tj = blockIdx.x / ntr; }
+  if (mode == 1) { ti = (e >> 6) & 63; tj = e & 63; }
+  else if (mode == 2) { ti = e & 4095; tj = 0; }
+  else if (mode == 3) { ti = 0; tj = e & 4095; }
   if (ti >= ntr || tj >= ntc) return;
   const int r0 = r_lo + ti * TR, c0 = c_lo + tj * TC;
-  if (mode == 0 && r0 >= p && c0 >= p) return;
   V* Fb = front_ptr<T>(arena, F, f, b, B);
   const int t = threadIdx.x;
   const int wave = t >> 6, lane = t & 63;
@@ -594,7 +594,7 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
       const bool ok = gr < r_hi && gk < K;
       const int grc = min(gr, r_hi - 1), gkc = min(gk, K - 1);
       int64_t addr;
-      if (mode == 0 || mode == 2) addr = (int64_t)grc + (int64_t)(k0 + gkc) * m;
+      if (mode == 2) addr = (int64_t)grc + (int64_t)(k0 + gkc) * m;
       else if (mode == 1 || mode == 4) addr = (int64_t)grc + (int64_t)gkc * m;
       else addr = (int64_t)grc + (int64_t)(k0 + gkc) * m;                  // Linv[row][k]
       ra[e] = Fb[addr];
@@ -640,7 +640,7 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
   load(0);
   // read-modify-write modes: with PRE (short K) the C tile is fetched with the first chunk
   // (clamped in-front addresses) so its latency overlaps the K loop instead of following it
-  constexpr bool RMW = mode == 0 || mode == 1 || mode == 4;
+  constexpr bool RMW = mode == 1 || mode == 4;
   V cpre[RMW && PRE ? 16 : 1];
   if (RMW && PRE)
     WaveAcc<T>::coords(lane, [&](int v, int j, int i) {
@@ -659,7 +659,7 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
   acc.each(lane, [&](int v, int j, int i, T re, T im) {
     const int gr = r0 + wrow * 32 + j;
     const int gc = c0 + wcol * 32 + i;
-    if (gr < r_hi && gc < c_hi && !(mode == 0 && gr >= p && gc >= p)) {
+    if (gr < r_hi && gc < c_hi) {
       V* c = Fb + (int64_t)gr + (int64_t)gc * m;
       V o;
       if (RMW && PRE) {
@@ -678,8 +678,8 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
 
 template <typename T, int mode, int TR = TILE, int TC = TILE, bool PRE = false>
 __global__ void __launch_bounds__(256, (PRE && sizeof(T) == 4) ? 3 : 1) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
-                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb, int nc) {
-  gemm_body<T, mode, TR, TC, PRE>(F, lvl, B, arena, k0, nb, nc, blockIdx.z);
+                                              typename cplx<T>::type* __restrict__ arena, int k0, int nb) {
+  gemm_body<T, mode, TR, TC, PRE>(F, lvl, B, arena, k0, nb, blockIdx.z);
 
 }
 
@@ -1163,9 +1163,22 @@ __global__ void k_axpy(int64_t n, double2* __restrict__ x, const double2* __rest
 
 // ---------------------------------------------------------------- plan
 // Fronts of one level that take one factorization path.
+// Work lists of one pivot panel [k0, k0 + nbl) over the fronts of a size bucket (entries
+// front << 12 | tile), so the panel launches have no empty workgroups
+struct PanelTiles {
+  const int* diag = nullptr;     // fronts with p > k0 (diagonal-block LU)
+  int ndiag = 0;
+  const int* t23 = nullptr;      // panel solves: row tiles of L21 = column tiles of U12
+  int n23 = 0;
+  const int* t4 = nullptr;       // left-looking update: n4c column-part tiles, then n4r row-part
+  int n4c = 0, n4r = 0;          //   tiles (those carry 2048 in the tile field)
+};
+
 struct Sub {
   int nf = 0, mmax = 0, pmax = 0, cbmax = 0;
   const int* fronts = nullptr;   // device list
+  std::vector<int> hfronts;      // host copy
+  std::vector<PanelTiles> panels;  // per pivot panel, for the panel width of panel_nb
   // Schur-complement tiles of these fronts, one entry per TILE x TILE tile of a contribution
   // block: front << 12 | tile row << 6 | tile column (so a launch has no empty workgroups)
   const int* tiles = nullptr;
@@ -1218,6 +1231,7 @@ struct Plan {
   int small_mmax[2] = {0, 0};    // threshold in use
   int solve_full_m = 0;          // fronts up to this order are solved by one workgroup each
   std::vector<void*> part_owned[2];
+  int panel_nb[2] = {0, 0};      // panel width the buckets' panel lists were built for (0: none)
   std::vector<int> hm, hp, hlvl_ptr, hlvl_fronts;
 };
 
@@ -1312,6 +1326,7 @@ int build_partition(Plan* P, int pi, int max_m) {
         Sub S;
         stats(part, S);
         if (put(part, &S.fronts)) return 3;
+        S.hfronts = part;
         std::vector<int> tl;
         for (int f : part) {
           const int q = P->hm[f] - P->hp[f];
@@ -1338,6 +1353,53 @@ int build_partition(Plan* P, int pi, int max_m) {
   SAW_HIP(hipMemcpy(d, flag.data(), nf, hipMemcpyHostToDevice));
   P->part_owned[pi].push_back(d);
   P->fused[pi] = (unsigned char*)d;
+  P->panel_nb[pi] = 0;
+  return 0;
+}
+
+// Panel work lists of every size bucket for panel width nb (the factorization's; a level's
+// panels are panel_width(nb, level) wide).  32-wide panels of buckets with m > 64 ("narrow")
+// run in 128 x 32 / 32 x 128 tiles, everything else in TILE x TILE tiles.
+int build_panel_lists(Plan* P, int pi, int nb) {
+  for (Level& L : P->levels) {
+    const int nbl = panel_width(nb, L);
+    for (Sub& G : L.big[pi]) {
+      const int R = (nbl == 32 && G.mmax > 64) ? 128 : TILE;
+      G.panels.clear();
+      for (int k0 = 0; k0 < G.pmax; k0 += nbl) {
+        std::vector<int> dg, t23, t4c, t4r;
+        for (int f : G.hfronts) {
+          const int m = P->hm[f], p = P->hp[f];
+          if (p <= k0) continue;
+          if ((m + R - 1) / R >= 2048) { sawfem_set_error("sawfem_mf: front too large for the panel tile lists"); return 2; }
+          const int k1 = std::min(p, k0 + nbl);
+          dg.push_back(f);
+          for (int t = 0; t < (m - k1 + R - 1) / R; ++t) t23.push_back(f << 12 | t);
+          if (k0 > 0) {
+            for (int t = 0; t < (m - k0 + R - 1) / R; ++t) t4c.push_back(f << 12 | t);
+            for (int t = 0; t < (m - k1 + R - 1) / R; ++t) t4r.push_back(f << 12 | 2048 | t);
+          }
+        }
+        PanelTiles PT;
+        PT.ndiag = (int)dg.size();
+        PT.n23 = (int)t23.size();
+        PT.n4c = (int)t4c.size();
+        PT.n4r = (int)t4r.size();
+        t4c.insert(t4c.end(), t4r.begin(), t4r.end());
+        for (auto [v, dst] : {std::pair<std::vector<int>*, const int**>{&dg, &PT.diag}, {&t23, &PT.t23}, {&t4c, &PT.t4}}) {
+          *dst = nullptr;
+          if (v->empty()) continue;
+          void* d = nullptr;
+          SAW_HIP(hipMalloc(&d, sizeof(int) * v->size()));
+          SAW_HIP(hipMemcpy(d, v->data(), sizeof(int) * v->size(), hipMemcpyHostToDevice));
+          P->part_owned[pi].push_back(d);
+          *dst = (const int*)d;
+        }
+        G.panels.push_back(PT);
+      }
+    }
+  }
+  P->panel_nb[pi] = nb;
   return 0;
 }
 
@@ -1348,6 +1410,10 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
   V* arena = (V*)arena_v;
   const T tiny = sizeof(T) == 4 ? (T)1e-30 : (T)1e-300;
   const int pi = sizeof(T) == 4 ? 0 : 1;
+  if (P->panel_nb[pi] != nb) {
+    const int rc = build_panel_lists(P, pi, nb);
+    if (rc) return rc;
+  }
   for (size_t li = 0; li < P->levels.size(); ++li) {
     const Level& L = P->levels[li];
     P->cur_level = (int)li;
@@ -1380,68 +1446,65 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
       // 32-wide panels use 128 x 32 / 32 x 128 tiles (no wave on columns outside the panel);
       // the bucket of fronts with m <= 64 keeps 64 x 64 tiles (measured equal or better there)
       const bool narrow = nbl == 32 && G.mmax > 64;
-      for (int k0 = 0; k0 < G.pmax; k0 += nbl) {
-        if (k0 > 0) {
+      for (size_t pk = 0; pk < G.panels.size(); ++pk) {
+        const PanelTiles& PT = G.panels[pk];
+        const int k0 = (int)pk * nbl;
+        if (PT.n4c + PT.n4r > 0) {
           Timed tm(P, s, K_GEMM_L);
           const bool pre = k0 <= SHORT_K;   // short reduction (K = k0): prefetch the C tile
+          const int* t4r = PT.t4 + PT.n4c;
           if (narrow) {         // column part in 128 x 32 tiles, row part in 32 x 128 tiles
-            const int nc = (G.mmax - k0 + 127) / 128;
-            const int nr = (G.mmax - k0 - 1 + 127) / 128;
             if (pre)
-              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32, true>), dim3(nc, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
-                                 arena, k0, nbl, nc);
+              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32, true>), dim3(PT.n4c, 1, B), dim3(256), 0, s, P->F, PT.t4, B,
+                                 arena, k0, nbl);
             else
-              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32>), dim3(nc, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena,
-                                 k0, nbl, nc);
-            if (nr > 0) {
+              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32>), dim3(PT.n4c, 1, B), dim3(256), 0, s, P->F, PT.t4, B, arena,
+                                 k0, nbl);
+            if (PT.n4r > 0) {
               if (pre)
-                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128, true>), dim3(nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
-                                   arena, k0, nbl, 0);
+                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128, true>), dim3(PT.n4r, 1, B), dim3(256), 0, s, P->F, t4r, B,
+                                   arena, k0, nbl);
               else
-                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128>), dim3(nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
-                                   arena, k0, nbl, 0);
+                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128>), dim3(PT.n4r, 1, B), dim3(256), 0, s, P->F, t4r, B, arena,
+                                   k0, nbl);
             }
+          } else if (pre) {
+            hipLaunchKernelGGL((k_gemm<T, 4, TILE, TILE, true>), dim3(PT.n4c + PT.n4r, 1, B), dim3(256), 0, s, P->F,
+                               PT.t4, B, arena, k0, nbl);
           } else {
-            const int nc = (G.mmax - k0 + TILE - 1) / TILE;
-            const int nr = (G.mmax - k0 - 1 + TILE - 1) / TILE;
-            if (pre)
-              hipLaunchKernelGGL((k_gemm<T, 4, TILE, TILE, true>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F,
-                                 G.fronts, B, arena, k0, nbl, nc);
-            else
-              hipLaunchKernelGGL((k_gemm<T, 4>), dim3(nc + nr, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
-                                 nbl, nc);
+            hipLaunchKernelGGL((k_gemm<T, 4>), dim3(PT.n4c + PT.n4r, 1, B), dim3(256), 0, s, P->F, PT.t4, B, arena,
+                               k0, nbl);
           }
         }
         const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
         // 32-wide blocks: one wave per block (its barriers cost next to nothing and more blocks
         // share a CU); 64-wide blocks: two waves
-        { Timed tm(P, s, K_DIAG);
-        hipLaunchKernelGGL(k_diag<T>, dim3(G.nf, B), dim3(nbl <= 32 ? 64 : 128), lds_diag, s, P->F, G.fronts, B, arena,
-                           k0, nbl, tiny, P->npert); }
-        const int rem = G.mmax - k0 - 1;
-        if (rem > 0) {
+        if (PT.ndiag > 0) {
+          Timed tm(P, s, K_DIAG);
+          hipLaunchKernelGGL(k_diag<T>, dim3(PT.ndiag, B), dim3(nbl <= 32 ? 64 : 128), lds_diag, s, P->F, PT.diag, B,
+                             arena, k0, nbl, tiny, P->npert);
+        }
+        if (PT.n23 > 0) {
           Timed tm(P, s, K_TRSM);
           if (narrow) {
-            const int nt = (rem + 127) / 128;
-            hipLaunchKernelGGL((k_gemm<T, 2, 128, 32>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
-                               nbl, 0);
-            hipLaunchKernelGGL((k_gemm<T, 3, 32, 128>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0,
-                               nbl, 0);
+            hipLaunchKernelGGL((k_gemm<T, 2, 128, 32>), dim3(PT.n23, 1, B), dim3(256), 0, s, P->F, PT.t23, B, arena,
+                               k0, nbl);
+            hipLaunchKernelGGL((k_gemm<T, 3, 32, 128>), dim3(PT.n23, 1, B), dim3(256), 0, s, P->F, PT.t23, B, arena,
+                               k0, nbl);
           } else {
-            const int nt = (rem + TILE - 1) / TILE;
-            hipLaunchKernelGGL((k_gemm<T, 2>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
-            hipLaunchKernelGGL((k_gemm<T, 3>), dim3(nt, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B, arena, k0, nbl, 0);
+            hipLaunchKernelGGL((k_gemm<T, 2>), dim3(PT.n23, 1, B), dim3(256), 0, s, P->F, PT.t23, B, arena, k0, nbl);
+            hipLaunchKernelGGL((k_gemm<T, 3>), dim3(PT.n23, 1, B), dim3(256), 0, s, P->F, PT.t23, B, arena, k0, nbl);
           }
         }
-    }
-    if (G.ntiles > 0) {
-      Timed tm(P, s, K_GEMM_S);
-      if (G.pmax <= SHORT_K)   // short reduction: the contribution-block read would follow the K loop
-        hipLaunchKernelGGL((k_gemm<T, 1, TILE, TILE, true>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles, B,
-                           arena, 0, nb, 0);
-      else
-        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles, B, arena, 0, nb, 0);
-    }
+      }
+      if (G.ntiles > 0) {
+        Timed tm(P, s, K_GEMM_S);
+        if (G.pmax <= SHORT_K)   // short reduction: the contribution-block read would follow the K loop
+          hipLaunchKernelGGL((k_gemm<T, 1, TILE, TILE, true>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles,
+                             B, arena, 0, nb);
+        else
+          hipLaunchKernelGGL((k_gemm<T, 1>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles, B, arena, 0, nb);
+      }
     }
   }
   SAW_HIP(hipGetLastError());
