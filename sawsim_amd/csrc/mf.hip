@@ -943,7 +943,8 @@ __global__ void __launch_bounds__(256) k_tri(FrontsDev F, const int* __restrict_
 
 }
 
-// Off-diagonal panels, one workgroup per 64-wide output tile of one front and frequency.
+// Off-diagonal panels, one workgroup per 64-wide output tile of one front and frequency; lvl is
+// a tile list (front << 12 | tile), the frequency comes from blockIdx.y.
 //  kind 0 (forward,  L21):   uf[r]  -= sum_k L[p + r][k] y[k]        r in CB rows
 //  kind 1 (backward, U12):   x_p[r] -= sum_j U[r][p + j] x_cb[j]      r in pivot rows
 //  kind 2 (forward,  U12^T): uf[j]  -= sum_k U[k][p + j] w[k]         j in CB rows
@@ -954,13 +955,13 @@ __device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ 
                                            double2* __restrict__ uarena, double2* __restrict__ x, int b) {
   using V = typename cplx<T>::type;
   extern __shared__ unsigned char smem_raw[];
-  const int f = lvl[blockIdx.y];
+  const int f = lvl[blockIdx.x] >> 12;
   const int m = F.m[f], p = F.p[f];
   const int q = m - p;
   const bool out_cb = (KIND == 0 || KIND == 2);
   const int nout = out_cb ? q : p;     // outputs
   const int nin = out_cb ? p : q;      // reduction length
-  const int o0 = blockIdx.x * 64;
+  const int o0 = (lvl[blockIdx.x] & 4095) * 64;
   if (o0 >= nout || nin == 0) return;
   const int no = min(64, nout - o0);
   const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
@@ -1037,7 +1038,7 @@ __global__ void __launch_bounds__(256) k_panel(FrontsDev F, const int* __restric
                                                const typename cplx<T>::type* __restrict__ arena,
                                                double2* __restrict__ uarena, double2* __restrict__ x,
                                                const int* __restrict__ bsel) {
-  panel_body<T, KIND, R>(F, lvl, B, n, arena, uarena, x, bsel ? bsel[blockIdx.z] : (int)blockIdx.z);
+  panel_body<T, KIND, R>(F, lvl, B, n, arena, uarena, x, bsel ? bsel[blockIdx.y] : (int)blockIdx.y);
 
 }
 
@@ -1183,6 +1184,11 @@ struct Sub {
   // block: front << 12 | tile row << 6 | tile column (so a launch has no empty workgroups)
   const int* tiles = nullptr;
   int ntiles = 0;
+  // solve panel tiles (front << 12 | 64-row tile) over the contribution rows (forward kinds)
+  // and over the pivot rows (backward kinds), for fronts with both
+  const int* stiles_cb = nullptr;
+  const int* stiles_p = nullptr;
+  int nstiles_cb = 0, nstiles_p = 0;
 };
 
 struct Level {
@@ -1337,6 +1343,16 @@ int build_partition(Plan* P, int pi, int max_m) {
         }
         S.ntiles = (int)tl.size();
         if (put(tl, &S.tiles)) return 3;
+        std::vector<int> scb, sp;
+        for (int f : part) {
+          const int q = P->hm[f] - P->hp[f], pp = P->hp[f];
+          if (q <= 0 || pp <= 0) continue;
+          for (int t = 0; t < (q + 63) / 64; ++t) scb.push_back(f << 12 | t);
+          for (int t = 0; t < (pp + 63) / 64; ++t) sp.push_back(f << 12 | t);
+        }
+        S.nstiles_cb = (int)scb.size();
+        S.nstiles_p = (int)sp.size();
+        if (put(scb, &S.stiles_cb) || put(sp, &S.stiles_p)) return 3;
         out.push_back(S);
       }
       return 0;
@@ -1564,17 +1580,17 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   auto panel = [&](const Level& L, int kind) {
     const bool out_cb = (kind == 0 || kind == 2);
     for (const Sub& S : (out_cb ? L.sbig[pi] : L.big[pi])) {
-      const int nout = out_cb ? S.cbmax : S.pmax;
-      const int nin = out_cb ? S.pmax : S.cbmax;
-      if (S.nf == 0 || nout == 0 || nin == 0) continue;
+      const int nt = out_cb ? S.nstiles_cb : S.nstiles_p;
+      const int* tl = out_cb ? S.stiles_cb : S.stiles_p;
+      if (nt == 0) continue;
       const size_t lds = sizeof(double2) * R * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
       Timed tm(P, s, out_cb ? K_FWD : K_BWD);
-      dim3 grid((nout + 63) / 64, S.nf, nsel);
+      dim3 grid(nt, nsel);
       switch (kind) {
-        case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
-        case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
-        case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel); break;
-        default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, A, U, X, bsel);
+        case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, tl, B, P->n, A, U, X, bsel); break;
+        case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, tl, B, P->n, A, U, X, bsel); break;
+        case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, tl, B, P->n, A, U, X, bsel); break;
+        default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, tl, B, P->n, A, U, X, bsel);
       }
     }
   };
