@@ -72,7 +72,7 @@ __device__ __forceinline__ void batched(int first, int count, int step, Load loa
 
 // ---------------------------------------------------------------- front assembly
 // Front f = sum of its children's contribution blocks (zero elsewhere), columns [c0, c1) of
-// one frequency per workgroup.  Threads cover rows in slabs of RS (64, 128 or 256) and CG =
+// one frequency per workgroup (lvl: work list front << 12 | column chunk, frequency blockIdx.y).  Threads cover rows in slabs of RS (64, 128 or 256) and CG =
 // 256 / RS columns at a time.  Two children (nested dissection): the parent-row -> child-CB
 // maps of a thread's row stay in registers and the values of CJ columns are loaded
 // unconditionally from clamped in-front addresses before any is selected or stored, so the
@@ -82,9 +82,9 @@ __device__ __forceinline__ void front_init_body(FrontsDev F, const int* __restri
                                                 typename cplx<T>::type* __restrict__ arena, int cols_per_wg, int b) {
   using V = typename cplx<T>::type;
   constexpr int CJ = 8;   // columns per thread whose loads are issued before any store
-  const int f = lvl[blockIdx.y];
+  const int f = lvl[blockIdx.x] >> 12;
   const int m = F.m[f];
-  const int c0 = blockIdx.x * cols_per_wg;
+  const int c0 = (lvl[blockIdx.x] & 4095) * cols_per_wg;
   if (c0 >= m) return;
   const int c1 = min(m, c0 + cols_per_wg);
   V* Fb = front_ptr<T>(arena, F, f, b, B);
@@ -163,7 +163,7 @@ __device__ __forceinline__ void front_init_body(FrontsDev F, const int* __restri
 template <typename T>
 __global__ void __launch_bounds__(256) k_front_init(FrontsDev F, const int* __restrict__ lvl, int B,
                                                     typename cplx<T>::type* __restrict__ arena, int cols_per_wg) {
-  front_init_body<T>(F, lvl, B, arena, cols_per_wg, blockIdx.z);
+  front_init_body<T>(F, lvl, B, arena, cols_per_wg, blockIdx.y);
 
 }
 
@@ -1189,6 +1189,11 @@ struct Sub {
   const int* stiles_cb = nullptr;
   const int* stiles_p = nullptr;
   int nstiles_cb = 0, nstiles_p = 0;
+  // front assembly: columns per workgroup (about 4096 entries of the bucket's largest front,
+  // in whole batches of 8) and the work list front << 12 | column chunk
+  int init_cpw = 8;
+  const int* itiles = nullptr;
+  int nitiles = 0;
 };
 
 struct Level {
@@ -1353,6 +1358,12 @@ int build_partition(Plan* P, int pi, int max_m) {
         S.nstiles_cb = (int)scb.size();
         S.nstiles_p = (int)sp.size();
         if (put(scb, &S.stiles_cb) || put(sp, &S.stiles_p)) return 3;
+        S.init_cpw = 8 * std::max(1, std::min(8, (4096 / std::max(1, S.mmax) + 4) / 8));
+        std::vector<int> it;
+        for (int f : part)
+          for (int c = 0; c < (P->hm[f] + S.init_cpw - 1) / S.init_cpw; ++c) it.push_back(f << 12 | c);
+        S.nitiles = (int)it.size();
+        if (put(it, &S.itiles)) return 3;
         out.push_back(S);
       }
       return 0;
@@ -1444,11 +1455,9 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
     }
     if (L.big[pi].empty()) continue;
     for (const Sub& G : L.big[pi]) {
-      // columns per workgroup: about 4096 entries of the largest front, in whole batches of 8
-      const int cpw = 8 * std::max(1, std::min(8, (4096 / std::max(1, G.mmax) + 4) / 8));
-      { Timed tm(P, s, K_INIT);
-      hipLaunchKernelGGL(k_front_init<T>, dim3((G.mmax + cpw - 1) / cpw, G.nf, B), dim3(256), 0, s, P->F, G.fronts, B,
-                         arena, cpw); }
+      if (G.nitiles == 0) continue;
+      Timed tm(P, s, K_INIT);
+      hipLaunchKernelGGL(k_front_init<T>, dim3(G.nitiles, B), dim3(256), 0, s, P->F, G.itiles, B, arena, G.init_cpw);
     }
     if (L.e1 > L.e0) { Timed tm(P, s, K_ENTRIES);
       hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), (B + ENT_FB - 1) / ENT_FB),
