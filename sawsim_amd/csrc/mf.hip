@@ -1242,6 +1242,7 @@ struct Plan {
   int small_mmax[2] = {0, 0};    // threshold in use
   int solve_full_m = 0;          // fronts up to this order are solved by one workgroup each
   std::vector<void*> part_owned[2];
+  std::vector<void*> panel_owned[2];   // the buckets' panel work lists (rebuilt per panel width)
   int panel_nb[2] = {0, 0};      // panel width the buckets' panel lists were built for (0: none)
   std::vector<int> hm, hp, hlvl_ptr, hlvl_fronts;
 };
@@ -1388,6 +1389,9 @@ int build_partition(Plan* P, int pi, int max_m) {
 // panels are panel_width(nb, level) wide).  32-wide panels of buckets with m > 64 ("narrow")
 // run in 128 x 32 / 32 x 128 tiles, everything else in TILE x TILE tiles.
 int build_panel_lists(Plan* P, int pi, int nb) {
+  for (void* d : P->panel_owned[pi]) (void)hipFree(d);
+  P->panel_owned[pi].clear();
+  P->panel_nb[pi] = 0;
   for (Level& L : P->levels) {
     const int nbl = panel_width(nb, L);
     for (Sub& G : L.big[pi]) {
@@ -1418,8 +1422,8 @@ int build_panel_lists(Plan* P, int pi, int nb) {
           if (v->empty()) continue;
           void* d = nullptr;
           SAW_HIP(hipMalloc(&d, sizeof(int) * v->size()));
+          P->panel_owned[pi].push_back(d);
           SAW_HIP(hipMemcpy(d, v->data(), sizeof(int) * v->size(), hipMemcpyHostToDevice));
-          P->part_owned[pi].push_back(d);
           *dst = (const int*)d;
         }
         G.panels.push_back(PT);
@@ -1755,8 +1759,10 @@ int sawfem_mf_destroy(void* plan) {
   Plan* P = (Plan*)plan;
   if (!P) return 0;
   for (void* d : P->owned) (void)hipFree(d);
-  for (int pi = 0; pi < 2; ++pi)
+  for (int pi = 0; pi < 2; ++pi) {
     for (void* d : P->part_owned[pi]) (void)hipFree(d);
+    for (void* d : P->panel_owned[pi]) (void)hipFree(d);
+  }
   for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
   delete P;
   return 0;
