@@ -204,6 +204,73 @@ __global__ void __launch_bounds__(256) k_front_entries(FrontsDev F, int64_t e0, 
   }
 }
 
+constexpr int LEAF_MAXPOS = 8192;   // largest column chunk (positions) k_leaf_front maps in LDS
+constexpr int LEAF_FB = 16;         // frequencies per k_leaf_front workgroup
+
+// Childless fronts (nested-dissection leaves) assembled in one pass: a workgroup takes columns
+// [c0, c1) of one front for LEAF_FB frequencies (grid y).  The front's A_aa entries in those columns
+// (ent[elo, ehi): entries of a front are sorted by column-major offset) are placed in an LDS map
+// position -> entry, then each thread writes its positions for all B frequencies: the entry
+// value K - w^2 M + i w beta D (rounded to T once) or zero.  This replaces the zero fill and the
+// entries' read-modify-write pass for these fronts (k_front_entries skips them).
+template <typename T>
+__global__ void __launch_bounds__(256) k_leaf_front(FrontsDev F, const int* __restrict__ items,
+                                                    const int64_t* __restrict__ erange, int cols_per_wg, int B,
+                                                    const double* __restrict__ omega, double beta,
+                                                    const int64_t* __restrict__ ent_slot, const int* __restrict__ ent_loc,
+                                                    const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
+                                                    const double2* __restrict__ Daa,
+                                                    typename cplx<T>::type* __restrict__ arena) {
+  using V = typename cplx<T>::type;
+  extern __shared__ int emap[];                  // [positions of the column chunk]
+  const int b0 = blockIdx.y * LEAF_FB, b1 = min(B, b0 + LEAF_FB);
+  const int f = items[blockIdx.x] >> 12;
+  const int m = F.m[f];
+  const int c0 = (items[blockIdx.x] & 4095) * cols_per_wg;
+  const int c1 = min(m, c0 + cols_per_wg);
+  const int base = c0 * m, npos = (c1 - c0) * m;
+  const int64_t elo = erange[2 * blockIdx.x], ehi = erange[2 * blockIdx.x + 1];
+  const int t = threadIdx.x;
+  for (int i = t; i < npos; i += blockDim.x) emap[i] = -1;
+  __syncthreads();
+  for (int64_t e = elo + t; e < ehi; e += blockDim.x) emap[ent_loc[e] - base] = (int)(e - elo);
+  __syncthreads();
+  const int64_t msq = (int64_t)m * m;
+  V* Fb = arena + F.off[f] * (int64_t)B + base;
+  // four positions per thread at a time: their K, M, D stay in registers over the frequencies
+  for (int g = 0; g < npos; g += 4 * 256) {
+    double2 kv[4], mv[4], dv[4];
+    bool has[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int pos = g + t + 256 * u;
+      const int idx = pos < npos ? emap[pos] : -1;
+      has[u] = idx >= 0;
+      kv[u] = mv[u] = dv[u] = make_double2(0, 0);
+      if (has[u]) {
+        const int64_t sl = ent_slot[elo + idx];
+        kv[u] = Kaa[sl];
+        mv[u] = Maa[sl];
+        if (Daa) dv[u] = Daa[sl];
+      }
+    }
+    for (int b = b0; b < b1; ++b) {
+      const double w = omega[b];
+      V* Fw = Fb + (int64_t)b * msq;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int pos = g + t + 256 * u;
+        if (pos < npos) {
+          V v;
+          v.x = (T)(kv[u].x - w * w * mv[u].x - w * beta * dv[u].y);
+          v.y = (T)(kv[u].y - w * w * mv[u].y + w * beta * dv[u].x);
+          Fw[pos] = v;
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- panel factorization
 // In-place inversion of the unit-lower and upper triangles of an n x n block D (column-major,
 // leading dimension ld, in LDS) by row operations (Gauss-Jordan):
@@ -1194,6 +1261,12 @@ struct Sub {
   int init_cpw = 8;
   const int* itiles = nullptr;
   int nitiles = 0;
+  // childless fronts assembled with their entries by k_leaf_front: work list (as itiles), the
+  // entry range of each item (pairs) and the LDS of its position map
+  const int* litiles = nullptr;
+  const int64_t* lranges = nullptr;
+  int nlitiles = 0;
+  size_t leaf_lds = 0;
 };
 
 struct Level {
@@ -1245,6 +1318,8 @@ struct Plan {
   std::vector<void*> panel_owned[2];   // the buckets' panel work lists (rebuilt per panel width)
   int panel_nb[2] = {0, 0};      // panel width the buckets' panel lists were built for (0: none)
   std::vector<int> hm, hp, hlvl_ptr, hlvl_fronts;
+  std::vector<int> hchild_ptr, hent_loc;       // host copies for the leaf-front work lists
+  std::vector<int64_t> hfent_lo, hfent_hi;
 };
 
 // Record a start/stop event pair around one launch when profiling is on.
@@ -1326,7 +1401,7 @@ int build_partition(Plan* P, int pi, int max_m) {
     };
     // size buckets by front order (edges roughly 1.7x apart): fronts of one bucket need grids
     // of similar size, so few workgroups of a launch find no tile
-    auto buckets = [&](std::vector<int> v, std::vector<Sub>& out) -> int {
+    auto buckets = [&](std::vector<int> v, std::vector<Sub>& out, bool factor) -> int {
       out.clear();
       std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return P->hm[a] < P->hm[b]; });
       static const int edges[] = {64, 128, 192, 320, 576, 1088, 1 << 30};
@@ -1360,19 +1435,44 @@ int build_partition(Plan* P, int pi, int max_m) {
         S.nstiles_p = (int)sp.size();
         if (put(scb, &S.stiles_cb) || put(sp, &S.stiles_p)) return 3;
         S.init_cpw = 8 * std::max(1, std::min(8, (4096 / std::max(1, S.mmax) + 4) / 8));
-        std::vector<int> it;
-        for (int f : part)
-          for (int c = 0; c < (P->hm[f] + S.init_cpw - 1) / S.init_cpw; ++c) it.push_back(f << 12 | c);
+        std::vector<int> it, lt;
+        std::vector<int64_t> lr;
+        for (int f : part) {
+          const int mf = P->hm[f], nch = P->hchild_ptr[f + 1] - P->hchild_ptr[f];
+          const int64_t elo = P->hfent_lo[f], ehi = P->hfent_hi[f];
+          const int* loc = P->hent_loc.data();
+          // leaves whose entries are in column-major order and whose chunk map fits LDS
+          const bool leaf = factor && nch == 0 && (int64_t)S.init_cpw * S.mmax <= LEAF_MAXPOS &&
+                            std::is_sorted(loc + elo, loc + ehi);
+          for (int c = 0; c < (mf + S.init_cpw - 1) / S.init_cpw; ++c) {
+            if (!leaf) { it.push_back(f << 12 | c); continue; }
+            const int c0 = c * S.init_cpw, c1 = std::min(mf, c0 + S.init_cpw);
+            lt.push_back(f << 12 | c);
+            lr.push_back(std::lower_bound(loc + elo, loc + ehi, c0 * mf) - loc);
+            lr.push_back(std::lower_bound(loc + elo, loc + ehi, c1 * mf) - loc);
+          }
+          if (leaf) flag[f] = 1;
+        }
         S.nitiles = (int)it.size();
-        if (put(it, &S.itiles)) return 3;
+        S.nlitiles = (int)lt.size();
+        S.leaf_lds = sizeof(int) * (size_t)S.init_cpw * S.mmax;
+        if (put(it, &S.itiles) || put(lt, &S.litiles)) return 3;
+        S.lranges = nullptr;
+        if (!lr.empty()) {
+          void* d = nullptr;
+          SAW_HIP(hipMalloc(&d, sizeof(int64_t) * lr.size()));
+          P->part_owned[pi].push_back(d);
+          SAW_HIP(hipMemcpy(d, lr.data(), sizeof(int64_t) * lr.size(), hipMemcpyHostToDevice));
+          S.lranges = (const int64_t*)d;
+        }
         out.push_back(S);
       }
       return 0;
     };
     stats(sm, L.small[pi]);
     stats(ssm, L.ssmall[pi]);
-    if (put(sm, &L.small[pi].fronts) || put(ssm, &L.ssmall[pi].fronts) || buckets(bg, L.big[pi]) ||
-        buckets(sbg, L.sbig[pi]))
+    if (put(sm, &L.small[pi].fronts) || put(ssm, &L.ssmall[pi].fronts) || buckets(bg, L.big[pi], true) ||
+        buckets(sbg, L.sbig[pi], false))
       return 3;
     for (int f : sm) flag[f] = 1;
   }
@@ -1459,9 +1559,16 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
     }
     if (L.big[pi].empty()) continue;
     for (const Sub& G : L.big[pi]) {
-      if (G.nitiles == 0) continue;
-      Timed tm(P, s, K_INIT);
-      hipLaunchKernelGGL(k_front_init<T>, dim3(G.nitiles, B), dim3(256), 0, s, P->F, G.itiles, B, arena, G.init_cpw);
+      if (G.nlitiles > 0) {
+        Timed tm(P, s, K_INIT);
+        hipLaunchKernelGGL(k_leaf_front<T>, dim3(G.nlitiles, (B + LEAF_FB - 1) / LEAF_FB), dim3(256), G.leaf_lds, s,
+                           P->F, G.litiles, G.lranges, G.init_cpw, B, omega, beta, P->ent_slot, P->ent_loc,
+                           (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, arena);
+      }
+      if (G.nitiles > 0) {
+        Timed tm(P, s, K_INIT);
+        hipLaunchKernelGGL(k_front_init<T>, dim3(G.nitiles, B), dim3(256), 0, s, P->F, G.itiles, B, arena, G.init_cpw);
+      }
     }
     if (L.e1 > L.e0) { Timed tm(P, s, K_ENTRIES);
       hipLaunchKernelGGL(k_front_entries<T>, dim3((unsigned)((L.e1 - L.e0 + 255) / 256), (B + ENT_FB - 1) / ENT_FB),
@@ -1702,7 +1809,11 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
     rc |= upload(P, &P->fent_lo, lo.data(), nf);
     rc |= upload(P, &P->fent_hi, hi.data(), nf);
     if (rc) { delete P; return 3; }
+    P->hfent_lo = lo;
+    P->hfent_hi = hi;
   }
+  P->hchild_ptr.assign(child_ptr, child_ptr + nf + 1);
+  P->hent_loc.assign(ent_loc, ent_loc + nent);
   P->hm.assign(m, m + nf);
   P->hp.assign(p, p + nf);
   P->hlvl_ptr.assign(lvl_ptr, lvl_ptr + nlev + 1);
