@@ -1317,6 +1317,12 @@ struct Plan {
   std::vector<void*> part_owned[2];
   std::vector<void*> panel_owned[2];   // the buckets' panel work lists (rebuilt per panel width)
   int panel_nb[2] = {0, 0};      // panel width the buckets' panel lists were built for (0: none)
+  // independent size buckets of a level run on side streams (default priority) next to the
+  // launch stream: fork after the level's assembly, join before the next level
+  static constexpr int NSIDE = 2;
+  bool overlap = true;
+  hipStream_t side[NSIDE] = {};
+  hipEvent_t fork_ev = nullptr, join_ev[NSIDE] = {};
   std::vector<int> hm, hp, hlvl_ptr, hlvl_fronts;
   std::vector<int> hchild_ptr, hent_loc;       // host copies for the leaf-front work lists
   std::vector<int64_t> hfent_lo, hfent_hi;
@@ -1357,6 +1363,32 @@ int upload(Plan* P, X** dst, const X* src, int64_t count) {
 // Panel width of a level: narrow panels for levels of small fronts.  The factorization and the
 // solves must agree on it (the diagonal blocks hold inverses of nbl x nbl triangles).
 int panel_width(int nb, const Level& L) { return (nb == 64 && L.pmax <= 128) ? 32 : nb; }
+
+// Side streams and events of the bucket overlap, created on first use.
+int ensure_side(Plan* P) {
+  if (P->fork_ev) return 0;
+  for (int k = 0; k < Plan::NSIDE; ++k) {
+    SAW_HIP(hipStreamCreateWithFlags(&P->side[k], hipStreamNonBlocking));
+    SAW_HIP(hipEventCreateWithFlags(&P->join_ev[k], hipEventDisableTiming));
+  }
+  SAW_HIP(hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming));
+  return 0;
+}
+
+// Fork: the side streams wait for everything issued on s so far.  Join: s waits for the side
+// streams.
+int fork_side(Plan* P, hipStream_t s) {
+  SAW_HIP(hipEventRecord(P->fork_ev, s));
+  for (int k = 0; k < Plan::NSIDE; ++k) SAW_HIP(hipStreamWaitEvent(P->side[k], P->fork_ev, 0));
+  return 0;
+}
+int join_side(Plan* P, hipStream_t s) {
+  for (int k = 0; k < Plan::NSIDE; ++k) {
+    SAW_HIP(hipEventRecord(P->join_ev[k], P->side[k]));
+    SAW_HIP(hipStreamWaitEvent(s, P->join_ev[k], 0));
+  }
+  return 0;
+}
 
 // Split every level into the fronts the fused small-front kernel takes (m <= max_m) and the
 // rest, for precision index pi; max_m <= 0 sends every front through the panel path.
@@ -1575,7 +1607,11 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
                          dim3(256), 0, s, P->F,
                          L.e0, L.e1, B, omega, beta, P->ent_front, P->ent_slot, P->ent_loc, (const double2*)Kaa,
                          (const double2*)Maa, (const double2*)Daa, arena, P->fused[pi]); }
-    for (const Sub& G : L.big[pi]) {
+    const bool fork = P->overlap && L.big[pi].size() > 1;
+    if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
+    for (size_t gi = 0; gi < L.big[pi].size(); ++gi) {
+      const Sub& G = L.big[pi][gi];
+      const hipStream_t sg = (!fork || gi == 0) ? s : P->side[(gi - 1) % Plan::NSIDE];
       // left-looking panels: update panel k0 with the factored panels before it (one GEMM with
       // K = k0 for its column and row parts), then its diagonal LU + inverses, then the two
       // panel solves; the contribution block waits for the final Schur complement
@@ -1586,29 +1622,29 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
         const PanelTiles& PT = G.panels[pk];
         const int k0 = (int)pk * nbl;
         if (PT.n4c + PT.n4r > 0) {
-          Timed tm(P, s, K_GEMM_L);
+          Timed tm(P, sg, K_GEMM_L);
           const bool pre = k0 <= SHORT_K;   // short reduction (K = k0): prefetch the C tile
           const int* t4r = PT.t4 + PT.n4c;
           if (narrow) {         // column part in 128 x 32 tiles, row part in 32 x 128 tiles
             if (pre)
-              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32, true>), dim3(PT.n4c, 1, B), dim3(256), 0, s, P->F, PT.t4, B,
+              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32, true>), dim3(PT.n4c, 1, B), dim3(256), 0, sg, P->F, PT.t4, B,
                                  arena, k0, nbl);
             else
-              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32>), dim3(PT.n4c, 1, B), dim3(256), 0, s, P->F, PT.t4, B, arena,
+              hipLaunchKernelGGL((k_gemm<T, 4, 128, 32>), dim3(PT.n4c, 1, B), dim3(256), 0, sg, P->F, PT.t4, B, arena,
                                  k0, nbl);
             if (PT.n4r > 0) {
               if (pre)
-                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128, true>), dim3(PT.n4r, 1, B), dim3(256), 0, s, P->F, t4r, B,
+                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128, true>), dim3(PT.n4r, 1, B), dim3(256), 0, sg, P->F, t4r, B,
                                    arena, k0, nbl);
               else
-                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128>), dim3(PT.n4r, 1, B), dim3(256), 0, s, P->F, t4r, B, arena,
+                hipLaunchKernelGGL((k_gemm<T, 4, 32, 128>), dim3(PT.n4r, 1, B), dim3(256), 0, sg, P->F, t4r, B, arena,
                                    k0, nbl);
             }
           } else if (pre) {
-            hipLaunchKernelGGL((k_gemm<T, 4, TILE, TILE, true>), dim3(PT.n4c + PT.n4r, 1, B), dim3(256), 0, s, P->F,
+            hipLaunchKernelGGL((k_gemm<T, 4, TILE, TILE, true>), dim3(PT.n4c + PT.n4r, 1, B), dim3(256), 0, sg, P->F,
                                PT.t4, B, arena, k0, nbl);
           } else {
-            hipLaunchKernelGGL((k_gemm<T, 4>), dim3(PT.n4c + PT.n4r, 1, B), dim3(256), 0, s, P->F, PT.t4, B, arena,
+            hipLaunchKernelGGL((k_gemm<T, 4>), dim3(PT.n4c + PT.n4r, 1, B), dim3(256), 0, sg, P->F, PT.t4, B, arena,
                                k0, nbl);
           }
         }
@@ -1616,31 +1652,34 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
         // 32-wide blocks: one wave per block (its barriers cost next to nothing and more blocks
         // share a CU); 64-wide blocks: two waves
         if (PT.ndiag > 0) {
-          Timed tm(P, s, K_DIAG);
-          hipLaunchKernelGGL(k_diag<T>, dim3(PT.ndiag, B), dim3(nbl <= 32 ? 64 : 128), lds_diag, s, P->F, PT.diag, B,
+          Timed tm(P, sg, K_DIAG);
+          hipLaunchKernelGGL(k_diag<T>, dim3(PT.ndiag, B), dim3(nbl <= 32 ? 64 : 128), lds_diag, sg, P->F, PT.diag, B,
                              arena, k0, nbl, tiny, P->npert);
         }
         if (PT.n23 > 0) {
-          Timed tm(P, s, K_TRSM);
+          Timed tm(P, sg, K_TRSM);
           if (narrow) {
-            hipLaunchKernelGGL((k_gemm<T, 2, 128, 32>), dim3(PT.n23, 1, B), dim3(256), 0, s, P->F, PT.t23, B, arena,
+            hipLaunchKernelGGL((k_gemm<T, 2, 128, 32>), dim3(PT.n23, 1, B), dim3(256), 0, sg, P->F, PT.t23, B, arena,
                                k0, nbl);
-            hipLaunchKernelGGL((k_gemm<T, 3, 32, 128>), dim3(PT.n23, 1, B), dim3(256), 0, s, P->F, PT.t23, B, arena,
+            hipLaunchKernelGGL((k_gemm<T, 3, 32, 128>), dim3(PT.n23, 1, B), dim3(256), 0, sg, P->F, PT.t23, B, arena,
                                k0, nbl);
           } else {
-            hipLaunchKernelGGL((k_gemm<T, 2>), dim3(PT.n23, 1, B), dim3(256), 0, s, P->F, PT.t23, B, arena, k0, nbl);
-            hipLaunchKernelGGL((k_gemm<T, 3>), dim3(PT.n23, 1, B), dim3(256), 0, s, P->F, PT.t23, B, arena, k0, nbl);
+            hipLaunchKernelGGL((k_gemm<T, 2>), dim3(PT.n23, 1, B), dim3(256), 0, sg, P->F, PT.t23, B, arena, k0, nbl);
+            hipLaunchKernelGGL((k_gemm<T, 3>), dim3(PT.n23, 1, B), dim3(256), 0, sg, P->F, PT.t23, B, arena, k0, nbl);
           }
         }
       }
-      if (G.ntiles > 0) {
-        Timed tm(P, s, K_GEMM_S);
-        if (G.pmax <= SHORT_K)   // short reduction: the contribution-block read would follow the K loop
-          hipLaunchKernelGGL((k_gemm<T, 1, TILE, TILE, true>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles,
-                             B, arena, 0, nb);
-        else
-          hipLaunchKernelGGL((k_gemm<T, 1>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles, B, arena, 0, nb);
-      }
+    }
+    if (fork && join_side(P, s)) return 3;
+    // the Schur complements after the join, one bucket after the other on the launch stream
+    for (const Sub& G : L.big[pi]) {
+      if (G.ntiles == 0) continue;
+      Timed tm(P, s, K_GEMM_S);
+      if (G.pmax <= SHORT_K)   // short reduction: the contribution-block read would follow the K loop
+        hipLaunchKernelGGL((k_gemm<T, 1, TILE, TILE, true>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles,
+                           B, arena, 0, nb);
+      else
+        hipLaunchKernelGGL((k_gemm<T, 1>), dim3(G.ntiles, 1, B), dim3(256), 0, s, P->F, G.tiles, B, arena, 0, nb);
     }
   }
   SAW_HIP(hipGetLastError());
@@ -1661,9 +1700,9 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   // forward passes take the one-workgroup set of the solve partition; backward passes keep the
   // factorization's partition (their CB-to-pivot product has only p active rows per workgroup,
   // which measured slower than the panel kernel on the small fronts)
-  auto tri_sub = [&](const Level& L, const Sub& S, int kind, bool full) {
+  auto tri_sub = [&](const Level& L, const Sub& S, int kind, bool full, hipStream_t st) {
     if (S.nf == 0) return;
-    Timed tm(P, s, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
+    Timed tm(P, st, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
     const int nbl = panel_width(nb, L);
     // kinds 0/1 keep only the nbl x nbl diagonal block in the tile; kinds 2/3 also stage 64
     // columns of a row panel through it
@@ -1678,7 +1717,7 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     if (lds > 64 * 1024)                                                                                 \
       (void)hipFuncSetAttribute((const void*)k_tri<T, K, R, FULL>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                 (int)lds);                                                               \
-    hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, dim3(256), lds, s, P->F, S.fronts, B, P->n, nbl, A, U, X, bsel); \
+    hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, dim3(256), lds, st, P->F, S.fronts, B, P->n, nbl, A, U, X, bsel); \
   } while (0)
     if (full) {
       switch (kind) { case 0: SAW_TRI(0, true); break; case 1: SAW_TRI(1, true); break;
@@ -1689,45 +1728,52 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     }
 #undef SAW_TRI
   };
-  auto tri = [&](const Level& L, int kind, bool full) {
-    const bool fw = kind == 0 || kind == 2;
-    if (full) {
-      tri_sub(L, fw ? L.ssmall[pi] : L.small[pi], kind, true);
-      return;
-    }
-    for (const Sub& S : (fw ? L.sbig[pi] : L.big[pi])) tri_sub(L, S, kind, false);
-  };
-  auto panel = [&](const Level& L, int kind) {
+  auto panel_sub = [&](const Sub& S, int kind, hipStream_t st) {
     const bool out_cb = (kind == 0 || kind == 2);
-    for (const Sub& S : (out_cb ? L.sbig[pi] : L.big[pi])) {
-      const int nt = out_cb ? S.nstiles_cb : S.nstiles_p;
-      const int* tl = out_cb ? S.stiles_cb : S.stiles_p;
-      if (nt == 0) continue;
-      const size_t lds = sizeof(double2) * R * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
-      Timed tm(P, s, out_cb ? K_FWD : K_BWD);
-      dim3 grid(nt, nsel);
-      switch (kind) {
-        case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, s, P->F, tl, B, P->n, A, U, X, bsel); break;
-        case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, s, P->F, tl, B, P->n, A, U, X, bsel); break;
-        case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, s, P->F, tl, B, P->n, A, U, X, bsel); break;
-        default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, s, P->F, tl, B, P->n, A, U, X, bsel);
-      }
+    const int nt = out_cb ? S.nstiles_cb : S.nstiles_p;
+    const int* tl = out_cb ? S.stiles_cb : S.stiles_p;
+    if (nt == 0) return;
+    const size_t lds = sizeof(double2) * R * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
+    Timed tm(P, st, out_cb ? K_FWD : K_BWD);
+    dim3 grid(nt, nsel);
+    switch (kind) {
+      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, st, P->F, tl, B, P->n, A, U, X, bsel); break;
+      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, st, P->F, tl, B, P->n, A, U, X, bsel); break;
+      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, st, P->F, tl, B, P->n, A, U, X, bsel); break;
+      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, st, P->F, tl, B, P->n, A, U, X, bsel);
     }
   };
+  // The one-workgroup set and the size buckets of a level are independent: with more than one,
+  // they run on the launch stream and the side streams between a fork and a join.
+  auto stream_of = [&](bool fork, size_t unit) { return (!fork || unit == 0) ? s : P->side[(unit - 1) % Plan::NSIDE]; };
   const int kf = transpose ? 2 : 0, kb = transpose ? 3 : 1;
   for (size_t li = 0; li < P->levels.size(); ++li) {
     const Level& L = P->levels[li];
     P->cur_level = (int)li;
-    tri(L, kf, true);
-    tri(L, kf, false);
-    panel(L, kf);
+    const bool full = L.ssmall[pi].nf > 0;
+    const bool fork = P->overlap && L.sbig[pi].size() + (full ? 1 : 0) > 1;
+    if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
+    if (full) tri_sub(L, L.ssmall[pi], kf, true, s);
+    for (size_t gi = 0; gi < L.sbig[pi].size(); ++gi) {
+      const hipStream_t st = stream_of(fork, gi + (full ? 1 : 0));
+      tri_sub(L, L.sbig[pi][gi], kf, false, st);
+      panel_sub(L.sbig[pi][gi], kf, st);
+    }
+    if (fork && join_side(P, s)) return 3;
   }
   for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
     const Level& L = P->levels[l];
     P->cur_level = l;
-    panel(L, kb);
-    tri(L, kb, false);
-    tri(L, kb, true);
+    const bool full = L.small[pi].nf > 0;
+    const bool fork = P->overlap && L.big[pi].size() + (full ? 1 : 0) > 1;
+    if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
+    for (size_t gi = 0; gi < L.big[pi].size(); ++gi) {
+      const hipStream_t st = stream_of(fork, gi + (full ? 1 : 0));
+      panel_sub(L.big[pi][gi], kb, st);
+      tri_sub(L, L.big[pi][gi], kb, false, st);
+    }
+    if (full) tri_sub(L, L.small[pi], kb, true, s);
+    if (fork && join_side(P, s)) return 3;
   }
   SAW_HIP(hipGetLastError());
   return 0;
@@ -1869,6 +1915,14 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
 int sawfem_mf_destroy(void* plan) {
   Plan* P = (Plan*)plan;
   if (!P) return 0;
+  if (P->fork_ev) {
+    (void)hipDeviceSynchronize();
+    for (int k = 0; k < Plan::NSIDE; ++k) {
+      (void)hipStreamDestroy(P->side[k]);
+      (void)hipEventDestroy(P->join_ev[k]);
+    }
+    (void)hipEventDestroy(P->fork_ev);
+  }
   for (void* d : P->owned) (void)hipFree(d);
   for (int pi = 0; pi < 2; ++pi) {
     for (void* d : P->part_owned[pi]) (void)hipFree(d);
