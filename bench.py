@@ -423,8 +423,12 @@ def main():
             lupdate_gemm_tflops=lupd_tf,
             solve=dict(bound='hbm', achieved_gbs=solve_gbs, peak_gbs=8000.0,
                        frac=(solve_gbs / 8000.0) if solve_gbs else None, bytes=solve_bytes, ms=solve_ms,
-                       note='forward+backward passes; algorithmic bytes = stored L/U entries x 8 B per point')),
+                       note='forward+backward passes; algorithmic bytes = stored L/U entries x 8 B per point; '
+                            'ms = summed HIP-event spans, which overlap across the size buckets of a level')),
         kernel_ms={k: v[0] for k, v in prof.items()},
+        kernel_ms_note='per kind, summed HIP-event spans of its launches; the size buckets of a level run their '
+                       'panel kernels and solves concurrently on side streams, so those sums exceed their wall '
+                       'time; the Schur complements run one at a time on the launch stream',
         kernel_ms_per_level=prof_levels,
         factor_flops_per_point=work['factor_flops'],
         cpu_baseline=cpu,
