@@ -111,7 +111,11 @@ int sawfem_mf_set_solve_full(void* plan, int max_m);
 
 /* Factor A(w_b) = K + i w_b beta D - w_b^2 M for B frequencies (omega: device doubles)
  * into arena (arena_elems * B entries; complex64 if prec == 0, complex128 if prec == 1).
- * Kaa/Maa/Daa: complex128 values on the active pattern (Daa may be NULL).  nb: panel width. */
+ * Kaa/Maa/Daa: complex128 values on the active pattern (Daa may be NULL).  nb: panel width.
+ * Work is ordered on `stream`: the independent size buckets of a tree level also run on two
+ * side streams of the plan (default priority, created on first use), forked from and joined
+ * back to `stream` within the call, so a caller sees ordinary stream semantics.  The same holds
+ * for sawfem_mf_solve.  A plan is used by one host thread at a time. */
 int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* omega, double beta, const void* Kaa,
                      const void* Maa, const void* Daa, void* arena, int nb);
 
