@@ -39,8 +39,9 @@ from .symbolic import analyse
 
 
 # in the forward passes, fronts up to this order are solved by one workgroup (pivot triangle +
-# panel) per frequency (measured on config 2: 192 is the best of 0/128/192/256)
-SOLVE_FULL_M = 192
+# panel) per frequency (measured on config 2 with the size buckets of a level on concurrent
+# streams: 256 is the best of 128/192/256/320/448/640)
+SOLVE_FULL_M = 256
 
 
 def ir_error(dq, dq_prev):
