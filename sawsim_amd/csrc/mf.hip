@@ -12,14 +12,18 @@
 // U (upper), rows/columns p.. of the trailing block hold the Schur complement that the
 // parent assembles (extend-add), and U12 = F[0:p, p:m], L21 = F[p:m, 0:p].
 //
-// Per level (leaves first):
+// Per level (leaves first), per front-size bucket:
+//   k_leaf_front    : childless fronts: zeros and their A(w) entries in one pass
 //   k_front_init    : F = sum of children's contribution blocks (gather, no atomics)
-//   k_front_entries : F += A(w) entries owned by this front (unique targets)
-//   for each pivot panel [k0, k0 + kb):
-//     k_diag        : LU of the kb x kb diagonal block (no pivoting, tiny-pivot guard)
-//     k_trsm        : L panel = A U_D^-1, U panel = L_D^-1 A
-//     k_gemm (L)    : update of the trailing rows/columns that are still pivots
-//   k_gemm (S)      : Schur complement F22 -= L21 U12 with K = p  (MFMA)
+//   k_front_entries : F += A(w) entries owned by the other fronts (unique targets)
+//   for each pivot panel [k0, k0 + kb) (left-looking):
+//     k_gemm mode 4 : the panel's column and row parts -= the factored panels before it (MFMA)
+//     k_diag        : LU of the kb x kb diagonal block (no pivoting, tiny-pivot guard), replaced
+//                     in place by the inverses of its triangles
+//     k_gemm 2 / 3  : L panel = A U_D^-1, U panel = L_D^-1 A  (MFMA)
+//   k_gemm mode 1   : Schur complement F22 -= L21 U12 with K = p  (MFMA)
+// Launches walk host-built work lists (one entry per workgroup); the buckets of a level run
+// their panel phases concurrently on side streams between a fork and a join.
 #include "common.h"
 
 #include <algorithm>
@@ -72,8 +76,9 @@ __device__ __forceinline__ void batched(int first, int count, int step, Load loa
 
 // ---------------------------------------------------------------- front assembly
 // Front f = sum of its children's contribution blocks (zero elsewhere), columns [c0, c1) of
-// one frequency per workgroup (lvl: work list front << 12 | column chunk, frequency blockIdx.y).  Threads cover rows in slabs of RS (64, 128 or 256) and CG =
-// 256 / RS columns at a time.  Two children (nested dissection): the parent-row -> child-CB
+// one frequency per workgroup (lvl: work list front << 12 | column chunk; frequency
+// blockIdx.y).  Threads cover rows in slabs of RS (64, 128 or 256) and CG = 256 / RS columns at
+// a time.  Two children (nested dissection): the parent-row -> child-CB
 // maps of a thread's row stay in registers and the values of CJ columns are loaded
 // unconditionally from clamped in-front addresses before any is selected or stored, so the
 // loads are in flight together; other child counts take a generic loop.
