@@ -239,3 +239,49 @@ def test_one_workgroup_solves_match_the_kernel_pair(precision):
     tol = 1e-12 if precision == 'fp64' else 1e-9
     assert np.max(np.abs(a['Q'] - b['Q']) / np.abs(a['Q'])) <= tol
     assert np.max(np.abs(a['dY'] - b['dY']).max(axis=1) / np.abs(a['dY']).max(axis=1)) <= 100 * tol
+
+
+def test_factor_and_solve_on_a_side_torch_stream():
+    """The plan forks the size buckets of a level onto its own streams and joins them back to
+    the caller's stream: factor + solve issued on a non-default torch stream give the same
+    arena and solution bit for bit as on the default stream."""
+    _say('[test_factor_and_solve_on_a_side_torch_stream]')
+    import torch
+    from sawsim_amd import _lib
+    from sawsim_amd.piezo_fem import stream_ptr
+    cell = workloads.double_layer()
+    eng = _engine(cell, 2 * np.pi * cell.band_hz[1])
+    # some level has fronts in more than one size bucket (m <= 64, 128, 192, ...): the fork runs
+    edges = [64, 128, 192, 320, 576, 1088]
+    mf = eng.mf
+    buckets = [np.unique(np.searchsorted(edges, mf.m[mf.lvl_fronts[mf.lvl_ptr[L]:mf.lvl_ptr[L + 1]]]))
+               for L in range(mf.lvl_ptr.size - 1)]
+    assert max(b.size for b in buckets) > 1
+    omegas = 2 * np.pi * np.linspace(1.6e9, 2.4e9, 5)
+    B, n = omegas.size, eng.n_active
+    om = torch.as_tensor(omegas, device='cuda')
+    g = torch.Generator(device='cpu').manual_seed(5)
+    rhs = torch.complex(torch.randn(B, 2, n, generator=g, dtype=torch.float64),
+                        torch.randn(B, 2, n, generator=g, dtype=torch.float64)).cuda()
+
+    def run():
+        s = stream_ptr()
+        arena = torch.empty(eng.mf.arena_elems * B, dtype=torch.complex64, device='cuda')
+        uarena = torch.empty(eng.mf.uarena_elems * B * 2, dtype=torch.complex128, device='cuda')
+        x = rhs.clone()
+        _lib.call('sawfem_mf_factor', eng._plan, s, 0, B, om.data_ptr(), 0.0, eng.Kaa.data_ptr(),
+                  eng.Maa.data_ptr(), None, arena.data_ptr(), 64)
+        _lib.call('sawfem_mf_solve', eng._plan, s, 0, B, 2, arena.data_ptr(), uarena.data_ptr(), x.data_ptr(), 0,
+                  64, None, 0)
+        return arena, x
+
+    torch.cuda.synchronize()
+    a0, x0 = run()
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        a1, x1 = run()
+    torch.cuda.synchronize()
+    assert torch.equal(a0, a1)
+    assert torch.equal(x0, x1)
