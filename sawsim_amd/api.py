@@ -195,6 +195,29 @@ def write_sweep_results(out: Path, fre, charge, voltage, *, mesh=None, fields=No
     return y
 
 
+def _write_mesh_only(model, cell, out, started):
+    """Mesh, metadata (task_kind 'mesh') and manifest of a mesh-only run (``solver.py:85-100``)."""
+    from .workloads import HexCell
+    out = Path(out)
+    if isinstance(cell, HexCell):
+        mesh = cell.mesh
+        np.savez_compressed(out / 'mesh.npz', position_m=mesh.pos_m, hexahedra=mesh.hexas, material_tags=mesh.hex_tag)
+        nodes, elements, dim, comps = int(mesh.pos_m.shape[0]), int(mesh.hexas.shape[0]), 3, 3
+    else:
+        m = cell.model
+        np.savez_compressed(out / 'mesh.npz', position_m=m.pos_m, quads=m.quads, material_tags=m.elem_tag)
+        nodes, elements, dim, comps = int(m.N), int(m.ne), 2, int(cell.ndisp)
+    _atomic_json(out / 'metadata.json', dict(task_kind='mesh', model_id=model.model_id, nodes=nodes,
+                                             elements=elements, mesh_dimension=dim, displacement_components=comps,
+                                             elapsed_seconds=time.time() - started, units={'position': 'm'},
+                                             input_file='input.json'))
+    media = {'.json': 'application/json', '.npz': 'application/octet-stream'}
+    artifacts = [dict(name=p.name, kind='data', media_type=media[p.suffix]) for p in sorted(out.iterdir())
+                 if p.suffix in media and p.name not in ('manifest.json', 'progress.json')]
+    artifacts.append(dict(name='manifest.json', kind='manifest', media_type='application/json'))
+    _atomic_json(out / 'manifest.json', dict(model_id=model.model_id, artifacts=artifacts))
+
+
 def _sweep_hex(model, cell, fre, out, progress, started):
     """Hex27 slab sweep and artifacts (``hex_solver.py:88-120``): Y = +i w Q / V per unit aperture."""
     from .hex27 import KM_SCALE
@@ -203,7 +226,8 @@ def _sweep_hex(model, cell, fre, out, progress, started):
     eng = build_hex_engine(cell)
     progress('sweep', 0, fre.size)
     V = float(c['voltage'])
-    Q, info = eng.sweep_charges(fre, V, precision=c['precision'], batch=c['batch'])
+    Q, info = eng.sweep_charges(fre, V, precision=c['precision'], batch=c['batch'],
+                                on_chunk=lambda done, total: progress('sweep', done, total))
     if not np.all(np.isfinite(Q)):
         raise ValueError('non-finite charge in the sweep')
     progress('postprocess', fre.size, fre.size)
@@ -249,11 +273,16 @@ def _sweep_hex(model, cell, fre, out, progress, started):
     _atomic_json(out / 'manifest.json', dict(model_id=model.model_id, artifacts=artifacts))
 
 
-def sweep(model: Model | dict, output_dir=None, *, on_progress: Optional[Callable] = None) -> Result:
+def sweep(model: Model | dict, output_dir=None, *, mesh_only: bool = False, on_progress: Optional[Callable] = None,
+          should_cancel: Optional[Callable] = None) -> Result:
     """Run the frequency sweep of ``model`` on the GPU and return a Result.
 
     A fresh temporary directory is used when ``output_dir`` is None; a directory that already
-    holds results is refused (``FileExistsError``), as in the reference.
+    holds results is refused (``FileExistsError``), as in the reference.  ``mesh_only`` writes
+    the mesh, metadata and manifest and stops before any solve (``solver.py:85-100``).
+    Cancellation, as in ``run_simulation`` (``solver.py:64-66``): a file named ``cancel`` in
+    the output directory or ``should_cancel()`` returning true, checked at every progress event
+    (stage boundaries and after each frequency chunk), raises ``InterruptedError``.
     """
     from . import params as prm
     from .piezo_fem import SCALE
@@ -267,14 +296,25 @@ def sweep(model: Model | dict, output_dir=None, *, on_progress: Optional[Callabl
     _atomic_json(out / 'input.json', model.to_dict())
     started = time.time()
 
+    def check_cancel():
+        if (out / 'cancel').exists() or (should_cancel is not None and should_cancel()):
+            raise InterruptedError('sweep cancelled')
+
     def progress(stage, done, total):
+        check_cancel()
         event = dict(stage=stage, completed=done, total=total, elapsed_seconds=round(time.time() - started, 2))
         _atomic_json(out / 'progress.json', event)
         if on_progress is not None:
             on_progress(event)
+        check_cancel()
 
     cell = model.cell()
     fre = cell.frequencies()
+    if mesh_only:
+        progress('mesh', 0, 1)
+        _write_mesh_only(model, cell, out, started)
+        progress('complete', 1, 1)
+        return Result(out)
     progress('assembly', 0, fre.size)
     from .workloads import HexCell
     if isinstance(cell, HexCell):
@@ -285,10 +325,13 @@ def sweep(model: Model | dict, output_dir=None, *, on_progress: Optional[Callabl
     eng = build_engine(cell, gradient=gradient)
     progress('sweep', 0, fre.size)
     V = float(c['voltage'])
+    def chunk_done(done, total):
+        progress('sweep', done, total)
     if gradient:
-        Q, dY, info = eng.sweep_charges(fre, V, precision=c['precision'], batch=c['batch'], gradient=True)
+        Q, dY, info = eng.sweep_charges(fre, V, precision=c['precision'], batch=c['batch'], gradient=True,
+                                        on_chunk=chunk_done)
     else:
-        Q, info = eng.sweep_charges(fre, V, precision=c['precision'], batch=c['batch'])
+        Q, info = eng.sweep_charges(fre, V, precision=c['precision'], batch=c['batch'], on_chunk=chunk_done)
         dY = None
     if not np.all(np.isfinite(Q)):
         raise ValueError('non-finite charge in the sweep')

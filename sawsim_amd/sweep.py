@@ -550,11 +550,12 @@ class FrequencySolver:
         per = self.arena_bytes(1, precision) + 16 * self.mf.uarena_elems * R + 16 * 12 * R * self.n_active
         return max(1, int(fraction * free // per))
 
-    def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, **kw):
+    def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, on_chunk=None, **kw):
         """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies (default: as many
         as fit the free HBM, see :meth:`auto_batch`).
 
         With ``gradient=True`` returns (Q, dY, info) where dY (P, n_params) is dY11/dp.
+        ``on_chunk(done, total)`` is called after each chunk (progress and cancellation points).
         """
         fre = np.asarray(fre, dtype=np.float64)
         if not batch:
@@ -579,6 +580,8 @@ class FrequencySolver:
             item['solved_per_pass'] = res.get('solved')
             info.append(item)
             del res
+            if on_chunk is not None:
+                on_chunk(min(i0 + batch, fre.size), fre.size)
         if gradient:
             return Q, dY, info
         return Q, info

@@ -56,3 +56,43 @@ def test_sweep_reproduces_the_stored_double_layer_curve(tmp_path):
     assert f['displacement_m'].shape[1] == 3 and np.all(np.isfinite(f['potential_v']))
     with pytest.raises(FileExistsError):
         sweep(Model('sp_double_layer', points=5), tmp_path / 'run')
+
+
+def test_mesh_only_writes_the_mesh_and_stops(tmp_path):
+    out = tmp_path / 'mesh'
+    r = sweep(Model('sp_double_layer', points=5), out, mesh_only=True)
+    meta = json.loads((out / 'metadata.json').read_text())
+    assert meta['task_kind'] == 'mesh' and meta['model_id'] == 'sp_double_layer' and meta['nodes'] > 0
+    m = np.load(out / 'mesh.npz')
+    assert m['quads'].shape[1] == 9 and m['position_m'].shape == (meta['nodes'], 2)
+    assert m['material_tags'].shape[0] == meta['elements']
+    names = {a['name'] for a in r.manifest['artifacts']}
+    assert {'input.json', 'mesh.npz', 'metadata.json', 'manifest.json'} <= names
+    assert not (out / 'curve.json').exists() and r.frequency_ghz.size == 0
+
+
+def test_cancellation_raises_before_any_solve(tmp_path):
+    events = []
+    with pytest.raises(InterruptedError):
+        sweep(Model('sp_double_layer', points=5), tmp_path / 'a', on_progress=events.append,
+              should_cancel=lambda: True)
+    assert events == [] and not (tmp_path / 'a' / 'curve.json').exists()
+    (tmp_path / 'b').mkdir()
+    (tmp_path / 'b' / 'cancel').write_text('')
+    with pytest.raises(InterruptedError):
+        sweep(Model('sp_double_layer', points=5), tmp_path / 'b', mesh_only=True)
+
+
+@pytest.mark.gpu
+def test_cancellation_between_frequency_chunks(tmp_path):
+    print('[test_cancellation_between_frequency_chunks]', flush=True)
+    sys.stdout.flush()
+    events = []
+
+    def cancel():
+        return any(e['stage'] == 'sweep' and e['completed'] > 0 for e in events)
+    with pytest.raises(InterruptedError):
+        sweep(Model('sp_double_layer', points=9, batch=3), tmp_path / 'run', on_progress=events.append,
+              should_cancel=cancel)
+    done = [e['completed'] for e in events if e['stage'] == 'sweep']
+    assert done[-1] == 3 and not (tmp_path / 'run' / 'curve.json').exists()
