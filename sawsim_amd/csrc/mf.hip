@@ -749,7 +749,7 @@ __device__ __forceinline__ void gemm_body(FrontsDev F, const int* __restrict__ l
 }
 
 template <typename T, int mode, int TR = TILE, int TC = TILE, bool PRE = false>
-__global__ void __launch_bounds__(256, (PRE && sizeof(T) == 4) ? 3 : 1) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
+__global__ void __launch_bounds__(256, (PRE && sizeof(T) == 4 && TC != 128) ? 3 : 1) k_gemm(FrontsDev F, const int* __restrict__ lvl, int B,
                                               typename cplx<T>::type* __restrict__ arena, int k0, int nb) {
   gemm_body<T, mode, TR, TC, PRE>(F, lvl, B, arena, k0, nb, blockIdx.z);
 
@@ -1139,24 +1139,30 @@ __global__ void k_residual(int n, int B, const int64_t* __restrict__ indptr, con
   if (i >= n) return;
   const int b0 = blockIdx.y * 8;
   const int nb = min(8, B - b0);
+  // the eight frequencies' accumulators stay in registers (fully unrolled, guarded by nb)
   double2 acc[8];
   double w[8];
+#pragma unroll
   for (int q = 0; q < 8; ++q) { acc[q] = make_double2(0, 0); w[q] = q < nb ? omega[b0 + q] : 0.0; }
   for (int64_t s0 = indptr[i]; s0 < indptr[i + 1]; ++s0) {
     const int j = indices[s0];
     const int64_t s = tslot ? tslot[s0] : s0;
     const double2 k = Kaa[s], mm = Maa[s];
     const double2 d = Daa ? Daa[s] : make_double2(0, 0);
-    for (int q = 0; q < nb; ++q) {
-      double2 a = make_double2(k.x - w[q] * w[q] * mm.x - w[q] * beta * d.y,
-                               k.y - w[q] * w[q] * mm.y + w[q] * beta * d.x);
-      acc[q] = cfma(acc[q], a, x[(int64_t)(b0 + q) * n + j]);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q < nb) {
+        double2 a = make_double2(k.x - w[q] * w[q] * mm.x - w[q] * beta * d.y,
+                                 k.y - w[q] * w[q] * mm.y + w[q] * beta * d.x);
+        acc[q] = cfma(acc[q], a, x[(int64_t)(b0 + q) * n + j]);
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (q < nb) {
+      const double2 bb = rhs[(int64_t)(b0 + q) * n + i];
+      r[(int64_t)(b0 + q) * n + i] = make_double2(bb.x - acc[q].x, bb.y - acc[q].y);
     }
-  }
-  for (int q = 0; q < nb; ++q) {
-    const double2 bb = rhs[(int64_t)(b0 + q) * n + i];
-    r[(int64_t)(b0 + q) * n + i] = make_double2(bb.x - acc[q].x, bb.y - acc[q].y);
-  }
 }
 
 // Frequency-contiguous residual: xt, rhst, rt are [n][Bp] (row-major, Bp = frequencies padded to

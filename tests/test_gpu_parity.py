@@ -285,3 +285,51 @@ def test_factor_and_solve_on_a_side_torch_stream():
     torch.cuda.synchronize()
     assert torch.equal(a0, a1)
     assert torch.equal(x0, x1)
+
+
+def test_frequency_major_and_transposed_residuals_agree():
+    """sawfem_residual ([B][n] vectors, eight frequencies per thread) and sawfem_residual_t
+    ([n][Bp] vectors, one wave per row and 64 frequencies) compute the same FP64 residual, and
+    both match r = rhs - A(w) x assembled on the host from the same values."""
+    _say('[test_frequency_major_and_transposed_residuals_agree]')
+    import scipy.sparse as sp
+    import torch
+    from sawsim_amd import _lib
+    from sawsim_amd.piezo_fem import stream_ptr
+    cell = workloads.config3(mesh_um=0.12, points=4)
+    eng = workloads.build_engine(cell)
+    n, B = eng.n_active, 11                       # 11 = one full and one partial group of 8
+    Bp = 64
+    omegas = 2 * np.pi * np.linspace(1.35e9, 1.5e9, B)
+    om = torch.as_tensor(omegas, device='cuda')
+    g = torch.Generator(device='cpu').manual_seed(7)
+    x = torch.complex(torch.randn(B, n, generator=g, dtype=torch.float64),
+                      torch.randn(B, n, generator=g, dtype=torch.float64)).cuda()
+    rhs = torch.complex(torch.randn(B, n, generator=g, dtype=torch.float64),
+                        torch.randn(B, n, generator=g, dtype=torch.float64)).cuda()
+    s = stream_ptr()
+    r1 = torch.empty_like(x)
+    _lib.call('sawfem_residual', s, n, B, eng.aa_indptr.data_ptr(), eng.aa_indices.data_ptr(), None,
+              eng.Kaa.data_ptr(), eng.Maa.data_ptr(), eng._dptr(eng.Daa), om.data_ptr(), eng.beta, x.data_ptr(),
+              rhs.data_ptr(), r1.data_ptr())
+    xt = torch.zeros((n, Bp), dtype=torch.complex128, device='cuda')
+    rt = torch.zeros((n, Bp), dtype=torch.complex128, device='cuda')
+    rhst = torch.zeros((n, Bp), dtype=torch.complex128, device='cuda')
+    xt[:, :B] = x.T
+    rhst[:, :B] = rhs.T
+    _lib.call('sawfem_residual_t', s, n, B, Bp, eng.aa_indptr.data_ptr(), eng.aa_indices.data_ptr(), None,
+              eng.Kaa.data_ptr(), eng.Maa.data_ptr(), eng._dptr(eng.Daa), om.data_ptr(), eng.beta, xt.data_ptr(),
+              rhst.data_ptr(), rt.data_ptr())
+    torch.cuda.synchronize()
+    r2 = rt[:, :B].T
+    scale = r1.abs().max().item()
+    assert (r1 - r2).abs().max().item() <= 1e-12 * scale
+    ip, ix = eng.aa_indptr.cpu().numpy(), eng.aa_indices.cpu().numpy()
+    K = sp.csr_matrix((eng.Kaa.cpu().numpy(), ix, ip), shape=(n, n))
+    M = sp.csr_matrix((eng.Maa.cpu().numpy(), ix, ip), shape=(n, n))
+    D = sp.csr_matrix((eng.Daa.cpu().numpy(), ix, ip), shape=(n, n)) if eng.Daa is not None else 0 * K
+    xs, rs = x.cpu().numpy(), rhs.cpu().numpy()
+    for b in (0, 7, 8, 10):
+        w = omegas[b]
+        ref = rs[b] - (K + 1j * w * eng.beta * D - w * w * M) @ xs[b]
+        assert np.abs(r1[b].cpu().numpy() - ref).max() <= 1e-10 * np.abs(ref).max()
