@@ -1289,7 +1289,7 @@ struct Level {
   Sub small[2];
   std::vector<Sub> big[2];
   // solves: one-workgroup (pivot triangle + panel) set and the two-kernel fronts, bucketed
-  Sub ssmall[2];
+  std::vector<Sub> ssmall[2];   // one-workgroup forward solves, in size buckets
   std::vector<Sub> sbig[2];
 };
 
@@ -1513,8 +1513,7 @@ int build_partition(Plan* P, int pi, int max_m) {
       return 0;
     };
     stats(sm, L.small[pi]);
-    stats(ssm, L.ssmall[pi]);
-    if (put(sm, &L.small[pi].fronts) || put(ssm, &L.ssmall[pi].fronts) || buckets(bg, L.big[pi], true) ||
+    if (put(sm, &L.small[pi].fronts) || buckets(ssm, L.ssmall[pi], false) || buckets(bg, L.big[pi], true) ||
         buckets(sbg, L.sbig[pi], false))
       return 3;
     for (int f : sm) flag[f] = 1;
@@ -1764,12 +1763,12 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   for (size_t li = 0; li < P->levels.size(); ++li) {
     const Level& L = P->levels[li];
     P->cur_level = (int)li;
-    const bool full = L.ssmall[pi].nf > 0;
-    const bool fork = P->overlap && L.sbig[pi].size() + (full ? 1 : 0) > 1;
+    const size_t nfull = L.ssmall[pi].size();
+    const bool fork = P->overlap && L.sbig[pi].size() + nfull > 1;
     if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
-    if (full) tri_sub(L, L.ssmall[pi], kf, true, s);
+    for (size_t gi = 0; gi < nfull; ++gi) tri_sub(L, L.ssmall[pi][gi], kf, true, stream_of(fork, gi));
     for (size_t gi = 0; gi < L.sbig[pi].size(); ++gi) {
-      const hipStream_t st = stream_of(fork, gi + (full ? 1 : 0));
+      const hipStream_t st = stream_of(fork, gi + nfull);
       tri_sub(L, L.sbig[pi][gi], kf, false, st);
       panel_sub(L.sbig[pi][gi], kf, st);
     }
