@@ -1410,8 +1410,11 @@ int build_partition(Plan* P, int pi, int max_m) {
   const int nf = P->nf;
   // Schur tile lists pack front (19 bits) and tile row / column (6 bits each)
   if (nf >= (1 << 19)) { sawfem_set_error("sawfem_mf: more than 524287 fronts"); return 2; }
-  for (int f = 0; f < nf; ++f)
+  for (int f = 0; f < nf; ++f) {
     if (P->hm[f] - P->hp[f] > 64 * TILE) { sawfem_set_error("sawfem_mf: contribution block above 4096 rows"); return 2; }
+    // assembly work lists pack the column chunk (>= 8 columns) in 12 bits
+    if (P->hm[f] > 8 * 4095) { sawfem_set_error("sawfem_mf: front order above 32760"); return 2; }
+  }
   std::vector<unsigned char> flag(nf, 0);
   auto put = [&](const std::vector<int>& v, const int** dst) -> int {
     *dst = nullptr;
