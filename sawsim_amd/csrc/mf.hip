@@ -1365,9 +1365,9 @@ int upload(Plan* P, X** dst, const X* src, int64_t count) {
   if (count <= 0) { *dst = nullptr; return 0; }
   void* d = nullptr;
   SAW_HIP(hipMalloc(&d, sizeof(X) * count));
-  SAW_HIP(hipMemcpy(d, src, sizeof(X) * count, hipMemcpyHostToDevice));
   P->owned.push_back(d);
   *dst = (X*)d;
+  SAW_HIP(hipMemcpy(d, src, sizeof(X) * count, hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -1376,11 +1376,12 @@ int upload(Plan* P, X** dst, const X* src, int64_t count) {
 int panel_width(int nb, const Level& L) { return (nb == 64 && L.pmax <= 128) ? 32 : nb; }
 
 // Side streams and events of the bucket overlap, created on first use.
+// A failure part-way leaves what was created for the next call (or the plan's destroy).
 int ensure_side(Plan* P) {
   if (P->fork_ev) return 0;
   for (int k = 0; k < Plan::NSIDE; ++k) {
-    SAW_HIP(hipStreamCreateWithFlags(&P->side[k], hipStreamNonBlocking));
-    SAW_HIP(hipEventCreateWithFlags(&P->join_ev[k], hipEventDisableTiming));
+    if (!P->side[k]) SAW_HIP(hipStreamCreateWithFlags(&P->side[k], hipStreamNonBlocking));
+    if (!P->join_ev[k]) SAW_HIP(hipEventCreateWithFlags(&P->join_ev[k], hipEventDisableTiming));
   }
   SAW_HIP(hipEventCreateWithFlags(&P->fork_ev, hipEventDisableTiming));
   return 0;
@@ -1421,9 +1422,9 @@ int build_partition(Plan* P, int pi, int max_m) {
     if (v.empty()) return 0;
     void* d = nullptr;
     SAW_HIP(hipMalloc(&d, sizeof(int) * v.size()));
-    SAW_HIP(hipMemcpy(d, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
     P->part_owned[pi].push_back(d);
     *dst = (const int*)d;
+    SAW_HIP(hipMemcpy(d, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice));
     return 0;
   };
   // the solves' one-workgroup set must fit the pivot-triangle kernel's LDS with the full front
@@ -1523,9 +1524,9 @@ int build_partition(Plan* P, int pi, int max_m) {
   }
   void* d = nullptr;
   SAW_HIP(hipMalloc(&d, nf));
-  SAW_HIP(hipMemcpy(d, flag.data(), nf, hipMemcpyHostToDevice));
   P->part_owned[pi].push_back(d);
   P->fused[pi] = (unsigned char*)d;
+  SAW_HIP(hipMemcpy(d, flag.data(), nf, hipMemcpyHostToDevice));
   P->panel_nb[pi] = 0;
   return 0;
 }
@@ -1822,7 +1823,7 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
   int64_t acc = 0, uacc = 0;
   for (int f = 0; f < nf; ++f) {
     if (m[f] <= 0 || p[f] < 0 || p[f] > m[f] || rows_ptr[f + 1] - rows_ptr[f] != m[f]) {
-      delete P;
+      sawfem_mf_destroy(P);
       sawfem_set_error("sawfem_mf_create: inconsistent front");
       return 1;
     }
@@ -1852,7 +1853,7 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
   rc |= upload(P, &P->ent_loc, ent_loc, nent);
   int zero = 0;
   rc |= upload(P, &P->npert, &zero, 1);
-  if (rc) { delete P; return 3; }
+  if (rc) { sawfem_mf_destroy(P); return 3; }
   // per-front entry ranges (entries are grouped by front within each level)
   {
     std::vector<int64_t> lo(nf, 0), hi(nf, 0);
@@ -1864,13 +1865,13 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
       if (!seen[f]) lo[f] = hi[f] = 0;
     for (int64_t e = 0; e < nent; ++e)
       if (e < lo[ent_front[e]] || e >= hi[ent_front[e]] || (e > lo[ent_front[e]] && ent_front[e - 1] != ent_front[e])) {
-        delete P;
+        sawfem_mf_destroy(P);
         sawfem_set_error("sawfem_mf_create: entries of a front are not contiguous");
         return 1;
       }
     rc |= upload(P, &P->fent_lo, lo.data(), nf);
     rc |= upload(P, &P->fent_hi, hi.data(), nf);
-    if (rc) { delete P; return 3; }
+    if (rc) { sawfem_mf_destroy(P); return 3; }
     P->hfent_lo = lo;
     P->hfent_hi = hi;
   }
@@ -1931,14 +1932,12 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
 int sawfem_mf_destroy(void* plan) {
   Plan* P = (Plan*)plan;
   if (!P) return 0;
-  if (P->fork_ev) {
-    (void)hipDeviceSynchronize();
-    for (int k = 0; k < Plan::NSIDE; ++k) {
-      (void)hipStreamDestroy(P->side[k]);
-      (void)hipEventDestroy(P->join_ev[k]);
-    }
-    (void)hipEventDestroy(P->fork_ev);
+  if (P->fork_ev || P->side[0]) (void)hipDeviceSynchronize();
+  for (int k = 0; k < Plan::NSIDE; ++k) {
+    if (P->side[k]) (void)hipStreamDestroy(P->side[k]);
+    if (P->join_ev[k]) (void)hipEventDestroy(P->join_ev[k]);
   }
+  if (P->fork_ev) (void)hipEventDestroy(P->fork_ev);
   for (void* d : P->owned) (void)hipFree(d);
   for (int pi = 0; pi < 2; ++pi) {
     for (void* d : P->part_owned[pi]) (void)hipFree(d);
