@@ -1,5 +1,6 @@
 """The 401-point Q9 stack at its full size (16,848 elements, 202,144 complex unknowns) on a
-12-frequency subset of its band: properties that hold at any size.
+12-frequency subset of its band, and the Hex27 slab at its full size (7,920 elements, 253,334
+complex unknowns) on 5 of its 201 frequencies: properties that hold at any size.
 
 * the mixed path (complex64 fronts, FP64 refinement, paired adjoint) against the batched FP64
   path (complex128 fronts, FP64 adjoint): Y11 within 1e-6 relative, dY11/dp within 1e-5 of the
@@ -62,3 +63,27 @@ def test_full_size_mixed_matches_fp64_and_batching_changes_nothing():
     c_err = np.abs(Qm5 - Qm) / np.abs(Qm)
     _say('mixed batch 5 vs 12', c_err.max())
     assert c_err.max() <= 2e-7
+
+
+def test_full_size_hex27_slab_mixed_matches_fp64():
+    _say('[test_full_size_hex27_slab_mixed_matches_fp64]')
+    import torch
+    cell = workloads.config4()
+    eng = workloads.build_engine(cell, gradient=True)
+    f = cell.frequencies()[::50]
+    assert f.size == 5
+    _say('n', eng.n_active, 'fronts', eng.mf.nf, 'arena entries x 5', eng.mf.arena_elems * 5)
+    assert eng.n_active == 253334 and eng.mf.arena_elems * f.size > 2 ** 31
+    Qm, dYm, info = eng.sweep_charges(f, precision='mixed', batch=5, gradient=True)
+    _say('mixed', info)
+    eng.release_workspace()
+    torch.cuda.empty_cache()
+    Q64, dY64, _ = eng.sweep_charges(f, precision='fp64', batch=3, gradient=True)
+    eng.release_workspace()
+    y_err = np.abs(Qm - Q64) / np.abs(Q64)
+    scale = np.abs(cell.params)
+    scale[12:15] = 1.0
+    g_err = np.abs((dYm - dY64) * scale).max(axis=1) / np.abs(dY64 * scale).max(axis=1)
+    _say('Y11 rel err', y_err.max(), 'dY11/dp rel err', g_err.max())
+    assert y_err.max() <= 1e-6
+    assert g_err.max() <= 1e-5
