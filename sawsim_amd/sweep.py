@@ -328,8 +328,12 @@ class FrequencySolver:
         """
         torch = self.torch
         dev = self.dev
-        omegas = np.asarray(omegas, dtype=np.float64)
+        omegas = np.asarray(omegas, dtype=np.float64).ravel()
         B, n = omegas.size, self.n_active
+        if B == 0:
+            raise ValueError('solve_batch: no frequencies')
+        if precision not in ('mixed', 'fp64'):
+            raise ValueError("precision must be 'mixed' or 'fp64', not %r" % (precision,))
         prec = 0 if precision == 'mixed' else 1
         nb = nb or (64 if prec == 0 else 32)
         paired = False
@@ -557,7 +561,12 @@ class FrequencySolver:
         With ``gradient=True`` returns (Q, dY, info) where dY (P, n_params) is dY11/dp.
         ``on_chunk(done, total)`` is called after each chunk (progress and cancellation points).
         """
-        fre = np.asarray(fre, dtype=np.float64)
+        fre = np.asarray(fre, dtype=np.float64).ravel()
+        if fre.size == 0:                      # nothing to solve: empty results, no launch
+            Q = np.empty(0, dtype=complex)
+            if gradient:
+                return Q, np.empty((0, self.n_params), dtype=complex), []
+            return Q, []
         if not batch:
             nb = self.auto_batch(precision, gradient)
             batch = -(-fre.size // -(-fre.size // nb))          # equal chunks

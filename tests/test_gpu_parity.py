@@ -333,3 +333,30 @@ def test_frequency_major_and_transposed_residuals_agree():
         w = omegas[b]
         ref = rs[b] - (K + 1j * w * eng.beta * D - w * w * M) @ xs[b]
         assert np.abs(r1[b].cpu().numpy() - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_edge_cases_empty_single_and_voltage_scaling():
+    """No frequencies: empty results and no launch; one frequency: the same charge as inside a
+    batch; Q is linear in the drive voltage (V = 2 doubles every right-hand side exactly, so the
+    charges double to rounding); an unknown precision is refused."""
+    _say('[test_edge_cases_empty_single_and_voltage_scaling]')
+    cell = workloads.config3(mesh_um=0.12, points=4)
+    eng = workloads.build_engine(cell, gradient=True)
+    Q0, dY0, info0 = eng.sweep_charges([], precision='mixed', gradient=True)
+    assert Q0.shape == (0,) and dY0.shape == (0, eng.n_params) and info0 == []
+    Qe, infoe = eng.sweep_charges(np.zeros(0), precision='fp64')
+    assert Qe.shape == (0,) and infoe == []
+    f = np.linspace(1.35e9, 1.5e9, 5)
+    Q1, dY1, _ = eng.sweep_charges(f, precision='mixed', gradient=True)
+    Qs, dYs, _ = eng.sweep_charges(f[2:3], precision='mixed', gradient=True)
+    _say('single vs batch', abs(Qs[0] - Q1[2]) / abs(Q1[2]))
+    assert abs(Qs[0] - Q1[2]) <= 1e-7 * abs(Q1[2])
+    assert np.abs(dYs[0] - dY1[2]).max() <= 1e-6 * np.abs(dY1[2]).max()
+    Qv1, _ = eng.sweep_charges(f, V=1.0, precision='mixed')
+    Qv2, _ = eng.sweep_charges(f, V=2.0, precision='mixed')
+    _say('V=2 vs 2 x V=1', np.max(np.abs(Qv2 - 2 * Qv1) / np.abs(2 * Qv1)))
+    assert np.allclose(Qv2, 2 * Qv1, rtol=1e-12, atol=0)
+    with pytest.raises(ValueError):
+        eng.sweep_charges(f, precision='fp32')
+    with pytest.raises(ValueError):
+        eng.solve_batch([], precision='mixed')
