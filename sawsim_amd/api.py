@@ -52,6 +52,32 @@ _DEFAULTS = dict(points=None, start_ghz=None, stop_ghz=None, voltage=1.0, precis
                  batch=None)
 
 
+def _check_fields(c: dict):
+    """The reference's bounds on the sweep fields (``src/sawsim/config.py:36-39, 122-123``):
+    3 <= points <= 1201 (an integer), start and stop > 0 with stop > start, voltage > 0; and this
+    backend's own: precision 'mixed' or 'fp64', batch a positive integer or None."""
+    import numbers
+
+    def integer(v):
+        return isinstance(v, numbers.Integral) and not isinstance(v, bool)
+
+    def positive(v):
+        return isinstance(v, numbers.Real) and not isinstance(v, bool) and v > 0
+    if c['points'] is not None and not (integer(c['points']) and 3 <= c['points'] <= 1201):
+        raise ValueError(f"points must be an integer in [3, 1201], not {c['points']!r}")
+    for k in ('start_ghz', 'stop_ghz'):
+        if c[k] is not None and not positive(c[k]):
+            raise ValueError(f'{k} must be > 0, not {c[k]!r}')
+    if c['start_ghz'] is not None and c['stop_ghz'] is not None and c['stop_ghz'] <= c['start_ghz']:
+        raise ValueError('stop_ghz must be above start_ghz')
+    if not positive(c['voltage']):
+        raise ValueError(f"voltage must be > 0, not {c['voltage']!r}")
+    if c['precision'] not in ('mixed', 'fp64'):
+        raise ValueError(f"precision must be 'mixed' or 'fp64', not {c['precision']!r}")
+    if c['batch'] is not None and not (integer(c['batch']) and c['batch'] >= 1):
+        raise ValueError(f"batch must be a positive integer, not {c['batch']!r}")
+
+
 def _atomic_json(path: Path, data):
     tmp = path.with_suffix('.tmp')
     tmp.write_text(json.dumps(data, allow_nan=False), encoding='utf-8')
@@ -70,6 +96,7 @@ class Model:
             raise ValueError(f'unknown fields {sorted(unknown)}')
         self.model_id = model_id
         self.config = dict(_DEFAULTS, **params, model_id=model_id)
+        _check_fields(self.config)
 
     @classmethod
     def from_json(cls, path) -> 'Model':

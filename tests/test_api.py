@@ -15,6 +15,12 @@ def test_model_fields_are_checked():
         Model('nope')
     with pytest.raises(ValueError):
         Model('config1', pitch=1.0)
+    # the reference's bounds on the sweep fields
+    for bad in (dict(points=2), dict(points=1202), dict(points=5.0), dict(points=True), dict(start_ghz=0),
+                dict(start_ghz=2.0, stop_ghz=1.5), dict(voltage=-1.0), dict(precision='fp32'), dict(batch=0)):
+        with pytest.raises(ValueError):
+            Model('sp_double_layer', **bad)
+    Model('sp_double_layer', points=3, start_ghz=1.5, stop_ghz=np.float64(2.7), voltage=2, batch=4)
 
 
 def test_result_directory_round_trip(tmp_path):
