@@ -155,13 +155,16 @@ def design_set(args):
         log(f'[bench] cell {i}: {c.model.ne} elements, n={engines[-1].n_active}, {time.time() - t0:.0f}s')
     setup = time.time() - t0
     P = cells[0].points
+    from sawsim_amd.sweep import check_workspace, hbm_available
+    budget = args.arena_gb * 1e9 if args.arena_gb else hbm_available()
     batches = []
     for eng in engines:
-        per = eng.arena_bytes(1, 'mixed')
-        b = max(1, min(P, int(args.arena_gb * 1e9 // per)))
+        b = min(P, eng.auto_batch('mixed', gradient=True, budget=budget))
         b = -(-P // -(-P // b))
         batches.append(b)
     ws_bytes = max(eng.arena_bytes(b, 'mixed') for eng, b in zip(engines, batches))
+    extra = max(eng.chunk_bytes(b, 'mixed', True) - eng.arena_bytes(b, 'mixed') for eng, b in zip(engines, batches))
+    check_workspace(ws_bytes + extra, what='shared front workspace and chunk vectors')
     shared = torch.empty(ws_bytes, dtype=torch.uint8, device='cuda')
     for eng in engines:
         eng._ws = shared                       # every cell reuses the one front workspace
@@ -260,7 +263,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--workload', default='config2')
     ap.add_argument('--batch', type=int, default=0, help='frequencies per factorization chunk (0: fit --arena-gb)')
-    ap.add_argument('--arena-gb', type=float, default=200.0, help='front arena per chunk when --batch is 0')
+    ap.add_argument('--arena-gb', type=float, default=0.0,
+                    help='device memory per chunk when --batch is 0 (0: what is free above the reserve)')
     ap.add_argument('--cells', type=int, default=64, help='design-set size (config5)')
     ap.add_argument('--plain-points', type=int, default=8)
     ap.add_argument('--check-points', type=int, default=0,
@@ -309,7 +313,8 @@ def main():
     nb = 64                                   # panel width of the complex64 factorization
     work = kernel_work(eng.mf, nb)
     per_freq = eng.arena_bytes(1, 'mixed')
-    batch = args.batch or max(1, min(P, int(args.arena_gb * 1e9 // per_freq)))
+    batch = args.batch or min(P, eng.auto_batch('mixed', gradient=True,
+                                                budget=args.arena_gb * 1e9 if args.arena_gb else None))
     nchunk = -(-P // batch)
     batch = -(-P // nchunk)
     log(f'[bench] batch {batch} x {nchunk} chunks, arena {per_freq * batch / 1e9:.1f} GB')

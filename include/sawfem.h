@@ -94,6 +94,13 @@ int sawfem_mf_destroy(void* plan);
 int64_t sawfem_mf_arena_elems(void* plan);   /* complex entries of one frequency's fronts */
 int64_t sawfem_mf_uarena_elems(void* plan);  /* complex128 entries of one frequency's update vectors */
 int sawfem_mf_perturbed(void* plan, int reset);
+/* Pivot guard of sawfem_mf_factor (static pivoting with perturbation, no row exchanges): a
+ * pivot of row g with |pivot| < sqrt(u_T) * s_g(w) is replaced by sqrt(u_T) * s_g(w) with its own
+ * phase (u_T the unit roundoff of the front precision, s_g(w) = max|K_g.| + w^2 max|M_g.| +
+ * w |beta| max|D_g.| the row scale of A(w)).  Reports the pivots replaced and the smallest
+ * |pivot| / s_g since the last reset (reset != 0 clears both).  Replaces the perturbation
+ * report of the PARDISO backend (sweep.py:349-353), which SawSim does not read. */
+int sawfem_mf_pivot_stats(void* plan, int reset, int64_t* perturbed, double* min_ratio);
 
 /* Fronts of order <= max_m are factored by one fused kernel each (assembly, partial LU and
  * diagonal-block inverses with the front in LDS) and solved by one workgroup each; the others

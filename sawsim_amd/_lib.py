@@ -30,6 +30,7 @@ SIGNATURES = {
     'sawfem_mf_arena_elems': [_vp],
     'sawfem_mf_uarena_elems': [_vp],
     'sawfem_mf_perturbed': [_vp, _i],
+    'sawfem_mf_pivot_stats': [_vp, _i, _vp, _vp],
     'sawfem_mf_set_small': [_vp, _i, _i],
     'sawfem_mf_set_solve_full': [_vp, _i],
     'sawfem_mf_profile_levels': [_vp, _vp, _i],

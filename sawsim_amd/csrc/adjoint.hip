@@ -20,6 +20,91 @@ namespace {
 
 using namespace q9;
 
+// Gauss-point gradients of component c in direction d of the node values v(c, a).
+template <typename Val>
+__device__ __forceinline__ double2 grad_at(const double (*dNq)[2], int c, int d, Val v) {
+  double2 s = make_double2(0, 0);
+  for (int a = 0; a < 9; ++a) {
+    const double g = dNq[a][d];
+    const double2 x = v(c, a);
+    s.x += g * x.x; s.y += g * x.y;
+  }
+  return s;
+}
+
+// Sensitivity sums of one pass: C rows [I0, I0 + NI) of Hc and, with EP, He and Hp; written to o
+// already multiplied by the element's codes.  dval/lval give the node values of d and l.
+template <int NDU, int I0, int NI, bool EP, typename DV, typename LV>
+__device__ __forceinline__ void adj_pass(const double (*dNx)[9][2], const double* dV, const double2* code,
+                                         double scale, DV dval, LV lval, double2* o) {
+  constexpr int ND = NDU + 1;
+  constexpr int NR = 2 * NDU;
+  double2 Hc[NI][NR], He[2][NR], Hp[2][2];
+  for (int i = 0; i < NI; ++i)
+    for (int j = 0; j < NR; ++j) Hc[i][j] = make_double2(0, 0);
+  for (int i = 0; i < NR; ++i) He[0][i] = He[1][i] = make_double2(0, 0);
+  Hp[0][0] = Hp[0][1] = Hp[1][0] = Hp[1][1] = make_double2(0, 0);
+  // one Gauss point at a time (unrolled, the gradients of all nine would be live together)
+#pragma unroll 1
+  for (int q = 0; q < 9; ++q) {
+    // compiler-only fence: keeps LDS-held node values from being hoisted out of the loop into
+    // registers (that would bring back the register demand the LDS copy avoids)
+    asm volatile("" ::: "memory");
+    double2 gd[ND][2], gl[ND][2];
+    for (int c = 0; c < ND; ++c)
+      for (int d = 0; d < 2; ++d) {
+        gd[c][d] = grad_at(dNx[q], c, d, dval);
+        gl[c][d] = grad_at(dNx[q], c, d, lval);
+      }
+    const double w = dV[q];
+    for (int ii = 0; ii < NI; ++ii) {
+      const int i = I0 + ii;
+      const double2 li = gl[i / 2][i % 2];
+      for (int j = 0; j < NR; ++j) {
+        const double2 pr = cmul(li, gd[j / 2][j % 2]);
+        Hc[ii][j].x += w * pr.x; Hc[ii][j].y += w * pr.y;
+      }
+    }
+    if (EP) {
+      for (int i = 0; i < NR; ++i) {
+        const double2 li = gl[i / 2][i % 2];
+        const double2 di = gd[i / 2][i % 2];
+        for (int k = 0; k < 2; ++k) {
+          const double2 a1 = cmul(li, gd[NDU][k]);
+          const double2 a2 = cmul(gl[NDU][k], di);
+          He[k][i].x += w * (a1.x + scale * a2.x);
+          He[k][i].y += w * (a1.y + scale * a2.y);
+        }
+      }
+      for (int j = 0; j < 2; ++j)
+        for (int k = 0; k < 2; ++k) {
+          const double2 pr = cmul(gl[NDU][j], gd[NDU][k]);
+          Hp[j][k].x += w * pr.x; Hp[j][k].y += w * pr.y;
+        }
+    }
+  }
+  for (int ii = 0; ii < NI; ++ii)
+    for (int j = 0; j < NR; ++j) {
+      const int i = I0 + ii;
+      const int ri = kStrainRow[i / 2][i % 2], rj = kStrainRow[j / 2][j % 2];
+      o[i * NR + j] = cmul(code[kT9Code[ri][rj]], Hc[ii][j]);
+    }
+  if (EP) {
+    for (int k = 0; k < 2; ++k)
+      for (int i = 0; i < NR; ++i) {
+        const int ri = kStrainRow[i / 2][i % 2];
+        o[NR * NR + k * NR + i] = cmul(code[kT9Code[k][ri]], He[k][i]);
+      }
+    for (int j = 0; j < 2; ++j)
+      for (int k = 0; k < 2; ++k) o[NR * NR + 2 * NR + j * 2 + k] = cmul(code[kTepsCode[j][k]], Hp[j][k]);
+  }
+}
+
+// One workgroup of 64 threads per element and 64 frequencies (thread = frequency).
+// NDU = 2 keeps the node values of d and l in registers and forms every sum in one pass.
+// NDU = 3 has 4/3 more node values and 9/4 more C sums than fit the register file, so its node
+// values sit in (dynamic) LDS, [2][ND * 9][64], each thread reading only its own column, and
+// the C sums are formed in two passes of three rows: no scratch memory in either case.
 template <int NDU>
 __global__ void __launch_bounds__(64) k_adj_elem(int ne0, const int* __restrict__ elems, const int* __restrict__ conn,
                                                  const double* __restrict__ xy, const int* __restrict__ elem_tag,
@@ -38,7 +123,9 @@ __global__ void __launch_bounds__(64) k_adj_elem(int ne0, const int* __restrict_
   __shared__ double Nq[9][9];
   __shared__ double dV[9];
   __shared__ double2 al[3];
+  __shared__ double2 code[6];
   __shared__ int dofs[9 * ND];
+  extern __shared__ double2 snode[];
   if (t < 9) {
     const int nd = conn[e * 9 + t];
     sx[t] = xy[2 * nd];
@@ -47,77 +134,40 @@ __global__ void __launch_bounds__(64) k_adj_elem(int ne0, const int* __restrict_
   }
   __syncthreads();
   if (t < 9) gauss_point(t, sx, sy, Nq[t], dNx[t], &dV[t]);
-  if (t == 32) pml_alpha(pmlset, npml, elem_tag[e], sx[8], sy[8], omega_top, al);
+  if (t == 32) {
+    pml_alpha(pmlset, npml, elem_tag[e], sx[8], sy[8], omega_top, al);
+    pml_codes(al, code);
+  }
   __syncthreads();
   const int b = blockIdx.y * 64 + t;
   if (b >= B) return;
-  double2 dvl[ND][9], lvl[ND][9];
-  for (int c = 0; c < ND; ++c)
-    for (int a = 0; a < 9; ++a) {
-      const int dof = dofs[c * 9 + a];
-      const int p = pos_rep[dof];
-      const bool z = zmask[dof] != 0;
-      double2 xv = p >= 0 ? x[(int64_t)b * n + p] : make_double2(0, 0);
-      double2 lv = p >= 0 ? lam[(int64_t)b * n + p] : make_double2(0, 0);
-      dvl[c][a] = z ? make_double2(V, 0) : xv;
-      lvl[c][a] = make_double2((z ? 1.0 : 0.0) - lv.x, -lv.y);
-    }
-  double2 Hc[NR][NR], He[2][NR], Hp[2][2];
-  for (int i = 0; i < NR; ++i) {
-    for (int j = 0; j < NR; ++j) Hc[i][j] = make_double2(0, 0);
-    He[0][i] = He[1][i] = make_double2(0, 0);
-  }
-  Hp[0][0] = Hp[0][1] = Hp[1][0] = Hp[1][1] = make_double2(0, 0);
-  for (int q = 0; q < 9; ++q) {
-    double2 gd[ND][2], gl[ND][2];
-    for (int c = 0; c < ND; ++c)
-      for (int d = 0; d < 2; ++d) {
-        double2 sd = make_double2(0, 0), sl = make_double2(0, 0);
-        for (int a = 0; a < 9; ++a) {
-          const double g = dNx[q][a][d];
-          sd.x += g * dvl[c][a].x; sd.y += g * dvl[c][a].y;
-          sl.x += g * lvl[c][a].x; sl.y += g * lvl[c][a].y;
-        }
-        gd[c][d] = sd;
-        gl[c][d] = sl;
-      }
-    const double w = dV[q];
-    for (int i = 0; i < NR; ++i) {
-      const double2 li = gl[i / 2][i % 2];
-      const double2 di = gd[i / 2][i % 2];
-      for (int j = 0; j < NR; ++j) {
-        const double2 pr = cmul(li, gd[j / 2][j % 2]);
-        Hc[i][j].x += w * pr.x; Hc[i][j].y += w * pr.y;
-      }
-      for (int k = 0; k < 2; ++k) {
-        const double2 a1 = cmul(li, gd[NDU][k]);
-        const double2 a2 = cmul(gl[NDU][k], di);
-        He[k][i].x += w * (a1.x + scale * a2.x);
-        He[k][i].y += w * (a1.y + scale * a2.y);
-      }
-    }
-    for (int j = 0; j < 2; ++j)
-      for (int k = 0; k < 2; ++k) {
-        const double2 pr = cmul(gl[NDU][j], gd[NDU][k]);
-        Hp[j][k].x += w * pr.x; Hp[j][k].y += w * pr.y;
-      }
-  }
-  double2 alc[3] = {al[0], al[1], al[2]};
-  double2 code[6];
-  pml_codes(alc, code);
   double2* o = out + ((int64_t)b * ne0 + ei) * NT;
-  for (int i = 0; i < NR; ++i)
-    for (int j = 0; j < NR; ++j) {
-      const int ri = kStrainRow[i / 2][i % 2], rj = kStrainRow[j / 2][j % 2];
-      o[i * NR + j] = cmul(code[kT9Code[ri][rj]], Hc[i][j]);
-    }
-  for (int k = 0; k < 2; ++k)
-    for (int i = 0; i < NR; ++i) {
-      const int ri = kStrainRow[i / 2][i % 2];
-      o[NR * NR + k * NR + i] = cmul(code[kT9Code[k][ri]], He[k][i]);
-    }
-  for (int j = 0; j < 2; ++j)
-    for (int k = 0; k < 2; ++k) o[NR * NR + 2 * NR + j * 2 + k] = cmul(code[kTepsCode[j][k]], Hp[j][k]);
+  auto node = [&](int c, int a, double2& dv, double2& lv) {
+    const int dof = dofs[c * 9 + a];
+    const int p = pos_rep[dof];
+    const bool z = zmask[dof] != 0;
+    const double2 xv = p >= 0 ? x[(int64_t)b * n + p] : make_double2(0, 0);
+    const double2 l = p >= 0 ? lam[(int64_t)b * n + p] : make_double2(0, 0);
+    dv = z ? make_double2(V, 0) : xv;
+    lv = make_double2((z ? 1.0 : 0.0) - l.x, -l.y);
+  };
+  if constexpr (NDU == 2) {
+    double2 dvl[ND][9], lvl[ND][9];
+    for (int c = 0; c < ND; ++c)
+      for (int a = 0; a < 9; ++a) node(c, a, dvl[c][a], lvl[c][a]);
+    auto dval = [&](int c, int a) { return dvl[c][a]; };
+    auto lval = [&](int c, int a) { return lvl[c][a]; };
+    adj_pass<NDU, 0, NR, true>(dNx, dV, code, scale, dval, lval, o);
+  } else {
+    double2* sd = snode;                  // [ND * 9][64]
+    double2* sl = snode + ND * 9 * 64;
+    for (int c = 0; c < ND; ++c)
+      for (int a = 0; a < 9; ++a) node(c, a, sd[(c * 9 + a) * 64 + t], sl[(c * 9 + a) * 64 + t]);
+    auto dval = [&](int c, int a) { return sd[(c * 9 + a) * 64 + t]; };
+    auto lval = [&](int c, int a) { return sl[(c * 9 + a) * 64 + t]; };
+    adj_pass<NDU, 0, NR / 2, true>(dNx, dV, code, scale, dval, lval, o);
+    adj_pass<NDU, NR / 2, NR / 2, false>(dNx, dV, code, scale, dval, lval, o);
+  }
 }
 
 // W[b][t] = sum_e out[b][e][t], elements in list order (deterministic).
@@ -157,9 +207,13 @@ int sawfem_adjoint_elem(void* stream, int ne0, const int* elems, const int* conn
     if (ndu == 2)
       hipLaunchKernelGGL(k_adj_elem<2>, grid, dim3(64), 0, s, ne0, elems, conn, xy, elem_tag, npml, ps, omega_top, N,
                          pos_rep, zmask, V, scale, B, n, (const double2*)x, (const double2*)lam, (double2*)work);
-    else
-      hipLaunchKernelGGL(k_adj_elem<3>, grid, dim3(64), 0, s, ne0, elems, conn, xy, elem_tag, npml, ps, omega_top, N,
+    else {
+      // node values of d and l for 64 frequencies: 2 * 36 * 64 complex128 = 72 KB of LDS
+      const int lds = 2 * 4 * 9 * 64 * (int)sizeof(double2);
+      SAW_HIP(hipFuncSetAttribute((const void*)k_adj_elem<3>, hipFuncAttributeMaxDynamicSharedMemorySize, lds));
+      hipLaunchKernelGGL(k_adj_elem<3>, grid, dim3(64), lds, s, ne0, elems, conn, xy, elem_tag, npml, ps, omega_top, N,
                          pos_rep, zmask, V, scale, B, n, (const double2*)x, (const double2*)lam, (double2*)work);
+    }
     SAW_HIP(hipGetLastError());
   }
   const int64_t tot = (int64_t)B * NT;

@@ -31,6 +31,9 @@ __global__ void __launch_bounds__(128) k_elem_q9(
   __shared__ double Nq[9][9];       // [q][a]
   __shared__ double dV[9];
   __shared__ double2 al[3];
+  // multiplier codes in LDS: they are indexed by table lookups, which a private array would
+  // have to serve from scratch memory
+  __shared__ double2 code[6];
   if (t < 9) {
     const int nd = conn[e * 9 + t];
     sx[t] = xy[2 * nd];
@@ -38,7 +41,10 @@ __global__ void __launch_bounds__(128) k_elem_q9(
   }
   __syncthreads();
   if (t < 9) gauss_point(t, sx, sy, Nq[t], dNx[t], &dV[t]);
-  if (t == 32) pml_alpha(pmlset, npml, elem_tag[e], sx[8], sy[8], omega_top, al);
+  if (t == 32) {
+    pml_alpha(pmlset, npml, elem_tag[e], sx[8], sy[8], omega_top, al);
+    pml_codes(al, code);
+  }
   __syncthreads();
   if (t >= 81) return;
   const int a = t / 9, b = t % 9;
@@ -51,9 +57,6 @@ __global__ void __launch_bounds__(128) k_elem_q9(
     G[1][1] += v * dNx[q][a][1] * dNx[q][b][1];
     GM += v * Nq[q][a] * Nq[q][b];
   }
-  double2 alc[3] = {al[0], al[1], al[2]};
-  double2 code[6];
-  pml_codes(alc, code);
   const int mat = mat_idx[e];
   const double2* Cm = C9 + mat * 81;
   const double2* Em = E9 + mat * 27;
@@ -98,7 +101,7 @@ __global__ void __launch_bounds__(128) k_elem_q9(
       }
     Kout[(ndu * nd + ndu) * 81 + t] = make_double2(-pp.x, -pp.y);
   }
-  const double2 a123 = cmul(cmul(alc[0], alc[1]), alc[2]);
+  const double2 a123 = cmul(cmul(al[0], al[1]), al[2]);
   const double rs = RHO[mat] * GM * scale;
   Me[(size_t)e * 81 + t] = make_double2(a123.x * rs, a123.y * rs);
 }

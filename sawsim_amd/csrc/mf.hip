@@ -276,6 +276,81 @@ __global__ void __launch_bounds__(256) k_leaf_front(FrontsDev F, const int* __re
   }
 }
 
+// ---------------------------------------------------------------- static pivoting
+// Pivot guard (static pivoting with perturbation, Li & Demmel, SuperLU_DIST): no row exchanges;
+// a pivot whose magnitude is below tau = sqrt(u_T) * s_g(w) (u_T the unit roundoff of the front
+// precision) is replaced by tau with the pivot's own phase (tau itself for a zero or non-finite
+// pivot) and counted.  sqrt(u_T) balances the perturbation against the element growth it
+// allows (a smaller tau leaves factors whose rounding errors exceed the perturbation).  s_g(w) = |K|_g + w^2 |M|_g +
+// w beta |D|_g bounds the largest entry of row g of A(w) (|X|_g: the largest magnitude in row g
+// of X, computed per factorization by k_row_scale).  The scale is per row because the rows of
+// this system span many orders of magnitude (displacement rows carry the stiffness scaling,
+// potential rows the permittivities): a matrix- or front-wide norm would flag every potential
+// pivot.  Refinement against the FP64 operator (or the complex128 refactorization) absorbs
+// the perturbation.
+struct PivotGuard {
+  const double* rs;          // [3][n]: row maxima of |K|, |M|, |D| (D: zero when absent)
+  const double* omega;
+  double beta, eps;
+  int n;
+  int* npert;                // replaced pivots
+  unsigned* pmin;            // smallest |pivot| / s_g seen, as float bits (non-negative floats
+                             // order like their bit patterns)
+};
+
+__device__ __forceinline__ double row_scale_at(const PivotGuard& G, int g, double w) {
+  return G.rs[g] + w * w * G.rs[G.n + g] + w * fabs(G.beta) * G.rs[2 * G.n + g];
+}
+
+// Returns the pivot to use; perturbed is set when it differs from piv.
+template <typename V>
+__device__ __forceinline__ V guard_pivot(V piv, double scale, double eps, double* ratio, bool* perturbed) {
+  const double mag = sqrt((double)piv.x * piv.x + (double)piv.y * piv.y);
+  const double tau = eps * scale;
+  *ratio = scale > 0 ? mag / scale : 0.0;
+  *perturbed = !(mag >= tau) || !(mag < 1e300);
+  if (!*perturbed) return piv;
+  if (mag > 0 && mag < 1e300) {
+    const double f = tau / mag;
+    V r; r.x = (decltype(piv.x))(piv.x * f); r.y = (decltype(piv.y))(piv.y * f); return r;
+  }
+  V r; r.x = (decltype(piv.x))tau; r.y = 0; return r;
+}
+
+__device__ __forceinline__ void note_pivots(const PivotGuard& G, double rmin, int nrep) {
+  if (nrep) atomicAdd(G.npert, nrep);
+  const float r = (float)rmin;
+  // most blocks find a smaller minimum already recorded and skip the atomic
+  if (r < __uint_as_float(*(volatile unsigned*)G.pmin)) atomicMin(G.pmin, __float_as_uint(r));
+}
+
+__global__ void k_fill_f64(int64_t n, double* __restrict__ x, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+// Row maxima of |K|, |M|, |D| over the A_aa entries (each entry is listed once, with the
+// front and the local position that give its row); max of non-negative doubles through
+// their bit patterns, so the result does not depend on the order of the atomics.
+__global__ void k_row_scale(int64_t nent, FrontsDev F, const int* __restrict__ ent_front,
+                            const int64_t* __restrict__ ent_slot, const int* __restrict__ ent_loc,
+                            const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
+                            const double2* __restrict__ Daa, int n, double* __restrict__ rs) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nent) return;
+  const int f = ent_front[e];
+  const int m = F.m[f];
+  const int g = F.rows[F.rows_ptr[f] + ent_loc[e] % m];
+  const int64_t sl = ent_slot[e];
+  auto put = [&](double* dst, double2 v) {
+    const double a = sqrt(v.x * v.x + v.y * v.y);
+    if (a > 0) atomicMax((unsigned long long*)dst, (unsigned long long)__double_as_longlong(a));
+  };
+  put(rs + g, Kaa[sl]);
+  put(rs + n + g, Maa[sl]);
+  if (Daa) put(rs + 2 * n + g, Daa[sl]);
+}
+
 // ---------------------------------------------------------------- panel factorization
 // In-place inversion of the unit-lower and upper triangles of an n x n block D (column-major,
 // leading dimension ld, in LDS) by row operations (Gauss-Jordan):
@@ -327,8 +402,8 @@ __device__ void invert_lu_block(V* D, int ld, int n) {
   __syncthreads();
 }
 
-// LU of the kb x kb diagonal block of pivot panel [k0, k0 + kb) (no pivoting; a pivot below
-// `tiny` is replaced by `tiny` and counted), then the inverses of its unit-lower and upper
+// LU of the kb x kb diagonal block of pivot panel [k0, k0 + kb) (no pivoting; small pivots
+// perturbed by the pivot guard), then the inverses of its unit-lower and upper
 // factors stored IN PLACE of the factors: the strictly lower triangle of the block holds
 // L_D^{-1} (unit diagonal implied) and the upper triangle U_D^{-1}.  Neither the panel solves
 // (GEMMs with these inverses) nor the triangular solves need L_D, U_D themselves.  Everything
@@ -336,10 +411,11 @@ __device__ void invert_lu_block(V* D, int ld, int n) {
 template <typename T>
 __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ lvl, int B,
                                           typename cplx<T>::type* __restrict__ arena, int k0, int nb,
-                                          T tiny, int* __restrict__ npert, int b) {
+                                          const PivotGuard& PG, int b) {
   using V = typename cplx<T>::type;
   extern __shared__ unsigned char smem_raw[];
   V* Ds = reinterpret_cast<V*>(smem_raw);   // [nb][nb+1], column-major with padding
+  double* rsc = reinterpret_cast<double*>(Ds + nb * (nb + 1));   // [nb] row scales of the pivots
   const int f = lvl[blockIdx.x];
   const int p = F.p[f];
   if (k0 >= p) return;
@@ -355,14 +431,20 @@ __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ l
   const int ic = min(i, kb - 1);
   batched<8>(tc, kb, ncg, [&](int j) { return G[ic + (int64_t)j * m]; },
              [&](int j, V v) { if (i < kb) Ds[i + j * ld] = v; });
+  if (tc == 0 && i < kb) rsc[i] = row_scale_at(PG, F.rows[F.rows_ptr[f] + k0 + i], PG.omega[b]);
   __syncthreads();
   // right-looking LU: multipliers of column k, then the rank-1 update of the trailing block
+  double rmin = 3.0e38;
+  int nrep = 0;
   for (int k = 0; k < kb; ++k) {
-    V piv = Ds[k + k * ld];
-    const T mag = piv.x * piv.x + piv.y * piv.y;
-    if (!(mag > tiny * tiny)) {
-      piv.x = tiny; piv.y = 0;
-      if (t == 0) { Ds[k + k * ld] = piv; atomicAdd(npert, 1); }
+    double ratio;
+    bool pert;
+    const V piv = guard_pivot(Ds[k + k * ld], rsc[k], PG.eps, &ratio, &pert);
+    rmin = fmin(rmin, ratio);
+    if (pert) {
+      ++nrep;
+      __syncthreads();           // every thread has read the old pivot
+      if (t == 0) Ds[k + k * ld] = piv;
     }
     if (tc == 0 && i > k && i < kb) Ds[i + k * ld] = cdiv(Ds[i + k * ld], piv);
     __syncthreads();
@@ -372,6 +454,7 @@ __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ l
     }
     __syncthreads();
   }
+  if (t == 0) note_pivots(PG, rmin, nrep);
   invert_lu_block(Ds, ld, kb);
   if (i < kb)
     for (int j = tc; j < kb; j += ncg) G[i + (int64_t)j * m] = Ds[i + j * ld];
@@ -380,9 +463,8 @@ __device__ __forceinline__ void diag_body(FrontsDev F, const int* __restrict__ l
 template <typename T>
 __global__ void __launch_bounds__(512) k_diag(FrontsDev F, const int* __restrict__ lvl, int B,
                                               typename cplx<T>::type* __restrict__ arena, int k0, int nb,
-                                              T tiny, int* __restrict__ npert) {
-  diag_body<T>(F, lvl, B, arena, k0, nb, tiny, npert, blockIdx.y);
-
+                                              const PivotGuard PG) {
+  diag_body<T>(F, lvl, B, arena, k0, nb, PG, blockIdx.y);
 }
 
 // ---------------------------------------------------------------- small fronts, fused
@@ -392,7 +474,7 @@ __global__ void __launch_bounds__(512) k_diag(FrontsDev F, const int* __restrict
 //     child-CB -> parent-row map rebuilt in LDS, children one after the other), then the
 //     A = K - w^2 M + i w beta D entries (FP64, rounded to T once);
 //  2. partial LU of the p pivots, right-looking with rank-1 updates of the whole trailing
-//     front (the Schur complement included); tiny pivots as in k_diag;
+//     front (the Schur complement included); pivot guard as in k_diag;
 //  3. write-back of everything but the nbl x nbl diagonal blocks of the pivot triangle;
 //     those are inverted in place (L_D^{-1} strictly below, U_D^{-1} on/above the diagonal,
 //     the storage the panel path and the solves use) and written last.
@@ -404,8 +486,8 @@ __global__ void __launch_bounds__(SMALL_NT) k_small_front(
     FrontsDev F, const int* __restrict__ lvl, int B, const double* __restrict__ omega, double beta,
     const int64_t* __restrict__ fent_lo, const int64_t* __restrict__ fent_hi, const int64_t* __restrict__ ent_slot,
     const int* __restrict__ ent_loc, const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
-    const double2* __restrict__ Daa, typename cplx<T>::type* __restrict__ arena, int nbl, T tiny,
-    int* __restrict__ npert) {
+    const double2* __restrict__ Daa, typename cplx<T>::type* __restrict__ arena, int nbl,
+    const PivotGuard PG) {
   using V = typename cplx<T>::type;
   extern __shared__ unsigned char smem_raw[];
   const int f = lvl[blockIdx.x];
@@ -459,12 +541,19 @@ __global__ void __launch_bounds__(SMALL_NT) k_small_front(
   //    the stores), lane owns rows k+1+lane and k+65+lane
   const int lane = t & 63, wave = t >> 6;
   constexpr int NW = SMALL_NT / 64;
+  const double wb = omega[b];
+  double rmin = 3.0e38;
+  int nrep = 0;
   for (int k = 0; k < p; ++k) {
-    V piv = S[k + k * ld];
-    const T mag = piv.x * piv.x + piv.y * piv.y;
-    if (!(mag > tiny * tiny)) {
-      piv.x = tiny; piv.y = 0;
-      if (t == 0) { S[k + k * ld] = piv; atomicAdd(npert, 1); }
+    double ratio;
+    bool pert;
+    const V piv = guard_pivot(S[k + k * ld], row_scale_at(PG, F.rows[F.rows_ptr[f] + k], wb), PG.eps, &ratio,
+                              &pert);
+    rmin = fmin(rmin, ratio);
+    if (pert) {
+      ++nrep;
+      __syncthreads();           // every thread has read the old pivot
+      if (t == 0) S[k + k * ld] = piv;
     }
     const V one{1, 0};
     const V ipiv = cdiv(one, piv);
@@ -495,6 +584,7 @@ __global__ void __launch_bounds__(SMALL_NT) k_small_front(
     }
     __syncthreads();
   }
+  if (t == 0) note_pivots(PG, rmin, nrep);
   // 3. write-back (diagonal blocks of the pivot triangle last)
   V* G = front_ptr<T>(arena, F, f, b, B);
   for (int idx = t; idx < m * m; idx += SMALL_NT) {
@@ -1314,6 +1404,9 @@ struct Plan {
   int64_t* ent_slot = nullptr;
   int* ent_loc = nullptr;
   int* npert = nullptr;
+  unsigned* pmin = nullptr;      // smallest |pivot| / row scale since the last reset (float bits)
+  double* rscale = nullptr;      // [3][n] row maxima of |K|, |M|, |D| of the last factorization
+  int64_t nent = 0;
   int mmax = 0;
   int pmax = 0;
   // fused small-front path: per-front entry ranges, per-precision front flags and the
@@ -1585,12 +1678,19 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
                 const void* Daa, void* arena_v, int nb) {
   using V = typename cplx<T>::type;
   V* arena = (V*)arena_v;
-  const T tiny = sizeof(T) == 4 ? (T)1e-30 : (T)1e-300;
   const int pi = sizeof(T) == 4 ? 0 : 1;
   if (P->panel_nb[pi] != nb) {
     const int rc = build_panel_lists(P, pi, nb);
     if (rc) return rc;
   }
+  // pivot guard: row scales of this operator, threshold sqrt(u_T) times the row scale
+  const int64_t n3 = 3 * (int64_t)P->n;
+  hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, n3, P->rscale, 0.0);
+  if (P->nent > 0)
+    hipLaunchKernelGGL(k_row_scale, dim3((unsigned)((P->nent + 255) / 256)), dim3(256), 0, s, P->nent, P->F,
+                       P->ent_front, P->ent_slot, P->ent_loc, (const double2*)Kaa, (const double2*)Maa,
+                       (const double2*)Daa, P->n, P->rscale);
+  const PivotGuard PG{P->rscale, omega, beta, sizeof(T) == 4 ? 0x1p-12 : 0x1.6a09e667f3bcdp-27, P->n, P->npert, P->pmin};
   for (size_t li = 0; li < P->levels.size(); ++li) {
     const Level& L = P->levels[li];
     P->cur_level = (int)li;
@@ -1601,7 +1701,7 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
       Timed tm(P, s, K_SMALL);
       hipLaunchKernelGGL(k_small_front<T>, dim3(S.nf, B), dim3(SMALL_NT), lds, s, P->F, S.fronts, B, omega, beta,
                          P->fent_lo, P->fent_hi, P->ent_slot, P->ent_loc, (const double2*)Kaa, (const double2*)Maa,
-                         (const double2*)Daa, arena, nbl, tiny, P->npert);
+                         (const double2*)Daa, arena, nbl, PG);
     }
     if (L.big[pi].empty()) continue;
     for (const Sub& G : L.big[pi]) {
@@ -1662,14 +1762,14 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
                                k0, nbl);
           }
         }
-        const size_t lds_diag = sizeof(V) * nbl * (nbl + 1);
+        const size_t lds_diag = sizeof(V) * nbl * (nbl + 1) + sizeof(double) * nbl;
         // 32-wide blocks: one wave per block (its barriers cost next to nothing and more blocks
         // share a CU); 64-wide blocks (levels of few large fronts, few workgroups): eight waves
         // share each step's updates (config 2: 7.2 -> 3.1 ms per top level)
         if (PT.ndiag > 0) {
           Timed tm(P, sg, K_DIAG);
           hipLaunchKernelGGL(k_diag<T>, dim3(PT.ndiag, B), dim3(nbl <= 32 ? 64 : 512), lds_diag, sg, P->F, PT.diag, B,
-                             arena, k0, nbl, tiny, P->npert);
+                             arena, k0, nbl, PG);
         }
         if (PT.n23 > 0) {
           Timed tm(P, sg, K_TRSM);
@@ -1853,6 +1953,13 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
   rc |= upload(P, &P->ent_loc, ent_loc, nent);
   int zero = 0;
   rc |= upload(P, &P->npert, &zero, 1);
+  const unsigned big = 0x7f7fffffu;   // FLT_MAX
+  rc |= upload(P, &P->pmin, &big, 1);
+  {
+    std::vector<double> rs0(3 * (size_t)n, 0.0);
+    rc |= upload(P, &P->rscale, rs0.data(), 3 * (int64_t)n);
+  }
+  P->nent = nent;
   if (rc) { sawfem_mf_destroy(P); return 3; }
   // per-front entry ranges (entries are grouped by front within each level)
   {
@@ -1974,7 +2081,7 @@ int sawfem_mf_set_solve_full(void* plan, int max_m) {
 int64_t sawfem_mf_arena_elems(void* plan) { return plan ? ((Plan*)plan)->arena_elems : -1; }
 int64_t sawfem_mf_uarena_elems(void* plan) { return plan ? ((Plan*)plan)->uarena_elems : -1; }
 
-// Number of tiny pivots replaced since the plan was created (or the counter last reset).
+// Number of pivots the guard replaced since the plan was created (or the counter last reset).
 int sawfem_mf_perturbed(void* plan, int reset) {
   Plan* P = (Plan*)plan;
   int h = 0;
@@ -1982,6 +2089,27 @@ int sawfem_mf_perturbed(void* plan, int reset) {
   if (hipMemcpy(&h, P->npert, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -1;
   if (reset) { int z = 0; (void)hipMemcpy(P->npert, &z, sizeof(int), hipMemcpyHostToDevice); }
   return h;
+}
+
+// Pivot statistics since the last reset: pivots replaced and the smallest |pivot| / row scale.
+int sawfem_mf_pivot_stats(void* plan, int reset, int64_t* perturbed, double* min_ratio) {
+  Plan* P = (Plan*)plan;
+  if (!P || !perturbed || !min_ratio) { sawfem_set_error("sawfem_mf_pivot_stats: invalid arguments"); return 1; }
+  int h = 0;
+  unsigned u = 0;
+  SAW_HIP(hipMemcpy(&h, P->npert, sizeof(int), hipMemcpyDeviceToHost));
+  SAW_HIP(hipMemcpy(&u, P->pmin, sizeof(unsigned), hipMemcpyDeviceToHost));
+  float r;
+  std::memcpy(&r, &u, sizeof(r));
+  *perturbed = h;
+  *min_ratio = r;
+  if (reset) {
+    const int z = 0;
+    const unsigned big = 0x7f7fffffu;
+    SAW_HIP(hipMemcpy(P->npert, &z, sizeof(int), hipMemcpyHostToDevice));
+    SAW_HIP(hipMemcpy(P->pmin, &big, sizeof(unsigned), hipMemcpyHostToDevice));
+  }
+  return 0;
 }
 
 // prec: 0 = complex64 fronts (mixed path), 1 = complex128 fronts (plain path).

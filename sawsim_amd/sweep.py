@@ -43,6 +43,27 @@ from .symbolic import analyse
 # streams: 256 is the best of 128/192/256/320/448/640)
 SOLVE_FULL_M = 256
 
+# device memory every workspace request leaves free for the HIP runtime (queues, code objects,
+# kernel scratch backing) and the caller's own tensors
+HBM_RESERVE_BYTES = 12 * 10 ** 9
+
+
+def hbm_available(keep=0):
+    """Device bytes a new workspace may take: free memory, plus what torch caches unused, plus
+    ``keep`` (a workspace that will be released for the new one), minus ``HBM_RESERVE_BYTES``."""
+    torch = _torch()
+    free, _ = torch.cuda.mem_get_info()
+    cached = torch.cuda.memory_reserved() - torch.cuda.memory_allocated()
+    return int(free + cached + keep - HBM_RESERVE_BYTES)
+
+
+def check_workspace(nbytes, keep=0, what='front workspace'):
+    """Raise ValueError (before anything is launched) when ``nbytes`` would break the reserve."""
+    avail = hbm_available(keep)
+    if nbytes > avail:
+        raise ValueError(f'{what} of {nbytes / 1e9:.1f} GB exceeds the {max(avail, 0) / 1e9:.1f} GB of device memory '
+                         f'available above the {HBM_RESERVE_BYTES / 1e9:.0f} GB reserve; use fewer frequencies per chunk')
+
 
 def ir_error(dq, dq_prev):
     """Estimated relative error of a refined quantity after its latest correction.
@@ -304,11 +325,19 @@ class FrequencySolver:
         nbytes = count * (8 if dtype == torch.complex64 else 16)
         ws = getattr(self, '_ws', None)
         if ws is None or ws.numel() < nbytes:
+            check_workspace(nbytes, keep=0 if ws is None else ws.numel())
             self._ws = None
             del ws
             torch.cuda.empty_cache()
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
         return self._ws[:nbytes].view(dtype)
+
+    def chunk_bytes(self, batch, precision='mixed', gradient=False):
+        """Device bytes one chunk of ``batch`` frequencies takes: front arena, update vectors
+        (two right-hand sides with the gradient) and about a dozen length-n vectors."""
+        R = 2 if gradient else 1
+        per = self.arena_bytes(1, precision) + 16 * self.mf.uarena_elems * R + 16 * 12 * R * self.n_active
+        return per * batch
 
     def release_workspace(self):
         """Free the arena workspace (it is otherwise kept for the next solve_batch)."""
@@ -358,6 +387,7 @@ class FrequencySolver:
             if gradient:
                 out.update(self._adjoint(om, omegas, V, prec, nb, arena, uarena, rhs, out['x'], ir_tol, max_ir))
         x, err = out['x'], out['err']
+        out['perturbed'], out['pivot_min_ratio'] = self.pivot_stats(reset=True)
         refactored = np.zeros(B, dtype=bool)
         if prec == 0 and fallback:
             bad = np.flatnonzero(~(err <= ir_tol) | (~(out.get('adj_err', err) <= ir_tol) if gradient else False))
@@ -371,13 +401,24 @@ class FrequencySolver:
                                            gradient=gradient, adjoint=adjoint)
                     tb = self.torch.as_tensor(part, device=self.dev)
                     x[tb] = sub['x']
+                    if 'lam' in out:
+                        out['lam'][tb] = sub['lam']
                     for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err'):
                         if key in out:
                             out[key][part] = sub[key]
+                    out['perturbed_fp64'] = out.get('perturbed_fp64', 0) + sub['perturbed']
                     del sub
                 refactored[bad] = True
         out['refactored'] = refactored
         return out
+
+    def pivot_stats(self, reset=True):
+        """(pivots the guard replaced, smallest |pivot| / row scale) since the last reset
+        (``sawfem_mf_pivot_stats``)."""
+        import ctypes as C
+        cnt, rmin = C.c_int64(), C.c_double()
+        _lib.call('sawfem_mf_pivot_stats', self._plan, int(reset), C.byref(cnt), C.byref(rmin))
+        return int(cnt.value), float(rmin.value)
 
     def _refine(self, prec, nb, arena, uarena, X, transpose, res, rhst, tslot, measure, ir_tol, max_ir):
         """Iterative refinement of X ([B][R][n], holding the first solve) in place.
@@ -544,15 +585,18 @@ class FrequencySolver:
         Wh = W.cpu().numpy().reshape(B, NT)
         return dict(dY=self._param_gradient(Wh, omegas, V))
 
-    def auto_batch(self, precision='mixed', gradient=False, fraction=0.6):
-        """Frequencies per chunk that fit ``fraction`` of the free HBM: front arena, update
-        vectors (two right-hand sides with the gradient) and about a dozen length-n vectors."""
-        free, _ = self.torch.cuda.mem_get_info()
-        ws = getattr(self, '_ws', None)
-        free += 0 if ws is None else ws.numel()
-        R = 2 if gradient else 1
-        per = self.arena_bytes(1, precision) + 16 * self.mf.uarena_elems * R + 16 * 12 * R * self.n_active
-        return max(1, int(fraction * free // per))
+    def auto_batch(self, precision='mixed', gradient=False, budget=None):
+        """Frequencies per chunk (:meth:`chunk_bytes`) that fit ``budget`` bytes, by default the
+        device memory available above the reserve (:func:`hbm_available`, counting the current
+        workspace as free)."""
+        if budget is None:
+            ws = getattr(self, '_ws', None)
+            budget = hbm_available(0 if ws is None else ws.numel())
+        per = self.chunk_bytes(1, precision, gradient)
+        if budget < per:
+            raise ValueError(f'one frequency needs {per / 1e9:.1f} GB of device memory; '
+                             f'{max(budget, 0) / 1e9:.1f} GB are available above the reserve')
+        return int(budget // per)
 
     def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, on_chunk=None, **kw):
         """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies (default: as many
@@ -577,7 +621,8 @@ class FrequencySolver:
             res = self.solve_batch(2 * np.pi * fre[i0:i0 + batch], V, precision=precision, gradient=gradient, **kw)
             Q[i0:i0 + batch] = res['Q']
             item = dict(ir_steps=res['ir_steps'], dq_max=float(np.max(res['dq'])),
-                        err_est_max=float(np.max(res['err'])), refactored=int(res['refactored'].sum()))
+                        err_est_max=float(np.max(res['err'])), refactored=int(res['refactored'].sum()),
+                        perturbed=res['perturbed'], pivot_min_ratio=res['pivot_min_ratio'])
             if gradient:
                 dY[i0:i0 + batch] = res['dY']
                 item['adj_dq_max'] = float(np.max(res['adj_dq']))

@@ -25,10 +25,43 @@ def _level_widths(mf, nb):
     return out
 
 
-def factor(mf, B, omegas, beta, Kaa, Maa, Daa, nb=32, dtype=np.complex128):
+def row_scales(mf, Kaa, Maa, Daa):
+    """Row maxima of |K|, |M|, |D| over the active entries (the pivot guard's row scale
+    s_g(w) = |K|_g + w^2 |M|_g + w |beta| |D|_g)."""
+    n = int(mf.n)
+    f = mf.ent_front
+    g = mf.rows[mf.rows_ptr[f] + mf.ent_loc % mf.m[f]]
+    out = []
+    for X in (Kaa, Maa, Daa):
+        r = np.zeros(n)
+        if X is not None:
+            np.maximum.at(r, g, np.abs(X[mf.ent_slot]))
+        out.append(r)
+    return out
+
+
+def guard_pivot(piv, scale, eps):
+    """The kernels' pivot guard: |piv| < eps * scale (or non-finite) -> eps * scale with the
+    pivot's phase (eps * scale for a zero pivot), eps = sqrt(unit roundoff) of the front
+    precision.  Returns (pivot, replaced)."""
+    mag = abs(complex(piv))
+    tau = eps * scale
+    if mag >= tau and np.isfinite(mag):
+        return piv, False
+    if 0 < mag < np.inf:
+        return piv * (tau / mag), True
+    return tau, True
+
+
+def factor(mf, B, omegas, beta, Kaa, Maa, Daa, nb=32, dtype=np.complex128, stats=None):
+    """Factor arena of B frequencies; ``stats`` (a dict, optional) receives the number of pivots
+    the guard replaced ('perturbed')."""
     off = mf.off
     arena = np.zeros(mf.arena_elems * B, dtype=dtype)
     widths = _level_widths(mf, nb)
+    eps = np.sqrt(2.0 ** -24 if np.dtype(dtype) == np.complex64 else 2.0 ** -53)
+    rk, rm, rd = row_scales(mf, Kaa, Maa, Daa)
+    nrep = 0
 
     def front(f, b):
         m = int(mf.m[f])
@@ -63,10 +96,14 @@ def factor(mf, B, omegas, beta, Kaa, Maa, Daa, nb=32, dtype=np.complex128):
             for f in fronts:
                 Fm = front(f, b)
                 m, p = int(mf.m[f]), int(mf.p[f])
+                rows = mf.rows[mf.rows_ptr[f]:mf.rows_ptr[f + 1]]
                 for k0 in range(0, p, nbl):
                     k1 = min(p, k0 + nbl)
                     D = Fm[k0:k1, k0:k1]
                     for k in range(k1 - k0):
+                        g = rows[k0 + k]
+                        D[k, k], rep = guard_pivot(D[k, k], rk[g] + w * w * rm[g] + w * abs(beta) * rd[g], eps)
+                        nrep += rep
                         D[k + 1:, k] /= D[k, k]
                         D[k + 1:, k + 1:] -= np.outer(D[k + 1:, k], D[k, k + 1:])
                     Lk = np.tril(D, -1) + np.eye(k1 - k0)
@@ -81,6 +118,8 @@ def factor(mf, B, omegas, beta, Kaa, Maa, Daa, nb=32, dtype=np.complex128):
                     Fm[k1:, k1:] -= np.where(mask, upd, 0)
                 if p < m and p > 0:
                     Fm[p:, p:] -= Fm[p:, :p] @ Fm[:p, p:]
+    if stats is not None:
+        stats['perturbed'] = nrep
     return arena
 
 

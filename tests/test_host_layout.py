@@ -113,3 +113,87 @@ def test_symbolic_covers_every_unknown_once():
         if f.parent >= 0:
             par = sym.fronts[f.parent]
             assert set(f.rows[f.p:].tolist()) <= set(par.rows.tolist())
+
+
+def _planted_zero_pivot_system():
+    """A small cell's active system at one frequency with the first pivot of a leaf front made
+    exactly zero (K_gg = w^2 M_gg): LU without pivoting breaks down there although A is
+    regular."""
+    cell = workloads.double_layer()
+    m = cell.model
+    wtop = 2 * np.pi * cell.band_hz[1]
+    eng = helpers.oracle_engine(cell, wtop)
+    kl = KLayout(m.quads, m.N, 4)
+    bl = BlochLayout(kl, *bc_periodic(m, cell.mesh.lines, ndisp=3))
+    sym = analyse(bl.aa_indptr, bl.aa_indices, bl.node_of, bl.comp_of, m.pos_m, period=2.17e-6, leaf_weight=60)
+    mf = build_mf_arrays(sym, bl.aa_indptr, bl.aa_indices)
+    r = np.repeat(np.arange(bl.n_active), np.diff(bl.aa_indptr))
+    Kaa = np.asarray(eng.Km_aa[r, bl.aa_indices]).ravel()
+    Maa = np.asarray(eng.Mm_aa[r, bl.aa_indices]).ravel()
+    w = 2 * np.pi * 2.0e9
+    leaf = int(mf.lvl_fronts[0])
+    g = int(mf.rows[mf.rows_ptr[leaf]])          # first pivot row of a childless front
+    dslot = int(np.flatnonzero((r == g) & (bl.aa_indices == g))[0])
+    Kaa = Kaa.copy()
+    Kaa[dslot] = w * w * Maa[dslot]
+    assert Kaa[dslot] - w * w * Maa[dslot] == 0
+    return mf, bl, r, Kaa, Maa, w
+
+
+def test_pivot_guard_replaces_a_planted_zero_pivot_and_refinement_recovers():
+    """The kernels' pivot guard on the emulated data flow: a planted exact-zero pivot is
+    replaced (sqrt(u) times its row scale) and counted and the factors stay finite in both
+    precisions; with the perturbed complex128 factors refinement reaches the direct solution of
+    the unperturbed system (the complex64 factors go to the complex128 refactorization when
+    their refinement stalls); without the guard the same factorization divides by zero."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    mf, bl, r, Kaa, Maa, w = _planted_zero_pivot_system()
+    n = bl.n_active
+    A = sp.csr_matrix((Kaa - w * w * Maa, bl.aa_indices, bl.aa_indptr), shape=(n, n))
+    rhs = np.random.default_rng(0).standard_normal(n) * np.abs(A).max(axis=1).toarray().ravel()
+    lu = spla.splu(A.tocsc())
+    x_ref = lu.solve(rhs.astype(complex))
+    for _ in range(4):     # SuperLU alone is ~1e-5 off on the potential rows of this system
+        x_ref = x_ref + lu.solve(rhs - A @ x_ref)
+    for dtype in (np.complex128, np.complex64):
+        st = {}
+        arena = mf_emulator.factor(mf, 1, np.array([w]), 0.0, Kaa, Maa, None, nb=32, dtype=dtype, stats=st)
+        assert st['perturbed'] >= 1
+        assert np.all(np.isfinite(arena))
+        if dtype == np.complex64:
+            continue
+        x = mf_emulator.solve(mf, 1, arena, rhs[None].astype(complex), nb=32)[0]
+        for _ in range(30):
+            d = mf_emulator.solve(mf, 1, arena, (rhs - A @ x)[None].astype(complex), nb=32)[0]
+            x = x + d
+            if np.linalg.norm(d) <= 1e-13 * np.linalg.norm(x):
+                break
+        assert np.linalg.norm(x - x_ref) <= 1e-9 * np.linalg.norm(x_ref)
+    # the guard is what keeps the factors finite: with eps = 0 nothing is replaced
+    import mf_emulator as em
+    keep = em.guard_pivot
+    try:
+        em.guard_pivot = lambda piv, scale, eps: (piv, False)
+        with np.errstate(all='ignore'):
+            try:
+                bad = em.factor(mf, 1, np.array([w]), 0.0, Kaa, Maa, None, nb=32)
+                broke = not np.all(np.isfinite(bad))
+            except np.linalg.LinAlgError:       # the zero pivot's triangle is singular
+                broke = True
+        assert broke
+    finally:
+        em.guard_pivot = keep
+
+
+def test_pivot_guard_leaves_regular_pivots_alone():
+    """On the unmodified system every pivot clears the guard by orders of magnitude: nothing is
+    replaced in either precision."""
+    mf, bl, r, Kaa, Maa, w = _planted_zero_pivot_system()
+    cell = workloads.double_layer()
+    eng = helpers.oracle_engine(cell, 2 * np.pi * cell.band_hz[1])
+    Kaa = np.asarray(eng.Km_aa[r, bl.aa_indices]).ravel()
+    for dtype in (np.complex128, np.complex64):
+        st = {}
+        mf_emulator.factor(mf, 1, np.array([w]), 0.0, Kaa, Maa, None, nb=32, dtype=dtype, stats=st)
+        assert st['perturbed'] == 0
