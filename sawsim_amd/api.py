@@ -32,49 +32,49 @@ from typing import Callable, Optional
 
 import numpy as np
 
+from . import config as _cfg
+
 TEMPLATES = {
-    'sp_double_layer': 'structured double-layer SP cell (reference template, ME1)',
-    'sp_quad_layer': 'structured quad-layer SP cell (reference template, ME1)',
-    'config1': 'sp_stack-style 3m film on Si, plane strain (ME0)',
-    'config2': 'config1 with a SiO2 overlay, refined mesh (ME0)',
-    'config3': 'config1 with Rayleigh damping and dielectric loss (ME0)',
+    'sp_double_layer': 'structured double-layer SP cell (reference template; ME0 or ME1)',
+    'sp_quad_layer': 'structured quad-layer SP cell (reference template; ME0 or ME1)',
+    'sp_stack': 'generic structured SP stack, 0-6 backing layers and 0-3 coatings (reference template)',
     'sp_2p5d_single_layer': 'Hex27 slab, piezo on its own PML (reference template)',
     'sp_2p5d_double_layer': 'Hex27 slab, piezo on cubic Si (reference template)',
     'sp_2p5d_triple_layer': 'Hex27 slab, piezo / SiO2 / poly-Si (reference template)',
     'sp_2p5d_quad_layer': 'Hex27 slab, piezo / SiO2 / poly-Si / anisotropic Si (reference template)',
+    'config1': 'sp_stack-style 3m film on Si, plane strain (ME0), 4312 elements',
+    'config2': 'config1 with a SiO2 overlay, refined mesh (ME0), 16848 elements',
+    'config3': 'config1 with Rayleigh damping and dielectric loss (ME0)',
     'config4': 'Hex27 slab, 3m film / SiO2 / poly-Si / Si, ~8k elements',
 }
-_BUILDERS = {'sp_double_layer': 'double_layer', 'sp_quad_layer': 'quad_layer', 'config1': 'config1',
-             'config2': 'config2', 'config3': 'config3', 'config4': 'config4'}
-_HEX = {'sp_2p5d_single_layer': 'single', 'sp_2p5d_double_layer': 'double', 'sp_2p5d_triple_layer': 'triple',
-        'sp_2p5d_quad_layer': 'quad'}
-_DEFAULTS = dict(points=None, start_ghz=None, stop_ghz=None, voltage=1.0, precision='mixed', gradient=False,
-                 batch=None)
+# the synthetic workload cells take the sweep fields only; their default bands and point counts
+_SYNTH = {'config1': ((1.3143, 1.5429), 101), 'config2': ((2.3114, 2.7134), 401),
+          'config3': ((1.3143, 1.5429), 201), 'config4': ((1.6816, 1.9741), 201)}
+_SYNTH_FIELDS = dict(points=None, start_ghz=None, stop_ghz=None, voltage=1.0)
+# this backend's own fields, beside the reference's
+_BACKEND = dict(precision='mixed', gradient=False, batch=None)
 
 
-def _check_fields(c: dict):
-    """The reference's bounds on the sweep fields (``src/sawsim/config.py:36-39, 122-123``):
-    3 <= points <= 1201 (an integer), start and stop > 0 with stop > start, voltage > 0; and this
-    backend's own: precision 'mixed' or 'fp64', batch a positive integer or None."""
-    import numbers
-
-    def integer(v):
-        return isinstance(v, numbers.Integral) and not isinstance(v, bool)
-
-    def positive(v):
-        return isinstance(v, numbers.Real) and not isinstance(v, bool) and v > 0
-    if c['points'] is not None and not (integer(c['points']) and 3 <= c['points'] <= 1201):
+def _check_sweep_fields(c: dict):
+    """The reference's bounds on the sweep fields of a synthetic cell (``config.py:36-39,
+    122-123``), after the cell's default band and point count are filled in: an integer
+    3 <= points <= 1201, finite start and stop > 0 with stop > start, finite voltage > 0."""
+    if not (_cfg.integer(c['points']) and 3 <= c['points'] <= 1201):
         raise ValueError(f"points must be an integer in [3, 1201], not {c['points']!r}")
-    for k in ('start_ghz', 'stop_ghz'):
-        if c[k] is not None and not positive(c[k]):
-            raise ValueError(f'{k} must be > 0, not {c[k]!r}')
-    if c['start_ghz'] is not None and c['stop_ghz'] is not None and c['stop_ghz'] <= c['start_ghz']:
+    for k in ('start_ghz', 'stop_ghz', 'voltage'):
+        if not (_cfg.finite_number(c[k]) and c[k] > 0):
+            raise ValueError(f'{k} must be a finite number > 0, not {c[k]!r}')
+    if c['stop_ghz'] <= c['start_ghz']:
         raise ValueError('stop_ghz must be above start_ghz')
-    if not positive(c['voltage']):
-        raise ValueError(f"voltage must be > 0, not {c['voltage']!r}")
+
+
+def _check_backend(c: dict):
+    """precision 'mixed' or 'fp64', gradient a bool, batch a positive integer or None."""
     if c['precision'] not in ('mixed', 'fp64'):
         raise ValueError(f"precision must be 'mixed' or 'fp64', not {c['precision']!r}")
-    if c['batch'] is not None and not (integer(c['batch']) and c['batch'] >= 1):
+    if not isinstance(c['gradient'], bool):
+        raise ValueError(f"gradient must be True or False, not {c['gradient']!r}")
+    if c['batch'] is not None and not (_cfg.integer(c['batch']) and c['batch'] >= 1):
         raise ValueError(f"batch must be a positive integer, not {c['batch']!r}")
 
 
@@ -85,18 +85,39 @@ def _atomic_json(path: Path, data):
 
 
 class Model:
-    """A unit-cell model: template id plus sweep fields (points, start_ghz, stop_ghz, voltage,
-    precision 'mixed' | 'fp64', gradient, batch)."""
+    """A unit-cell model: template id plus fields (``sawsim.Model``, ``api.py:12-35``).
+
+    Reference templates (``sp_double_layer``, ``sp_quad_layer``, ``sp_stack``, ``sp_2p5d_*``) take
+    every ``SimulationConfig`` field (pitch_um, electrode_um, substrate_um, metal_ratio, mesh_um,
+    layers, coatings, Euler angles, mode_extension, aperture_um, beta_dk, eta_eps, start_ghz,
+    stop_ghz, points, voltage, ...), over the template's defaults and checked against its ranges
+    (:mod:`sawsim_amd.config`).  The synthetic workload cells (``config1`` .. ``config4``) take the
+    sweep fields only.  Every model also takes this backend's precision ('mixed' | 'fp64'),
+    gradient and batch.  Invalid fields raise ValueError, as the reference's validation does.
+    """
 
     def __init__(self, model_id: str = 'sp_double_layer', **params):
-        if model_id not in TEMPLATES:
+        backend = {k: params.pop(k) for k in list(params) if k in _BACKEND}
+        if model_id in _SYNTH:
+            unknown = set(params) - set(_SYNTH_FIELDS)
+            if unknown:
+                raise ValueError(f'unknown fields {sorted(unknown)} (a synthetic workload cell takes '
+                                 f'{sorted(_SYNTH_FIELDS)})')
+            (lo, hi), pts = _SYNTH[model_id]
+            conf = dict(_SYNTH_FIELDS, **params)
+            conf['points'] = pts if conf['points'] is None else conf['points']
+            conf['start_ghz'] = lo if conf['start_ghz'] is None else conf['start_ghz']
+            conf['stop_ghz'] = hi if conf['stop_ghz'] is None else conf['stop_ghz']
+            _check_sweep_fields(conf)
+        elif model_id in TEMPLATES or model_id in _cfg.UNSUPPORTED:
+            conf = _cfg.validate(model_id, params)
+        else:
             raise ValueError(f'unknown model_id {model_id!r}; available: {sorted(TEMPLATES)}')
-        unknown = set(params) - set(_DEFAULTS)
-        if unknown:
-            raise ValueError(f'unknown fields {sorted(unknown)}')
         self.model_id = model_id
-        self.config = dict(_DEFAULTS, **params, model_id=model_id)
-        _check_fields(self.config)
+        self.config = dict(conf, **dict(_BACKEND, **backend), model_id=model_id)
+        _check_backend(self.config)
+        if self.config['gradient'] and model_id not in _SYNTH:
+            raise ValueError('dY11/dp is computed for the parameterised cells config1-config4 only')
 
     @classmethod
     def from_json(cls, path) -> 'Model':
@@ -114,22 +135,10 @@ class Model:
         """The workload cell (mesh, materials, band) this model sweeps."""
         from . import workloads
         c = self.config
-        if self.model_id in _HEX:
-            name = _HEX[self.model_id]
-            builder = lambda **kw: workloads.hex_template(name, **kw)
-        else:
-            builder = workloads.WORKLOADS[_BUILDERS[self.model_id]]
-        kw = {}
-        if c['points']:
-            kw['points'] = int(c['points'])
-        if self.model_id in _HEX:
-            kw.setdefault('points', 201)
-        cell = builder(**kw)
-        if c['start_ghz'] is not None or c['stop_ghz'] is not None:
-            lo = cell.band_hz[0] if c['start_ghz'] is None else c['start_ghz'] * 1e9
-            hi = cell.band_hz[1] if c['stop_ghz'] is None else c['stop_ghz'] * 1e9
-            cell = builder(band=(lo, hi), **kw)
-        return cell
+        if self.model_id not in _SYNTH:
+            return _cfg.build_cell(c)
+        band = (c['start_ghz'] * 1e9, c['stop_ghz'] * 1e9)
+        return workloads.WORKLOADS[self.model_id](points=int(c['points']), band=band)
 
     def __repr__(self):
         c = self.config
