@@ -177,13 +177,20 @@ def _layer_material(rng):
     return mt.Material(j(2330.0), C, np.zeros((3, 6)), np.eye(3) * j(11.68) * mt.EPS0)
 
 
-def design_cell(seed, points=128, ne_range=(2000, 20000)):
+def design_target(seed, ne_range=(2000, 20000)):
+    """Element-count target of design cell ``seed`` (the last draw of :func:`design_cell`)."""
+    return float(design_cell(seed, ne_range=ne_range, draws_only=True))
+
+
+def design_cell(seed, points=128, ne_range=(2000, 20000), ne_target=None, draws_only=False):
     """One cell of the design set: pitch U(0.8, 2.0) µm, 0-3 backings and 0-2 coatings with
     thicknesses U(0.1, 0.8) pitch (the deepest backing fills the stack to 5 pitch; without
     backings the piezo is the substrate), isotropic/cubic layer records and a 3m piezo, all
     jittered ±5 %, Euler (0, U(-50, -35)°, 0); Q9 plane strain with a mesh size that puts the
     element count near a target drawn uniformly from ``ne_range``; Al electrodes 0.08 pitch thick;
-    1 pitch bottom PML; ``points`` frequencies over [0.7, 1.3] x (3100 m/s) / (2 pitch)."""
+    1 pitch bottom PML; ``points`` frequencies over [0.7, 1.3] x (3100 m/s) / (2 pitch).
+    ``ne_target`` replaces the drawn element count (the same cell on a coarser or finer mesh);
+    ``draws_only`` returns the drawn target without meshing."""
     rng = np.random.default_rng(seed)
     pitch = float(rng.uniform(0.8, 2.0))
     nb, nc = int(rng.integers(0, 4)), int(rng.integers(0, 3))
@@ -200,6 +207,10 @@ def design_cell(seed, points=128, ne_range=(2000, 20000)):
     mats = ([piezo_material(k, euler), _al()] + [_layer_material(rng) for _ in range(nb)]
             + [_layer_material(rng) for _ in range(nc)])
     target = float(rng.uniform(*ne_range))
+    if draws_only:
+        return target
+    if ne_target is not None:
+        target = float(ne_target)
     build = lambda h: stack_cell(pitch, 0.5, 0.08 * pitch, piezo, backs, coats, h, pml_pitches=1.0)
     lo, hi = 0.005 * pitch, 0.5 * pitch                  # mesh size bisection on log scale
     for _ in range(30):

@@ -1,0 +1,111 @@
+"""GPU parity on the benchmark's own workloads and on the reference's stored curves at every
+frequency: config 1 at its full size, cells of the config-5 design set, and the 801-point
+sp_double_layer / sp_quad_layer curves (|Y| and fr/fa)."""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import helpers
+from sawsim_amd import workloads
+from sawsim_amd.metrics import resonances
+
+pytestmark = pytest.mark.gpu
+DATA = Path(__file__).parent / 'data'
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle_q(cell, f):
+    ref = helpers.oracle_engine(cell, 2 * np.pi * cell.band_hz[1])
+    return np.array([helpers.refined_charge(ref, 2 * np.pi * fz)[0] for fz in f])
+
+
+def test_config1_full_size_vs_oracle():
+    """Config 1 as the benchmark names it (mesh 0.045 um: 4312 Q9 elements, 51752 unknowns), five
+    frequencies across its band: FP64 path within 1e-8 and mixed path within 1e-6 of the refined
+    SuperLU oracle."""
+    _say('[test_config1_full_size_vs_oracle]')
+    cell = workloads.config1()
+    assert cell.model.ne == 4312
+    eng = workloads.build_engine(cell)
+    f = cell.frequencies(5)
+    Qo = _oracle_q(cell, f)
+    Q64, _ = eng.sweep_charges(f, precision='fp64')
+    Qm, info = eng.sweep_charges(f, precision='mixed')
+    e64 = np.abs(Q64 - Qo) / np.abs(Qo)
+    em = np.abs(Qm - Qo) / np.abs(Qo)
+    _say('n', eng.n_active, 'fp64', e64, 'mixed', em, info)
+    assert eng.n_active == 51752
+    assert e64.max() <= 1e-8 and em.max() <= 1e-6
+
+
+# seeds of the smallest, median and largest element targets of the 64-cell design set
+DESIGN_SEEDS = {'smallest': 34, 'median': 49, 'largest': 0}
+
+
+@pytest.mark.parametrize('which', list(DESIGN_SEEDS))
+def test_design_cell_vs_oracle(which):
+    """A design-set cell (its pitch, layer stack, materials and band) on a coarse mesh of about
+    2000 elements, three frequencies: FP64 within 1e-8, mixed within 1e-6 of the oracle."""
+    seed = DESIGN_SEEDS[which]
+    _say(f'[test_design_cell_vs_oracle] {which} seed {seed}')
+    assert int(np.argsort([workloads.design_target(s) for s in range(64)])[{'smallest': 0, 'median': 32,
+                                                                              'largest': 63}[which]]) == seed
+    cell = workloads.design_cell(seed, points=3, ne_target=2000)
+    eng = workloads.build_engine(cell)
+    f = cell.frequencies()
+    Qo = _oracle_q(cell, f)
+    Q64, _ = eng.sweep_charges(f, precision='fp64')
+    Qm, info = eng.sweep_charges(f, precision='mixed')
+    e64, em = np.abs(Q64 - Qo) / np.abs(Qo), np.abs(Qm - Qo) / np.abs(Qo)
+    _say(cell.model.ne, 'elements; fp64', e64, 'mixed', em)
+    assert e64.max() <= 1e-8 and em.max() <= 1e-6
+
+
+def test_design_cell_full_size_mixed_vs_fp64_with_gradient():
+    """The median design cell at its full size (13k elements), 16 of its 128 frequencies: the
+    mixed path's Y11 within 1e-6 and dY11/dp within 1e-5 of the FP64 path with its FP64 adjoint."""
+    _say('[test_design_cell_full_size_mixed_vs_fp64_with_gradient]')
+    cell = workloads.design_cell(DESIGN_SEEDS['median'])
+    eng = workloads.build_engine(cell, gradient=True)
+    f = cell.frequencies()[::8]
+    Q, dY, info = eng.sweep_charges(f, precision='mixed', gradient=True)
+    Q64, dY64, _ = eng.sweep_charges(f, precision='fp64', gradient=True)
+    scale = np.abs(cell.params)
+    scale[12:15] = 1.0
+    ey = np.abs(Q - Q64) / np.abs(Q64)
+    eg = np.abs((dY - dY64) * scale).max(axis=1) / np.abs(dY64 * scale).max(axis=1)
+    _say(cell.model.ne, 'elements; Y11', ey.max(), 'dY/dp', eg.max(), info)
+    assert ey.max() <= 1e-6 and eg.max() <= 1e-5
+
+
+@pytest.mark.parametrize('name', ['sp_double_layer', 'sp_quad_layer'])
+def test_stored_curve_at_every_frequency(name):
+    """All 801 stored frequencies of the reference's curve (tests/data/<name>.npz, its own
+    regression test: tests/test_templates.py:14-39) swept on the GPU in mixed precision: |Y|
+    within the fixture's regression_rtol everywhere with a median deviation below 1e-6, and fr/fa
+    of the GPU curve within 1 kHz of fr/fa of the stored curve (the reference's fr/fa agreement
+    check, tests/test_agent.py:54-61)."""
+    _say(f'[test_stored_curve_at_every_frequency] {name}')
+    meta = json.loads((DATA / f'{name}.json').read_text())
+    d = np.load(DATA / f'{name}.npz')
+    f, stored = d['frequency_hz'], d['sawsim_magnitude']
+    builder = workloads.double_layer if name == 'sp_double_layer' else workloads.quad_layer
+    cell = builder(points=f.size, band=(f[0], f[-1]))
+    assert np.max(np.abs(cell.frequencies() - f)) <= 1.0          # the grid lands within 1 Hz
+    eng = workloads.build_engine(cell)
+    Q, info = eng.sweep_charges(f, precision='mixed')
+    y = np.abs(-1j * 2 * np.pi * f * Q)
+    rel = np.abs(y - stored) / stored
+    r_gpu, r_ref = resonances(f, y), resonances(f, stored)
+    _say(name, 'max', rel.max(), 'median', np.median(rel), 'fr', r_gpu['fr_hz'], r_ref['fr_hz'], 'fa',
+         r_gpu['fa_hz'], r_ref['fa_hz'], 'refactored', sum(i['refactored'] for i in info))
+    assert rel.max() <= meta['regression_rtol'] and np.median(rel) <= 1e-6
+    assert r_ref['fr_hz'] is not None and r_ref['fa_hz'] is not None
+    assert abs(r_gpu['fr_hz'] - r_ref['fr_hz']) <= 1e3 and abs(r_gpu['fa_hz'] - r_ref['fa_hz']) <= 1e3
