@@ -185,6 +185,18 @@ int sawfem_adjoint_elem(void* stream, int ne0, const int* elems, const int* conn
                         double omega_top, int ndu, int N, const int* pos_rep, const unsigned char* zmask, double V,
                         double scale, int B, int n, const void* x, const void* lam, void* work, void* W);
 
+/* sawfem_adjoint_elem without the work argument: allocates nothing.  No counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The B * ne0 * NT complex128 element sums live
+ * in a static device array of 64 MiB per GPU (held from library load until the process ends);
+ * larger requests return 1.  Every call on one GPU uses that GPU's array: calls that overlap on
+ * one GPU race. */
+int sawfem_adjoint_elem_noalloc(void* stream, int ne0, const int* elems, const int* conn, const double* xy,
+                                const int* elem_tag, int npml, const int* pml_tag, const int* pml_kind,
+                                const double* pml_par, double omega_top, int ndu, int N, const int* pos_rep,
+                                const unsigned char* zmask, double V, double scale, int B, int n, const void* x,
+                                const void* lam, void* W);
+
 #ifdef __cplusplus
 }
 #endif

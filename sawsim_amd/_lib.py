@@ -46,6 +46,8 @@ SIGNATURES = {
     'sawfem_axpy': [_vp, _i64, _vp, _vp],
     'sawfem_adjoint_elem': [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _i, _vp, _vp, _d, _d, _i, _i,
                             _vp, _vp, _vp, _vp],
+    'sawfem_adjoint_elem_noalloc': [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _i, _vp, _vp, _d, _d, _i,
+                                    _i, _vp, _vp, _vp],
 }
 _RESTYPE = {'sawfem_last_error': C.c_char_p, 'sawfem_mf_arena_elems': _i64, 'sawfem_mf_uarena_elems': _i64}
 

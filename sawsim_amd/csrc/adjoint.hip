@@ -16,7 +16,42 @@
 // elements of the parameterised material only.
 #include "q9.h"
 
+#include <mutex>
+
+// Scratch of sawfem_adjoint_elem_noalloc: the per-element sensitivity sums (B * ne0 * NT
+// complex128) live in this file-scope device array instead of a caller workspace.  Each GPU the
+// library runs on holds its own copy from module load until the process ends.
+constexpr size_t kAdjScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kAdjScratchBytes];
+
 namespace {
+
+// Device address of `scratch` per device, looked up once per device with that device current;
+// host code uses these addresses, never the symbol itself.
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+int scratch_address(void** out) {
+  int dev = -1;
+  SAW_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    sawfem_set_error("sawfem_adjoint_elem_noalloc: device ordinal out of range");
+    return 1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    SAW_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      sawfem_set_error("sawfem_adjoint_elem_noalloc: no device address for the scratch array");
+      return 3;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
 
 using namespace q9;
 
@@ -221,6 +256,31 @@ int sawfem_adjoint_elem(void* stream, int ne0, const int* elems, const int* conn
                      (const double2*)work, (double2*)W);
   SAW_HIP(hipGetLastError());
   return 0;
+}
+
+// The same computation with the scratch array above in place of `work` (no device memory is
+// allocated).  B * ne0 * NT * 16 bytes must fit the array (64 MiB); the kernels write every
+// scratch entry they read, so nothing needs clearing.  Calls that overlap on one GPU race on
+// the array.
+int sawfem_adjoint_elem_noalloc(void* stream, int ne0, const int* elems, const int* conn, const double* xy,
+                                const int* elem_tag, int npml, const int* pml_tag, const int* pml_kind,
+                                const double* pml_par, double omega_top, int ndu, int N, const int* pos_rep,
+                                const unsigned char* zmask, double V, double scale, int B, int n, const void* x,
+                                const void* lam, void* W) {
+  if (ne0 < 0 || B <= 0 || (ndu != 2 && ndu != 3)) {
+    sawfem_set_error("sawfem_adjoint_elem_noalloc: invalid arguments");
+    return 1;
+  }
+  const int NR = 2 * ndu, NT = NR * NR + 2 * NR + 4;
+  if ((size_t)B * (size_t)ne0 * (size_t)NT * sizeof(double2) > kAdjScratchBytes) {
+    sawfem_set_error("sawfem_adjoint_elem_noalloc: B * ne0 * NT complex128 sums exceed the 64 MiB scratch array");
+    return 1;
+  }
+  void* work = nullptr;
+  const int rc = scratch_address(&work);
+  if (rc) return rc;
+  return sawfem_adjoint_elem(stream, ne0, elems, conn, xy, elem_tag, npml, pml_tag, pml_kind, pml_par, omega_top,
+                             ndu, N, pos_rep, zmask, V, scale, B, n, x, lam, work, W);
 }
 
 }  // extern "C"
