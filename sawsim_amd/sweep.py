@@ -33,7 +33,7 @@ import numpy as np
 
 from . import _lib
 from .layout import BlochLayout
-from .piezo_fem import DeviceCSR, Model, _torch, stream_ptr
+from .piezo_fem import DeviceCSR, Model, _torch, stream_ptr, timed_call
 from .plan import build_mf_arrays
 from .symbolic import analyse
 
@@ -45,7 +45,7 @@ SOLVE_FULL_M = 256
 
 # device memory every workspace request leaves free for the HIP runtime (queues, code objects,
 # kernel scratch backing) and the caller's own tensors
-HBM_RESERVE_BYTES = 12 * 10 ** 9
+HBM_RESERVE_BYTES = 16 * 10 ** 9
 
 
 def hbm_available(keep=0):
@@ -196,12 +196,17 @@ class FrequencySolver:
         bl = self.bloch
         n = self.n_active
         s = stream_ptr()
-        _lib.call('sawfem_gather3', s, bl.nnz_aa, self._aa[0].data_ptr(), self._aa[1].data_ptr(),
-                  K.values.data_ptr(), M.values.data_ptr(), None, self.Kaa.data_ptr(), self.Maa.data_ptr(), None)
-        _lib.call('sawfem_gather3', s, n, self._rh[0].data_ptr(), self._rh[1].data_ptr(),
-                  K.values.data_ptr(), M.values.data_ptr(), None, self.rK.data_ptr(), self.rM.data_ptr(), None)
-        _lib.call('sawfem_gather3', s, n + 1, self._kt[0].data_ptr(), self._kt[1].data_ptr(),
-                  K.values.data_ptr(), None, None, self._kts.data_ptr(), None, None)
+        # K3 reads 16 B per listed source entry and source, the 8 B index, and writes 16 B per
+        # output and source
+        g3 = lambda ptr, nout, nsrc: int(ptr[1].numel() * (16 * nsrc + 8) + 8 * (nout + 1) + 16 * nout * nsrc)
+        timed_call('k_gather3', g3(self._aa, bl.nnz_aa, 2), 'sawfem_gather3', s, bl.nnz_aa, self._aa[0].data_ptr(),
+                   self._aa[1].data_ptr(), K.values.data_ptr(), M.values.data_ptr(), None, self.Kaa.data_ptr(),
+                   self.Maa.data_ptr(), None)
+        timed_call('k_gather3', g3(self._rh, n, 2), 'sawfem_gather3', s, n, self._rh[0].data_ptr(),
+                   self._rh[1].data_ptr(), K.values.data_ptr(), M.values.data_ptr(), None, self.rK.data_ptr(),
+                   self.rM.data_ptr(), None)
+        timed_call('k_gather3', g3(self._kt, n + 1, 1), 'sawfem_gather3', s, n + 1, self._kt[0].data_ptr(),
+                   self._kt[1].data_ptr(), K.values.data_ptr(), None, None, self._kts.data_ptr(), None, None)
         self.kts_a = self._kts[:n]
         self.kts_z = complex(self._kts[n].item())
         self._sym = None
@@ -333,11 +338,21 @@ class FrequencySolver:
         return self._ws[:nbytes].view(dtype)
 
     def chunk_bytes(self, batch, precision='mixed', gradient=False):
-        """Device bytes one chunk of ``batch`` frequencies takes: front arena, update vectors
-        (two right-hand sides with the gradient) and about a dozen length-n vectors."""
+        """Device bytes one chunk of ``batch`` frequencies takes at its peak, counted from what
+        solve_batch allocates: front arena; update vectors (two right-hand sides with the
+        gradient); right-hand sides, solutions, residuals and their temporaries ([B][R][n]); the
+        three frequency-major residual buffers ([n][Bp], Bp = R B padded to 64); the element sums
+        of the gradient contraction; plus 10 %."""
         R = 2 if gradient else 1
-        per = self.arena_bytes(1, precision) + 16 * self.mf.uarena_elems * R + 16 * 12 * R * self.n_active
-        return per * batch
+        n = self.n_active
+        Bp = -(-R * batch // 64) * 64
+        vecs = (12 if gradient else 4) * batch + 3 * Bp
+        work = 0
+        if gradient and getattr(self, '_grad', None) is not None:
+            nd = self.layout.nd - 1
+            work = 16 * batch * self._grad['ne0'] * ((2 * nd) ** 2 + 4 * nd + 4)
+        total = self.arena_bytes(batch, precision) + 16 * self.mf.uarena_elems * R * batch + 16 * n * vecs + work
+        return int(1.1 * total)
 
     def release_workspace(self):
         """Free the arena workspace (it is otherwise kept for the next solve_batch)."""
@@ -596,7 +611,10 @@ class FrequencySolver:
         if budget < per:
             raise ValueError(f'one frequency needs {per / 1e9:.1f} GB of device memory; '
                              f'{max(budget, 0) / 1e9:.1f} GB are available above the reserve')
-        return int(budget // per)
+        b = max(1, int(budget // per))
+        while b > 1 and self.chunk_bytes(b, precision, gradient) > budget:
+            b = max(1, int(b * budget / self.chunk_bytes(b, precision, gradient)) - 1)
+        return b
 
     def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, on_chunk=None, **kw):
         """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies (default: as many

@@ -13,6 +13,13 @@ Covers what no closed-form or stored-curve check pins:
 * ``golden_cfg3_coarse.npz``: the reference CPU sweep (real-block SuperLU, as its fallback
   backend) of config 3 on a coarse mesh at 5 frequencies, and central finite differences of
   Y11 for the 17 parameters (relative step 1e-6; 1e-6 rad absolute for zero angles).
+* ``golden_hex27_tiny.npz``: the Hex27 path (``saw2d/saw3d``: elements, assembly, boundary,
+  bc_solve, loaded as files without the package ``__init__``, which imports Gmsh):
+  K and M of ``form_stiffness_mass_3d`` on six elements of a slab with three PML records
+  (z, x and x+z stretch, each with a rotated 3m piezo record) and two plain records; and, on a
+  84-element single-layer slab with a piezo PML, the boundary sets of
+  ``boundary_idx_vect_3ds``, the two fold operators, the active set and Q of
+  ``FrequencySolver3D.solve`` (SuperLU) at 3 frequencies, plus Q with 3 FP64 refinement steps.
 """
 from __future__ import annotations
 
@@ -153,11 +160,99 @@ def cfg3(rmat, rfem, rsweep, rcrys, npts=5):
     print('cfg3: N', mesh.N, 'ne', mesh.ne)
 
 
+def _ref_saw3d(ref):
+    """The reference's saw3d modules as files of a stand-in package (its __init__ is not run)."""
+    import importlib
+    import types
+    d = Path(ref) / 'sawsim' / 'saw2d' / 'saw3d'
+    pkg = types.ModuleType('_ref_saw3d')
+    pkg.__path__ = [str(d)]
+    sys.modules['_ref_saw3d'] = pkg
+    return {n: importlib.import_module(f'_ref_saw3d.{n}') for n in ('elements', 'assembly', 'boundary', 'bc_solve')}
+
+
+def hex_tiny(ref):
+    import scipy.sparse.linalg as spla
+    from sawsim_amd.hexmesh import template_slab
+    from sawsim_amd.workloads import hex_record_materials
+    r = _ref_saw3d(ref)
+    euler = (7.0, 48.0, -11.0)
+    mats = hex_record_materials(euler, 'al', [])          # rotated sp_baseline piezo, Al electrode
+    rec = lambda m, ids: dict(rho=m.rho, C=np.asarray(m.C), e=np.asarray(m.e), eps=np.asarray(m.eps),
+                              indx=np.asarray(ids, dtype=np.int64))
+    # full slab: 84 elements, piezo on its own 2-pitch PML (z stretch)
+    mesh = template_slab('single', 1.085, 0.217, 0.17, 1.2, 0.5, 0.5, [])
+    tm, tp = mesh.info['tag_map'], mesh.info['tag_pml']
+    mat_list = [rec(mats[i], np.flatnonzero(mesh.hex_tag == t)) for t, i in tm.items() if t != tp]
+    bottom = float(mesh.pos_m[mesh.sets['bottom'], 2].min())
+    pml = dict(region=3, za=bottom + mesh.info['pml_thickness_m'], zp=bottom, xa=0.0, xp=0.0, dmax=1e11, n=3,
+               **rec(mats[tm[tp]], np.flatnonzero(mesh.hex_tag == tp)))
+    f = np.array([1.8e9, 2.1e9, 2.4e9])
+    wtop = 2 * np.pi * 2.7e9
+    M, K = r['assembly'].form_stiffness_mass_3d(mesh.hexas, mesh.pos_m, mat_list, [pml], wtop)
+    N = mesh.pos_m.shape[0]
+    bnd = r['boundary'].boundary_idx_vect_3ds(mesh.sets, mesh.pos_m, 3 * N)
+    t_fb, t_lr = r['bc_solve'].build_periodic_operators(bnd, 4 * N)
+    eng = r['bc_solve'].FrequencySolver3D(K, M, bnd, t_fb, t_lr)
+    q_raw, q_ref = [], []
+    for fz in f:
+        w = 2 * np.pi * fz
+        q_raw.append(eng.solve(w, 1.0)[1])
+        a = (eng.k_aa - w * w * eng.m_aa).tocsc()
+        rhs = -np.asarray((eng.k_az - w * w * eng.m_az).sum(axis=1)).ravel()
+        lu = spla.splu(a)
+        x = lu.solve(rhs)
+        for _ in range(3):
+            x = x + lu.solve(rhs - a @ x)
+        disp = np.zeros(eng.gdof, dtype=complex)
+        disp[eng.active] = x
+        r['bc_solve'].apply_periodic_values(disp, bnd, 1.0)
+        q_ref.append((eng.kz_rows @ disp).sum())
+    for t in (t_fb, t_lr):
+        assert np.all(np.diff(t.indptr) == 1)            # one 1 per row: a representative map
+    out = dict(slab_pos=mesh.pos_m, slab_hexas=mesh.hexas, slab_hex_tag=mesh.hex_tag, slab_euler=np.array(euler),
+               slab_frequency_hz=f, slab_omega_top=wtop, slab_q=np.array(q_raw), slab_q_refined=np.array(q_ref),
+               slab_active=eng.active, slab_zheng=np.asarray(eng.zheng), t_fb=t_fb.indices, t_lr=t_lr.indices,
+               slab_K_nnz=K.nnz)
+    out.update({f'slab_sets_{k}': np.asarray(v) for k, v in mesh.sets.items()})
+    for part in ('machine', 'electric'):
+        for k, v in getattr(bnd, part).items():
+            out[f'bnd_{part}_{k}'] = np.asarray(v)
+    # six elements with three PML records (z, x, x+z stretch) and two plain records
+    pe = np.flatnonzero(mesh.hex_tag == tp)
+    top = np.flatnonzero(mesh.hex_tag != tp)
+    al = np.flatnonzero(mesh.hex_tag == [t for t, i in tm.items() if i == 1][0])
+    pz = np.setdiff1d(top, al)
+    pick = np.array([pz[0], pz[len(pz) // 2], al[0], pe[0], pe[len(pe) // 2], pe[-1]])
+    nodes, inv = np.unique(mesh.hexas[pick], return_inverse=True)
+    elem = inv.reshape(pick.size, 27)
+    coord = mesh.pos_m[nodes]
+    xs, zs = coord[:, 0], coord[:, 2]
+    sub_mats = [rec(mats[0], [0, 1]), rec(mats[1], [2])]
+    base = dict(dmax=1e11, n=3, xa=float(xs.max() + 0.3e-6), xp=float(xs.min() - 0.1e-6),
+                za=float(zs.max() + 0.2e-6), zp=float(zs.min() - 0.05e-6))
+    m2 = hex_record_materials((20.0, -30.0, 35.0), 'al', [])[0]
+    sub_pml = [dict(region=3, **base, **rec(mats[0], [3])), dict(region=5, **base, **rec(m2, [4])),
+               dict(region=2, **base, **rec(mats[0], [5]))]
+    Ms, Ks = r['assembly'].form_stiffness_mass_3d(elem, coord, sub_mats, sub_pml, wtop)
+    out.update(sub_elem=elem, sub_coord=coord, sub_K_indptr=Ks.indptr, sub_K_indices=Ks.indices, sub_K_data=Ks.data,
+               sub_M_indptr=Ms.indptr, sub_M_indices=Ms.indices, sub_M_data=Ms.data, sub_omega=wtop)
+    for i, m in enumerate(sub_mats):
+        out.update({f'sub_mat{i}_{k}': np.asarray(v) for k, v in m.items()})
+    for i, m in enumerate(sub_pml):
+        out.update({f'sub_pml{i}_{k}': np.asarray(v) for k, v in m.items()})
+    np.savez_compressed(HERE / 'golden_hex27_tiny.npz', **out)
+    print('hex27: slab N', N, 'ne', mesh.ne, 'K nnz', K.nnz, 'active', eng.active.size, 'Q', q_raw, q_ref,
+          '| sub N', nodes.size, 'K nnz', Ks.nnz)
+
+
 if __name__ == '__main__':
     ap = argparse.ArgumentParser()
     ap.add_argument('--ref', default='/root/reference/src')
     ap.add_argument('--what', default='tiny,cfg3')
     a = ap.parse_args()
+    if 'hex27' in a.what:
+        hex_tiny(a.ref)
     mods = _ref_modules(a.ref)
     if 'tiny' in a.what:
         tiny(*mods)

@@ -263,3 +263,79 @@ def test_oracle_reproduces_stored_slab_curves(name):
         assert abs(y - own) / own <= 5e-5
         ref = d['reference_magnitude'][i]
         assert abs(y - ref) / ref <= meta['reference_rtol']
+
+
+# ------------------------------------------------------------------ reference golden vectors
+def hex27_golden():
+    """tests/golden/golden_hex27_tiny.npz (written by tests/golden/make_golden.py from the
+    reference's saw3d modules) unpacked into meshes, records and expected outputs."""
+    g = np.load(Path(__file__).parent / 'golden' / 'golden_hex27_tiny.npz')
+    import scipy.sparse as sp
+
+    def recs(prefix, n):
+        out = []
+        for i in range(n):
+            keys = [k for k in g.files if k.startswith(f'{prefix}{i}_')]
+            out.append({k[len(f'{prefix}{i}_'):]: (g[k].item() if g[k].ndim == 0 else g[k]) for k in keys})
+        return out
+    nsub = g['sub_coord'].shape[0]
+    csr = lambda p: sp.csr_matrix((g[p + '_data'], g[p + '_indices'], g[p + '_indptr']), shape=(4 * nsub, 4 * nsub))
+    sets = {k[len('slab_sets_'):]: g[k] for k in g.files if k.startswith('slab_sets_')}
+    bnd = {part: {k[len(f'bnd_{part}_'):]: g[k] for k in g.files if k.startswith(f'bnd_{part}_')}
+           for part in ('machine', 'electric')}
+    return g, recs('sub_mat', 2), recs('sub_pml', 3), csr('sub_K'), csr('sub_M'), sets, bnd
+
+
+def test_oracle_assembly_matches_reference_golden_with_x_z_and_xz_stretch():
+    """K and M of the reference's form_stiffness_mass_3d on six elements with PML records of
+    region 3 (z), 5 (x) and 2 (x and z) stretch, each with a rotated 3m record: the oracle's
+    entries within 1e-12 of the largest, on the same pattern."""
+    g, mats, pmls, Kr, Mr, _, _ = hex27_golden()
+    M, K = fem3d.assemble(g['sub_elem'], g['sub_coord'], mats, pmls, float(g['sub_omega']))
+    for ours, ref in ((K, Kr), (M, Mr)):
+        d = (ours.tocsr() - ref).tocsr()
+        assert abs(d).max() <= 1e-12 * abs(ref).max()
+        assert (abs(ref) > 0).sum() == (abs(ours) > 0).sum()
+
+
+def test_boundary_sets_and_fold_match_reference_golden():
+    """The reference's boundary_idx_vect_3ds sets and its two fold operators on the 84-element
+    slab: the product's host sets and the oracle's equal them element for element, and the
+    product's representative map equals the composite fold T_fb T_lr."""
+    g, _, _, _, _, sets, bnd = hex27_golden()
+    pos = g['slab_pos']
+    N = pos.shape[0]
+    b = hex27.boundary_idx_vect_3ds(sets, pos, 3 * N)
+    m, el = fem3d.boundary_dofs(sets, pos, 3 * N)
+    for part, ours, orc in (('machine', b.machine, m), ('electric', b.electric, el)):
+        assert set(ours) == set(bnd[part]) == set(orc)
+        for k, ref in bnd[part].items():
+            assert np.array_equal(np.asarray(ours[k]), ref) and np.array_equal(np.asarray(orc[k]), ref), (part, k)
+    gd = 4 * N
+    rep = hex27.fold_map(b, gd)
+    t_fb, t_lr = g['t_fb'], g['t_lr']
+    assert np.array_equal(rep, t_lr[t_fb])             # one 1 per row: (T_fb T_lr)[d] = lr(fb(d))
+    assert np.array_equal(np.setdiff1d(np.arange(gd), hex27.prescribed_dofs(b)), g['slab_active'])
+
+
+def test_oracle_slab_charge_matches_reference_golden():
+    """Q of the reference's FrequencySolver3D (SuperLU + 3 refinement steps on its own system)
+    at three frequencies: the oracle, on its own assembly and folds, within 1e-9."""
+    g, _, _, _, _, sets, _ = hex27_golden()
+    mats = workloads.hex_record_materials(tuple(g['slab_euler']), 'al', [])
+    mesh = hexmesh.template_slab('single', 1.085, 0.217, 0.17, 1.2, 0.5, 0.5, [])
+    assert np.array_equal(mesh.pos_m, g['slab_pos']) and np.array_equal(mesh.hexas, g['slab_hexas'])
+    cell = workloads.HexCell(mesh=mesh, materials=mats, band_hz=(1.8e9, 2.7e9), points=3)
+    M, K = fem3d.assemble(mesh.hexas, mesh.pos_m, cell.material_list(), cell.pml_list(), float(g['slab_omega_top']))
+    assert K.nnz >= int(g['slab_K_nnz'])
+    m, el = fem3d.boundary_dofs(mesh.sets, mesh.pos_m, 3 * mesh.N)
+    eng = fem3d.Solver3D(K, M, m, el)
+    for w, qr in zip(2 * np.pi * g['slab_frequency_hz'], g['slab_q_refined']):
+        a = (eng.k_aa - w * w * eng.m_aa).tocsc()
+        rhs = -np.asarray((eng.k_az - w * w * eng.m_az).sum(axis=1)).ravel()
+        lu = spla.splu(a)
+        x = lu.solve(rhs)
+        for _ in range(3):
+            x = x + lu.solve(rhs - a @ x)
+        q = (eng.kz_rows @ eng.expand(x)).sum()
+        assert abs(q - qr) <= 1e-9 * abs(qr), (q, qr)
