@@ -27,7 +27,7 @@ import numpy as np
 
 from . import _lib
 from .layout import KLayout
-from .piezo_fem import DeviceCSR, _torch, stream_ptr
+from .piezo_fem import DeviceCSR, _torch, stream_ptr, timed_call
 
 KM_SCALE = 6.723363908816403e-11            # conditioning scale of the u rows (assembly.py:24)
 _X_REGIONS = {1, 5, 6, 7, 8, 9, 10, 11}     # PML regions that stretch x (assembly.py:27)
@@ -229,9 +229,10 @@ def assemble_h27(elem, coord, material_list, pml_list, omega, damping=None):
     Ke = torch.empty(ne * 16 * 729, dtype=torch.complex128, device=dev)
     Me = torch.empty(ne * 729, dtype=torch.complex128, device=dev)
     s = stream_ptr()
-    _lib.call('sawfem_elem_h27', s, ne, g.conn.data_ptr(), g.xyz.data_ptr(), dm.data_ptr(), dpml.data_ptr(),
-              dC.data_ptr(), dE.data_ptr(), dP.data_ptr(), drho.data_ptr(), npml, kinds.ctypes.data, par.ctypes.data,
-              float(omega), KM_SCALE, Ke.data_ptr(), Me.data_ptr())
+    timed_call('k_elem_h27', ne * (16 * 17 * 729 + 27 * (4 + 24) + 8), 'sawfem_elem_h27', s, ne, g.conn.data_ptr(),
+               g.xyz.data_ptr(), dm.data_ptr(), dpml.data_ptr(), dC.data_ptr(), dE.data_ptr(), dP.data_ptr(),
+               drho.data_ptr(), npml, kinds.ctypes.data, par.ctypes.data, float(omega), KM_SCALE, Ke.data_ptr(),
+               Me.data_ptr())
     K = torch.empty(kl.nnz, dtype=torch.complex128, device=dev)
     M = torch.empty(kl.nnz, dtype=torch.complex128, device=dev)
     mask = np.zeros(ne, dtype=np.uint8)
@@ -243,9 +244,10 @@ def assemble_h27(elem, coord, material_list, pml_list, omega, damping=None):
                 mask[np.asarray(mat['indx'], dtype=np.int64)] = 1
         D = torch.empty(kl.nnz, dtype=torch.complex128, device=dev)
     dmask = t(mask)
-    _lib.call('sawfem_gather_h27', s, kl.nnz_node, g.pair_row.data_ptr(), g.node_ptr.data_ptr(), g.gptr.data_ptr(),
-              g.gidx.data_ptr(), dmask.data_ptr(), Ke.data_ptr(), Me.data_ptr(), K.data_ptr(), M.data_ptr(),
-              D.data_ptr() if D is not None else None)
+    nbytes = Ke.numel() * 16 + Me.numel() * 16 + g.gidx.numel() * 8 + kl.nnz * 16 * (3 if D is not None else 2)
+    timed_call('k_gather_h27', nbytes, 'sawfem_gather_h27', s, kl.nnz_node, g.pair_row.data_ptr(),
+               g.node_ptr.data_ptr(), g.gptr.data_ptr(), g.gidx.data_ptr(), dmask.data_ptr(), Ke.data_ptr(),
+               Me.data_ptr(), K.data_ptr(), M.data_ptr(), D.data_ptr() if D is not None else None)
     del Ke, Me
     out = [DeviceCSR(kl, K), DeviceCSR(kl, M)] + ([DeviceCSR(kl, D)] if D is not None else [])
     for mat in out:

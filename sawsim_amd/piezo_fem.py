@@ -76,6 +76,24 @@ def stream_ptr():
     return torch.cuda.current_stream().cuda_stream
 
 
+# Optional timing of the assembly and Bloch-gather launches: when set to a list, every such launch
+# appends (kernel, start event, stop event, algorithmic HBM bytes) recorded on the launch stream.
+KERNEL_LOG = None
+
+
+def timed_call(kernel, nbytes, name, *args):
+    """``_lib.call(name, *args)``, bracketed by events into KERNEL_LOG when it is a list."""
+    if KERNEL_LOG is None:
+        return _lib.call(name, *args)
+    torch = _torch()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    rc = _lib.call(name, *args)
+    b.record()
+    KERNEL_LOG.append((kernel, a, b, int(nbytes)))
+    return rc
+
+
 class DeviceCSR:
     """Complex128 values in HBM on a shared structural pattern."""
 
@@ -154,10 +172,12 @@ def assemble_device(model: Model, materials, elem_mat_idx, pml_regions, omega_to
     Me = torch.empty(ne * 81, dtype=torch.complex128, device=dev)
     tags, kinds, par = _pml_arrays(pml_regions)
     s = stream_ptr()
-    _lib.call('sawfem_elem_q9', s, ne, nd - 1, g.conn.data_ptr(), g.xy.data_ptr(), midx.data_ptr(), g.tag.data_ptr(),
-              dC9.data_ptr(), dE9.data_ptr(), dEP.data_ptr(), drho.data_ptr(), len(pml_regions),
-              tags.ctypes.data, kinds.ctypes.data, par.ctypes.data, float(omega_top), SCALE,
-              Ke.data_ptr(), Me.data_ptr())
+    # K1 writes the element matrices; it reads 9 node indices and coordinates per element
+    timed_call('k_elem_q9', ne * (16 * (nd * nd + 1) * 81 + 9 * (4 + 16) + 8), 'sawfem_elem_q9', s, ne, nd - 1,
+               g.conn.data_ptr(), g.xy.data_ptr(), midx.data_ptr(), g.tag.data_ptr(),
+               dC9.data_ptr(), dE9.data_ptr(), dEP.data_ptr(), drho.data_ptr(), len(pml_regions),
+               tags.ctypes.data, kinds.ctypes.data, par.ctypes.data, float(omega_top), SCALE,
+               Ke.data_ptr(), Me.data_ptr())
     K = torch.empty(kl.nnz, dtype=torch.complex128, device=dev)
     M = torch.empty(kl.nnz, dtype=torch.complex128, device=dev)
     D = None
@@ -166,9 +186,11 @@ def assemble_device(model: Model, materials, elem_mat_idx, pml_regions, omega_to
         D = torch.empty(kl.nnz, dtype=torch.complex128, device=dev)
         dm = np.asarray(damping_mask, dtype=bool)[np.asarray(elem_mat_idx)]
         dmask = t(dm.astype(np.uint8))
-    _lib.call('sawfem_gather_kmd', s, kl.nnz_node, nd, nd - 1, g.pair_row.data_ptr(), g.node_ptr.data_ptr(),
-              g.gptr.data_ptr(), g.gidx.data_ptr(), dmask.data_ptr(), Ke.data_ptr(), Me.data_ptr(),
-              K.data_ptr(), M.data_ptr(), D.data_ptr() if D is not None else None)
+    # K2 reads every element entry once (Ke, Me) and its code, writes K, M (and D)
+    nbytes = Ke.numel() * 16 + Me.numel() * 16 + g.gidx.numel() * 8 + kl.nnz * 16 * (3 if D is not None else 2)
+    timed_call('k_gather_kmd', nbytes, 'sawfem_gather_kmd', s, kl.nnz_node, nd, nd - 1, g.pair_row.data_ptr(),
+               g.node_ptr.data_ptr(), g.gptr.data_ptr(), g.gidx.data_ptr(), dmask.data_ptr(), Ke.data_ptr(),
+               Me.data_ptr(), K.data_ptr(), M.data_ptr(), D.data_ptr() if D is not None else None)
     out = (DeviceCSR(kl, K), DeviceCSR(kl, M))
     if D is not None:
         out = out + (DeviceCSR(kl, D),)
