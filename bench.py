@@ -361,7 +361,9 @@ def main():
                                                 budget=args.arena_gb * 1e9 if args.arena_gb else None))
     nchunk = -(-P // batch)
     batch = -(-P // nchunk)
-    log(f'[bench] batch {batch} x {nchunk} chunks, arena {per_freq * batch / 1e9:.1f} GB')
+    from sawsim_amd.sweep import hbm_available
+    log(f'[bench] batch {batch} x {nchunk} chunks, arena {per_freq * batch / 1e9:.1f} GB, chunk '
+        f'{eng.chunk_bytes(batch, "mixed", True) / 1e9:.1f} GB of {hbm_available() / 1e9:.1f} GB available')
 
     def sweep():
         return eng.sweep_charges(freqs, precision='mixed', batch=batch, gradient=True)

@@ -342,7 +342,7 @@ class FrequencySolver:
         solve_batch allocates: front arena; update vectors (two right-hand sides with the
         gradient); right-hand sides, solutions, residuals and their temporaries ([B][R][n]); the
         three frequency-major residual buffers ([n][Bp], Bp = R B padded to 64); the element sums
-        of the gradient contraction; plus 10 %."""
+        of the gradient contraction; 10 % on top of everything but the arena, whose size is exact."""
         R = 2 if gradient else 1
         n = self.n_active
         Bp = -(-R * batch // 64) * 64
@@ -351,8 +351,8 @@ class FrequencySolver:
         if gradient and getattr(self, '_grad', None) is not None:
             nd = self.layout.nd - 1
             work = 16 * batch * self._grad['ne0'] * ((2 * nd) ** 2 + 4 * nd + 4)
-        total = self.arena_bytes(batch, precision) + 16 * self.mf.uarena_elems * R * batch + 16 * n * vecs + work
-        return int(1.1 * total)
+        other = 16 * self.mf.uarena_elems * R * batch + 16 * n * vecs + work
+        return int(self.arena_bytes(batch, precision) + 1.1 * other)
 
     def release_workspace(self):
         """Free the arena workspace (it is otherwise kept for the next solve_batch)."""
@@ -611,10 +611,11 @@ class FrequencySolver:
         if budget < per:
             raise ValueError(f'one frequency needs {per / 1e9:.1f} GB of device memory; '
                              f'{max(budget, 0) / 1e9:.1f} GB are available above the reserve')
-        b = max(1, int(budget // per))
-        while b > 1 and self.chunk_bytes(b, precision, gradient) > budget:
-            b = max(1, int(b * budget / self.chunk_bytes(b, precision, gradient)) - 1)
-        return b
+        lo, hi = 1, max(1, int(budget // self.arena_bytes(1, precision))) + 1   # chunk_bytes(hi) > budget
+        while hi - lo > 1:                      # chunk_bytes grows with the batch: bisect
+            mid = (lo + hi) // 2
+            lo, hi = (mid, hi) if self.chunk_bytes(mid, precision, gradient) <= budget else (lo, mid)
+        return lo
 
     def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, on_chunk=None, **kw):
         """Charges Q(f) for all frequencies, in chunks of ``batch`` frequencies (default: as many
