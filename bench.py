@@ -316,6 +316,8 @@ def main():
     ap.add_argument('--small-max', type=int, default=-1, help='largest front for the fused kernel (-1: default)')
     ap.add_argument('--solve-full', type=int, default=-1,
                     help='largest front solved by one workgroup (-1: the library default)')
+    ap.add_argument('--panel-lu', type=int, default=-1,
+                    help='largest front order whose 32-wide panels use the one-kernel panel LU (-1: default)')
     ap.add_argument('--copy-mib', type=int, default=0,
                     help='only run a plain device copy of this many MiB (HBM counter calibration) and exit')
     args = ap.parse_args()
@@ -348,6 +350,8 @@ def main():
         _lib.call_value('sawfem_mf_set_small', eng._plan, 0, args.small_max)
     if args.solve_full >= 0:
         eng.set_solve_full(args.solve_full)
+    if args.panel_lu >= 0:
+        eng.set_panel_lu(args.panel_lu)
     torch.cuda.synchronize()
     log(f'[bench] {args.workload}: n={eng.n_active} fronts={eng.mf.nf} setup {time.time() - t0:.1f}s '
         f'(analysis {eng.analysis_seconds:.1f}s)')

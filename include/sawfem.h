@@ -115,6 +115,12 @@ int sawfem_mf_set_small(void* plan, int prec, int max_m);
  * Returns the order in use (capped at 1024), or -1.  A tuning knob of the frontal solves
  * (part of the PARDISO/SuperLU replacement, sweep.py:349-353); no separate reference interface. */
 int sawfem_mf_set_solve_full(void* plan, int max_m);
+/* In the factorization, the 32-wide pivot panels of front-size buckets whose largest front has
+ * order <= max_m are factored by one kernel per panel (diagonal block LU and inverses and both
+ * panel solves, one wave per front and frequency) instead of the diagonal-block kernel and two
+ * MFMA panel-solve GEMMs; 0 (the default) keeps the latter.  Returns the order in use, or -1.
+ * A kernel-selection knob of the PARDISO/SuperLU replacement (sweep.py:349-353). */
+int sawfem_mf_set_panel_lu(void* plan, int max_m);
 
 /* Factor A(w_b) = K + i w_b beta D - w_b^2 M for B frequencies (omega: device doubles)
  * into arena (arena_elems * B entries; complex64 if prec == 0, complex128 if prec == 1).

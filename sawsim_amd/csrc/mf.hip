@@ -467,6 +467,66 @@ __global__ void __launch_bounds__(512) k_diag(FrontsDev F, const int* __restrict
   diag_body<T>(F, lvl, B, arena, k0, nb, PG, blockIdx.y);
 }
 
+// Diagonal block and both panel solves of a 32-wide pivot panel in one wave per front and
+// frequency (fronts of moderate order, where the three separate launches are latency-bound):
+// diag_body factors and inverts the kb x kb block in LDS, then each lane takes one row of the
+// column panel, L21[i, :] = A21[i, :] U_D^{-1}, or one column of the row panel,
+// U12[:, j] = L_D^{-1} A12[:, j]: the kb inputs are loaded into registers (the row's loads are
+// coalesced across lanes, a column is contiguous), the triangular products read the inverse
+// from LDS (the same address on every lane), and the results overwrite the inputs.
+constexpr int PANEL_LU_NB = 32;
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_panel_lu(FrontsDev F, const int* __restrict__ lvl, int B,
+                                                 typename cplx<T>::type* __restrict__ arena, int k0,
+                                                 const PivotGuard PG) {
+  using V = typename cplx<T>::type;
+  constexpr int NB = PANEL_LU_NB;
+  const int b = blockIdx.y;
+  diag_body<T>(F, lvl, B, arena, k0, NB, PG, b);
+  const int f = lvl[blockIdx.x];
+  const int p = F.p[f];
+  if (k0 >= p) return;
+  const int m = F.m[f];
+  const int kb = min(NB, p - k0), k1 = k0 + kb;
+  const int q = m - k1;
+  extern __shared__ unsigned char smem_raw[];
+  const V* Ds = reinterpret_cast<const V*>(smem_raw);   // L_D^{-1} below, U_D^{-1} on/above
+  constexpr int ld = NB + 1;
+  V* Fb = front_ptr<T>(arena, F, f, b, B);
+  for (int it = threadIdx.x; it < 2 * q; it += 64) {
+    V a[NB];
+    if (it < q) {                 // row i of the column panel
+      const int i = k1 + it;
+#pragma unroll
+      for (int k = 0; k < NB; ++k) a[k] = k < kb ? Fb[(int64_t)i + (int64_t)(k0 + k) * m] : czero<V>();
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (c < kb) {
+          V acc = czero<V>();
+#pragma unroll
+          for (int k = 0; k <= c; ++k) acc = cfma(acc, a[k], Ds[k + c * ld]);
+          Fb[(int64_t)i + (int64_t)(k0 + c) * m] = acc;
+        }
+      }
+    } else {                      // column j of the row panel
+      const int j = k1 + (it - q);
+      const V* col = Fb + (int64_t)k0 + (int64_t)j * m;
+#pragma unroll
+      for (int r = 0; r < NB; ++r) a[r] = r < kb ? col[r] : czero<V>();
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        if (r < kb) {
+          V acc = a[r];
+#pragma unroll
+          for (int k = 0; k < r; ++k) acc = cfma(acc, Ds[r + k * ld], a[k]);
+          Fb[(int64_t)(k0 + r) + (int64_t)j * m] = acc;
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------- small fronts, fused
 // One workgroup of 512 threads per (front, frequency) for fronts whose m x m block fits in
 // LDS (m <= 128): the whole factorization of the front happens there.
@@ -1418,6 +1478,7 @@ struct Plan {
   int lds_max = 64 * 1024;       // LDS one workgroup may allocate on this device
   int small_mmax[2] = {0, 0};    // threshold in use
   int solve_full_m = 0;          // fronts up to this order are solved by one workgroup each
+  int panel_lu_m = 0;            // 32-wide panels of buckets up to this order: k_panel_lu
   std::vector<void*> part_owned[2];
   std::vector<void*> panel_owned[2];   // the buckets' panel work lists (rebuilt per panel width)
   int panel_nb[2] = {0, 0};      // panel width the buckets' panel lists were built for (0: none)
@@ -1732,6 +1793,8 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
       // 32-wide panels use 128 x 32 / 32 x 128 tiles (no wave on columns outside the panel);
       // the bucket of fronts with m <= 64 keeps 64 x 64 tiles (measured equal or better there)
       const bool narrow = nbl == 32 && G.mmax > 64;
+      // diagonal block + panel solves in one wave per front (k_panel_lu) for these buckets
+      const bool panel_lu = nbl == PANEL_LU_NB && G.mmax <= P->panel_lu_m;
       for (size_t pk = 0; pk < G.panels.size(); ++pk) {
         const PanelTiles& PT = G.panels[pk];
         const int k0 = (int)pk * nbl;
@@ -1766,12 +1829,16 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
         // 32-wide blocks: one wave per block (its barriers cost next to nothing and more blocks
         // share a CU); 64-wide blocks (levels of few large fronts, few workgroups): eight waves
         // share each step's updates (config 2: 7.2 -> 3.1 ms per top level)
-        if (PT.ndiag > 0) {
+        if (PT.ndiag > 0 && panel_lu) {
+          Timed tm(P, sg, K_DIAG);
+          hipLaunchKernelGGL(k_panel_lu<T>, dim3(PT.ndiag, B), dim3(64), lds_diag, sg, P->F, PT.diag, B, arena, k0,
+                             PG);
+        } else if (PT.ndiag > 0) {
           Timed tm(P, sg, K_DIAG);
           hipLaunchKernelGGL(k_diag<T>, dim3(PT.ndiag, B), dim3(nbl <= 32 ? 64 : 512), lds_diag, sg, P->F, PT.diag, B,
                              arena, k0, nbl, PG);
         }
-        if (PT.n23 > 0) {
+        if (PT.n23 > 0 && !panel_lu) {
           Timed tm(P, sg, K_TRSM);
           if (narrow) {
             hipLaunchKernelGGL((k_gemm<T, 2, 128, 32>), dim3(PT.n23, 1, B), dim3(256), 0, sg, P->F, PT.t23, B, arena,
@@ -2069,6 +2136,16 @@ int sawfem_mf_set_small(void* plan, int prec, int max_m) {
 // triangle and off-diagonal panel together, k_tri with FULL) instead of the pivot-triangle +
 // panel kernel pair; 0 keeps the pair for every front that the factorization did not fuse.
 // Returns the order in use.
+// 32-wide pivot panels of size buckets whose largest front has order <= max_m are factored by
+// one kernel per panel (diagonal block and both panel solves, one wave per front and frequency)
+// instead of the diagonal-block kernel and two MFMA panel-solve GEMMs; 0 keeps the latter.
+int sawfem_mf_set_panel_lu(void* plan, int max_m) {
+  Plan* P = (Plan*)plan;
+  if (!P || max_m < 0) { sawfem_set_error("sawfem_mf_set_panel_lu: invalid arguments"); return -1; }
+  P->panel_lu_m = max_m;
+  return P->panel_lu_m;
+}
+
 int sawfem_mf_set_solve_full(void* plan, int max_m) {
   Plan* P = (Plan*)plan;
   if (!P || max_m < 0) { sawfem_set_error("sawfem_mf_set_solve_full: invalid arguments"); return -1; }

@@ -42,6 +42,9 @@ from .symbolic import analyse
 # panel) per frequency (measured on config 2 with the size buckets of a level on concurrent
 # streams: 256 is the best of 128/192/256/320/448/640)
 SOLVE_FULL_M = 256
+# 32-wide pivot panels of size buckets up to this front order are factored by one kernel per
+# panel (diagonal block and both panel solves in one wave per front and frequency)
+PANEL_LU_M = 256
 
 # device memory every workspace request leaves free for the HIP runtime (queues, code objects,
 # kernel scratch backing) and the caller's own tensors
@@ -160,7 +163,13 @@ class FrequencySolver:
         self.analysis_seconds = time.time() - t0
         self._plan = self._create_plan()
         self.set_solve_full(SOLVE_FULL_M)
+        self.set_panel_lu(PANEL_LU_M)
         self.stats = dict(self.symbolic.stats)
+
+    def set_panel_lu(self, max_m):
+        """Factor the 32-wide panels of size buckets up to front order ``max_m`` with one kernel
+        per panel (``sawfem_mf_set_panel_lu``); 0 uses the diagonal-block kernel and two GEMMs."""
+        return _lib.call_value('sawfem_mf_set_panel_lu', self._plan, int(max_m))
 
     def set_solve_full(self, max_m):
         """Solve fronts of order <= ``max_m`` with one workgroup each (``sawfem_mf_set_solve_full``)."""
