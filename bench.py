@@ -318,6 +318,8 @@ def main():
                     help='largest front solved by one workgroup (-1: the library default)')
     ap.add_argument('--panel-lu', type=int, default=-1,
                     help='largest front order whose 32-wide panels use the one-kernel panel LU (-1: default)')
+    ap.add_argument('--tri-lds', type=int, default=-1,
+                    help='LDS bytes for the factor block of the LDS-staged solves (-1: default, 0: off)')
     ap.add_argument('--copy-mib', type=int, default=0,
                     help='only run a plain device copy of this many MiB (HBM counter calibration) and exit')
     args = ap.parse_args()
@@ -352,6 +354,8 @@ def main():
         eng.set_solve_full(args.solve_full)
     if args.panel_lu >= 0:
         eng.set_panel_lu(args.panel_lu)
+    if args.tri_lds >= 0:
+        eng.set_tri_lds(args.tri_lds)
     torch.cuda.synchronize()
     log(f'[bench] {args.workload}: n={eng.n_active} fronts={eng.mf.nf} setup {time.time() - t0:.1f}s '
         f'(analysis {eng.analysis_seconds:.1f}s)')

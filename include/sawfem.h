@@ -121,6 +121,12 @@ int sawfem_mf_set_solve_full(void* plan, int max_m);
  * MFMA panel-solve GEMMs; 0 (the default) keeps the latter.  Returns the order in use, or -1.
  * A kernel-selection knob of the PARDISO/SuperLU replacement (sweep.py:349-353). */
 int sawfem_mf_set_panel_lu(void* plan, int max_m);
+/* In the solves (not transposed), fronts of the one-workgroup set (sawfem_mf_set_solve_full) whose
+ * size bucket's largest factor block (m * p entries of the front precision) fits max_bytes are
+ * solved with that block staged in LDS by one coalesced load, forward and backward; 0 (the
+ * default) keeps the pivot-triangle kernels.  Clipped to the device's LDS per workgroup; returns
+ * the bytes in use, or -1.  A kernel-selection knob of the PARDISO/SuperLU replacement. */
+int sawfem_mf_set_tri_lds(void* plan, int max_bytes);
 
 /* Factor A(w_b) = K + i w_b beta D - w_b^2 M for B frequencies (omega: device doubles)
  * into arena (arena_elems * B entries; complex64 if prec == 0, complex128 if prec == 1).

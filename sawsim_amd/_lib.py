@@ -34,6 +34,7 @@ SIGNATURES = {
     'sawfem_mf_set_small': [_vp, _i, _i],
     'sawfem_mf_set_solve_full': [_vp, _i],
     'sawfem_mf_set_panel_lu': [_vp, _i],
+    'sawfem_mf_set_tri_lds': [_vp, _i],
     'sawfem_mf_profile_levels': [_vp, _vp, _i],
     'sawfem_mf_factor': [_vp, _vp, _i, _i, _vp, _d, _vp, _vp, _vp, _vp, _i],
     'sawfem_mf_solve': [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _i],

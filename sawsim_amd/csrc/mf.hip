@@ -1165,6 +1165,183 @@ __global__ void __launch_bounds__(512) k_tri(FrontsDev F, const int* __restrict_
 
 }
 
+// Whole-front solves of small fronts with the front's factor block staged in LDS, so the solve
+// itself touches no global memory between the gather and the write-back (the pivot-triangle +
+// panel kernels wait on a global load per diagonal block and per panel row chunk).  Persistent
+// 256-thread workgroups walk the (front, frequency) items of a bucket; the next item's block is
+// loaded into registers while the current one is solved from LDS, so loads and solves overlap.
+//  kind 0 (forward):  the first p columns of the front (L11 and L21, contiguous in the
+//                     column-major front) -> Pl[k m + i] = F[i][k];
+//                     w = gathered right-hand sides and children's updates over all m rows;
+//                     per diagonal block: w_blk = L_D^{-1} w_blk, then w[below] -= L[below][blk] w_blk;
+//                     pivot rows to x, the other rows to the front's update vector
+//  kind 1 (backward): the first p rows of every column (U11 and U12) -> Pl[j p + i] = F[i][j];
+//                     w = x over all m rows (the contribution-block rows are final);
+//                     w[0:p] -= U12 w[p:m], then per diagonal block, last first:
+//                     w_blk = U_D^{-1} w_blk, w[above] -= U[above][blk] w_blk; pivot rows to x
+// The diagonal blocks hold L_D^{-1} strictly below and U_D^{-1} on/above the diagonal (nbl wide).
+// block entries a thread prefetches: blocks up to 4096 complex64 or 2048 complex128 entries
+template <typename T> constexpr int tri_lds_npf() { return sizeof(T) == 4 ? 16 : 8; }
+
+// Loads of a front's factor block (kind 0: first p columns; kind 1: first p rows of every column)
+// into registers, NPF entries per thread, clamped to the block so every address is valid.
+template <typename T, int KIND, int NPF>
+__device__ __forceinline__ void tri_lds_fetch(const FrontsDev& F, const typename cplx<T>::type* arena, int f, int b,
+                                              int B, T (&px)[NPF], T (&py)[NPF]) {
+  using V = typename cplx<T>::type;
+  const int t = threadIdx.x;
+  const int m = F.m[f], p = F.p[f], mp = m * p;
+  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
+#pragma unroll
+  for (int u = 0; u < NPF; ++u) {
+    const int idx = min(t + 256 * u, mp - 1);
+    V v;
+    if (KIND == 0) {
+      v = Fb[idx];
+    } else {
+      const int j = idx / p;
+      v = Fb[(int64_t)(idx - j * p) + (int64_t)j * m];
+    }
+    px[u] = v.x;         // components kept as scalars: arrays of vector types went to scratch
+    py[u] = v.y;
+  }
+}
+
+template <typename T, int KIND, int R>
+__global__ void __launch_bounds__(256) k_tri_lds(FrontsDev F, const int* __restrict__ lvl, int nf, int B, int n,
+                                                 int nbl, const typename cplx<T>::type* __restrict__ arena,
+                                                 double2* __restrict__ uarena, double2* __restrict__ x,
+                                                 const int* __restrict__ bsel, int nsel) {
+  using V = typename cplx<T>::type;
+  static_assert(KIND == 0 || KIND == 1, "forward L or backward U");
+  constexpr int NPF = tri_lds_npf<T>();
+  const int t = threadIdx.x;
+  const int nitems = nf * nsel;
+  extern __shared__ unsigned char smem_raw[];
+  double2* w = reinterpret_cast<double2*>(smem_raw);            // [R][m]
+  double2* part = w + R * 256;                                   // [R][4][64] (m <= 256)
+  V* Pl = reinterpret_cast<V*>(part + R * 4 * 64);               // [p][m] (kind 0) or [m][p] (kind 1)
+  T px[NPF], py[NPF];
+  auto front_of = [&](int it) { return lvl[it % nf]; };
+  auto freq_of = [&](int it) { return bsel ? bsel[it / nf] : it / nf; };
+  int item = blockIdx.x;
+  if (item < nitems) tri_lds_fetch<T, KIND, NPF>(F, arena, front_of(item), freq_of(item), B, px, py);
+  for (; item < nitems; item += gridDim.x) {
+    const int f = front_of(item), b = freq_of(item);
+    const int m = F.m[f], p = F.p[f], mp = m * p;
+    const int* rows = F.rows + F.rows_ptr[f];
+#pragma unroll
+    for (int u = 0; u < NPF; ++u)
+      if (t + 256 * u < mp) {
+        V v;
+        v.x = px[u];
+        v.y = py[u];
+        Pl[t + 256 * u] = v;
+      }
+    if (KIND == 0) {
+      solve_gather<R, true>(F, f, b, B, n, m, p, rows, x, uarena, w);
+    } else {
+      for (int i = t; i < m; i += 256)
+#pragma unroll
+        for (int r = 0; r < R; ++r) w[r * m + i] = x[((int64_t)b * R + r) * n + rows[i]];
+    }
+    __syncthreads();
+    if (item + (int)gridDim.x < nitems)     // in flight during the solve below
+      tri_lds_fetch<T, KIND, NPF>(F, arena, front_of(item + gridDim.x), freq_of(item + gridDim.x), B, px, py);
+    const int i64 = t & 63, g = t >> 6;     // row of a 64-row slab, quarter of a reduction
+    if (KIND == 1 && p < m) {
+      // w[i] -= sum_j U[i][p + j] w[p + j]: four partial sums per pivot row
+      const int q = m - p;
+      for (int i0 = 0; i0 < p; i0 += 64) {
+        const int i = i0 + i64;
+        double2 s[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[r] = make_double2(0, 0);
+        if (i < p)
+          for (int j = g; j < q; j += 4) {
+            const double2 a = to_d2(Pl[(p + j) * p + i]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * m + p + j]);
+          }
+#pragma unroll
+        for (int r = 0; r < R; ++r) part[(r * 4 + g) * 64 + i64] = s[r];
+        __syncthreads();
+        if (t < 64 && i < p)
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            double2 tot = part[r * 256 + t];
+            for (int gg = 1; gg < 4; ++gg) tot = d2add(tot, part[(r * 4 + gg) * 64 + t]);
+            w[r * m + i] = make_double2(w[r * m + i].x - tot.x, w[r * m + i].y - tot.y);
+          }
+        __syncthreads();
+      }
+    }
+    const int nblk = (p + nbl - 1) / nbl;
+    for (int bi = 0; bi < nblk; ++bi) {
+      const int J0 = KIND == 0 ? bi * nbl : (nblk - 1 - bi) * nbl;
+      const int nbk = min(nbl, p - J0);
+      {   // diagonal block: four partial sums per row
+        double2 s[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[r] = make_double2(0, 0);
+        if (i64 < nbk) {
+          if (KIND == 0) {        // L_D^{-1} (unit lower): w_i + sum_{k<i} Li[i][k] w_k
+            for (int k = g; k < i64; k += 4) {
+              const double2 a = to_d2(Pl[(J0 + k) * m + J0 + i64]);
+#pragma unroll
+              for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * m + J0 + k]);
+            }
+          } else {                // U_D^{-1}: sum_{k>=i} Ui[i][k] w_k
+            for (int k = i64 + g; k < nbk; k += 4) {
+              const double2 a = to_d2(Pl[(J0 + k) * p + J0 + i64]);
+#pragma unroll
+              for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * m + J0 + k]);
+            }
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) part[(r * 4 + g) * 64 + i64] = s[r];
+      }
+      __syncthreads();
+      if (t < nbk)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          double2 tot = part[r * 256 + t];
+          for (int gg = 1; gg < 4; ++gg) tot = d2add(tot, part[(r * 4 + gg) * 64 + t]);
+          if (KIND == 0) tot = d2add(tot, w[r * m + J0 + t]);
+          w[r * m + J0 + t] = tot;
+        }
+      __syncthreads();
+      // the other rows of the block's columns: forward below the block (all m rows), backward above
+      const int lo = KIND == 0 ? J0 + nbk : 0, hi = KIND == 0 ? m : J0;
+      for (int i = lo + t; i < hi; i += 256) {
+        double2 s[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[r] = w[r * m + i];
+        for (int k = 0; k < nbk; ++k) {
+          const double2 a = to_d2(KIND == 0 ? Pl[(J0 + k) * m + i] : Pl[(J0 + k) * p + i]);
+#pragma unroll
+          for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * m + J0 + k]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) w[r * m + i] = s[r];
+      }
+      __syncthreads();
+    }
+    for (int i = t; i < m; i += 256) {
+      if (i < p) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[((int64_t)b * R + r) * n + rows[i]] = w[r * m + i];
+      } else if (KIND == 0) {
+        double2* uf = uarena + F.uoff[f] * B * R + (int64_t)b * R * (m - p) + (i - p);
+#pragma unroll
+        for (int r = 0; r < R; ++r) uf[(int64_t)r * (m - p)] = w[r * m + i];
+      }
+    }
+    __syncthreads();          // w and Pl are rewritten by the next item
+  }
+}
+
 // Off-diagonal panels, one workgroup per 64-wide output tile of one front and frequency; lvl is
 // a tile list (front << 12 | tile), the frequency comes from blockIdx.y.
 //  kind 0 (forward,  L21):   uf[r]  -= sum_k L[p + r][k] y[k]        r in CB rows
@@ -1405,6 +1582,7 @@ struct PanelTiles {
 
 struct Sub {
   int nf = 0, mmax = 0, pmax = 0, cbmax = 0;
+  int64_t mpmax = 0;               // largest m * p (factor block of the LDS-staged solves)
   const int* fronts = nullptr;   // device list
   std::vector<int> hfronts;      // host copy
   std::vector<PanelTiles> panels;  // per pivot panel, for the panel width of panel_nb
@@ -1476,9 +1654,12 @@ struct Plan {
   unsigned char* fused[2] = {nullptr, nullptr};
   int small_limit[2] = {0, 0};   // device LDS bound per precision
   int lds_max = 64 * 1024;       // LDS one workgroup may allocate on this device
+  int lds_per_cu = 64 * 1024;    // LDS of one CU
+  int ncu = 1;                   // compute units
   int small_mmax[2] = {0, 0};    // threshold in use
   int solve_full_m = 0;          // fronts up to this order are solved by one workgroup each
   int panel_lu_m = 0;            // 32-wide panels of buckets up to this order: k_panel_lu
+  int tri_lds_bytes = 0;         // one-workgroup solve buckets whose factor block fits: k_tri_lds
   std::vector<void*> part_owned[2];
   std::vector<void*> panel_owned[2];   // the buckets' panel work lists (rebuilt per panel width)
   int panel_nb[2] = {0, 0};      // panel width the buckets' panel lists were built for (0: none)
@@ -1598,6 +1779,7 @@ int build_partition(Plan* P, int pi, int max_m) {
         S.mmax = std::max(S.mmax, P->hm[f]);
         S.pmax = std::max(S.pmax, P->hp[f]);
         S.cbmax = std::max(S.cbmax, P->hm[f] - P->hp[f]);
+        S.mpmax = std::max(S.mpmax, (int64_t)P->hm[f] * P->hp[f]);
       }
     };
     // size buckets by front order (edges roughly 1.7x apart): fronts of one bucket need grids
@@ -1931,13 +2113,46 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   // they run on the launch stream and the side streams between a fork and a join.
   auto stream_of = [&](bool fork, size_t unit) { return (!fork || unit == 0) ? s : P->side[(unit - 1) % Plan::NSIDE]; };
   const int kf = transpose ? 2 : 0, kb = transpose ? 3 : 1;
+  // one-workgroup buckets whose factor block fits the LDS budget: whole-front solves from LDS
+  auto lds_bytes = [&](const Sub& S) {
+    return sizeof(V) * (size_t)S.mpmax + sizeof(double2) * R * (256 + 4 * 64);
+  };
+  auto lds_ok = [&](const Sub& S) {
+    return !transpose && P->tri_lds_bytes > 0 && S.mmax <= 256 && S.mpmax <= 256 * tri_lds_npf<T>() &&
+           sizeof(V) * (size_t)S.mpmax <= (size_t)P->tri_lds_bytes && lds_bytes(S) <= (size_t)P->lds_max;
+  };
+  auto tri_lds = [&](const Level& L, const Sub& S, int kind, hipStream_t st) {
+    if (S.nf == 0) return;
+    Timed tm(P, st, kind == 0 ? K_FWD : K_BWD);
+    const size_t lds = lds_bytes(S);
+    const int nbl = panel_width(nb, L);
+    // persistent grid: as many workgroups as the CUs hold at this LDS size (at most 8 a CU)
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (size_t)P->lds_per_cu / lds));
+    const int64_t nitems = (int64_t)S.nf * nsel;
+    const unsigned grid = (unsigned)std::min<int64_t>(nitems, (int64_t)P->ncu * per_cu);
+    if (kind == 0) {
+      if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 0, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((k_tri_lds<T, 0, R>), dim3(grid), dim3(256), lds, st, P->F, S.fronts, S.nf, B, P->n, nbl, A,
+                         U, X, bsel, nsel);
+    } else {
+      if (lds > 64 * 1024)
+        (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 1, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL((k_tri_lds<T, 1, R>), dim3(grid), dim3(256), lds, st, P->F, S.fronts, S.nf, B, P->n, nbl, A,
+                         U, X, bsel, nsel);
+    }
+  };
   for (size_t li = 0; li < P->levels.size(); ++li) {
     const Level& L = P->levels[li];
     P->cur_level = (int)li;
     const size_t nfull = L.ssmall[pi].size();
     const bool fork = P->overlap && L.sbig[pi].size() + nfull > 1;
     if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
-    for (size_t gi = 0; gi < nfull; ++gi) tri_sub(L, L.ssmall[pi][gi], kf, true, stream_of(fork, gi));
+    for (size_t gi = 0; gi < nfull; ++gi) {
+      const Sub& S = L.ssmall[pi][gi];
+      if (lds_ok(S)) tri_lds(L, S, 0, stream_of(fork, gi));
+      else tri_sub(L, S, kf, true, stream_of(fork, gi));
+    }
     for (size_t gi = 0; gi < L.sbig[pi].size(); ++gi) {
       const hipStream_t st = stream_of(fork, gi + nfull);
       tri_sub(L, L.sbig[pi][gi], kf, false, st);
@@ -1948,6 +2163,26 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
     const Level& L = P->levels[l];
     P->cur_level = l;
+    if (!transpose && P->tri_lds_bytes > 0) {
+      // the forward partition: one-workgroup buckets (LDS-staged where their block fits, else
+      // the whole-front pivot-triangle kernel), the rest through the panel + triangle pair
+      const size_t nfull = L.ssmall[pi].size();
+      const bool fork = P->overlap && L.sbig[pi].size() + nfull > 1;
+      if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
+      for (size_t gi = 0; gi < L.sbig[pi].size(); ++gi) {
+        const hipStream_t st = stream_of(fork, gi);
+        panel_sub(L.sbig[pi][gi], kb, st);
+        tri_sub(L, L.sbig[pi][gi], kb, false, st);
+      }
+      for (size_t gi = 0; gi < nfull; ++gi) {
+        const Sub& S = L.ssmall[pi][gi];
+        const hipStream_t st = stream_of(fork, gi + L.sbig[pi].size());
+        if (lds_ok(S)) tri_lds(L, S, 1, st);
+        else tri_sub(L, S, kb, true, st);
+      }
+      if (fork && join_side(P, s)) return 3;
+      continue;
+    }
     const bool full = L.small[pi].nf > 0;
     const bool fork = P->overlap && L.big[pi].size() + (full ? 1 : 0) > 1;
     if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
@@ -2061,6 +2296,12 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
         hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
       lds = 64 * 1024;
     P->lds_max = lds;
+    int ldscu = lds, ncu = 1;
+    if (hipDeviceGetAttribute(&ldscu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess)
+      ldscu = lds;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1) ncu = 1;
+    P->lds_per_cu = std::max(ldscu, lds);
+    P->ncu = ncu;
     for (int pi = 0; pi < 2; ++pi) {
       const size_t sz = pi == 0 ? sizeof(c64) : sizeof(c128);
       int mm = 0;
@@ -2136,6 +2377,16 @@ int sawfem_mf_set_small(void* plan, int prec, int max_m) {
 // triangle and off-diagonal panel together, k_tri with FULL) instead of the pivot-triangle +
 // panel kernel pair; 0 keeps the pair for every front that the factorization did not fuse.
 // Returns the order in use.
+// One-workgroup solve buckets (fronts up to the solve_full order) whose largest factor block
+// (m p entries of the front precision) fits max_bytes of LDS are solved with that block staged in
+// LDS, forward and backward; 0 keeps the pivot-triangle kernels.  Returns the bytes in use.
+int sawfem_mf_set_tri_lds(void* plan, int max_bytes) {
+  Plan* P = (Plan*)plan;
+  if (!P || max_bytes < 0) { sawfem_set_error("sawfem_mf_set_tri_lds: invalid arguments"); return -1; }
+  P->tri_lds_bytes = std::min(max_bytes, P->lds_max);
+  return P->tri_lds_bytes;
+}
+
 // 32-wide pivot panels of size buckets whose largest front has order <= max_m are factored by
 // one kernel per panel (diagonal block and both panel solves, one wave per front and frequency)
 // instead of the diagonal-block kernel and two MFMA panel-solve GEMMs; 0 keeps the latter.

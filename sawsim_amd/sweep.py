@@ -45,6 +45,11 @@ SOLVE_FULL_M = 256
 # 32-wide pivot panels of size buckets up to this front order are factored by one kernel per
 # panel (diagonal block and both panel solves in one wave per front and frequency)
 PANEL_LU_M = 256
+# one-workgroup solve buckets whose largest factor block fits this many bytes are solved with
+# the block staged in LDS by persistent workgroups (forward and backward); off by default:
+# on config 2 it shortens the level-0 solves (193 -> 152 ms of kernel time per sweep) but the
+# persistent grids crowd out the concurrent buckets (291 vs 294 pts/s); the tuner may pick it
+TRI_LDS_BYTES = 0
 
 # device memory every workspace request leaves free for the HIP runtime (queues, code objects,
 # kernel scratch backing) and the caller's own tensors
@@ -164,12 +169,18 @@ class FrequencySolver:
         self._plan = self._create_plan()
         self.set_solve_full(SOLVE_FULL_M)
         self.set_panel_lu(PANEL_LU_M)
+        self.set_tri_lds(TRI_LDS_BYTES)
         self.stats = dict(self.symbolic.stats)
 
     def set_panel_lu(self, max_m):
         """Factor the 32-wide panels of size buckets up to front order ``max_m`` with one kernel
         per panel (``sawfem_mf_set_panel_lu``); 0 uses the diagonal-block kernel and two GEMMs."""
         return _lib.call_value('sawfem_mf_set_panel_lu', self._plan, int(max_m))
+
+    def set_tri_lds(self, max_bytes):
+        """Solve one-workgroup buckets whose factor block fits ``max_bytes`` from LDS
+        (``sawfem_mf_set_tri_lds``); 0 keeps the pivot-triangle kernels."""
+        return _lib.call_value('sawfem_mf_set_tri_lds', self._plan, int(max_bytes))
 
     def set_solve_full(self, max_m):
         """Solve fronts of order <= ``max_m`` with one workgroup each (``sawfem_mf_set_solve_full``)."""
