@@ -213,6 +213,16 @@ def design_set(args):
     log(f'[bench] config5: {len(cells)} cells, {sum(ne)} elements ({min(ne)}-{max(ne)}), setup {setup:.0f}s, '
         f'workspace {ws_bytes / 1e9:.0f} GB')
 
+    tuning = None
+    if not args.no_autotune:
+        from sawsim_amd.tune import autotune, _SETTERS
+        big = int(np.argmax(ne))
+        tuning = autotune(engines[big], cells[big].frequencies(), 'mixed', gradient=True, batch=batches[big])
+        for eng in engines:
+            for knob, v in tuning['choice'].items():
+                getattr(eng, _SETTERS[knob])(v)
+        log(f"[bench] autotune on cell {big}: {tuning['choice']}")
+
     def sweep(i):
         return engines[i].sweep_charges(cells[i].frequencies(), precision='mixed', batch=batches[i], gradient=True)
     for _ in range(args.warmup):
@@ -274,7 +284,7 @@ def design_set(args):
         config=dict(workload=f'config5: {len(cells)} design cells (seeds 0..{len(cells) - 1}), {sum(ne)} Q9 elements '
                              f'({min(ne)}-{max(ne)} per cell), {P} points each, swept back to back',
                     cells=len(cells), points_per_cell=P, elements=ne,
-                    n_active=[int(e.n_active) for e in engines], batches=batches,
+                    n_active=[int(e.n_active) for e in engines], batches=batches, tuning=tuning,
                     gradient='dY11/dp for 17 parameters by adjoint'),
         latency_ms_per_cell=dict(p50=float(np.percentile(lat_ms, 50)), p90=float(np.percentile(lat_ms, 90)),
                                  p99=float(np.percentile(lat_ms, 99)), max=float(lat_ms.max()),
@@ -320,6 +330,8 @@ def main():
                     help='largest front order whose 32-wide panels use the one-kernel panel LU (-1: default)')
     ap.add_argument('--tri-lds', type=int, default=-1,
                     help='LDS bytes for the factor block of the LDS-staged solves (-1: default, 0: off)')
+    ap.add_argument('--no-autotune', action='store_true',
+                    help='keep the library defaults of the kernel-selection knobs (no tuning run)')
     ap.add_argument('--copy-mib', type=int, default=0,
                     help='only run a plain device copy of this many MiB (HBM counter calibration) and exit')
     args = ap.parse_args()
@@ -372,6 +384,14 @@ def main():
     from sawsim_amd.sweep import hbm_available
     log(f'[bench] batch {batch} x {nchunk} chunks, arena {per_freq * batch / 1e9:.1f} GB, chunk '
         f'{eng.chunk_bytes(batch, "mixed", True) / 1e9:.1f} GB of {hbm_available() / 1e9:.1f} GB available')
+
+    tuning = None
+    if not args.no_autotune:
+        from sawsim_amd.tune import autotune
+        t0 = time.time()
+        tuning = autotune(eng, freqs, 'mixed', gradient=True, batch=batch)
+        log(f"[bench] autotune {time.time() - t0:.1f}s: {tuning['choice']} "
+            f"({ {k: {c: round(v, 1) for c, v in d.items()} for k, d in tuning['ms'].items()} } ms)")
 
     def sweep():
         return eng.sweep_charges(freqs, precision='mixed', batch=batch, gradient=True)
@@ -457,7 +477,8 @@ def main():
                              f'{P} frequency points, batch {batch}',
                     points=P, n_active=eng.n_active, fronts=eng.mf.nf, batch=batch,
                     gradient=f'dY11/dp for {eng.n_params} parameters (3m constants, Euler angles, beta_dK, '
-                             f'tan delta{", rho" if hex_cell else ""}) by adjoint'),
+                             f'tan delta{", rho" if hex_cell else ""}) by adjoint',
+                    tuning=tuning),
         y11_max_rel_err_vs_fp64=y_err,
         y11_check_points=int(len(idx)),
         fr_fa_vs_fp64=frfa,

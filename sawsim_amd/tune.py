@@ -1,0 +1,101 @@
+"""Kernel-selection tuner of the batched multifrontal solver.
+
+The plan has three kernel-selection knobs whose best setting depends on the cell's front sizes
+and on the chunk size:
+
+* ``panel_lu`` (factorization): 32-wide pivot panels of size buckets up to this front order are
+  factored by one kernel per panel instead of the diagonal-block kernel and two MFMA GEMMs;
+* ``solve_full`` (solves): fronts up to this order are solved by one workgroup each;
+* ``tri_lds`` (solves): one-workgroup buckets whose factor block fits this many bytes are solved
+  from LDS by persistent workgroups.
+
+:func:`autotune` times the candidates on one chunk of the actual sweep, with HIP events on the
+launch stream, one knob at a time (coordinate descent: the factorization knob on the
+factorization, the solve knobs on a forward + backward pass with the chunk's right-hand sides),
+keeps the fastest setting of each, and records timings and choice in ``eng.tuning``.  The
+results of the solver do not depend on the knobs beyond rounding (the GPU tests compare the
+paths); only the time does.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+
+CANDIDATES = {
+    'panel_lu': (0, 128, 256, 512),
+    'solve_full': (128, 192, 256, 384),
+    'tri_lds': (0, 16384, 32768),
+}
+_SETTERS = {'panel_lu': 'set_panel_lu', 'solve_full': 'set_solve_full', 'tri_lds': 'set_tri_lds'}
+
+
+def _defaults():
+    from . import sweep
+    return dict(panel_lu=sweep.PANEL_LU_M, solve_full=sweep.SOLVE_FULL_M, tri_lds=sweep.TRI_LDS_BYTES)
+
+
+def autotune(eng, freqs, precision='mixed', gradient=True, batch=None, reps=3, candidates=None, margin=0.02):
+    """Pick the fastest setting of each knob for chunks of the sweep over ``freqs`` (Hz).
+
+    ``batch``: frequencies of the timed chunk (default: what one chunk of the sweep holds).  A
+    candidate replaces the library default only when it is faster by more than ``margin`` (run to
+    run noise of these timings is about 1 %).  Returns ``dict(choice=..., ms=...)`` (also kept in
+    ``eng.tuning``); the engine is left set to the choice.
+    """
+    from .piezo_fem import stream_ptr
+    torch = eng.torch
+    cand = dict(CANDIDATES, **(candidates or {}))
+    freqs = np.asarray(freqs, dtype=float).ravel()
+    B = int(min(freqs.size, batch or eng.auto_batch(precision, gradient)))
+    B = -(-freqs.size // -(-freqs.size // B))           # the equal chunks sweep_charges uses
+    prec = 0 if precision == 'mixed' else 1
+    nb = 64 if prec == 0 else 32
+    R = 2 if gradient else 1
+    n = eng.n_active
+    om = torch.as_tensor(2 * np.pi * freqs[:B], device=eng.dev)
+    arena = eng._arena(eng.mf.arena_elems * B, torch.complex64 if prec == 0 else torch.complex128)
+    uarena = torch.empty(max(1, eng.mf.uarena_elems * B * R), dtype=torch.complex128, device=eng.dev)
+    gen = torch.Generator(device=eng.dev).manual_seed(0)
+    x0 = torch.randn((B, R, n), dtype=torch.complex128, device=eng.dev, generator=gen)
+    x = torch.empty_like(x0)
+
+    def timed(fn):
+        best = None
+        for _ in range(reps + 1):                     # the first call warms up (work lists, code)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            ms = a.elapsed_time(b)
+            best = ms if best is None or ms < best else best
+        return best
+
+    def factor():
+        _lib.call('sawfem_mf_factor', eng._plan, stream_ptr(), prec, B, om.data_ptr(), eng.beta, eng.Kaa.data_ptr(),
+                  eng.Maa.data_ptr(), eng._dptr(eng.Daa), arena.data_ptr(), nb)
+
+    def solve():
+        x.copy_(x0)
+        _lib.call('sawfem_mf_solve', eng._plan, stream_ptr(), prec, B, R, arena.data_ptr(), uarena.data_ptr(),
+                  x.data_ptr(), 0, nb, None, 0)
+
+    choice, ms = {}, {}
+    default = _defaults()
+    for knob, fn in (('panel_lu', factor), ('solve_full', solve), ('tri_lds', solve)):
+        setter = getattr(eng, _SETTERS[knob])
+        ms[knob] = {}
+        values = list(dict.fromkeys((default[knob],) + tuple(cand[knob])))
+        for v in values:
+            setter(v)
+            ms[knob][v] = timed(fn)
+        best = min(ms[knob], key=ms[knob].get)
+        choice[knob] = best if ms[knob][best] < (1 - margin) * ms[knob][default[knob]] else default[knob]
+        setter(choice[knob])
+        if knob == 'panel_lu':
+            factor()                                 # the arena the solve knobs are timed on
+    torch.cuda.synchronize()
+    eng.pivot_stats(reset=True)                      # the tuning factorizations are not the sweep's
+    eng.tuning = dict(choice=choice, ms=ms, batch=B)
+    return eng.tuning
