@@ -2,9 +2,16 @@
  *
  * Plain C ABI: device pointers are passed as void*, sizes as int/int64_t, the stream as
  * a hipStream_t cast to void*.  Complex values are interleaved (re, im) doubles
- * ("complex128") unless stated.  Every function returns 0 on success and a non-zero
- * status otherwise (1 invalid argument, 3 HIP runtime error); sawfem_last_error()
- * returns the thread-local message of the last failure.
+ * ("complex128") unless stated.  Status codes (int return):
+ *   0 success
+ *   1 invalid argument
+ *   2 plan limit exceeded (front count or order beyond what the plan encodes)
+ *   3 HIP runtime error
+ *   4 out of device memory (an allocation failed, or a caller workspace is too small)
+ *   5 non-finite result (sawfem_sweep_chunk: a charge is NaN or infinite)
+ *   6 success, with the complex128 refactorization used for some frequencies
+ *     (sawfem_sweep_chunk; which ones is in its flags output)
+ * sawfem_last_error() returns the thread-local message of the last failure.
  *
  * Reference interfaces replaced (SawSim, Python):
  *   saw2d/piezo_fem.py:283-367 assemble / :373-405 assemble_ME0  -> sawfem_elem_q9 + sawfem_gather_kmd
@@ -18,6 +25,7 @@
  *   saw2d/sweep.py:337-342 rhs = -(Ks - w^2 Ms)[active] V          -> sawfem_rhs
  *   saw2d/sweep.py:362 Q = kts . disp                               -> sawfem_charge
  *   (no counterpart; refinement against the FP64 operator)          -> sawfem_residual, sawfem_axpy
+ *   saw2d/sweep.py:335-364 FrequencySolver.solve (whole chunk)     -> sawfem_sweep_chunk
  */
 #ifndef SAWFEM_H
 #define SAWFEM_H
@@ -196,6 +204,40 @@ int sawfem_adjoint_elem(void* stream, int ne0, const int* elems, const int* conn
                         const int* elem_tag, int npml, const int* pml_tag, const int* pml_kind, const double* pml_par,
                         double omega_top, int ndu, int N, const int* pos_rep, const unsigned char* zmask, double V,
                         double scale, int B, int n, const void* x, const void* lam, void* work, void* W);
+
+/* Active operator of one cell for sawfem_sweep_chunk (all pointers device memory):
+ * A(w) = Kaa + i w beta Daa - w^2 Maa on the CSR pattern (indptr, indices) of the n active
+ * unknowns; rhs(w) = -V (rK + i w beta rD - w^2 rM); Q = kts_a . x + V kts_z. */
+typedef struct sawfem_operator {
+  int n;
+  const int64_t* indptr;
+  const int* indices;
+  const void* Kaa;
+  const void* Maa;
+  const void* Daa;        /* NULL without damping (then rD NULL too) */
+  const void* rK;
+  const void* rM;
+  const void* rD;
+  const void* kts_a;
+  double kts_z_re, kts_z_im;
+  double beta;
+} sawfem_operator;
+
+/* Bytes of the workspace sawfem_sweep_chunk needs for B frequencies in precision prec. */
+int64_t sawfem_sweep_chunk_bytes(void* plan, int n, int B, int prec);
+
+/* Q(w_b) of B frequencies (omega: host, rad/s): factor (prec 0 complex64 / 1 complex128),
+ * right-hand sides, solve, and refinement against the FP64 operator until the estimated
+ * relative error of each Q is <= ir_tol (at most max_ir corrections); with fallback, the
+ * complex64 frequencies that do not converge are refactored and refined in complex128.  Q (host,
+ * 2B doubles re/im), err (host, B: estimated relative error), flags (host, B: bit 0 refactored
+ * in complex128, bit 1 not converged in the last precision tried).  Returns 0, 6 (done, some
+ * frequencies refactored), 5 (a charge is not finite; outputs filled) or an error status.
+ * workspace: device memory of at least sawfem_sweep_chunk_bytes(plan, n, B, prec) bytes.
+ * Replaces FrequencySolver.solve of saw2d/sweep.py:335-364 for a whole chunk of frequencies. */
+int sawfem_sweep_chunk(void* plan, void* stream, const sawfem_operator* op, int B, const double* omega, double V,
+                       int prec, double ir_tol, int max_ir, int fallback, void* workspace, int64_t workspace_bytes,
+                       double* Q, double* err, int* flags);
 
 /* sawfem_adjoint_elem without the work argument: allocates nothing.  No counterpart in the
  * reference; for callers that cannot allocate device memory (code under HIP graph capture, tight

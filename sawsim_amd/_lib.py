@@ -48,10 +48,23 @@ SIGNATURES = {
     'sawfem_axpy': [_vp, _i64, _vp, _vp],
     'sawfem_adjoint_elem': [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _i, _vp, _vp, _d, _d, _i, _i,
                             _vp, _vp, _vp, _vp],
+    'sawfem_sweep_chunk_bytes': [_vp, _i, _i, _i],
+    'sawfem_sweep_chunk': [_vp, _vp, _vp, _i, _vp, _d, _i, _d, _i, _i, _vp, _i64, _vp, _vp, _vp],
     'sawfem_adjoint_elem_noalloc': [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _i, _vp, _vp, _d, _d, _i,
                                     _i, _vp, _vp, _vp],
 }
-_RESTYPE = {'sawfem_last_error': C.c_char_p, 'sawfem_mf_arena_elems': _i64, 'sawfem_mf_uarena_elems': _i64}
+_RESTYPE = {'sawfem_last_error': C.c_char_p, 'sawfem_mf_arena_elems': _i64, 'sawfem_mf_uarena_elems': _i64,
+            'sawfem_sweep_chunk_bytes': _i64}
+
+# status codes of the C ABI (include/sawfem.h)
+OK, INVALID, PLAN_LIMIT, HIP_ERROR, OUT_OF_MEMORY, NON_FINITE, REFACTORED = 0, 1, 2, 3, 4, 5, 6
+
+
+class SweepOperator(C.Structure):
+    """``sawfem_operator``: the active operator of one cell (device pointers)."""
+    _fields_ = [('n', _i), ('indptr', _vp), ('indices', _vp), ('Kaa', _vp), ('Maa', _vp), ('Daa', _vp),
+                ('rK', _vp), ('rM', _vp), ('rD', _vp), ('kts_a', _vp), ('kts_z_re', _d), ('kts_z_im', _d),
+                ('beta', _d)]
 
 _lib = None
 

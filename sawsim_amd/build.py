@@ -10,7 +10,7 @@ import sys
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SOURCES = ['csrc/assemble.hip', 'csrc/mf.hip', 'csrc/adjoint.hip', 'csrc/h27.hip']
+SOURCES = ['csrc/assemble.hip', 'csrc/mf.hip', 'csrc/adjoint.hip', 'csrc/h27.hip', 'csrc/refine.hip']
 OUT = HERE / 'libsawfem_hip.so'
 
 FLAGS = ['--offload-arch=gfx950', '-O3', '-std=c++17', '-fPIC', '-shared',

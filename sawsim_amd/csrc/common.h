@@ -39,8 +39,12 @@ template <> __device__ __forceinline__ double2 from_d2<double2>(double2 a) { ret
 // thread-local last error for the C ABI
 const char* sawfem_set_error(const char* msg);
 
-#define SAW_HIP(call)                                                        \
-  do {                                                                       \
-    hipError_t e_ = (call);                                                  \
-    if (e_ != hipSuccess) { sawfem_set_error(hipGetErrorString(e_)); return 3; } \
+// status 4 for a failed device allocation, 3 for any other HIP runtime error
+#define SAW_HIP(call)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) {                                                                   \
+      sawfem_set_error(hipGetErrorString(e_));                                                \
+      return e_ == hipErrorOutOfMemory ? 4 : 3;                                               \
+    }                                                                                         \
   } while (0)

@@ -374,6 +374,42 @@ class FrequencySolver:
         other = 16 * self.mf.uarena_elems * R * batch + 16 * n * vecs + work
         return int(self.arena_bytes(batch, precision) + 1.1 * other)
 
+    def _workspace(self, nbytes):
+        """``nbytes`` of the kept workspace (grown like the front arena)."""
+        return self._arena(-(-int(nbytes) // 8), self.torch.complex64).view(self.torch.uint8)[:int(nbytes)]
+
+    def sweep_chunk_native(self, omegas, V=1.0, precision='mixed', ir_tol=1e-8, max_ir=8, fallback=True):
+        """One chunk of frequencies through the C ABI alone (``sawfem_sweep_chunk``: factor, solve,
+        refinement and the complex128 refactorization in the library).  Returns (Q complex (B,),
+        estimated relative error (B,), flags (B,): bit 0 refactored in complex128, bit 1 not
+        converged, status: 0, 5 non-finite charge or 6 some frequencies refactored)."""
+        import ctypes as C
+        om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64).ravel())
+        B = om.size
+        if B == 0:
+            raise ValueError('sweep_chunk_native: no frequencies')
+        if precision not in ('mixed', 'fp64'):
+            raise ValueError("precision must be 'mixed' or 'fp64', not %r" % (precision,))
+        prec = 0 if precision == 'mixed' else 1
+        op = _lib.SweepOperator(self.n_active, self.aa_indptr.data_ptr(), self.aa_indices.data_ptr(),
+                                self.Kaa.data_ptr(), self.Maa.data_ptr(), self._dptr(self.Daa), self.rK.data_ptr(),
+                                self.rM.data_ptr(), self._dptr(self.rD), self.kts_a.data_ptr(), self.kts_z.real,
+                                self.kts_z.imag, self.beta)
+        nbytes = _lib.load().sawfem_sweep_chunk_bytes(self._plan, self.n_active, B, prec)
+        if nbytes < 0:
+            raise _lib.SawfemError('sawfem_sweep_chunk_bytes failed')
+        ws = self._workspace(nbytes)
+        Q = np.empty(2 * B)
+        err = np.empty(B)
+        flags = np.zeros(B, dtype=np.int32)
+        rc = _lib.raw('sawfem_sweep_chunk', self._plan, stream_ptr(), C.byref(op), B, om.ctypes.data, float(V), prec,
+                      float(ir_tol), int(max_ir), int(bool(fallback)), ws.data_ptr(), int(nbytes), Q.ctypes.data,
+                      err.ctypes.data, flags.ctypes.data)
+        if rc not in (_lib.OK, _lib.NON_FINITE, _lib.REFACTORED):
+            msg = _lib.load().sawfem_last_error().decode(errors='replace')
+            raise _lib.SawfemError(f'sawfem_sweep_chunk failed with status {rc}: {msg}')
+        return Q.view(complex), err, flags, rc
+
     def release_workspace(self):
         """Free the arena workspace (it is otherwise kept for the next solve_batch)."""
         self._ws = None
