@@ -93,6 +93,84 @@ int sawfem_adjoint_h27(void* stream, int ne0, const int* elems, const int* conn,
 int sawfem_gather3(void* stream, int64_t nout, const int64_t* ptr, const int64_t* idx, const void* s0,
                    const void* s1, const void* s2, void* d0, void* d1, void* d2);
 
+/* Host symbolic analysis (no GPU call): nested-dissection ordering of the active system and the
+ * multifrontal plan arrays.  n active unknowns with pattern aa_indptr (n+1) / aa_indices (CSR,
+ * int64); sv_of[i] in [0, nxy) groups unknowns into supervariables (merged mesh nodes, or slab
+ * columns) with coordinates xy (nxy*2 doubles, x then y; the cuts are made on these), comp_of[i]
+ * orders the unknowns within one; period > 0 makes x periodic with that period (0: not
+ * periodic); leaf_weight is the largest sub-domain (in unknowns) that is not dissected further.
+ * The result is read with sawfem_host_size / sawfem_host_copy (element counts; int32 or int64
+ * arrays as listed) and freed with sawfem_host_destroy; sawfem_mf_create_sym builds the device
+ * plan from it.  Replaces the per-frequency analysis phase of PARDISO / SuperLU
+ * (sweep.py:326-333, 349-353), done once per cell here. */
+enum {
+  SAWFEM_SYM_M = 0,          /* int32 [nf]     front order */
+  SAWFEM_SYM_P = 1,          /* int32 [nf]     pivots of the front */
+  SAWFEM_SYM_ROWS_PTR = 2,   /* int64 [nf+1] */
+  SAWFEM_SYM_ROWS = 3,       /* int32          front rows (active ids), pivots first, elimination order */
+  SAWFEM_SYM_CHILD_PTR = 4,  /* int32 [nf+1] */
+  SAWFEM_SYM_CHILD_IDS = 5,  /* int32 */
+  SAWFEM_SYM_PARENT = 6,     /* int32 [nf]     -1 for a root */
+  SAWFEM_SYM_LEVEL = 7,      /* int32 [nf]     0 for a leaf */
+  SAWFEM_SYM_INV_PTR = 8,    /* int64          one map per (front, child) pair */
+  SAWFEM_SYM_INV = 9,        /* int32          parent row -> child contribution index or -1 */
+  SAWFEM_SYM_LVL_PTR = 10,   /* int32 [nlev+1] */
+  SAWFEM_SYM_LVL_FRONTS = 11,/* int32 [nf]     fronts by level, then id */
+  SAWFEM_SYM_ENT_FRONT = 12, /* int32 [nnz]    owner front of each A_aa entry, sorted */
+  SAWFEM_SYM_ENT_SLOT = 13,  /* int64 [nnz]    the entry's CSR position */
+  SAWFEM_SYM_ENT_LOC = 14,   /* int32 [nnz]    column-major offset in the owner front */
+  SAWFEM_SYM_ENT_LVL_PTR = 15,/* int64 [nlev+1] */
+  SAWFEM_SYM_PERM = 16,      /* int64 [n]      active unknowns in elimination order */
+  SAWFEM_SYM_PIV_SV_PTR = 17,/* int64 [nf+1] */
+  SAWFEM_SYM_PIV_SV = 18,    /* int64 [nsv]    supervariables in elimination order */
+  SAWFEM_SYM_ROWS_SV_PTR = 19,/* int64 [nf+1] */
+  SAWFEM_SYM_ROWS_SV = 20,   /* int64          front rows as supervariables */
+  SAWFEM_SYM_SV_IDS = 21     /* int64 [nsv]    sv_of value of each supervariable (ascending) */
+};
+int sawfem_analyse(void** sym, int64_t n, const int64_t* aa_indptr, const int64_t* aa_indices, const int64_t* sv_of,
+                   const int64_t* comp_of, int64_t nxy, const double* xy, double period, int64_t leaf_weight);
+int sawfem_mf_create_sym(void** plan, const void* sym);
+
+/* Host Bloch reduction layout (no GPU call): the structural pattern of
+ * A_aa = (T^T K T)[active, active] and the gather lists the device gathers use
+ * (sawfem_gather3), over the K slots of the structural pattern.  K's slots: node adjacency
+ * node_ptr (N+1) / node_idx, nd components per node, slot = a*nd*nnz_node + nd*node_ptr[i] +
+ * b*deg[i] + (k - node_ptr[i]) for component block (a, b) of node pair k of row node i; dof of
+ * (a, node) = a*N + node, or (interleaved != 0, nd = 4) 3*node + a for a < 3 and 3N + node.
+ * rep[d]: Bloch representative of dof d (rep[rep[d]] = rep[d]); pos[d]: active position of d or
+ * -1; is_zheng / is_fu: 0/1 flags of the driven-electrode and grounded dofs.  Output arrays
+ * (int64, read with sawfem_host_size / sawfem_host_copy, freed with sawfem_host_destroy):
+ *   AA_INDPTR / AA_INDICES  the A_aa pattern (CSR, columns ascending);
+ *   AA_PTR / AA_IDX         per A_aa entry, the K slots whose merged row and column map to it;
+ *   RHS_PTR / RHS_IDX       per active row, the K slots with merged row there and merged column
+ *                           on the electrode (the Km@s column, sweep.py:303);
+ *   KTS_PTR / KTS_IDX       charge weights on the unmerged K (sweep.py:305): per active position,
+ *                           slots of an electrode row whose merged column is that position (column
+ *                           neither electrode nor grounded); entry n_active: slots of an electrode
+ *                           row and electrode column.
+ * Every list is in ascending slot order.  Replaces the per-cell set-up of
+ * FrequencySolver.__init__ (sweep.py:283-314: union of slave into master rows and columns,
+ * active extraction). */
+enum {
+  SAWFEM_BLOCH_AA_INDPTR = 32,
+  SAWFEM_BLOCH_AA_INDICES = 33,
+  SAWFEM_BLOCH_AA_PTR = 34,
+  SAWFEM_BLOCH_AA_IDX = 35,
+  SAWFEM_BLOCH_RHS_PTR = 36,
+  SAWFEM_BLOCH_RHS_IDX = 37,
+  SAWFEM_BLOCH_KTS_PTR = 38,
+  SAWFEM_BLOCH_KTS_IDX = 39
+};
+int sawfem_bloch_layout(void** out, int nd, int64_t N, int interleaved, const int64_t* node_ptr,
+                        const int64_t* node_idx, const int64_t* rep, const int64_t* pos, int64_t n_active,
+                        const unsigned char* is_zheng, const unsigned char* is_fu);
+
+/* Arrays of a host result (sawfem_analyse, sawfem_bloch_layout): element count of array
+ * `what` (-1 for an id the result does not have), copy into dst, free the result. */
+int64_t sawfem_host_size(const void* res, int what);
+int sawfem_host_copy(const void* res, int what, void* dst);
+int sawfem_host_destroy(void* res);
+
 /* Multifrontal plan from the host symbolic analysis (all pointers are host memory). */
 int sawfem_mf_create(void** plan, int n, int nf, const int* m, const int* p, const int64_t* rows_ptr,
                      const int* rows, const int* child_ptr, const int* child_ids, const int64_t* inv_ptr,

@@ -9,6 +9,8 @@ import ctypes as C
 import os
 from pathlib import Path
 
+import numpy as np
+
 LIB_NAME = 'libsawfem_hip.so'
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
@@ -27,6 +29,12 @@ SIGNATURES = {
     'sawfem_mf_create': [C.POINTER(_vp), _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _i64, _vp,
                          _vp, _vp, _vp],
     'sawfem_mf_destroy': [_vp],
+    'sawfem_analyse': [C.POINTER(_vp), _i64, _vp, _vp, _vp, _vp, _i64, _vp, _d, _i64],
+    'sawfem_bloch_layout': [C.POINTER(_vp), _i, _i64, _i, _vp, _vp, _vp, _vp, _i64, _vp, _vp],
+    'sawfem_host_size': [_vp, _i],
+    'sawfem_host_copy': [_vp, _i, _vp],
+    'sawfem_host_destroy': [_vp],
+    'sawfem_mf_create_sym': [C.POINTER(_vp), _vp],
     'sawfem_mf_arena_elems': [_vp],
     'sawfem_mf_uarena_elems': [_vp],
     'sawfem_mf_perturbed': [_vp, _i],
@@ -54,7 +62,7 @@ SIGNATURES = {
                                     _i, _vp, _vp, _vp],
 }
 _RESTYPE = {'sawfem_last_error': C.c_char_p, 'sawfem_mf_arena_elems': _i64, 'sawfem_mf_uarena_elems': _i64,
-            'sawfem_sweep_chunk_bytes': _i64}
+            'sawfem_sweep_chunk_bytes': _i64, 'sawfem_host_size': _i64}
 
 # status codes of the C ABI (include/sawfem.h)
 OK, INVALID, PLAN_LIMIT, HIP_ERROR, OUT_OF_MEMORY, NON_FINITE, REFACTORED = 0, 1, 2, 3, 4, 5, 6
@@ -112,3 +120,16 @@ def call_value(name, *args):
 
 def raw(name, *args):
     return getattr(load(), name)(*args)
+
+
+def host_arrays(handle, spec):
+    """Copy the arrays ``spec`` = {name: (id, dtype)} out of a host result and free it."""
+    try:
+        out = {}
+        for name, (what, dt) in spec.items():
+            a = np.empty(call_value('sawfem_host_size', handle, what), dtype=dt)
+            call('sawfem_host_copy', handle, what, a.ctypes.data)
+            out[name] = a
+        return out
+    finally:
+        call('sawfem_host_destroy', handle)

@@ -23,9 +23,12 @@ Gather lists (``ptr``, ``idx``) drive a deterministic, atomic-free reduction:
 """
 from __future__ import annotations
 
+import ctypes as C
 from dataclasses import dataclass
 
 import numpy as np
+
+from . import _lib
 
 
 def _csr_from_sorted_keys(keys, nrows, ncols):
@@ -197,6 +200,9 @@ class BlochLayout:
     * ``kts``: charge weights on the *unmerged* K (``sweep.py:305``):
       ``Q = sum_p kts_a[p] x[p] + V * kts_z``, where disp[slave] = disp[master]
       and the zheng/fu values are written last (``sweep.py:356-362``).
+
+    The merge map and the active set are set up here; the pattern and the lists over the
+    K slots (millions of entries) are built by ``sawfem_bloch_layout`` (host C++).
     """
 
     def __init__(self, kl: KLayout, fix, zheng, fu, groups=None, rep=None):
@@ -224,29 +230,20 @@ class BlochLayout:
         is_z[self.zheng] = True
         is_f = np.zeros(gdof, dtype=bool)
         is_f[self.fu] = True
-        rows, cols = kl.slot_rows_cols()
-        mr, mc = rep[rows], rep[cols]
-        pr, pc = pos[mr], pos[mc]
-        # A_aa
-        sel = np.flatnonzero((pr >= 0) & (pc >= 0))
-        keys = pr[sel] * n + pc[sel]
-        ukeys, inv = np.unique(keys, return_inverse=True)
-        self.aa_indptr, self.aa_indices = _csr_from_sorted_keys(ukeys, n, n)
-        self.nnz_aa = int(ukeys.size)
-        ptr, order = _gather_lists(inv, self.nnz_aa)
-        self.aa = GatherList(ptr, sel[order])
-        # rhs column: merged row active, merged column in zheng
-        sel = np.flatnonzero((pr >= 0) & is_z[mc])
-        ptr, order = _gather_lists(pr[sel], n)
-        self.rhs = GatherList(ptr, sel[order])
-        # charge weights on unmerged K: row in zheng
-        zr = is_z[rows]
-        sel = np.flatnonzero(zr & (pc >= 0) & ~is_z[cols] & ~is_f[cols])
-        tgt = pc[sel]
-        selz = np.flatnonzero(zr & is_z[cols])
-        tgt = np.concatenate([tgt, np.full(selz.size, n)])
-        ptr, order = _gather_lists(tgt, n + 1)
-        self.kts = GatherList(ptr, np.concatenate([sel, selz])[order])
+        # A_aa pattern and the three gather lists, built natively (sawfem_bloch_layout)
+        rep = np.ascontiguousarray(rep, dtype=np.int64)
+        h = C.c_void_p()
+        _lib.call('sawfem_bloch_layout', C.byref(h), kl.nd, kl.N, int(kl.numbering == 'interleaved'),
+                  np.ascontiguousarray(kl.node_ptr, dtype=np.int64).ctypes.data,
+                  np.ascontiguousarray(kl.node_idx, dtype=np.int64).ctypes.data, rep.ctypes.data, pos.ctypes.data, n,
+                  is_z.view(np.uint8).ctypes.data, is_f.view(np.uint8).ctypes.data)
+        a = _lib.host_arrays(h, {k: (32 + i, np.int64) for i, k in enumerate(
+            ('aa_indptr', 'aa_indices', 'aa_ptr', 'aa_idx', 'rhs_ptr', 'rhs_idx', 'kts_ptr', 'kts_idx'))})
+        self.aa_indptr, self.aa_indices = a['aa_indptr'], a['aa_indices']
+        self.nnz_aa = int(self.aa_indices.size)
+        self.aa = GatherList(a['aa_ptr'], a['aa_idx'])
+        self.rhs = GatherList(a['rhs_ptr'], a['rhs_idx'])
+        self.kts = GatherList(a['kts_ptr'], a['kts_idx'])
         # merged-node view of the active unknowns (supervariables for the ordering)
         self.node_of, self.comp_of = kl.node_comp(self.active)
 

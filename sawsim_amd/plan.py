@@ -1,6 +1,6 @@
 """Host arrays of the multifrontal numeric plan (input of ``sawfem_mf_create``).
 
-From the symbolic analysis and the active pattern this builds, once per cell:
+``sawfem_analyse`` (``csrc/analyse.cpp``) builds them with the ordering, once per cell:
 
 * per front: order ``m``, pivots ``p``, row list (active ids, pivots first);
 * per (front, child): the map parent-row -> child contribution-block index (or -1),
@@ -15,8 +15,6 @@ from __future__ import annotations
 from dataclasses import dataclass
 
 import numpy as np
-
-from .symbolic import Symbolic
 
 
 @dataclass
@@ -56,70 +54,6 @@ class MFArrays:
     @property
     def uarena_elems(self):
         return int((self.m - self.p).sum())
-
-
-def build_mf_arrays(sym: Symbolic, aa_indptr: np.ndarray, aa_indices: np.ndarray) -> MFArrays:
-    F = sym.fronts
-    nf = len(F)
-    n = sym.n
-    m = np.array([f.m for f in F], dtype=np.int32)
-    p = np.array([f.p for f in F], dtype=np.int32)
-    rows_ptr = np.concatenate([[0], np.cumsum(m.astype(np.int64))])
-    rows = np.concatenate([f.rows for f in F]).astype(np.int32)
-    child_ptr = np.concatenate([[0], np.cumsum([len(f.children) for f in F])]).astype(np.int32)
-    child_ids = np.array([c for f in F for c in f.children], dtype=np.int32)
-    # parent-row -> child CB index maps
-    scratch = np.full(n, -1, dtype=np.int32)
-    inv_parts = []
-    for f in F:
-        for c in f.children:
-            g = F[c]
-            cb = g.rows[g.p:]
-            scratch[cb] = np.arange(cb.size, dtype=np.int32)
-            inv = scratch[f.rows].copy()
-            if (inv >= 0).sum() != cb.size:
-                raise AssertionError('child contribution rows missing from parent front')
-            inv_parts.append(inv)
-            scratch[cb] = -1
-    inv_ptr = np.concatenate([[0], np.cumsum([x.size for x in inv_parts])]).astype(np.int64)
-    inv = np.concatenate(inv_parts).astype(np.int32) if inv_parts else np.zeros(0, dtype=np.int32)
-    # levels
-    lvl_fronts = np.concatenate(sym.levels).astype(np.int32)
-    lvl_ptr = np.concatenate([[0], np.cumsum([lv.size for lv in sym.levels])]).astype(np.int32)
-    level_of = np.empty(nf, dtype=np.int64)
-    for L, lv in enumerate(sym.levels):
-        level_of[lv] = L
-    # entry ownership
-    owner = np.empty(n, dtype=np.int64)
-    for fid, f in enumerate(F):
-        owner[f.rows[:f.p]] = fid
-    rank = np.empty(n, dtype=np.int64)
-    rank[sym.perm] = np.arange(n)
-    r = np.repeat(np.arange(n, dtype=np.int64), np.diff(aa_indptr))
-    c = aa_indices.astype(np.int64)
-    own = np.where(rank[r] <= rank[c], owner[r], owner[c])
-    # local index lookup table keyed by front * n + active id
-    fk = np.repeat(np.arange(nf, dtype=np.int64), m.astype(np.int64)) * n + rows.astype(np.int64)
-    lk = np.concatenate([np.arange(x, dtype=np.int64) for x in m])
-    o = np.argsort(fk)
-    fk, lk = fk[o], lk[o]
-
-    def local(front, ids):
-        key = front * n + ids
-        pos = np.searchsorted(fk, key)
-        if not np.all(fk[np.minimum(pos, fk.size - 1)] == key):
-            raise AssertionError('A_aa entry outside its owner front')
-        return lk[pos]
-    li, lj = local(own, r), local(own, c)
-    loc = li + lj * m[own].astype(np.int64)
-    order = np.lexsort((loc, own, level_of[own]))      # by level, front, column-major offset
-    ent_front = own[order].astype(np.int32)
-    ent_slot = np.arange(r.size, dtype=np.int64)[order]
-    ent_loc = loc[order].astype(np.int32)
-    ent_lvl_ptr = np.searchsorted(level_of[ent_front], np.arange(len(sym.levels) + 1)).astype(np.int64)
-    return MFArrays(n=n, m=m, p=p, rows_ptr=rows_ptr, rows=rows, child_ptr=child_ptr, child_ids=child_ids,
-                    inv_ptr=inv_ptr, inv=inv, lvl_ptr=lvl_ptr, lvl_fronts=lvl_fronts, ent_front=ent_front,
-                    ent_slot=ent_slot, ent_loc=ent_loc, ent_lvl_ptr=ent_lvl_ptr)
 
 
 def panel_width(nb: int, level_pmax: int) -> int:

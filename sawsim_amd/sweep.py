@@ -34,7 +34,6 @@ import numpy as np
 from . import _lib
 from .layout import BlochLayout
 from .piezo_fem import DeviceCSR, Model, _torch, stream_ptr, timed_call
-from .plan import build_mf_arrays
 from .symbolic import analyse
 
 
@@ -164,7 +163,7 @@ class FrequencySolver:
         t0 = time.time()
         self.symbolic = analyse(bl.aa_indptr, bl.aa_indices, sv_of, bl.comp_of, np.asarray(sv_xy),
                                 period=period, leaf_weight=leaf_weight)
-        self.mf = build_mf_arrays(self.symbolic, bl.aa_indptr, bl.aa_indices)
+        self.mf = self.symbolic.mf
         self.analysis_seconds = time.time() - t0
         self._plan = self._create_plan()
         self.set_solve_full(SOLVE_FULL_M)

@@ -8,7 +8,7 @@ from oracle import sweep as osw
 from sawsim_amd import materials as mt, workloads
 from sawsim_amd.layout import BlochLayout, KLayout
 from sawsim_amd.piezo_fem import Model
-from sawsim_amd.plan import build_mf_arrays, kernel_work
+from sawsim_amd.plan import kernel_work
 from sawsim_amd.symbolic import analyse
 from sawsim_amd.sweep import bc_periodic
 
@@ -78,7 +78,7 @@ def test_plan_arrays_drive_a_correct_factorization(nb):
     kl = KLayout(m.quads, m.N, 4)
     bl = BlochLayout(kl, *bc_periodic(m, cell.mesh.lines, ndisp=3))
     sym = analyse(bl.aa_indptr, bl.aa_indices, bl.node_of, bl.comp_of, m.pos_m, period=2.17e-6, leaf_weight=60)
-    mf = build_mf_arrays(sym, bl.aa_indptr, bl.aa_indices)
+    mf = sym.mf
     r = np.repeat(np.arange(bl.n_active), np.diff(bl.aa_indptr))
     Kaa = np.asarray(eng.Km_aa[r, bl.aa_indices]).ravel()
     Maa = np.asarray(eng.Mm_aa[r, bl.aa_indices]).ravel()
@@ -126,7 +126,7 @@ def _planted_zero_pivot_system():
     kl = KLayout(m.quads, m.N, 4)
     bl = BlochLayout(kl, *bc_periodic(m, cell.mesh.lines, ndisp=3))
     sym = analyse(bl.aa_indptr, bl.aa_indices, bl.node_of, bl.comp_of, m.pos_m, period=2.17e-6, leaf_weight=60)
-    mf = build_mf_arrays(sym, bl.aa_indptr, bl.aa_indices)
+    mf = sym.mf
     r = np.repeat(np.arange(bl.n_active), np.diff(bl.aa_indptr))
     Kaa = np.asarray(eng.Km_aa[r, bl.aa_indices]).ravel()
     Maa = np.asarray(eng.Mm_aa[r, bl.aa_indices]).ravel()
