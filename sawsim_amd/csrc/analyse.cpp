@@ -488,30 +488,9 @@ struct Analysis : HostResult {
 
 // Bloch reduction of the structural K pattern to the active system (sweep.py:283-314): the
 // A_aa pattern of (T^T K T)[active, active] and three gather lists over K slots (see
-// sawfem_bloch_layout).  Slots are enumerated in ascending order, so every list is in
-// ascending slot order within its target.
+// sawfem_bloch_layout), every list in ascending slot order within its target.
 struct Bloch : HostResult {
   vector<i64> aa_indptr, aa_indices, aa_ptr, aa_idx, rhs_ptr, rhs_idx, kts_ptr, kts_idx;
-
-  // calls f(slot, row dof, column dof) for every structural slot, slots ascending: slot =
-  // a*nd*nnz_node + nd*node_ptr[i] + b*deg[i] + (k - node_ptr[i]) for component block (a, b)
-  // of node pair k in row node i
-  template <typename F>
-  static void slots(int nd, i64 N, bool interleaved, const i64* node_ptr, const i64* node_idx, F&& f) {
-    const i64 nnz_node = node_ptr[N];
-    auto dof = [&](int a, i64 node) -> i64 {
-      if (!interleaved) return a * N + node;
-      return a < 3 ? 3 * node + a : 3 * N + node;
-    };
-    for (int a = 0; a < nd; ++a)
-      for (i64 i = 0; i < N; ++i) {
-        const i64 k0 = node_ptr[i], deg = node_ptr[i + 1] - k0, row = dof(a, i);
-        for (int b = 0; b < nd; ++b) {
-          const i64 s0 = a * nd * nnz_node + nd * k0 + b * deg;
-          for (i64 k = 0; k < deg; ++k) f(s0 + k, row, dof(b, node_idx[k0 + k]));
-        }
-      }
-  }
 
   int build(int nd, i64 N, bool interleaved, const i64* node_ptr, const i64* node_idx, const i64* rep,
             const i64* pos, i64 n, const unsigned char* is_z, const unsigned char* is_f) {
@@ -523,64 +502,95 @@ struct Bloch : HostResult {
       if (node_ptr[i + 1] < node_ptr[i]) return fail(1, "sawfem_bloch_layout: node_ptr not ascending");
     for (i64 k = 0; k < node_ptr[N]; ++k)
       if (node_idx[k] < 0 || node_idx[k] >= N) return fail(1, "sawfem_bloch_layout: node index out of range");
-    auto each = [&](auto&& f) { slots(nd, N, interleaved, node_ptr, node_idx, f); };
-    // slots of each merged active row, ascending
-    vector<i64> rcnt(n + 1, 0);
-    rhs_ptr.assign(n + 1, 0);
-    kts_ptr.assign(n + 2, 0);
-    each([&](i64, i64 r, i64 c) {
-      const i64 pr = pos[rep[r]], pc = pos[rep[c]];
-      if (pr >= 0 && pc >= 0) ++rcnt[pr + 1];
-      if (pr >= 0 && is_z[rep[c]]) ++rhs_ptr[pr + 1];
-      if (is_z[r]) {
-        if (pc >= 0 && !is_z[c] && !is_f[c]) ++kts_ptr[pc + 1];
-        else if (is_z[c]) ++kts_ptr[n + 1];
-      }
-    });
-    for (i64 j = 0; j < n; ++j) rcnt[j + 1] += rcnt[j], rhs_ptr[j + 1] += rhs_ptr[j];
-    for (i64 j = 0; j <= n; ++j) kts_ptr[j + 1] += kts_ptr[j];
-    vector<i64> bs(rcnt[n]), bc(rcnt[n]);
-    rhs_idx.resize(rhs_ptr[n]);
-    kts_idx.resize(kts_ptr[n + 1]);
+    const i64 nnz_node = node_ptr[N];
+    // K row d = (component a, node i); its slots are one ascending range: block (a, b) of node
+    // pair k at a*nd*nnz_node + nd*node_ptr[i] + b*deg[i] + (k - node_ptr[i])
+    auto comp = [&](i64 d) -> int {
+      if (!interleaved) return (int)(d / N);
+      return d < 3 * N ? (int)(d % 3) : 3;
+    };
+    auto node = [&](i64 d) -> i64 {
+      if (!interleaved) return d % N;
+      return d < 3 * N ? d / 3 : d - 3 * N;
+    };
+    auto dof = [&](int a, i64 nd_) -> i64 {
+      if (!interleaved) return a * N + nd_;
+      return a < 3 ? 3 * nd_ + a : 3 * N + nd_;
+    };
+    auto start = [&](i64 d) { return comp(d) * nd * nnz_node + nd * node_ptr[node(d)]; };
+    // calls f(slot, column dof) for the slots of K row d, ascending
+    auto row_slots = [&](i64 d, auto&& f) {
+      const i64 i = node(d), k0 = node_ptr[i], deg = node_ptr[i + 1] - k0, s0 = start(d);
+      for (int b = 0; b < nd; ++b)
+        for (i64 k = 0; k < deg; ++k) f(s0 + b * deg + k, dof(b, node_idx[k0 + k]));
+    };
+    // the K rows merged into each active row, by ascending slot range
+    vector<i64> rptr(n + 1, 0), rrows;
+    for (i64 d = 0; d < gdof; ++d)
+      if (pos[rep[d]] >= 0) ++rptr[pos[rep[d]] + 1];
+    for (i64 j = 0; j < n; ++j) rptr[j + 1] += rptr[j];
+    rrows.resize(rptr[n]);
     {
-      vector<i64> fa(rcnt.begin(), rcnt.end() - 1), fr(rhs_ptr.begin(), rhs_ptr.end() - 1),
-          fk(kts_ptr.begin(), kts_ptr.end() - 1);
-      each([&](i64 s, i64 r, i64 c) {
-        const i64 pr = pos[rep[r]], pc = pos[rep[c]];
-        if (pr >= 0 && pc >= 0) {
-          const i64 t = fa[pr]++;
-          bs[t] = s;
-          bc[t] = pc;
-        }
-        if (pr >= 0 && is_z[rep[c]]) rhs_idx[fr[pr]++] = s;
-        if (is_z[r]) {
-          if (pc >= 0 && !is_z[c] && !is_f[c]) kts_idx[fk[pc]++] = s;
-          else if (is_z[c]) kts_idx[fk[n]++] = s;
-        }
-      });
+      vector<i64> fill(rptr.begin(), rptr.end() - 1);
+      for (i64 d = 0; d < gdof; ++d)
+        if (pos[rep[d]] >= 0) rrows[fill[pos[rep[d]]]++] = d;
     }
-    // per active row: distinct merged columns ascending, each with its slots ascending
-    vector<i64> mark(n, -1), rk(n, 0), cols, cnt;
+    // merged active position and electrode flag of every column dof
+    vector<i64> prep(gdof);
+    vector<unsigned char> zrep(gdof);
+    for (i64 d = 0; d < gdof; ++d) prep[d] = pos[rep[d]], zrep[d] = is_z[rep[d]];
+    // per active row: its slots ascending with their merged columns; the distinct columns
+    // ascending, each with its slots; the electrode-column slots
+    vector<i64> mark(n, -1), rk(n, 0), ents, entc, cols, cnt;
     aa_indptr.assign(n + 1, 0);
     aa_ptr.assign(1, 0);
-    aa_idx.resize(bs.size());
+    rhs_ptr.assign(n + 1, 0);
     for (i64 pr = 0; pr < n; ++pr) {
+      std::sort(rrows.begin() + rptr[pr], rrows.begin() + rptr[pr + 1],
+                [&](i64 a, i64 b) { return start(a) < start(b); });
+      ents.clear();
+      entc.clear();
       cols.clear();
-      for (i64 t = rcnt[pr]; t < rcnt[pr + 1]; ++t)
-        if (mark[bc[t]] != pr) mark[bc[t]] = pr, cols.push_back(bc[t]);
+      for (i64 t = rptr[pr]; t < rptr[pr + 1]; ++t)
+        row_slots(rrows[t], [&](i64 s, i64 c) {
+          const i64 pc = prep[c];
+          if (pc >= 0) {
+            ents.push_back(s);
+            entc.push_back(pc);
+            if (mark[pc] != pr) mark[pc] = pr, cols.push_back(pc);
+          }
+          if (zrep[c]) rhs_idx.push_back(s);
+        });
+      rhs_ptr[pr + 1] = (i64)rhs_idx.size();
       std::sort(cols.begin(), cols.end());
       cnt.assign(cols.size() + 1, 0);
       for (size_t q = 0; q < cols.size(); ++q) rk[cols[q]] = (i64)q;
-      for (i64 t = rcnt[pr]; t < rcnt[pr + 1]; ++t) ++cnt[rk[bc[t]] + 1];
+      for (i64 c : entc) ++cnt[rk[c] + 1];
       const i64 base = aa_ptr.back();
       for (size_t q = 0; q < cols.size(); ++q) {
         cnt[q + 1] += cnt[q];
         aa_ptr.push_back(base + cnt[q + 1]);
       }
-      for (i64 t = rcnt[pr]; t < rcnt[pr + 1]; ++t) aa_idx[base + cnt[rk[bc[t]]]++] = bs[t];
+      aa_idx.resize(base + (i64)ents.size());
+      for (size_t t = 0; t < ents.size(); ++t) aa_idx[base + cnt[rk[entc[t]]]++] = ents[t];
       aa_indices.insert(aa_indices.end(), cols.begin(), cols.end());
       aa_indptr[pr + 1] = (i64)aa_indices.size();
     }
+    // charge weights: slots of the (unmerged) electrode rows, by target then slot
+    vector<std::pair<i64, i64>> kt;
+    for (i64 d = 0; d < gdof; ++d) {
+      if (!is_z[d]) continue;
+      row_slots(d, [&](i64 s, i64 c) {
+        const i64 pc = pos[rep[c]];
+        if (pc >= 0 && !is_z[c] && !is_f[c]) kt.emplace_back(pc, s);
+        else if (is_z[c]) kt.emplace_back(n, s);
+      });
+    }
+    std::sort(kt.begin(), kt.end());
+    kts_ptr.assign(n + 2, 0);
+    kts_idx.resize(kt.size());
+    for (size_t t = 0; t < kt.size(); ++t) ++kts_ptr[kt[t].first + 1], kts_idx[t] = kt[t].second;
+    for (i64 j = 0; j <= n; ++j) kts_ptr[j + 1] += kts_ptr[j];
     return 0;
   }
 
