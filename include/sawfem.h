@@ -165,6 +165,12 @@ int sawfem_bloch_layout(void** out, int nd, int64_t N, int interleaved, const in
                         const int64_t* node_idx, const int64_t* rep, const int64_t* pos, int64_t n_active,
                         const unsigned char* is_zheng, const unsigned char* is_fu);
 
+/* Host (no GPU call): tslot[e] = position of the transposed entry (c, r) of every entry e =
+ * (r, c) of a structurally symmetric CSR pattern with ascending columns in every row (status 1
+ * otherwise).  Used by the transposed residual and the paired adjoint; no counterpart in the
+ * reference (it solves the adjoint nowhere). */
+int sawfem_transpose_slots(int64_t n, const int64_t* indptr, const int64_t* indices, int64_t* tslot);
+
 /* Arrays of a host result (sawfem_analyse, sawfem_bloch_layout): element count of array
  * `what` (-1 for an id the result does not have), copy into dst, free the result. */
 int64_t sawfem_host_size(const void* res, int what);

@@ -31,6 +31,7 @@ SIGNATURES = {
     'sawfem_mf_destroy': [_vp],
     'sawfem_analyse': [C.POINTER(_vp), _i64, _vp, _vp, _vp, _vp, _i64, _vp, _d, _i64],
     'sawfem_bloch_layout': [C.POINTER(_vp), _i, _i64, _i, _vp, _vp, _vp, _vp, _i64, _vp, _vp],
+    'sawfem_transpose_slots': [_i64, _vp, _vp, _vp],
     'sawfem_host_size': [_vp, _i],
     'sawfem_host_copy': [_vp, _i, _vp],
     'sawfem_host_destroy': [_vp],

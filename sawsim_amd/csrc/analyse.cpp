@@ -662,6 +662,23 @@ int sawfem_bloch_layout(void** out, int nd, int64_t N, int interleaved, const in
   return 0;
 }
 
+int sawfem_transpose_slots(int64_t n, const int64_t* indptr, const int64_t* indices, int64_t* tslot) {
+  if (n <= 0 || !indptr || !indices || !tslot) return fail(1, "sawfem_transpose_slots: invalid argument");
+  // rows are visited in ascending order, so the visits to column c meet row c's entries in
+  // their (ascending) column order: a cursor per row finds each transposed entry
+  vector<i64> cur(indptr, indptr + n);
+  for (i64 r = 0; r < n; ++r)
+    for (i64 e = indptr[r]; e < indptr[r + 1]; ++e) {
+      const i64 c = indices[e];
+      if (c < 0 || c >= n) return fail(1, "sawfem_transpose_slots: column index out of range");
+      const i64 q = cur[c]++;
+      if (q >= indptr[c + 1] || indices[q] != r)
+        return fail(1, "sawfem_transpose_slots: pattern is not structurally symmetric with ascending columns");
+      tslot[e] = q;
+    }
+  return 0;
+}
+
 int64_t sawfem_host_size(const void* res, int what) {
   if (!res) return -1;
   return static_cast<const HostResult*>(res)->item(what).count;

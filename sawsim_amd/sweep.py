@@ -281,12 +281,11 @@ class FrequencySolver:
         t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device=self.dev)
         n = self.n_active
         r = np.repeat(np.arange(n, dtype=np.int64), np.diff(bl.aa_indptr))
-        c = bl.aa_indices.astype(np.int64)
-        key = r * n + c
-        tkey = c * n + r
-        self.tslot = t(np.searchsorted(key, tkey).astype(np.int64))
-        if not np.array_equal(key[np.searchsorted(key, tkey)], tkey):
-            raise ValueError('active pattern is not structurally symmetric')
+        ip = np.ascontiguousarray(bl.aa_indptr, dtype=np.int64)
+        ix = np.ascontiguousarray(bl.aa_indices, dtype=np.int64)
+        tslot = np.empty(ix.size, dtype=np.int64)
+        _lib.call('sawfem_transpose_slots', n, ip.ctypes.data, ix.ctypes.data, tslot.ctypes.data)
+        self.tslot = t(tslot)
         is_phi = bl.comp_of == (self.layout.nd - 1)
         self._dscale = t(np.where(is_phi, 1.0 / scale, 1.0))
         self._arow = t(r)

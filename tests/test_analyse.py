@@ -151,3 +151,24 @@ def test_native_bloch_layout_rejects_bad_input():
                   bl.rep.ctypes.data, pos.ctypes.data, bl.n_active, z.ctypes.data, z.ctypes.data)
     assert rc == _lib.INVALID and b'out of range' in _lib.load().sawfem_last_error()
     assert _lib.raw('sawfem_bloch_layout', C.byref(h), 4, kl.N, 1, None, None, None, None, 0, None, None) == 1
+
+
+def test_transpose_slots_matches_sorted_search():
+    """sawfem_transpose_slots against a search of the sorted (row, column) keys, and its
+    rejection of a pattern that is not structurally symmetric."""
+    (ip, ix, _, _, _), _ = _q9_case(workloads.double_layer(), 4)
+    n = ip.size - 1
+    r = np.repeat(np.arange(n, dtype=np.int64), np.diff(ip))
+    key, tkey = r * n + ix, ix * n + r
+    want = np.searchsorted(key, tkey)
+    assert np.array_equal(key[want], tkey)
+    got = np.empty(ix.size, dtype=np.int64)
+    _lib.call('sawfem_transpose_slots', n, ip.ctypes.data, ix.ctypes.data, got.ctypes.data)
+    assert np.array_equal(got, want)
+    # drop one off-diagonal entry: its transpose has no partner
+    e = int(np.flatnonzero(r != ix)[5])
+    ip2 = ip.copy()
+    ip2[r[e] + 1:] -= 1
+    ix2 = np.delete(ix, e)
+    out = np.empty(ix2.size, dtype=np.int64)
+    assert _lib.raw('sawfem_transpose_slots', n, ip2.ctypes.data, ix2.ctypes.data, out.ctypes.data) == _lib.INVALID
