@@ -171,6 +171,13 @@ def assembly_roofline(eng, cell, reps=3):
     return out
 
 
+def _mem():
+    import torch
+    free, total = torch.cuda.mem_get_info()
+    return (f'free {free / 1e9:.1f} of {total / 1e9:.1f} GB, torch allocated {torch.cuda.memory_allocated() / 1e9:.1f} '
+            f'reserved {torch.cuda.memory_reserved() / 1e9:.1f} GB')
+
+
 def design_set(args):
     """Config 5: the design set of 64 seeded stack cells (2k-20k Q9 elements, 128 points each,
     Y11 + dY11/dp for 17 parameters), swept back to back on one shared HBM workspace after every
@@ -206,6 +213,8 @@ def design_set(args):
         extra[i] = engines[i].chunk_bytes(batches[i], 'mixed', True) - engines[i].arena_bytes(batches[i], 'mixed')
         ws_bytes = max(eng.arena_bytes(b, 'mixed') for eng, b in zip(engines, batches))
     check_workspace(ws_bytes + max(extra), what='shared front workspace and chunk vectors')
+    log(f'[bench] memory: {_mem()}; budget {budget / 1e9:.1f} GB, largest chunk {max(extra) / 1e9:.1f} GB '
+        f'beside the workspace')
     shared = torch.empty(ws_bytes, dtype=torch.uint8, device='cuda')
     for eng in engines:
         eng._ws = shared                       # every cell reuses the one front workspace
@@ -221,7 +230,7 @@ def design_set(args):
         for eng in engines:
             for knob, v in tuning['choice'].items():
                 getattr(eng, _SETTERS[knob])(v)
-        log(f"[bench] autotune on cell {big}: {tuning['choice']}")
+        log(f"[bench] autotune on cell {big}: {tuning['choice']}; memory: {_mem()}")
 
     def sweep(i):
         return engines[i].sweep_charges(cells[i].frequencies(), precision='mixed', batch=batches[i], gradient=True)
@@ -231,6 +240,7 @@ def design_set(args):
     for eng in engines:
         _lib.call('sawfem_mf_profile', eng._plan, 1)
     lat, results = [], []
+    torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     for i in range(len(cells)):
         t1 = time.perf_counter()
@@ -238,6 +248,7 @@ def design_set(args):
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t1)
     dt = time.perf_counter() - t0
+    log(f'[bench] peak torch allocation over the sweeps {torch.cuda.max_memory_allocated() / 1e9:.1f} GB; {_mem()}')
     schur_ms = schur_n = 0.0
     schur_flops = 0.0
     for eng, c in zip(engines, cells):

@@ -7,11 +7,11 @@
 // steps on the host (one Q read-back per step).
 //
 // Refinement (the same rule as the Python engine): after each correction, dq = |dQ| / |Q| per
-// frequency; with the observed rate r = dq / dq_prev < 1/2 the corrections still to come sum to
-// about dq r / (1 - r), otherwise the estimate is dq itself.  A frequency leaves the refinement
-// when that estimate is <= ir_tol; from the second correction on, one whose correction does not
-// shrink by half while above ir_tol has stalled.  Stalled and unconverged frequencies of a
-// complex64 chunk are refactored and refined in complex128.
+// frequency.  A frequency leaves the refinement when dq <= ir_tol; from the second correction
+// on, one whose correction does not shrink by half while above ir_tol has stalled.  Stalled and
+// unconverged frequencies of a complex64 chunk are refactored and refined in complex128.  The
+// reported error estimate extrapolates the observed rate r = dq / dq_prev: dq r / (1 - r) for
+// r < 1/2 (the corrections still to come), dq otherwise.
 #include "common.h"
 #include "../../include/sawfem.h"
 
@@ -134,13 +134,13 @@ int refine_pass(void* plan, hipStream_t s, const sawfem_operator* op, int B, con
       const double dq = std::hypot(dr, di) / std::max(std::hypot(qn[2 * b], qn[2 * b + 1]), 1e-300);
       const double eb = ir_error(dq, dqp[b]);
       if (act[b]) {
-        if (steps >= 2 && dq > 0.5 * dqp[b] && eb > ir_tol) stalled[b] = 1;
+        if (steps >= 2 && dq > 0.5 * dqp[b] && dq > ir_tol) stalled[b] = 1;
         e[b] = eb;
         dqp[b] = dq;
       }
       q[2 * b] = qn[2 * b];
       q[2 * b + 1] = qn[2 * b + 1];
-      act[b] = act[b] && e[b] > ir_tol && !stalled[b];
+      act[b] = act[b] && dqp[b] > ir_tol && !stalled[b];
       any = any || act[b];
     }
     if (!any) break;
@@ -149,7 +149,7 @@ int refine_pass(void* plan, hipStream_t s, const sawfem_operator* op, int B, con
     Q[2 * b] = q[2 * b];
     Q[2 * b + 1] = q[2 * b + 1];
     err[b] = e[b];
-    bad[b] = !(e[b] <= ir_tol);
+    bad[b] = !(dqp[b] <= ir_tol);
   }
   if (steps_out) *steps_out = std::min(steps, max_ir);
   return 0;

@@ -79,7 +79,11 @@ def ir_error(dq, dq_prev):
     (inf after the first).  Refinement converges linearly; with the observed rate
     r = dq / dq_prev below 1/2, the corrections still to come sum to about dq r / (1 - r)
     (geometric series).  Otherwise (first step, slow or stalled contraction) the estimate is
-    dq itself.
+    dq itself.  This is the reported estimate; the refinement stops on dq itself (see
+    ``FrequencySolver._refine``), because the rate can change: complex64 factors can remove the
+    dominant error in one step and then converge slowly on a component they resolve poorly
+    (seen on a design cell near resonance: corrections 4e-4, 2.4e-6, 8.6e-8, 9.3e-8 with a true
+    error of 7e-7 after the third), which an extrapolation from the fast step hides.
     """
     dq = np.asarray(dq, dtype=float)
     prev = np.asarray(dq_prev, dtype=float)
@@ -111,6 +115,41 @@ def bc_periodic(model: Model, lines, tag_bottom=21, tag_left=19, tag_right=20, t
     groups = [np.array([a, b]) for a, b in zip(masters, slaves)]
     fix = np.concatenate([bottom + a * N for a in range(ndisp)])
     return fix, zheng + ndisp * N, fu + ndisp * N, groups
+
+
+class _ResidualOp:
+    """FP64 residual r = rhs - A(w) x of a chunk (A^T with a transposed-slot map), through the
+    frequency-major layout: x and rhs are transposed to [n][Bp] (Bp = B padded to 64), the
+    residual kernel reads each A_aa entry once for all frequencies, r is transposed back.
+
+    A module-level class on purpose: the [n][Bp] buffers must be freed when the chunk's solve
+    returns (a class defined inside a method sits in a reference cycle, and its buffers would
+    wait for Python's cyclic garbage collector)."""
+
+    def __init__(self, solver, B, om):
+        torch = solver.torch
+        self.solver, self.B, self.om = solver, B, om
+        n = solver.n_active
+        self.Bp = -(-B // 64) * 64
+        self.xt = torch.empty((n, self.Bp), dtype=torch.complex128, device=solver.dev)
+        self.rt = torch.empty((n, self.Bp), dtype=torch.complex128, device=solver.dev)
+
+    def to_t(self, v):
+        sv = self.solver
+        n = sv.n_active
+        out = sv.torch.empty((n, self.Bp), dtype=sv.torch.complex128, device=sv.dev)
+        _lib.call('sawfem_transpose', stream_ptr(), self.B, n, n, self.Bp, v.data_ptr(), out.data_ptr())
+        return out
+
+    def __call__(self, x, rhst, r, tslot):
+        sv = self.solver
+        n, B, Bp = sv.n_active, self.B, self.Bp
+        s = stream_ptr()
+        _lib.call('sawfem_transpose', s, B, n, n, Bp, x.data_ptr(), self.xt.data_ptr())
+        _lib.call('sawfem_residual_t', s, n, B, Bp, sv.aa_indptr.data_ptr(), sv.aa_indices.data_ptr(),
+                  sv._dptr(tslot), sv.Kaa.data_ptr(), sv.Maa.data_ptr(), sv._dptr(sv.Daa), self.om.data_ptr(),
+                  sv.beta, self.xt.data_ptr(), rhst.data_ptr(), self.rt.data_ptr())
+        _lib.call('sawfem_transpose', s, n, B, Bp, n, self.rt.data_ptr(), r.data_ptr())
 
 
 class FrequencySolver:
@@ -459,7 +498,8 @@ class FrequencySolver:
         out['perturbed'], out['pivot_min_ratio'] = self.pivot_stats(reset=True)
         refactored = np.zeros(B, dtype=bool)
         if prec == 0 and fallback:
-            bad = np.flatnonzero(~(err <= ir_tol) | (~(out.get('adj_err', err) <= ir_tol) if gradient else False))
+            dq = out['dq']
+            bad = np.flatnonzero(~(dq <= ir_tol) | (~(out.get('adj_dq', dq) <= ir_tol) if gradient else False))
             if bad.size:
                 del arena, uarena
                 # complex128 refactorization in chunks that fit the workspace of this call
@@ -492,11 +532,14 @@ class FrequencySolver:
     def _refine(self, prec, nb, arena, uarena, X, transpose, res, rhst, tslot, measure, ir_tol, max_ir):
         """Iterative refinement of X ([B][R][n], holding the first solve) in place.
 
-        Each step: FP64 residual of every frequency, a solve of the frequencies whose estimated
-        error (:func:`ir_error` of each quantity ``measure()`` returns, per frequency) is still
-        above ``ir_tol`` (converged frequencies are left out of the solve and their correction
-        is zero), update.  Returns (values, last changes, error estimates, steps, history of the
-        largest change of the first quantity).
+        Each step: FP64 residual of every frequency, a solve of the frequencies still active,
+        update.  A frequency converges when the relative change of every quantity ``measure()``
+        returns is at most ``ir_tol`` (the remaining error is then below ir_tol for any rate
+        under 1/2, and below 100 ir_tol even for a slow tail at rate 0.99); from the second step
+        on, one whose change above ir_tol does not halve has stalled (left to the complex128
+        refactorization).  Converged and stalled frequencies are left out of the solve (their
+        correction is zero).  Returns (values, last changes, error estimates (:func:`ir_error`,
+        reported), steps, history of the largest change of the first quantity).
         """
         torch = self.torch
         s = stream_ptr()
@@ -529,12 +572,12 @@ class FrequencySolver:
                 # a correction that does not shrink by half after the first step: refinement with
                 # these factors does not converge here (left to the complex128 refactorization)
                 if steps >= 2:
-                    stalled |= active & (dq > 0.5 * dqs[k]) & (e > ir_tol)
+                    stalled |= active & (dq > 0.5 * dqs[k]) & (dq > ir_tol)
                 errs[k][active] = e[active]
                 dqs[k][active] = dq[active]
                 vals[k] = new[k]
             hist.append(float(dqs[0][active].max()))
-            active = np.any([e > ir_tol for e in errs], axis=0) & ~stalled & ~self._stalled
+            active = np.any([d > ir_tol for d in dqs], axis=0) & ~stalled & ~self._stalled
             self._stalled |= stalled
             if not active.any():
                 break
@@ -595,27 +638,7 @@ class FrequencySolver:
 
     def _residual_op(self, B, om):
         """r = rhs - A(w) x (or A^T with tslot) through the frequency-contiguous residual kernel."""
-        torch = self.torch
-        n = self.n_active
-        Bp = -(-B // 64) * 64
-        xt = torch.empty((n, Bp), dtype=torch.complex128, device=self.dev)
-        rt = torch.empty((n, Bp), dtype=torch.complex128, device=self.dev)
-        solver = self
-
-        class Op:
-            def to_t(self, v):
-                out = torch.empty((n, Bp), dtype=torch.complex128, device=solver.dev)
-                _lib.call('sawfem_transpose', stream_ptr(), B, n, n, Bp, v.data_ptr(), out.data_ptr())
-                return out
-
-            def __call__(self, x, rhst, r, tslot):
-                s = stream_ptr()
-                _lib.call('sawfem_transpose', s, B, n, n, Bp, x.data_ptr(), xt.data_ptr())
-                _lib.call('sawfem_residual_t', s, n, B, Bp, solver.aa_indptr.data_ptr(), solver.aa_indices.data_ptr(),
-                          solver._dptr(tslot), solver.Kaa.data_ptr(), solver.Maa.data_ptr(), solver._dptr(solver.Daa),
-                          om.data_ptr(), solver.beta, xt.data_ptr(), rhst.data_ptr(), rt.data_ptr())
-                _lib.call('sawfem_transpose', s, n, B, Bp, n, rt.data_ptr(), r.data_ptr())
-        return Op()
+        return _ResidualOp(self, B, om)
 
     def _adjoint(self, om, omegas, V, prec, nb, arena, uarena, rhs, x, ir_tol, max_ir):
         """lambda = A^-T kts_a by transposed solves (refined with the transposed FP64 residual
