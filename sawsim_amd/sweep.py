@@ -498,8 +498,8 @@ class FrequencySolver:
         out['perturbed'], out['pivot_min_ratio'] = self.pivot_stats(reset=True)
         refactored = np.zeros(B, dtype=bool)
         if prec == 0 and fallback:
-            dq = out['dq']
-            bad = np.flatnonzero(~(dq <= ir_tol) | (~(out.get('adj_dq', dq) <= ir_tol) if gradient else False))
+            conv = out['converged'] & out.get('adj_converged', True)
+            bad = np.flatnonzero(~conv)
             if bad.size:
                 del arena, uarena
                 # complex128 refactorization in chunks that fit the workspace of this call
@@ -512,7 +512,7 @@ class FrequencySolver:
                     x[tb] = sub['x']
                     if 'lam' in out:
                         out['lam'][tb] = sub['lam']
-                    for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err'):
+                    for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err', 'converged', 'adj_converged'):
                         if key in out:
                             out[key][part] = sub[key]
                     out['perturbed_fp64'] = out.get('perturbed_fp64', 0) + sub['perturbed']
@@ -529,7 +529,7 @@ class FrequencySolver:
         _lib.call('sawfem_mf_pivot_stats', self._plan, int(reset), C.byref(cnt), C.byref(rmin))
         return int(cnt.value), float(rmin.value)
 
-    def _refine(self, prec, nb, arena, uarena, X, transpose, res, rhst, tslot, measure, ir_tol, max_ir):
+    def _refine(self, prec, nb, arena, uarena, X, transpose, res, rhst, tslot, measure, ir_tol, max_ir, dwr=None):
         """Iterative refinement of X ([B][R][n], holding the first solve) in place.
 
         Each step: FP64 residual of every frequency, a solve of the frequencies still active,
@@ -538,8 +538,19 @@ class FrequencySolver:
         under 1/2, and below 100 ir_tol even for a slow tail at rate 0.99); from the second step
         on, one whose change above ir_tol does not halve has stalled (left to the complex128
         refactorization).  Converged and stalled frequencies are left out of the solve (their
-        correction is zero).  Returns (values, last changes, error estimates (:func:`ir_error`,
-        reported), steps, history of the largest change of the first quantity).
+        correction is zero).
+
+        ``dwr(r)``, when given, returns for every quantity the dual-weighted residual: the
+        adjoint solution contracted with the residual r = b - A x of the current iterate, which
+        is the quantity's error to first order (for Q = k^T x, Q - Q* = -lambda*^T r exactly, with
+        lambda* = A^-T k; the computed lambda replaces lambda* at second-order cost).  A frequency
+        whose relative dual-weighted residuals are all within ``ir_tol`` converges on the residual
+        alone, before that step's solve.
+
+        Returns (values, last changes, error estimates, steps, history of the largest change of
+        the first quantity); ``self._converged`` marks the converged frequencies and
+        ``self._dwr_ok`` those accepted on their dual-weighted residual.  The estimates are the
+        dual-weighted residuals where accepted on them, :func:`ir_error` otherwise.
         """
         torch = self.torch
         s = stream_ptr()
@@ -549,13 +560,23 @@ class FrequencySolver:
         dqs = [np.full(B, np.inf) for _ in vals]
         errs = [np.full(B, np.inf) for _ in vals]
         active = np.ones(B, dtype=bool)
+        dwr_ok = np.zeros(B, dtype=bool)
         r = torch.empty_like(X)
-        hist, steps = [], 0
+        hist, applied = [], 0
         self._solved = [B]                 # frequencies solved per pass (first solve, then each step)
         self._stalled = np.zeros(B, dtype=bool)
         for steps in range(1, max_ir + 1):
-            self._solved.append(int(active.sum()))
             res(X, rhst, r, tslot)
+            if dwr is not None:
+                eta = [np.abs(a) / np.maximum(np.abs(v), 1e-300) for a, v in zip(dwr(r), vals)]
+                ok = active & np.all([e <= ir_tol for e in eta], axis=0)
+                for k, e in enumerate(eta):
+                    errs[k][ok] = e[ok]
+                dwr_ok |= ok
+                active &= ~ok
+                if not active.any():
+                    break
+            self._solved.append(int(active.sum()))
             sel, nsel = None, 0
             if not active.all():
                 r[torch.as_tensor(~active, device=self.dev)] = 0
@@ -564,6 +585,7 @@ class FrequencySolver:
             _lib.call('sawfem_mf_solve', self._plan, s, prec, B, R, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(),
                       transpose, nb, sel, nsel)
             _lib.call('sawfem_axpy', s, X.numel(), X.data_ptr(), r.data_ptr())
+            applied = steps
             new = measure()
             stalled = np.zeros(B, dtype=bool)
             for k in range(len(vals)):
@@ -577,11 +599,13 @@ class FrequencySolver:
                 dqs[k][active] = dq[active]
                 vals[k] = new[k]
             hist.append(float(dqs[0][active].max()))
-            active = np.any([d > ir_tol for d in dqs], axis=0) & ~stalled & ~self._stalled
+            active = active & np.any([d > ir_tol for d in dqs], axis=0) & ~stalled & ~self._stalled
             self._stalled |= stalled
             if not active.any():
                 break
-        return vals, dqs, errs, steps, hist
+        self._dwr_ok = dwr_ok
+        self._converged = dwr_ok | (np.all([d <= ir_tol for d in dqs], axis=0) & ~self._stalled)
+        return vals, dqs, errs, applied, hist
 
     def _solve_single(self, om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir):
         """x = A^-1 rhs refined against the FP64 operator until Q's estimated error <= ir_tol."""
@@ -600,7 +624,8 @@ class FrequencySolver:
         res = self._residual_op(B, om)
         (Q,), (dq,), (err,), steps, hist = self._refine(prec, nb, arena, uarena, x, 0, res, res.to_t(rhs), None,
                                                          measure, ir_tol, max_ir)
-        return dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist, solved=list(self._solved))
+        return dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist, solved=list(self._solved),
+                    converged=self._converged.copy())
 
     def _solve_paired(self, om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir):
         """Forward and adjoint systems as two right-hand sides of one refined solve.
@@ -625,13 +650,19 @@ class FrequencySolver:
                       X.data_ptr(), Qd.data_ptr())
             mu = (X[:, 1] * d * rhs).sum(dim=1)
             return [Qd[0::2].cpu().numpy().copy(), mu.cpu().numpy()]
+        def dwr(r):
+            # Q - Q* = -lambda^T r_x (forward residual); mu - mu* = -(D x)^T r_z (adjoint residual)
+            eq = (X[:, 1] * d * r[:, 0]).sum(dim=1)
+            em = (X[:, 0] * d * r[:, 1]).sum(dim=1)
+            return [eq.cpu().numpy(), em.cpu().numpy()]
         res = self._residual_op(2 * B, om.repeat_interleave(2))
         (Q, mu), (dq, dm), (err, aerr), steps, hist = self._refine(prec, nb, arena, uarena, X, 0, res, res.to_t(rhs2),
-                                                                  None, measure, ir_tol, max_ir)
+                                                                  None, measure, ir_tol, max_ir, dwr=dwr)
         x = X[:, 0].contiguous()
         lam = (X[:, 1] * d).contiguous()
         del X, rhs2
-        out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist, solved=list(self._solved))
+        out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist, solved=list(self._solved),
+                   converged=self._converged.copy(), dwr_accepted=int(self._dwr_ok.sum()))
         out.update(self._contract(omegas, V, x, lam))
         out.update(adj_dq=dm, adj_err=aerr, adj_steps=steps, lam=lam, paired=True)
         return out
@@ -654,7 +685,7 @@ class FrequencySolver:
         _, (dq,), (err,), steps, _ = self._refine(prec, nb, arena, uarena, lam, 1, res, res.to_t(gvec), self.tslot,
                                                   measure, ir_tol, max_ir)
         out = self._contract(omegas, V, x, lam)
-        out.update(adj_dq=dq, adj_err=err, adj_steps=steps, lam=lam)
+        out.update(adj_dq=dq, adj_err=err, adj_steps=steps, lam=lam, adj_converged=self._converged.copy())
         return out
 
     def _contract(self, omegas, V, x, lam):
@@ -728,6 +759,8 @@ class FrequencySolver:
             item['points'] = int(min(batch, fre.size - i0))
             item['dq_hist'] = res.get('dq_hist')
             item['solved_per_pass'] = res.get('solved')
+            if 'dwr_accepted' in res:
+                item['accepted_on_residual'] = res['dwr_accepted']
             info.append(item)
             del res
             if on_chunk is not None:
