@@ -329,7 +329,7 @@ def design_set(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--workload', default='config2')
     ap.add_argument('--batch', type=int, default=0, help='frequencies per factorization chunk (0: fit --arena-gb)')
