@@ -31,7 +31,13 @@ PEAK_FP64_MFMA = 78.6
 # HBM bytes per launch of the Schur GEMM k_gemm<float, 1> on the config-2 / config-4 sweeps:
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (separate runs), each scaled by what the counter reads for
 # a plain 1 GiB device copy (`--copy-mib 1024`: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000)
+# HBM bytes per launch of the Schur-complement GEMM, from two rocprofv3 --pmc runs (FETCH_SIZE, then
+# WRITE_SIZE, each calibrated on a 1 GiB device copy): profiles/config2_schur_traffic.md and
+# profiles/config4_schur_traffic.md hold the commands and the per-dispatch counts.  The counters
+# cannot be read inside a timed run (a --pmc run serializes the kernels), so the figure is copied
+# from those summaries, and the kernel it describes has not changed since they were taken.
 SCHUR_TRAFFIC_BYTES = {'config2': 1.823e10, 'config4': 1.620e10}
+SCHUR_TRAFFIC_SOURCE = {'config2': 'profiles/config2_schur_traffic.md', 'config4': 'profiles/config4_schur_traffic.md'}
 
 
 def log(*a):
@@ -518,7 +524,8 @@ def main():
         roofline=dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
                       frac=(achieved / PEAK_FP32_MFMA) if achieved else None,
                       traffic=SCHUR_TRAFFIC_BYTES.get(args.workload),
-                      traffic_note='HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy',
+                      traffic_note='HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy, '
+                                   f'copied from {SCHUR_TRAFFIC_SOURCE.get(args.workload, "(no counter run)")}',
                       kernel='k_gemm<float, 1, 64, 64, *> (Schur complement, complex64 on FP32 MFMA)',
                       launches=schur_n, kernel_ms=schur_ms, algorithmic_flops=schur_flops,
                       algorithmic_bytes_per_launch=work['schur_bytes'] * P * args.steps / max(schur_n, 1),

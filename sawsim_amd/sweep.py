@@ -512,7 +512,7 @@ class FrequencySolver:
                     x[tb] = sub['x']
                     if 'lam' in out:
                         out['lam'][tb] = sub['lam']
-                    for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err', 'converged', 'adj_converged'):
+                    for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err', 'converged', 'adj_converged', 'dwr_mask'):
                         if key in out:
                             out[key][part] = sub[key]
                     out['perturbed_fp64'] = out.get('perturbed_fp64', 0) + sub['perturbed']
@@ -662,7 +662,8 @@ class FrequencySolver:
         lam = (X[:, 1] * d).contiguous()
         del X, rhs2
         out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist, solved=list(self._solved),
-                   converged=self._converged.copy(), dwr_accepted=int(self._dwr_ok.sum()))
+                   converged=self._converged.copy(), dwr_accepted=int(self._dwr_ok.sum()),
+                   dwr_mask=self._dwr_ok.copy())
         out.update(self._contract(omegas, V, x, lam))
         out.update(adj_dq=dm, adj_err=aerr, adj_steps=steps, lam=lam, paired=True)
         return out
