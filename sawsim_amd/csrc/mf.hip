@@ -925,6 +925,7 @@ template <> struct SolveTile<float> { static constexpr int SB = 64; };    // wid
 template <> struct SolveTile<double> { static constexpr int SB = 32; };   // widest panel, complex128
 
 __device__ __forceinline__ double2 d2add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 d2sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 
 // w[r][i] (i < p) = x_r[rows[i]] + children's update entries; uf_r[i - p] (i >= p) = children's
 // entries.  w is [R][p] in LDS, or [R][m] holding the CB rows too when ALL is set.
@@ -1001,18 +1002,38 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
     // CB -> pivot rows: kind 1: w_i -= sum_j U[i][p + j] w_{p+j};  kind 3: w_k -= sum_j L[p + j][k] w_{p+j}
     const int q = m - p;
     if (KIND == 1) {
-      for (int i = t; i < p; i += blockDim.x) {
+      // 64 pivot rows at a time, the q CB columns split over G = blockDim / 64 thread groups so
+      // that every thread has loads in flight (one thread per row left 3/4 of the workgroup idle
+      // and each row a chain of q / 8 dependent load batches); the G partial sums meet in the
+      // tile region, unused until the diagonal blocks (R * blockDim double2 fit it: checked by
+      // the launch)
+      double2* red = reinterpret_cast<double2*>(tile);
+      const int G = blockDim.x / 64;
+      const int i = t & 63, g = t >> 6;
+      for (int i0 = 0; i0 < p; i0 += 64) {
         double2 s[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) s[r] = w[r * W + i];
-        const V* row = Fb + (int64_t)i + (int64_t)p * m;
-        batched<8>(0, q, 1, [&](int j) { return row[(int64_t)j * m]; }, [&](int j, V v) {
-          const double2 a = to_d2(v);
+        for (int r = 0; r < R; ++r) s[r] = make_double2(0, 0);
+        if (i0 + i < p) {
+          const V* row = Fb + (int64_t)(i0 + i) + (int64_t)p * m;
+          batched<8>(g, q, G, [&](int j) { return row[(int64_t)j * m]; }, [&](int j, V v) {
+            const double2 a = to_d2(v);
 #pragma unroll
-          for (int r = 0; r < R; ++r) s[r] = cfms(s[r], a, w[r * W + p + j]);
-        });
+            for (int r = 0; r < R; ++r) s[r] = cfma(s[r], a, w[r * W + p + j]);
+          });
+        }
 #pragma unroll
-        for (int r = 0; r < R; ++r) w[r * W + i] = s[r];
+        for (int r = 0; r < R; ++r) red[(r * G + g) * 64 + i] = s[r];
+        __syncthreads();
+        if (t < 64 && i0 + t < p) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            double2 tot = red[r * G * 64 + t];
+            for (int gg = 1; gg < G; ++gg) tot = d2add(tot, red[(r * G + gg) * 64 + t]);
+            w[r * W + i0 + t] = d2sub(w[r * W + i0 + t], tot);
+          }
+        }
+        __syncthreads();
       }
     } else {
       for (int c0 = 0; c0 < p; c0 += 64) {
@@ -2061,9 +2082,8 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   double2* X = (double2*)x;
   // fronts of the fused set (m <= the small-front threshold): one workgroup does the whole
   // front; the others: pivot-triangle kernel + panel kernel
-  // forward passes take the one-workgroup set of the solve partition; backward passes keep the
-  // factorization's partition (their CB-to-pivot product has only p active rows per workgroup,
-  // which measured slower than the panel kernel on the small fronts)
+  // forward and backward passes take the one-workgroup set of the solve partition (the
+  // transposed passes keep the factorization's partition)
   auto tri_sub = [&](const Level& L, const Sub& S, int kind, bool full, hipStream_t st) {
     if (S.nf == 0) return;
     Timed tm(P, st, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
@@ -2163,9 +2183,10 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   for (int l = (int)P->levels.size() - 1; l >= 0; --l) {
     const Level& L = P->levels[l];
     P->cur_level = l;
-    if (!transpose && P->tri_lds_bytes > 0) {
+    if (!transpose) {
       // the forward partition: one-workgroup buckets (LDS-staged where their block fits, else
-      // the whole-front pivot-triangle kernel), the rest through the panel + triangle pair
+      // the whole-front pivot-triangle kernel, whose CB-to-pivot product splits the CB columns
+      // over the workgroup), the rest through the panel + triangle pair
       const size_t nfull = L.ssmall[pi].size();
       const bool fork = P->overlap && L.sbig[pi].size() + nfull > 1;
       if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
