@@ -202,6 +202,26 @@ def design_set(args):
     P = cells[0].points
     from sawsim_amd.sweep import check_workspace, hbm_available
     torch.cuda.empty_cache()          # the cells' assembly temporaries
+    ne = [c.model.ne for c in cells]
+    if args.share:
+        for eng in engines:
+            eng.set_share(args.share)
+    tuning = None
+    if not args.no_autotune:
+        # kernel knobs and factorization sharing tuned on the largest cell, applied to every cell
+        from sawsim_amd.tune import autotune, tune_share, _SETTERS
+        big = int(np.argmax(ne))
+        fb = cells[big].frequencies()
+        tuning = autotune(engines[big], fb, 'mixed', gradient=True)
+        if not args.share:
+            tuning['share'] = tune_share(engines[big], fb, 'mixed', gradient=True)
+        for eng in engines:
+            for knob, v in tuning['choice'].items():
+                getattr(eng, _SETTERS[knob])(v)
+            eng.set_share(engines[big].share)
+        engines[big].release_workspace()
+        log(f"[bench] autotune on cell {big}: {tuning['choice']}, share {engines[big].share} "
+            f"({tuning.get('share', {}).get('ms')} ms); memory: {_mem()}")
     budget = args.arena_gb * 1e9 if args.arena_gb else hbm_available()
     batches = []
     for eng in engines:
@@ -209,34 +229,23 @@ def design_set(args):
         b = -(-P // -(-P // b))
         batches.append(b)
     # every cell's chunk has to fit beside the shared workspace, which the largest arena sets
-    ws_bytes = max(eng.arena_bytes(b, 'mixed') for eng, b in zip(engines, batches))
-    extra = [eng.chunk_bytes(b, 'mixed', True) - eng.arena_bytes(b, 'mixed') for eng, b in zip(engines, batches)]
+    ws_bytes = max(eng.chunk_arena_bytes(b, 'mixed') for eng, b in zip(engines, batches))
+    extra = [eng.chunk_bytes(b, 'mixed', True) - eng.chunk_arena_bytes(b, 'mixed') for eng, b in zip(engines, batches)]
     while ws_bytes + max(extra) > budget:
-        i = int(np.argmax([eng.arena_bytes(b, 'mixed') + x for eng, b, x in zip(engines, batches, extra)]))
+        i = int(np.argmax([eng.chunk_arena_bytes(b, 'mixed') + x for eng, b, x in zip(engines, batches, extra)]))
         if batches[i] == 1:
             break
         batches[i] = -(-P // (-(-P // batches[i]) + 1))       # one more chunk for that cell
-        extra[i] = engines[i].chunk_bytes(batches[i], 'mixed', True) - engines[i].arena_bytes(batches[i], 'mixed')
-        ws_bytes = max(eng.arena_bytes(b, 'mixed') for eng, b in zip(engines, batches))
+        extra[i] = engines[i].chunk_bytes(batches[i], 'mixed', True) - engines[i].chunk_arena_bytes(batches[i], 'mixed')
+        ws_bytes = max(eng.chunk_arena_bytes(b, 'mixed') for eng, b in zip(engines, batches))
     check_workspace(ws_bytes + max(extra), what='shared front workspace and chunk vectors')
     log(f'[bench] memory: {_mem()}; budget {budget / 1e9:.1f} GB, largest chunk {max(extra) / 1e9:.1f} GB '
         f'beside the workspace')
     shared = torch.empty(ws_bytes, dtype=torch.uint8, device='cuda')
     for eng in engines:
         eng._ws = shared                       # every cell reuses the one front workspace
-    ne = [c.model.ne for c in cells]
     log(f'[bench] config5: {len(cells)} cells, {sum(ne)} elements ({min(ne)}-{max(ne)}), setup {setup:.0f}s, '
-        f'workspace {ws_bytes / 1e9:.0f} GB')
-
-    tuning = None
-    if not args.no_autotune:
-        from sawsim_amd.tune import autotune, _SETTERS
-        big = int(np.argmax(ne))
-        tuning = autotune(engines[big], cells[big].frequencies(), 'mixed', gradient=True, batch=batches[big])
-        for eng in engines:
-            for knob, v in tuning['choice'].items():
-                getattr(eng, _SETTERS[knob])(v)
-        log(f"[bench] autotune on cell {big}: {tuning['choice']}; memory: {_mem()}")
+        f'workspace {ws_bytes / 1e9:.0f} GB, share {engines[0].share}, chunks {[-(-P // b) for b in batches]}')
 
     def sweep(i):
         return engines[i].sweep_charges(cells[i].frequencies(), precision='mixed', batch=batches[i], gradient=True)
@@ -257,25 +266,35 @@ def design_set(args):
     log(f'[bench] peak torch allocation over the sweeps {torch.cuda.max_memory_allocated() / 1e9:.1f} GB; {_mem()}')
     schur_ms = schur_n = 0.0
     schur_flops = 0.0
-    for eng, c in zip(engines, cells):
+    for eng, c, res in zip(engines, cells, results):
         prof, _ = profile_read(eng)
         schur_ms += prof['gemm_s'][0]
         schur_n += prof['gemm_s'][1]
-        schur_flops += kernel_work(eng.mf, 64)['schur_flops'] * c.points
+        schur_flops += kernel_work(eng.mf, 64)['schur_flops'] * sum(it['factorizations'] for it in res[2])
         _lib.call('sawfem_mf_profile', eng._plan, 0)
     npts = sum(c.points for c in cells)
     value = npts / dt
     lat_ms = np.array(lat) * 1e3
     log(f'[bench] design set: {dt:.2f}s for {npts} points -> {value:.1f} pts/s; latency p50 '
         f'{np.percentile(lat_ms, 50):.0f} ms p90 {np.percentile(lat_ms, 90):.0f} ms')
-    # accuracy against the FP64 path on every cell, at every 8th of its points
+    # accuracy against the FP64 path on every cell, at every 8th of its points, on one shared
+    # workspace sized for the largest cell's FP64 chunk
+    k = np.arange(0, P, 8)
+    for eng in engines:
+        eng._ws = None
+    del shared
+    torch.cuda.empty_cache()
+    fp64_batch = [min(k.size, eng.auto_batch('fp64', gradient=True)) for eng in engines]
+    shared = torch.empty(max(eng.arena_bytes(b, 'fp64') for eng, b in zip(engines, fp64_batch)), dtype=torch.uint8,
+                         device='cuda')
+    for eng in engines:
+        eng._ws = shared
     errs, gerrs = [], []
     t1 = time.perf_counter()
     for i in range(len(cells)):
         Q, dY, _ = results[i]
-        k = np.arange(0, P, 8)
-        Q64, dY64, _ = engines[i].sweep_charges(cells[i].frequencies()[k], precision='fp64',
-                                                batch=max(1, min(k.size, batches[i] // 2)), gradient=True)
+        Q64, dY64, _ = engines[i].sweep_charges(cells[i].frequencies()[k], precision='fp64', gradient=True,
+                                                batch=fp64_batch[i])
         errs.append(float(np.max(np.abs(Q[k] - Q64) / np.abs(Q64))))
         scale = np.abs(cells[i].params)
         scale[12:15] = 1.0
@@ -305,6 +324,8 @@ def design_set(args):
                              f'({min(ne)}-{max(ne)} per cell), {P} points each, swept back to back',
                     cells=len(cells), points_per_cell=P, elements=ne,
                     n_active=[int(e.n_active) for e in engines], batches=batches, tuning=tuning,
+                    share=engines[0].share,
+                    factorizations=int(sum(it['factorizations'] for r in results for it in r[2])),
                     gradient='dY11/dp for 17 parameters by adjoint'),
         latency_ms_per_cell=dict(p50=float(np.percentile(lat_ms, 50)), p90=float(np.percentile(lat_ms, 90)),
                                  p99=float(np.percentile(lat_ms, 99)), max=float(lat_ms.max()),
@@ -355,6 +376,8 @@ def main():
                     help='largest front order whose 32-wide panels use the one-kernel panel LU (-1: default)')
     ap.add_argument('--tri-lds', type=int, default=-1,
                     help='LDS bytes for the factor block of the LDS-staged solves (-1: default, 0: off)')
+    ap.add_argument('--share', type=int, default=0,
+                    help='frequencies per complex64 factorization (0: chosen by the tuner, 1 without it)')
     ap.add_argument('--no-autotune', action='store_true',
                     help='keep the library defaults of the kernel-selection knobs (no tuning run)')
     ap.add_argument('--copy-mib', type=int, default=0,
@@ -402,22 +425,33 @@ def main():
                                          for k, v in asm.items()))
     nb = 64                                   # panel width of the complex64 factorization
     work = kernel_work(eng.mf, nb)
-    per_freq = eng.arena_bytes(1, 'mixed')
-    batch = args.batch or min(P, eng.auto_batch('mixed', gradient=True,
-                                                budget=args.arena_gb * 1e9 if args.arena_gb else None))
-    nchunk = -(-P // batch)
-    batch = -(-P // nchunk)
-    from sawsim_amd.sweep import hbm_available
-    log(f'[bench] batch {batch} x {nchunk} chunks, arena {per_freq * batch / 1e9:.1f} GB, chunk '
-        f'{eng.chunk_bytes(batch, "mixed", True) / 1e9:.1f} GB of {hbm_available() / 1e9:.1f} GB available')
+    budget = args.arena_gb * 1e9 if args.arena_gb else None
 
+    def chunking():
+        b = args.batch or min(P, eng.auto_batch('mixed', gradient=True, budget=budget))
+        return -(-P // -(-P // b)), -(-P // b)
+
+    if args.share:
+        eng.set_share(args.share)
     tuning = None
     if not args.no_autotune:
-        from sawsim_amd.tune import autotune
+        from sawsim_amd.tune import autotune, tune_share
         t0 = time.time()
-        tuning = autotune(eng, freqs, 'mixed', gradient=True, batch=batch)
+        tuning = autotune(eng, freqs, 'mixed', gradient=True, batch=chunking()[0])
         log(f"[bench] autotune {time.time() - t0:.1f}s: {tuning['choice']} "
             f"({ {k: {c: round(v, 1) for c, v in d.items()} for k, d in tuning['ms'].items()} } ms)")
+        if not args.share:
+            t0 = time.time()
+            tuning['share'] = tune_share(eng, freqs, 'mixed', gradient=True)
+            log(f"[bench] factorization sharing {time.time() - t0:.1f}s: share {tuning['share']['choice']} "
+                f"(ms per frequency { {c: round(v, 2) for c, v in tuning['share']['ms_per_frequency'].items()} } "
+                f"over chunks of {tuning['share']['batch']} frequencies)")
+    batch, nchunk = chunking()
+    from sawsim_amd.sweep import hbm_available
+    log(f'[bench] share {eng.share}: batch {batch} x {nchunk} chunks, arena '
+        f'{eng.chunk_arena_bytes(batch, "mixed") / 1e9:.1f} GB, chunk '
+        f'{eng.chunk_bytes(batch, "mixed", True) / 1e9:.1f} GB of '
+        f'{hbm_available(0 if getattr(eng, "_ws", None) is None else eng._ws.numel()) / 1e9:.1f} GB available')
 
     def sweep():
         return eng.sweep_charges(freqs, precision='mixed', batch=batch, gradient=True)
@@ -439,8 +473,7 @@ def main():
     # accuracy of the mixed path against the FP64 path (batched complex128 fronts, FP64 adjoint)
     idx = np.arange(P) if not args.check_points else np.linspace(0, P - 1, min(args.check_points, P)).astype(int)
     t0 = time.perf_counter()
-    Q64, dY64, _ = eng.sweep_charges(freqs[idx], precision='fp64', batch=max(1, min(len(idx), batch // 2)),
-                                     gradient=True)
+    Q64, dY64, _ = eng.sweep_charges(freqs[idx], precision='fp64', gradient=True)
     log(f'[bench] FP64 check sweep of {len(idx)} points: {time.perf_counter() - t0:.1f}s')
     y_err_pts = np.abs(Q[idx] - Q64) / np.abs(Q64)
     y_err = float(np.max(y_err_pts))
@@ -467,9 +500,12 @@ def main():
     log(f'[bench] mixed vs fp64: Y11 {y_err:.2e}, dY/dp {g_err:.2e}')
     # dominant MFMA kernel: the Schur-complement GEMM k_gemm<float, 1> (one launch per level)
     schur_ms, schur_n = prof['gemm_s']
-    schur_flops = work['schur_flops'] * P * args.steps
+    # factorizations per sweep: one per `share` frequencies, plus the complex64 refactorizations
+    # of frequencies whose shared factors did not converge
+    nfact = sum(it['factorizations'] for it in info)
+    schur_flops = work['schur_flops'] * nfact * args.steps
     achieved = schur_flops / (schur_ms * 1e-3) / 1e12 if schur_ms > 0 else None
-    lupd_tf = work['lupd_flops'] * P * args.steps / (prof['gemm_l'][0] * 1e-3) / 1e12 if prof['gemm_l'][0] else None
+    lupd_tf = work['lupd_flops'] * nfact * args.steps / (prof['gemm_l'][0] * 1e-3) / 1e12 if prof['gemm_l'][0] else None
     # solves: each forward+backward pass streams the stored L/U entries (complex64) once
     # (passes over the factors: paired forward+adjoint solves share each pass)
     nsolve_pts = sum(sum(it['solved_per_pass']) if it.get('paired_adjoint')
@@ -482,10 +518,11 @@ def main():
     nnz_aa = int(eng.bloch.nnz_aa)
     def chunk_flops(it):
         if it.get('paired_adjoint'):       # two right-hand sides per solved frequency and pass
-            return (it['points'] * work['factor_flops'] + 2 * sum(it['solved_per_pass']) * work['solve_flops']
-                    + 2 * it['points'] * it['ir_steps'] * 8 * nnz_aa)
-        return it['points'] * (work['factor_flops'] + (2 + it['ir_steps'] + it.get('adj_steps', 0)) * work['solve_flops']
-                               + (it['ir_steps'] + it.get('adj_steps', 0)) * 8 * nnz_aa)
+            return (it['factorizations'] * work['factor_flops'] + 2 * sum(it['solved_per_pass']) * work['solve_flops']
+                    + 2 * sum(it['solved_per_pass'][1:]) * 8 * nnz_aa)
+        return (it['factorizations'] * work['factor_flops']
+                + it['points'] * ((2 + it['ir_steps'] + it.get('adj_steps', 0)) * work['solve_flops']
+                                  + (it['ir_steps'] + it.get('adj_steps', 0)) * 8 * nnz_aa))
     sweep_flops = args.steps * sum(chunk_flops(it) for it in info)
     sweep_tf = sweep_flops / dt / 1e12
     cpu = None
@@ -505,7 +542,7 @@ def main():
                     points=P, n_active=eng.n_active, fronts=eng.mf.nf, batch=batch,
                     gradient=f'dY11/dp for {eng.n_params} parameters (3m constants, Euler angles, beta_dK, '
                              f'tan delta{", rho" if hex_cell else ""}) by adjoint',
-                    tuning=tuning),
+                    tuning=tuning, share=eng.share, factorizations_per_sweep=nfact),
         y11_max_rel_err_vs_fp64=y_err,
         y11_check_points=int(len(idx)),
         y11_rel_err_per_point=dict(freq_hz=[float(f) for f in freqs[idx]],

@@ -241,6 +241,13 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
  * NULL for all B. */
 int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const void* arena, void* uarena, void* x,
                     int transpose, int nb, const int* bsel, int nsel);
+/* sawfem_mf_solve with factors shared between frequencies: right-hand side b (of B) is solved
+ * with the factors in slot fslot[b] (device array of B ints in [0, Bf)) of an arena factored for
+ * Bf frequencies.  Refined against each frequency's own FP64 operator, the factors of a nearby
+ * frequency act as a preconditioner (a few more correction steps instead of a factorization).
+ * No counterpart in the reference (it factors every frequency). */
+int sawfem_mf_solve_shared(void* plan, void* stream, int prec, int B, int nrhs, const void* arena, int Bf,
+                           const int* fslot, void* uarena, void* x, int transpose, int nb, const int* bsel, int nsel);
 
 /* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;
  * used by the benchmark).  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S,

@@ -25,6 +25,7 @@
 // Launches walk host-built work lists (one entry per workgroup); the buckets of a level run
 // their panel phases concurrently on side streams between a fork and a join.
 #include "common.h"
+#include "../../include/sawfem.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -48,6 +49,10 @@ struct FrontsDev {
   const int* child_ids;
   const int64_t* inv_ptr;  // per child slot: parent-row -> child CB index map (length m_parent)
   const int* inv;
+  // solves only: right-hand side b uses the factors of arena slot fslot[b] of an arena of Bf
+  // frequencies (null: slot b of an arena of B, one factorization per right-hand side)
+  const int* fslot = nullptr;
+  int Bf = 0;
 };
 
 template <typename T>
@@ -55,6 +60,13 @@ __device__ __forceinline__ typename cplx<T>::type* front_ptr(typename cplx<T>::t
                                                              int f, int b, int B) {
   const int64_t m = F.m[f];
   return arena + F.off[f] * (int64_t)B + (int64_t)b * m * m;
+}
+
+// The factors a solve of right-hand side b reads (see FrontsDev::fslot).
+template <typename T>
+__device__ __forceinline__ typename cplx<T>::type* factor_ptr(typename cplx<T>::type* arena, const FrontsDev& F,
+                                                              int f, int b, int B) {
+  return F.fslot ? front_ptr<T>(arena, F, f, F.fslot[b], F.Bf) : front_ptr<T>(arena, F, f, b, B);
 }
 
 // Strided loop over idx = first, first + step, ... < count with its global loads issued U at a
@@ -986,7 +998,7 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
   const int f = lvl[blockIdx.x];
   const int m = F.m[f], p = F.p[f];
   const int W = FULL ? m : p;
-  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
+  const V* Fb = factor_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
   const int t = threadIdx.x;
   constexpr bool fwd = (KIND == 0 || KIND == 2);
@@ -1212,7 +1224,7 @@ __device__ __forceinline__ void tri_lds_fetch(const FrontsDev& F, const typename
   using V = typename cplx<T>::type;
   const int t = threadIdx.x;
   const int m = F.m[f], p = F.p[f], mp = m * p;
-  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
+  const V* Fb = factor_ptr<T>(const_cast<V*>(arena), F, f, b, B);
 #pragma unroll
   for (int u = 0; u < NPF; ++u) {
     const int idx = min(t + 256 * u, mp - 1);
@@ -1384,7 +1396,7 @@ __device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ 
   const int o0 = (lvl[blockIdx.x] & 4095) * 64;
   if (o0 >= nout || nin == 0) return;
   const int no = min(64, nout - o0);
-  const V* Fb = front_ptr<T>(const_cast<V*>(arena), F, f, b, B);
+  const V* Fb = factor_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
   double2* uf = uarena + F.uoff[f] * B * R + (int64_t)b * R * q;   // + r * q
   constexpr int VC = 256;                                     // input entries staged per chunk
@@ -2073,8 +2085,11 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
 
 template <typename T, int R>
 int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose, int nb,
-               const int* bsel, int nsel) {
+               const int* bsel, int nsel, const int* fslot, int Bf) {
   using V = typename cplx<T>::type;
+  FrontsDev FS = P->F;         // the plan's fronts with this call's factor-slot map
+  FS.fslot = fslot;
+  FS.Bf = Bf;
   constexpr int LD = SolveTile<T>::SB + 1;   // panel kernels' transposition tile
   const int pi = sizeof(T) == 4 ? 0 : 1;
   const V* A = (const V*)arena;
@@ -2103,7 +2118,7 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     if (lds > 64 * 1024)                                                                                 \
       (void)hipFuncSetAttribute((const void*)k_tri<T, K, R, FULL>, hipFuncAttributeMaxDynamicSharedMemorySize, \
                                 (int)lds);                                                               \
-    hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, blk, lds, st, P->F, S.fronts, B, P->n, nbl, A, U, X, bsel); \
+    hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, blk, lds, st, FS, S.fronts, B, P->n, nbl, A, U, X, bsel); \
   } while (0)
     if (full) {
       switch (kind) { case 0: SAW_TRI(0, true); break; case 1: SAW_TRI(1, true); break;
@@ -2123,10 +2138,10 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     Timed tm(P, st, out_cb ? K_FWD : K_BWD);
     dim3 grid(nt, nsel);
     switch (kind) {
-      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, st, P->F, tl, B, P->n, A, U, X, bsel); break;
-      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, st, P->F, tl, B, P->n, A, U, X, bsel); break;
-      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, st, P->F, tl, B, P->n, A, U, X, bsel); break;
-      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, st, P->F, tl, B, P->n, A, U, X, bsel);
+      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, st, FS, tl, B, P->n, A, U, X, bsel); break;
+      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, st, FS, tl, B, P->n, A, U, X, bsel); break;
+      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, st, FS, tl, B, P->n, A, U, X, bsel); break;
+      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, st, FS, tl, B, P->n, A, U, X, bsel);
     }
   };
   // The one-workgroup set and the size buckets of a level are independent: with more than one,
@@ -2153,12 +2168,12 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     if (kind == 0) {
       if (lds > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 0, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL((k_tri_lds<T, 0, R>), dim3(grid), dim3(256), lds, st, P->F, S.fronts, S.nf, B, P->n, nbl, A,
+      hipLaunchKernelGGL((k_tri_lds<T, 0, R>), dim3(grid), dim3(256), lds, st, FS, S.fronts, S.nf, B, P->n, nbl, A,
                          U, X, bsel, nsel);
     } else {
       if (lds > 64 * 1024)
         (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 1, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL((k_tri_lds<T, 1, R>), dim3(grid), dim3(256), lds, st, P->F, S.fronts, S.nf, B, P->n, nbl, A,
+      hipLaunchKernelGGL((k_tri_lds<T, 1, R>), dim3(grid), dim3(256), lds, st, FS, S.fronts, S.nf, B, P->n, nbl, A,
                          U, X, bsel, nsel);
     }
   };
@@ -2481,10 +2496,15 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
 // NULL for all B): solve only these frequencies of the chunk; the others' x is untouched.
 int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const void* arena, void* uarena, void* x,
                     int transpose, int nb, const int* bsel, int nsel) {
+  return sawfem_mf_solve_shared(plan, stream, prec, B, nrhs, arena, B, nullptr, uarena, x, transpose, nb, bsel, nsel);
+}
+
+int sawfem_mf_solve_shared(void* plan, void* stream, int prec, int B, int nrhs, const void* arena, int Bf,
+                           const int* fslot, void* uarena, void* x, int transpose, int nb, const int* bsel, int nsel) {
   Plan* P = (Plan*)plan;
   if (!bsel) nsel = B;
   if (!P || B <= 0 || !arena || !uarena || !x || (prec != 0 && prec != 1) || (nb != 32 && nb != 64) ||
-      (prec == 1 && nb > 32) || (nrhs != 1 && nrhs != 2) || nsel < 0 || nsel > B) {
+      (prec == 1 && nb > 32) || (nrhs != 1 && nrhs != 2) || nsel < 0 || nsel > B || Bf <= 0 || (!fslot && Bf != B)) {
     sawfem_set_error("sawfem_mf_solve: invalid arguments");
     return 1;
   }
@@ -2498,10 +2518,10 @@ int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const v
   hipStream_t s = (hipStream_t)stream;
   if (nsel == 0) return 0;
   if (nrhs == 1)
-    return prec == 0 ? solve_impl<float, 1>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel)
-                     : solve_impl<double, 1>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel);
-  return prec == 0 ? solve_impl<float, 2>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel)
-                   : solve_impl<double, 2>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel);
+    return prec == 0 ? solve_impl<float, 1>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel, fslot, Bf)
+                     : solve_impl<double, 1>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel, fslot, Bf);
+  return prec == 0 ? solve_impl<float, 2>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel, fslot, Bf)
+                   : solve_impl<double, 2>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel, fslot, Bf);
 }
 
 // Per-kernel event timing: enable (1) / disable (0); turning it on clears what was recorded.

@@ -41,6 +41,8 @@ from .symbolic import analyse
 # panel) per frequency (measured on config 2 with the size buckets of a level on concurrent
 # streams: 256 is the best of 128/192/256/320/448/640)
 SOLVE_FULL_M = 256
+# consecutive frequencies per complex64 factorization (FrequencySolver.solve_batch)
+SHARE = 1
 # 32-wide pivot panels of size buckets up to this front order are factored by one kernel per
 # panel (diagonal block and both panel solves in one wave per front and frequency)
 PANEL_LU_M = 256
@@ -205,6 +207,7 @@ class FrequencySolver:
         self.mf = self.symbolic.mf
         self.analysis_seconds = time.time() - t0
         self._plan = self._create_plan()
+        self.share, self._fslot, self._Bf = SHARE, None, 0
         self.set_solve_full(SOLVE_FULL_M)
         self.set_panel_lu(PANEL_LU_M)
         self.set_tri_lds(TRI_LDS_BYTES)
@@ -374,7 +377,14 @@ class FrequencySolver:
 
     # ---------------------------------------------------------------- numeric
     def arena_bytes(self, batch, precision):
+        """Bytes of ``batch`` frequencies' fronts."""
         return self.mf.arena_elems * batch * (8 if precision == 'mixed' else 16)
+
+    def chunk_arena_bytes(self, batch, precision, share=None):
+        """Front bytes of a chunk of ``batch`` frequencies: one factorization per ``share``
+        of them in mixed precision (default ``self.share``)."""
+        share = max(1, int(self.share if share is None else share)) if precision == 'mixed' else 1
+        return self.arena_bytes(-(-batch // share), precision)
 
     def _dptr(self, t):
         return None if t is None else t.data_ptr()
@@ -394,13 +404,16 @@ class FrequencySolver:
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
         return self._ws[:nbytes].view(dtype)
 
-    def chunk_bytes(self, batch, precision='mixed', gradient=False):
+    def chunk_bytes(self, batch, precision='mixed', gradient=False, share=None):
         """Device bytes one chunk of ``batch`` frequencies takes at its peak, counted from what
         solve_batch allocates: front arena; update vectors (two right-hand sides with the
         gradient); right-hand sides, solutions, residuals and their temporaries ([B][R][n]); the
         three frequency-major residual buffers ([n][Bp], Bp = R B padded to 64); the element sums
-        of the gradient contraction; 10 % on top of everything but the arena, whose size is exact."""
+        of the gradient contraction; 10 % on top of everything but the arena, whose size is exact.
+        With ``share`` (default ``self.share``; mixed precision) the arena holds one
+        factorization per ``share`` frequencies."""
         R = 2 if gradient else 1
+        share = max(1, int(self.share if share is None else share)) if precision == 'mixed' else 1
         n = self.n_active
         Bp = -(-R * batch // 64) * 64
         vecs = (12 if gradient else 4) * batch + 3 * Bp
@@ -409,7 +422,7 @@ class FrequencySolver:
             nd = self.layout.nd - 1
             work = 16 * batch * self._grad['ne0'] * ((2 * nd) ** 2 + 4 * nd + 4)
         other = 16 * self.mf.uarena_elems * R * batch + 16 * n * vecs + work
-        return int(self.arena_bytes(batch, precision) + 1.1 * other)
+        return int(self.arena_bytes(-(-batch // share), precision) + 1.1 * other)
 
     def _workspace(self, nbytes):
         """``nbytes`` of the kept workspace (grown like the front arena)."""
@@ -453,15 +466,24 @@ class FrequencySolver:
         self.torch.cuda.empty_cache()
 
     def solve_batch(self, omegas, V=1.0, precision='mixed', ir_tol=1e-8, max_ir=8, nb=None, fallback=True,
-                    gradient=False, adjoint='auto'):
+                    gradient=False, adjoint='auto', share=None):
         """Solve all ``omegas`` in one batched factorization.
 
         Returns dict with ``x`` (B, n) complex128 device tensor, ``Q`` (B,) complex numpy,
         ``ir_steps``, ``dq`` (last relative change of Q per frequency), ``err`` (its
-        estimated remaining error), ``refactored``; with ``gradient`` also ``dY`` and the
-        adjoint's ``adj_*`` entries.  ``adjoint``: 'auto' pairs the adjoint with the forward
-        solve when :meth:`scaled_symmetric` holds, 'paired' requires it, 'transposed' always
-        uses separate transposed solves.
+        estimated remaining error), ``refactored`` (complex128 refactorization used),
+        ``own_factors`` (refactored in complex64 after sharing failed); with ``gradient`` also
+        ``dY`` and the adjoint's ``adj_*`` entries.  ``adjoint``: 'auto' pairs the adjoint with
+        the forward solve when :meth:`scaled_symmetric` holds, 'paired' requires it,
+        'transposed' always uses separate transposed solves.
+
+        ``share`` (mixed precision; default ``self.share``): consecutive frequencies per
+        factorization.  Groups of ``share`` frequencies are factored once, at the mean of their
+        angular frequencies, and each frequency is refined against its own FP64 operator with
+        those factors (``sawfem_mf_solve_shared``): a nearby frequency's factors precondition
+        the refinement, which takes a few more steps instead of a factorization.  Frequencies
+        that do not converge that way are refactored with their own complex64 factors, and
+        those that still do not, in complex128.
         """
         torch = self.torch
         dev = self.dev
@@ -479,11 +501,21 @@ class FrequencySolver:
             if adjoint == 'paired' and not paired:
                 raise ValueError('D A is not symmetric: the paired adjoint does not apply')
         R = 2 if paired else 1
+        share = max(1, int(self.share if share is None else share)) if prec == 0 else 1
+        share = min(share, B)
         s = stream_ptr()
         om = torch.as_tensor(omegas, device=dev)
-        arena = self._arena(self.mf.arena_elems * B, torch.complex64 if prec == 0 else torch.complex128)
+        group = np.arange(B) // share
+        Bf = int(group[-1]) + 1
+        if share > 1:
+            om_f = torch.as_tensor(np.bincount(group, weights=omegas) / np.bincount(group), device=dev)
+            self._fslot = torch.as_tensor(group.astype(np.int32), device=dev)
+        else:
+            om_f, self._fslot = om, None
+        self._Bf = Bf
+        arena = self._arena(self.mf.arena_elems * Bf, torch.complex64 if prec == 0 else torch.complex128)
         uarena = torch.empty(max(1, self.mf.uarena_elems * B * R), dtype=torch.complex128, device=dev)
-        _lib.call('sawfem_mf_factor', self._plan, s, prec, B, om.data_ptr(), self.beta, self.Kaa.data_ptr(),
+        _lib.call('sawfem_mf_factor', self._plan, s, prec, Bf, om_f.data_ptr(), self.beta, self.Kaa.data_ptr(),
                   self.Maa.data_ptr(), self._dptr(self.Daa), arena.data_ptr(), nb)
         rhs = torch.empty((B, n), dtype=torch.complex128, device=dev)
         _lib.call('sawfem_rhs', s, n, B, om.data_ptr(), self.beta, float(V), self.rK.data_ptr(), self.rM.data_ptr(),
@@ -494,20 +526,29 @@ class FrequencySolver:
             out = self._solve_single(om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir)
             if gradient:
                 out.update(self._adjoint(om, omegas, V, prec, nb, arena, uarena, rhs, out['x'], ir_tol, max_ir))
+        self._fslot = None
         x, err = out['x'], out['err']
         out['perturbed'], out['pivot_min_ratio'] = self.pivot_stats(reset=True)
+        out['share'], out['factorizations'] = share, Bf
         refactored = np.zeros(B, dtype=bool)
+        own = np.zeros(B, dtype=bool)
         if prec == 0 and fallback:
             conv = out['converged'] & out.get('adj_converged', True)
             bad = np.flatnonzero(~conv)
             if bad.size:
                 del arena, uarena
-                # complex128 refactorization in chunks that fit the workspace of this call
-                nfit = max(1, int(self._ws.numel() // self.arena_bytes(1, 'fp64')))
+                # shared factors: their own complex64 factorization first (which falls back to
+                # complex128 itself); own factors: complex128.  In chunks that fit the workspace.
+                sub_prec = 'mixed' if share > 1 else 'fp64'
+                nfit = max(1, int(self._ws.numel() // self.arena_bytes(1, sub_prec)))
                 for c0 in range(0, bad.size, nfit):
                     part = bad[c0:c0 + nfit]
-                    sub = self.solve_batch(omegas[part], V, precision='fp64', ir_tol=ir_tol, max_ir=max_ir,
-                                           gradient=gradient, adjoint=adjoint)
+                    sub = self.solve_batch(omegas[part], V, precision=sub_prec, ir_tol=ir_tol, max_ir=max_ir,
+                                           gradient=gradient, adjoint=adjoint, share=1)
+                    if sub_prec == 'mixed':
+                        own[part] = True
+                        refactored[part] = sub['refactored']
+                        out['factorizations'] += sub['factorizations']
                     tb = self.torch.as_tensor(part, device=self.dev)
                     x[tb] = sub['x']
                     if 'lam' in out:
@@ -515,11 +556,29 @@ class FrequencySolver:
                     for key in ('Q', 'dq', 'err', 'dY', 'adj_dq', 'adj_err', 'converged', 'adj_converged', 'dwr_mask'):
                         if key in out:
                             out[key][part] = sub[key]
-                    out['perturbed_fp64'] = out.get('perturbed_fp64', 0) + sub['perturbed']
+                    out['perturbed_' + sub_prec] = out.get('perturbed_' + sub_prec, 0) + sub['perturbed']
                     del sub
-                refactored[bad] = True
+                if sub_prec == 'fp64':
+                    refactored[bad] = True
         out['refactored'] = refactored
+        out['own_factors'] = own
         return out
+
+    def set_share(self, k):
+        """Consecutive frequencies per complex64 factorization (see :meth:`solve_batch`)."""
+        k = int(k)
+        if k < 1:
+            raise ValueError('share must be >= 1')
+        self.share = k
+        return k
+
+    def _solve(self, prec, B, R, arena, uarena, vec, transpose, nb, sel=None, nsel=0):
+        """Forward + backward solve of B right-hand sides with the chunk's factors (shared when
+        ``solve_batch`` factored groups of frequencies)."""
+        fs = self._fslot
+        _lib.call('sawfem_mf_solve_shared', self._plan, stream_ptr(), prec, B, R, arena.data_ptr(),
+                  self._Bf if fs is not None else B, None if fs is None else fs.data_ptr(), uarena.data_ptr(),
+                  vec.data_ptr(), transpose, nb, sel, nsel)
 
     def pivot_stats(self, reset=True):
         """(pivots the guard replaced, smallest |pivot| / row scale) since the last reset
@@ -582,8 +641,7 @@ class FrequencySolver:
                 r[torch.as_tensor(~active, device=self.dev)] = 0
                 idx = torch.as_tensor(np.flatnonzero(active).astype(np.int32), device=self.dev)
                 sel, nsel = idx.data_ptr(), int(idx.numel())
-            _lib.call('sawfem_mf_solve', self._plan, s, prec, B, R, arena.data_ptr(), uarena.data_ptr(), r.data_ptr(),
-                      transpose, nb, sel, nsel)
+            self._solve(prec, B, R, arena, uarena, r, transpose, nb, sel, nsel)
             _lib.call('sawfem_axpy', s, X.numel(), X.data_ptr(), r.data_ptr())
             applied = steps
             new = measure()
@@ -613,8 +671,7 @@ class FrequencySolver:
         s = stream_ptr()
         B, n = rhs.shape
         x = rhs.clone()
-        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), x.data_ptr(), 0,
-                  nb, None, 0)
+        self._solve(prec, B, 1, arena, uarena, x, 0, nb)
         Qd = torch.empty(B, dtype=torch.complex128, device=self.dev)
 
         def measure():
@@ -641,8 +698,7 @@ class FrequencySolver:
         d = self._dscale
         X = torch.stack([rhs, (self.kts_a / d).unsqueeze(0).expand(B, n)], dim=1).contiguous()   # [B][2][n]
         rhs2 = X.clone()
-        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 2, arena.data_ptr(), uarena.data_ptr(), X.data_ptr(), 0,
-                  nb, None, 0)
+        self._solve(prec, B, 2, arena, uarena, X, 0, nb)
         Qd = torch.empty(2 * B, dtype=torch.complex128, device=self.dev)
 
         def measure():
@@ -679,8 +735,7 @@ class FrequencySolver:
         B, n = x.shape
         gvec = self.kts_a.unsqueeze(0).expand(B, n).contiguous()
         lam = gvec.clone()
-        _lib.call('sawfem_mf_solve', self._plan, s, prec, B, 1, arena.data_ptr(), uarena.data_ptr(), lam.data_ptr(), 1,
-                  nb, None, 0)
+        self._solve(prec, B, 1, arena, uarena, lam, 1, nb)
         measure = lambda: [(lam * rhs).sum(dim=1).cpu().numpy()]
         res = self._residual_op(B, om)
         _, (dq,), (err,), steps, _ = self._refine(prec, nb, arena, uarena, lam, 1, res, res.to_t(gvec), self.tslot,
@@ -709,21 +764,23 @@ class FrequencySolver:
         Wh = W.cpu().numpy().reshape(B, NT)
         return dict(dY=self._param_gradient(Wh, omegas, V))
 
-    def auto_batch(self, precision='mixed', gradient=False, budget=None):
+    def auto_batch(self, precision='mixed', gradient=False, budget=None, share=None):
         """Frequencies per chunk (:meth:`chunk_bytes`) that fit ``budget`` bytes, by default the
         device memory available above the reserve (:func:`hbm_available`, counting the current
         workspace as free)."""
         if budget is None:
             ws = getattr(self, '_ws', None)
             budget = hbm_available(0 if ws is None else ws.numel())
-        per = self.chunk_bytes(1, precision, gradient)
+        share = max(1, int(self.share if share is None else share)) if precision == 'mixed' else 1
+        per = self.chunk_bytes(1, precision, gradient, share)
         if budget < per:
             raise ValueError(f'one frequency needs {per / 1e9:.1f} GB of device memory; '
                              f'{max(budget, 0) / 1e9:.1f} GB are available above the reserve')
-        lo, hi = 1, max(1, int(budget // self.arena_bytes(1, precision))) + 1   # chunk_bytes(hi) > budget
+        # chunk_bytes(hi) > budget: its arena alone exceeds the budget
+        lo, hi = 1, share * (max(1, int(budget // self.arena_bytes(1, precision))) + 1)
         while hi - lo > 1:                      # chunk_bytes grows with the batch: bisect
             mid = (lo + hi) // 2
-            lo, hi = (mid, hi) if self.chunk_bytes(mid, precision, gradient) <= budget else (lo, mid)
+            lo, hi = (mid, hi) if self.chunk_bytes(mid, precision, gradient, share) <= budget else (lo, mid)
         return lo
 
     def sweep_charges(self, fre, V=1.0, precision='mixed', batch=None, gradient=False, on_chunk=None, **kw):
@@ -750,7 +807,8 @@ class FrequencySolver:
             Q[i0:i0 + batch] = res['Q']
             item = dict(ir_steps=res['ir_steps'], dq_max=float(np.max(res['dq'])),
                         err_est_max=float(np.max(res['err'])), refactored=int(res['refactored'].sum()),
-                        perturbed=res['perturbed'], pivot_min_ratio=res['pivot_min_ratio'])
+                        perturbed=res['perturbed'], pivot_min_ratio=res['pivot_min_ratio'], share=res['share'],
+                        factorizations=res['factorizations'], own_factors=int(res['own_factors'].sum()))
             if gradient:
                 dY[i0:i0 + batch] = res['dY']
                 item['adj_dq_max'] = float(np.max(res['adj_dq']))

@@ -1,7 +1,10 @@
 """Kernel-selection tuner of the batched multifrontal solver.
 
-The plan has three kernel-selection knobs whose best setting depends on the cell's front sizes
-and on the chunk size:
+The engine has one algorithmic knob, ``share`` (consecutive frequencies per complex64
+factorization, :meth:`sweep.FrequencySolver.solve_batch`), whose best value depends on how far
+apart the sweep's frequencies are relative to the cell's resonances: :func:`tune_share` times
+each candidate on a block of the sweep's own frequencies.  The plan has three kernel-selection
+knobs whose best setting depends on the cell's front sizes and on the chunk size:
 
 * ``panel_lu`` (factorization): 32-wide pivot panels of size buckets up to this front order are
   factored by one kernel per panel instead of the diagonal-block kernel and two MFMA GEMMs;
@@ -99,3 +102,46 @@ def autotune(eng, freqs, precision='mixed', gradient=True, batch=None, reps=3, c
     eng.pivot_stats(reset=True)                      # the tuning factorizations are not the sweep's
     eng.tuning = dict(choice=choice, ms=ms, batch=B)
     return eng.tuning
+
+
+SHARE_CANDIDATES = (1, 2, 4, 8)
+
+
+def tune_share(eng, freqs, precision='mixed', gradient=True, candidates=SHARE_CANDIDATES, margin=0.02):
+    """Pick the factorization sharing of the sweep over ``freqs`` (Hz).  Each candidate solves the
+    chunk it would run in the sweep: as many of the sweep's frequencies as fit the device memory
+    at that sharing (more with more sharing: the arena holds one factorization per ``share``
+    frequencies), a contiguous block from the middle of the band, refinement and fallbacks
+    included, timed with the device synchronized.  The choice minimizes the time per frequency;
+    a candidate replaces share 1 only when it is faster by more than ``margin``.  Returns
+    ``dict(choice=..., ms=..., batch=...)``; the engine is left set to the choice."""
+    import time
+    torch = eng.torch
+    freqs = np.asarray(freqs, dtype=float).ravel()
+    P = freqs.size
+    candidates = tuple(dict.fromkeys((1,) + tuple(candidates)))
+    ws = getattr(eng, '_ws', None)
+    keep = 0 if ws is None else ws.numel()
+    del ws                       # a reference here would keep the workspace alive when it grows
+    from .sweep import hbm_available
+    budget = hbm_available(keep)
+    batch = {c: int(min(P, eng.auto_batch(precision, gradient, budget=budget, share=c))) for c in candidates}
+    # the largest chunk arena up front, so that no candidate's timing includes growing it
+    eng._arena(max(eng.chunk_arena_bytes(b, precision, c) for c, b in batch.items()) // 8, torch.complex64)
+    ms, per = {}, {}
+    for c in candidates:
+        B = batch[c]
+        lo = max(0, min(P - B, P // 2 - B // 2))
+        om = 2 * np.pi * freqs[lo:lo + B]
+        eng.solve_batch(om[:min(B, 2 * c)], precision=precision, gradient=gradient, share=c)     # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.solve_batch(om, precision=precision, gradient=gradient, share=c)
+        torch.cuda.synchronize()
+        ms[c] = 1e3 * (time.perf_counter() - t0)
+        per[c] = ms[c] / B
+    best = min(per, key=per.get)
+    choice = best if per[best] < (1 - margin) * per[1] else 1
+    eng.set_share(choice)
+    eng.pivot_stats(reset=True)
+    return dict(choice=choice, ms=ms, batch=batch, ms_per_frequency=per)
