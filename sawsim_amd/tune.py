@@ -104,15 +104,16 @@ def autotune(eng, freqs, precision='mixed', gradient=True, batch=None, reps=3, c
     return eng.tuning
 
 
-SHARE_CANDIDATES = (1, 2, 4, 8)
+SHARE_CANDIDATES = (1, 2, 4, 8, 16)
 
 
-def tune_share(eng, freqs, precision='mixed', gradient=True, candidates=SHARE_CANDIDATES, margin=0.02):
+def tune_share(eng, freqs, precision='mixed', gradient=True, candidates=SHARE_CANDIDATES, margin=0.02, reps=2):
     """Pick the factorization sharing of the sweep over ``freqs`` (Hz).  Each candidate solves the
     chunk it would run in the sweep: as many of the sweep's frequencies as fit the device memory
     at that sharing (more with more sharing: the arena holds one factorization per ``share``
     frequencies), a contiguous block from the middle of the band, refinement and fallbacks
-    included, timed with the device synchronized.  The choice minimizes the time per frequency;
+    included, timed with the device synchronized (the fastest of ``reps`` runs).  The choice
+    minimizes the time per frequency;
     a candidate replaces share 1 only when it is faster by more than ``margin``.  Returns
     ``dict(choice=..., ms=..., batch=...)``; the engine is left set to the choice."""
     import time
@@ -133,12 +134,16 @@ def tune_share(eng, freqs, precision='mixed', gradient=True, candidates=SHARE_CA
         B = batch[c]
         lo = max(0, min(P - B, P // 2 - B // 2))
         om = 2 * np.pi * freqs[lo:lo + B]
-        eng.solve_batch(om[:min(B, 2 * c)], precision=precision, gradient=gradient, share=c)     # warm-up
+        # warm-up with this chunk's vector sizes (one factorization for all of it, one step), so
+        # that the timed call finds the allocator's blocks and the kernels' code in place
+        eng.solve_batch(om, precision=precision, gradient=gradient, share=B, max_ir=1, fallback=False)
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        eng.solve_batch(om, precision=precision, gradient=gradient, share=c)
-        torch.cuda.synchronize()
-        ms[c] = 1e3 * (time.perf_counter() - t0)
+        for _ in range(reps):                       # the fastest of reps runs
+            t0 = time.perf_counter()
+            eng.solve_batch(om, precision=precision, gradient=gradient, share=c)
+            torch.cuda.synchronize()
+            t = 1e3 * (time.perf_counter() - t0)
+            ms[c] = min(ms.get(c, t), t)
         per[c] = ms[c] / B
     best = min(per, key=per.get)
     choice = best if per[best] < (1 - margin) * per[1] else 1
