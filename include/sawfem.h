@@ -249,6 +249,27 @@ int sawfem_mf_solve(void* plan, void* stream, int prec, int B, int nrhs, const v
 int sawfem_mf_solve_shared(void* plan, void* stream, int prec, int B, int nrhs, const void* arena, int Bf,
                            const int* fslot, void* uarena, void* x, int transpose, int nb, const int* bsel, int nsel);
 
+/* sawfem_mf_solve_shared for uniform groups, complex64 factors: frequencies b of the chunk
+ * ([B][nrhs][n] vectors in x) share the factors of arena slot b / G (the arena holds ceil(B/G)
+ * factorizations).  One workgroup solves the vectors of several frequencies of a group
+ * together (up to the block size set by sawfem_mf_set_group_rc, default 16 vectors), so each
+ * factor entry is read once for all of them.  gsel (device int[ngsel], or NULL for all):
+ * the full groups (G frequencies) to solve; tail (0/1): whether the last, partial group
+ * (frequencies G floor(B/G) .. B-1) is solved.  The vectors of unselected groups are left
+ * untouched; every frequency of a selected group is solved (a zero right-hand side gives a zero
+ * solution).  No counterpart in the reference (it factors every frequency). */
+int sawfem_mf_solve_groups(void* plan, void* stream, int B, int nrhs, const void* arena, int G, void* uarena,
+                           void* x, int transpose, int nb, const int* gsel, int ngsel, int tail);
+/* Largest block of vectors one workgroup of sawfem_mf_solve_groups takes (1..16; the block
+ * size used is the largest R 2^k within it that divides G nrhs and fits the LDS).  Returns the
+ * value in use or -1.  No counterpart in the reference. */
+int sawfem_mf_set_group_rc(void* plan, int rc_max);
+/* sawfem_mf_solve_groups kernel selection: the one-workgroup solve buckets use the LDS-staged
+ * block kernel where the padded vectors and the factor block of blocks of 4 vectors fit
+ * lds_bytes of LDS (default 48 KiB; 0: never); the other kernels take blocks of at most rc_max
+ * vectors (default 4).  Returns rc_max in use or -1.  No counterpart in the reference. */
+int sawfem_mf_set_group_tri(void* plan, int rc_max, int lds_bytes);
+
 /* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;
  * used by the benchmark).  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S,
  * 6 fwd, 7 bwd, 8 fused small fronts.  profile_read waits for the recorded events and

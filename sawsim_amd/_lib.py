@@ -48,6 +48,9 @@ SIGNATURES = {
     'sawfem_mf_factor': [_vp, _vp, _i, _i, _vp, _d, _vp, _vp, _vp, _vp, _i],
     'sawfem_mf_solve': [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _i],
     'sawfem_mf_solve_shared': [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _vp, _i],
+    'sawfem_mf_solve_groups': [_vp, _vp, _i, _i, _vp, _i, _vp, _vp, _i, _i, _vp, _i, _i],
+    'sawfem_mf_set_group_rc': [_vp, _i],
+    'sawfem_mf_set_group_tri': [_vp, _i, _i],
     'sawfem_mf_profile': [_vp, _i],
     'sawfem_mf_profile_read': [_vp, _vp, _vp],
     'sawfem_rhs': [_vp, _i, _i, _vp, _d, _d, _vp, _vp, _vp, _vp],

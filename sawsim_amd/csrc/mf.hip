@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -49,10 +50,19 @@ struct FrontsDev {
   const int* child_ids;
   const int64_t* inv_ptr;  // per child slot: parent-row -> child CB index map (length m_parent)
   const int* inv;
-  // solves only: right-hand side b uses the factors of arena slot fslot[b] of an arena of Bf
-  // frequencies (null: slot b of an arena of B, one factorization per right-hand side)
+  // solves only.  A solve launch works on blocks of R vectors (R = the kernels' template
+  // argument); block b is vectors b R .. b R + R - 1 of the vector array.  Block b reads the
+  // factors of arena slot fslot[b] of an arena of Bf factorizations, or slot b / vdiv when
+  // fslot is null and vdiv > 0 (uniform groups), or slot b of an arena of B (one factorization
+  // per block).  Grid row y is block sel[y / cpg] * cpg + y % cpg with a selection list (cpg
+  // blocks per selected entry), else block boff + y.  nvec: vectors of the whole call (the
+  // update vectors of front f start at uoff[f] * nvec).
   const int* fslot = nullptr;
   int Bf = 0;
+  int vdiv = 0;
+  int cpg = 1;
+  int boff = 0;
+  int64_t nvec = 0;
 };
 
 template <typename T>
@@ -66,7 +76,14 @@ __device__ __forceinline__ typename cplx<T>::type* front_ptr(typename cplx<T>::t
 template <typename T>
 __device__ __forceinline__ typename cplx<T>::type* factor_ptr(typename cplx<T>::type* arena, const FrontsDev& F,
                                                               int f, int b, int B) {
-  return F.fslot ? front_ptr<T>(arena, F, f, F.fslot[b], F.Bf) : front_ptr<T>(arena, F, f, b, B);
+  if (F.fslot) return front_ptr<T>(arena, F, f, F.fslot[b], F.Bf);
+  if (F.vdiv > 0) return front_ptr<T>(arena, F, f, b / F.vdiv, F.Bf);
+  return front_ptr<T>(arena, F, f, b, B);
+}
+
+// Vector block of grid row y of a solve launch (see FrontsDev).
+__device__ __forceinline__ int solve_block(const FrontsDev& F, const int* sel, int y) {
+  return sel ? sel[y / F.cpg] * F.cpg + y % F.cpg : F.boff + y;
 }
 
 // Strided loop over idx = first, first + step, ... < count with its global loads issued U at a
@@ -927,11 +944,12 @@ __global__ void __launch_bounds__(256, (PRE && sizeof(T) == 4 && TC != 128) ? 3 
 //   * a panel kernel over 64-wide tiles of the off-diagonal panel (L21, U12 or their
 //     transposes), many workgroups per front, coalesced loads (transposed panels go through
 //     an LDS tile).
-// Each frequency carries R right-hand sides (1 or 2) that share its factors: every factor
-// entry is read once and applied to all R vectors.
-// Vectors: x [B][R][n] complex128 (pivot rows of each front are owned by that front); the
-// update vectors of front f (its contribution to the parent's rows) live in uarena at
-// uoff[f] * B * R + (b * R + r) * (m - p).
+// A workgroup solves one block of R right-hand sides that share one factorization (the one or
+// two vectors of a frequency, or those of a group of frequencies factored together): every
+// factor entry is read once and applied to all R vectors.
+// Vectors: x [nvec][n] complex128, block b = rows b R .. b R + R - 1 (pivot rows of each front
+// are owned by that front); the update vectors of front f (its contribution to the parent's
+// rows) live in uarena at uoff[f] * nvec + (b * R + r) * (m - p).
 template <typename T> struct SolveTile;
 template <> struct SolveTile<float> { static constexpr int SB = 64; };    // widest panel, complex64
 template <> struct SolveTile<double> { static constexpr int SB = 32; };   // widest panel, complex128
@@ -956,7 +974,7 @@ __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, i
       if (ci >= 0) {
         const int ch = F.child_ids[slot];
         const int qc = F.m[ch] - F.p[ch];
-        const double2* u = uarena + F.uoff[ch] * B * R + (int64_t)b * R * qc + ci;
+        const double2* u = uarena + F.uoff[ch] * F.nvec + (int64_t)b * R * qc + ci;
 #pragma unroll
         for (int r = 0; r < R; ++r) v[r] = d2add(v[r], u[(int64_t)r * qc]);
       }
@@ -968,11 +986,20 @@ __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, i
 #pragma unroll
       for (int r = 0; r < R; ++r) w[r * p + i] = v[r];
     } else {
-      double2* uf = uarena + F.uoff[f] * B * R + (int64_t)b * R * q + (i - p);
+      double2* uf = uarena + F.uoff[f] * F.nvec + (int64_t)b * R * q + (i - p);
 #pragma unroll
       for (int r = 0; r < R; ++r) uf[(int64_t)r * q] = v[r];
     }
   }
+}
+
+// Bytes of k_tri's tile region: the diagonal block (kinds 0/1) or a 64-row panel tile (kinds
+// 2/3) of factor entries, and with FULL kind 1 at least the R x threads partial sums of the
+// CB-to-pivot product, which use the region before the diagonal blocks do.  A multiple of 16.
+__host__ __device__ inline size_t tri_tile_bytes(size_t vsize, int kind, int R, bool full, int nbl, int threads) {
+  const size_t t = vsize * (size_t)(kind >= 2 ? 64 : nbl) * (size_t)(nbl + 1);
+  const size_t red = (full && kind == 1) ? sizeof(double2) * (size_t)R * (size_t)threads : 0;
+  return ((t > red ? t : red) + 15) / 16 * 16;
 }
 
 // Pivot triangle.  kind 0: L11 (unit lower, forward)   1: U11 (upper, backward)
@@ -992,7 +1019,7 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
   constexpr bool ROWTILE = KIND >= 2;          // the tile also stages 64 columns of a row panel
   extern __shared__ unsigned char smem_raw[];
   V* tile = reinterpret_cast<V*>(smem_raw);                                   // [nbl or 64][LD]
-  double2* part = reinterpret_cast<double2*>(smem_raw + sizeof(V) * (ROWTILE ? 64 : nbl) * LD);  // [R][4][64]
+  double2* part = reinterpret_cast<double2*>(smem_raw + tri_tile_bytes(sizeof(V), KIND, R, FULL, nbl, blockDim.x));  // [R][4][PS]
   const int PS = nbl <= 32 ? 32 : 64;                                         // rows of a block
   double2* w = part + R * 4 * PS;                                             // [R][W]
   const int f = lvl[blockIdx.x];
@@ -1017,8 +1044,8 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
       // 64 pivot rows at a time, the q CB columns split over G = blockDim / 64 thread groups so
       // that every thread has loads in flight (one thread per row left 3/4 of the workgroup idle
       // and each row a chain of q / 8 dependent load batches); the G partial sums meet in the
-      // tile region, unused until the diagonal blocks (R * blockDim double2 fit it: checked by
-      // the launch)
+      // tile region, unused until the diagonal blocks (sized for R * blockDim double2 by
+      // tri_tile_bytes)
       double2* red = reinterpret_cast<double2*>(tile);
       const int G = blockDim.x / 64;
       const int i = t & 63, g = t >> 6;
@@ -1182,7 +1209,7 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
 #pragma unroll
       for (int r = 0; r < R; ++r) x[((int64_t)b * R + r) * n + rows[i]] = w[r * W + i];
     } else if (fwd) {
-      double2* uf = uarena + F.uoff[f] * B * R + (int64_t)b * R * (m - p) + (i - p);
+      double2* uf = uarena + F.uoff[f] * F.nvec + (int64_t)b * R * (m - p) + (i - p);
 #pragma unroll
       for (int r = 0; r < R; ++r) uf[(int64_t)r * (m - p)] = w[r * W + i];
     }
@@ -1194,7 +1221,7 @@ __global__ void __launch_bounds__(512) k_tri(FrontsDev F, const int* __restrict_
                                              const typename cplx<T>::type* __restrict__ arena,
                                              double2* __restrict__ uarena, double2* __restrict__ x,
                                              const int* __restrict__ bsel) {
-  tri_body<T, KIND, R, FULL>(F, lvl, B, n, nbl, arena, uarena, x, bsel ? bsel[blockIdx.y] : (int)blockIdx.y);
+  tri_body<T, KIND, R, FULL>(F, lvl, B, n, nbl, arena, uarena, x, solve_block(F, bsel, blockIdx.y));
 
 }
 
@@ -1256,7 +1283,7 @@ __global__ void __launch_bounds__(256) k_tri_lds(FrontsDev F, const int* __restr
   V* Pl = reinterpret_cast<V*>(part + R * 4 * 64);               // [p][m] (kind 0) or [m][p] (kind 1)
   T px[NPF], py[NPF];
   auto front_of = [&](int it) { return lvl[it % nf]; };
-  auto freq_of = [&](int it) { return bsel ? bsel[it / nf] : it / nf; };
+  auto freq_of = [&](int it) { return solve_block(F, bsel, it / nf); };
   int item = blockIdx.x;
   if (item < nitems) tri_lds_fetch<T, KIND, NPF>(F, arena, front_of(item), freq_of(item), B, px, py);
   for (; item < nitems; item += gridDim.x) {
@@ -1366,12 +1393,153 @@ __global__ void __launch_bounds__(256) k_tri_lds(FrontsDev F, const int* __restr
 #pragma unroll
         for (int r = 0; r < R; ++r) x[((int64_t)b * R + r) * n + rows[i]] = w[r * m + i];
       } else if (KIND == 0) {
-        double2* uf = uarena + F.uoff[f] * B * R + (int64_t)b * R * (m - p) + (i - p);
+        double2* uf = uarena + F.uoff[f] * F.nvec + (int64_t)b * R * (m - p) + (i - p);
 #pragma unroll
         for (int r = 0; r < R; ++r) uf[(int64_t)r * (m - p)] = w[r * m + i];
       }
     }
     __syncthreads();          // w and Pl are rewritten by the next item
+  }
+}
+
+// Whole-front solves of one block of RC vectors (RC = 4, 8 or 16: the vectors of a group of
+// frequencies that share one complex64 factorization), the vectors staged in LDS.  Every step
+// is a small product whose (row, vector) outputs are spread over the 256 threads: TPR = RC / 4
+// threads per row, four vectors each, so one factor entry serves four vectors of a thread and
+// the TPR threads of a row read it together (one cache line per wave and step: the rows of a
+// column are contiguous).  A thread's factor loads are issued eight at a time; only the vectors
+// use LDS (m (RC + 1) entries), so several workgroups share a CU and hide each other's loads.
+//  kind 0 (forward):  w = the block's vectors at the pivot rows plus the children's update
+//                     vectors, over all m rows; per diagonal block w_blk = L_D^{-1} w_blk, then
+//                     w[below] -= L[below][blk] w_blk; pivot rows to x, the other rows to the
+//                     front's update vectors
+//  kind 1 (backward): w = x over all m rows (the contribution-block rows are final);
+//                     w[0:p] -= U12 w[p:m], then per diagonal block, last first,
+//                     w_blk = U_D^{-1} w_blk, w[above] -= U[above][blk] w_blk; pivot rows to x
+// The diagonal blocks hold L_D^{-1} strictly below and U_D^{-1} on/above the diagonal (nbl wide).
+// w rows are padded to RC + 1 entries (the gather's row-strided LDS writes then spread over the
+// banks).
+constexpr int GRP_NB = 8;      // factor loads of a thread in flight together
+template <int KIND, int RC>
+__global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
+                                                 const float2* __restrict__ arena, double2* __restrict__ uarena,
+                                                 double2* __restrict__ x, const int* __restrict__ sel) {
+  static_assert(KIND == 0 || KIND == 1, "forward L or backward U");
+  constexpr int VPT = 4, TPR = RC / VPT, RPP = 256 / TPR, WS = RC + 1;
+  static_assert(RC % VPT == 0 && RC <= 16, "blocks of 4, 8 or 16 vectors");
+  extern __shared__ unsigned char smem_raw[];
+  const int f = lvl[blockIdx.x];
+  const int b = solve_block(F, sel, blockIdx.y);
+  const int m = F.m[f], p = F.p[f], q = m - p;
+  double2* w = reinterpret_cast<double2*>(smem_raw);                  // [m][WS]
+  const float2* Fb = factor_ptr<float>(const_cast<float2*>(arena), F, f, b, B);
+  const int* rows = F.rows + F.rows_ptr[f];
+  const int t = threadIdx.x;
+  const int64_t v0 = (int64_t)b * RC;           // first vector of the block
+  // vectors, row index fastest over the threads (runs of x rows coalesce); kind 0 keeps zero
+  // at the contribution-block rows before the children's updates
+  const int mr = m * RC;
+  batched<8>(t, mr, 256,
+             [&](int idx) {
+               const int r = idx / m, i = idx - r * m;
+               return x[(v0 + r) * n + rows[i]];
+             },
+             [&](int idx, double2 v) {
+               const int r = idx / m, i = idx - r * m;
+               w[i * WS + r] = (KIND == 1 || i < p) ? v : make_double2(0, 0);
+             });
+  if (KIND == 0) {
+    const int c0 = F.child_ptr[f], nch = F.child_ptr[f + 1] - c0;
+    for (int c = 0; c < nch; ++c) {
+      const int* inv = F.inv + F.inv_ptr[c0 + c];
+      const int ch = F.child_ids[c0 + c];
+      const int qc = F.m[ch] - F.p[ch];
+      const double2* u = uarena + F.uoff[ch] * F.nvec + v0 * qc;
+      // each thread adds to the w entries it wrote above (same index -> thread map): no barrier
+      batched<8>(t, mr, 256,
+                 [&](int idx) {
+                   const int r = idx / m, i = idx - r * m;
+                   return u[(int64_t)r * qc + max(inv[i], 0)];
+                 },
+                 [&](int idx, double2 v) {
+                   const int r = idx / m, i = idx - r * m;
+                   if (inv[i] >= 0) w[i * WS + r] = d2add(w[i * WS + r], v);
+                 });
+    }
+  }
+  __syncthreads();
+  const int tr = t / TPR, r0 = (t % TPR) * VPT;     // row slot, first vector of the thread
+  double2 s[VPT];
+  // s -= sum_{k in [k0, k1)} F[i][c0 + k] w[wrow0 + k] (entries of row i of the column-major front)
+  auto row_update = [&](int i, int c0, int k0, int k1, int wrow0) {
+    const float2* col = Fb + i + (int64_t)c0 * m;
+    batched<GRP_NB>(k0, k1, 1, [&](int k) { return col[(int64_t)k * m]; }, [&](int k, float2 e) {
+      const double2 a = to_d2(e);
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) s[v] = cfms(s[v], a, w[(wrow0 + k) * WS + r0 + v]);
+    });
+  };
+  if (KIND == 1 && q > 0) {
+    // contribution-block rows -> pivot rows: w[i] -= sum_j U[i][p + j] w[p + j]
+    for (int i = tr; i < p; i += RPP) {
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) s[v] = w[i * WS + r0 + v];
+      row_update(i, p, 0, q, p);
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) w[i * WS + r0 + v] = s[v];
+    }
+    __syncthreads();
+  }
+  const int nblk = (p + nbl - 1) / nbl;
+  for (int bi = 0; bi < nblk; ++bi) {
+    const int J0 = KIND == 0 ? bi * nbl : (nblk - 1 - bi) * nbl;
+    const int nbk = min(nbl, p - J0);
+    // diagonal block (nbk <= 64 <= RPP rows: one row per thread slot), from the old values
+    const bool own = tr < nbk;
+    if (own) {
+      const int i = tr;
+      if (KIND == 0) {          // L_D^{-1} (unit lower): w_i + sum_{k<i} Li[i][k] w_k
+#pragma unroll
+        for (int v = 0; v < VPT; ++v) s[v] = w[(J0 + i) * WS + r0 + v];
+        const float2* row = Fb + (J0 + i) + (int64_t)J0 * m;
+        batched<GRP_NB>(0, i, 1, [&](int k) { return row[(int64_t)k * m]; }, [&](int k, float2 e) {
+          const double2 a = to_d2(e);
+#pragma unroll
+          for (int v = 0; v < VPT; ++v) s[v] = cfma(s[v], a, w[(J0 + k) * WS + r0 + v]);
+        });
+      } else {                  // U_D^{-1}: sum_{k>=i} Ui[i][k] w_k
+#pragma unroll
+        for (int v = 0; v < VPT; ++v) s[v] = make_double2(0, 0);
+        const float2* row = Fb + (J0 + i) + (int64_t)J0 * m;
+        batched<GRP_NB>(i, nbk, 1, [&](int k) { return row[(int64_t)k * m]; }, [&](int k, float2 e) {
+          const double2 a = to_d2(e);
+#pragma unroll
+          for (int v = 0; v < VPT; ++v) s[v] = cfma(s[v], a, w[(J0 + k) * WS + r0 + v]);
+        });
+      }
+    }
+    __syncthreads();
+    if (own) {
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) w[(J0 + tr) * WS + r0 + v] = s[v];
+    }
+    __syncthreads();
+    // the other rows of the block's columns: forward below the block (all m rows), backward above
+    const int lo = KIND == 0 ? J0 + nbk : 0, hi = KIND == 0 ? m : J0;
+    for (int i = lo + tr; i < hi; i += RPP) {
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) s[v] = w[i * WS + r0 + v];
+      row_update(i, J0, 0, nbk, J0);
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) w[i * WS + r0 + v] = s[v];
+    }
+    __syncthreads();
+  }
+  double2* uf = uarena + F.uoff[f] * F.nvec + v0 * q;
+  for (int idx = t; idx < mr; idx += 256) {
+    const int r = idx / m, i = idx - r * m;
+    if (i < p) x[(v0 + r) * n + rows[i]] = w[i * WS + r];
+    else if (KIND == 0) uf[(int64_t)r * q + (i - p)] = w[i * WS + r];
   }
 }
 
@@ -1398,7 +1566,7 @@ __device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ 
   const int no = min(64, nout - o0);
   const V* Fb = factor_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
-  double2* uf = uarena + F.uoff[f] * B * R + (int64_t)b * R * q;   // + r * q
+  double2* uf = uarena + F.uoff[f] * F.nvec + (int64_t)b * R * q;   // + r * q
   constexpr int VC = 256;                                     // input entries staged per chunk
   double2* v = reinterpret_cast<double2*>(smem_raw);        // [R][VC] input vector chunk
   double2* part = v + R * VC;                                 // [R][4][64] partial sums
@@ -1470,7 +1638,7 @@ __global__ void __launch_bounds__(256) k_panel(FrontsDev F, const int* __restric
                                                const typename cplx<T>::type* __restrict__ arena,
                                                double2* __restrict__ uarena, double2* __restrict__ x,
                                                const int* __restrict__ bsel) {
-  panel_body<T, KIND, R>(F, lvl, B, n, arena, uarena, x, bsel ? bsel[blockIdx.y] : (int)blockIdx.y);
+  panel_body<T, KIND, R>(F, lvl, B, n, arena, uarena, x, solve_block(F, bsel, blockIdx.y));
 
 }
 
@@ -1693,6 +1861,9 @@ struct Plan {
   int solve_full_m = 0;          // fronts up to this order are solved by one workgroup each
   int panel_lu_m = 0;            // 32-wide panels of buckets up to this order: k_panel_lu
   int tri_lds_bytes = 0;         // one-workgroup solve buckets whose factor block fits: k_tri_lds
+  int group_rc = 16;             // grouped solves: vectors per workgroup of k_grp_lds at most
+  int group_rc_tri = 4;          // grouped solves: vectors per workgroup of the other kernels
+  int grp_lds_bytes = 48 * 1024;   // one-workgroup buckets whose k_grp_lds LDS fits: k_grp_lds
   std::vector<void*> part_owned[2];
   std::vector<void*> panel_owned[2];   // the buckets' panel work lists (rebuilt per panel width)
   int panel_nb[2] = {0, 0};      // panel width the buckets' panel lists were built for (0: none)
@@ -2083,65 +2254,228 @@ int factor_impl(Plan* P, hipStream_t s, int B, const double* omega, double beta,
   return 0;
 }
 
-template <typename T, int R>
-int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, void* x, int transpose, int nb,
-               const int* bsel, int nsel, const int* fslot, int Bf) {
+// One solve call: B frequencies of R vectors each ([B][R][n]).  Factor slots: uniform groups of
+// G consecutive frequencies share slot b / G (G > 0; sel lists full groups, tail says whether
+// the last, partial group is solved), or frequency b reads slot fslot[b] of Bf (G == 0; sel
+// lists frequencies).  rc_max bounds the vectors one workgroup takes (G > 0).
+struct SolveCall {
+  int B, R, G;
+  const int* fslot;
+  int Bf;
+  const int* sel;
+  int nsel;
+  bool tail;
+  int rc_max;       // k_grp_lds
+  int rc_tri;       // the pivot-triangle, panel and k_tri_lds kernels
+};
+
+// One launch of a solve kernel over a set of vector blocks: block size rc (the kernels' R), the
+// fronts with the set's block -> factor slot and grid row -> block mapping, the selection and
+// the grid rows.
+struct SolveSet {
+  int rc;
+  FrontsDev F;
+  const int* sel;
+  int ny;
+};
+
+// The launch sets of a kernel whose LDS need for blocks of rc vectors is lds_of(rc): groups
+// of G frequencies go in blocks of rc = R 2^k vectors (2^k dividing G, rc <= rc_max, the
+// largest whose LDS fits), cpg = G R / rc blocks per group; the frequencies of a partial last
+// group in blocks of R.  Without groups: one set of blocks of R.  Returns the number of sets.
+template <typename LdsOf>
+int solve_sets(const Plan* P, const SolveCall& C, int rc_max, LdsOf&& lds_of, SolveSet (&out)[2]) {
+  FrontsDev F = P->F;
+  F.nvec = (int64_t)C.B * C.R;
+  if (C.G <= 0) {
+    F.fslot = C.fslot;
+    F.Bf = C.Bf;
+    out[0] = SolveSet{C.R, F, C.sel, C.nsel};
+    return C.nsel > 0 ? 1 : 0;
+  }
+  const int nfull = C.B / C.G, ntail = C.B - nfull * C.G;
+  F.Bf = nfull + (ntail > 0 ? 1 : 0);
+  int rc = C.R;
+  while (rc * 2 <= std::min(rc_max, 16) && (C.G * C.R) % (rc * 2) == 0 && lds_of(rc * 2) <= (size_t)P->lds_max)
+    rc *= 2;
+  int k = 0;
+  const int ng = C.sel ? C.nsel : nfull;
+  if (ng > 0) {
+    FrontsDev Fm = F;
+    Fm.cpg = C.G * C.R / rc;
+    Fm.vdiv = Fm.cpg;
+    out[k++] = SolveSet{rc, Fm, C.sel, ng * Fm.cpg};
+  }
+  if (ntail > 0 && C.tail) {
+    FrontsDev Ft = F;
+    Ft.vdiv = C.G;
+    Ft.boff = nfull * C.G;
+    out[k++] = SolveSet{C.R, Ft, nullptr, ntail};
+  }
+  return k;
+}
+
+// Calls fn(std::integral_constant<int, rc>) for the block sizes the solve kernels are built for.
+template <typename T, typename Fn>
+bool with_rc(int rc, Fn&& fn) {
+  switch (rc) {
+    case 1: fn(std::integral_constant<int, 1>{}); return true;
+    case 2: fn(std::integral_constant<int, 2>{}); return true;
+    default: break;
+  }
+  if constexpr (sizeof(T) == 4) {     // shared (grouped) factors exist in complex64 only
+    switch (rc) {
+      case 4: fn(std::integral_constant<int, 4>{}); return true;
+      case 8: fn(std::integral_constant<int, 8>{}); return true;
+      case 16: fn(std::integral_constant<int, 16>{}); return true;
+      default: break;
+    }
+  }
+  return false;
+}
+
+template <typename T>
+int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, void* uarena, void* x, int transpose,
+               int nb) {
   using V = typename cplx<T>::type;
-  FrontsDev FS = P->F;         // the plan's fronts with this call's factor-slot map
-  FS.fslot = fslot;
-  FS.Bf = Bf;
   constexpr int LD = SolveTile<T>::SB + 1;   // panel kernels' transposition tile
   const int pi = sizeof(T) == 4 ? 0 : 1;
   const V* A = (const V*)arena;
   double2* U = (double2*)uarena;
   double2* X = (double2*)x;
+  const int B = C.B;
+  bool bad = false;           // a block size without kernels (not reached: rc is R or R 2^k <= 16)
   // fronts of the fused set (m <= the small-front threshold): one workgroup does the whole
   // front; the others: pivot-triangle kernel + panel kernel
   // forward and backward passes take the one-workgroup set of the solve partition (the
   // transposed passes keep the factorization's partition)
+  // pivot-triangle kernel (with FULL the whole front) on one launch set
+  auto tri_launch = [&](const Level& L, const Sub& S, int kind, bool full, hipStream_t st, const SolveSet& Q) {
+    const int nbl = panel_width(nb, L);
+    const int ps = nbl <= 32 ? 32 : 64;     // partial sums: 4 x ps per vector
+    // 64-wide panels (levels of few large fronts): 512 threads share the row-panel updates
+    const dim3 blk(!full && nbl == 64 ? 512 : 256);
+    auto lds_of = [&](int rc) {
+      return tri_tile_bytes(sizeof(V), kind, rc, full, nbl, (int)blk.x) +
+             sizeof(double2) * rc * (4 * ps + (full ? S.mmax : S.pmax));
+    };
+    {
+      const size_t lds = lds_of(Q.rc);
+      const dim3 grid(S.nf, Q.ny);
+      bad |= !with_rc<T>(Q.rc, [&](auto c) {
+        constexpr int RC = decltype(c)::value;
+        // above 64 KB the kernel must be allowed the larger dynamic LDS (within the device
+        // limit: sawfem_mf_solve checks blocks of R, larger blocks are chosen to fit)
+#define SAW_TRI(K, FULL)                                                                                   \
+  do {                                                                                                    \
+    if (lds > 64 * 1024)                                                                                  \
+      (void)hipFuncSetAttribute((const void*)k_tri<T, K, RC, FULL>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)lds);                                                                \
+    hipLaunchKernelGGL((k_tri<T, K, RC, FULL>), grid, blk, lds, st, Q.F, S.fronts, B, P->n, nbl, A, U, X, Q.sel); \
+  } while (0)
+        if (full) {
+          switch (kind) { case 0: SAW_TRI(0, true); break; case 1: SAW_TRI(1, true); break;
+                          case 2: SAW_TRI(2, true); break; default: SAW_TRI(3, true); }
+        } else {
+          switch (kind) { case 0: SAW_TRI(0, false); break; case 1: SAW_TRI(1, false); break;
+                          case 2: SAW_TRI(2, false); break; default: SAW_TRI(3, false); }
+        }
+#undef SAW_TRI
+      });
+    }
+  };
   auto tri_sub = [&](const Level& L, const Sub& S, int kind, bool full, hipStream_t st) {
     if (S.nf == 0) return;
     Timed tm(P, st, (kind == 0 || kind == 2) ? K_FWD : K_BWD);
     const int nbl = panel_width(nb, L);
-    // kinds 0/1 keep only the nbl x nbl diagonal block in the tile; kinds 2/3 also stage 64
-    // columns of a row panel through it
-    const int tile_rows = kind <= 1 ? nbl : 64;
-    const int ps = nbl <= 32 ? 32 : 64;     // partial sums: 4 x ps per vector
-    const size_t lds = sizeof(V) * tile_rows * (nbl + 1) + sizeof(double2) * R * (4 * ps + (full ? S.mmax : S.pmax));
-    const dim3 grid(S.nf, nsel);
-    // 64-wide panels (levels of few large fronts): 512 threads share the row-panel updates
-    const dim3 blk(!full && nbl == 64 ? 512 : 256);
-    // above 64 KB the kernel must be allowed the larger dynamic LDS (checked against the device
-    // limit by sawfem_mf_solve)
-#define SAW_TRI(K, FULL)                                                                                  \
-  do {                                                                                                   \
-    if (lds > 64 * 1024)                                                                                 \
-      (void)hipFuncSetAttribute((const void*)k_tri<T, K, R, FULL>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                (int)lds);                                                               \
-    hipLaunchKernelGGL((k_tri<T, K, R, FULL>), grid, blk, lds, st, FS, S.fronts, B, P->n, nbl, A, U, X, bsel); \
+    const int ps = nbl <= 32 ? 32 : 64;
+    const int threads = !full && nbl == 64 ? 512 : 256;
+    auto lds_of = [&](int rc) {
+      return tri_tile_bytes(sizeof(V), kind, rc, full, nbl, threads) +
+             sizeof(double2) * rc * (4 * ps + (full ? S.mmax : S.pmax));
+    };
+    SolveSet sets[2];
+    const int ns = solve_sets(P, C, C.rc_tri, lds_of, sets);
+    for (int k = 0; k < ns; ++k) tri_launch(L, S, kind, full, st, sets[k]);
+  };
+  // grouped calls, one-workgroup buckets (kinds 0/1): k_grp_lds on blocks of 4 or more vectors
+  // whose LDS (the padded vectors and the factor block) fits the budget; smaller blocks (a
+  // partial last group) through the whole-front pivot-triangle kernel
+  auto grp_lds_of = [&](const Sub& S, int rc) {
+    return sizeof(double2) * (size_t)S.mmax * (rc + 1);
+  };
+  auto grp_ok = [&](const Sub& S) {
+    return sizeof(T) == 4 && C.G > 0 && !transpose && C.rc_max >= 4 && (C.G * C.R) % 4 == 0 &&
+           grp_lds_of(S, 4) <= (size_t)std::min(P->grp_lds_bytes, P->lds_max);
+  };
+  auto grp_sub = [&](const Level& L, const Sub& S, int kind, hipStream_t st) {
+    if (S.nf == 0) return;
+    Timed tm(P, st, kind == 0 ? K_FWD : K_BWD);
+    const int nbl = panel_width(nb, L);
+    const size_t budget = (size_t)std::min(P->grp_lds_bytes, P->lds_max);
+    SolveSet sets[2];
+    const int ns = solve_sets(P, C, C.rc_max, [&](int rc) { return grp_lds_of(S, rc) <= budget ? grp_lds_of(S, rc)
+                                                                                              : (size_t)-1; }, sets);
+    for (int k = 0; k < ns; ++k) {
+      const SolveSet& Q = sets[k];
+      if (Q.rc < 4) {
+        // blocks of one frequency (the partial last group): the pivot-triangle kernel's block size
+        const int threads = 256;
+        const int ps = nbl <= 32 ? 32 : 64;
+        const size_t lds_t = tri_tile_bytes(sizeof(V), kind, Q.rc, true, nbl, threads) +
+                             sizeof(double2) * Q.rc * (4 * ps + S.mmax);
+        if (lds_t > (size_t)P->lds_max) { bad = true; continue; }
+        tri_launch(L, S, kind, true, st, Q);
+        continue;
+      }
+      const size_t lds = grp_lds_of(S, Q.rc);
+      const dim3 grid(S.nf, Q.ny);
+      if constexpr (sizeof(T) == 4) {
+#define SAW_GRP(K, RCV)                                                                                     \
+  do {                                                                                                      \
+    if (lds > 64 * 1024)                                                                                    \
+      (void)hipFuncSetAttribute((const void*)k_grp_lds<K, RCV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)lds);                                                                  \
+    hipLaunchKernelGGL((k_grp_lds<K, RCV>), grid, dim3(256), lds, st, Q.F, S.fronts, B, P->n, nbl, A, U, X,   \
+                       Q.sel);                                                                              \
   } while (0)
-    if (full) {
-      switch (kind) { case 0: SAW_TRI(0, true); break; case 1: SAW_TRI(1, true); break;
-                      case 2: SAW_TRI(2, true); break; default: SAW_TRI(3, true); }
-    } else {
-      switch (kind) { case 0: SAW_TRI(0, false); break; case 1: SAW_TRI(1, false); break;
-                      case 2: SAW_TRI(2, false); break; default: SAW_TRI(3, false); }
+        if (kind == 0) {
+          if (Q.rc == 4) SAW_GRP(0, 4); else if (Q.rc == 8) SAW_GRP(0, 8); else SAW_GRP(0, 16);
+        } else {
+          if (Q.rc == 4) SAW_GRP(1, 4); else if (Q.rc == 8) SAW_GRP(1, 8); else SAW_GRP(1, 16);
+        }
+#undef SAW_GRP
+      }
     }
-#undef SAW_TRI
   };
   auto panel_sub = [&](const Sub& S, int kind, hipStream_t st) {
     const bool out_cb = (kind == 0 || kind == 2);
     const int nt = out_cb ? S.nstiles_cb : S.nstiles_p;
     const int* tl = out_cb ? S.stiles_cb : S.stiles_p;
     if (nt == 0) return;
-    const size_t lds = sizeof(double2) * R * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
+    auto lds_of = [&](int rc) {
+      return sizeof(double2) * rc * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
+    };
     Timed tm(P, st, out_cb ? K_FWD : K_BWD);
-    dim3 grid(nt, nsel);
-    switch (kind) {
-      case 0: hipLaunchKernelGGL((k_panel<T, 0, R>), grid, dim3(256), lds, st, FS, tl, B, P->n, A, U, X, bsel); break;
-      case 1: hipLaunchKernelGGL((k_panel<T, 1, R>), grid, dim3(256), lds, st, FS, tl, B, P->n, A, U, X, bsel); break;
-      case 2: hipLaunchKernelGGL((k_panel<T, 2, R>), grid, dim3(256), lds, st, FS, tl, B, P->n, A, U, X, bsel); break;
-      default: hipLaunchKernelGGL((k_panel<T, 3, R>), grid, dim3(256), lds, st, FS, tl, B, P->n, A, U, X, bsel);
+    SolveSet sets[2];
+    const int ns = solve_sets(P, C, C.rc_tri, lds_of, sets);
+    for (int k = 0; k < ns; ++k) {
+      const SolveSet& Q = sets[k];
+      const size_t lds = lds_of(Q.rc);
+      const dim3 grid(nt, Q.ny);
+      bad |= !with_rc<T>(Q.rc, [&](auto c) {
+        constexpr int RC = decltype(c)::value;
+#define SAW_PANEL(K)                                                                                      \
+  do {                                                                                                    \
+    if (lds > 64 * 1024)                                                                                  \
+      (void)hipFuncSetAttribute((const void*)k_panel<T, K, RC>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)lds);                                                                \
+    hipLaunchKernelGGL((k_panel<T, K, RC>), grid, dim3(256), lds, st, Q.F, tl, B, P->n, A, U, X, Q.sel); \
+  } while (0)
+        switch (kind) { case 0: SAW_PANEL(0); break; case 1: SAW_PANEL(1); break;
+                        case 2: SAW_PANEL(2); break; default: SAW_PANEL(3); }
+#undef SAW_PANEL
+      });
     }
   };
   // The one-workgroup set and the size buckets of a level are independent: with more than one,
@@ -2149,32 +2483,42 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
   auto stream_of = [&](bool fork, size_t unit) { return (!fork || unit == 0) ? s : P->side[(unit - 1) % Plan::NSIDE]; };
   const int kf = transpose ? 2 : 0, kb = transpose ? 3 : 1;
   // one-workgroup buckets whose factor block fits the LDS budget: whole-front solves from LDS
-  auto lds_bytes = [&](const Sub& S) {
-    return sizeof(V) * (size_t)S.mpmax + sizeof(double2) * R * (256 + 4 * 64);
+  auto lds_bytes = [&](const Sub& S, int rc) {
+    return sizeof(V) * (size_t)S.mpmax + sizeof(double2) * rc * (256 + 4 * 64);
   };
   auto lds_ok = [&](const Sub& S) {
     return !transpose && P->tri_lds_bytes > 0 && S.mmax <= 256 && S.mpmax <= 256 * tri_lds_npf<T>() &&
-           sizeof(V) * (size_t)S.mpmax <= (size_t)P->tri_lds_bytes && lds_bytes(S) <= (size_t)P->lds_max;
+           sizeof(V) * (size_t)S.mpmax <= (size_t)P->tri_lds_bytes && lds_bytes(S, C.R) <= (size_t)P->lds_max;
   };
   auto tri_lds = [&](const Level& L, const Sub& S, int kind, hipStream_t st) {
     if (S.nf == 0) return;
     Timed tm(P, st, kind == 0 ? K_FWD : K_BWD);
-    const size_t lds = lds_bytes(S);
     const int nbl = panel_width(nb, L);
-    // persistent grid: as many workgroups as the CUs hold at this LDS size (at most 8 a CU)
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (size_t)P->lds_per_cu / lds));
-    const int64_t nitems = (int64_t)S.nf * nsel;
-    const unsigned grid = (unsigned)std::min<int64_t>(nitems, (int64_t)P->ncu * per_cu);
-    if (kind == 0) {
-      if (lds > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 0, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL((k_tri_lds<T, 0, R>), dim3(grid), dim3(256), lds, st, FS, S.fronts, S.nf, B, P->n, nbl, A,
-                         U, X, bsel, nsel);
-    } else {
-      if (lds > 64 * 1024)
-        (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 1, R>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      hipLaunchKernelGGL((k_tri_lds<T, 1, R>), dim3(grid), dim3(256), lds, st, FS, S.fronts, S.nf, B, P->n, nbl, A,
-                         U, X, bsel, nsel);
+    SolveSet sets[2];
+    const int ns = solve_sets(P, C, C.rc_tri, [&](int rc) { return lds_bytes(S, rc); }, sets);
+    for (int k = 0; k < ns; ++k) {
+      const SolveSet& Q = sets[k];
+      const size_t lds = lds_bytes(S, Q.rc);
+      // persistent grid: as many workgroups as the CUs hold at this LDS size (at most 8 a CU)
+      const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (size_t)P->lds_per_cu / lds));
+      const int64_t nitems = (int64_t)S.nf * Q.ny;
+      const unsigned grid = (unsigned)std::min<int64_t>(nitems, (int64_t)P->ncu * per_cu);
+      bad |= !with_rc<T>(Q.rc, [&](auto c) {
+        constexpr int RC = decltype(c)::value;
+        if (kind == 0) {
+          if (lds > 64 * 1024)
+            (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 0, RC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds);
+          hipLaunchKernelGGL((k_tri_lds<T, 0, RC>), dim3(grid), dim3(256), lds, st, Q.F, S.fronts, S.nf, B, P->n,
+                             nbl, A, U, X, Q.sel, Q.ny);
+        } else {
+          if (lds > 64 * 1024)
+            (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 1, RC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)lds);
+          hipLaunchKernelGGL((k_tri_lds<T, 1, RC>), dim3(grid), dim3(256), lds, st, Q.F, S.fronts, S.nf, B, P->n,
+                             nbl, A, U, X, Q.sel, Q.ny);
+        }
+      });
     }
   };
   for (size_t li = 0; li < P->levels.size(); ++li) {
@@ -2185,7 +2529,8 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
     for (size_t gi = 0; gi < nfull; ++gi) {
       const Sub& S = L.ssmall[pi][gi];
-      if (lds_ok(S)) tri_lds(L, S, 0, stream_of(fork, gi));
+      if (grp_ok(S)) grp_sub(L, S, 0, stream_of(fork, gi));
+      else if (lds_ok(S)) tri_lds(L, S, 0, stream_of(fork, gi));
       else tri_sub(L, S, kf, true, stream_of(fork, gi));
     }
     for (size_t gi = 0; gi < L.sbig[pi].size(); ++gi) {
@@ -2213,7 +2558,8 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
       for (size_t gi = 0; gi < nfull; ++gi) {
         const Sub& S = L.ssmall[pi][gi];
         const hipStream_t st = stream_of(fork, gi + L.sbig[pi].size());
-        if (lds_ok(S)) tri_lds(L, S, 1, st);
+        if (grp_ok(S)) grp_sub(L, S, 1, st);
+        else if (lds_ok(S)) tri_lds(L, S, 1, st);
         else tri_sub(L, S, kb, true, st);
       }
       if (fork && join_side(P, s)) return 3;
@@ -2230,7 +2576,20 @@ int solve_impl(Plan* P, hipStream_t s, int B, const void* arena, void* uarena, v
     if (full) tri_sub(L, L.small[pi], kb, true, s);
     if (fork && join_side(P, s)) return 3;
   }
+  if (bad) { sawfem_set_error("sawfem_mf_solve: no kernels for the block size"); return 1; }
   SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+// LDS of the pivot-block kernel for blocks of nrhs vectors: the widest tile (64 x 65 complex64
+// or 64 x 33 complex128, both <= 64 * 33 * 16 bytes), partial sums and the pivot entries of the
+// vectors.  Grouped solves take larger blocks only where they fit.
+int solve_lds_check(const Plan* P, int nrhs) {
+  const size_t tile = 64 * 33 * 16;
+  if ((size_t)nrhs * (P->pmax + 256) * sizeof(double2) + tile > (size_t)P->lds_max) {
+    sawfem_set_error("sawfem_mf_solve: front too large for the LDS solve");
+    return 1;
+  }
   return 0;
 }
 
@@ -2508,20 +2867,44 @@ int sawfem_mf_solve_shared(void* plan, void* stream, int prec, int B, int nrhs, 
     sawfem_set_error("sawfem_mf_solve: invalid arguments");
     return 1;
   }
-  // LDS of the pivot-block kernel: the widest tile (64 x 65 complex64 or 64 x 33 complex128,
-  // both <= 64 * 33 * 16 bytes), partial sums and the pivot entries of the nrhs vectors
-  const size_t tile = 64 * 33 * 16;
-  if ((size_t)nrhs * (P->pmax + 256) * sizeof(double2) + tile > (size_t)P->lds_max) {
-    sawfem_set_error("sawfem_mf_solve: front too large for the LDS solve");
+  if (solve_lds_check(P, nrhs)) return 1;
+  if (nsel == 0) return 0;
+  const SolveCall C{B, nrhs, 0, fslot, Bf, bsel, nsel, false, nrhs, nrhs};
+  hipStream_t s = (hipStream_t)stream;
+  return prec == 0 ? solve_impl<float>(P, s, C, arena, uarena, x, transpose, nb)
+                   : solve_impl<double>(P, s, C, arena, uarena, x, transpose, nb);
+}
+
+int sawfem_mf_solve_groups(void* plan, void* stream, int B, int nrhs, const void* arena, int G, void* uarena,
+                           void* x, int transpose, int nb, const int* gsel, int ngsel, int tail) {
+  Plan* P = (Plan*)plan;
+  const int nfull = G > 0 ? B / G : 0;
+  if (!gsel) ngsel = nfull;
+  if (!P || B <= 0 || G <= 0 || !arena || !uarena || !x || (nb != 32 && nb != 64) || (nrhs != 1 && nrhs != 2) ||
+      ngsel < 0 || ngsel > nfull || (tail != 0 && tail != 1)) {
+    sawfem_set_error("sawfem_mf_solve_groups: invalid arguments");
     return 1;
   }
-  hipStream_t s = (hipStream_t)stream;
-  if (nsel == 0) return 0;
-  if (nrhs == 1)
-    return prec == 0 ? solve_impl<float, 1>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel, fslot, Bf)
-                     : solve_impl<double, 1>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel, fslot, Bf);
-  return prec == 0 ? solve_impl<float, 2>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel, fslot, Bf)
-                   : solve_impl<double, 2>(P, s, B, arena, uarena, x, transpose, nb, bsel, nsel, fslot, Bf);
+  if (solve_lds_check(P, nrhs)) return 1;
+  const bool do_tail = tail && B % G != 0;
+  if (ngsel == 0 && !do_tail) return 0;
+  const SolveCall C{B, nrhs, G, nullptr, 0, gsel, ngsel, do_tail, P->group_rc, P->group_rc_tri};
+  return solve_impl<float>(P, (hipStream_t)stream, C, arena, uarena, x, transpose, nb);
+}
+
+int sawfem_mf_set_group_rc(void* plan, int rc_max) {
+  Plan* P = (Plan*)plan;
+  if (!P || rc_max < 1) { sawfem_set_error("sawfem_mf_set_group_rc: invalid arguments"); return -1; }
+  P->group_rc = std::min(rc_max, 16);
+  return P->group_rc;
+}
+
+int sawfem_mf_set_group_tri(void* plan, int rc_max, int lds_bytes) {
+  Plan* P = (Plan*)plan;
+  if (!P || rc_max < 1 || lds_bytes < 0) { sawfem_set_error("sawfem_mf_set_group_tri: invalid arguments"); return -1; }
+  P->group_rc_tri = std::min(rc_max, 16);
+  P->grp_lds_bytes = std::min(lds_bytes, P->lds_max);
+  return P->group_rc_tri;
 }
 
 // Per-kernel event timing: enable (1) / disable (0); turning it on clears what was recorded.

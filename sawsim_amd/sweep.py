@@ -207,7 +207,7 @@ class FrequencySolver:
         self.mf = self.symbolic.mf
         self.analysis_seconds = time.time() - t0
         self._plan = self._create_plan()
-        self.share, self._fslot, self._Bf = SHARE, None, 0
+        self.share, self._group = SHARE, 0
         self.set_solve_full(SOLVE_FULL_M)
         self.set_panel_lu(PANEL_LU_M)
         self.set_tri_lds(TRI_LDS_BYTES)
@@ -222,6 +222,11 @@ class FrequencySolver:
         """Solve one-workgroup buckets whose factor block fits ``max_bytes`` from LDS
         (``sawfem_mf_set_tri_lds``); 0 keeps the pivot-triangle kernels."""
         return _lib.call_value('sawfem_mf_set_tri_lds', self._plan, int(max_bytes))
+
+    def set_group_rc(self, rc_max):
+        """At most ``rc_max`` vectors per workgroup in the solves of shared factors
+        (``sawfem_mf_set_group_rc``; 1..16)."""
+        return _lib.call_value('sawfem_mf_set_group_rc', self._plan, int(rc_max))
 
     def set_solve_full(self, max_m):
         """Solve fronts of order <= ``max_m`` with one workgroup each (``sawfem_mf_set_solve_full``)."""
@@ -509,10 +514,9 @@ class FrequencySolver:
         Bf = int(group[-1]) + 1
         if share > 1:
             om_f = torch.as_tensor(np.bincount(group, weights=omegas) / np.bincount(group), device=dev)
-            self._fslot = torch.as_tensor(group.astype(np.int32), device=dev)
         else:
-            om_f, self._fslot = om, None
-        self._Bf = Bf
+            om_f = om
+        self._group = share if share > 1 else 0
         arena = self._arena(self.mf.arena_elems * Bf, torch.complex64 if prec == 0 else torch.complex128)
         uarena = torch.empty(max(1, self.mf.uarena_elems * B * R), dtype=torch.complex128, device=dev)
         _lib.call('sawfem_mf_factor', self._plan, s, prec, Bf, om_f.data_ptr(), self.beta, self.Kaa.data_ptr(),
@@ -526,7 +530,7 @@ class FrequencySolver:
             out = self._solve_single(om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir)
             if gradient:
                 out.update(self._adjoint(om, omegas, V, prec, nb, arena, uarena, rhs, out['x'], ir_tol, max_ir))
-        self._fslot = None
+        self._group = 0
         x, err = out['x'], out['err']
         out['perturbed'], out['pivot_min_ratio'] = self.pivot_stats(reset=True)
         out['share'], out['factorizations'] = share, Bf
@@ -572,13 +576,33 @@ class FrequencySolver:
         self.share = k
         return k
 
-    def _solve(self, prec, B, R, arena, uarena, vec, transpose, nb, sel=None, nsel=0):
-        """Forward + backward solve of B right-hand sides with the chunk's factors (shared when
-        ``solve_batch`` factored groups of frequencies)."""
-        fs = self._fslot
-        _lib.call('sawfem_mf_solve_shared', self._plan, stream_ptr(), prec, B, R, arena.data_ptr(),
-                  self._Bf if fs is not None else B, None if fs is None else fs.data_ptr(), uarena.data_ptr(),
-                  vec.data_ptr(), transpose, nb, sel, nsel)
+    def _solve(self, prec, B, R, arena, uarena, vec, transpose, nb, active=None):
+        """Forward + backward solve of the B x R right-hand sides ``vec`` in place with the chunk's
+        factors; ``active`` (bool [B], or None for all) selects the frequencies to solve.
+
+        When ``solve_batch`` factored groups of G frequencies (complex64), the vectors of a group
+        are solved together (``sawfem_mf_solve_groups``): a group is solved when any of its
+        frequencies is active, and the others of the group must hold zero right-hand sides
+        (their solution is then zero)."""
+        G = self._group
+        idx = None
+        if G > 1:
+            nfull = B // G
+            tail = 1
+            if active is not None:
+                ga = np.zeros(nfull + 1, dtype=bool)
+                np.logical_or.at(ga, np.arange(B) // G, active)
+                tail = int(ga[nfull]) if B % G else 0
+                idx = self.torch.as_tensor(np.flatnonzero(ga[:nfull]).astype(np.int32), device=self.dev)
+            _lib.call('sawfem_mf_solve_groups', self._plan, stream_ptr(), B, R, arena.data_ptr(), G,
+                      uarena.data_ptr(), vec.data_ptr(), transpose, nb, None if idx is None else idx.data_ptr(),
+                      0 if idx is None else int(idx.numel()), tail)
+            return
+        if active is not None:
+            idx = self.torch.as_tensor(np.flatnonzero(active).astype(np.int32), device=self.dev)
+        _lib.call('sawfem_mf_solve', self._plan, stream_ptr(), prec, B, R, arena.data_ptr(), uarena.data_ptr(),
+                  vec.data_ptr(), transpose, nb, None if idx is None else idx.data_ptr(),
+                  0 if idx is None else int(idx.numel()))
 
     def pivot_stats(self, reset=True):
         """(pivots the guard replaced, smallest |pivot| / row scale) since the last reset
@@ -636,12 +660,11 @@ class FrequencySolver:
                 if not active.any():
                     break
             self._solved.append(int(active.sum()))
-            sel, nsel = None, 0
+            sel = None
             if not active.all():
                 r[torch.as_tensor(~active, device=self.dev)] = 0
-                idx = torch.as_tensor(np.flatnonzero(active).astype(np.int32), device=self.dev)
-                sel, nsel = idx.data_ptr(), int(idx.numel())
-            self._solve(prec, B, R, arena, uarena, r, transpose, nb, sel, nsel)
+                sel = active
+            self._solve(prec, B, R, arena, uarena, r, transpose, nb, sel)
             _lib.call('sawfem_axpy', s, X.numel(), X.data_ptr(), r.data_ptr())
             applied = steps
             new = measure()

@@ -45,6 +45,74 @@ def test_identity_slot_map_matches_the_plain_solve():
     assert rc == _lib.INVALID
 
 
+@pytest.mark.parametrize('G', [2, 3, 4, 8])
+def test_grouped_solve_matches_the_slot_map_solve(G):
+    """sawfem_mf_solve_groups (the vectors of a group of frequencies in one workgroup) against
+    sawfem_mf_solve_shared with the slot map b -> b // G, for one and two vectors per frequency,
+    both transposes, a partial last group and a selection of groups (unselected vectors
+    untouched): bit for bit through the pivot-triangle and panel kernels at every block size
+    (their per-vector operations do not depend on the block size); with the LDS block kernel
+    on the one-workgroup fronts (blocks of 4, 8, 16 vectors; other summation order) within
+    1e-10 of the largest entry of each vector."""
+    _say(f'[test_grouped_solve_matches_the_slot_map_solve] G {G}')
+    cell = workloads.config3(mesh_um=0.12, points=6)
+    eng = workloads.build_engine(cell)
+    torch = eng.torch
+    B = 11
+    f = np.linspace(cell.frequencies()[0], cell.frequencies()[-1], B)
+    Bf = -(-B // G)
+    grp = np.arange(B) // G
+    om_f = torch.as_tensor(2 * np.pi * np.bincount(grp, weights=f) / np.bincount(grp), device=eng.dev)
+    n = eng.n_active
+    arena = eng._arena(eng.mf.arena_elems * Bf, torch.complex64)
+    uarena = torch.empty(eng.mf.uarena_elems * B * 2, dtype=torch.complex128, device=eng.dev)
+    s = stream_ptr()
+    _lib.call('sawfem_mf_factor', eng._plan, s, 0, Bf, om_f.data_ptr(), eng.beta, eng.Kaa.data_ptr(),
+              eng.Maa.data_ptr(), eng._dptr(eng.Daa), arena.data_ptr(), 64)
+    slot = torch.as_tensor(grp.astype(np.int32), device=eng.dev)
+    gen = torch.Generator(device=eng.dev).manual_seed(G)
+    nfull = B // G
+    for R in (1, 2):
+        x0 = torch.randn((B, R, n), dtype=torch.complex128, device=eng.dev, generator=gen)
+        for tr in (0, 1):
+            ref = x0.clone()
+            _lib.call('sawfem_mf_solve_shared', eng._plan, s, 0, B, R, arena.data_ptr(), Bf, slot.data_ptr(),
+                      uarena.data_ptr(), ref.data_ptr(), tr, 64, None, 0)
+            for rc in (1, 2, 4, 16):
+                assert _lib.call_value('sawfem_mf_set_group_tri', eng._plan, rc, 0) == rc
+                got = x0.clone()
+                _lib.call('sawfem_mf_solve_groups', eng._plan, s, B, R, arena.data_ptr(), G, uarena.data_ptr(),
+                          got.data_ptr(), tr, 64, None, 0, 1)
+                assert torch.equal(got, ref), (R, tr, rc)
+            _lib.call_value('sawfem_mf_set_group_tri', eng._plan, 4, 160 * 1024)
+            for rc in (4, 8, 16):
+                assert _lib.call_value('sawfem_mf_set_group_rc', eng._plan, rc) == rc
+                got = x0.clone()
+                _lib.call('sawfem_mf_solve_groups', eng._plan, s, B, R, arena.data_ptr(), G, uarena.data_ptr(),
+                          got.data_ptr(), tr, 64, None, 0, 1)
+                err = ((got - ref).abs().amax(dim=2) / ref.abs().amax(dim=2)).max().item()
+                assert err <= 1e-10, (R, tr, rc, err)
+            _lib.call_value('sawfem_mf_set_group_rc', eng._plan, 16)
+            # groups 1 .. nfull-1 selected, the partial group not: group 0 and the tail untouched
+            _lib.call_value('sawfem_mf_set_group_tri', eng._plan, 4, 0)
+            sel = torch.arange(1, nfull, dtype=torch.int32, device=eng.dev)
+            got = x0.clone()
+            _lib.call('sawfem_mf_solve_groups', eng._plan, s, B, R, arena.data_ptr(), G, uarena.data_ptr(),
+                      got.data_ptr(), tr, 64, sel.data_ptr() if sel.numel() else None, int(sel.numel()), 0)
+            if sel.numel() == 0:      # a NULL list means every full group
+                assert torch.equal(got[:nfull * G], ref[:nfull * G])
+            else:
+                assert torch.equal(got[:G], x0[:G])
+                assert torch.equal(got[G:nfull * G], ref[G:nfull * G])
+            assert torch.equal(got[nfull * G:], x0[nfull * G:])
+    _lib.call_value('sawfem_mf_set_group_tri', eng._plan, 4, 48 * 1024)
+    # invalid group sizes and selections are refused before any launch
+    assert _lib.raw('sawfem_mf_solve_groups', eng._plan, s, B, 1, arena.data_ptr(), 0, uarena.data_ptr(),
+                    x0.data_ptr(), 0, 64, None, 0, 1) == _lib.INVALID
+    assert _lib.raw('sawfem_mf_solve_groups', eng._plan, s, B, 1, arena.data_ptr(), G, uarena.data_ptr(),
+                    x0.data_ptr(), 0, 64, slot.data_ptr(), nfull + 1, 1) == _lib.INVALID
+
+
 @pytest.mark.parametrize('share', [2, 4])
 def test_shared_factors_meet_the_fp64_accuracy(share):
     """Config 1 as the benchmark names it (51,752 unknowns), 24 consecutive frequencies of its
