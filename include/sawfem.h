@@ -296,6 +296,16 @@ int sawfem_residual_t(void* stream, int n, int B, int Bp, const int64_t* indptr,
                       const int64_t* tslot, const void* Kaa, const void* Maa, const void* Daa, const double* omega,
                       double beta, const void* xt, const void* rhst, void* rt);
 
+/* sawfem_residual_t over ng row groups (gptr int32[ng + 1], grows int32: the rows of group g
+ * are grows[gptr[g] .. gptr[g + 1]), at most 4, with identical column lists, e.g. the active
+ * unknowns of one merged mesh node; every row in exactly one group).  One wave per group and 64
+ * frequencies reads each x row once for the group's rows; per row the result equals
+ * sawfem_residual_t's bit for bit.  Replaces the same residual as sawfem_residual (the FP64
+ * A(w) x of SawSim saw2d/sweep.py:335-364's solve, here the refinement residual). */
+int sawfem_residual_sv(void* stream, int ng, int B, int Bp, const int* gptr, const int* grows, const int64_t* indptr,
+                       const int* indices, const int64_t* tslot, const void* Kaa, const void* Maa, const void* Daa,
+                       const double* omega, double beta, const void* xt, const void* rhst, void* rt);
+
 /* out (C rows, leading dimension ld_out) = transpose of in (R rows, leading dimension ld_in). */
 int sawfem_transpose(void* stream, int R, int C, int ld_in, int ld_out, const void* in, void* out);
 

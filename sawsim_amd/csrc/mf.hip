@@ -1405,10 +1405,11 @@ __global__ void __launch_bounds__(256) k_tri_lds(FrontsDev F, const int* __restr
 // Whole-front solves of one block of RC vectors (RC = 4, 8 or 16: the vectors of a group of
 // frequencies that share one complex64 factorization), the vectors staged in LDS.  Every step
 // is a small product whose (row, vector) outputs are spread over the 256 threads: TPR = RC / 4
-// threads per row, four vectors each, so one factor entry serves four vectors of a thread and
-// the TPR threads of a row read it together (one cache line per wave and step: the rows of a
-// column are contiguous).  A thread's factor loads are issued eight at a time; only the vectors
-// use LDS (m (RC + 1) entries), so several workgroups share a CU and hide each other's loads.
+// threads per row, four vectors each, so one factor entry serves four vectors of a thread.  The
+// TPR threads of a row split its factor entries (every TPR-th, eight loads of a thread in
+// flight, coalesced over the rows of a column) and pass them to each other by DPP, so a row of
+// a 32-wide block arrives in one round trip; only the vectors use LDS (m (RC + 1) entries), so
+// several workgroups share a CU and hide each other's loads.
 //  kind 0 (forward):  w = the block's vectors at the pivot rows plus the children's update
 //                     vectors, over all m rows; per diagonal block w_blk = L_D^{-1} w_blk, then
 //                     w[below] -= L[below][blk] w_blk; pivot rows to x, the other rows to the
@@ -1420,6 +1421,59 @@ __global__ void __launch_bounds__(256) k_tri_lds(FrontsDev F, const int* __restr
 // w rows are padded to RC + 1 entries (the gather's row-strided LDS writes then spread over the
 // banks).
 constexpr int GRP_NB = 8;      // factor loads of a thread in flight together
+
+// Value v of lane S of this lane's TPR-lane row group (TPR consecutive lanes: 4 = a quad, 2 =
+// a pair within the quad, 1 = itself), by a DPP quad permutation (no LDS).  All lanes of a
+// group must be active.
+template <int TPR, int S>
+__device__ __forceinline__ float grp_bcast(float v) {
+  if constexpr (TPR == 1) {
+    return v;
+  } else {
+    constexpr int ctrl = TPR == 4 ? S * 0x55 : (S | (S << 2) | ((S + 2) << 4) | ((S + 2) << 6));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), ctrl, 0xF, 0xF, false));
+  }
+}
+
+// s[v] op= sum_{k in [k0, k1)} col[k m] w[(wrow0 + k) WS + r0 + v] for the TPR threads of one row:
+// each loads every TPR-th entry (GRP_NB loads in flight), the entries reach the others of the
+// group by grp_bcast, so a batch covers TPR * GRP_NB entries of the row in one round trip.
+// SUB: subtract (update) or add (diagonal-block product).  k0, k1 must be equal for the TPR
+// threads of a row.
+template <int TPR, int VPT, int WS, bool SUB>
+__device__ __forceinline__ void grp_row_product(const float2* __restrict__ col, int64_t m, int k0, int k1, int q,
+                                                const double2* __restrict__ w, int wrow0, int r0, double2 (&s)[VPT]) {
+  for (int kb = k0; kb < k1; kb += TPR * GRP_NB) {
+    float ex[GRP_NB], ey[GRP_NB];
+#pragma unroll
+    for (int u = 0; u < GRP_NB; ++u) {
+      const int kc = min(kb + u * TPR + q, k1 - 1);
+      const float2 e = col[(int64_t)kc * m];
+      ex[u] = e.x;
+      ey[u] = e.y;
+    }
+#pragma unroll
+    for (int u = 0; u < GRP_NB; ++u) {
+      auto step = [&](auto sc) {
+        constexpr int S = decltype(sc)::value;
+        const int k = kb + u * TPR + S;
+        const double2 a = make_double2((double)grp_bcast<TPR, S>(ex[u]), (double)grp_bcast<TPR, S>(ey[u]));
+        if (k < k1) {
+#pragma unroll
+          for (int v = 0; v < VPT; ++v)
+            s[v] = SUB ? cfms(s[v], a, w[(wrow0 + k) * WS + r0 + v]) : cfma(s[v], a, w[(wrow0 + k) * WS + r0 + v]);
+        }
+      };
+      step(std::integral_constant<int, 0>{});
+      if constexpr (TPR > 1) step(std::integral_constant<int, 1>{});
+      if constexpr (TPR > 2) {
+        step(std::integral_constant<int, 2>{});
+        step(std::integral_constant<int, 3>{});
+      }
+    }
+  }
+}
+
 template <int KIND, int RC>
 __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
                                                  const float2* __restrict__ arena, double2* __restrict__ uarena,
@@ -1470,14 +1524,10 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
   __syncthreads();
   const int tr = t / TPR, r0 = (t % TPR) * VPT;     // row slot, first vector of the thread
   double2 s[VPT];
+  const int q4 = t % TPR;                           // thread within the row's group
   // s -= sum_{k in [k0, k1)} F[i][c0 + k] w[wrow0 + k] (entries of row i of the column-major front)
   auto row_update = [&](int i, int c0, int k0, int k1, int wrow0) {
-    const float2* col = Fb + i + (int64_t)c0 * m;
-    batched<GRP_NB>(k0, k1, 1, [&](int k) { return col[(int64_t)k * m]; }, [&](int k, float2 e) {
-      const double2 a = to_d2(e);
-#pragma unroll
-      for (int v = 0; v < VPT; ++v) s[v] = cfms(s[v], a, w[(wrow0 + k) * WS + r0 + v]);
-    });
+    grp_row_product<TPR, VPT, WS, true>(Fb + i + (int64_t)c0 * m, m, k0, k1, q4, w, wrow0, r0, s);
   };
   if (KIND == 1 && q > 0) {
     // contribution-block rows -> pivot rows: w[i] -= sum_j U[i][p + j] w[p + j]
@@ -1501,21 +1551,11 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
       if (KIND == 0) {          // L_D^{-1} (unit lower): w_i + sum_{k<i} Li[i][k] w_k
 #pragma unroll
         for (int v = 0; v < VPT; ++v) s[v] = w[(J0 + i) * WS + r0 + v];
-        const float2* row = Fb + (J0 + i) + (int64_t)J0 * m;
-        batched<GRP_NB>(0, i, 1, [&](int k) { return row[(int64_t)k * m]; }, [&](int k, float2 e) {
-          const double2 a = to_d2(e);
-#pragma unroll
-          for (int v = 0; v < VPT; ++v) s[v] = cfma(s[v], a, w[(J0 + k) * WS + r0 + v]);
-        });
+        grp_row_product<TPR, VPT, WS, false>(Fb + (J0 + i) + (int64_t)J0 * m, m, 0, i, q4, w, J0, r0, s);
       } else {                  // U_D^{-1}: sum_{k>=i} Ui[i][k] w_k
 #pragma unroll
         for (int v = 0; v < VPT; ++v) s[v] = make_double2(0, 0);
-        const float2* row = Fb + (J0 + i) + (int64_t)J0 * m;
-        batched<GRP_NB>(i, nbk, 1, [&](int k) { return row[(int64_t)k * m]; }, [&](int k, float2 e) {
-          const double2 a = to_d2(e);
-#pragma unroll
-          for (int v = 0; v < VPT; ++v) s[v] = cfma(s[v], a, w[(J0 + k) * WS + r0 + v]);
-        });
+        grp_row_product<TPR, VPT, WS, false>(Fb + (J0 + i) + (int64_t)J0 * m, m, i, nbk, q4, w, J0, r0, s);
       }
     }
     __syncthreads();
@@ -1719,6 +1759,58 @@ __global__ void __launch_bounds__(256) k_residual_t(int n, int B, int Bp, const 
   }
   const double2 r = rhst[i * Bp + b];
   rt[i * Bp + b] = make_double2(r.x - acc.x, r.y - acc.y);
+}
+
+// The same residual over row groups: the rows of a group (at most RES_G, listed in
+// grows[gptr[g] .. gptr[g + 1])) have identical column lists, as the active unknowns of one
+// merged mesh node do, so one wave per group and 64 frequencies reads each x row once for all
+// the group's rows.  Per row the products are summed in the order of k_residual_t.
+constexpr int RES_G = 4;
+__global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, const int* __restrict__ gptr,
+                                                     const int* __restrict__ grows, const int64_t* __restrict__ indptr,
+                                                     const int* __restrict__ indices, const int64_t* __restrict__ tslot,
+                                                     const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
+                                                     const double2* __restrict__ Daa,
+                                                     const double* __restrict__ omega, double beta,
+                                                     const double2* __restrict__ xt, const double2* __restrict__ rhst,
+                                                     double2* __restrict__ rt) {
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= ng) return;
+  const int b = blockIdx.y * 64 + lane;
+  const double w = b < B ? omega[b] : 0.0;
+  const double w2 = w * w, wb = w * beta;
+  const int g0 = gptr[g], cnt = gptr[g + 1] - g0;
+  int64_t base[RES_G];
+#pragma unroll
+  for (int t = 0; t < RES_G; ++t) base[t] = indptr[grows[g0 + min(t, cnt - 1)]];
+  const int64_t len = indptr[grows[g0] + 1] - base[0];
+  const int* cols = indices + base[0];
+  double2 acc[RES_G];
+#pragma unroll
+  for (int t = 0; t < RES_G; ++t) acc[t] = make_double2(0, 0);
+#pragma unroll 2
+  for (int64_t k = 0; k < len; ++k) {
+    const double2 xv = xt[(int64_t)cols[k] * Bp + b];
+#pragma unroll
+    for (int t = 0; t < RES_G; ++t) {
+      if (t < cnt) {
+        const int64_t s = tslot ? tslot[base[t] + k] : base[t] + k;
+        const double2 kk = Kaa[s], mm = Maa[s];
+        const double2 d = Daa ? Daa[s] : make_double2(0, 0);
+        const double2 a = make_double2(kk.x - w2 * mm.x - wb * d.y, kk.y - w2 * mm.y + wb * d.x);
+        acc[t] = cfma(acc[t], a, xv);
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < RES_G; ++t) {
+    if (t < cnt) {
+      const int64_t i = grows[g0 + t];
+      const double2 r = rhst[i * Bp + b];
+      rt[i * Bp + b] = make_double2(r.x - acc[t].x, r.y - acc[t].y);
+    }
+  }
 }
 
 // out[j][i] = in[i][j] for an R x C complex128 matrix (tiled through LDS).
@@ -2978,6 +3070,20 @@ int sawfem_residual_t(void* stream, int n, int B, int Bp, const int64_t* indptr,
   hipLaunchKernelGGL(k_residual_t, dim3((n + 3) / 4, Bp / 64), dim3(256), 0, (hipStream_t)stream, n, B, Bp, indptr,
                      indices, tslot, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, omega, beta,
                      (const double2*)xt, (const double2*)rhst, (double2*)rt);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_residual_sv(void* stream, int ng, int B, int Bp, const int* gptr, const int* grows, const int64_t* indptr,
+                       const int* indices, const int64_t* tslot, const void* Kaa, const void* Maa, const void* Daa,
+                       const double* omega, double beta, const void* xt, const void* rhst, void* rt) {
+  if (ng <= 0 || B <= 0 || Bp < B || Bp % 64 || !gptr || !grows) {
+    sawfem_set_error("sawfem_residual_sv: invalid arguments");
+    return 1;
+  }
+  hipLaunchKernelGGL(k_residual_sv, dim3((ng + 3) / 4, Bp / 64), dim3(256), 0, (hipStream_t)stream, ng, B, Bp, gptr,
+                     grows, indptr, indices, tslot, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa,
+                     omega, beta, (const double2*)xt, (const double2*)rhst, (double2*)rt);
   SAW_HIP(hipGetLastError());
   return 0;
 }

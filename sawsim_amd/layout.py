@@ -255,3 +255,47 @@ class BlochLayout:
         disp[self.zheng] = V
         disp[self.fu] = 0.0
         return disp
+
+
+def residual_groups(indptr, indices, sv_of, gmax=4):
+    """Row groups for the grouped residual (``sawfem_residual_sv``): the rows of one
+    supervariable (``sv_of``, e.g. the active unknowns of one merged mesh node) in groups of at
+    most ``gmax`` whose column lists are identical; a row whose list differs from its group's
+    first row's is split off into a group of its own.  Returns (gptr int32 [ng + 1], grows
+    int32 [n]): group g holds rows grows[gptr[g]:gptr[g + 1]], every row exactly once."""
+    indptr = np.asarray(indptr, dtype=np.int64)
+    indices = np.asarray(indices)
+    sv_of = np.asarray(sv_of)
+    n = indptr.size - 1
+    order = np.argsort(sv_of, kind='stable').astype(np.int64)
+    sv = sv_of[order]
+    pos = np.arange(n)
+    run_start = np.flatnonzero(np.r_[True, sv[1:] != sv[:-1]])
+    run_id = np.cumsum(np.r_[True, sv[1:] != sv[:-1]]) - 1
+    rank = pos - run_start[run_id]                       # position within the supervariable
+    key = run_id * n + rank // gmax                      # candidate groups of <= gmax rows
+    start = np.flatnonzero(np.r_[True, key[1:] != key[:-1]])
+    gid = np.cumsum(np.r_[True, key[1:] != key[:-1]]) - 1
+    lead = order[start[gid]]                             # first row of each row's group
+    length = indptr[order + 1] - indptr[order]
+    same = length == (indptr[lead + 1] - indptr[lead])
+    # entries: row r's k-th column against its leader's k-th column
+    cand = np.flatnonzero(same & (order != lead))
+    if cand.size:
+        ln = length[cand]
+        rows_e = np.repeat(cand, ln)
+        k = np.arange(ln.sum()) - np.repeat(np.cumsum(ln) - ln, ln)
+        eq = indices[indptr[order[rows_e]] + k] == indices[indptr[lead[rows_e]] + k]
+        bad = np.unique(rows_e[~eq])
+        same[bad] = False
+    # rows that differ from their leader become groups of their own (placed after the others)
+    keep = same | (order == lead)
+    rows_kept = order[keep]
+    gk = gid[keep]
+    counts = np.bincount(gk, minlength=gid.max() + 1 if n else 0)
+    counts = counts[counts > 0]
+    loners = order[~keep]
+    grows = np.concatenate([rows_kept, loners]).astype(np.int32)
+    sizes = np.concatenate([counts, np.ones(loners.size, dtype=np.int64)])
+    gptr = np.r_[0, np.cumsum(sizes)].astype(np.int32)
+    return gptr, grows

@@ -32,7 +32,7 @@ import time
 import numpy as np
 
 from . import _lib
-from .layout import BlochLayout
+from .layout import BlochLayout, residual_groups
 from .piezo_fem import DeviceCSR, Model, _torch, stream_ptr, timed_call
 from .symbolic import analyse
 
@@ -122,7 +122,9 @@ def bc_periodic(model: Model, lines, tag_bottom=21, tag_left=19, tag_right=20, t
 class _ResidualOp:
     """FP64 residual r = rhs - A(w) x of a chunk (A^T with a transposed-slot map), through the
     frequency-major layout: x and rhs are transposed to [n][Bp] (Bp = B padded to 64), the
-    residual kernel reads each A_aa entry once for all frequencies, r is transposed back.
+    residual kernel reads each A_aa entry once for all frequencies and each x row once per
+    row group (the active unknowns of one merged node share their column list), r is
+    transposed back.
 
     A module-level class on purpose: the [n][Bp] buffers must be freed when the chunk's solve
     returns (a class defined inside a method sits in a reference cycle, and its buffers would
@@ -148,9 +150,11 @@ class _ResidualOp:
         n, B, Bp = sv.n_active, self.B, self.Bp
         s = stream_ptr()
         _lib.call('sawfem_transpose', s, B, n, n, Bp, x.data_ptr(), self.xt.data_ptr())
-        _lib.call('sawfem_residual_t', s, n, B, Bp, sv.aa_indptr.data_ptr(), sv.aa_indices.data_ptr(),
-                  sv._dptr(tslot), sv.Kaa.data_ptr(), sv.Maa.data_ptr(), sv._dptr(sv.Daa), self.om.data_ptr(),
-                  sv.beta, self.xt.data_ptr(), rhst.data_ptr(), self.rt.data_ptr())
+        gptr, grows, ng = sv.res_groups
+        _lib.call('sawfem_residual_sv', s, ng, B, Bp, gptr.data_ptr(), grows.data_ptr(), sv.aa_indptr.data_ptr(),
+                  sv.aa_indices.data_ptr(), sv._dptr(tslot), sv.Kaa.data_ptr(), sv.Maa.data_ptr(),
+                  sv._dptr(sv.Daa), self.om.data_ptr(), sv.beta, self.xt.data_ptr(), rhst.data_ptr(),
+                  self.rt.data_ptr())
         _lib.call('sawfem_transpose', s, n, B, Bp, n, self.rt.data_ptr(), r.data_ptr())
 
 
@@ -194,6 +198,8 @@ class FrequencySolver:
         self._kt = (t(bl.kts.ptr), t(bl.kts.idx))
         self.aa_indptr = t(bl.aa_indptr.astype(np.int64))
         self.aa_indices = t(bl.aa_indices.astype(np.int32))
+        gptr, grows = residual_groups(bl.aa_indptr, bl.aa_indices, bl.node_of)
+        self.res_groups = (t(gptr), t(grows), int(gptr.size - 1))
         z = lambda k: torch.empty(k, dtype=torch.complex128, device=dev)
         self.Kaa, self.Maa, self.rK, self.rM = z(bl.nnz_aa), z(bl.nnz_aa), z(n), z(n)
         self.Daa = self.rD = None

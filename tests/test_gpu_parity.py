@@ -360,3 +360,38 @@ def test_edge_cases_empty_single_and_voltage_scaling():
         eng.sweep_charges(f, precision='fp32')
     with pytest.raises(ValueError):
         eng.solve_batch([], precision='mixed')
+
+
+@pytest.mark.parametrize('transposed', [False, True])
+def test_grouped_residual_matches_the_row_residual(transposed):
+    """sawfem_residual_sv (one wave per merged node's rows, each x row read once for the group)
+    equals sawfem_residual_t bit for bit, for A and, with the transposed-slot map, A^T; on the
+    config-3 coarse cell (groups of three active unknowns) and its padded frequency tile."""
+    _say(f'[test_grouped_residual_matches_the_row_residual] transposed={transposed}')
+    import torch
+    from sawsim_amd import _lib
+    from sawsim_amd.piezo_fem import stream_ptr
+    cell = workloads.config3(mesh_um=0.12, points=4)
+    eng = workloads.build_engine(cell, gradient=transposed)
+    n, B, Bp = eng.n_active, 70, 128
+    gptr, grows, ng = eng.res_groups
+    assert ng < n
+    om = torch.as_tensor(2 * np.pi * np.linspace(1.35e9, 1.5e9, B), device='cuda')
+    g = torch.Generator(device='cpu').manual_seed(11)
+    xt = torch.complex(torch.randn(n, Bp, generator=g, dtype=torch.float64),
+                       torch.randn(n, Bp, generator=g, dtype=torch.float64)).cuda()
+    rhst = torch.complex(torch.randn(n, Bp, generator=g, dtype=torch.float64),
+                         torch.randn(n, Bp, generator=g, dtype=torch.float64)).cuda()
+    ts = eng._dptr(eng.tslot) if transposed else None
+    s = stream_ptr()
+    r1 = torch.zeros_like(xt)
+    r2 = torch.zeros_like(xt)
+    _lib.call('sawfem_residual_t', s, n, B, Bp, eng.aa_indptr.data_ptr(), eng.aa_indices.data_ptr(), ts,
+              eng.Kaa.data_ptr(), eng.Maa.data_ptr(), eng._dptr(eng.Daa), om.data_ptr(), eng.beta, xt.data_ptr(),
+              rhst.data_ptr(), r1.data_ptr())
+    _lib.call('sawfem_residual_sv', s, ng, B, Bp, gptr.data_ptr(), grows.data_ptr(), eng.aa_indptr.data_ptr(),
+              eng.aa_indices.data_ptr(), ts, eng.Kaa.data_ptr(), eng.Maa.data_ptr(), eng._dptr(eng.Daa),
+              om.data_ptr(), eng.beta, xt.data_ptr(), rhst.data_ptr(), r2.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(r1[:, :B], r2[:, :B])
+    assert torch.equal(r1, r2)        # the padded columns of the tile too (omega 0 there)

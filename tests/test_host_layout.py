@@ -197,3 +197,27 @@ def test_pivot_guard_leaves_regular_pivots_alone():
         st = {}
         mf_emulator.factor(mf, 1, np.array([w]), 0.0, Kaa, Maa, None, nb=32, dtype=dtype, stats=st)
         assert st['perturbed'] == 0
+
+
+@pytest.mark.parametrize('cell', [workloads.config1(mesh_um=0.12, points=3), workloads.config2(mesh_um=0.3, points=3)],
+                         ids=['config1_coarse', 'config2_coarse'])
+def test_residual_groups_share_column_lists(cell):
+    """Row groups of the grouped FP64 residual: every active row exactly once, at most four rows
+    a group, one merged node a group, identical column lists (checked row by row here), and
+    almost every node in one group (the merge keeps a node's unknowns on one pattern)."""
+    from sawsim_amd.layout import residual_groups
+    m = cell.model
+    kl = KLayout(m.quads, m.N, cell.ndisp + 1)
+    bl = BlochLayout(kl, *bc_periodic(m, cell.mesh.lines, ndisp=cell.ndisp))
+    gptr, grows = residual_groups(bl.aa_indptr, bl.aa_indices, bl.node_of)
+    assert np.array_equal(np.sort(grows), np.arange(bl.n_active))
+    sizes = np.diff(gptr)
+    assert sizes.min() >= 1 and sizes.max() <= 4
+    ip, ix = bl.aa_indptr, bl.aa_indices
+    for g in range(sizes.size):
+        r = grows[gptr[g]:gptr[g + 1]]
+        assert np.unique(bl.node_of[r]).size == 1
+        for x in r[1:]:
+            assert np.array_equal(ix[ip[x]:ip[x + 1]], ix[ip[r[0]]:ip[r[0] + 1]])
+    nodes = np.unique(bl.node_of).size
+    assert sizes.size <= 1.05 * nodes, (sizes.size, nodes)
