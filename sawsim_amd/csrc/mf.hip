@@ -1763,9 +1763,14 @@ __global__ void __launch_bounds__(256) k_residual_t(int n, int B, int Bp, const 
 
 // The same residual over row groups: the rows of a group (at most RES_G, listed in
 // grows[gptr[g] .. gptr[g + 1])) have identical column lists, as the active unknowns of one
-// merged mesh node do, so one wave per group and 64 frequencies reads each x row once for all
-// the group's rows.  Per row the products are summed in the order of k_residual_t.
+// merged mesh node do, so one wave per group reads each x row once for all the group's rows.
+// The wave covers FT tiles of 64 frequencies (ft <= FT of them in the last column of the grid):
+// the group's column indices and K, M, D values are the same for every lane, and with the group
+// index made wave-uniform (readfirstlane) they arrive by scalar loads, each used for the wave's
+// FT x 64 frequencies (per-lane vector loads of them bound the kernel by address processing).
+// Per row the products are summed in the order of k_residual_t, bit for bit the same result.
 constexpr int RES_G = 4;
+template <int FT>
 __global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, const int* __restrict__ gptr,
                                                      const int* __restrict__ grows, const int64_t* __restrict__ indptr,
                                                      const int* __restrict__ indices, const int64_t* __restrict__ tslot,
@@ -1775,31 +1780,45 @@ __global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, cons
                                                      const double2* __restrict__ xt, const double2* __restrict__ rhst,
                                                      double2* __restrict__ rt) {
   const int lane = threadIdx.x & 63;
-  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int g = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (g >= ng) return;
-  const int b = blockIdx.y * 64 + lane;
-  const double w = b < B ? omega[b] : 0.0;
-  const double w2 = w * w, wb = w * beta;
+  const int t0 = blockIdx.y * FT;
+  const int ft = min(FT, Bp / 64 - t0);
+  double w2[FT], wb[FT];
+#pragma unroll
+  for (int f = 0; f < FT; ++f) {
+    const int b = (t0 + f) * 64 + lane;
+    const double w = (f < ft && b < B) ? omega[b] : 0.0;
+    w2[f] = w * w;
+    wb[f] = w * beta;
+  }
   const int g0 = gptr[g], cnt = gptr[g + 1] - g0;
   int64_t base[RES_G];
 #pragma unroll
   for (int t = 0; t < RES_G; ++t) base[t] = indptr[grows[g0 + min(t, cnt - 1)]];
   const int64_t len = indptr[grows[g0] + 1] - base[0];
   const int* cols = indices + base[0];
-  double2 acc[RES_G];
+  double2 acc[RES_G][FT];
 #pragma unroll
-  for (int t = 0; t < RES_G; ++t) acc[t] = make_double2(0, 0);
-#pragma unroll 2
+  for (int t = 0; t < RES_G; ++t)
+#pragma unroll
+    for (int f = 0; f < FT; ++f) acc[t][f] = make_double2(0, 0);
   for (int64_t k = 0; k < len; ++k) {
-    const double2 xv = xt[(int64_t)cols[k] * Bp + b];
+    const int64_t j = cols[k];
+    double2 xv[FT];
+#pragma unroll
+    for (int f = 0; f < FT; ++f) xv[f] = f < ft ? xt[j * Bp + (t0 + f) * 64 + lane] : make_double2(0, 0);
 #pragma unroll
     for (int t = 0; t < RES_G; ++t) {
       if (t < cnt) {
         const int64_t s = tslot ? tslot[base[t] + k] : base[t] + k;
         const double2 kk = Kaa[s], mm = Maa[s];
         const double2 d = Daa ? Daa[s] : make_double2(0, 0);
-        const double2 a = make_double2(kk.x - w2 * mm.x - wb * d.y, kk.y - w2 * mm.y + wb * d.x);
-        acc[t] = cfma(acc[t], a, xv);
+#pragma unroll
+        for (int f = 0; f < FT; ++f) {
+          const double2 a = make_double2(kk.x - w2[f] * mm.x - wb[f] * d.y, kk.y - w2[f] * mm.y + wb[f] * d.x);
+          acc[t][f] = cfma(acc[t][f], a, xv[f]);
+        }
       }
     }
   }
@@ -1807,8 +1826,14 @@ __global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, cons
   for (int t = 0; t < RES_G; ++t) {
     if (t < cnt) {
       const int64_t i = grows[g0 + t];
-      const double2 r = rhst[i * Bp + b];
-      rt[i * Bp + b] = make_double2(r.x - acc[t].x, r.y - acc[t].y);
+#pragma unroll
+      for (int f = 0; f < FT; ++f) {
+        if (f < ft) {
+          const int64_t o = i * Bp + (t0 + f) * 64 + lane;
+          const double2 r = rhst[o];
+          rt[o] = make_double2(r.x - acc[t][f].x, r.y - acc[t][f].y);
+        }
+      }
     }
   }
 }
@@ -3081,9 +3106,16 @@ int sawfem_residual_sv(void* stream, int ng, int B, int Bp, const int* gptr, con
     sawfem_set_error("sawfem_residual_sv: invalid arguments");
     return 1;
   }
-  hipLaunchKernelGGL(k_residual_sv, dim3((ng + 3) / 4, Bp / 64), dim3(256), 0, (hipStream_t)stream, ng, B, Bp, gptr,
-                     grows, indptr, indices, tslot, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa,
-                     omega, beta, (const double2*)xt, (const double2*)rhst, (double2*)rt);
+  // tiles of 64 frequencies per wave: all of them up to 8, else columns of 8 (the last one shorter)
+  const int nt = Bp / 64;
+  const int ft = nt >= 8 ? 8 : (nt >= 4 ? 4 : (nt >= 2 ? 2 : 1));
+  const dim3 grid((ng + 3) / 4, (nt + ft - 1) / ft);
+#define SAW_RES(F)                                                                                                 \
+  hipLaunchKernelGGL(k_residual_sv<F>, grid, dim3(256), 0, (hipStream_t)stream, ng, B, Bp, gptr, grows, indptr,     \
+                     indices, tslot, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, omega, beta,    \
+                     (const double2*)xt, (const double2*)rhst, (double2*)rt)
+  switch (ft) { case 8: SAW_RES(8); break; case 4: SAW_RES(4); break; case 2: SAW_RES(2); break; default: SAW_RES(1); }
+#undef SAW_RES
   SAW_HIP(hipGetLastError());
   return 0;
 }

@@ -362,18 +362,20 @@ def test_edge_cases_empty_single_and_voltage_scaling():
         eng.solve_batch([], precision='mixed')
 
 
-@pytest.mark.parametrize('transposed', [False, True])
-def test_grouped_residual_matches_the_row_residual(transposed):
-    """sawfem_residual_sv (one wave per merged node's rows, each x row read once for the group)
-    equals sawfem_residual_t bit for bit, for A and, with the transposed-slot map, A^T; on the
-    config-3 coarse cell (groups of three active unknowns) and its padded frequency tile."""
-    _say(f'[test_grouped_residual_matches_the_row_residual] transposed={transposed}')
+@pytest.mark.parametrize('transposed,B', [(False, 40), (False, 70), (True, 70), (False, 520), (True, 520)])
+def test_grouped_residual_matches_the_row_residual(transposed, B):
+    """sawfem_residual_sv (one wave per merged node's rows, each x row read once for the group,
+    1, 2 or 8 frequency tiles per wave) equals sawfem_residual_t bit for bit, for A and, with the
+    transposed-slot map, A^T; on the config-3 coarse cell (groups of three active unknowns), with
+    1, 2 and 9 tiles of 64 frequencies (the last column of 8 tiles holding one) and the padded
+    frequencies of the last tile."""
+    _say(f'[test_grouped_residual_matches_the_row_residual] transposed={transposed} B={B}')
     import torch
     from sawsim_amd import _lib
     from sawsim_amd.piezo_fem import stream_ptr
     cell = workloads.config3(mesh_um=0.12, points=4)
     eng = workloads.build_engine(cell, gradient=transposed)
-    n, B, Bp = eng.n_active, 70, 128
+    n, Bp = eng.n_active, -(-B // 64) * 64
     gptr, grows, ng = eng.res_groups
     assert ng < n
     om = torch.as_tensor(2 * np.pi * np.linspace(1.35e9, 1.5e9, B), device='cuda')
