@@ -265,10 +265,17 @@ int sawfem_mf_solve_groups(void* plan, void* stream, int B, int nrhs, const void
  * value in use or -1.  No counterpart in the reference. */
 int sawfem_mf_set_group_rc(void* plan, int rc_max);
 /* sawfem_mf_solve_groups kernel selection: the one-workgroup solve buckets use the LDS-staged
- * block kernel where the padded vectors and the factor block of blocks of 4 vectors fit
- * lds_bytes of LDS (default 48 KiB; 0: never); the other kernels take blocks of at most rc_max
- * vectors (default 4).  Returns rc_max in use or -1.  No counterpart in the reference. */
+ * block kernel where the padded vectors of blocks of 4 vectors fit lds_bytes of LDS (default
+ * 48 KiB), the other buckets the block kernels for the pivot rows and the off-diagonal panels;
+ * lds_bytes = 0 turns both off.  The pivot-triangle and panel kernels that remain (blocks that do
+ * not fit, a partial last group, transposed solves) take blocks of at most rc_max vectors
+ * (default 4).  Returns rc_max in use or -1.  No counterpart in the reference. */
 int sawfem_mf_set_group_tri(void* plan, int rc_max, int lds_bytes);
+/* sawfem_mf_solve_groups: the block kernels for pivot rows and off-diagonal panels take the
+ * two-kernel buckets whose largest pivot block is at most max_p (default 256; 0: none); larger
+ * ones keep the pivot-triangle + panel pair.  Returns the value in use or -1.  No counterpart in
+ * the reference. */
+int sawfem_mf_set_group_big(void* plan, int max_p);
 
 /* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;
  * used by the benchmark).  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S,

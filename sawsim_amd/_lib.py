@@ -51,6 +51,7 @@ SIGNATURES = {
     'sawfem_mf_solve_groups': [_vp, _vp, _i, _i, _vp, _i, _vp, _vp, _i, _i, _vp, _i, _i],
     'sawfem_mf_set_group_rc': [_vp, _i],
     'sawfem_mf_set_group_tri': [_vp, _i, _i],
+    'sawfem_mf_set_group_big': [_vp, _i],
     'sawfem_mf_profile': [_vp, _i],
     'sawfem_mf_profile_read': [_vp, _vp, _vp],
     'sawfem_rhs': [_vp, _i, _i, _vp, _d, _d, _vp, _vp, _vp, _vp],

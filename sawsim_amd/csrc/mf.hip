@@ -1474,7 +1474,11 @@ __device__ __forceinline__ void grp_row_product(const float2* __restrict__ col, 
   }
 }
 
-template <int KIND, int RC>
+// PIV: the pivot rows only (fronts whose off-diagonal panels k_grp_panel handles): w holds the
+// p pivot rows, kind 0 adds the children's updates of the pivot rows only (k_grp_panel forms
+// the update vectors), kind 1 starts from x at the pivot rows (k_grp_panel has already
+// subtracted U12 times the contribution-block rows).
+template <int KIND, int RC, bool PIV = false>
 __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
                                                  const float2* __restrict__ arena, double2* __restrict__ uarena,
                                                  double2* __restrict__ x, const int* __restrict__ sel) {
@@ -1485,21 +1489,22 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
   const int f = lvl[blockIdx.x];
   const int b = solve_block(F, sel, blockIdx.y);
   const int m = F.m[f], p = F.p[f], q = m - p;
-  double2* w = reinterpret_cast<double2*>(smem_raw);                  // [m][WS]
+  const int W = PIV ? p : m;                    // rows held in w
+  double2* w = reinterpret_cast<double2*>(smem_raw);                  // [W][WS]
   const float2* Fb = factor_ptr<float>(const_cast<float2*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
   const int t = threadIdx.x;
   const int64_t v0 = (int64_t)b * RC;           // first vector of the block
   // vectors, row index fastest over the threads (runs of x rows coalesce); kind 0 keeps zero
   // at the contribution-block rows before the children's updates
-  const int mr = m * RC;
+  const int mr = W * RC;
   batched<8>(t, mr, 256,
              [&](int idx) {
-               const int r = idx / m, i = idx - r * m;
+               const int r = idx / W, i = idx - r * W;
                return x[(v0 + r) * n + rows[i]];
              },
              [&](int idx, double2 v) {
-               const int r = idx / m, i = idx - r * m;
+               const int r = idx / W, i = idx - r * W;
                w[i * WS + r] = (KIND == 1 || i < p) ? v : make_double2(0, 0);
              });
   if (KIND == 0) {
@@ -1512,11 +1517,11 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
       // each thread adds to the w entries it wrote above (same index -> thread map): no barrier
       batched<8>(t, mr, 256,
                  [&](int idx) {
-                   const int r = idx / m, i = idx - r * m;
+                   const int r = idx / W, i = idx - r * W;
                    return u[(int64_t)r * qc + max(inv[i], 0)];
                  },
                  [&](int idx, double2 v) {
-                   const int r = idx / m, i = idx - r * m;
+                   const int r = idx / W, i = idx - r * W;
                    if (inv[i] >= 0) w[i * WS + r] = d2add(w[i * WS + r], v);
                  });
     }
@@ -1529,7 +1534,7 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
   auto row_update = [&](int i, int c0, int k0, int k1, int wrow0) {
     grp_row_product<TPR, VPT, WS, true>(Fb + i + (int64_t)c0 * m, m, k0, k1, q4, w, wrow0, r0, s);
   };
-  if (KIND == 1 && q > 0) {
+  if (KIND == 1 && !PIV && q > 0) {
     // contribution-block rows -> pivot rows: w[i] -= sum_j U[i][p + j] w[p + j]
     for (int i = tr; i < p; i += RPP) {
 #pragma unroll
@@ -1565,7 +1570,7 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
     }
     __syncthreads();
     // the other rows of the block's columns: forward below the block (all m rows), backward above
-    const int lo = KIND == 0 ? J0 + nbk : 0, hi = KIND == 0 ? m : J0;
+    const int lo = KIND == 0 ? J0 + nbk : 0, hi = KIND == 0 ? W : J0;
     for (int i = lo + tr; i < hi; i += RPP) {
 #pragma unroll
       for (int v = 0; v < VPT; ++v) s[v] = w[i * WS + r0 + v];
@@ -1577,9 +1582,89 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
   }
   double2* uf = uarena + F.uoff[f] * F.nvec + v0 * q;
   for (int idx = t; idx < mr; idx += 256) {
-    const int r = idx / m, i = idx - r * m;
+    const int r = idx / W, i = idx - r * W;
     if (i < p) x[(v0 + r) * n + rows[i]] = w[i * WS + r];
     else if (KIND == 0) uf[(int64_t)r * q + (i - p)] = w[i * WS + r];
+  }
+}
+
+// Off-diagonal panel of one front for one block of RC vectors sharing its factors, 64 output
+// rows per workgroup (lvl: tile list front << 12 | tile, as k_panel), 64 TPR threads: TPR =
+// RC / 4 threads per row, four vectors each, the row's factor entries split over them and
+// passed on by DPP (grp_row_product); the input vector entries are staged in LDS in chunks of
+// GRP_VC, [k][WS] as k_grp_lds's w.
+//  kind 0 (forward,  L21): uf[o] = sum_c child_c[o] - sum_{k<p} L[p + o][k] y[k]   o: the front's
+//                          contribution-block rows; y = x at the pivot rows (solved); the
+//                          children's update vectors are gathered here (k_grp_lds with PIV
+//                          takes only the pivot rows)
+//  kind 1 (backward, U12): x[o] -= sum_{j<q} U[o][p + j] x[p + j]                   o: pivot rows
+constexpr int GRP_VC = 128;
+template <int KIND, int RC>
+__global__ void __launch_bounds__(256) k_grp_panel(FrontsDev F, const int* __restrict__ lvl, int B, int n,
+                                                   const float2* __restrict__ arena, double2* __restrict__ uarena,
+                                                   double2* __restrict__ x, const int* __restrict__ sel) {
+  static_assert(KIND == 0 || KIND == 1, "forward L21 or backward U12");
+  constexpr int VPT = 4, TPR = RC / VPT, WS = RC + 1, NT = 64 * TPR;
+  static_assert(RC % VPT == 0 && RC <= 16, "blocks of 4, 8 or 16 vectors");
+  extern __shared__ unsigned char smem_raw[];
+  double2* vin = reinterpret_cast<double2*>(smem_raw);               // [GRP_VC][WS]
+  const int f = lvl[blockIdx.x] >> 12;
+  const int o0 = (lvl[blockIdx.x] & 4095) * 64;
+  const int m = F.m[f], p = F.p[f], q = m - p;
+  const int nout = KIND == 0 ? q : p, nin = KIND == 0 ? p : q;
+  if (o0 >= nout || nin == 0) return;
+  const int b = solve_block(F, sel, blockIdx.y);
+  const float2* Fb = factor_ptr<float>(const_cast<float2*>(arena), F, f, b, B);
+  const int* rows = F.rows + F.rows_ptr[f];
+  const int t = threadIdx.x;
+  const int o = t / TPR, q4 = t % TPR, r0 = q4 * VPT;
+  const bool act = o0 + o < nout;
+  const int64_t v0 = (int64_t)b * RC;
+  const int in0 = KIND == 0 ? 0 : p;                                  // first input row of the front
+  const int frow = (KIND == 0 ? p : 0) + o0 + o;                      // the thread's factor row
+  double2 s[VPT];
+#pragma unroll
+  for (int v = 0; v < VPT; ++v) s[v] = make_double2(0, 0);
+  for (int c0 = 0; c0 < nin; c0 += GRP_VC) {
+    const int nc = min(GRP_VC, nin - c0);
+    __syncthreads();
+    batched<8>(t, nc * RC, NT,
+               [&](int idx) {
+                 const int r = idx / nc, k = idx - r * nc;
+                 return x[(v0 + r) * n + rows[in0 + c0 + k]];
+               },
+               [&](int idx, double2 v) {
+                 const int r = idx / nc, k = idx - r * nc;
+                 vin[k * WS + r] = v;
+               });
+    __syncthreads();
+    if (act)
+      grp_row_product<TPR, VPT, WS, true>(Fb + frow + (int64_t)(in0 + c0) * m, m, 0, nc, q4, vin, 0, r0, s);
+  }
+  if (!act) return;
+  if (KIND == 0) {
+    const int i = p + o0 + o;                                         // front row
+    const int cp0 = F.child_ptr[f], nch = F.child_ptr[f + 1] - cp0;
+    for (int c = 0; c < nch; ++c) {
+      const int ci = F.inv[F.inv_ptr[cp0 + c] + i];
+      if (ci >= 0) {
+        const int ch = F.child_ids[cp0 + c];
+        const int qc = F.m[ch] - F.p[ch];
+        const double2* u = uarena + F.uoff[ch] * F.nvec + (v0 + r0) * qc + ci;
+#pragma unroll
+        for (int v = 0; v < VPT; ++v) s[v] = d2add(s[v], u[(int64_t)v * qc]);
+      }
+    }
+    double2* uf = uarena + F.uoff[f] * F.nvec + (v0 + r0) * q + (o0 + o);
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) uf[(int64_t)v * q] = s[v];
+  } else {
+    const int row = rows[o0 + o];
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) {
+      double2* d = x + (v0 + r0 + v) * n + row;
+      *d = d2add(*d, s[v]);
+    }
   }
 }
 
@@ -1981,6 +2066,7 @@ struct Plan {
   int group_rc = 16;             // grouped solves: vectors per workgroup of k_grp_lds at most
   int group_rc_tri = 4;          // grouped solves: vectors per workgroup of the other kernels
   int grp_lds_bytes = 48 * 1024;   // one-workgroup buckets whose k_grp_lds LDS fits: k_grp_lds
+  int grp_big_pmax = 256;          // two-kernel buckets with pivot blocks up to this: k_grp_lds + k_grp_panel
   std::vector<void*> part_owned[2];
   std::vector<void*> panel_owned[2];   // the buckets' panel work lists (rebuilt per panel width)
   int panel_nb[2] = {0, 0};      // panel width the buckets' panel lists were built for (0: none)
@@ -2565,20 +2651,17 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
       }
     }
   };
-  auto panel_sub = [&](const Sub& S, int kind, hipStream_t st) {
+  auto panel_lds_of = [&](int kind, int rc) {
+    return sizeof(double2) * rc * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
+  };
+  // off-diagonal panel kernel on one launch set
+  auto panel_launch = [&](const Sub& S, int kind, hipStream_t st, const SolveSet& Q) {
     const bool out_cb = (kind == 0 || kind == 2);
     const int nt = out_cb ? S.nstiles_cb : S.nstiles_p;
     const int* tl = out_cb ? S.stiles_cb : S.stiles_p;
     if (nt == 0) return;
-    auto lds_of = [&](int rc) {
-      return sizeof(double2) * rc * (256 + 256) + ((kind >= 2) ? sizeof(V) * 64 * LD : 0);
-    };
-    Timed tm(P, st, out_cb ? K_FWD : K_BWD);
-    SolveSet sets[2];
-    const int ns = solve_sets(P, C, C.rc_tri, lds_of, sets);
-    for (int k = 0; k < ns; ++k) {
-      const SolveSet& Q = sets[k];
-      const size_t lds = lds_of(Q.rc);
+    {
+      const size_t lds = panel_lds_of(kind, Q.rc);
       const dim3 grid(nt, Q.ny);
       bad |= !with_rc<T>(Q.rc, [&](auto c) {
         constexpr int RC = decltype(c)::value;
@@ -2593,6 +2676,69 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
                         case 2: SAW_PANEL(2); break; default: SAW_PANEL(3); }
 #undef SAW_PANEL
       });
+    }
+  };
+  auto panel_sub = [&](const Sub& S, int kind, hipStream_t st) {
+    const bool out_cb = (kind == 0 || kind == 2);
+    if ((out_cb ? S.nstiles_cb : S.nstiles_p) == 0) return;
+    Timed tm(P, st, out_cb ? K_FWD : K_BWD);
+    SolveSet sets[2];
+    const int ns = solve_sets(P, C, C.rc_tri, [&](int rc) { return panel_lds_of(kind, rc); }, sets);
+    for (int k = 0; k < ns; ++k) panel_launch(S, kind, st, sets[k]);
+  };
+  // grouped calls, two-kernel buckets (kinds 0/1): the pivot rows by k_grp_lds (PIV) and the
+  // off-diagonal panel by k_grp_panel, on blocks of 4 or more vectors whose pivot rows fit the
+  // LDS; forward: pivot rows, then the panel (which also gathers the children's updates of the
+  // contribution-block rows); backward: the panel, then the pivot rows.  A partial last group
+  // goes through the pivot-triangle + panel pair.
+  auto grp_big_lds_of = [&](const Sub& S, int rc) { return sizeof(double2) * (size_t)S.pmax * (rc + 1); };
+  auto grp_big_ok = [&](const Sub& S) {
+    return sizeof(T) == 4 && C.G > 0 && !transpose && C.rc_max >= 4 && (C.G * C.R) % 4 == 0 &&
+           P->grp_lds_bytes > 0 && S.pmax <= P->grp_big_pmax && grp_big_lds_of(S, 4) <= (size_t)P->lds_max;
+  };
+  auto grp_big = [&](const Level& L, const Sub& S, int kind, hipStream_t st) {
+    if (S.nf == 0) return;
+    Timed tm(P, st, kind == 0 ? K_FWD : K_BWD);
+    const int nbl = panel_width(nb, L);
+    SolveSet sets[2];
+    const int ns = solve_sets(P, C, C.rc_max, [&](int rc) { return grp_big_lds_of(S, rc); }, sets);
+    for (int k = 0; k < ns; ++k) {
+      const SolveSet& Q = sets[k];
+      if (Q.rc < 4) {
+        if (kind == 0) { tri_launch(L, S, 0, false, st, Q); panel_launch(S, 0, st, Q); }
+        else { panel_launch(S, 1, st, Q); tri_launch(L, S, 1, false, st, Q); }
+        continue;
+      }
+      if constexpr (sizeof(T) == 4) {
+        const size_t lds = grp_big_lds_of(S, Q.rc);
+        const int ntile = kind == 0 ? S.nstiles_cb : S.nstiles_p;
+        const int* tl = kind == 0 ? S.stiles_cb : S.stiles_p;
+#define SAW_GRP_PIV(K, RCV)                                                                                       \
+  do {                                                                                                            \
+    if (lds > 64 * 1024)                                                                                          \
+      (void)hipFuncSetAttribute((const void*)k_grp_lds<K, RCV, true>, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                (int)lds);                                                                        \
+    hipLaunchKernelGGL((k_grp_lds<K, RCV, true>), dim3(S.nf, Q.ny), dim3(256), lds, st, Q.F, S.fronts, B, P->n,   \
+                       nbl, A, U, X, Q.sel);                                                                      \
+  } while (0)
+#define SAW_GRP_PAN(K, RCV)                                                                                       \
+  do {                                                                                                            \
+    if (ntile > 0)                                                                                                \
+      hipLaunchKernelGGL((k_grp_panel<K, RCV>), dim3(ntile, Q.ny), dim3(16 * RCV), sizeof(double2) * GRP_VC *     \
+                         (RCV + 1), st, Q.F, tl, B, P->n, A, U, X, Q.sel);                                        \
+  } while (0)
+        if (kind == 0) {
+          if (Q.rc == 4) { SAW_GRP_PIV(0, 4); SAW_GRP_PAN(0, 4); }
+          else if (Q.rc == 8) { SAW_GRP_PIV(0, 8); SAW_GRP_PAN(0, 8); }
+          else { SAW_GRP_PIV(0, 16); SAW_GRP_PAN(0, 16); }
+        } else {
+          if (Q.rc == 4) { SAW_GRP_PAN(1, 4); SAW_GRP_PIV(1, 4); }
+          else if (Q.rc == 8) { SAW_GRP_PAN(1, 8); SAW_GRP_PIV(1, 8); }
+          else { SAW_GRP_PAN(1, 16); SAW_GRP_PIV(1, 16); }
+        }
+#undef SAW_GRP_PIV
+#undef SAW_GRP_PAN
+      }
     }
   };
   // The one-workgroup set and the size buckets of a level are independent: with more than one,
@@ -2652,8 +2798,12 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
     }
     for (size_t gi = 0; gi < L.sbig[pi].size(); ++gi) {
       const hipStream_t st = stream_of(fork, gi + nfull);
-      tri_sub(L, L.sbig[pi][gi], kf, false, st);
-      panel_sub(L.sbig[pi][gi], kf, st);
+      if (grp_big_ok(L.sbig[pi][gi])) {
+        grp_big(L, L.sbig[pi][gi], 0, st);
+      } else {
+        tri_sub(L, L.sbig[pi][gi], kf, false, st);
+        panel_sub(L.sbig[pi][gi], kf, st);
+      }
     }
     if (fork && join_side(P, s)) return 3;
   }
@@ -2669,8 +2819,12 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
       if (fork && (ensure_side(P) || fork_side(P, s))) return 3;
       for (size_t gi = 0; gi < L.sbig[pi].size(); ++gi) {
         const hipStream_t st = stream_of(fork, gi);
-        panel_sub(L.sbig[pi][gi], kb, st);
-        tri_sub(L, L.sbig[pi][gi], kb, false, st);
+        if (grp_big_ok(L.sbig[pi][gi])) {
+          grp_big(L, L.sbig[pi][gi], 1, st);
+        } else {
+          panel_sub(L.sbig[pi][gi], kb, st);
+          tri_sub(L, L.sbig[pi][gi], kb, false, st);
+        }
       }
       for (size_t gi = 0; gi < nfull; ++gi) {
         const Sub& S = L.ssmall[pi][gi];
@@ -3014,6 +3168,13 @@ int sawfem_mf_set_group_rc(void* plan, int rc_max) {
   if (!P || rc_max < 1) { sawfem_set_error("sawfem_mf_set_group_rc: invalid arguments"); return -1; }
   P->group_rc = std::min(rc_max, 16);
   return P->group_rc;
+}
+
+int sawfem_mf_set_group_big(void* plan, int max_p) {
+  Plan* P = (Plan*)plan;
+  if (!P || max_p < 0) { sawfem_set_error("sawfem_mf_set_group_big: invalid arguments"); return -1; }
+  P->grp_big_pmax = max_p;
+  return P->grp_big_pmax;
 }
 
 int sawfem_mf_set_group_tri(void* plan, int rc_max, int lds_bytes) {
