@@ -468,6 +468,20 @@ def main():
     _lib.call('sawfem_mf_profile', eng._plan, 0)
     value = args.steps * P / dt
     log(f'[bench] mixed sweep + adjoint {dt / args.steps:.3f}s/step -> {value:.1f} pts/s; ir {info}')
+    # the same sweep with every factorization one replayed HIP graph (the timed region above runs
+    # the factorizations launch by launch: its per-launch HIP events feed the roofline)
+    sweep()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        sweep()
+    torch.cuda.synchronize()
+    graph = dict(pts_per_s=2 * P / (time.perf_counter() - t0),
+                 nodes_per_factorization=int(_lib.call_value('sawfem_mf_graph_nodes', eng._plan)),
+                 note='2 sweeps after the timed region, factorizations as captured HIP graphs '
+                      '(sawfem_mf_factor_graph), no per-launch events')
+    log(f'[bench] graph-captured factorizations: {graph["pts_per_s"]:.1f} pts/s, '
+        f'{graph["nodes_per_factorization"]} graph nodes per factorization')
     # accuracy of the mixed path against the plain FP64 path (with its FP64 adjoint) on a sample
     torch.cuda.empty_cache()          # the FP64 chunks have another size than the timed ones
     # accuracy of the mixed path against the FP64 path (batched complex128 fronts, FP64 adjoint)
@@ -585,6 +599,7 @@ def main():
                        'time; the Schur complements run one at a time on the launch stream',
         kernel_ms_per_level=prof_levels,
         factor_flops_per_point=work['factor_flops'],
+        factor_graph=graph,
         cpu_baseline=cpu,
     )
     print(json.dumps(out), flush=True)

@@ -230,6 +230,21 @@ int sawfem_mf_set_tri_lds(void* plan, int max_bytes);
 int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* omega, double beta, const void* Kaa,
                      const void* Maa, const void* Daa, void* arena, int nb);
 
+/* sawfem_mf_factor as one HIP graph launch: the factorization's launch sequence (every tree
+ * level, the side-stream forks and joins) is captured once per argument set (prec, B, the
+ * omega/Kaa/Maa/Daa/arena pointers, beta, nb) and replayed with hipGraphLaunch on `stream`;
+ * the values behind the pointers are read when the graph runs, so a caller that keeps its
+ * buffers (new frequencies written into the same omega buffer, the same workspace) replays one
+ * graph per chunk size.  Up to four graphs per plan (least recently used replaced); changing a
+ * kernel-selection knob drops them.  With per-launch profiling on it runs eagerly.  Same results
+ * as sawfem_mf_factor bit for bit.  The north star's graph-captured batched factorization; in the
+ * reference the per-frequency backend call (SawSim saw2d/sweep.py:349-353). */
+int sawfem_mf_factor_graph(void* plan, void* stream, int prec, int B, const double* omega, double beta,
+                           const void* Kaa, const void* Maa, const void* Daa, void* arena, int nb);
+/* Nodes (kernel launches and stream dependencies) of the graph the last sawfem_mf_factor_graph
+ * replayed, or -1.  No counterpart in the reference. */
+int64_t sawfem_mf_graph_nodes(void* plan);
+
 /* Forward/backward substitution with the factors of sawfem_mf_factor; x: B x nrhs x n
  * complex128 (nrhs = 1 or 2 right-hand sides per frequency, sharing its factors: each factor
  * entry is read once for all of them), right-hand sides in, solutions out.

@@ -46,6 +46,8 @@ SIGNATURES = {
     'sawfem_mf_set_tri_lds': [_vp, _i],
     'sawfem_mf_profile_levels': [_vp, _vp, _i],
     'sawfem_mf_factor': [_vp, _vp, _i, _i, _vp, _d, _vp, _vp, _vp, _vp, _i],
+    'sawfem_mf_factor_graph': [_vp, _vp, _i, _i, _vp, _d, _vp, _vp, _vp, _vp, _i],
+    'sawfem_mf_graph_nodes': [_vp],
     'sawfem_mf_solve': [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _i],
     'sawfem_mf_solve_shared': [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _vp, _i],
     'sawfem_mf_solve_groups': [_vp, _vp, _i, _i, _vp, _i, _vp, _vp, _i, _i, _vp, _i, _i],
@@ -69,7 +71,7 @@ SIGNATURES = {
                                     _i, _vp, _vp, _vp],
 }
 _RESTYPE = {'sawfem_last_error': C.c_char_p, 'sawfem_mf_arena_elems': _i64, 'sawfem_mf_uarena_elems': _i64,
-            'sawfem_sweep_chunk_bytes': _i64, 'sawfem_host_size': _i64}
+            'sawfem_sweep_chunk_bytes': _i64, 'sawfem_host_size': _i64, 'sawfem_mf_graph_nodes': _i64}
 
 # status codes of the C ABI (include/sawfem.h)
 OK, INVALID, PLAN_LIMIT, HIP_ERROR, OUT_OF_MEMORY, NON_FINITE, REFACTORED = 0, 1, 2, 3, 4, 5, 6

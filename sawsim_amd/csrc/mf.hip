@@ -2079,7 +2079,31 @@ struct Plan {
   std::vector<int> hm, hp, hlvl_ptr, hlvl_fronts;
   std::vector<int> hchild_ptr, hent_loc;       // host copies for the leaf-front work lists
   std::vector<int64_t> hfent_lo, hfent_hi;
+  // captured factorizations (sawfem_mf_factor_graph): the launch sequence of one argument set,
+  // replayed by hipGraphLaunch; dropped when a kernel-selection knob changes
+  struct FactorGraph {
+    int prec, B, nb;
+    const void *omega, *Kaa, *Maa, *Daa, *arena;
+    double beta;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    size_t nodes;
+    uint64_t used;
+  };
+  static constexpr int NGRAPH = 4;
+  std::vector<FactorGraph> graphs;
+  uint64_t graph_clock = 0;
+  size_t graph_nodes = 0;          // nodes of the graph the last sawfem_mf_factor_graph ran
+  hipStream_t capture = nullptr;   // stream the factorizations are captured on
 };
+
+void drop_graphs(Plan* P) {
+  for (auto& g : P->graphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  P->graphs.clear();
+}
 
 // Record a start/stop event pair around one launch when profiling is on.
 struct Timed {
@@ -3025,6 +3049,8 @@ int sawfem_mf_destroy(void* plan) {
     for (void* d : P->panel_owned[pi]) (void)hipFree(d);
   }
   for (hipEvent_t e : P->ev_pool) (void)hipEventDestroy(e);
+  drop_graphs(P);
+  if (P->capture) (void)hipStreamDestroy(P->capture);
   delete P;
   return 0;
 }
@@ -3035,6 +3061,7 @@ int sawfem_mf_destroy(void* plan) {
 int sawfem_mf_set_small(void* plan, int prec, int max_m) {
   Plan* P = (Plan*)plan;
   if (!P || (prec != 0 && prec != 1)) { sawfem_set_error("sawfem_mf_set_small: invalid arguments"); return -1; }
+  drop_graphs(P);
   if (build_partition(P, prec, max_m)) return -1;
   return P->small_mmax[prec];
 }
@@ -3059,6 +3086,7 @@ int sawfem_mf_set_tri_lds(void* plan, int max_bytes) {
 int sawfem_mf_set_panel_lu(void* plan, int max_m) {
   Plan* P = (Plan*)plan;
   if (!P || max_m < 0) { sawfem_set_error("sawfem_mf_set_panel_lu: invalid arguments"); return -1; }
+  drop_graphs(P);
   P->panel_lu_m = max_m;
   return P->panel_lu_m;
 }
@@ -3066,6 +3094,7 @@ int sawfem_mf_set_panel_lu(void* plan, int max_m) {
 int sawfem_mf_set_solve_full(void* plan, int max_m) {
   Plan* P = (Plan*)plan;
   if (!P || max_m < 0) { sawfem_set_error("sawfem_mf_set_solve_full: invalid arguments"); return -1; }
+  drop_graphs(P);
   P->solve_full_m = std::min(max_m, 1024);
   for (int pi = 0; pi < 2; ++pi)
     if (build_partition(P, pi, P->small_mmax[pi])) return -1;
@@ -3119,6 +3148,65 @@ int sawfem_mf_factor(void* plan, void* stream, int prec, int B, const double* om
   return prec == 0 ? factor_impl<float>(P, s, B, omega, beta, Kaa, Maa, Daa, arena, nb)
                    : factor_impl<double>(P, s, B, omega, beta, Kaa, Maa, Daa, arena, nb);
 }
+
+int sawfem_mf_factor_graph(void* plan, void* stream, int prec, int B, const double* omega, double beta,
+                           const void* Kaa, const void* Maa, const void* Daa, void* arena, int nb) {
+  Plan* P = (Plan*)plan;
+  if (!P || B <= 0 || !omega || !Kaa || !Maa || !arena || (nb != 32 && nb != 64) || (prec != 0 && prec != 1) ||
+      (prec == 1 && nb > 32)) {
+    sawfem_set_error("sawfem_mf_factor_graph: invalid arguments");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  // per-launch event timing does not go into graphs: the profiled factorization runs eagerly
+  if (P->profile) return sawfem_mf_factor(plan, stream, prec, B, omega, beta, Kaa, Maa, Daa, arena, nb);
+  Plan::FactorGraph* hit = nullptr;
+  for (auto& g : P->graphs)
+    if (g.prec == prec && g.B == B && g.nb == nb && g.omega == omega && g.Kaa == Kaa && g.Maa == Maa &&
+        g.Daa == Daa && g.arena == arena && g.beta == beta)
+      hit = &g;
+  if (!hit) {
+    // everything the launches need from the host side first (work lists, side streams), so the
+    // capture holds launches and stream dependencies only
+    const int pi = prec == 0 ? 0 : 1;
+    if (P->panel_nb[pi] != nb && build_panel_lists(P, pi, nb)) return 1;
+    if (P->overlap && ensure_side(P)) return 3;
+    if (!P->capture) SAW_HIP(hipStreamCreateWithFlags(&P->capture, hipStreamNonBlocking));
+    SAW_HIP(hipStreamBeginCapture(P->capture, hipStreamCaptureModeThreadLocal));
+    const int rc = prec == 0 ? factor_impl<float>(P, P->capture, B, omega, beta, Kaa, Maa, Daa, arena, nb)
+                             : factor_impl<double>(P, P->capture, B, omega, beta, Kaa, Maa, Daa, arena, nb);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(P->capture, &graph);
+    if (rc || ec != hipSuccess || !graph) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (!rc) sawfem_set_error("sawfem_mf_factor_graph: capture failed");
+      return rc ? rc : 3;
+    }
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      sawfem_set_error("sawfem_mf_factor_graph: graph instantiation failed");
+      return 3;
+    }
+    size_t nodes = 0;
+    (void)hipGraphGetNodes(graph, nullptr, &nodes);
+    if ((int)P->graphs.size() >= Plan::NGRAPH) {      // replace the least recently used
+      auto lru = std::min_element(P->graphs.begin(), P->graphs.end(),
+                                  [](const Plan::FactorGraph& a, const Plan::FactorGraph& b) { return a.used < b.used; });
+      (void)hipGraphExecDestroy(lru->exec);
+      (void)hipGraphDestroy(lru->graph);
+      P->graphs.erase(lru);
+    }
+    P->graphs.push_back(Plan::FactorGraph{prec, B, nb, omega, Kaa, Maa, Daa, arena, beta, graph, exec, nodes, 0});
+    hit = &P->graphs.back();
+  }
+  hit->used = ++P->graph_clock;
+  P->graph_nodes = hit->nodes;
+  SAW_HIP(hipGraphLaunch(hit->exec, s));
+  return 0;
+}
+
+int64_t sawfem_mf_graph_nodes(void* plan) { return plan ? (int64_t)((Plan*)plan)->graph_nodes : -1; }
 
 // x: [B][nrhs][n] complex128 right-hand sides (nrhs = 1 or 2 per frequency, sharing that
 // frequency's factors), overwritten with the solutions.  nb must be the value the arena was

@@ -214,6 +214,7 @@ class FrequencySolver:
         self.analysis_seconds = time.time() - t0
         self._plan = self._create_plan()
         self.share, self._group = SHARE, 0
+        self.use_graph, self._om_buf = True, None
         self.set_solve_full(SOLVE_FULL_M)
         self.set_panel_lu(PANEL_LU_M)
         self.set_tri_lds(TRI_LDS_BYTES)
@@ -525,8 +526,7 @@ class FrequencySolver:
         self._group = share if share > 1 else 0
         arena = self._arena(self.mf.arena_elems * Bf, torch.complex64 if prec == 0 else torch.complex128)
         uarena = torch.empty(max(1, self.mf.uarena_elems * B * R), dtype=torch.complex128, device=dev)
-        _lib.call('sawfem_mf_factor', self._plan, s, prec, Bf, om_f.data_ptr(), self.beta, self.Kaa.data_ptr(),
-                  self.Maa.data_ptr(), self._dptr(self.Daa), arena.data_ptr(), nb)
+        self._factor(prec, Bf, om_f, arena, nb)
         rhs = torch.empty((B, n), dtype=torch.complex128, device=dev)
         _lib.call('sawfem_rhs', s, n, B, om.data_ptr(), self.beta, float(V), self.rK.data_ptr(), self.rM.data_ptr(),
                   self._dptr(self.rD), rhs.data_ptr())
@@ -573,6 +573,21 @@ class FrequencySolver:
         out['refactored'] = refactored
         out['own_factors'] = own
         return out
+
+    def _factor(self, prec, Bf, om_f, arena, nb):
+        """Factor the chunk: with ``use_graph`` one HIP graph launch (``sawfem_mf_factor_graph``),
+        the frequencies copied into a buffer the engine keeps, so every chunk of the same size
+        (and workspace) replays one captured graph; else launch by launch (``sawfem_mf_factor``)."""
+        s = stream_ptr()
+        if not self.use_graph:
+            _lib.call('sawfem_mf_factor', self._plan, s, prec, Bf, om_f.data_ptr(), self.beta, self.Kaa.data_ptr(),
+                      self.Maa.data_ptr(), self._dptr(self.Daa), arena.data_ptr(), nb)
+            return
+        if self._om_buf is None or self._om_buf.numel() < Bf:
+            self._om_buf = self.torch.empty(max(Bf, 64), dtype=self.torch.float64, device=self.dev)
+        self._om_buf[:Bf].copy_(om_f)
+        _lib.call('sawfem_mf_factor_graph', self._plan, s, prec, Bf, self._om_buf.data_ptr(), self.beta,
+                  self.Kaa.data_ptr(), self.Maa.data_ptr(), self._dptr(self.Daa), arena.data_ptr(), nb)
 
     def set_share(self, k):
         """Consecutive frequencies per complex64 factorization (see :meth:`solve_batch`)."""
