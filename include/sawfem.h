@@ -335,6 +335,14 @@ int sawfem_transpose(void* stream, int R, int C, int ld_in, int ld_out, const vo
 int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, double kz_im, double V,
                   const void* x, void* Q);
 
+/* out[k] = sum_i a[b sa + i] w[i] c[b sc + i] for k < nb, b = sel[k] (device int, or k when sel is
+ * NULL): complex128 a, c (strides sa, sc in complex entries), real weights w (NULL: 1), no
+ * conjugation.  The refinement's duality products lambda^T D b and dual-weighted residuals
+ * lambda^T D r in one pass over the two vectors; the quantities of SawSim's charge reduction
+ * (saw2d/sweep.py:360-364, Q = kts . disp) carried over to the adjoint. */
+int sawfem_wdot(void* stream, int n, int nb, const void* a, int64_t sa, const double* w, const void* c, int64_t sc,
+                const int* sel, void* out);
+
 /* x += d (n complex128). */
 int sawfem_axpy(void* stream, int64_t n, void* x, const void* d);
 

@@ -1963,6 +1963,39 @@ __global__ void __launch_bounds__(CHARGE_NT) k_charge(int n, const double2* __re
   if (threadIdx.x == 0) Q[b] = make_double2(red[0].x + V * kts_z.x, red[0].y + V * kts_z.y);
 }
 
+// out[k] = sum_i a_b[i] w[i] c_b[i] (no conjugation; w real, or 1 when null) for b = sel[k] (or
+// k): the refinement's duality products and dual-weighted residuals in one pass over the two
+// vectors, one 1024-thread block per product, fixed-order tree reduction (deterministic).
+constexpr int WDOT_NT = 1024;
+__global__ void __launch_bounds__(WDOT_NT) k_wdot(int n, const double2* __restrict__ a, int64_t sa,
+                                                  const double* __restrict__ w, const double2* __restrict__ c,
+                                                  int64_t sc, const int* __restrict__ sel, double2* __restrict__ out) {
+  __shared__ double2 red[WDOT_NT];
+  const int k = blockIdx.x;
+  const int64_t b = sel ? sel[k] : k;
+  const double2* ab = a + b * sa;
+  const double2* cb = c + b * sc;
+  double2 s = make_double2(0, 0);
+#pragma unroll 4
+  for (int i = threadIdx.x; i < n; i += WDOT_NT) {
+    double2 u = ab[i];
+    const double wi = w ? w[i] : 1.0;
+    u.x *= wi;
+    u.y *= wi;
+    s = cfma(s, u, cb[i]);
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = WDOT_NT / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[threadIdx.x].x += red[threadIdx.x + h].x;
+      red[threadIdx.x].y += red[threadIdx.x + h].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[k] = red[0];
+}
+
 __global__ void k_axpy(int64_t n, double2* __restrict__ x, const double2* __restrict__ d) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -3384,6 +3417,19 @@ int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, d
   if (n <= 0 || B <= 0) { sawfem_set_error("sawfem_charge: invalid arguments"); return 1; }
   hipLaunchKernelGGL(k_charge, dim3(B), dim3(CHARGE_NT), 0, (hipStream_t)stream, n, (const double2*)kts_a,
                      make_double2(kz_re, kz_im), V, (const double2*)x, (double2*)Q);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_wdot(void* stream, int n, int nb, const void* a, int64_t sa, const double* w, const void* c, int64_t sc,
+                const int* sel, void* out) {
+  if (n <= 0 || nb < 0 || !a || !c || !out || sa < 0 || sc < 0) {
+    sawfem_set_error("sawfem_wdot: invalid arguments");
+    return 1;
+  }
+  if (nb == 0) return 0;
+  hipLaunchKernelGGL(k_wdot, dim3(nb), dim3(WDOT_NT), 0, (hipStream_t)stream, n, (const double2*)a, sa, w,
+                     (const double2*)c, sc, sel, (double2*)out);
   SAW_HIP(hipGetLastError());
   return 0;
 }

@@ -745,16 +745,24 @@ class FrequencySolver:
         self._solve(prec, B, 2, arena, uarena, X, 0, nb)
         Qd = torch.empty(2 * B, dtype=torch.complex128, device=self.dev)
 
+        w3 = torch.empty((3, B), dtype=torch.complex128, device=self.dev)
+        ev = 16 * n                      # bytes per [n] vector
+
+        def wdot(a_ptr, sa, c_ptr, sc, out):
+            # out[b] = sum_i a_b[i] d[i] c_b[i] (one pass over both vectors)
+            _lib.call('sawfem_wdot', s, n, B, a_ptr, sa, d.data_ptr(), c_ptr, sc, None, out.data_ptr())
+
         def measure():
             _lib.call('sawfem_charge', s, n, 2 * B, self.kts_a.data_ptr(), self.kts_z.real, self.kts_z.imag, float(V),
                       X.data_ptr(), Qd.data_ptr())
-            mu = (X[:, 1] * d * rhs).sum(dim=1)
-            return [Qd[0::2].cpu().numpy().copy(), mu.cpu().numpy()]
+            wdot(X.data_ptr() + ev, 2 * n, rhs.data_ptr(), n, w3[0])              # mu = (D lambda)^T b
+            return [Qd[0::2].cpu().numpy().copy(), w3[0].cpu().numpy()]
         def dwr(r):
             # Q - Q* = -lambda^T r_x (forward residual); mu - mu* = -(D x)^T r_z (adjoint residual)
-            eq = (X[:, 1] * d * r[:, 0]).sum(dim=1)
-            em = (X[:, 0] * d * r[:, 1]).sum(dim=1)
-            return [eq.cpu().numpy(), em.cpu().numpy()]
+            wdot(X.data_ptr() + ev, 2 * n, r.data_ptr(), 2 * n, w3[1])
+            wdot(X.data_ptr(), 2 * n, r.data_ptr() + ev, 2 * n, w3[2])
+            e = w3[1:].cpu().numpy()
+            return [e[0], e[1]]
         res = self._residual_op(2 * B, om.repeat_interleave(2))
         (Q, mu), (dq, dm), (err, aerr), steps, hist = self._refine(prec, nb, arena, uarena, X, 0, res, res.to_t(rhs2),
                                                                   None, measure, ir_tol, max_ir, dwr=dwr)

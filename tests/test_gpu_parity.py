@@ -397,3 +397,29 @@ def test_grouped_residual_matches_the_row_residual(transposed, B):
     torch.cuda.synchronize()
     assert torch.equal(r1[:, :B], r2[:, :B])
     assert torch.equal(r1, r2)        # the padded columns of the tile too (omega 0 there)
+
+
+def test_weighted_dot_matches_torch():
+    """sawfem_wdot (the refinement's duality products in one pass) against torch's product and sum,
+    strided rows of a [B][2][n] array, with and without a selection and weights."""
+    _say('[test_weighted_dot_matches_torch]')
+    import torch
+    from sawsim_amd import _lib
+    from sawsim_amd.piezo_fem import stream_ptr
+    g = torch.Generator(device='cpu').manual_seed(5)
+    B, n = 9, 50_003
+    X = torch.complex(torch.randn(B, 2, n, generator=g, dtype=torch.float64),
+                      torch.randn(B, 2, n, generator=g, dtype=torch.float64)).cuda()
+    c = torch.complex(torch.randn(B, n, generator=g, dtype=torch.float64),
+                      torch.randn(B, n, generator=g, dtype=torch.float64)).cuda()
+    w = torch.rand(n, generator=g, dtype=torch.float64).cuda() + 0.5
+    out = torch.empty(B, dtype=torch.complex128, device='cuda')
+    s = stream_ptr()
+    _lib.call('sawfem_wdot', s, n, B, X.data_ptr() + 16 * n, 2 * n, w.data_ptr(), c.data_ptr(), n, None, out.data_ptr())
+    ref = (X[:, 1] * w * c).sum(dim=1)
+    assert ((out - ref).abs().max() / ref.abs().max()).item() <= 1e-13
+    sel = torch.tensor([7, 2, 4], dtype=torch.int32, device='cuda')
+    _lib.call('sawfem_wdot', s, n, 3, X.data_ptr(), 2 * n, None, X.data_ptr() + 16 * n, 2 * n, sel.data_ptr(),
+              out.data_ptr())
+    ref = (X[:, 0] * X[:, 1]).sum(dim=1)[sel.long()]
+    assert ((out[:3] - ref).abs().max() / ref.abs().max()).item() <= 1e-13
