@@ -322,7 +322,7 @@ int sawfem_residual_t(void* stream, int n, int B, int Bp, const int64_t* indptr,
  * are grows[gptr[g] .. gptr[g + 1]), at most 4, with identical column lists, e.g. the active
  * unknowns of one merged mesh node; every row in exactly one group).  One wave per group and 64
  * frequencies reads each x row once for the group's rows; per row the result equals
- * sawfem_residual_t's bit for bit.  Replaces the same residual as sawfem_residual (the FP64
+ * sawfem_residual_t's bit for bit.  rhst may be NULL (then rt = -A x).  Replaces the same residual as sawfem_residual (the FP64
  * A(w) x of SawSim saw2d/sweep.py:335-364's solve, here the refinement residual). */
 int sawfem_residual_sv(void* stream, int ng, int B, int Bp, const int* gptr, const int* grows, const int64_t* indptr,
                        const int* indices, const int64_t* tslot, const void* Kaa, const void* Maa, const void* Daa,
@@ -330,6 +330,12 @@ int sawfem_residual_sv(void* stream, int ng, int B, int Bp, const int* gptr, con
 
 /* out (C rows, leading dimension ld_out) = transpose of in (R rows, leading dimension ld_in). */
 int sawfem_transpose(void* stream, int R, int C, int ld_in, int ld_out, const void* in, void* out);
+/* Mapped transpose: for r < R, c < C, out[omap[c] * ld_out + r] = add[omap[c] * ld_out + r] +
+ * in[imap[r] * ld_in + c] (imap, omap: device int row maps, NULL for the identity; add: NULL for
+ * none).  Gathers the active vectors of a refinement step into the frequency-major layout of
+ * sawfem_residual_sv and scatters rhs - A x back to their rows.  No counterpart in the reference. */
+int sawfem_transpose_map(void* stream, int R, int C, int ld_in, int ld_out, const void* in, void* out, const int* imap,
+                         const int* omap, const void* add);
 
 /* Q[b] = sum_i kts_a[i] x[b][i] + V kts_z  (sweep.py:356-362). */
 int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, double kz_im, double V,

@@ -1915,10 +1915,36 @@ __global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, cons
       for (int f = 0; f < FT; ++f) {
         if (f < ft) {
           const int64_t o = i * Bp + (t0 + f) * 64 + lane;
-          const double2 r = rhst[o];
+          const double2 r = rhst ? rhst[o] : make_double2(0, 0);
           rt[o] = make_double2(r.x - acc[t][f].x, r.y - acc[t][f].y);
         }
       }
+    }
+  }
+}
+
+// Mapped transpose (tiled through LDS): out row omap[c] (or c), column r =
+// add[omap[c]][r] (or 0) + in[imap[r]][c] (input row imap[r], or r), for r < R, c < C.  Gathers the
+// refinement's active vectors into the frequency-major layout and scatters the residuals back.
+__global__ void __launch_bounds__(256) k_transpose_map(int R, int C, int ld_in, int ld_out,
+                                                       const double2* __restrict__ in, double2* __restrict__ out,
+                                                       const int* __restrict__ imap, const int* __restrict__ omap,
+                                                       const double2* __restrict__ add) {
+  __shared__ double2 tile[32][33];
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + tx;
+    if (r < R && c < C) tile[k][tx] = in[(int64_t)(imap ? imap[r] : r) * ld_in + c];
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + k, r = r0 + tx;
+    if (r < R && c < C) {
+      const int64_t o = (int64_t)(omap ? omap[c] : c) * ld_out + r;
+      double2 v = tile[tx][k];
+      if (add) v = d2add(v, add[o]);
+      out[o] = v;
     }
   }
 }
@@ -3408,6 +3434,19 @@ int sawfem_transpose(void* stream, int R, int C, int ld_in, int ld_out, const vo
   if (R == 0 || C == 0) return 0;
   hipLaunchKernelGGL(k_transpose, dim3((C + 31) / 32, (R + 31) / 32), dim3(256), 0, (hipStream_t)stream, R, C, ld_in,
                      ld_out, (const double2*)in, (double2*)out);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_transpose_map(void* stream, int R, int C, int ld_in, int ld_out, const void* in, void* out, const int* imap,
+                         const int* omap, const void* add) {
+  if (R < 0 || C < 0 || (!imap && ld_in < C) || (!omap && ld_out < R) || !in || !out) {
+    sawfem_set_error("sawfem_transpose_map: invalid arguments");
+    return 1;
+  }
+  if (R == 0 || C == 0) return 0;
+  hipLaunchKernelGGL(k_transpose_map, dim3((C + 31) / 32, (R + 31) / 32), dim3(256), 0, (hipStream_t)stream, R, C,
+                     ld_in, ld_out, (const double2*)in, (double2*)out, imap, omap, (const double2*)add);
   SAW_HIP(hipGetLastError());
   return 0;
 }

@@ -138,19 +138,48 @@ class _ResidualOp:
         self.xt = torch.empty((n, self.Bp), dtype=torch.complex128, device=solver.dev)
         self.rt = torch.empty((n, self.Bp), dtype=torch.complex128, device=solver.dev)
 
+    # below this fraction of active vectors a step gathers them (compact residual)
+    COMPACT = 0.9
+
     def to_t(self, v):
+        """The right-hand sides ``v`` ([B][n]) transposed for the residual kernel; ``v`` itself is
+        kept for the compact steps, which add it while scattering back."""
         sv = self.solver
         n = sv.n_active
+        self.rhs = v
         out = sv.torch.empty((n, self.Bp), dtype=sv.torch.complex128, device=sv.dev)
         _lib.call('sawfem_transpose', stream_ptr(), self.B, n, n, self.Bp, v.data_ptr(), out.data_ptr())
         return out
 
-    def __call__(self, x, rhst, r, tslot):
+    def __call__(self, x, rhst, r, tslot, active=None, per=1):
+        """r = rhs - A x.  ``active`` (bool per frequency, ``per`` vectors each): when fewer than
+        COMPACT of them are active, only their vectors are gathered, evaluated and scattered back
+        (the rows of the other vectors of r are left as they are)."""
         sv = self.solver
         n, B, Bp = sv.n_active, self.B, self.Bp
         s = stream_ptr()
-        _lib.call('sawfem_transpose', s, B, n, n, Bp, x.data_ptr(), self.xt.data_ptr())
         gptr, grows, ng = sv.res_groups
+        if active is not None and active.sum() * per < self.COMPACT * B:
+            torch = sv.torch
+            vec = (np.flatnonzero(active)[:, None] * per + np.arange(per)).ravel().astype(np.int32)
+            nsel = int(vec.size)
+            if nsel == 0:
+                return
+            sel = torch.as_tensor(vec, device=sv.dev)
+            Bc = -(-nsel // 64) * 64
+            xt = self.xt.view(-1)[:n * Bc].view(n, Bc)
+            rt = self.rt.view(-1)[:n * Bc].view(n, Bc)
+            om = self.om[sel.long()].contiguous()
+            _lib.call('sawfem_transpose_map', s, nsel, n, n, Bc, x.data_ptr(), xt.data_ptr(), sel.data_ptr(), None,
+                      None)
+            _lib.call('sawfem_residual_sv', s, ng, nsel, Bc, gptr.data_ptr(), grows.data_ptr(),
+                      sv.aa_indptr.data_ptr(), sv.aa_indices.data_ptr(), sv._dptr(tslot), sv.Kaa.data_ptr(),
+                      sv.Maa.data_ptr(), sv._dptr(sv.Daa), om.data_ptr(), sv.beta, xt.data_ptr(), None,
+                      rt.data_ptr())
+            _lib.call('sawfem_transpose_map', s, n, nsel, Bc, n, rt.data_ptr(), r.data_ptr(), None, sel.data_ptr(),
+                      self.rhs.data_ptr())
+            return
+        _lib.call('sawfem_transpose', s, B, n, n, Bp, x.data_ptr(), self.xt.data_ptr())
         _lib.call('sawfem_residual_sv', s, ng, B, Bp, gptr.data_ptr(), grows.data_ptr(), sv.aa_indptr.data_ptr(),
                   sv.aa_indices.data_ptr(), sv._dptr(tslot), sv.Kaa.data_ptr(), sv.Maa.data_ptr(),
                   sv._dptr(sv.Daa), self.om.data_ptr(), sv.beta, self.xt.data_ptr(), rhst.data_ptr(),
@@ -670,7 +699,7 @@ class FrequencySolver:
         self._solved = [B]                 # frequencies solved per pass (first solve, then each step)
         self._stalled = np.zeros(B, dtype=bool)
         for steps in range(1, max_ir + 1):
-            res(X, rhst, r, tslot)
+            res(X, rhst, r, tslot, active, R)
             if dwr is not None:
                 eta = [np.abs(a) / np.maximum(np.abs(v), 1e-300) for a, v in zip(dwr(r), vals)]
                 ok = active & np.all([e <= ir_tol for e in eta], axis=0)

@@ -423,3 +423,35 @@ def test_weighted_dot_matches_torch():
               out.data_ptr())
     ref = (X[:, 0] * X[:, 1]).sum(dim=1)[sel.long()]
     assert ((out[:3] - ref).abs().max() / ref.abs().max()).item() <= 1e-13
+
+
+@pytest.mark.parametrize('transposed', [False, True])
+def test_compact_residual_matches_the_full_residual(transposed):
+    """A refinement step with few active frequencies gathers their vectors (sawfem_transpose_map),
+    evaluates rhs - A x for them only and scatters back: the active rows equal the full residual's
+    bit for bit, the other rows are left untouched."""
+    _say(f'[test_compact_residual_matches_the_full_residual] transposed={transposed}')
+    import torch
+    from sawsim_amd.sweep import _ResidualOp
+    cell = workloads.config3(mesh_um=0.12, points=4)
+    eng = workloads.build_engine(cell, gradient=transposed)
+    n, Bf, per = eng.n_active, 37, 2
+    om = torch.as_tensor(2 * np.pi * np.linspace(1.35e9, 1.5e9, Bf), device='cuda').repeat_interleave(per)
+    g = torch.Generator(device='cpu').manual_seed(3)
+    X = torch.complex(torch.randn(Bf, per, n, generator=g, dtype=torch.float64),
+                      torch.randn(Bf, per, n, generator=g, dtype=torch.float64)).cuda()
+    rhs = torch.complex(torch.randn(Bf * per, n, generator=g, dtype=torch.float64),
+                        torch.randn(Bf * per, n, generator=g, dtype=torch.float64)).cuda()
+    ts = eng.tslot if transposed else None
+    res = _ResidualOp(eng, Bf * per, om)
+    rhst = res.to_t(rhs)
+    full = torch.empty_like(X)
+    res(X, rhst, full, ts)
+    active = np.zeros(Bf, dtype=bool)
+    active[[0, 5, 6, 20, 36]] = True
+    part = torch.full_like(X, 7.0)
+    res(X, rhst, part, ts, active, per)
+    torch.cuda.synchronize()
+    a = torch.as_tensor(active, device='cuda')
+    assert torch.equal(part[a], full[a])
+    assert bool((part[~a] == 7.0).all())
