@@ -8,9 +8,10 @@ are done once per cell before the timed region (inputs resident in HBM).
 
 Printed JSON line: metric/value (whole-job frequency points per second), the plain FP64
 GPU path (one frequency per launch) measured on a sample of points, the Y11 error of the
-mixed path against the FP64 path, a roofline entry for the dominant kernel (the MFMA
-update GEMM, timed live with HIP events) and a CPU baseline (the SuperLU oracle on a
-bounded sample of the same workload: one frequency per worker process).
+mixed path against the FP64 path, a roofline entry for the dominant kernel (the grouped
+block-solve kernel against HBM, timed live with HIP events), one for the MFMA update GEMM
+(`mfma_roofline`) and a CPU baseline (the SuperLU oracle on a bounded sample of the same
+workload: one frequency per worker process).
 """
 from __future__ import annotations
 
@@ -38,6 +39,28 @@ PEAK_FP64_MFMA = 78.6
 # from those summaries, and the kernel it describes has not changed since they were taken.
 SCHUR_TRAFFIC_BYTES = {'config2': 1.823e10, 'config4': 1.620e10}
 SCHUR_TRAFFIC_SOURCE = {'config2': 'profiles/config2_schur_traffic.md', 'config4': 'profiles/config4_schur_traffic.md'}
+# HBM bytes per launch of the grouped solve kernel k_grp_lds (whole fronts), measured the same way
+GRP_TRAFFIC_BYTES = {}
+GRP_TRAFFIC_SOURCE = {}
+
+
+def grp_roofline(grp, workload):
+    """Roofline entry of the grouped block-solve kernel k_grp_lds (every block size and kind on the
+    whole fronts of a grouped solve), the sweep's largest kernel by time: algorithmic bytes (factor
+    entries read once per block of vectors, vectors gathered and written; counted by the library per
+    launch) over its HIP-event time in the timed region, against HBM."""
+    if not grp['launches']:
+        return None
+    gbs = grp['bytes'] / (grp['ms'] * 1e-3) / 1e9
+    return dict(bound='hbm', achieved=gbs, peak=8000.0, unit='GB/s', frac=gbs / 8000.0,
+                traffic=GRP_TRAFFIC_BYTES.get(workload),
+                traffic_note=('HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy, '
+                              f'from {GRP_TRAFFIC_SOURCE[workload]}') if workload in GRP_TRAFFIC_SOURCE else None,
+                kernel='k_grp_lds<kind, 8|16, false> (grouped whole-front solves, complex64 factors, FP64 vectors)',
+                launches=grp['launches'], kernel_ms=grp['ms'], algorithmic_bytes=grp['bytes'],
+                algorithmic_bytes_per_launch=grp['bytes'] / grp['launches'],
+                avg_launch_ms=grp['ms'] / grp['launches'],
+                note='latency-bound: 62 % of wave cycles waiting, VALU 15 % busy (rocprofv3 counters)')
 
 
 def log(*a):
@@ -464,6 +487,10 @@ def main():
         Q, dY, info = sweep()
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    import ctypes as C
+    g_ms, g_bytes, g_n = C.c_double(), C.c_double(), C.c_int64()
+    _lib.call('sawfem_mf_profile_grp', eng._plan, C.byref(g_ms), C.byref(g_bytes), C.byref(g_n))
+    grp = dict(ms=g_ms.value, bytes=g_bytes.value, launches=g_n.value)
     prof, prof_levels = profile_read(eng)
     _lib.call('sawfem_mf_profile', eng._plan, 0)
     value = args.steps * P / dt
@@ -572,7 +599,8 @@ def main():
         speedup_vs_plain_fp64=dict(central_fd=value / plain_fd, fp64_adjoint=value / plain_adj,
                                    y11_only=value / plain) if pp > 0 else None,
         ir=info,
-        roofline=dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
+        roofline=grp_roofline(grp, args.workload),
+        mfma_roofline=dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
                       frac=(achieved / PEAK_FP32_MFMA) if achieved else None,
                       traffic=SCHUR_TRAFFIC_BYTES.get(args.workload),
                       traffic_note='HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy, '

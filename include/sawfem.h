@@ -301,6 +301,11 @@ int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count);
 /* The same totals per level of the assembly tree (leaves first), ms[level * 9 + kind], from the
  * last profile_read; returns the number of levels written (at most max_levels) or -1. */
 int sawfem_mf_profile_levels(void* plan, double* ms, int max_levels);
+/* The grouped block-solve kernel (k_grp_lds on whole fronts, every block size and kind) since
+ * profiling was turned on: summed HIP-event time of its launches (ms), their algorithmic bytes
+ * (factor entries read once per block, vectors gathered and written) and the launch count.  Read
+ * it before sawfem_mf_profile_read, which releases the events.  No counterpart in the reference. */
+int sawfem_mf_profile_grp(void* plan, double* ms, double* bytes, int64_t* launches);
 
 /* out[b][i] = -V (rK[i] + i w_b beta rD[i] - w_b^2 rM[i])  (sweep.py:338-342). */
 int sawfem_rhs(void* stream, int n, int B, const double* omega, double beta, double V, const void* rK,

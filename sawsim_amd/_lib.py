@@ -60,6 +60,7 @@ SIGNATURES = {
     'sawfem_residual': [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
     'sawfem_residual_t': [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
     'sawfem_transpose_map': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
+    'sawfem_mf_profile_grp': [_vp, _vp, _vp, _vp],
     'sawfem_wdot': [_vp, _i, _i, _vp, _i64, _vp, _vp, _i64, _vp, _vp],
     'sawfem_residual_sv': [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
     'sawfem_transpose': [_vp, _i, _i, _i, _i, _vp, _vp],

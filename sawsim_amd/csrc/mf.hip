@@ -1500,8 +1500,10 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
   const int mr = W * RC;
   batched<8>(t, mr, 256,
              [&](int idx) {
+               // kind 0 uses x at the pivot rows only: the other lanes re-read the last pivot row
+               // (a line the wave loads anyway) instead of a contribution-block row
                const int r = idx / W, i = idx - r * W;
-               return x[(v0 + r) * n + rows[i]];
+               return x[(v0 + r) * n + rows[KIND == 0 ? min(i, p - 1) : i]];
              },
              [&](int idx, double2 v) {
                const int r = idx / W, i = idx - r * W;
@@ -2137,6 +2139,10 @@ struct Plan {
   hipEvent_t fork_ev = nullptr, join_ev[NSIDE] = {};
   std::vector<int> hm, hp, hlvl_ptr, hlvl_fronts;
   std::vector<int> hchild_ptr, hent_loc;       // host copies for the leaf-front work lists
+  std::vector<int> hchild_cb;                  // per front: sum of its children's contribution-block rows
+  // k_grp_lds launches under profiling: HIP-event pair and algorithmic bytes of each
+  struct GrpMark { size_t i0; double bytes; };
+  std::vector<GrpMark> grp_marks;
   std::vector<int64_t> hfent_lo, hfent_hi;
   // captured factorizations (sawfem_mf_factor_graph): the launch sequence of one argument set,
   // replayed by hipGraphLaunch; dropped when a kernel-selection knob changes
@@ -2694,6 +2700,26 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
     return sizeof(T) == 4 && C.G > 0 && !transpose && C.rc_max >= 4 && (C.G * C.R) % 4 == 0 &&
            grp_lds_of(S, 4) <= (size_t)std::min(P->grp_lds_bytes, P->lds_max);
   };
+  // algorithmic bytes of one block of rc vectors over the fronts of S (k_grp_lds, whole fronts):
+  // the factor entries the kind reads once (kind 0: the unit-lower triangle's stored inverse blocks
+  // below their diagonals and the column panels below them; kind 1: U12, the diagonal blocks on and
+  // above their diagonals and the column panels above them) at 8 B, the vectors gathered and
+  // written at 16 B per entry (kind 0: the pivot rows in and out, the children's update vectors
+  // in, the front's update vector out; kind 1: all m rows in, the pivot rows out) and the rows list
+  auto grp_bytes = [&](const Sub& S, int kind, int rc, int nbl) {
+    double b = 0;
+    for (int f : S.hfronts) {
+      const double m = P->hm[f], p = P->hp[f], q = m - p;
+      double ent = kind == 0 ? 0 : p * q;
+      for (int J0 = 0; J0 < (int)p; J0 += nbl) {
+        const double nbk = std::min<double>(nbl, p - J0);
+        ent += kind == 0 ? nbk * (nbk - 1) / 2 + (m - J0 - nbk) * nbk : nbk * (nbk + 1) / 2 + J0 * nbk;
+      }
+      const double vec = kind == 0 ? p + P->hchild_cb[f] + p + q : m + p;
+      b += 8 * ent + 16.0 * rc * vec + 4 * m;
+    }
+    return b;
+  };
   auto grp_sub = [&](const Level& L, const Sub& S, int kind, hipStream_t st) {
     if (S.nf == 0) return;
     Timed tm(P, st, kind == 0 ? K_FWD : K_BWD);
@@ -2716,6 +2742,22 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
       }
       const size_t lds = grp_lds_of(S, Q.rc);
       const dim3 grid(S.nf, Q.ny);
+      // profiling: an event pair around this launch alone and its algorithmic bytes (the
+      // roofline input of the grouped solve kernel)
+      size_t gi0 = 0;
+      if (P->profile) {
+        while (P->ev_pool.size() < P->ev_used + 2) {
+          hipEvent_t e;
+          if (hipEventCreate(&e) != hipSuccess) break;
+          P->ev_pool.push_back(e);
+        }
+        if (P->ev_pool.size() >= P->ev_used + 2) {
+          gi0 = P->ev_used;
+          P->ev_used += 2;
+          (void)hipEventRecord(P->ev_pool[gi0], st);
+          P->grp_marks.push_back({gi0, (double)Q.ny * grp_bytes(S, kind, Q.rc, nbl)});
+        }
+      }
       if constexpr (sizeof(T) == 4) {
 #define SAW_GRP(K, RCV)                                                                                     \
   do {                                                                                                      \
@@ -2732,6 +2774,8 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
         }
 #undef SAW_GRP
       }
+      if (P->profile && !P->grp_marks.empty() && P->grp_marks.back().i0 == gi0 && gi0 + 1 < P->ev_pool.size())
+        (void)hipEventRecord(P->ev_pool[gi0 + 1], st);
     }
   };
   auto panel_lds_of = [&](int kind, int rc) {
@@ -3034,6 +3078,9 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
     P->hfent_hi = hi;
   }
   P->hchild_ptr.assign(child_ptr, child_ptr + nf + 1);
+  P->hchild_cb.assign(nf, 0);
+  for (int f = 0; f < nf; ++f)
+    for (int c = child_ptr[f]; c < child_ptr[f + 1]; ++c) P->hchild_cb[f] += m[child_ids[c]] - p[child_ids[c]];
   P->hent_loc.assign(ent_loc, ent_loc + nent);
   P->hm.assign(m, m + nf);
   P->hp.assign(p, p + nf);
@@ -3339,6 +3386,27 @@ int sawfem_mf_profile(void* plan, int enable) {
   P->profile = enable != 0;
   P->ev_used = 0;
   P->ev_marks.clear();
+  P->grp_marks.clear();
+  return 0;
+}
+
+// The grouped block-solve kernel (k_grp_lds on whole fronts) under profiling: summed HIP-event
+// time (ms), algorithmic bytes and launches since profiling was turned on; waits for the events.
+int sawfem_mf_profile_grp(void* plan, double* ms, double* bytes, int64_t* launches) {
+  Plan* P = (Plan*)plan;
+  if (!P || !ms || !bytes || !launches) { sawfem_set_error("sawfem_mf_profile_grp: invalid arguments"); return 1; }
+  *ms = 0;
+  *bytes = 0;
+  *launches = 0;
+  for (auto& g : P->grp_marks) {
+    SAW_HIP(hipEventSynchronize(P->ev_pool[g.i0 + 1]));
+    float t = 0;
+    SAW_HIP(hipEventElapsedTime(&t, P->ev_pool[g.i0], P->ev_pool[g.i0 + 1]));
+    *ms += t;
+    *bytes += g.bytes;
+    *launches += 1;
+  }
+  P->grp_marks.clear();
   return 0;
 }
 
@@ -3360,6 +3428,7 @@ int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count) {
   }
   P->ev_used = 0;
   P->ev_marks.clear();
+  P->grp_marks.clear();   // their events are released with the pool
   return 0;
 }
 
