@@ -40,8 +40,9 @@ PEAK_FP64_MFMA = 78.6
 SCHUR_TRAFFIC_BYTES = {'config2': 1.823e10, 'config4': 1.620e10}
 SCHUR_TRAFFIC_SOURCE = {'config2': 'profiles/config2_schur_traffic.md', 'config4': 'profiles/config4_schur_traffic.md'}
 # HBM bytes per launch of the grouped solve kernel k_grp_lds (whole fronts), measured the same way
-GRP_TRAFFIC_BYTES = {}
-GRP_TRAFFIC_SOURCE = {}
+# (FETCH_SIZE + WRITE_SIZE of the four k_grp_lds whole-front instances over their dispatches)
+GRP_TRAFFIC_BYTES = {'config2': 1.931e9}
+GRP_TRAFFIC_SOURCE = {'config2': 'profiles/config2_grouped_solve_traffic.md'}
 
 
 def grp_roofline(grp, workload):
