@@ -402,6 +402,20 @@ int sawfem_sweep_chunk(void* plan, void* stream, const sawfem_operator* op, int 
                        int prec, double ir_tol, int max_ir, int fallback, void* workspace, int64_t workspace_bytes,
                        double* Q, double* err, int* flags);
 
+/* sawfem_sweep_chunk with factorization sharing (complex64): groups of `share` consecutive
+ * frequencies are factored once, at the mean of their angular frequencies, each frequency is
+ * refined against its own FP64 operator with those factors (the same stopping and stall rule),
+ * and the solves take the vectors of a group together (sawfem_mf_solve_groups).  With fallback,
+ * frequencies that do not converge are refined with their own complex64 factors (flags bit 2),
+ * and those that still do not in complex128 (bit 0).  share = 1 is sawfem_sweep_chunk; prec 1
+ * ignores share.  workspace: at least sawfem_sweep_chunk_share_bytes(plan, n, B, prec, share).
+ * Replaces FrequencySolver.solve of saw2d/sweep.py:335-364 for a chunk (the reference factors
+ * every frequency). */
+int64_t sawfem_sweep_chunk_share_bytes(void* plan, int n, int B, int prec, int share);
+int sawfem_sweep_chunk_share(void* plan, void* stream, const sawfem_operator* op, int B, const double* omega,
+                             double V, int prec, int share, double ir_tol, int max_ir, int fallback, void* workspace,
+                             int64_t workspace_bytes, double* Q, double* err, int* flags);
+
 /* sawfem_adjoint_elem without the work argument: allocates nothing.  No counterpart in the
  * reference; for callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  The B * ne0 * NT complex128 element sums live

@@ -70,11 +70,13 @@ SIGNATURES = {
                             _vp, _vp, _vp, _vp],
     'sawfem_sweep_chunk_bytes': [_vp, _i, _i, _i],
     'sawfem_sweep_chunk': [_vp, _vp, _vp, _i, _vp, _d, _i, _d, _i, _i, _vp, _i64, _vp, _vp, _vp],
+    'sawfem_sweep_chunk_share_bytes': [_vp, _i, _i, _i, _i],
+    'sawfem_sweep_chunk_share': [_vp, _vp, _vp, _i, _vp, _d, _i, _i, _d, _i, _i, _vp, _i64, _vp, _vp, _vp],
     'sawfem_adjoint_elem_noalloc': [_vp, _i, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _i, _vp, _vp, _d, _d, _i,
                                     _i, _vp, _vp, _vp],
 }
 _RESTYPE = {'sawfem_last_error': C.c_char_p, 'sawfem_mf_arena_elems': _i64, 'sawfem_mf_uarena_elems': _i64,
-            'sawfem_sweep_chunk_bytes': _i64, 'sawfem_host_size': _i64, 'sawfem_mf_graph_nodes': _i64}
+            'sawfem_sweep_chunk_bytes': _i64, 'sawfem_sweep_chunk_share_bytes': _i64, 'sawfem_host_size': _i64, 'sawfem_mf_graph_nodes': _i64}
 
 # status codes of the C ABI (include/sawfem.h)
 OK, INVALID, PLAN_LIMIT, HIP_ERROR, OUT_OF_MEMORY, NON_FINITE, REFACTORED = 0, 1, 2, 3, 4, 5, 6

@@ -38,21 +38,29 @@ struct Layout {
   size_t arena, uarena, vec, tvec, qd, om, sel, keep, total;
 };
 
-Layout layout_of(void* plan, int n, int B, int prec) {
+// share: consecutive frequencies per complex64 factorization (1: one each)
+Layout layout_of(void* plan, int n, int B, int prec, int share = 1) {
   Layout L{};
   const int64_t ae = sawfem_mf_arena_elems(plan), ue = sawfem_mf_uarena_elems(plan);
   const int Bp = (B + 63) / 64 * 64;
+  const int Bf = prec == 0 ? (B + share - 1) / share : B;
   // the arena also holds the complex128 refactorization of at least one frequency
-  L.arena = align256(std::max<size_t>((size_t)ae * B * (prec == 0 ? 8 : 16), (size_t)ae * 16));
+  L.arena = align256(std::max<size_t>((size_t)ae * Bf * (prec == 0 ? 8 : 16), (size_t)ae * 16));
   L.uarena = align256((size_t)std::max<int64_t>(1, ue) * B * 16);
   L.vec = align256((size_t)n * B * 16);         // rhs, x, r: [B][n]
   L.tvec = align256((size_t)n * Bp * 16);       // rhst, xt, rt: [n][Bp]
   L.qd = align256((size_t)B * 16);
-  L.om = align256((size_t)B * 8);
+  L.om = align256((size_t)2 * B * 8);          // the frequencies, then the group means
   L.sel = align256((size_t)B * 4);
   L.keep = align256((size_t)B);
   L.total = L.arena + L.uarena + 3 * L.vec + 3 * L.tvec + L.qd + L.om + L.sel + L.keep;
   return L;
+}
+
+size_t hsel_count(const std::vector<unsigned char>& a) {
+  size_t c = 0;
+  for (unsigned char v : a) c += v ? 1 : 0;
+  return c;
 }
 
 double ir_error(double dq, double dq_prev) {
@@ -68,11 +76,15 @@ double ir_error(double dq, double dq_prev) {
 
 // One precision pass over B frequencies (omega on the host).  Q (2B doubles), err (B) and
 // stalled/unconverged flags (B) out.
+// With share G > 1 (complex64): groups of G consecutive frequencies are factored once, at the mean
+// of their angular frequencies, and each frequency is refined against its own FP64 operator with
+// those factors; the solves take the vectors of a group together (sawfem_mf_solve_groups).
 int refine_pass(void* plan, hipStream_t s, const sawfem_operator* op, int B, const double* omega, double V, int prec,
                 double ir_tol, int max_ir, unsigned char* ws, int64_t ws_bytes, double* Q, double* err,
-                unsigned char* bad, int* steps_out) {
+                unsigned char* bad, int* steps_out, int share = 1) {
   const int n = op->n;
-  const Layout L = layout_of(plan, n, B, prec);
+  const int G = prec == 0 ? std::max(1, std::min(share, B)) : 1;
+  const Layout L = layout_of(plan, n, B, prec, G);
   if ((int64_t)L.total > ws_bytes) {
     sawfem_set_error("sawfem_sweep_chunk: workspace smaller than sawfem_sweep_chunk_bytes");
     return 4;
@@ -92,12 +104,48 @@ int refine_pass(void* plan, hipStream_t s, const sawfem_operator* op, int B, con
   unsigned char* keep = p;
   const int Bp = (B + 63) / 64 * 64;
   const int nb = prec == 0 ? 64 : 32;
-  SAW_HIP(hipMemcpyAsync(om, omega, sizeof(double) * B, hipMemcpyHostToDevice, s));
-  SAW_CALL(sawfem_mf_factor(plan, s, prec, B, om, op->beta, op->Kaa, op->Maa, op->Daa, arena, nb));
+  // frequencies, then (shared factors) the group means summed in frequency order
+  const int Bf = (B + G - 1) / G;
+  std::vector<double> hom(omega, omega + B);
+  for (int g = 0; g < Bf && G > 1; ++g) {
+    double sum = 0;
+    int cnt = 0;
+    for (int b = g * G; b < std::min(B, (g + 1) * G); ++b, ++cnt) sum += omega[b];
+    hom.push_back(sum / cnt);
+  }
+  SAW_HIP(hipMemcpyAsync(om, hom.data(), sizeof(double) * hom.size(), hipMemcpyHostToDevice, s));
+  const double* omf = G > 1 ? om + B : om;
+  SAW_CALL(sawfem_mf_factor(plan, s, prec, Bf, omf, op->beta, op->Kaa, op->Maa, op->Daa, arena, nb));
   SAW_CALL(sawfem_rhs(s, n, B, om, op->beta, V, op->rK, op->rM, op->rD, rhs));
   hipLaunchKernelGGL(k_copy, dim3((unsigned)(((int64_t)n * B + 255) / 256)), dim3(256), 0, s, (int64_t)n * B, rhs, x);
   SAW_HIP(hipGetLastError());
-  SAW_CALL(sawfem_mf_solve(plan, s, prec, B, 1, arena, uarena, x, 0, nb, nullptr, 0));
+  // solve of the selected frequencies (G > 1: every group with one of them; the others' right-hand
+  // sides are zero)
+  std::vector<int> hgsel;
+  auto solve = [&](double2* v, const std::vector<unsigned char>* active) -> int {
+    if (G == 1) {
+      if (!active) return sawfem_mf_solve(plan, s, prec, B, 1, arena, uarena, v, 0, nb, nullptr, 0);
+      return sawfem_mf_solve(plan, s, prec, B, 1, arena, uarena, v, 0, nb, sel, (int)hsel_count(*active));
+    }
+    const int nfull = B / G;
+    int tail = B % G ? 1 : 0;
+    if (!active) return sawfem_mf_solve_groups(plan, s, B, 1, arena, G, uarena, v, 0, nb, nullptr, 0, tail);
+    hgsel.clear();
+    for (int g = 0; g < nfull; ++g) {
+      bool any = false;
+      for (int b = g * G; b < (g + 1) * G; ++b) any = any || (*active)[b];
+      if (any) hgsel.push_back(g);
+    }
+    bool tany = false;
+    for (int b = nfull * G; b < B; ++b) tany = tany || (*active)[b];
+    tail = tany ? 1 : 0;
+    if (!hgsel.empty())
+      SAW_HIP(hipMemcpyAsync(sel, hgsel.data(), sizeof(int) * hgsel.size(), hipMemcpyHostToDevice, s));
+    if (hgsel.empty() && !tail) return 0;
+    return sawfem_mf_solve_groups(plan, s, B, 1, arena, G, uarena, v, 0, nb, hgsel.empty() ? nullptr : sel,
+                                  (int)hgsel.size(), tail);
+  };
+  SAW_CALL(solve(x, nullptr));
   SAW_CALL(sawfem_transpose(s, B, n, n, Bp, rhs, rhst));
   std::vector<double> q(2 * B), qn(2 * B), dqp(B, INFINITY), e(B, INFINITY);
   std::vector<unsigned char> act(B, 1), stalled(B, 0);
@@ -121,11 +169,11 @@ int refine_pass(void* plan, hipStream_t s, const sawfem_operator* op, int B, con
     const bool all = (int)hsel.size() == B;
     if (!all) {
       SAW_HIP(hipMemcpyAsync(keep, act.data(), B, hipMemcpyHostToDevice, s));
-      SAW_HIP(hipMemcpyAsync(sel, hsel.data(), sizeof(int) * hsel.size(), hipMemcpyHostToDevice, s));
+      if (G == 1) SAW_HIP(hipMemcpyAsync(sel, hsel.data(), sizeof(int) * hsel.size(), hipMemcpyHostToDevice, s));
       hipLaunchKernelGGL(k_zero_rows, dim3((n + 255) / 256, B), dim3(256), 0, s, n, B, keep, r);
       SAW_HIP(hipGetLastError());
     }
-    SAW_CALL(sawfem_mf_solve(plan, s, prec, B, 1, arena, uarena, r, 0, nb, all ? nullptr : sel, (int)hsel.size()));
+    SAW_CALL(solve(r, all ? nullptr : &act));
     SAW_CALL(sawfem_axpy(s, (int64_t)n * B, x, r));
     SAW_CALL(charge(qn.data()));
     bool any = false;
@@ -164,12 +212,24 @@ int64_t sawfem_sweep_chunk_bytes(void* plan, int n, int B, int prec) {
   return (int64_t)layout_of(plan, n, B, prec).total;
 }
 
+int64_t sawfem_sweep_chunk_share_bytes(void* plan, int n, int B, int prec, int share) {
+  if (!plan || n <= 0 || B <= 0 || (prec != 0 && prec != 1) || share < 1) return -1;
+  return (int64_t)layout_of(plan, n, B, prec, prec == 0 ? std::min(share, B) : 1).total;
+}
+
 int sawfem_sweep_chunk(void* plan, void* stream, const sawfem_operator* op, int B, const double* omega, double V,
                        int prec, double ir_tol, int max_ir, int fallback, void* workspace, int64_t workspace_bytes,
                        double* Q, double* err, int* flags) {
+  return sawfem_sweep_chunk_share(plan, stream, op, B, omega, V, prec, 1, ir_tol, max_ir, fallback, workspace,
+                                  workspace_bytes, Q, err, flags);
+}
+
+int sawfem_sweep_chunk_share(void* plan, void* stream, const sawfem_operator* op, int B, const double* omega,
+                             double V, int prec, int share, double ir_tol, int max_ir, int fallback, void* workspace,
+                             int64_t workspace_bytes, double* Q, double* err, int* flags) {
   if (!plan || !op || !omega || !workspace || !Q || !err || !flags || B <= 0 || op->n <= 0 || !op->indptr ||
       !op->indices || !op->Kaa || !op->Maa || !op->rK || !op->rM || !op->kts_a || (prec != 0 && prec != 1) ||
-      max_ir < 1 || !(ir_tol > 0) || !(V > 0)) {
+      max_ir < 1 || !(ir_tol > 0) || !(V > 0) || share < 1) {
     sawfem_set_error("sawfem_sweep_chunk: invalid arguments");
     return 1;
   }
@@ -180,10 +240,35 @@ int sawfem_sweep_chunk(void* plan, void* stream, const sawfem_operator* op, int 
   hipStream_t s = (hipStream_t)stream;
   std::vector<unsigned char> bad(B, 0);
   int steps = 0;
+  const int G = prec == 0 ? std::min(share, B) : 1;
   SAW_CALL(refine_pass(plan, s, op, B, omega, V, prec, ir_tol, max_ir, (unsigned char*)workspace, workspace_bytes, Q,
-                       err, bad.data(), &steps));
+                       err, bad.data(), &steps, G));
   for (int b = 0; b < B; ++b) flags[b] = bad[b] ? 2 : 0;       // bit 1: not converged in this precision
   bool refactored = false;
+  if (prec == 0 && fallback && G > 1) {
+    // shared factors that did not converge: the frequency's own complex64 factors first
+    std::vector<int> todo;
+    for (int b = 0; b < B; ++b)
+      if (bad[b]) todo.push_back(b);
+    int nfit = 0;
+    while (nfit < (int)todo.size() && sawfem_sweep_chunk_bytes(plan, op->n, nfit + 1, 0) <= workspace_bytes) ++nfit;
+    for (size_t c0 = 0; nfit > 0 && c0 < todo.size(); c0 += nfit) {
+      const int k = (int)std::min<size_t>(nfit, todo.size() - c0);
+      std::vector<double> om(k), q(2 * k), e(k);
+      std::vector<unsigned char> bb(k);
+      for (int j = 0; j < k; ++j) om[j] = omega[todo[c0 + j]];
+      SAW_CALL(refine_pass(plan, s, op, k, om.data(), V, 0, ir_tol, max_ir, (unsigned char*)workspace,
+                           workspace_bytes, q.data(), e.data(), bb.data(), nullptr, 1));
+      for (int j = 0; j < k; ++j) {
+        const int b = todo[c0 + j];
+        Q[2 * b] = q[2 * j];
+        Q[2 * b + 1] = q[2 * j + 1];
+        err[b] = e[j];
+        bad[b] = bb[j];
+        flags[b] = 4 | (bb[j] ? 2 : 0);                          // bit 2: own complex64 factors
+      }
+    }
+  }
   if (prec == 0 && fallback) {
     std::vector<int> todo;
     for (int b = 0; b < B; ++b)
@@ -208,7 +293,7 @@ int sawfem_sweep_chunk(void* plan, void* stream, const sawfem_operator* op, int 
         Q[2 * b] = q[2 * j];
         Q[2 * b + 1] = q[2 * j + 1];
         err[b] = e[j];
-        flags[b] = 1 | (bb[j] ? 2 : 0);                          // bit 0: refactored in complex128
+        flags[b] = (flags[b] & 4) | 1 | (bb[j] ? 2 : 0);         // bit 0: refactored in complex128
       }
     }
     refactored = !todo.empty();

@@ -469,11 +469,12 @@ class FrequencySolver:
         """``nbytes`` of the kept workspace (grown like the front arena)."""
         return self._arena(-(-int(nbytes) // 8), self.torch.complex64).view(self.torch.uint8)[:int(nbytes)]
 
-    def sweep_chunk_native(self, omegas, V=1.0, precision='mixed', ir_tol=1e-8, max_ir=8, fallback=True):
-        """One chunk of frequencies through the C ABI alone (``sawfem_sweep_chunk``: factor, solve,
-        refinement and the complex128 refactorization in the library).  Returns (Q complex (B,),
-        estimated relative error (B,), flags (B,): bit 0 refactored in complex128, bit 1 not
-        converged, status: 0, 5 non-finite charge or 6 some frequencies refactored)."""
+    def sweep_chunk_native(self, omegas, V=1.0, precision='mixed', ir_tol=1e-8, max_ir=8, fallback=True, share=1):
+        """One chunk of frequencies through the C ABI alone (``sawfem_sweep_chunk_share``: factor,
+        solve, refinement and the fallbacks in the library; ``share`` consecutive frequencies per
+        complex64 factorization).  Returns (Q complex (B,), estimated relative error (B,), flags
+        (B,): bit 0 refactored in complex128, bit 1 not converged, bit 2 own complex64 factors after
+        sharing, status: 0, 5 non-finite charge or 6 some frequencies refactored)."""
         import ctypes as C
         om = np.ascontiguousarray(np.asarray(omegas, dtype=np.float64).ravel())
         B = om.size
@@ -486,16 +487,16 @@ class FrequencySolver:
                                 self.Kaa.data_ptr(), self.Maa.data_ptr(), self._dptr(self.Daa), self.rK.data_ptr(),
                                 self.rM.data_ptr(), self._dptr(self.rD), self.kts_a.data_ptr(), self.kts_z.real,
                                 self.kts_z.imag, self.beta)
-        nbytes = _lib.load().sawfem_sweep_chunk_bytes(self._plan, self.n_active, B, prec)
+        nbytes = _lib.load().sawfem_sweep_chunk_share_bytes(self._plan, self.n_active, B, prec, int(share))
         if nbytes < 0:
-            raise _lib.SawfemError('sawfem_sweep_chunk_bytes failed')
+            raise _lib.SawfemError('sawfem_sweep_chunk_share_bytes failed')
         ws = self._workspace(nbytes)
         Q = np.empty(2 * B)
         err = np.empty(B)
         flags = np.zeros(B, dtype=np.int32)
-        rc = _lib.raw('sawfem_sweep_chunk', self._plan, stream_ptr(), C.byref(op), B, om.ctypes.data, float(V), prec,
-                      float(ir_tol), int(max_ir), int(bool(fallback)), ws.data_ptr(), int(nbytes), Q.ctypes.data,
-                      err.ctypes.data, flags.ctypes.data)
+        rc = _lib.raw('sawfem_sweep_chunk_share', self._plan, stream_ptr(), C.byref(op), B, om.ctypes.data, float(V),
+                      prec, int(share), float(ir_tol), int(max_ir), int(bool(fallback)), ws.data_ptr(), int(nbytes),
+                      Q.ctypes.data, err.ctypes.data, flags.ctypes.data)
         if rc not in (_lib.OK, _lib.NON_FINITE, _lib.REFACTORED):
             msg = _lib.load().sawfem_last_error().decode(errors='replace')
             raise _lib.SawfemError(f'sawfem_sweep_chunk failed with status {rc}: {msg}')

@@ -61,3 +61,25 @@ def test_native_chunk_status_codes():
     eng.Kaa[0] = complex(np.nan, 0.0)
     Q, err, flags, rc = eng.sweep_chunk_native(om)
     assert rc == _lib.NON_FINITE and not np.all(np.isfinite(Q))
+
+
+@pytest.mark.parametrize('max_ir', [8, 1])
+def test_native_shared_chunk_equals_solve_batch(max_ir):
+    """sawfem_sweep_chunk_share against solve_batch(share=3) on 7 frequencies spread over the band
+    (groups of 3, 3 and a partial group of 1): Q bit for bit and the same fallbacks (the
+    frequencies far from their group's mean take their own complex64 factors; with one
+    correction allowed they go on to complex128)."""
+    _say(f'[test_native_shared_chunk_equals_solve_batch] max_ir={max_ir}')
+    cell = workloads.config3(mesh_um=0.09, points=7)
+    eng = workloads.build_engine(cell)
+    om = 2 * np.pi * cell.frequencies()
+    ref = eng.solve_batch(om, precision='mixed', share=3, max_ir=max_ir)
+    Q, err, flags, rc = eng.sweep_chunk_native(om, share=3, max_ir=max_ir)
+    _say(rc, flags, ref['factorizations'], ref['own_factors'], ref['refactored'])
+    assert np.array_equal(Q, ref['Q'])
+    assert np.array_equal((flags & 4) != 0, ref['own_factors'])
+    assert np.array_equal((flags & 1) != 0, ref['refactored'])
+    assert ref['factorizations'] == 3 + int(ref['own_factors'].sum())
+    assert rc == (_lib.REFACTORED if ref['refactored'].any() else _lib.OK)
+    if max_ir == 1:
+        assert ref['refactored'].all()
