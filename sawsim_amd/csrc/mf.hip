@@ -1440,38 +1440,64 @@ __device__ __forceinline__ float grp_bcast(float v) {
 // group by grp_bcast, so a batch covers TPR * GRP_NB entries of the row in one round trip.
 // SUB: subtract (update) or add (diagonal-block product).  k0, k1 must be equal for the TPR
 // threads of a row.
+// One batch of it: this thread's GRP_NB entries of [kb, kb + TPR GRP_NB) (clamped to k1 - 1;
+// kb < k1), then their use.
+template <int TPR>
+__device__ __forceinline__ void grp_load(const float2* __restrict__ col, int64_t m, int kb, int k1, int q,
+                                         float (&ex)[GRP_NB], float (&ey)[GRP_NB]) {
+#pragma unroll
+  for (int u = 0; u < GRP_NB; ++u) {
+    const int kc = min(kb + u * TPR + q, k1 - 1);
+    const float2 e = col[(int64_t)kc * m];
+    ex[u] = e.x;
+    ey[u] = e.y;
+  }
+}
+
+template <int TPR, int VPT, int WS, bool SUB>
+__device__ __forceinline__ void grp_use(const float (&ex)[GRP_NB], const float (&ey)[GRP_NB], int kb, int k1,
+                                        const double2* __restrict__ w, int wrow0, int r0, double2 (&s)[VPT]) {
+#pragma unroll
+  for (int u = 0; u < GRP_NB; ++u) {
+    auto step = [&](auto sc) {
+      constexpr int S = decltype(sc)::value;
+      const int k = kb + u * TPR + S;
+      const double2 a = make_double2((double)grp_bcast<TPR, S>(ex[u]), (double)grp_bcast<TPR, S>(ey[u]));
+      if (k < k1) {
+#pragma unroll
+        for (int v = 0; v < VPT; ++v)
+          s[v] = SUB ? cfms(s[v], a, w[(wrow0 + k) * WS + r0 + v]) : cfma(s[v], a, w[(wrow0 + k) * WS + r0 + v]);
+      }
+    };
+    step(std::integral_constant<int, 0>{});
+    if constexpr (TPR > 1) step(std::integral_constant<int, 1>{});
+    if constexpr (TPR > 2) {
+      step(std::integral_constant<int, 2>{});
+      step(std::integral_constant<int, 3>{});
+    }
+  }
+}
+
 template <int TPR, int VPT, int WS, bool SUB>
 __device__ __forceinline__ void grp_row_product(const float2* __restrict__ col, int64_t m, int k0, int k1, int q,
                                                 const double2* __restrict__ w, int wrow0, int r0, double2 (&s)[VPT]) {
   for (int kb = k0; kb < k1; kb += TPR * GRP_NB) {
     float ex[GRP_NB], ey[GRP_NB];
-#pragma unroll
-    for (int u = 0; u < GRP_NB; ++u) {
-      const int kc = min(kb + u * TPR + q, k1 - 1);
-      const float2 e = col[(int64_t)kc * m];
-      ex[u] = e.x;
-      ey[u] = e.y;
-    }
-#pragma unroll
-    for (int u = 0; u < GRP_NB; ++u) {
-      auto step = [&](auto sc) {
-        constexpr int S = decltype(sc)::value;
-        const int k = kb + u * TPR + S;
-        const double2 a = make_double2((double)grp_bcast<TPR, S>(ex[u]), (double)grp_bcast<TPR, S>(ey[u]));
-        if (k < k1) {
-#pragma unroll
-          for (int v = 0; v < VPT; ++v)
-            s[v] = SUB ? cfms(s[v], a, w[(wrow0 + k) * WS + r0 + v]) : cfma(s[v], a, w[(wrow0 + k) * WS + r0 + v]);
-        }
-      };
-      step(std::integral_constant<int, 0>{});
-      if constexpr (TPR > 1) step(std::integral_constant<int, 1>{});
-      if constexpr (TPR > 2) {
-        step(std::integral_constant<int, 2>{});
-        step(std::integral_constant<int, 3>{});
-      }
-    }
+    grp_load<TPR>(col, m, kb, k1, q, ex, ey);
+    grp_use<TPR, VPT, WS, SUB>(ex, ey, kb, k1, w, wrow0, r0, s);
   }
+}
+
+// grp_row_product whose first batch (at k0) grp_load has already brought into ex, ey: the same
+// operations in the same order.
+template <int TPR, int VPT, int WS, bool SUB>
+__device__ __forceinline__ void grp_row_product_pre(const float2* __restrict__ col, int64_t m, int k0, int k1, int q,
+                                                    const double2* __restrict__ w, int wrow0, int r0,
+                                                    double2 (&s)[VPT], const float (&ex)[GRP_NB],
+                                                    const float (&ey)[GRP_NB]) {
+  if (k0 >= k1) return;
+  grp_use<TPR, VPT, WS, SUB>(ex, ey, k0, k1, w, wrow0, r0, s);
+  grp_row_product<TPR, VPT, WS, SUB>(col, m, k0 + TPR * GRP_NB, k1, q, w, wrow0, r0, s);
 }
 
 // PIV: the pivot rows only (fronts whose off-diagonal panels k_grp_panel handles): w holds the
@@ -1497,20 +1523,71 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
   const int64_t v0 = (int64_t)b * RC;           // first vector of the block
   // vectors, row index fastest over the threads (runs of x rows coalesce); kind 0 keeps zero
   // at the contribution-block rows before the children's updates
+  const int tr = t / TPR, r0 = (t % TPR) * VPT;     // row slot, first vector of the thread
+  const int q4 = t % TPR;                           // thread within the row's group
+  const int nblk = (p + nbl - 1) / nbl;
+  // The factor entries of the first product do not depend on the vectors: this thread's first
+  // batch of them is loaded before the gather, so its round trip overlaps the gather's.  First
+  // product: forward, row tr of the first diagonal block (k in [0, tr)); backward with
+  // contribution-block rows, row tr of U12 (k in [0, q)).
+  const bool cb_first = KIND == 1 && !PIV && q > 0;
+  const int nbf = min(nbl, p);
+  float pex[GRP_NB], pey[GRP_NB];
+  if (KIND == 0 && tr > 0 && tr < nbf) grp_load<TPR>(Fb + tr, m, 0, tr, q4, pex, pey);
+  if (KIND == 1 && cb_first && tr < p) grp_load<TPR>(Fb + tr + (int64_t)p * m, m, 0, q, q4, pex, pey);
   const int mr = W * RC;
-  batched<8>(t, mr, 256,
-             [&](int idx) {
-               // kind 0 uses x at the pivot rows only: the other lanes re-read the last pivot row
-               // (a line the wave loads anyway) instead of a contribution-block row
-               const int r = idx / W, i = idx - r * W;
-               return x[(v0 + r) * n + rows[KIND == 0 ? min(i, p - 1) : i]];
-             },
-             [&](int idx, double2 v) {
-               const int r = idx / W, i = idx - r * W;
-               w[i * WS + r] = (KIND == 1 || i < p) ? v : make_double2(0, 0);
-             });
-  if (KIND == 0) {
-    const int c0 = F.child_ptr[f], nch = F.child_ptr[f + 1] - c0;
+  const int c0 = F.child_ptr[f], nch = KIND == 0 ? F.child_ptr[f + 1] - c0 : 0;
+  int q2[2] = {0, 0};
+  if (nch == 2) {
+    q2[0] = F.m[F.child_ids[c0]] - F.p[F.child_ids[c0]];
+    q2[1] = F.m[F.child_ids[c0 + 1]] - F.p[F.child_ids[c0 + 1]];
+  }
+  if (nch == 2 && q2[0] > 0 && q2[1] > 0) {
+    // two children (nested dissection): x and both update vectors in one pass, the three loads of
+    // an entry in flight together (a row outside a child's update reads that vector's first
+    // entry, a line in use anyway); sums in the order of the general loop below
+    const int* inv0 = F.inv + F.inv_ptr[c0];
+    const int* inv1 = F.inv + F.inv_ptr[c0 + 1];
+    const double2* u0 = uarena + F.uoff[F.child_ids[c0]] * F.nvec + v0 * q2[0];
+    const double2* u1 = uarena + F.uoff[F.child_ids[c0 + 1]] * F.nvec + v0 * q2[1];
+    constexpr int U = 4;
+    for (int base = t; base < mr; base += U * 256) {
+      double2 a[U], b0[U], b1[U];
+      int k0[U], k1[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = min(base + u * 256, mr - 1);
+        const int r = idx / W, i = idx - r * W;
+        k0[u] = inv0[i];
+        k1[u] = inv1[i];
+        a[u] = x[(v0 + r) * n + rows[min(i, p - 1)]];
+        b0[u] = u0[(int64_t)r * q2[0] + max(k0[u], 0)];
+        b1[u] = u1[(int64_t)r * q2[1] + max(k1[u], 0)];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int idx = base + u * 256;
+        if (idx < mr) {
+          const int r = idx / W, i = idx - r * W;
+          double2 v = i < p ? a[u] : make_double2(0, 0);
+          if (k0[u] >= 0) v = d2add(v, b0[u]);
+          if (k1[u] >= 0) v = d2add(v, b1[u]);
+          w[i * WS + r] = v;
+        }
+      }
+    }
+  } else {
+    batched<8>(t, mr, 256,
+               [&](int idx) {
+                 // kind 0 uses x at the pivot rows only: the other lanes re-read the last pivot row
+                 // (a line the wave loads anyway) instead of a contribution-block row
+                 const int r = idx / W, i = idx - r * W;
+                 return x[(v0 + r) * n + rows[KIND == 0 ? min(i, p - 1) : i]];
+               },
+               [&](int idx, double2 v) {
+                 const int r = idx / W, i = idx - r * W;
+                 w[i * WS + r] = (KIND == 1 || i < p) ? v : make_double2(0, 0);
+               });
     for (int c = 0; c < nch; ++c) {
       const int* inv = F.inv + F.inv_ptr[c0 + c];
       const int ch = F.child_ids[c0 + c];
@@ -1529,16 +1606,22 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
     }
   }
   __syncthreads();
-  const int tr = t / TPR, r0 = (t % TPR) * VPT;     // row slot, first vector of the thread
   double2 s[VPT];
-  const int q4 = t % TPR;                           // thread within the row's group
   // s -= sum_{k in [k0, k1)} F[i][c0 + k] w[wrow0 + k] (entries of row i of the column-major front)
   auto row_update = [&](int i, int c0, int k0, int k1, int wrow0) {
     grp_row_product<TPR, VPT, WS, true>(Fb + i + (int64_t)c0 * m, m, k0, k1, q4, w, wrow0, r0, s);
   };
-  if (KIND == 1 && !PIV && q > 0) {
-    // contribution-block rows -> pivot rows: w[i] -= sum_j U[i][p + j] w[p + j]
-    for (int i = tr; i < p; i += RPP) {
+  if (cb_first) {
+    // contribution-block rows -> pivot rows: w[i] -= sum_j U[i][p + j] w[p + j] (row tr from the
+    // batch loaded before the gather)
+    if (tr < p) {
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) s[v] = w[tr * WS + r0 + v];
+      grp_row_product_pre<TPR, VPT, WS, true>(Fb + tr + (int64_t)p * m, m, 0, q, q4, w, p, r0, s, pex, pey);
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) w[tr * WS + r0 + v] = s[v];
+    }
+    for (int i = tr + RPP; i < p; i += RPP) {
 #pragma unroll
       for (int v = 0; v < VPT; ++v) s[v] = w[i * WS + r0 + v];
       row_update(i, p, 0, q, p);
@@ -1547,24 +1630,27 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
     }
     __syncthreads();
   }
-  const int nblk = (p + nbl - 1) / nbl;
+  // diagonal block (nbk <= 64 <= RPP rows: one row per thread slot), from the old values:
+  // s = L_D^{-1} w (forward, unit lower: w_i + sum_{k<i} Li[i][k] w_k) or U_D^{-1} w (backward:
+  // sum_{k>=i} Ui[i][k] w_k) for row slot tr < nbk
+  auto diag = [&](int J0, int nbk, auto pre) {
+    const int i = tr;
+    const float2* col = Fb + (J0 + i) + (int64_t)J0 * m;
+#pragma unroll
+    for (int v = 0; v < VPT; ++v) s[v] = KIND == 0 ? w[(J0 + i) * WS + r0 + v] : make_double2(0, 0);
+    const int k0 = KIND == 0 ? 0 : i, k1 = KIND == 0 ? i : nbk;
+    if constexpr (decltype(pre)::value)
+      grp_row_product_pre<TPR, VPT, WS, false>(col, m, k0, k1, q4, w, J0, r0, s, pex, pey);
+    else
+      grp_row_product<TPR, VPT, WS, false>(col, m, k0, k1, q4, w, J0, r0, s);
+  };
+  // forward: the first block's product from the batch loaded before the gather, ahead of the loop
+  if (KIND == 0 && tr < nbf) diag(0, nbf, std::true_type{});
   for (int bi = 0; bi < nblk; ++bi) {
     const int J0 = KIND == 0 ? bi * nbl : (nblk - 1 - bi) * nbl;
     const int nbk = min(nbl, p - J0);
-    // diagonal block (nbk <= 64 <= RPP rows: one row per thread slot), from the old values
     const bool own = tr < nbk;
-    if (own) {
-      const int i = tr;
-      if (KIND == 0) {          // L_D^{-1} (unit lower): w_i + sum_{k<i} Li[i][k] w_k
-#pragma unroll
-        for (int v = 0; v < VPT; ++v) s[v] = w[(J0 + i) * WS + r0 + v];
-        grp_row_product<TPR, VPT, WS, false>(Fb + (J0 + i) + (int64_t)J0 * m, m, 0, i, q4, w, J0, r0, s);
-      } else {                  // U_D^{-1}: sum_{k>=i} Ui[i][k] w_k
-#pragma unroll
-        for (int v = 0; v < VPT; ++v) s[v] = make_double2(0, 0);
-        grp_row_product<TPR, VPT, WS, false>(Fb + (J0 + i) + (int64_t)J0 * m, m, i, nbk, q4, w, J0, r0, s);
-      }
-    }
+    if (own && (KIND == 1 || bi > 0)) diag(J0, nbk, std::false_type{});
     __syncthreads();
     if (own) {
 #pragma unroll
@@ -1647,7 +1733,27 @@ __global__ void __launch_bounds__(256) k_grp_panel(FrontsDev F, const int* __res
   if (KIND == 0) {
     const int i = p + o0 + o;                                         // front row
     const int cp0 = F.child_ptr[f], nch = F.child_ptr[f + 1] - cp0;
-    for (int c = 0; c < nch; ++c) {
+    if (nch == 2) {
+      // two children: both row maps, then both update vectors, in flight together (the sums in
+      // the order of the general loop below)
+      const int ci0 = F.inv[F.inv_ptr[cp0] + i], ci1 = F.inv[F.inv_ptr[cp0 + 1] + i];
+      const int ch0 = F.child_ids[cp0], ch1 = F.child_ids[cp0 + 1];
+      const int qc0 = F.m[ch0] - F.p[ch0], qc1 = F.m[ch1] - F.p[ch1];
+      const double2* u0 = uarena + F.uoff[ch0] * F.nvec + (v0 + r0) * qc0 + ci0;
+      const double2* u1 = uarena + F.uoff[ch1] * F.nvec + (v0 + r0) * qc1 + ci1;
+      double2 a0[VPT], a1[VPT];
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) {
+        if (ci0 >= 0) a0[v] = u0[(int64_t)v * qc0];
+        if (ci1 >= 0) a1[v] = u1[(int64_t)v * qc1];
+      }
+#pragma unroll
+      for (int v = 0; v < VPT; ++v) {
+        if (ci0 >= 0) s[v] = d2add(s[v], a0[v]);
+        if (ci1 >= 0) s[v] = d2add(s[v], a1[v]);
+      }
+    }
+    for (int c = 0; c < (nch == 2 ? 0 : nch); ++c) {
       const int ci = F.inv[F.inv_ptr[cp0 + c] + i];
       if (ci >= 0) {
         const int ch = F.child_ids[cp0 + c];
@@ -2279,13 +2385,20 @@ int build_partition(Plan* P, int pi, int max_m) {
       }
     };
     // size buckets by front order (edges roughly 1.7x apart): fronts of one bucket need grids
-    // of similar size, so few workgroups of a launch find no tile
-    auto buckets = [&](std::vector<int> v, std::vector<Sub>& out, bool factor) -> int {
+    // of similar size, so few workgroups of a launch find no tile.  The one-workgroup solves
+    // reserve LDS for their bucket's largest front (16 vectors: 272 B a row), so up to 180 (the
+    // largest 16-vector block within 48 KB) their edges follow the LDS occupancy steps of a
+    // 160 KB CU: 7, 6, 5, 4, 3 workgroups.
+    auto buckets = [&](std::vector<int> v, std::vector<Sub>& out, bool factor, bool solve1 = false) -> int {
       out.clear();
       std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return P->hm[a] < P->hm[b]; });
-      static const int edges[] = {64, 128, 192, 320, 576, 1088, 1 << 30};
+      static const int edges_f[] = {64, 128, 192, 320, 576, 1088, 1 << 30};
+      static const int edges_s[] = {64, 84, 98, 117, 147, 180, 320, 576, 1088, 1 << 30};
+      const int* e0 = solve1 ? edges_s : edges_f;
+      const int ne = solve1 ? (int)(sizeof(edges_s) / sizeof(int)) : (int)(sizeof(edges_f) / sizeof(int));
       size_t i = 0;
-      for (int e : edges) {
+      for (int ei = 0; ei < ne; ++ei) {
+        const int e = e0[ei];
         std::vector<int> part;
         while (i < v.size() && P->hm[v[i]] <= e) part.push_back(v[i++]);
         if (part.empty()) continue;
@@ -2349,7 +2462,7 @@ int build_partition(Plan* P, int pi, int max_m) {
       return 0;
     };
     stats(sm, L.small[pi]);
-    if (put(sm, &L.small[pi].fronts) || buckets(ssm, L.ssmall[pi], false) || buckets(bg, L.big[pi], true) ||
+    if (put(sm, &L.small[pi].fronts) || buckets(ssm, L.ssmall[pi], false, true) || buckets(bg, L.big[pi], true) ||
         buckets(sbg, L.sbig[pi], false))
       return 3;
     for (int f : sm) flag[f] = 1;
