@@ -1,10 +1,11 @@
 #!/usr/bin/env python
 """Frequency-sweep throughput of the MI355X piezo-FEM backend.
 
-One step = one full sweep of the workload: every frequency point factored (complex FP32
-fronts on MFMA, batched over frequencies in chunks that fit HBM), solved, refined against
-the FP64 operator, reduced to Y11.  Assembly, Bloch reduction and the symbolic analysis
-are done once per cell before the timed region (inputs resident in HBM).
+One step = one full sweep of the workload: the frequency points factored in groups of
+neighbouring frequencies (one complex FP32 factorization per group on MFMA, replayed as a HIP
+graph, in chunks that fit HBM), every point solved and refined against its own FP64 operator,
+reduced to Y11 and dY11/dp.  Assembly, Bloch reduction and the symbolic analysis are done once
+per cell before the timed region (inputs resident in HBM).
 
 Printed JSON line: metric/value (whole-job frequency points per second), the plain FP64
 GPU path (one frequency per launch) measured on a sample of points, the Y11 error of the
@@ -29,23 +30,37 @@ import numpy as np  # noqa: E402
 
 PEAK_FP32_MFMA = 157.3   # TFLOP/s dense, MI355X
 PEAK_FP64_MFMA = 78.6
-# HBM bytes per launch of the Schur GEMM k_gemm<float, 1> on the config-2 / config-4 sweeps:
-# rocprofv3 FETCH_SIZE and WRITE_SIZE (separate runs), each scaled by what the counter reads for
-# a plain 1 GiB device copy (`--copy-mib 1024`: FETCH_SIZE x 2.000, WRITE_SIZE x 1.000)
-# HBM bytes per launch of the Schur-complement GEMM, from two rocprofv3 --pmc runs (FETCH_SIZE, then
-# WRITE_SIZE, each calibrated on a 1 GiB device copy): profiles/config2_schur_traffic.md and
-# profiles/config4_schur_traffic.md hold the commands and the per-dispatch counts.  The counters
-# cannot be read inside a timed run (a --pmc run serializes the kernels), so the figure is copied
-# from those summaries, and the kernel it describes has not changed since they were taken.
-SCHUR_TRAFFIC_BYTES = {'config2': 1.823e10, 'config4': 1.620e10}
-SCHUR_TRAFFIC_SOURCE = {'config2': 'profiles/config2_schur_traffic.md', 'config4': 'profiles/config4_schur_traffic.md'}
-# HBM bytes per launch of the grouped solve kernel k_grp_lds (whole fronts), measured the same way
-# (FETCH_SIZE + WRITE_SIZE of the four k_grp_lds whole-front instances over their dispatches)
-GRP_TRAFFIC_BYTES = {'config2': 1.931e9}
-GRP_TRAFFIC_SOURCE = {'config2': 'profiles/config2_grouped_solve_traffic.md'}
+# HBM bytes per launch of the grouped solve kernel k_grp_lds (whole fronts: the four instances
+# over their dispatches) and of the Schur-complement GEMM k_gemm<float, 1>, each from two rocprofv3
+# --pmc runs (FETCH_SIZE, then WRITE_SIZE, calibrated on a 1 GiB device copy), with the
+# algorithmic bytes per launch the benchmark reported in the same command and the factorization
+# sharing it ran at.  The profiles/ summaries hold the commands and the per-kernel counts.  The
+# counters cannot be read inside a timed run (a --pmc run serializes the kernels), so the figures
+# are copied from those summaries.  Launch sizes follow the sharing (a launch covers every
+# factorization or vector block of a chunk): at another sharing the entry is the measured ratio
+# of HBM to algorithmic bytes times the run's own algorithmic bytes per launch.
+COUNTER_TRAFFIC = {
+    ('grp', 'config2'): dict(share=8, hbm=1.216e9, alg=7.216e8,
+                             source='profiles/config2_grouped_solve_traffic.md'),
+    ('schur', 'config2'): dict(share=8, hbm=6.978e9, alg=3.624e9, source='profiles/config2_schur_traffic.md'),
+    ('schur', 'config4'): dict(share=1, hbm=1.620e10, alg=8.48e9, source='profiles/config4_schur_traffic.md'),
+}
 
 
-def grp_roofline(grp, workload):
+def counter_traffic(kernel, workload, share, alg_per_launch):
+    """(HBM bytes per launch, note) of COUNTER_TRAFFIC for this run, or (None, None)."""
+    e = COUNTER_TRAFFIC.get((kernel, workload))
+    if e is None:
+        return None, None
+    note = 'HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy'
+    if share == e['share']:
+        return e['hbm'], f"{note}, from {e['source']} (share {share}, as this run)"
+    ratio = e['hbm'] / e['alg']
+    return ratio * alg_per_launch, (f"{note}: {e['source']} measured {ratio:.2f}x the algorithmic bytes at share "
+                                    f"{e['share']}; times this run's algorithmic bytes per launch (share {share})")
+
+
+def grp_roofline(grp, workload, share):
     """Roofline entry of the grouped block-solve kernel k_grp_lds (every block size and kind on the
     whole fronts of a grouped solve), the sweep's largest kernel by time: algorithmic bytes (factor
     entries read once per block of vectors, vectors gathered and written; counted by the library per
@@ -53,15 +68,15 @@ def grp_roofline(grp, workload):
     if not grp['launches']:
         return None
     gbs = grp['bytes'] / (grp['ms'] * 1e-3) / 1e9
+    traffic, note = counter_traffic('grp', workload, share, grp['bytes'] / grp['launches'])
     return dict(bound='hbm', achieved=gbs, peak=8000.0, unit='GB/s', frac=gbs / 8000.0,
-                traffic=GRP_TRAFFIC_BYTES.get(workload),
-                traffic_note=('HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy, '
-                              f'from {GRP_TRAFFIC_SOURCE[workload]}') if workload in GRP_TRAFFIC_SOURCE else None,
+                traffic=traffic, traffic_note=note,
                 kernel='k_grp_lds<kind, 8|16, false> (grouped whole-front solves, complex64 factors, FP64 vectors)',
                 launches=grp['launches'], kernel_ms=grp['ms'], algorithmic_bytes=grp['bytes'],
                 algorithmic_bytes_per_launch=grp['bytes'] / grp['launches'],
                 avg_launch_ms=grp['ms'] / grp['launches'],
-                note='latency-bound: 62 % of wave cycles waiting, VALU 15 % busy (rocprofv3 counters)')
+                note='latency-bound: chains of dependent loads per front; SQ counters: 62 % of wave cycles waiting, '
+                     'VALU 15 % busy')
 
 
 def log(*a):
@@ -482,7 +497,9 @@ def main():
     for _ in range(args.warmup):
         sweep()
     torch.cuda.synchronize()
-    _lib.call('sawfem_mf_profile', eng._plan, 1)
+    # timed region: factorizations as replayed HIP graphs; HIP events around the grouped solve
+    # kernel's launches only (profile mode 2: the roofline input)
+    _lib.call('sawfem_mf_profile', eng._plan, 2)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         Q, dY, info = sweep()
@@ -492,24 +509,26 @@ def main():
     g_ms, g_bytes, g_n = C.c_double(), C.c_double(), C.c_int64()
     _lib.call('sawfem_mf_profile_grp', eng._plan, C.byref(g_ms), C.byref(g_bytes), C.byref(g_n))
     grp = dict(ms=g_ms.value, bytes=g_bytes.value, launches=g_n.value)
-    prof, prof_levels = profile_read(eng)
     _lib.call('sawfem_mf_profile', eng._plan, 0)
     value = args.steps * P / dt
     log(f'[bench] mixed sweep + adjoint {dt / args.steps:.3f}s/step -> {value:.1f} pts/s; ir {info}')
-    # the same sweep with every factorization one replayed HIP graph (the timed region above runs
-    # the factorizations launch by launch: its per-launch HIP events feed the roofline)
-    sweep()
-    torch.cuda.synchronize()
+    # per-kernel breakdown: NPROF more sweeps with every launch timed (profile mode 1; the
+    # factorizations then run launch by launch, not as graphs)
+    nprof = 1
+    _lib.call('sawfem_mf_profile', eng._plan, 1)
     t0 = time.perf_counter()
-    for _ in range(2):
+    for _ in range(nprof):
         sweep()
     torch.cuda.synchronize()
-    graph = dict(pts_per_s=2 * P / (time.perf_counter() - t0),
+    eager = dict(pts_per_s=nprof * P / (time.perf_counter() - t0),
                  nodes_per_factorization=int(_lib.call_value('sawfem_mf_graph_nodes', eng._plan)),
-                 note='2 sweeps after the timed region, factorizations as captured HIP graphs '
-                      '(sawfem_mf_factor_graph), no per-launch events')
-    log(f'[bench] graph-captured factorizations: {graph["pts_per_s"]:.1f} pts/s, '
-        f'{graph["nodes_per_factorization"]} graph nodes per factorization')
+                 note=f'{nprof} sweep(s) after the timed region with every launch timed by HIP events and the '
+                      'factorizations launch by launch (the timed region replays each factorization as one '
+                      'captured HIP graph of nodes_per_factorization kernels)')
+    prof, prof_levels = profile_read(eng)
+    _lib.call('sawfem_mf_profile', eng._plan, 0)
+    log(f'[bench] profiled sweep, eager factorizations: {eager["pts_per_s"]:.1f} pts/s; '
+        f'{eager["nodes_per_factorization"]} graph nodes per factorization in the timed region')
     # accuracy of the mixed path against the plain FP64 path (with its FP64 adjoint) on a sample
     torch.cuda.empty_cache()          # the FP64 chunks have another size than the timed ones
     # accuracy of the mixed path against the FP64 path (batched complex128 fronts, FP64 adjoint)
@@ -545,13 +564,15 @@ def main():
     # factorizations per sweep: one per `share` frequencies, plus the complex64 refactorizations
     # of frequencies whose shared factors did not converge
     nfact = sum(it['factorizations'] for it in info)
-    schur_flops = work['schur_flops'] * nfact * args.steps
+    schur_flops = work['schur_flops'] * nfact * nprof
     achieved = schur_flops / (schur_ms * 1e-3) / 1e12 if schur_ms > 0 else None
-    lupd_tf = work['lupd_flops'] * nfact * args.steps / (prof['gemm_l'][0] * 1e-3) / 1e12 if prof['gemm_l'][0] else None
+    schur_alg = work['schur_bytes'] * nfact * nprof / max(schur_n, 1)
+    schur_traffic, schur_traffic_note = counter_traffic('schur', args.workload, eng.share, schur_alg)
+    lupd_tf = work['lupd_flops'] * nfact * nprof / (prof['gemm_l'][0] * 1e-3) / 1e12 if prof['gemm_l'][0] else None
     # solves: each forward+backward pass streams the stored L/U entries (complex64) once
     # (passes over the factors: paired forward+adjoint solves share each pass)
     nsolve_pts = sum(sum(it['solved_per_pass']) if it.get('paired_adjoint')
-                     else it['points'] * (2 + it['ir_steps'] + it.get('adj_steps', 0)) for it in info) * args.steps
+                     else it['points'] * (2 + it['ir_steps'] + it.get('adj_steps', 0)) for it in info) * nprof
     solve_bytes = nsolve_pts * work['lu_entries'] * 8
     solve_ms = prof['fwd'][0] + prof['bwd'][0]
     solve_gbs = solve_bytes / (solve_ms * 1e-3) / 1e9 if solve_ms else None
@@ -600,16 +621,14 @@ def main():
         speedup_vs_plain_fp64=dict(central_fd=value / plain_fd, fp64_adjoint=value / plain_adj,
                                    y11_only=value / plain) if pp > 0 else None,
         ir=info,
-        roofline=grp_roofline(grp, args.workload),
+        roofline=grp_roofline(grp, args.workload, eng.share),
         mfma_roofline=dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
                       frac=(achieved / PEAK_FP32_MFMA) if achieved else None,
-                      traffic=SCHUR_TRAFFIC_BYTES.get(args.workload),
-                      traffic_note='HBM bytes per launch, rocprofv3 FETCH_SIZE+WRITE_SIZE scaled by a 1 GiB copy, '
-                                   f'copied from {SCHUR_TRAFFIC_SOURCE.get(args.workload, "(no counter run)")}',
+                      traffic=schur_traffic, traffic_note=schur_traffic_note,
                       kernel='k_gemm<float, 1, 64, 64, *> (Schur complement, complex64 on FP32 MFMA)',
                       launches=schur_n, kernel_ms=schur_ms, algorithmic_flops=schur_flops,
-                      algorithmic_bytes_per_launch=work['schur_bytes'] * P * args.steps / max(schur_n, 1),
-                      avg_launch_ms=schur_ms / max(schur_n, 1)),
+                      algorithmic_bytes_per_launch=schur_alg, avg_launch_ms=schur_ms / max(schur_n, 1),
+                      note=f'from the profiled sweep after the timed region ({nprof} sweep)'),
         other_kernels=dict(
             whole_sweep=dict(bound='mfma', achieved=sweep_tf, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
                              frac=sweep_tf / PEAK_FP32_MFMA, algorithmic_flops=sweep_flops,
@@ -628,7 +647,7 @@ def main():
                        'time; the Schur complements run one at a time on the launch stream',
         kernel_ms_per_level=prof_levels,
         factor_flops_per_point=work['factor_flops'],
-        factor_graph=graph,
+        profiled_sweep=eager,
         cpu_baseline=cpu,
     )
     print(json.dumps(out), flush=True)

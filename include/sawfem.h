@@ -293,9 +293,11 @@ int sawfem_mf_set_group_tri(void* plan, int rc_max, int lds_bytes);
 int sawfem_mf_set_group_big(void* plan, int max_p);
 
 /* Per-kernel HIP-event timing of the factor/solve launches (no counterpart in the reference;
- * used by the benchmark).  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S,
- * 6 fwd, 7 bwd, 8 fused small fronts.  profile_read waits for the recorded events and
- * returns totals in ms[9] / count[9]. */
+ * used by the benchmark): enable 1 times every launch (factorizations then run launch by launch,
+ * not as graphs), 2 only the grouped solve kernel (sawfem_mf_profile_grp; graphs allowed), 0
+ * turns it off.  Kinds: 0 init, 1 entries, 2 diag, 3 trsm, 4 gemm-L, 5 gemm-S, 6 fwd, 7 bwd,
+ * 8 fused small fronts.  profile_read waits for the recorded events and returns totals in
+ * ms[9] / count[9]. */
 int sawfem_mf_profile(void* plan, int enable);
 int sawfem_mf_profile_read(void* plan, double* ms, int64_t* count);
 /* The same totals per level of the assembly tree (leaves first), ms[level * 9 + kind], from the

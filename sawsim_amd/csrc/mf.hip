@@ -2196,7 +2196,7 @@ enum Kind { K_INIT = 0, K_ENTRIES, K_DIAG, K_TRSM, K_GEMM_L, K_GEMM_S, K_FWD, K_
 
 struct Plan {
   int n = 0, nf = 0;
-  bool profile = false;
+  int profile = 0;               // 0 off, 1 every launch (factorizations eager), 2 grouped solves only
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   struct Mark { int kind, level; size_t i0; };
@@ -2276,14 +2276,14 @@ void drop_graphs(Plan* P) {
   P->graphs.clear();
 }
 
-// Record a start/stop event pair around one launch when profiling is on.
+// Record a start/stop event pair around one launch when every launch is profiled (mode 1).
 struct Timed {
-  Plan* P; hipStream_t s; int kind; size_t i0;
-  Timed(Plan* p, hipStream_t st, int k) : P(p), s(st), kind(k), i0(0) {
-    if (!P->profile) return;
+  Plan* P; hipStream_t s; int kind; size_t i0; bool on;
+  Timed(Plan* p, hipStream_t st, int k) : P(p), s(st), kind(k), i0(0), on(p->profile == 1) {
+    if (!on) return;
     while (P->ev_pool.size() < P->ev_used + 2) {
       hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) { P->profile = false; return; }
+      if (hipEventCreate(&e) != hipSuccess) { P->profile = 0; on = false; return; }
       P->ev_pool.push_back(e);
     }
     i0 = P->ev_used;
@@ -2291,7 +2291,7 @@ struct Timed {
     (void)hipEventRecord(P->ev_pool[i0], s);
   }
   ~Timed() {
-    if (!P->profile) return;
+    if (!on) return;
     (void)hipEventRecord(P->ev_pool[i0 + 1], s);
     P->ev_marks.push_back({kind, P->cur_level, i0});
   }
@@ -3377,8 +3377,9 @@ int sawfem_mf_factor_graph(void* plan, void* stream, int prec, int B, const doub
     return 1;
   }
   hipStream_t s = (hipStream_t)stream;
-  // per-launch event timing does not go into graphs: the profiled factorization runs eagerly
-  if (P->profile) return sawfem_mf_factor(plan, stream, prec, B, omega, beta, Kaa, Maa, Daa, arena, nb);
+  // per-launch event timing does not go into graphs: with every launch profiled the factorization
+  // runs eagerly
+  if (P->profile == 1) return sawfem_mf_factor(plan, stream, prec, B, omega, beta, Kaa, Maa, Daa, arena, nb);
   Plan::FactorGraph* hit = nullptr;
   for (auto& g : P->graphs)
     if (g.prec == prec && g.B == B && g.nb == nb && g.omega == omega && g.Kaa == Kaa && g.Maa == Maa &&
@@ -3492,11 +3493,12 @@ int sawfem_mf_set_group_tri(void* plan, int rc_max, int lds_bytes) {
   return P->group_rc_tri;
 }
 
-// Per-kernel event timing: enable (1) / disable (0); turning it on clears what was recorded.
+// Per-kernel event timing: 1 every launch, 2 the grouped solve kernel only (factorizations may run
+// as graphs), 0 off; turning it on clears what was recorded.
 int sawfem_mf_profile(void* plan, int enable) {
   Plan* P = (Plan*)plan;
-  if (!P) { sawfem_set_error("sawfem_mf_profile: null plan"); return 1; }
-  P->profile = enable != 0;
+  if (!P || enable < 0 || enable > 2) { sawfem_set_error("sawfem_mf_profile: invalid arguments"); return 1; }
+  P->profile = enable;
   P->ev_used = 0;
   P->ev_marks.clear();
   P->grp_marks.clear();

@@ -112,10 +112,12 @@ def tune_share(eng, freqs, precision='mixed', gradient=True, candidates=SHARE_CA
     chunk it would run in the sweep: as many of the sweep's frequencies as fit the device memory
     at that sharing (more with more sharing: the arena holds one factorization per ``share``
     frequencies), a contiguous block from the middle of the band, refinement and fallbacks
-    included, timed with the device synchronized (the fastest of ``reps`` runs).  The choice
-    minimizes the time per frequency;
-    a candidate replaces share 1 only when it is faster by more than ``margin``.  Returns
-    ``dict(choice=..., ms=..., batch=...)``; the engine is left set to the choice."""
+    included, timed with the device synchronized (the fastest of ``reps`` runs).  The choice is
+    the smallest sharing whose time per frequency is within ``margin`` of the fastest candidate's
+    (less sharing takes fewer refinement steps, so near-ties go to it and the choice does not
+    flip between runs; in particular share 1 stays unless another is faster by more than
+    ``margin``).  Returns ``dict(choice=..., ms=..., batch=...)``; the engine is left set to the
+    choice."""
     import time
     torch = eng.torch
     freqs = np.asarray(freqs, dtype=float).ravel()
@@ -145,8 +147,8 @@ def tune_share(eng, freqs, precision='mixed', gradient=True, candidates=SHARE_CA
             t = 1e3 * (time.perf_counter() - t0)
             ms[c] = min(ms.get(c, t), t)
         per[c] = ms[c] / B
-    best = min(per, key=per.get)
-    choice = best if per[best] < (1 - margin) * per[1] else 1
+    fastest = min(per.values())
+    choice = min(c for c in per if per[c] <= (1 + margin) * fastest)
     eng.set_share(choice)
     eng.pivot_stats(reset=True)
     return dict(choice=choice, ms=ms, batch=batch, ms_per_frequency=per)
