@@ -498,8 +498,11 @@ def main():
         sweep()
     torch.cuda.synchronize()
     # timed region: factorizations as replayed HIP graphs; HIP events around the grouped solve
-    # kernel's launches only (profile mode 2: the roofline input)
-    _lib.call('sawfem_mf_profile', eng._plan, 2)
+    # kernel's launches only (profile mode 2: the roofline input).  Without sharing no grouped
+    # solves run: every launch is timed instead (mode 1, factorizations launch by launch) and the
+    # Schur GEMM is the roofline entry.
+    mode = 2 if eng.share > 1 else 1
+    _lib.call('sawfem_mf_profile', eng._plan, mode)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         Q, dY, info = sweep()
@@ -509,26 +512,30 @@ def main():
     g_ms, g_bytes, g_n = C.c_double(), C.c_double(), C.c_int64()
     _lib.call('sawfem_mf_profile_grp', eng._plan, C.byref(g_ms), C.byref(g_bytes), C.byref(g_n))
     grp = dict(ms=g_ms.value, bytes=g_bytes.value, launches=g_n.value)
+    if mode == 1:
+        prof, prof_levels = profile_read(eng)
     _lib.call('sawfem_mf_profile', eng._plan, 0)
     value = args.steps * P / dt
     log(f'[bench] mixed sweep + adjoint {dt / args.steps:.3f}s/step -> {value:.1f} pts/s; ir {info}')
-    # per-kernel breakdown: NPROF more sweeps with every launch timed (profile mode 1; the
-    # factorizations then run launch by launch, not as graphs)
-    nprof = 1
-    _lib.call('sawfem_mf_profile', eng._plan, 1)
-    t0 = time.perf_counter()
-    for _ in range(nprof):
-        sweep()
-    torch.cuda.synchronize()
-    eager = dict(pts_per_s=nprof * P / (time.perf_counter() - t0),
-                 nodes_per_factorization=int(_lib.call_value('sawfem_mf_graph_nodes', eng._plan)),
-                 note=f'{nprof} sweep(s) after the timed region with every launch timed by HIP events and the '
-                      'factorizations launch by launch (the timed region replays each factorization as one '
-                      'captured HIP graph of nodes_per_factorization kernels)')
-    prof, prof_levels = profile_read(eng)
-    _lib.call('sawfem_mf_profile', eng._plan, 0)
-    log(f'[bench] profiled sweep, eager factorizations: {eager["pts_per_s"]:.1f} pts/s; '
-        f'{eager["nodes_per_factorization"]} graph nodes per factorization in the timed region')
+    nprof, eager = args.steps, None
+    if mode == 2:
+        # per-kernel breakdown: NPROF more sweeps with every launch timed (profile mode 1; the
+        # factorizations then run launch by launch, not as graphs)
+        nprof = 1
+        _lib.call('sawfem_mf_profile', eng._plan, 1)
+        t0 = time.perf_counter()
+        for _ in range(nprof):
+            sweep()
+        torch.cuda.synchronize()
+        eager = dict(pts_per_s=nprof * P / (time.perf_counter() - t0),
+                     nodes_per_factorization=int(_lib.call_value('sawfem_mf_graph_nodes', eng._plan)),
+                     note=f'{nprof} sweep(s) after the timed region with every launch timed by HIP events and the '
+                          'factorizations launch by launch (the timed region replays each factorization as one '
+                          'captured HIP graph of nodes_per_factorization kernels)')
+        prof, prof_levels = profile_read(eng)
+        _lib.call('sawfem_mf_profile', eng._plan, 0)
+        log(f'[bench] profiled sweep, eager factorizations: {eager["pts_per_s"]:.1f} pts/s; '
+            f'{eager["nodes_per_factorization"]} graph nodes per factorization in the timed region')
     # accuracy of the mixed path against the plain FP64 path (with its FP64 adjoint) on a sample
     torch.cuda.empty_cache()          # the FP64 chunks have another size than the timed ones
     # accuracy of the mixed path against the FP64 path (batched complex128 fronts, FP64 adjoint)
@@ -588,6 +595,14 @@ def main():
                                   + (it['ir_steps'] + it.get('adj_steps', 0)) * 8 * nnz_aa))
     sweep_flops = args.steps * sum(chunk_flops(it) for it in info)
     sweep_tf = sweep_flops / dt / 1e12
+    mfma_roofline_entry = dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
+                               frac=(achieved / PEAK_FP32_MFMA) if achieved else None,
+                               traffic=schur_traffic, traffic_note=schur_traffic_note,
+                               kernel='k_gemm<float, 1, 64, 64, *> (Schur complement, complex64 on FP32 MFMA)',
+                               launches=schur_n, kernel_ms=schur_ms, algorithmic_flops=schur_flops,
+                               algorithmic_bytes_per_launch=schur_alg, avg_launch_ms=schur_ms / max(schur_n, 1),
+                               note=(f'from the profiled sweep after the timed region ({nprof} sweep)' if eager
+                                     else 'from the timed region (every launch timed)'))
     cpu = None
     if not args.no_cpu:
         try:
@@ -621,14 +636,10 @@ def main():
         speedup_vs_plain_fp64=dict(central_fd=value / plain_fd, fp64_adjoint=value / plain_adj,
                                    y11_only=value / plain) if pp > 0 else None,
         ir=info,
-        roofline=grp_roofline(grp, args.workload, eng.share),
-        mfma_roofline=dict(bound='mfma', achieved=achieved, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
-                      frac=(achieved / PEAK_FP32_MFMA) if achieved else None,
-                      traffic=schur_traffic, traffic_note=schur_traffic_note,
-                      kernel='k_gemm<float, 1, 64, 64, *> (Schur complement, complex64 on FP32 MFMA)',
-                      launches=schur_n, kernel_ms=schur_ms, algorithmic_flops=schur_flops,
-                      algorithmic_bytes_per_launch=schur_alg, avg_launch_ms=schur_ms / max(schur_n, 1),
-                      note=f'from the profiled sweep after the timed region ({nprof} sweep)'),
+        roofline=grp_roofline(grp, args.workload, eng.share) or dict(
+            mfma_roofline_entry, note='no grouped solves ran (share 1): the Schur GEMM, timed in the timed region, '
+                                      'is the roofline entry'),
+        mfma_roofline=mfma_roofline_entry,
         other_kernels=dict(
             whole_sweep=dict(bound='mfma', achieved=sweep_tf, peak=PEAK_FP32_MFMA, unit='TFLOP/s',
                              frac=sweep_tf / PEAK_FP32_MFMA, algorithmic_flops=sweep_flops,
