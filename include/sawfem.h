@@ -275,6 +275,13 @@ int sawfem_mf_solve_shared(void* plan, void* stream, int prec, int B, int nrhs, 
  * solution).  No counterpart in the reference (it factors every frequency). */
 int sawfem_mf_solve_groups(void* plan, void* stream, int B, int nrhs, const void* arena, int G, void* uarena,
                            void* x, int transpose, int nb, const int* gsel, int ngsel, int tail);
+/* sawfem_mf_solve_groups (forward + backward) on vectors stored row-major: entry i of vector
+ * b nrhs + r at x[i ldx + b nrhs + r], ldx >= B nrhs, so the vectors of one row are contiguous
+ * and the solve's gathers and scatters read whole lines; the update vectors in uarena are stored
+ * row-major as well.  The same operations as sawfem_mf_solve_groups, so the same results bit for
+ * bit.  No counterpart in the reference. */
+int sawfem_mf_solve_groups_rows(void* plan, void* stream, int B, int nrhs, const void* arena, int G, void* uarena,
+                                void* x, int64_t ldx, int nb, const int* gsel, int ngsel, int tail);
 /* Largest block of vectors one workgroup of sawfem_mf_solve_groups takes (1..16; the block
  * size used is the largest R 2^k within it that divides G nrhs and fits the LDS).  Returns the
  * value in use or -1.  No counterpart in the reference. */
@@ -328,12 +335,16 @@ int sawfem_residual_t(void* stream, int n, int B, int Bp, const int64_t* indptr,
 /* sawfem_residual_t over ng row groups (gptr int32[ng + 1], grows int32: the rows of group g
  * are grows[gptr[g] .. gptr[g + 1]), at most 4, with identical column lists, e.g. the active
  * unknowns of one merged mesh node; every row in exactly one group).  One wave per group and 64
- * frequencies reads each x row once for the group's rows; per row the result equals
- * sawfem_residual_t's bit for bit.  rhst may be NULL (then rt = -A x).  Replaces the same residual as sawfem_residual (the FP64
- * A(w) x of SawSim saw2d/sweep.py:335-364's solve, here the refinement residual). */
-int sawfem_residual_sv(void* stream, int ng, int B, int Bp, const int* gptr, const int* grows, const int64_t* indptr,
-                       const int* indices, const int64_t* tslot, const void* Kaa, const void* Maa, const void* Daa,
-                       const double* omega, double beta, const void* xt, const void* rhst, void* rt);
+ * columns reads each x row once for the group's rows; per row the result equals
+ * sawfem_residual_t's bit for bit.  B columns c of the [n][Bp] arrays xt, rhst, rt, stored at
+ * column cmap[c] (device int, or NULL: column c, then Bp >= B); omega is indexed by the stored
+ * column; the other columns are not touched.  rhst may be NULL (then rt = -A x).  Replaces the
+ * same residual as sawfem_residual (the FP64 A(w) x of SawSim saw2d/sweep.py:335-364's solve,
+ * here the refinement residual). */
+int sawfem_residual_sv(void* stream, int ng, int B, int Bp, const int* cmap, const int* gptr, const int* grows,
+                       const int64_t* indptr, const int* indices, const int64_t* tslot, const void* Kaa,
+                       const void* Maa, const void* Daa, const double* omega, double beta, const void* xt,
+                       const void* rhst, void* rt);
 
 /* out (C rows, leading dimension ld_out) = transpose of in (R rows, leading dimension ld_in). */
 int sawfem_transpose(void* stream, int R, int C, int ld_in, int ld_out, const void* in, void* out);
@@ -355,6 +366,16 @@ int sawfem_charge(void* stream, int n, int B, const void* kts_a, double kz_re, d
  * (saw2d/sweep.py:360-364, Q = kts . disp) carried over to the adjoint. */
 int sawfem_wdot(void* stream, int n, int nb, const void* a, int64_t sa, const double* w, const void* c, int64_t sc,
                 const int* sel, void* out);
+
+/* sawfem_charge and sawfem_wdot on vectors stored row-major (entry i of column j at [i ld + j]):
+ * Q[k] = sum_i kts_a[i] x[i][cols[k]] + V kts_z, out[k] = sum_i a[i][acol[k]] w[i] c[i][ccol[k]]
+ * for k < nc (column maps: device int, NULL for k).  Summed in the order of sawfem_charge /
+ * sawfem_wdot, so equal to them bit for bit on the same vectors.  work: 1024 nc complex128
+ * partial sums.  No counterpart in the reference (sweep.py:356-364's charge reduction). */
+int sawfem_charge_rows(void* stream, int n, int nc, int64_t ld, const void* kts_a, double kz_re, double kz_im,
+                       double V, const void* x, const int* cols, void* work, void* Q);
+int sawfem_wdot_rows(void* stream, int n, int nc, int64_t ld, const void* a, const int* acol, const double* w,
+                     const void* c, const int* ccol, void* work, void* out);
 
 /* x += d (n complex128). */
 int sawfem_axpy(void* stream, int64_t n, void* x, const void* d);

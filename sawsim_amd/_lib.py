@@ -51,6 +51,7 @@ SIGNATURES = {
     'sawfem_mf_solve': [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _i, _i, _vp, _i],
     'sawfem_mf_solve_shared': [_vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _i, _vp, _i],
     'sawfem_mf_solve_groups': [_vp, _vp, _i, _i, _vp, _i, _vp, _vp, _i, _i, _vp, _i, _i],
+    'sawfem_mf_solve_groups_rows': [_vp, _vp, _i, _i, _vp, _i, _vp, _vp, _i64, _i, _vp, _i, _i],
     'sawfem_mf_set_group_rc': [_vp, _i],
     'sawfem_mf_set_group_tri': [_vp, _i, _i],
     'sawfem_mf_set_group_big': [_vp, _i],
@@ -62,7 +63,9 @@ SIGNATURES = {
     'sawfem_transpose_map': [_vp, _i, _i, _i, _i, _vp, _vp, _vp, _vp, _vp],
     'sawfem_mf_profile_grp': [_vp, _vp, _vp, _vp],
     'sawfem_wdot': [_vp, _i, _i, _vp, _i64, _vp, _vp, _i64, _vp, _vp],
-    'sawfem_residual_sv': [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
+    'sawfem_residual_sv': [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
+    'sawfem_charge_rows': [_vp, _i, _i, _i64, _vp, _d, _d, _d, _vp, _vp, _vp, _vp],
+    'sawfem_wdot_rows': [_vp, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'sawfem_transpose': [_vp, _i, _i, _i, _i, _vp, _vp],
     'sawfem_charge': [_vp, _i, _i, _vp, _d, _d, _d, _vp, _vp],
     'sawfem_axpy': [_vp, _i64, _vp, _vp],

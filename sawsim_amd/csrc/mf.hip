@@ -56,14 +56,35 @@ struct FrontsDev {
   // fslot is null and vdiv > 0 (uniform groups), or slot b of an arena of B (one factorization
   // per block).  Grid row y is block sel[y / cpg] * cpg + y % cpg with a selection list (cpg
   // blocks per selected entry), else block boff + y.  nvec: vectors of the whole call (the
-  // update vectors of front f start at uoff[f] * nvec).
+  // update vectors of front f start at uoff[f] * nvec).  Vector layout (xat, uat): vector-major
+  // with xld = 0 (x[v][n], a front's update vectors [v][q]); with xld > 0 row-major (x[n][xld],
+  // a front's update vectors [q][nvec]), in which the vectors of one row are contiguous.
   const int* fslot = nullptr;
   int Bf = 0;
   int vdiv = 0;
   int cpg = 1;
   int boff = 0;
   int64_t nvec = 0;
+  int64_t xld = 0;
 };
+
+// Index of entry (vector v, row i) of the solve vectors x (n rows) and of entry (vector v,
+// contribution-block row j) of a front's update vectors (q rows, from uarena + uoff[f] nvec).
+__device__ __forceinline__ int64_t xat(const FrontsDev& F, int n, int64_t v, int64_t i) {
+  return F.xld ? i * F.xld + v : v * n + i;
+}
+__device__ __forceinline__ int64_t uat(const FrontsDev& F, int q, int64_t v, int64_t j) {
+  return F.xld ? j * F.nvec + v : v * q + j;
+}
+// The same with the layout fixed at compile time (XR: row-major).
+template <bool XR>
+__device__ __forceinline__ int64_t xat_t(const FrontsDev& F, int n, int64_t v, int64_t i) {
+  return XR ? i * F.xld + v : v * n + i;
+}
+template <bool XR>
+__device__ __forceinline__ int64_t uat_t(const FrontsDev& F, int q, int64_t v, int64_t j) {
+  return XR ? j * F.nvec + v : v * q + j;
+}
 
 template <typename T>
 __device__ __forceinline__ typename cplx<T>::type* front_ptr(typename cplx<T>::type* arena, const FrontsDev& F,
@@ -967,16 +988,16 @@ __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, i
   for (int i = threadIdx.x; i < m; i += blockDim.x) {
     double2 v[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[r] = (i < p) ? x[((int64_t)b * R + r) * n + rows[i]] : make_double2(0, 0);
+    for (int r = 0; r < R; ++r) v[r] = (i < p) ? x[xat(F, n, (int64_t)b * R + r, rows[i])] : make_double2(0, 0);
     for (int c = 0; c < nch; ++c) {
       const int slot = F.child_ptr[f] + c;
       const int ci = F.inv[F.inv_ptr[slot] + i];
       if (ci >= 0) {
         const int ch = F.child_ids[slot];
         const int qc = F.m[ch] - F.p[ch];
-        const double2* u = uarena + F.uoff[ch] * F.nvec + (int64_t)b * R * qc + ci;
+        const double2* u = uarena + F.uoff[ch] * F.nvec;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[r] = d2add(v[r], u[(int64_t)r * qc]);
+        for (int r = 0; r < R; ++r) v[r] = d2add(v[r], u[uat(F, qc, (int64_t)b * R + r, ci)]);
       }
     }
     if (ALL) {
@@ -986,9 +1007,9 @@ __device__ __forceinline__ void solve_gather(const FrontsDev& F, int f, int b, i
 #pragma unroll
       for (int r = 0; r < R; ++r) w[r * p + i] = v[r];
     } else {
-      double2* uf = uarena + F.uoff[f] * F.nvec + (int64_t)b * R * q + (i - p);
+      double2* uf = uarena + F.uoff[f] * F.nvec;
 #pragma unroll
-      for (int r = 0; r < R; ++r) uf[(int64_t)r * q] = v[r];
+      for (int r = 0; r < R; ++r) uf[uat(F, q, (int64_t)b * R + r, i - p)] = v[r];
     }
   }
 }
@@ -1034,7 +1055,7 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
   } else {
     for (int i = t; i < W; i += blockDim.x)
 #pragma unroll
-      for (int r = 0; r < R; ++r) w[r * W + i] = x[((int64_t)b * R + r) * n + rows[i]];
+      for (int r = 0; r < R; ++r) w[r * W + i] = x[xat(F, n, (int64_t)b * R + r, rows[i])];
   }
   __syncthreads();
   if (FULL && !fwd && p < m) {
@@ -1207,11 +1228,11 @@ __device__ __forceinline__ void tri_body(FrontsDev F, const int* __restrict__ lv
   for (int i = t; i < W; i += blockDim.x) {
     if (i < p) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) x[((int64_t)b * R + r) * n + rows[i]] = w[r * W + i];
+      for (int r = 0; r < R; ++r) x[xat(F, n, (int64_t)b * R + r, rows[i])] = w[r * W + i];
     } else if (fwd) {
-      double2* uf = uarena + F.uoff[f] * F.nvec + (int64_t)b * R * (m - p) + (i - p);
+      double2* uf = uarena + F.uoff[f] * F.nvec;
 #pragma unroll
-      for (int r = 0; r < R; ++r) uf[(int64_t)r * (m - p)] = w[r * W + i];
+      for (int r = 0; r < R; ++r) uf[uat(F, m - p, (int64_t)b * R + r, i - p)] = w[r * W + i];
     }
   }
 }
@@ -1303,7 +1324,7 @@ __global__ void __launch_bounds__(256) k_tri_lds(FrontsDev F, const int* __restr
     } else {
       for (int i = t; i < m; i += 256)
 #pragma unroll
-        for (int r = 0; r < R; ++r) w[r * m + i] = x[((int64_t)b * R + r) * n + rows[i]];
+        for (int r = 0; r < R; ++r) w[r * m + i] = x[xat(F, n, (int64_t)b * R + r, rows[i])];
     }
     __syncthreads();
     if (item + (int)gridDim.x < nitems)     // in flight during the solve below
@@ -1391,11 +1412,11 @@ __global__ void __launch_bounds__(256) k_tri_lds(FrontsDev F, const int* __restr
     for (int i = t; i < m; i += 256) {
       if (i < p) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) x[((int64_t)b * R + r) * n + rows[i]] = w[r * m + i];
+        for (int r = 0; r < R; ++r) x[xat(F, n, (int64_t)b * R + r, rows[i])] = w[r * m + i];
       } else if (KIND == 0) {
-        double2* uf = uarena + F.uoff[f] * F.nvec + (int64_t)b * R * (m - p) + (i - p);
+        double2* uf = uarena + F.uoff[f] * F.nvec;
 #pragma unroll
-        for (int r = 0; r < R; ++r) uf[(int64_t)r * (m - p)] = w[r * m + i];
+        for (int r = 0; r < R; ++r) uf[uat(F, m - p, (int64_t)b * R + r, i - p)] = w[r * m + i];
       }
     }
     __syncthreads();          // w and Pl are rewritten by the next item
@@ -1504,7 +1525,7 @@ __device__ __forceinline__ void grp_row_product_pre(const float2* __restrict__ c
 // p pivot rows, kind 0 adds the children's updates of the pivot rows only (k_grp_panel forms
 // the update vectors), kind 1 starts from x at the pivot rows (k_grp_panel has already
 // subtracted U12 times the contribution-block rows).
-template <int KIND, int RC, bool PIV = false>
+template <int KIND, int RC, bool PIV = false, bool XR = false>
 __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restrict__ lvl, int B, int n, int nbl,
                                                  const float2* __restrict__ arena, double2* __restrict__ uarena,
                                                  double2* __restrict__ x, const int* __restrict__ sel) {
@@ -1536,6 +1557,16 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
   if (KIND == 0 && tr > 0 && tr < nbf) grp_load<TPR>(Fb + tr, m, 0, tr, q4, pex, pey);
   if (KIND == 1 && cb_first && tr < p) grp_load<TPR>(Fb + tr + (int64_t)p * m, m, 0, q, q4, pex, pey);
   const int mr = W * RC;
+  // entry idx of the block's W x RC vector entries: the row fastest over the threads in the
+  // vector-major layout, the vector fastest in the row-major one (coalesced either way)
+  auto split = [W](int idx) -> int2 {    // (vector r, row i)
+    if (XR) {
+      const int i = idx / RC;
+      return make_int2(idx - i * RC, i);
+    }
+    const int r = idx / W;
+    return make_int2(r, idx - r * W);
+  };
   const int c0 = F.child_ptr[f], nch = KIND == 0 ? F.child_ptr[f + 1] - c0 : 0;
   int q2[2] = {0, 0};
   if (nch == 2) {
@@ -1548,8 +1579,8 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
     // entry, a line in use anyway); sums in the order of the general loop below
     const int* inv0 = F.inv + F.inv_ptr[c0];
     const int* inv1 = F.inv + F.inv_ptr[c0 + 1];
-    const double2* u0 = uarena + F.uoff[F.child_ids[c0]] * F.nvec + v0 * q2[0];
-    const double2* u1 = uarena + F.uoff[F.child_ids[c0 + 1]] * F.nvec + v0 * q2[1];
+    const double2* u0 = uarena + F.uoff[F.child_ids[c0]] * F.nvec;
+    const double2* u1 = uarena + F.uoff[F.child_ids[c0 + 1]] * F.nvec;
     constexpr int U = 4;
     for (int base = t; base < mr; base += U * 256) {
       double2 a[U], b0[U], b1[U];
@@ -1557,18 +1588,20 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int idx = min(base + u * 256, mr - 1);
-        const int r = idx / W, i = idx - r * W;
+        const int2 ri = split(idx);
+        const int r = ri.x, i = ri.y;
         k0[u] = inv0[i];
         k1[u] = inv1[i];
-        a[u] = x[(v0 + r) * n + rows[min(i, p - 1)]];
-        b0[u] = u0[(int64_t)r * q2[0] + max(k0[u], 0)];
-        b1[u] = u1[(int64_t)r * q2[1] + max(k1[u], 0)];
+        a[u] = x[xat_t<XR>(F, n, v0 + r, rows[min(i, p - 1)])];
+        b0[u] = u0[uat_t<XR>(F, q2[0], v0 + r, max(k0[u], 0))];
+        b1[u] = u1[uat_t<XR>(F, q2[1], v0 + r, max(k1[u], 0))];
       }
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const int idx = base + u * 256;
         if (idx < mr) {
-          const int r = idx / W, i = idx - r * W;
+          const int2 ri = split(idx);
+          const int r = ri.x, i = ri.y;
           double2 v = i < p ? a[u] : make_double2(0, 0);
           if (k0[u] >= 0) v = d2add(v, b0[u]);
           if (k1[u] >= 0) v = d2add(v, b1[u]);
@@ -1581,26 +1614,30 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
                [&](int idx) {
                  // kind 0 uses x at the pivot rows only: the other lanes re-read the last pivot row
                  // (a line the wave loads anyway) instead of a contribution-block row
-                 const int r = idx / W, i = idx - r * W;
-                 return x[(v0 + r) * n + rows[KIND == 0 ? min(i, p - 1) : i]];
+                 const int2 ri = split(idx);
+                 const int r = ri.x, i = ri.y;
+                 return x[xat_t<XR>(F, n, v0 + r, rows[KIND == 0 ? min(i, p - 1) : i])];
                },
                [&](int idx, double2 v) {
-                 const int r = idx / W, i = idx - r * W;
+                 const int2 ri = split(idx);
+                 const int r = ri.x, i = ri.y;
                  w[i * WS + r] = (KIND == 1 || i < p) ? v : make_double2(0, 0);
                });
     for (int c = 0; c < nch; ++c) {
       const int* inv = F.inv + F.inv_ptr[c0 + c];
       const int ch = F.child_ids[c0 + c];
       const int qc = F.m[ch] - F.p[ch];
-      const double2* u = uarena + F.uoff[ch] * F.nvec + v0 * qc;
+      const double2* u = uarena + F.uoff[ch] * F.nvec;
       // each thread adds to the w entries it wrote above (same index -> thread map): no barrier
       batched<8>(t, mr, 256,
                  [&](int idx) {
-                   const int r = idx / W, i = idx - r * W;
-                   return u[(int64_t)r * qc + max(inv[i], 0)];
+                   const int2 ri = split(idx);
+                   const int r = ri.x, i = ri.y;
+                   return u[uat_t<XR>(F, qc, v0 + r, max(inv[i], 0))];
                  },
                  [&](int idx, double2 v) {
-                   const int r = idx / W, i = idx - r * W;
+                   const int2 ri = split(idx);
+                   const int r = ri.x, i = ri.y;
                    if (inv[i] >= 0) w[i * WS + r] = d2add(w[i * WS + r], v);
                  });
     }
@@ -1668,11 +1705,12 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
     }
     __syncthreads();
   }
-  double2* uf = uarena + F.uoff[f] * F.nvec + v0 * q;
+  double2* uf = uarena + F.uoff[f] * F.nvec;
   for (int idx = t; idx < mr; idx += 256) {
-    const int r = idx / W, i = idx - r * W;
-    if (i < p) x[(v0 + r) * n + rows[i]] = w[i * WS + r];
-    else if (KIND == 0) uf[(int64_t)r * q + (i - p)] = w[i * WS + r];
+    const int2 ri = split(idx);
+    const int r = ri.x, i = ri.y;
+    if (i < p) x[xat_t<XR>(F, n, v0 + r, rows[i])] = w[i * WS + r];
+    else if (KIND == 0) uf[uat_t<XR>(F, q, v0 + r, i - p)] = w[i * WS + r];
   }
 }
 
@@ -1687,7 +1725,7 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
 //                          takes only the pivot rows)
 //  kind 1 (backward, U12): x[o] -= sum_{j<q} U[o][p + j] x[p + j]                   o: pivot rows
 constexpr int GRP_VC = 128;
-template <int KIND, int RC>
+template <int KIND, int RC, bool XR = false>
 __global__ void __launch_bounds__(256) k_grp_panel(FrontsDev F, const int* __restrict__ lvl, int B, int n,
                                                    const float2* __restrict__ arena, double2* __restrict__ uarena,
                                                    double2* __restrict__ x, const int* __restrict__ sel) {
@@ -1718,11 +1756,12 @@ __global__ void __launch_bounds__(256) k_grp_panel(FrontsDev F, const int* __res
     __syncthreads();
     batched<8>(t, nc * RC, NT,
                [&](int idx) {
-                 const int r = idx / nc, k = idx - r * nc;
-                 return x[(v0 + r) * n + rows[in0 + c0 + k]];
+                 // the row (vector-major layout) or the vector (row-major) fastest over the threads
+                 const int r = XR ? idx % RC : idx / nc, k = XR ? idx / RC : idx - r * nc;
+                 return x[xat_t<XR>(F, n, v0 + r, rows[in0 + c0 + k])];
                },
                [&](int idx, double2 v) {
-                 const int r = idx / nc, k = idx - r * nc;
+                 const int r = XR ? idx % RC : idx / nc, k = XR ? idx / RC : idx - r * nc;
                  vin[k * WS + r] = v;
                });
     __syncthreads();
@@ -1739,13 +1778,13 @@ __global__ void __launch_bounds__(256) k_grp_panel(FrontsDev F, const int* __res
       const int ci0 = F.inv[F.inv_ptr[cp0] + i], ci1 = F.inv[F.inv_ptr[cp0 + 1] + i];
       const int ch0 = F.child_ids[cp0], ch1 = F.child_ids[cp0 + 1];
       const int qc0 = F.m[ch0] - F.p[ch0], qc1 = F.m[ch1] - F.p[ch1];
-      const double2* u0 = uarena + F.uoff[ch0] * F.nvec + (v0 + r0) * qc0 + ci0;
-      const double2* u1 = uarena + F.uoff[ch1] * F.nvec + (v0 + r0) * qc1 + ci1;
+      const double2* u0 = uarena + F.uoff[ch0] * F.nvec;
+      const double2* u1 = uarena + F.uoff[ch1] * F.nvec;
       double2 a0[VPT], a1[VPT];
 #pragma unroll
       for (int v = 0; v < VPT; ++v) {
-        if (ci0 >= 0) a0[v] = u0[(int64_t)v * qc0];
-        if (ci1 >= 0) a1[v] = u1[(int64_t)v * qc1];
+        if (ci0 >= 0) a0[v] = u0[uat_t<XR>(F, qc0, v0 + r0 + v, ci0)];
+        if (ci1 >= 0) a1[v] = u1[uat_t<XR>(F, qc1, v0 + r0 + v, ci1)];
       }
 #pragma unroll
       for (int v = 0; v < VPT; ++v) {
@@ -1758,19 +1797,19 @@ __global__ void __launch_bounds__(256) k_grp_panel(FrontsDev F, const int* __res
       if (ci >= 0) {
         const int ch = F.child_ids[cp0 + c];
         const int qc = F.m[ch] - F.p[ch];
-        const double2* u = uarena + F.uoff[ch] * F.nvec + (v0 + r0) * qc + ci;
+        const double2* u = uarena + F.uoff[ch] * F.nvec;
 #pragma unroll
-        for (int v = 0; v < VPT; ++v) s[v] = d2add(s[v], u[(int64_t)v * qc]);
+        for (int v = 0; v < VPT; ++v) s[v] = d2add(s[v], u[uat_t<XR>(F, qc, v0 + r0 + v, ci)]);
       }
     }
-    double2* uf = uarena + F.uoff[f] * F.nvec + (v0 + r0) * q + (o0 + o);
+    double2* uf = uarena + F.uoff[f] * F.nvec;
 #pragma unroll
-    for (int v = 0; v < VPT; ++v) uf[(int64_t)v * q] = s[v];
+    for (int v = 0; v < VPT; ++v) uf[uat_t<XR>(F, q, v0 + r0 + v, o0 + o)] = s[v];
   } else {
     const int row = rows[o0 + o];
 #pragma unroll
     for (int v = 0; v < VPT; ++v) {
-      double2* d = x + (v0 + r0 + v) * n + row;
+      double2* d = x + xat_t<XR>(F, n, v0 + r0 + v, row);
       *d = d2add(*d, s[v]);
     }
   }
@@ -1799,7 +1838,7 @@ __device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ 
   const int no = min(64, nout - o0);
   const V* Fb = factor_ptr<T>(const_cast<V*>(arena), F, f, b, B);
   const int* rows = F.rows + F.rows_ptr[f];
-  double2* uf = uarena + F.uoff[f] * F.nvec + (int64_t)b * R * q;   // + r * q
+  double2* uf = uarena + F.uoff[f] * F.nvec;                  // + uat(F, q, b R + r, j)
   constexpr int VC = 256;                                     // input entries staged per chunk
   double2* v = reinterpret_cast<double2*>(smem_raw);        // [R][VC] input vector chunk
   double2* part = v + R * VC;                                 // [R][4][64] partial sums
@@ -1814,7 +1853,7 @@ __device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ 
     __syncthreads();
     for (int i = t; i < nv; i += blockDim.x)
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[r * VC + i] = x[((int64_t)b * R + r) * n + rows[(out_cb ? 0 : p) + v0 + i]];
+      for (int r = 0; r < R; ++r) v[r * VC + i] = x[xat(F, n, (int64_t)b * R + r, rows[(out_cb ? 0 : p) + v0 + i])];
     __syncthreads();
     if (KIND == 0 || KIND == 1) {
       // direct: output o is a row of the panel; coalesced across o
@@ -1856,10 +1895,10 @@ __device__ __forceinline__ void panel_body(FrontsDev F, const int* __restrict__ 
       double2 tot = part[r * 256 + t];
       for (int gg = 1; gg < 4; ++gg) tot = d2add(tot, part[r * 256 + gg * 64 + t]);
       if (out_cb) {
-        double2* d = uf + (int64_t)r * q + o0 + t;
+        double2* d = uf + uat(F, q, (int64_t)b * R + r, o0 + t);
         d->x -= tot.x; d->y -= tot.y;
       } else {
-        double2* d = x + ((int64_t)b * R + r) * n + rows[o0 + t];
+        double2* d = x + xat(F, n, (int64_t)b * R + r, rows[o0 + t]);
         d->x -= tot.x; d->y -= tot.y;
       }
     }
@@ -1962,9 +2001,12 @@ __global__ void __launch_bounds__(256) k_residual_t(int n, int B, int Bp, const 
 // index made wave-uniform (readfirstlane) they arrive by scalar loads, each used for the wave's
 // FT x 64 frequencies (per-lane vector loads of them bound the kernel by address processing).
 // Per row the products are summed in the order of k_residual_t, bit for bit the same result.
+// Columns: B logical columns c (tiles of 64) stored at column cmap[c] (or c) of the [n][Bp]
+// arrays; omega is indexed by the stored column.  Lanes past B read and write nothing.
 constexpr int RES_G = 4;
 template <int FT>
-__global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, const int* __restrict__ gptr,
+__global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, const int* __restrict__ cmap,
+                                                     const int* __restrict__ gptr,
                                                      const int* __restrict__ grows, const int64_t* __restrict__ indptr,
                                                      const int* __restrict__ indices, const int64_t* __restrict__ tslot,
                                                      const double2* __restrict__ Kaa, const double2* __restrict__ Maa,
@@ -1976,12 +2018,14 @@ __global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, cons
   const int g = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
   if (g >= ng) return;
   const int t0 = blockIdx.y * FT;
-  const int ft = min(FT, Bp / 64 - t0);
+  const int ft = min(FT, (B + 63) / 64 - t0);
   double w2[FT], wb[FT];
+  int col[FT];          // stored column of this lane in tile f, or -1
 #pragma unroll
   for (int f = 0; f < FT; ++f) {
-    const int b = (t0 + f) * 64 + lane;
-    const double w = (f < ft && b < B) ? omega[b] : 0.0;
+    const int c = (t0 + f) * 64 + lane;
+    col[f] = (f < ft && c < B) ? (cmap ? cmap[c] : c) : -1;
+    const double w = col[f] >= 0 ? omega[col[f]] : 0.0;
     w2[f] = w * w;
     wb[f] = w * beta;
   }
@@ -2000,7 +2044,7 @@ __global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, cons
     const int64_t j = cols[k];
     double2 xv[FT];
 #pragma unroll
-    for (int f = 0; f < FT; ++f) xv[f] = f < ft ? xt[j * Bp + (t0 + f) * 64 + lane] : make_double2(0, 0);
+    for (int f = 0; f < FT; ++f) xv[f] = col[f] >= 0 ? xt[j * Bp + col[f]] : make_double2(0, 0);
 #pragma unroll
     for (int t = 0; t < RES_G; ++t) {
       if (t < cnt) {
@@ -2021,8 +2065,8 @@ __global__ void __launch_bounds__(256) k_residual_sv(int ng, int B, int Bp, cons
       const int64_t i = grows[g0 + t];
 #pragma unroll
       for (int f = 0; f < FT; ++f) {
-        if (f < ft) {
-          const int64_t o = i * Bp + (t0 + f) * 64 + lane;
+        if (col[f] >= 0) {
+          const int64_t o = i * Bp + col[f];
           const double2 r = rhst ? rhst[o] : make_double2(0, 0);
           rt[o] = make_double2(r.x - acc[t][f].x, r.y - acc[t][f].y);
         }
@@ -2128,6 +2172,59 @@ __global__ void __launch_bounds__(WDOT_NT) k_wdot(int n, const double2* __restri
     __syncthreads();
   }
   if (threadIdx.x == 0) out[k] = red[0];
+}
+
+// The same products on vectors stored row-major (entry i of column c at [i ld + c]), in two
+// stages that keep k_charge's / k_wdot's summation order exactly: stage 1 forms, for every
+// column, the 1024 strided partial sums (thread t of the column-major kernels: entries t, t +
+// 1024, ... in order), 64 columns x 16 partial sums per workgroup so that the row loads coalesce
+// over the columns; stage 2 reduces each column's 1024 partial sums by the same tree.
+//   charge: a = kts_a (a vector), c = x at column ccol[k] (or k)
+//   wdot:   a = a at column acol[k] times w (or 1), c = c at column ccol[k]
+template <bool CHARGE>
+__global__ void __launch_bounds__(1024) k_rows_partial(int n, int nc, int64_t ld, const double2* __restrict__ a,
+                                                       const int* __restrict__ acol, const double* __restrict__ w,
+                                                       const double2* __restrict__ c, const int* __restrict__ ccol,
+                                                       double2* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 64 + lane;
+  const int t = blockIdx.y * 16 + (threadIdx.x >> 6);
+  if (k >= nc) return;
+  const int64_t ca = CHARGE ? 0 : (acol ? acol[k] : k);
+  const int64_t cc = ccol ? ccol[k] : k;
+  double2 s = make_double2(0, 0);
+  for (int i = t; i < n; i += 1024) {
+    if (CHARGE) {
+      s = cfma(s, a[i], c[(int64_t)i * ld + cc]);
+    } else {
+      double2 u = a[(int64_t)i * ld + ca];
+      const double wi = w ? w[i] : 1.0;
+      u.x *= wi;
+      u.y *= wi;
+      s = cfma(s, u, c[(int64_t)i * ld + cc]);
+    }
+  }
+  part[(int64_t)t * nc + k] = s;
+}
+
+template <bool CHARGE>
+__global__ void __launch_bounds__(1024) k_rows_reduce(int nc, const double2* __restrict__ part, double2 kts_z,
+                                                      double V, double2* __restrict__ out) {
+  __shared__ double2 red[1024];
+  const int k = blockIdx.x;
+  red[threadIdx.x] = part[(int64_t)threadIdx.x * nc + k];
+  __syncthreads();
+  for (int h = 512; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[threadIdx.x].x += red[threadIdx.x + h].x;
+      red[threadIdx.x].y += red[threadIdx.x + h].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    if (CHARGE) out[k] = make_double2(red[0].x + V * kts_z.x, red[0].y + V * kts_z.y);
+    else out[k] = red[0];
+  }
 }
 
 __global__ void k_axpy(int64_t n, double2* __restrict__ x, const double2* __restrict__ d) {
@@ -2240,6 +2337,9 @@ struct Plan {
   // independent size buckets of a level run on side streams (default priority) next to the
   // launch stream: fork after the level's assembly, join before the next level
   static constexpr int NSIDE = 2;
+  // grouped solves: the partial last group's launches, concurrent with the full groups'
+  hipStream_t tail_stream = nullptr;
+  hipEvent_t tail_fork = nullptr, tail_join = nullptr;
   bool overlap = true;
   hipStream_t side[NSIDE] = {};
   hipEvent_t fork_ev = nullptr, join_ev[NSIDE] = {};
@@ -2336,6 +2436,12 @@ int join_side(Plan* P, hipStream_t s) {
     SAW_HIP(hipEventRecord(P->join_ev[k], P->side[k]));
     SAW_HIP(hipStreamWaitEvent(s, P->join_ev[k], 0));
   }
+  return 0;
+}
+int ensure_tail(Plan* P) {
+  if (!P->tail_stream) SAW_HIP(hipStreamCreateWithFlags(&P->tail_stream, hipStreamNonBlocking));
+  if (!P->tail_fork) SAW_HIP(hipEventCreateWithFlags(&P->tail_fork, hipEventDisableTiming));
+  if (!P->tail_join) SAW_HIP(hipEventCreateWithFlags(&P->tail_join, hipEventDisableTiming));
   return 0;
 }
 
@@ -2672,6 +2778,7 @@ struct SolveCall {
   bool tail;
   int rc_max;       // k_grp_lds
   int rc_tri;       // the pivot-triangle, panel and k_tri_lds kernels
+  int64_t ldx = 0;  // 0: x as [B][R][n]; else x as [n][ldx] (see FrontsDev::xld)
 };
 
 // One launch of a solve kernel over a set of vector blocks: block size rc (the kernels' R), the
@@ -2682,6 +2789,7 @@ struct SolveSet {
   FrontsDev F;
   const int* sel;
   int ny;
+  bool tail = false;   // the partial last group of a grouped call
 };
 
 // The launch sets of a kernel whose LDS need for blocks of rc vectors is lds_of(rc): groups
@@ -2692,6 +2800,7 @@ template <typename LdsOf>
 int solve_sets(const Plan* P, const SolveCall& C, int rc_max, LdsOf&& lds_of, SolveSet (&out)[2]) {
   FrontsDev F = P->F;
   F.nvec = (int64_t)C.B * C.R;
+  F.xld = C.ldx;
   if (C.G <= 0) {
     F.fslot = C.fslot;
     F.Bf = C.Bf;
@@ -2715,7 +2824,7 @@ int solve_sets(const Plan* P, const SolveCall& C, int rc_max, LdsOf&& lds_of, So
     FrontsDev Ft = F;
     Ft.vdiv = C.G;
     Ft.boff = nfull * C.G;
-    out[k++] = SolveSet{C.R, Ft, nullptr, ntail};
+    out[k++] = SolveSet{C.R, Ft, nullptr, ntail, true};
   }
   return k;
 }
@@ -2750,12 +2859,14 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
   double2* X = (double2*)x;
   const int B = C.B;
   bool bad = false;           // a block size without kernels (not reached: rc is R or R 2^k <= 16)
+  hipStream_t tail_s = nullptr;   // the partial last group's stream (grouped calls), see below
   // fronts of the fused set (m <= the small-front threshold): one workgroup does the whole
   // front; the others: pivot-triangle kernel + panel kernel
   // forward and backward passes take the one-workgroup set of the solve partition (the
   // transposed passes keep the factorization's partition)
   // pivot-triangle kernel (with FULL the whole front) on one launch set
   auto tri_launch = [&](const Level& L, const Sub& S, int kind, bool full, hipStream_t st, const SolveSet& Q) {
+    if (Q.tail && tail_s) st = tail_s;
     const int nbl = panel_width(nb, L);
     const int ps = nbl <= 32 ? 32 : 64;     // partial sums: 4 x ps per vector
     // 64-wide panels (levels of few large fronts): 512 threads share the row-panel updates
@@ -2872,20 +2983,22 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
         }
       }
       if constexpr (sizeof(T) == 4) {
-#define SAW_GRP(K, RCV)                                                                                     \
+#define SAW_GRP_X(K, RCV, XRV)                                                                              \
   do {                                                                                                      \
     if (lds > 64 * 1024)                                                                                    \
-      (void)hipFuncSetAttribute((const void*)k_grp_lds<K, RCV>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                (int)lds);                                                                  \
-    hipLaunchKernelGGL((k_grp_lds<K, RCV>), grid, dim3(256), lds, st, Q.F, S.fronts, B, P->n, nbl, A, U, X,   \
-                       Q.sel);                                                                              \
+      (void)hipFuncSetAttribute((const void*)k_grp_lds<K, RCV, false, XRV>,                                 \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                      \
+    hipLaunchKernelGGL((k_grp_lds<K, RCV, false, XRV>), grid, dim3(256), lds, st, Q.F, S.fronts, B, P->n, nbl, \
+                       A, U, X, Q.sel);                                                                     \
   } while (0)
+#define SAW_GRP(K, RCV) do { if (Q.F.xld) SAW_GRP_X(K, RCV, true); else SAW_GRP_X(K, RCV, false); } while (0)
         if (kind == 0) {
           if (Q.rc == 4) SAW_GRP(0, 4); else if (Q.rc == 8) SAW_GRP(0, 8); else SAW_GRP(0, 16);
         } else {
           if (Q.rc == 4) SAW_GRP(1, 4); else if (Q.rc == 8) SAW_GRP(1, 8); else SAW_GRP(1, 16);
         }
 #undef SAW_GRP
+#undef SAW_GRP_X
       }
       if (P->profile && !P->grp_marks.empty() && P->grp_marks.back().i0 == gi0 && gi0 + 1 < P->ev_pool.size())
         (void)hipEventRecord(P->ev_pool[gi0 + 1], st);
@@ -2896,6 +3009,7 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
   };
   // off-diagonal panel kernel on one launch set
   auto panel_launch = [&](const Sub& S, int kind, hipStream_t st, const SolveSet& Q) {
+    if (Q.tail && tail_s) st = tail_s;
     const bool out_cb = (kind == 0 || kind == 2);
     const int nt = out_cb ? S.nstiles_cb : S.nstiles_p;
     const int* tl = out_cb ? S.stiles_cb : S.stiles_p;
@@ -2953,20 +3067,22 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
         const size_t lds = grp_big_lds_of(S, Q.rc);
         const int ntile = kind == 0 ? S.nstiles_cb : S.nstiles_p;
         const int* tl = kind == 0 ? S.stiles_cb : S.stiles_p;
-#define SAW_GRP_PIV(K, RCV)                                                                                       \
+#define SAW_GRP_PIV_X(K, RCV, XRV)                                                                                \
   do {                                                                                                            \
     if (lds > 64 * 1024)                                                                                          \
-      (void)hipFuncSetAttribute((const void*)k_grp_lds<K, RCV, true>, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                (int)lds);                                                                        \
-    hipLaunchKernelGGL((k_grp_lds<K, RCV, true>), dim3(S.nf, Q.ny), dim3(256), lds, st, Q.F, S.fronts, B, P->n,   \
-                       nbl, A, U, X, Q.sel);                                                                      \
+      (void)hipFuncSetAttribute((const void*)k_grp_lds<K, RCV, true, XRV>,                                        \
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                            \
+    hipLaunchKernelGGL((k_grp_lds<K, RCV, true, XRV>), dim3(S.nf, Q.ny), dim3(256), lds, st, Q.F, S.fronts, B,    \
+                       P->n, nbl, A, U, X, Q.sel);                                                                \
   } while (0)
-#define SAW_GRP_PAN(K, RCV)                                                                                       \
+#define SAW_GRP_PAN_X(K, RCV, XRV)                                                                                \
   do {                                                                                                            \
     if (ntile > 0)                                                                                                \
-      hipLaunchKernelGGL((k_grp_panel<K, RCV>), dim3(ntile, Q.ny), dim3(16 * RCV), sizeof(double2) * GRP_VC *     \
+      hipLaunchKernelGGL((k_grp_panel<K, RCV, XRV>), dim3(ntile, Q.ny), dim3(16 * RCV), sizeof(double2) * GRP_VC * \
                          (RCV + 1), st, Q.F, tl, B, P->n, A, U, X, Q.sel);                                        \
   } while (0)
+#define SAW_GRP_PIV(K, RCV) do { if (Q.F.xld) SAW_GRP_PIV_X(K, RCV, true); else SAW_GRP_PIV_X(K, RCV, false); } while (0)
+#define SAW_GRP_PAN(K, RCV) do { if (Q.F.xld) SAW_GRP_PAN_X(K, RCV, true); else SAW_GRP_PAN_X(K, RCV, false); } while (0)
         if (kind == 0) {
           if (Q.rc == 4) { SAW_GRP_PIV(0, 4); SAW_GRP_PAN(0, 4); }
           else if (Q.rc == 8) { SAW_GRP_PIV(0, 8); SAW_GRP_PAN(0, 8); }
@@ -2978,6 +3094,8 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
         }
 #undef SAW_GRP_PIV
 #undef SAW_GRP_PAN
+#undef SAW_GRP_PIV_X
+#undef SAW_GRP_PAN_X
       }
     }
   };
@@ -3001,6 +3119,7 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
     const int ns = solve_sets(P, C, C.rc_tri, [&](int rc) { return lds_bytes(S, rc); }, sets);
     for (int k = 0; k < ns; ++k) {
       const SolveSet& Q = sets[k];
+      const hipStream_t sq = (Q.tail && tail_s) ? tail_s : st;
       const size_t lds = lds_bytes(S, Q.rc);
       // persistent grid: as many workgroups as the CUs hold at this LDS size (at most 8 a CU)
       const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(8, (size_t)P->lds_per_cu / lds));
@@ -3012,18 +3131,27 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
           if (lds > 64 * 1024)
             (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 0, RC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds);
-          hipLaunchKernelGGL((k_tri_lds<T, 0, RC>), dim3(grid), dim3(256), lds, st, Q.F, S.fronts, S.nf, B, P->n,
+          hipLaunchKernelGGL((k_tri_lds<T, 0, RC>), dim3(grid), dim3(256), lds, sq, Q.F, S.fronts, S.nf, B, P->n,
                              nbl, A, U, X, Q.sel, Q.ny);
         } else {
           if (lds > 64 * 1024)
             (void)hipFuncSetAttribute((const void*)k_tri_lds<T, 1, RC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                       (int)lds);
-          hipLaunchKernelGGL((k_tri_lds<T, 1, RC>), dim3(grid), dim3(256), lds, st, Q.F, S.fronts, S.nf, B, P->n,
+          hipLaunchKernelGGL((k_tri_lds<T, 1, RC>), dim3(grid), dim3(256), lds, sq, Q.F, S.fronts, S.nf, B, P->n,
                              nbl, A, U, X, Q.sel, Q.ny);
         }
       });
     }
   };
+  // A grouped call's partial last group (its own vectors, update vectors and factor slot) runs on
+  // a stream of its own, concurrently with the full groups: forked here, joined at the end.  Its
+  // launches keep their order on that stream, which is all they depend on.
+  if (C.G > 0 && C.tail && B % C.G != 0 && P->overlap) {
+    if (ensure_tail(P)) return 3;
+    SAW_HIP(hipEventRecord(P->tail_fork, s));
+    SAW_HIP(hipStreamWaitEvent(P->tail_stream, P->tail_fork, 0));
+    tail_s = P->tail_stream;
+  }
   for (size_t li = 0; li < P->levels.size(); ++li) {
     const Level& L = P->levels[li];
     P->cur_level = (int)li;
@@ -3086,6 +3214,10 @@ int solve_impl(Plan* P, hipStream_t s, const SolveCall& C, const void* arena, vo
     }
     if (full) tri_sub(L, L.small[pi], kb, true, s);
     if (fork && join_side(P, s)) return 3;
+  }
+  if (tail_s) {
+    SAW_HIP(hipEventRecord(P->tail_join, tail_s));
+    SAW_HIP(hipStreamWaitEvent(s, P->tail_join, 0));
   }
   if (bad) { sawfem_set_error("sawfem_mf_solve: no kernels for the block size"); return 1; }
   SAW_HIP(hipGetLastError());
@@ -3256,12 +3388,15 @@ int sawfem_mf_create(void** out, int n, int nf, const int* m, const int* p, cons
 int sawfem_mf_destroy(void* plan) {
   Plan* P = (Plan*)plan;
   if (!P) return 0;
-  if (P->fork_ev || P->side[0]) (void)hipDeviceSynchronize();
+  if (P->fork_ev || P->side[0] || P->tail_stream) (void)hipDeviceSynchronize();
   for (int k = 0; k < Plan::NSIDE; ++k) {
     if (P->side[k]) (void)hipStreamDestroy(P->side[k]);
     if (P->join_ev[k]) (void)hipEventDestroy(P->join_ev[k]);
   }
   if (P->fork_ev) (void)hipEventDestroy(P->fork_ev);
+  if (P->tail_stream) (void)hipStreamDestroy(P->tail_stream);
+  if (P->tail_fork) (void)hipEventDestroy(P->tail_fork);
+  if (P->tail_join) (void)hipEventDestroy(P->tail_join);
   for (void* d : P->owned) (void)hipFree(d);
   for (int pi = 0; pi < 2; ++pi) {
     for (void* d : P->part_owned[pi]) (void)hipFree(d);
@@ -3471,6 +3606,27 @@ int sawfem_mf_solve_groups(void* plan, void* stream, int B, int nrhs, const void
   return solve_impl<float>(P, (hipStream_t)stream, C, arena, uarena, x, transpose, nb);
 }
 
+// sawfem_mf_solve_groups on vectors stored row-major: entry (row i, vector b nrhs + r) of x at
+// x[i ldx + b nrhs + r], ldx >= B nrhs; forward + backward solves.  The update vectors in uarena
+// are stored row-major too, so a call's uarena is not shared with a vector-major call's.
+int sawfem_mf_solve_groups_rows(void* plan, void* stream, int B, int nrhs, const void* arena, int G, void* uarena,
+                                void* x, int64_t ldx, int nb, const int* gsel, int ngsel, int tail) {
+  Plan* P = (Plan*)plan;
+  const int nfull = G > 0 ? B / G : 0;
+  if (!gsel) ngsel = nfull;
+  if (!P || B <= 0 || G <= 0 || !arena || !uarena || !x || (nb != 32 && nb != 64) || (nrhs != 1 && nrhs != 2) ||
+      ngsel < 0 || ngsel > nfull || (tail != 0 && tail != 1) || ldx < (int64_t)B * nrhs) {
+    sawfem_set_error("sawfem_mf_solve_groups_rows: invalid arguments");
+    return 1;
+  }
+  if (solve_lds_check(P, nrhs)) return 1;
+  const bool do_tail = tail && B % G != 0;
+  if (ngsel == 0 && !do_tail) return 0;
+  SolveCall C{B, nrhs, G, nullptr, 0, gsel, ngsel, do_tail, P->group_rc, P->group_rc_tri};
+  C.ldx = ldx;
+  return solve_impl<float>(P, (hipStream_t)stream, C, arena, uarena, x, 0, nb);
+}
+
 int sawfem_mf_set_group_rc(void* plan, int rc_max) {
   Plan* P = (Plan*)plan;
   if (!P || rc_max < 1) { sawfem_set_error("sawfem_mf_set_group_rc: invalid arguments"); return -1; }
@@ -3591,21 +3747,22 @@ int sawfem_residual_t(void* stream, int n, int B, int Bp, const int64_t* indptr,
   return 0;
 }
 
-int sawfem_residual_sv(void* stream, int ng, int B, int Bp, const int* gptr, const int* grows, const int64_t* indptr,
-                       const int* indices, const int64_t* tslot, const void* Kaa, const void* Maa, const void* Daa,
-                       const double* omega, double beta, const void* xt, const void* rhst, void* rt) {
-  if (ng <= 0 || B <= 0 || Bp < B || Bp % 64 || !gptr || !grows) {
+int sawfem_residual_sv(void* stream, int ng, int B, int Bp, const int* cmap, const int* gptr, const int* grows,
+                       const int64_t* indptr, const int* indices, const int64_t* tslot, const void* Kaa,
+                       const void* Maa, const void* Daa, const double* omega, double beta, const void* xt,
+                       const void* rhst, void* rt) {
+  if (ng <= 0 || B <= 0 || Bp < 1 || (!cmap && Bp < B) || !gptr || !grows) {
     sawfem_set_error("sawfem_residual_sv: invalid arguments");
     return 1;
   }
-  // tiles of 64 frequencies per wave: all of them up to 8, else columns of 8 (the last one shorter)
-  const int nt = Bp / 64;
+  // tiles of 64 columns per wave: all of them up to 8, else columns of 8 (the last one shorter)
+  const int nt = (B + 63) / 64;
   const int ft = nt >= 8 ? 8 : (nt >= 4 ? 4 : (nt >= 2 ? 2 : 1));
   const dim3 grid((ng + 3) / 4, (nt + ft - 1) / ft);
 #define SAW_RES(F)                                                                                                 \
-  hipLaunchKernelGGL(k_residual_sv<F>, grid, dim3(256), 0, (hipStream_t)stream, ng, B, Bp, gptr, grows, indptr,     \
-                     indices, tslot, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, omega, beta,    \
-                     (const double2*)xt, (const double2*)rhst, (double2*)rt)
+  hipLaunchKernelGGL(k_residual_sv<F>, grid, dim3(256), 0, (hipStream_t)stream, ng, B, Bp, cmap, gptr, grows,      \
+                     indptr, indices, tslot, (const double2*)Kaa, (const double2*)Maa, (const double2*)Daa, omega,  \
+                     beta, (const double2*)xt, (const double2*)rhst, (double2*)rt)
   switch (ft) { case 8: SAW_RES(8); break; case 4: SAW_RES(4); break; case 2: SAW_RES(2); break; default: SAW_RES(1); }
 #undef SAW_RES
   SAW_HIP(hipGetLastError());
@@ -3653,6 +3810,36 @@ int sawfem_wdot(void* stream, int n, int nb, const void* a, int64_t sa, const do
   if (nb == 0) return 0;
   hipLaunchKernelGGL(k_wdot, dim3(nb), dim3(WDOT_NT), 0, (hipStream_t)stream, n, (const double2*)a, sa, w,
                      (const double2*)c, sc, sel, (double2*)out);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_charge_rows(void* stream, int n, int nc, int64_t ld, const void* kts_a, double kz_re, double kz_im, double V,
+                       const void* x, const int* cols, void* work, void* Q) {
+  if (n <= 0 || nc <= 0 || ld < 1 || (!cols && ld < nc) || !kts_a || !x || !work || !Q) {
+    sawfem_set_error("sawfem_charge_rows: invalid arguments");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_rows_partial<true>, dim3((nc + 63) / 64, 64), dim3(1024), 0, s, n, nc, ld,
+                     (const double2*)kts_a, nullptr, nullptr, (const double2*)x, cols, (double2*)work);
+  hipLaunchKernelGGL(k_rows_reduce<true>, dim3(nc), dim3(1024), 0, s, nc, (const double2*)work,
+                     make_double2(kz_re, kz_im), V, (double2*)Q);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_wdot_rows(void* stream, int n, int nc, int64_t ld, const void* a, const int* acol, const double* w,
+                     const void* c, const int* ccol, void* work, void* out) {
+  if (n <= 0 || nc <= 0 || ld < 1 || ((!acol || !ccol) && ld < nc) || !a || !c || !work || !out) {
+    sawfem_set_error("sawfem_wdot_rows: invalid arguments");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_rows_partial<false>, dim3((nc + 63) / 64, 64), dim3(1024), 0, s, n, nc, ld,
+                     (const double2*)a, acol, w, (const double2*)c, ccol, (double2*)work);
+  hipLaunchKernelGGL(k_rows_reduce<false>, dim3(nc), dim3(1024), 0, s, nc, (const double2*)work,
+                     make_double2(0, 0), 0.0, (double2*)out);
   SAW_HIP(hipGetLastError());
   return 0;
 }

@@ -124,17 +124,21 @@ class _ResidualOp:
     frequency-major layout: x and rhs are transposed to [n][Bp] (Bp = B padded to 64), the
     residual kernel reads each A_aa entry once for all frequencies and each x row once per
     row group (the active unknowns of one merged node share their column list), r is
-    transposed back.
+    transposed back.  With ``ld`` the operands are already row-major [n][ld] (the grouped
+    refinement keeps its vectors so) and nothing is transposed.
 
     A module-level class on purpose: the [n][Bp] buffers must be freed when the chunk's solve
     returns (a class defined inside a method sits in a reference cycle, and its buffers would
     wait for Python's cyclic garbage collector)."""
 
-    def __init__(self, solver, B, om):
+    def __init__(self, solver, B, om, ld=0):
         torch = solver.torch
-        self.solver, self.B, self.om = solver, B, om
+        self.solver, self.B, self.om, self.ld = solver, B, om, ld
         n = solver.n_active
         self.Bp = -(-B // 64) * 64
+        if ld:                 # x, rhs and r already row-major [n][ld]: no staging buffers
+            self.xt = self.rt = None
+            return
         self.xt = torch.empty((n, self.Bp), dtype=torch.complex128, device=solver.dev)
         self.rt = torch.empty((n, self.Bp), dtype=torch.complex128, device=solver.dev)
 
@@ -154,11 +158,24 @@ class _ResidualOp:
     def __call__(self, x, rhst, r, tslot, active=None, per=1):
         """r = rhs - A x.  ``active`` (bool per frequency, ``per`` vectors each): when fewer than
         COMPACT of them are active, only their vectors are gathered, evaluated and scattered back
-        (the rows of the other vectors of r are left as they are)."""
+        (the rows of the other vectors of r are left as they are).  Row-major operands (``ld``)
+        are read and written in place, the active vectors through a column map."""
         sv = self.solver
         n, B, Bp = sv.n_active, self.B, self.Bp
         s = stream_ptr()
         gptr, grows, ng = sv.res_groups
+        if self.ld:
+            cmap, nc = None, B
+            if active is not None and active.sum() * per < self.COMPACT * B:
+                vec = (np.flatnonzero(active)[:, None] * per + np.arange(per)).ravel().astype(np.int32)
+                if vec.size == 0:
+                    return
+                cmap, nc = sv.torch.as_tensor(vec, device=sv.dev), int(vec.size)
+            _lib.call('sawfem_residual_sv', s, ng, nc, self.ld, None if cmap is None else cmap.data_ptr(),
+                      gptr.data_ptr(), grows.data_ptr(), sv.aa_indptr.data_ptr(), sv.aa_indices.data_ptr(),
+                      sv._dptr(tslot), sv.Kaa.data_ptr(), sv.Maa.data_ptr(), sv._dptr(sv.Daa), self.om.data_ptr(),
+                      sv.beta, x.data_ptr(), rhst.data_ptr(), r.data_ptr())
+            return
         if active is not None and active.sum() * per < self.COMPACT * B:
             torch = sv.torch
             vec = (np.flatnonzero(active)[:, None] * per + np.arange(per)).ravel().astype(np.int32)
@@ -172,7 +189,7 @@ class _ResidualOp:
             om = self.om[sel.long()].contiguous()
             _lib.call('sawfem_transpose_map', s, nsel, n, n, Bc, x.data_ptr(), xt.data_ptr(), sel.data_ptr(), None,
                       None)
-            _lib.call('sawfem_residual_sv', s, ng, nsel, Bc, gptr.data_ptr(), grows.data_ptr(),
+            _lib.call('sawfem_residual_sv', s, ng, nsel, Bc, None, gptr.data_ptr(), grows.data_ptr(),
                       sv.aa_indptr.data_ptr(), sv.aa_indices.data_ptr(), sv._dptr(tslot), sv.Kaa.data_ptr(),
                       sv.Maa.data_ptr(), sv._dptr(sv.Daa), om.data_ptr(), sv.beta, xt.data_ptr(), None,
                       rt.data_ptr())
@@ -180,7 +197,7 @@ class _ResidualOp:
                       self.rhs.data_ptr())
             return
         _lib.call('sawfem_transpose', s, B, n, n, Bp, x.data_ptr(), self.xt.data_ptr())
-        _lib.call('sawfem_residual_sv', s, ng, B, Bp, gptr.data_ptr(), grows.data_ptr(), sv.aa_indptr.data_ptr(),
+        _lib.call('sawfem_residual_sv', s, ng, B, Bp, None, gptr.data_ptr(), grows.data_ptr(), sv.aa_indptr.data_ptr(),
                   sv.aa_indices.data_ptr(), sv._dptr(tslot), sv.Kaa.data_ptr(), sv.Maa.data_ptr(),
                   sv._dptr(sv.Daa), self.om.data_ptr(), sv.beta, self.xt.data_ptr(), rhst.data_ptr(),
                   self.rt.data_ptr())
@@ -627,14 +644,15 @@ class FrequencySolver:
         self.share = k
         return k
 
-    def _solve(self, prec, B, R, arena, uarena, vec, transpose, nb, active=None):
+    def _solve(self, prec, B, R, arena, uarena, vec, transpose, nb, active=None, ld=0):
         """Forward + backward solve of the B x R right-hand sides ``vec`` in place with the chunk's
         factors; ``active`` (bool [B], or None for all) selects the frequencies to solve.
 
         When ``solve_batch`` factored groups of G frequencies (complex64), the vectors of a group
         are solved together (``sawfem_mf_solve_groups``): a group is solved when any of its
         frequencies is active, and the others of the group must hold zero right-hand sides
-        (their solution is then zero)."""
+        (their solution is then zero).  ``ld``: ``vec`` is row-major [n][ld] (grouped solves only,
+        ``sawfem_mf_solve_groups_rows``), else [B][R][n]."""
         G = self._group
         idx = None
         if G > 1:
@@ -645,10 +663,19 @@ class FrequencySolver:
                 np.logical_or.at(ga, np.arange(B) // G, active)
                 tail = int(ga[nfull]) if B % G else 0
                 idx = self.torch.as_tensor(np.flatnonzero(ga[:nfull]).astype(np.int32), device=self.dev)
+            if ld:
+                if transpose:
+                    raise ValueError('row-major vectors are solved without transposition')
+                _lib.call('sawfem_mf_solve_groups_rows', self._plan, stream_ptr(), B, R, arena.data_ptr(), G,
+                          uarena.data_ptr(), vec.data_ptr(), ld, nb, None if idx is None else idx.data_ptr(),
+                          0 if idx is None else int(idx.numel()), tail)
+                return
             _lib.call('sawfem_mf_solve_groups', self._plan, stream_ptr(), B, R, arena.data_ptr(), G,
                       uarena.data_ptr(), vec.data_ptr(), transpose, nb, None if idx is None else idx.data_ptr(),
                       0 if idx is None else int(idx.numel()), tail)
             return
+        if ld:
+            raise ValueError('row-major vectors need grouped factorizations')
         if active is not None:
             idx = self.torch.as_tensor(np.flatnonzero(active).astype(np.int32), device=self.dev)
         _lib.call('sawfem_mf_solve', self._plan, stream_ptr(), prec, B, R, arena.data_ptr(), uarena.data_ptr(),
@@ -663,8 +690,11 @@ class FrequencySolver:
         _lib.call('sawfem_mf_pivot_stats', self._plan, int(reset), C.byref(cnt), C.byref(rmin))
         return int(cnt.value), float(rmin.value)
 
-    def _refine(self, prec, nb, arena, uarena, X, transpose, res, rhst, tslot, measure, ir_tol, max_ir, dwr=None):
-        """Iterative refinement of X ([B][R][n], holding the first solve) in place.
+    def _refine(self, prec, nb, arena, uarena, X, transpose, res, rhst, tslot, measure, ir_tol, max_ir, dwr=None,
+                rows=None):
+        """Iterative refinement of X ([B][R][n], holding the first solve) in place; with
+        ``rows=(B, R, ld)`` X is row-major [n][ld] (vector b R + r in column b R + r; grouped
+        solves only), and so are rhst and the residuals.
 
         Each step: FP64 residual of every frequency, a solve of the frequencies still active,
         update.  A frequency converges when the relative change of every quantity ``measure()``
@@ -688,14 +718,17 @@ class FrequencySolver:
         """
         torch = self.torch
         s = stream_ptr()
-        B = X.shape[0]
-        R = X.shape[1] if X.dim() == 3 else 1
+        if rows:
+            B, R, ld = rows
+        else:
+            B, ld = X.shape[0], 0
+            R = X.shape[1] if X.dim() == 3 else 1
         vals = measure()
         dqs = [np.full(B, np.inf) for _ in vals]
         errs = [np.full(B, np.inf) for _ in vals]
         active = np.ones(B, dtype=bool)
         dwr_ok = np.zeros(B, dtype=bool)
-        r = torch.empty_like(X)
+        r = torch.zeros_like(X) if ld else torch.empty_like(X)     # row-major: padding columns stay 0
         hist, applied = [], 0
         self._solved = [B]                 # frequencies solved per pass (first solve, then each step)
         self._stalled = np.zeros(B, dtype=bool)
@@ -713,9 +746,13 @@ class FrequencySolver:
             self._solved.append(int(active.sum()))
             sel = None
             if not active.all():
-                r[torch.as_tensor(~active, device=self.dev)] = 0
+                if ld:
+                    cols = (np.flatnonzero(~active)[:, None] * R + np.arange(R)).ravel()
+                    r[:, torch.as_tensor(cols, device=self.dev)] = 0
+                else:
+                    r[torch.as_tensor(~active, device=self.dev)] = 0
                 sel = active
-            self._solve(prec, B, R, arena, uarena, r, transpose, nb, sel)
+            self._solve(prec, B, R, arena, uarena, r, transpose, nb, sel, ld=ld)
             _lib.call('sawfem_axpy', s, X.numel(), X.data_ptr(), r.data_ptr())
             applied = steps
             new = measure()
@@ -770,6 +807,8 @@ class FrequencySolver:
         s = stream_ptr()
         B, n = rhs.shape
         d = self._dscale
+        if self._group > 1:
+            return self._solve_paired_rows(om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir)
         X = torch.stack([rhs, (self.kts_a / d).unsqueeze(0).expand(B, n)], dim=1).contiguous()   # [B][2][n]
         rhs2 = X.clone()
         self._solve(prec, B, 2, arena, uarena, X, 0, nb)
@@ -799,6 +838,59 @@ class FrequencySolver:
         x = X[:, 0].contiguous()
         lam = (X[:, 1] * d).contiguous()
         del X, rhs2
+        return self._paired_out(omegas, V, x, lam, Q, steps, dq, err, hist, dm, aerr)
+
+    def _solve_paired_rows(self, om, omegas, V, prec, nb, arena, uarena, rhs, ir_tol, max_ir):
+        """:meth:`_solve_paired` with grouped factorizations, its vectors kept row-major for the
+        whole refinement: column 2b holds frequency b's forward vector, column 2b + 1 its adjoint
+        (X: [n][ld], ld = 2B padded to 64).  The grouped solves then gather whole lines, and the
+        residual reads and writes X and r in place (no transposes).  The charge and duality
+        products run in the summation order of the [B][2][n] path, so every result is the same
+        bit for bit."""
+        torch = self.torch
+        s = stream_ptr()
+        B, n = rhs.shape
+        d = self._dscale
+        nv = 2 * B
+        ld = -(-nv // 64) * 64
+        c128 = dict(dtype=torch.complex128, device=self.dev)
+        X = torch.zeros((n, ld), **c128)
+        X[:, 0:nv:2] = rhs.t()
+        X[:, 1:nv:2] = (self.kts_a / d).unsqueeze(1)
+        rhs2 = X.clone()
+        self._solve(prec, B, 2, arena, uarena, X, 0, nb, ld=ld)
+        Qd = torch.empty(nv, **c128)
+        w3 = torch.empty((3, B), **c128)
+        work = torch.empty(1024 * nv, **c128)            # partial sums of the row-major products
+        even = torch.arange(0, nv, 2, dtype=torch.int32, device=self.dev)
+        odd = even + 1
+
+        def wdot(acol, c, ccol, out):
+            # out[b] = sum_i X[i][acol[b]] d[i] c[i][ccol[b]]
+            _lib.call('sawfem_wdot_rows', s, n, B, ld, X.data_ptr(), acol.data_ptr(), d.data_ptr(), c.data_ptr(),
+                      ccol.data_ptr(), work.data_ptr(), out.data_ptr())
+
+        def measure():
+            _lib.call('sawfem_charge_rows', s, n, nv, ld, self.kts_a.data_ptr(), self.kts_z.real, self.kts_z.imag,
+                      float(V), X.data_ptr(), None, work.data_ptr(), Qd.data_ptr())
+            wdot(odd, rhs2, even, w3[0])                                          # mu = (D lambda)^T b
+            return [Qd[0::2].cpu().numpy().copy(), w3[0].cpu().numpy()]
+        def dwr(r):
+            # Q - Q* = -lambda^T r_x (forward residual); mu - mu* = -(D x)^T r_z (adjoint residual)
+            wdot(odd, r, even, w3[1])
+            wdot(even, r, odd, w3[2])
+            e = w3[1:].cpu().numpy()
+            return [e[0], e[1]]
+        res = self._residual_op(nv, om.repeat_interleave(2), ld=ld)
+        (Q, mu), (dq, dm), (err, aerr), steps, hist = self._refine(prec, nb, arena, uarena, X, 0, res, rhs2, None,
+                                                                  measure, ir_tol, max_ir, dwr=dwr, rows=(B, 2, ld))
+        x = X[:, 0:nv:2].t().contiguous()
+        lam = (X[:, 1:nv:2].t() * d).contiguous()
+        del X, rhs2, work
+        return self._paired_out(omegas, V, x, lam, Q, steps, dq, err, hist, dm, aerr)
+
+    def _paired_out(self, omegas, V, x, lam, Q, steps, dq, err, hist, dm, aerr):
+        """The paired solve's result: the refinement record, dY11/dp from x and lambda ([B][n])."""
         out = dict(x=x, Q=Q, ir_steps=steps, dq=dq, err=err, dq_hist=hist, solved=list(self._solved),
                    converged=self._converged.copy(), dwr_accepted=int(self._dwr_ok.sum()),
                    dwr_mask=self._dwr_ok.copy())
@@ -806,9 +898,10 @@ class FrequencySolver:
         out.update(adj_dq=dm, adj_err=aerr, adj_steps=steps, lam=lam, paired=True)
         return out
 
-    def _residual_op(self, B, om):
-        """r = rhs - A(w) x (or A^T with tslot) through the frequency-contiguous residual kernel."""
-        return _ResidualOp(self, B, om)
+    def _residual_op(self, B, om, ld=0):
+        """r = rhs - A(w) x (or A^T with tslot) through the frequency-contiguous residual kernel
+        (row-major operands [n][ld] when ``ld``)."""
+        return _ResidualOp(self, B, om, ld)
 
     def _adjoint(self, om, omegas, V, prec, nb, arena, uarena, rhs, x, ir_tol, max_ir):
         """lambda = A^-T kts_a by transposed solves (refined with the transposed FP64 residual

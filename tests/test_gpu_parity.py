@@ -391,12 +391,24 @@ def test_grouped_residual_matches_the_row_residual(transposed, B):
     _lib.call('sawfem_residual_t', s, n, B, Bp, eng.aa_indptr.data_ptr(), eng.aa_indices.data_ptr(), ts,
               eng.Kaa.data_ptr(), eng.Maa.data_ptr(), eng._dptr(eng.Daa), om.data_ptr(), eng.beta, xt.data_ptr(),
               rhst.data_ptr(), r1.data_ptr())
-    _lib.call('sawfem_residual_sv', s, ng, B, Bp, gptr.data_ptr(), grows.data_ptr(), eng.aa_indptr.data_ptr(),
+    _lib.call('sawfem_residual_sv', s, ng, B, Bp, None, gptr.data_ptr(), grows.data_ptr(), eng.aa_indptr.data_ptr(),
               eng.aa_indices.data_ptr(), ts, eng.Kaa.data_ptr(), eng.Maa.data_ptr(), eng._dptr(eng.Daa),
               om.data_ptr(), eng.beta, xt.data_ptr(), rhst.data_ptr(), r2.data_ptr())
     torch.cuda.synchronize()
     assert torch.equal(r1[:, :B], r2[:, :B])
-    assert torch.equal(r1, r2)        # the padded columns of the tile too (omega 0 there)
+    assert not r2[:, B:].any()        # the padded columns of the last tile are not touched
+    # a column map: a third of the columns in shuffled order, each equal to the full residual's
+    # column, every other column untouched
+    cols = np.random.default_rng(B).permutation(B)[:max(1, B // 3)].astype(np.int32)
+    cmap = torch.as_tensor(cols, device='cuda')
+    r3 = torch.zeros_like(xt)
+    _lib.call('sawfem_residual_sv', s, ng, cols.size, Bp, cmap.data_ptr(), gptr.data_ptr(), grows.data_ptr(),
+              eng.aa_indptr.data_ptr(), eng.aa_indices.data_ptr(), ts, eng.Kaa.data_ptr(), eng.Maa.data_ptr(),
+              eng._dptr(eng.Daa), om.data_ptr(), eng.beta, xt.data_ptr(), rhst.data_ptr(), r3.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(r3[:, cmap.long()], r1[:, cmap.long()])
+    rest = np.setdiff1d(np.arange(Bp), cols)
+    assert not r3[:, torch.as_tensor(rest, device='cuda')].any()
 
 
 def test_weighted_dot_matches_torch():
@@ -423,6 +435,53 @@ def test_weighted_dot_matches_torch():
               out.data_ptr())
     ref = (X[:, 0] * X[:, 1]).sum(dim=1)[sel.long()]
     assert ((out[:3] - ref).abs().max() / ref.abs().max()).item() <= 1e-13
+
+
+def test_row_major_charge_and_weighted_dot_match_bit_for_bit():
+    """sawfem_charge_rows / sawfem_wdot_rows on the vectors of a [B][2][n] array stored row-major
+    ([n][ld], column k = vector k) equal sawfem_charge / sawfem_wdot on the array itself bit for
+    bit (the same summation order), with the identity and with shuffled column maps, weights and
+    no weights."""
+    _say('[test_row_major_charge_and_weighted_dot_match_bit_for_bit]')
+    import torch
+    from sawsim_amd import _lib
+    from sawsim_amd.piezo_fem import stream_ptr
+    g = torch.Generator(device='cpu').manual_seed(8)
+    B, n = 37, 70_001
+    nv, ld = 2 * B, 128
+    cx = lambda *shape: torch.complex(torch.randn(*shape, generator=g, dtype=torch.float64),
+                                      torch.randn(*shape, generator=g, dtype=torch.float64)).cuda()
+    X, C, kts = cx(B, 2, n), cx(B, 2, n), cx(n)
+    w = torch.rand(n, generator=g, dtype=torch.float64).cuda() + 0.5
+    Xr = torch.zeros(n, ld, dtype=torch.complex128, device='cuda')
+    Xr[:, :nv] = X.reshape(nv, n).t()
+    Cr = torch.zeros_like(Xr)
+    Cr[:, :nv] = C.reshape(nv, n).t()
+    work = torch.empty(1024 * nv, dtype=torch.complex128, device='cuda')
+    s = stream_ptr()
+    q1 = torch.empty(nv, dtype=torch.complex128, device='cuda')
+    q2 = torch.empty_like(q1)
+    _lib.call('sawfem_charge', s, n, nv, kts.data_ptr(), 0.25, -1.5, 2.0, X.data_ptr(), q1.data_ptr())
+    _lib.call('sawfem_charge_rows', s, n, nv, ld, kts.data_ptr(), 0.25, -1.5, 2.0, Xr.data_ptr(), None,
+              work.data_ptr(), q2.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(q1, q2)
+    cols = torch.tensor([9, 0, 73, 40], dtype=torch.int32, device='cuda')
+    _lib.call('sawfem_charge_rows', s, n, 4, ld, kts.data_ptr(), 0.25, -1.5, 2.0, Xr.data_ptr(), cols.data_ptr(),
+              work.data_ptr(), q2.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(q1[cols.long()], q2[:4])
+    even = torch.arange(0, nv, 2, dtype=torch.int32, device='cuda')
+    odd = even + 1
+    for wp in (w.data_ptr(), None):
+        o1 = torch.empty(B, dtype=torch.complex128, device='cuda')
+        o2 = torch.empty_like(o1)
+        # a = vector 2b + 1 of X, c = vector 2b of C (strides of the [B][2][n] array)
+        _lib.call('sawfem_wdot', s, n, B, X.data_ptr() + 16 * n, 2 * n, wp, C.data_ptr(), 2 * n, None, o1.data_ptr())
+        _lib.call('sawfem_wdot_rows', s, n, B, ld, Xr.data_ptr(), odd.data_ptr(), wp, Cr.data_ptr(), even.data_ptr(),
+                  work.data_ptr(), o2.data_ptr())
+        torch.cuda.synchronize()
+        assert torch.equal(o1, o2)
 
 
 @pytest.mark.parametrize('transposed', [False, True])

@@ -45,6 +45,19 @@ def test_identity_slot_map_matches_the_plain_solve():
     assert rc == _lib.INVALID
 
 
+def _rows_solve(eng, s, B, R, arena, G, uarena, x0, sel, nsel, tail):
+    """sawfem_mf_solve_groups_rows on a row-major copy of x0 ([B][R][n]), returned as [B][R][n]."""
+    torch = eng.torch
+    n = eng.n_active
+    ld = -(-B * R // 64) * 64 + 64           # padded beyond the vectors: padding must stay untouched
+    xr = torch.full((n, ld), 7.0 + 3.0j, dtype=torch.complex128, device=eng.dev)
+    xr[:, :B * R] = x0.reshape(B * R, n).t()
+    _lib.call('sawfem_mf_solve_groups_rows', eng._plan, s, B, R, arena.data_ptr(), G, uarena.data_ptr(),
+              xr.data_ptr(), ld, 64, sel, nsel, tail)
+    assert (xr[:, B * R:] == 7.0 + 3.0j).all()
+    return xr[:, :B * R].t().reshape(B, R, n)
+
+
 @pytest.mark.parametrize('G', [2, 3, 4, 8])
 def test_grouped_solve_matches_the_slot_map_solve(G):
     """sawfem_mf_solve_groups (the vectors of a group of frequencies in one workgroup) against
@@ -53,7 +66,8 @@ def test_grouped_solve_matches_the_slot_map_solve(G):
     untouched): bit for bit through the pivot-triangle and panel kernels at every block size
     (their per-vector operations do not depend on the block size); with the LDS block kernel
     on the one-workgroup fronts (blocks of 4, 8, 16 vectors; other summation order) within
-    1e-10 of the largest entry of each vector."""
+    1e-10 of the largest entry of each vector.  Every call is repeated on the vectors stored
+    row-major (sawfem_mf_solve_groups_rows): bit for bit the same."""
     _say(f'[test_grouped_solve_matches_the_slot_map_solve] G {G}')
     cell = workloads.config3(mesh_um=0.12, points=6)
     eng = workloads.build_engine(cell)
@@ -84,6 +98,8 @@ def test_grouped_solve_matches_the_slot_map_solve(G):
                 _lib.call('sawfem_mf_solve_groups', eng._plan, s, B, R, arena.data_ptr(), G, uarena.data_ptr(),
                           got.data_ptr(), tr, 64, None, 0, 1)
                 assert torch.equal(got, ref), (R, tr, rc)
+                if tr == 0:
+                    assert torch.equal(_rows_solve(eng, s, B, R, arena, G, uarena, x0, None, 0, 1), got), (R, rc)
             _lib.call_value('sawfem_mf_set_group_tri', eng._plan, 4, 160 * 1024)
             for rc in (4, 8, 16):
                 assert _lib.call_value('sawfem_mf_set_group_rc', eng._plan, rc) == rc
@@ -92,6 +108,8 @@ def test_grouped_solve_matches_the_slot_map_solve(G):
                           got.data_ptr(), tr, 64, None, 0, 1)
                 err = ((got - ref).abs().amax(dim=2) / ref.abs().amax(dim=2)).max().item()
                 assert err <= 1e-10, (R, tr, rc, err)
+                if tr == 0:
+                    assert torch.equal(_rows_solve(eng, s, B, R, arena, G, uarena, x0, None, 0, 1), got), (R, rc)
             _lib.call_value('sawfem_mf_set_group_rc', eng._plan, 16)
             # groups 1 .. nfull-1 selected, the partial group not: group 0 and the tail untouched
             _lib.call_value('sawfem_mf_set_group_tri', eng._plan, 4, 0)
@@ -105,12 +123,19 @@ def test_grouped_solve_matches_the_slot_map_solve(G):
                 assert torch.equal(got[:G], x0[:G])
                 assert torch.equal(got[G:nfull * G], ref[G:nfull * G])
             assert torch.equal(got[nfull * G:], x0[nfull * G:])
+            if tr == 0:
+                gr = _rows_solve(eng, s, B, R, arena, G, uarena, x0, sel.data_ptr() if sel.numel() else None,
+                                 int(sel.numel()), 0)
+                assert torch.equal(gr, got)
     _lib.call_value('sawfem_mf_set_group_tri', eng._plan, 4, 48 * 1024)
     # invalid group sizes and selections are refused before any launch
     assert _lib.raw('sawfem_mf_solve_groups', eng._plan, s, B, 1, arena.data_ptr(), 0, uarena.data_ptr(),
                     x0.data_ptr(), 0, 64, None, 0, 1) == _lib.INVALID
     assert _lib.raw('sawfem_mf_solve_groups', eng._plan, s, B, 1, arena.data_ptr(), G, uarena.data_ptr(),
                     x0.data_ptr(), 0, 64, slot.data_ptr(), nfull + 1, 1) == _lib.INVALID
+    # a row stride below the vector count is refused
+    assert _lib.raw('sawfem_mf_solve_groups_rows', eng._plan, s, B, 1, arena.data_ptr(), G, uarena.data_ptr(),
+                    x0.data_ptr(), B - 1, 64, None, 0, 1) == _lib.INVALID
 
 
 @pytest.mark.parametrize('share', [2, 4])
