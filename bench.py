@@ -40,9 +40,9 @@ PEAK_FP64_MFMA = 78.6
 # factorization or vector block of a chunk): at another sharing the entry is the measured ratio
 # of HBM to algorithmic bytes times the run's own algorithmic bytes per launch.
 COUNTER_TRAFFIC = {
-    ('grp', 'config2'): dict(share=8, hbm=1.216e9, alg=7.216e8,
+    ('grp', 'config2'): dict(share=16, hbm=8.113e8, alg=6.786e8,
                              source='profiles/config2_grouped_solve_traffic.md'),
-    ('schur', 'config2'): dict(share=8, hbm=6.978e9, alg=3.624e9, source='profiles/config2_schur_traffic.md'),
+    ('schur', 'config2'): dict(share=16, hbm=3.562e9, alg=1.848e9, source='profiles/config2_schur_traffic.md'),
     ('schur', 'config4'): dict(share=1, hbm=1.620e10, alg=8.48e9, source='profiles/config4_schur_traffic.md'),
 }
 
@@ -71,12 +71,13 @@ def grp_roofline(grp, workload, share):
     traffic, note = counter_traffic('grp', workload, share, grp['bytes'] / grp['launches'])
     return dict(bound='hbm', achieved=gbs, peak=8000.0, unit='GB/s', frac=gbs / 8000.0,
                 traffic=traffic, traffic_note=note,
-                kernel='k_grp_lds<kind, 8|16, false> (grouped whole-front solves, complex64 factors, FP64 vectors)',
+                kernel='k_grp_lds<kind, 8|16, false, true> (grouped whole-front solves on row-major vectors, '
+                       'complex64 factors, FP64 vectors)',
                 launches=grp['launches'], kernel_ms=grp['ms'], algorithmic_bytes=grp['bytes'],
                 algorithmic_bytes_per_launch=grp['bytes'] / grp['launches'],
                 avg_launch_ms=grp['ms'] / grp['launches'],
-                note='latency-bound: chains of dependent loads per front; SQ counters: 62 % of wave cycles waiting, '
-                     'VALU 15 % busy')
+                note='latency-bound: each front is a chain of dependent loads (gather, diagonal blocks, update '
+                     'rows, scatter)')
 
 
 def log(*a):

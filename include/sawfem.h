@@ -376,6 +376,12 @@ int sawfem_charge_rows(void* stream, int n, int nc, int64_t ld, const void* kts_
                        double V, const void* x, const int* cols, void* work, void* Q);
 int sawfem_wdot_rows(void* stream, int n, int nc, int64_t ld, const void* a, const int* acol, const double* w,
                      const void* c, const int* ccol, void* work, void* out);
+/* Both duality products of the paired refinement in one pass, on row-major arrays whose columns
+ * pair up per frequency (2b forward, 2b + 1 adjoint): out1[b] = sum_i X[i][2b + 1] w[i] C[i][2b],
+ * out2[b] = sum_i X[i][2b] w[i] C[i][2b + 1] for b < nb (ld >= 2 nb), each equal bit for bit to
+ * sawfem_wdot on the same vectors.  work: 2048 nb complex128.  No counterpart in the reference. */
+int sawfem_wdot_pairs_rows(void* stream, int n, int nb, int64_t ld, const void* X, const double* w, const void* C,
+                           void* work, void* out1, void* out2);
 
 /* x += d (n complex128). */
 int sawfem_axpy(void* stream, int64_t n, void* x, const void* d);

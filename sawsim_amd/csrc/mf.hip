@@ -1550,11 +1550,14 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
   // The factor entries of the first product do not depend on the vectors: this thread's first
   // batch of them is loaded before the gather, so its round trip overlaps the gather's.  First
   // product: forward, row tr of the first diagonal block (k in [0, tr)); backward with
-  // contribution-block rows, row tr of U12 (k in [0, q)).
+  // contribution-block rows, row tr of U12 (k in [0, q)).  Not in the row-major forward kernel:
+  // there the batch held across the gather costs 19 registers (80 -> 99, 6 -> 4 waves a SIMD),
+  // more than the round trip it saves.
+  constexpr bool pre_fwd = KIND == 0 && !XR;
   const bool cb_first = KIND == 1 && !PIV && q > 0;
   const int nbf = min(nbl, p);
   float pex[GRP_NB], pey[GRP_NB];
-  if (KIND == 0 && tr > 0 && tr < nbf) grp_load<TPR>(Fb + tr, m, 0, tr, q4, pex, pey);
+  if (pre_fwd && tr > 0 && tr < nbf) grp_load<TPR>(Fb + tr, m, 0, tr, q4, pex, pey);
   if (KIND == 1 && cb_first && tr < p) grp_load<TPR>(Fb + tr + (int64_t)p * m, m, 0, q, q4, pex, pey);
   const int mr = W * RC;
   // entry idx of the block's W x RC vector entries: the row fastest over the threads in the
@@ -1682,12 +1685,12 @@ __global__ void __launch_bounds__(256) k_grp_lds(FrontsDev F, const int* __restr
       grp_row_product<TPR, VPT, WS, false>(col, m, k0, k1, q4, w, J0, r0, s);
   };
   // forward: the first block's product from the batch loaded before the gather, ahead of the loop
-  if (KIND == 0 && tr < nbf) diag(0, nbf, std::true_type{});
+  if (pre_fwd && tr < nbf) diag(0, nbf, std::true_type{});
   for (int bi = 0; bi < nblk; ++bi) {
     const int J0 = KIND == 0 ? bi * nbl : (nblk - 1 - bi) * nbl;
     const int nbk = min(nbl, p - J0);
     const bool own = tr < nbk;
-    if (own && (KIND == 1 || bi > 0)) diag(J0, nbk, std::false_type{});
+    if (own && (!pre_fwd || bi > 0)) diag(J0, nbk, std::false_type{});
     __syncthreads();
     if (own) {
 #pragma unroll
@@ -2118,6 +2121,18 @@ __global__ void __launch_bounds__(256) k_transpose(int R, int C, int ld_in, int 
   }
 }
 
+// s + a b with every product and sum rounded on its own (no contraction into FMAs): the charge
+// and duality products below use it, so that their forms over either vector layout agree bit for
+// bit whatever fusing the compiler would choose in each loop.
+__device__ __forceinline__ double2 cfma_nc(double2 s, double2 a, double2 b) {
+#pragma clang fp contract(off)
+  return make_double2(s.x + (a.x * b.x - a.y * b.y), s.y + (a.x * b.y + a.y * b.x));
+}
+__device__ __forceinline__ double2 charge_out(double2 sum, double V, double2 kts_z) {
+#pragma clang fp contract(off)
+  return make_double2(sum.x + V * kts_z.x, sum.y + V * kts_z.y);
+}
+
 // Q[b] = sum_p kts_a[p] x[b][p] + V kts_z: one 1024-thread block per frequency (many loads in
 // flight for the long dot product), fixed-order tree reduction (deterministic)
 constexpr int CHARGE_NT = 1024;
@@ -2128,7 +2143,7 @@ __global__ void __launch_bounds__(CHARGE_NT) k_charge(int n, const double2* __re
   const int b = blockIdx.x;
   double2 s = make_double2(0, 0);
 #pragma unroll 4
-  for (int i = threadIdx.x; i < n; i += CHARGE_NT) s = cfma(s, kts_a[i], x[(int64_t)b * n + i]);
+  for (int i = threadIdx.x; i < n; i += CHARGE_NT) s = cfma_nc(s, kts_a[i], x[(int64_t)b * n + i]);
   red[threadIdx.x] = s;
   __syncthreads();
   for (int h = CHARGE_NT / 2; h > 0; h >>= 1) {
@@ -2138,7 +2153,7 @@ __global__ void __launch_bounds__(CHARGE_NT) k_charge(int n, const double2* __re
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) Q[b] = make_double2(red[0].x + V * kts_z.x, red[0].y + V * kts_z.y);
+  if (threadIdx.x == 0) Q[b] = charge_out(red[0], V, kts_z);
 }
 
 // out[k] = sum_i a_b[i] w[i] c_b[i] (no conjugation; w real, or 1 when null) for b = sel[k] (or
@@ -2160,7 +2175,7 @@ __global__ void __launch_bounds__(WDOT_NT) k_wdot(int n, const double2* __restri
     const double wi = w ? w[i] : 1.0;
     u.x *= wi;
     u.y *= wi;
-    s = cfma(s, u, cb[i]);
+    s = cfma_nc(s, u, cb[i]);
   }
   red[threadIdx.x] = s;
   __syncthreads();
@@ -2195,16 +2210,46 @@ __global__ void __launch_bounds__(1024) k_rows_partial(int n, int nc, int64_t ld
   double2 s = make_double2(0, 0);
   for (int i = t; i < n; i += 1024) {
     if (CHARGE) {
-      s = cfma(s, a[i], c[(int64_t)i * ld + cc]);
+      s = cfma_nc(s, a[i], c[(int64_t)i * ld + cc]);
     } else {
       double2 u = a[(int64_t)i * ld + ca];
       const double wi = w ? w[i] : 1.0;
       u.x *= wi;
       u.y *= wi;
-      s = cfma(s, u, c[(int64_t)i * ld + cc]);
+      s = cfma_nc(s, u, c[(int64_t)i * ld + cc]);
     }
   }
   part[(int64_t)t * nc + k] = s;
+}
+
+// Both duality products of the paired refinement in one pass over two row-major arrays whose
+// columns pair up per frequency (2b: forward vector, 2b + 1: adjoint): out1[b] = sum_i X[i][2b +
+// 1] w[i] C[i][2b], out2[b] = sum_i X[i][2b] w[i] C[i][2b + 1], stage 1 as k_rows_partial (each a
+// k_wdot order), a lane reading its frequency's two adjacent columns of each array.
+__global__ void __launch_bounds__(1024) k_rows_partial_pair(int n, int nb, int64_t ld, const double2* __restrict__ X,
+                                                            const double* __restrict__ w,
+                                                            const double2* __restrict__ C, double2* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 64 + lane;
+  const int t = blockIdx.y * 16 + (threadIdx.x >> 6);
+  if (b >= nb) return;
+  double2 s1 = make_double2(0, 0), s2 = make_double2(0, 0);
+  for (int i = t; i < n; i += 1024) {
+    const double2* xr = X + (int64_t)i * ld + 2 * b;
+    const double2* cr = C + (int64_t)i * ld + 2 * b;
+    const double2 x0 = xr[0], x1 = xr[1], c0 = cr[0], c1 = cr[1];
+    const double wi = w ? w[i] : 1.0;
+    double2 u = x1;
+    u.x *= wi;
+    u.y *= wi;
+    s1 = cfma_nc(s1, u, c0);
+    u = x0;
+    u.x *= wi;
+    u.y *= wi;
+    s2 = cfma_nc(s2, u, c1);
+  }
+  part[(int64_t)t * nb + b] = s1;
+  part[(int64_t)(1024 + t) * nb + b] = s2;
 }
 
 template <bool CHARGE>
@@ -2222,7 +2267,7 @@ __global__ void __launch_bounds__(1024) k_rows_reduce(int nc, const double2* __r
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    if (CHARGE) out[k] = make_double2(red[0].x + V * kts_z.x, red[0].y + V * kts_z.y);
+    if (CHARGE) out[k] = charge_out(red[0], V, kts_z);
     else out[k] = red[0];
   }
 }
@@ -3840,6 +3885,24 @@ int sawfem_wdot_rows(void* stream, int n, int nc, int64_t ld, const void* a, con
                      (const double2*)a, acol, w, (const double2*)c, ccol, (double2*)work);
   hipLaunchKernelGGL(k_rows_reduce<false>, dim3(nc), dim3(1024), 0, s, nc, (const double2*)work,
                      make_double2(0, 0), 0.0, (double2*)out);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_wdot_pairs_rows(void* stream, int n, int nb, int64_t ld, const void* X, const double* w, const void* C,
+                           void* work, void* out1, void* out2) {
+  if (n <= 0 || nb <= 0 || ld < 2 * (int64_t)nb || !X || !C || !work || !out1 || !out2) {
+    sawfem_set_error("sawfem_wdot_pairs_rows: invalid arguments");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  double2* part = (double2*)work;
+  hipLaunchKernelGGL(k_rows_partial_pair, dim3((nb + 63) / 64, 64), dim3(1024), 0, s, n, nb, ld, (const double2*)X, w,
+                     (const double2*)C, part);
+  hipLaunchKernelGGL(k_rows_reduce<false>, dim3(nb), dim3(1024), 0, s, nb, part, make_double2(0, 0), 0.0,
+                     (double2*)out1);
+  hipLaunchKernelGGL(k_rows_reduce<false>, dim3(nb), dim3(1024), 0, s, nb, part + (int64_t)1024 * nb,
+                     make_double2(0, 0), 0.0, (double2*)out2);
   SAW_HIP(hipGetLastError());
   return 0;
 }

@@ -860,26 +860,23 @@ class FrequencySolver:
         rhs2 = X.clone()
         self._solve(prec, B, 2, arena, uarena, X, 0, nb, ld=ld)
         Qd = torch.empty(nv, **c128)
-        w3 = torch.empty((3, B), **c128)
+        w4 = torch.empty((4, B), **c128)
         work = torch.empty(1024 * nv, **c128)            # partial sums of the row-major products
-        even = torch.arange(0, nv, 2, dtype=torch.int32, device=self.dev)
-        odd = even + 1
 
-        def wdot(acol, c, ccol, out):
-            # out[b] = sum_i X[i][acol[b]] d[i] c[i][ccol[b]]
-            _lib.call('sawfem_wdot_rows', s, n, B, ld, X.data_ptr(), acol.data_ptr(), d.data_ptr(), c.data_ptr(),
-                      ccol.data_ptr(), work.data_ptr(), out.data_ptr())
+        def wdots(c, out1, out2):
+            # out1[b] = sum_i X[i][2b + 1] d[i] c[i][2b], out2[b] = sum_i X[i][2b] d[i] c[i][2b + 1]
+            _lib.call('sawfem_wdot_pairs_rows', s, n, B, ld, X.data_ptr(), d.data_ptr(), c.data_ptr(), work.data_ptr(),
+                      out1.data_ptr(), out2.data_ptr())
 
         def measure():
             _lib.call('sawfem_charge_rows', s, n, nv, ld, self.kts_a.data_ptr(), self.kts_z.real, self.kts_z.imag,
                       float(V), X.data_ptr(), None, work.data_ptr(), Qd.data_ptr())
-            wdot(odd, rhs2, even, w3[0])                                          # mu = (D lambda)^T b
-            return [Qd[0::2].cpu().numpy().copy(), w3[0].cpu().numpy()]
+            wdots(rhs2, w4[0], w4[3])                                             # mu = (D lambda)^T b
+            return [Qd[0::2].cpu().numpy().copy(), w4[0].cpu().numpy()]
         def dwr(r):
             # Q - Q* = -lambda^T r_x (forward residual); mu - mu* = -(D x)^T r_z (adjoint residual)
-            wdot(odd, r, even, w3[1])
-            wdot(even, r, odd, w3[2])
-            e = w3[1:].cpu().numpy()
+            wdots(r, w4[1], w4[2])
+            e = w4[1:3].cpu().numpy()
             return [e[0], e[1]]
         res = self._residual_op(nv, om.repeat_interleave(2), ld=ld)
         (Q, mu), (dq, dm), (err, aerr), steps, hist = self._refine(prec, nb, arena, uarena, X, 0, res, rhs2, None,

@@ -438,10 +438,10 @@ def test_weighted_dot_matches_torch():
 
 
 def test_row_major_charge_and_weighted_dot_match_bit_for_bit():
-    """sawfem_charge_rows / sawfem_wdot_rows on the vectors of a [B][2][n] array stored row-major
-    ([n][ld], column k = vector k) equal sawfem_charge / sawfem_wdot on the array itself bit for
-    bit (the same summation order), with the identity and with shuffled column maps, weights and
-    no weights."""
+    """sawfem_charge_rows / sawfem_wdot_rows / sawfem_wdot_pairs_rows on the vectors of a [B][2][n]
+    array stored row-major ([n][ld], column k = vector k) equal sawfem_charge / sawfem_wdot on the
+    array itself bit for bit (the same summation order), with the identity and with shuffled
+    column maps, weights and no weights."""
     _say('[test_row_major_charge_and_weighted_dot_match_bit_for_bit]')
     import torch
     from sawsim_amd import _lib
@@ -482,6 +482,14 @@ def test_row_major_charge_and_weighted_dot_match_bit_for_bit():
                   work.data_ptr(), o2.data_ptr())
         torch.cuda.synchronize()
         assert torch.equal(o1, o2)
+        # both products of a frequency's column pair in one pass (sawfem_wdot_pairs_rows)
+        o3 = torch.empty_like(o1)
+        p1, p2 = torch.empty_like(o1), torch.empty_like(o1)
+        _lib.call('sawfem_wdot', s, n, B, X.data_ptr(), 2 * n, wp, C.data_ptr() + 16 * n, 2 * n, None, o3.data_ptr())
+        _lib.call('sawfem_wdot_pairs_rows', s, n, B, ld, Xr.data_ptr(), wp, Cr.data_ptr(), work.data_ptr(),
+                  p1.data_ptr(), p2.data_ptr())
+        torch.cuda.synchronize()
+        assert torch.equal(p1, o1) and torch.equal(p2, o3)
 
 
 @pytest.mark.parametrize('transposed', [False, True])
