@@ -294,7 +294,7 @@ int sawfem_mf_set_group_rc(void* plan, int rc_max);
  * (default 4).  Returns rc_max in use or -1.  No counterpart in the reference. */
 int sawfem_mf_set_group_tri(void* plan, int rc_max, int lds_bytes);
 /* sawfem_mf_solve_groups: the block kernels for pivot rows and off-diagonal panels take the
- * two-kernel buckets whose largest pivot block is at most max_p (default 256; 0: none); larger
+ * two-kernel buckets whose largest pivot block is at most max_p (default 1024; 0: none); larger
  * ones keep the pivot-triangle + panel pair.  Returns the value in use or -1.  No counterpart in
  * the reference. */
 int sawfem_mf_set_group_big(void* plan, int max_p);

@@ -2375,7 +2375,7 @@ struct Plan {
   int group_rc = 16;             // grouped solves: vectors per workgroup of k_grp_lds at most
   int group_rc_tri = 4;          // grouped solves: vectors per workgroup of the other kernels
   int grp_lds_bytes = 48 * 1024;   // one-workgroup buckets whose k_grp_lds LDS fits: k_grp_lds
-  int grp_big_pmax = 256;          // two-kernel buckets with pivot blocks up to this: k_grp_lds + k_grp_panel
+  int grp_big_pmax = 1024;         // two-kernel buckets with pivot blocks up to this: k_grp_lds + k_grp_panel
   std::vector<void*> part_owned[2];
   std::vector<void*> panel_owned[2];   // the buckets' panel work lists (rebuilt per panel width)
   int panel_nb[2] = {0, 0};      // panel width the buckets' panel lists were built for (0: none)
