@@ -382,6 +382,14 @@ int sawfem_wdot_rows(void* stream, int n, int nc, int64_t ld, const void* a, con
  * sawfem_wdot on the same vectors.  work: 2048 nb complex128.  No counterpart in the reference. */
 int sawfem_wdot_pairs_rows(void* stream, int n, int nb, int64_t ld, const void* X, const double* w, const void* C,
                            void* work, void* out1, void* out2);
+/* One update of the paired refinement on row-major column pairs, with its measurements, in one
+ * pass: X[i][c] += R[i][c] for c < 2 nb, then Q[c] = sum_i kts_a[i] X[i][c] + V kts_z (c < 2 nb)
+ * and mu[b] = sum_i X[i][2b + 1] w[i] rhs[i][2b] (b < nb), each equal bit for bit to sawfem_axpy,
+ * sawfem_charge and sawfem_wdot run one after another.  work: 3072 nb complex128.  No counterpart
+ * in the reference. */
+int sawfem_rows_update_measure(void* stream, int n, int nb, int64_t ld, void* X, const void* R, const void* kts_a,
+                               double kz_re, double kz_im, double V, const double* w, const void* rhs, void* work,
+                               void* Q, void* mu);
 
 /* x += d (n complex128). */
 int sawfem_axpy(void* stream, int64_t n, void* x, const void* d);

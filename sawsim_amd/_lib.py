@@ -67,6 +67,7 @@ SIGNATURES = {
     'sawfem_charge_rows': [_vp, _i, _i, _i64, _vp, _d, _d, _d, _vp, _vp, _vp, _vp],
     'sawfem_wdot_rows': [_vp, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     'sawfem_wdot_pairs_rows': [_vp, _i, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    'sawfem_rows_update_measure': [_vp, _i, _i, _i64, _vp, _vp, _vp, _d, _d, _d, _vp, _vp, _vp, _vp, _vp],
     'sawfem_transpose': [_vp, _i, _i, _i, _i, _vp, _vp],
     'sawfem_charge': [_vp, _i, _i, _vp, _d, _d, _d, _vp, _vp],
     'sawfem_axpy': [_vp, _i64, _vp, _vp],

@@ -2252,6 +2252,46 @@ __global__ void __launch_bounds__(1024) k_rows_partial_pair(int n, int nb, int64
   part[(int64_t)(1024 + t) * nb + b] = s2;
 }
 
+// One refinement update of the paired row-major vectors and its measurements in one pass: X +=
+// R on the column pairs (2b forward, 2b + 1 adjoint: k_axpy's sums), then the charges of both
+// updated columns (k_charge's products) and mu[b] = sum_i X[i][2b + 1] w[i] rhs[i][2b] (k_wdot's),
+// stage 1 as k_rows_partial (pq: [1024][2 nb], pm: [1024][nb]).
+__global__ void __launch_bounds__(1024) k_rows_update_measure(int n, int nb, int64_t ld, double2* __restrict__ X,
+                                                              const double2* __restrict__ R,
+                                                              const double2* __restrict__ kts_a,
+                                                              const double* __restrict__ w,
+                                                              const double2* __restrict__ rhs,
+                                                              double2* __restrict__ pq, double2* __restrict__ pm) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * 64 + lane;
+  const int t = blockIdx.y * 16 + (threadIdx.x >> 6);
+  if (b >= nb) return;
+  double2 q0 = make_double2(0, 0), q1 = make_double2(0, 0), mu = make_double2(0, 0);
+  for (int i = t; i < n; i += 1024) {
+    double2* xr = X + (int64_t)i * ld + 2 * b;
+    const double2* rr = R + (int64_t)i * ld + 2 * b;
+    double2 x0 = xr[0], x1 = xr[1];
+    const double2 r0 = rr[0], r1 = rr[1];
+    x0.x += r0.x;
+    x0.y += r0.y;
+    x1.x += r1.x;
+    x1.y += r1.y;
+    xr[0] = x0;
+    xr[1] = x1;
+    const double2 k = kts_a[i];
+    q0 = cfma_nc(q0, k, x0);
+    q1 = cfma_nc(q1, k, x1);
+    const double wi = w[i];
+    double2 u = x1;
+    u.x *= wi;
+    u.y *= wi;
+    mu = cfma_nc(mu, u, rhs[(int64_t)i * ld + 2 * b]);
+  }
+  pq[(int64_t)t * (2 * nb) + 2 * b] = q0;
+  pq[(int64_t)t * (2 * nb) + 2 * b + 1] = q1;
+  pm[(int64_t)t * nb + b] = mu;
+}
+
 template <bool CHARGE>
 __global__ void __launch_bounds__(1024) k_rows_reduce(int nc, const double2* __restrict__ part, double2 kts_z,
                                                       double V, double2* __restrict__ out) {
@@ -3903,6 +3943,26 @@ int sawfem_wdot_pairs_rows(void* stream, int n, int nb, int64_t ld, const void* 
                      (double2*)out1);
   hipLaunchKernelGGL(k_rows_reduce<false>, dim3(nb), dim3(1024), 0, s, nb, part + (int64_t)1024 * nb,
                      make_double2(0, 0), 0.0, (double2*)out2);
+  SAW_HIP(hipGetLastError());
+  return 0;
+}
+
+int sawfem_rows_update_measure(void* stream, int n, int nb, int64_t ld, void* X, const void* R, const void* kts_a,
+                               double kz_re, double kz_im, double V, const double* w, const void* rhs, void* work,
+                               void* Q, void* mu) {
+  if (n <= 0 || nb <= 0 || ld < 2 * (int64_t)nb || !X || !R || !kts_a || !w || !rhs || !work || !Q || !mu) {
+    sawfem_set_error("sawfem_rows_update_measure: invalid arguments");
+    return 1;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  double2* pq = (double2*)work;
+  double2* pm = pq + (int64_t)1024 * 2 * nb;
+  hipLaunchKernelGGL(k_rows_update_measure, dim3((nb + 63) / 64, 64), dim3(1024), 0, s, n, nb, ld, (double2*)X,
+                     (const double2*)R, (const double2*)kts_a, w, (const double2*)rhs, pq, pm);
+  hipLaunchKernelGGL(k_rows_reduce<true>, dim3(2 * nb), dim3(1024), 0, s, 2 * nb, pq, make_double2(kz_re, kz_im), V,
+                     (double2*)Q);
+  hipLaunchKernelGGL(k_rows_reduce<false>, dim3(nb), dim3(1024), 0, s, nb, pm, make_double2(0, 0), 0.0,
+                     (double2*)mu);
   SAW_HIP(hipGetLastError());
   return 0;
 }

@@ -691,10 +691,11 @@ class FrequencySolver:
         return int(cnt.value), float(rmin.value)
 
     def _refine(self, prec, nb, arena, uarena, X, transpose, res, rhst, tslot, measure, ir_tol, max_ir, dwr=None,
-                rows=None):
+                rows=None, update=None):
         """Iterative refinement of X ([B][R][n], holding the first solve) in place; with
         ``rows=(B, R, ld)`` X is row-major [n][ld] (vector b R + r in column b R + r; grouped
-        solves only), and so are rhst and the residuals.
+        solves only), and so are rhst and the residuals.  ``update(r)``, when given, applies the
+        correction X += r and returns what ``measure()`` would then (one fused pass).
 
         Each step: FP64 residual of every frequency, a solve of the frequencies still active,
         update.  A frequency converges when the relative change of every quantity ``measure()``
@@ -753,9 +754,12 @@ class FrequencySolver:
                     r[torch.as_tensor(~active, device=self.dev)] = 0
                 sel = active
             self._solve(prec, B, R, arena, uarena, r, transpose, nb, sel, ld=ld)
-            _lib.call('sawfem_axpy', s, X.numel(), X.data_ptr(), r.data_ptr())
+            if update is not None:
+                new = update(r)                          # X += r and measure() in one pass
+            else:
+                _lib.call('sawfem_axpy', s, X.numel(), X.data_ptr(), r.data_ptr())
+                new = measure()
             applied = steps
-            new = measure()
             stalled = np.zeros(B, dtype=bool)
             for k in range(len(vals)):
                 dq = np.abs(new[k] - vals[k]) / np.maximum(np.abs(new[k]), 1e-300)
@@ -861,7 +865,7 @@ class FrequencySolver:
         self._solve(prec, B, 2, arena, uarena, X, 0, nb, ld=ld)
         Qd = torch.empty(nv, **c128)
         w4 = torch.empty((4, B), **c128)
-        work = torch.empty(1024 * nv, **c128)            # partial sums of the row-major products
+        work = torch.empty(3072 * B, **c128)             # partial sums of the row-major products
 
         def wdots(c, out1, out2):
             # out1[b] = sum_i X[i][2b + 1] d[i] c[i][2b], out2[b] = sum_i X[i][2b] d[i] c[i][2b + 1]
@@ -878,9 +882,16 @@ class FrequencySolver:
             wdots(r, w4[1], w4[2])
             e = w4[1:3].cpu().numpy()
             return [e[0], e[1]]
+        def update(r):
+            # X += r, then measure()'s charges and mu, in one pass over X, r and the right-hand sides
+            _lib.call('sawfem_rows_update_measure', s, n, B, ld, X.data_ptr(), r.data_ptr(), self.kts_a.data_ptr(),
+                      self.kts_z.real, self.kts_z.imag, float(V), d.data_ptr(), rhs2.data_ptr(), work.data_ptr(),
+                      Qd.data_ptr(), w4[0].data_ptr())
+            return [Qd[0::2].cpu().numpy().copy(), w4[0].cpu().numpy()]
         res = self._residual_op(nv, om.repeat_interleave(2), ld=ld)
         (Q, mu), (dq, dm), (err, aerr), steps, hist = self._refine(prec, nb, arena, uarena, X, 0, res, rhs2, None,
-                                                                  measure, ir_tol, max_ir, dwr=dwr, rows=(B, 2, ld))
+                                                                  measure, ir_tol, max_ir, dwr=dwr, rows=(B, 2, ld),
+                                                                  update=update)
         x = X[:, 0:nv:2].t().contiguous()
         lam = (X[:, 1:nv:2].t() * d).contiguous()
         del X, rhs2, work

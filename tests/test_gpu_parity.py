@@ -490,6 +490,23 @@ def test_row_major_charge_and_weighted_dot_match_bit_for_bit():
                   p1.data_ptr(), p2.data_ptr())
         torch.cuda.synchronize()
         assert torch.equal(p1, o1) and torch.equal(p2, o3)
+    # the refinement update with its measurements in one pass (sawfem_rows_update_measure) against
+    # sawfem_axpy, sawfem_charge_rows and sawfem_wdot_rows one after another
+    Rr = torch.zeros_like(Xr)
+    Rr[:, :nv] = cx(nv, n).t()
+    X1, X2 = Xr.clone(), Xr.clone()
+    qa, qb = torch.empty(nv, dtype=torch.complex128, device='cuda'), torch.empty(nv, dtype=torch.complex128, device='cuda')
+    ma, mb = torch.empty(B, dtype=torch.complex128, device='cuda'), torch.empty(B, dtype=torch.complex128, device='cuda')
+    work3 = torch.empty(3072 * B, dtype=torch.complex128, device='cuda')
+    _lib.call('sawfem_axpy', s, X1.numel(), X1.data_ptr(), Rr.data_ptr())
+    _lib.call('sawfem_charge_rows', s, n, nv, ld, kts.data_ptr(), 0.25, -1.5, 2.0, X1.data_ptr(), None,
+              work.data_ptr(), qa.data_ptr())
+    _lib.call('sawfem_wdot_rows', s, n, B, ld, X1.data_ptr(), odd.data_ptr(), w.data_ptr(), Cr.data_ptr(),
+              even.data_ptr(), work.data_ptr(), ma.data_ptr())
+    _lib.call('sawfem_rows_update_measure', s, n, B, ld, X2.data_ptr(), Rr.data_ptr(), kts.data_ptr(), 0.25, -1.5, 2.0,
+              w.data_ptr(), Cr.data_ptr(), work3.data_ptr(), qb.data_ptr(), mb.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(X1, X2) and torch.equal(qa, qb) and torch.equal(ma, mb)
 
 
 @pytest.mark.parametrize('transposed', [False, True])
