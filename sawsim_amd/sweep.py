@@ -858,9 +858,11 @@ class FrequencySolver:
         nv = 2 * B
         ld = -(-nv // 64) * 64
         c128 = dict(dtype=torch.complex128, device=self.dev)
+        # [B][2][n] right-hand sides transposed (tiled) into the row-major array
+        V2 = torch.stack([rhs, (self.kts_a / d).unsqueeze(0).expand(B, n)], dim=1)
         X = torch.zeros((n, ld), **c128)
-        X[:, 0:nv:2] = rhs.t()
-        X[:, 1:nv:2] = (self.kts_a / d).unsqueeze(1)
+        _lib.call('sawfem_transpose', s, nv, n, n, ld, V2.data_ptr(), X.data_ptr())
+        del V2
         rhs2 = X.clone()
         self._solve(prec, B, 2, arena, uarena, X, 0, nb, ld=ld)
         Qd = torch.empty(nv, **c128)
@@ -892,9 +894,11 @@ class FrequencySolver:
         (Q, mu), (dq, dm), (err, aerr), steps, hist = self._refine(prec, nb, arena, uarena, X, 0, res, rhs2, None,
                                                                   measure, ir_tol, max_ir, dwr=dwr, rows=(B, 2, ld),
                                                                   update=update)
-        x = X[:, 0:nv:2].t().contiguous()
-        lam = (X[:, 1:nv:2].t() * d).contiguous()
-        del X, rhs2, work
+        XT = torch.empty((B, 2, n), **c128)              # back to [B][2][n] by one tiled transpose
+        _lib.call('sawfem_transpose', s, n, nv, ld, n, X.data_ptr(), XT.data_ptr())
+        x = XT[:, 0].contiguous()
+        lam = (XT[:, 1] * d).contiguous()
+        del X, XT, rhs2, work
         return self._paired_out(omegas, V, x, lam, Q, steps, dq, err, hist, dm, aerr)
 
     def _paired_out(self, omegas, V, x, lam, Q, steps, dq, err, hist, dm, aerr):
